@@ -1,0 +1,304 @@
+"""cubql_amd -- MI355X (gfx950) BVH build-and-query, Python host binding.
+
+Python mirror of the reference's host interface for the build-and-query
+path, over the C ABI in ``include/cubql_amd/cubql_capi.h``:
+
+* ``gpuBuilder(boxes, BuildConfig(...))``  -- cuBQL::gpuBuilder
+  (builder/cuda.h:91-99)
+* ``cuda.radixBuilder / cuda.radix30Builder`` -- cuBQL::cuda::radixBuilder
+  (builder/cuda.h:134-140)
+* ``cuda.free(bvh)`` -- cuBQL::cuda::free (builder/cuda.h:155-161)
+* ``points.findClosest / points.findKNN`` -- batched versions of the device
+  functions in queries/points/findClosest.h and queries/points/knn.h
+
+Device memory and streams come from PyTorch (plumbing only): BVH arrays are
+allocated through a memory resource backed by torch's caching allocator,
+and work is enqueued on torch's current stream unless a stream is given.
+
+The compute path is the HIP library ``libcubql_amd.so``; importing this
+package on a machine where it is missing raises immediately -- there is no
+CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from typing import Optional
+
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcubql_amd.so")
+
+SPATIAL_MEDIAN, SAH, ELH = 0, 1, 2
+
+
+class CubqlError(RuntimeError):
+    pass
+
+
+class Box3f(C.Structure):
+    _fields_ = [("lower", C.c_float * 3), ("upper", C.c_float * 3)]
+
+
+class BVH3f(C.Structure):
+    """Byte-identical to BinaryBVH<float,3> (bvh.h:71-111)."""
+    _fields_ = [("nodes", C.c_void_p), ("numNodes", C.c_uint32),
+                ("primIDs", C.c_void_p), ("numPrims", C.c_uint32)]
+
+
+class BuildConfig(C.Structure):
+    """Byte-identical to cuBQL::BuildConfig (bvh.h:31-64)."""
+    _fields_ = [("maxAllowedLeafSize", C.c_int), ("makeLeafThreshold", C.c_int),
+                ("buildMethod", C.c_int)]
+
+    def __init__(self, makeLeafThreshold: int = 0, maxAllowedLeafSize: int = 1 << 15,
+                 buildMethod: int = SPATIAL_MEDIAN):
+        super().__init__(maxAllowedLeafSize, makeLeafThreshold, buildMethod)
+
+    def enableSAH(self):
+        self.buildMethod = SAH
+        return self
+
+
+_MALLOC_CB = C.CFUNCTYPE(C.c_int, C.POINTER(C.c_void_p), C.c_size_t, C.c_void_p, C.c_void_p)
+_FREE_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p)
+
+
+class MemResource(C.Structure):
+    _fields_ = [("malloc_fn", _MALLOC_CB), ("free_fn", _FREE_CB), ("user", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libcubql_amd.so (raises if it was never built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CubqlError(f"{LIB_PATH} not found: build it with __graft_entry__.build() "
+                         "(or make -C cubql_amd/csrc); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    V, U32, I, F = C.c_void_p, C.c_uint32, C.c_int, C.c_float
+    P = C.POINTER
+    sigs = {
+        "cubql_last_error_string": (C.c_char_p, []),
+        "cubql_version": (C.c_char_p, []),
+        "cubql_gpu_builder_f3": (I, [P(BVH3f), V, U32, P(BuildConfig), V, P(MemResource)]),
+        "cubql_radix_builder_f3": (I, [P(BVH3f), V, U32, P(BuildConfig), V, P(MemResource)]),
+        "cubql_radix30_builder_f3": (I, [P(BVH3f), V, U32, P(BuildConfig), V, P(MemResource)]),
+        "cubql_free_f3": (I, [P(BVH3f), V, P(MemResource)]),
+        "cubql_refit_f3": (I, [P(BVH3f), V, V, P(MemResource)]),
+        "cubql_fcp_points_f3": (I, [P(BVH3f), V, V, U32, F, V, V, V]),
+        "cubql_knn_points_f3": (I, [P(BVH3f), V, V, U32, I, F, V, V, V, V]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(L, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names declared in the C ABI header that the loaded library exports."""
+    L = lib()
+    return {n for n in _declared_symbols() if hasattr(L, n)}
+
+
+def _declared_symbols():
+    hdr = os.path.join(os.path.dirname(HERE), "include", "cubql_amd", "cubql_capi.h")
+    import re
+    names = set()
+    with open(hdr) as f:
+        for m in re.finditer(r"\b(cubql_[a-z0-9_]+)\s*\(", f.read()):
+            names.add(m.group(1))
+    return names
+
+
+def _check(status: int, what: str):
+    if status != 0:
+        msg = lib().cubql_last_error_string()
+        raise CubqlError(f"{what} failed ({status}): {msg.decode() if msg else ''}")
+
+
+def _stream_handle(stream) -> int:
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class TorchMemory:
+    """cubql_mem_resource backed by torch's caching allocator."""
+
+    def __init__(self, device=None):
+        self.device = torch.device(device) if device is not None else torch.device(
+            "cuda", torch.cuda.current_device())
+        self.live = {}
+
+        def _malloc(pp, size, stream, user):
+            try:
+                t = torch.empty(int(size), dtype=torch.uint8, device=self.device)
+                self.live[t.data_ptr()] = t
+                pp[0] = t.data_ptr()
+                return 0
+            except Exception:  # noqa: BLE001 -- reported as a status code
+                return 1
+
+        def _free(p, stream, user):
+            self.live.pop(p, None)
+            return 0
+
+        self._m = _MALLOC_CB(_malloc)
+        self._f = _FREE_CB(_free)
+        self.resource = MemResource(self._m, self._f, None)
+
+    def tensor(self, ptr: int) -> torch.Tensor:
+        return self.live[ptr]
+
+
+class BinaryBVH:
+    """A BinaryBVH<float,3> whose arrays live in torch-managed device memory."""
+
+    def __init__(self, mem: TorchMemory):
+        self.mem = mem
+        self.c = BVH3f()
+
+    @property
+    def numNodes(self) -> int:
+        return int(self.c.numNodes)
+
+    @property
+    def numPrims(self) -> int:
+        return int(self.c.numPrims)
+
+    def nodes_tensor(self) -> torch.Tensor:
+        """[numNodes, 8] float32 view (lo xyz, hi xyz, admin word as 2 x u32)."""
+        if not self.c.nodes or self.numNodes == 0:
+            return torch.empty((0, 8), dtype=torch.float32, device=self.mem.device)
+        t = self.mem.tensor(self.c.nodes)
+        return t[: self.numNodes * 32].view(torch.float32).view(self.numNodes, 8)
+
+    def prim_ids_tensor(self) -> torch.Tensor:
+        if not self.c.primIDs:
+            return torch.empty((0,), dtype=torch.int32, device=self.mem.device)
+        t = self.mem.tensor(self.c.primIDs)
+        return t[: self.numPrims * 4].view(torch.int32)
+
+    def free(self, stream=None):
+        if self.c.nodes or self.c.primIDs:
+            _check(lib().cubql_free_f3(C.byref(self.c), C.c_void_p(_stream_handle(stream)),
+                                       C.byref(self.mem.resource)), "cuda::free")
+
+    def __del__(self):
+        try:
+            if torch is not None and (self.c.nodes or self.c.primIDs):
+                self.mem.live.pop(self.c.nodes, None)
+                self.mem.live.pop(self.c.primIDs, None)
+        except Exception:  # noqa: BLE001
+            pass
+
+
+def _as_boxes(boxes: torch.Tensor) -> torch.Tensor:
+    if boxes.dtype != torch.float32 or not boxes.is_cuda:
+        raise CubqlError("boxes must be a float32 CUDA tensor of shape [n,6] (lower xyz, upper xyz)")
+    return boxes.contiguous().view(-1, 6)
+
+
+def _build(fn_name: str, boxes: torch.Tensor, config: Optional[BuildConfig], stream, mem):
+    boxes = _as_boxes(boxes)
+    config = config or BuildConfig()
+    mem = mem or TorchMemory(boxes.device)
+    bvh = BinaryBVH(mem)
+    fn = getattr(lib(), fn_name)
+    _check(fn(C.byref(bvh.c), _ptr(boxes), boxes.shape[0], C.byref(config),
+              C.c_void_p(_stream_handle(stream)), C.byref(mem.resource)), fn_name)
+    return bvh
+
+
+def gpuBuilder(boxes: torch.Tensor, config: Optional[BuildConfig] = None, stream=None,
+               mem: Optional[TorchMemory] = None) -> BinaryBVH:
+    """cuBQL::gpuBuilder (builder/cuda.h:91-99) over [n,6] float32 boxes."""
+    return _build("cubql_gpu_builder_f3", boxes, config, stream, mem)
+
+
+def refit(bvh: BinaryBVH, boxes: torch.Tensor, stream=None):
+    boxes = _as_boxes(boxes)
+    _check(lib().cubql_refit_f3(C.byref(bvh.c), _ptr(boxes), C.c_void_p(_stream_handle(stream)),
+                                C.byref(bvh.mem.resource)), "refit")
+
+
+class cuda:  # noqa: N801 -- mirrors the reference namespace cuBQL::cuda
+    @staticmethod
+    def radixBuilder(boxes, config=None, stream=None, mem=None) -> BinaryBVH:
+        """cuBQL::cuda::radixBuilder (builder/cuda.h:134-140), 63-bit keys."""
+        return _build("cubql_radix_builder_f3", boxes, config, stream, mem)
+
+    @staticmethod
+    def radix30Builder(boxes, config=None, stream=None, mem=None) -> BinaryBVH:
+        """30-bit Morton variant of radixBuilder (radix.h:34-38)."""
+        return _build("cubql_radix30_builder_f3", boxes, config, stream, mem)
+
+    @staticmethod
+    def free(bvh: BinaryBVH, stream=None):
+        bvh.free(stream)
+
+
+def _as_points(p: torch.Tensor, what: str) -> torch.Tensor:
+    if p.dtype != torch.float32 or not p.is_cuda:
+        raise CubqlError(f"{what} must be a float32 CUDA tensor of shape [n,3]")
+    return p.contiguous().view(-1, 3)
+
+
+class points:  # noqa: N801 -- mirrors cuBQL::points
+    @staticmethod
+    def findClosest(bvh: BinaryBVH, data: torch.Tensor, queries: torch.Tensor,
+                    sqrMaxSearchDist: float = math.inf, stream=None):
+        """Batched points::findClosest: returns (int32 ids, float32 sqrDist)."""
+        data = _as_points(data, "data")
+        queries = _as_points(queries, "queries")
+        nq = queries.shape[0]
+        ids = torch.empty(nq, dtype=torch.int32, device=queries.device)
+        d2 = torch.empty(nq, dtype=torch.float32, device=queries.device)
+        _check(lib().cubql_fcp_points_f3(C.byref(bvh.c), _ptr(data), _ptr(queries), nq,
+                                         float(sqrMaxSearchDist), _ptr(ids), _ptr(d2),
+                                         C.c_void_p(_stream_handle(stream))), "findClosest")
+        return ids, d2
+
+    @staticmethod
+    def findKNN(bvh: BinaryBVH, data: torch.Tensor, queries: torch.Tensor, k: int,
+                sqrMaxSearchDist: float = math.inf, stream=None):
+        """Batched points::findKNN: (ids [nq,k], sqrDist [nq,k], numFound [nq])."""
+        data = _as_points(data, "data")
+        queries = _as_points(queries, "queries")
+        nq = queries.shape[0]
+        ids = torch.empty((nq, k), dtype=torch.int32, device=queries.device)
+        d2 = torch.empty((nq, k), dtype=torch.float32, device=queries.device)
+        nf = torch.empty(nq, dtype=torch.int32, device=queries.device)
+        _check(lib().cubql_knn_points_f3(C.byref(bvh.c), _ptr(data), _ptr(queries), nq, int(k),
+                                         float(sqrMaxSearchDist), _ptr(ids), _ptr(d2), _ptr(nf),
+                                         C.c_void_p(_stream_handle(stream))), "findKNN")
+        return ids, d2, nf
+
+
+def points_to_boxes(p: torch.Tensor) -> torch.Tensor:
+    """box3f().including(p) per point (samples/s01_closestPoint_points_gpu/closestPoint.cu:16-22)."""
+    p = p.contiguous().view(-1, 3)
+    return torch.cat([p, p], dim=1).contiguous()
+
+
+def nodes_to_numpy(bvh: BinaryBVH):
+    """Host copy of nodes as a structured array (lo[3], hi[3], oc u64)."""
+    import numpy as np
+    dt = np.dtype([("lo", "<f4", 3), ("hi", "<f4", 3), ("oc", "<u8")])
+    raw = bvh.nodes_tensor().contiguous().cpu().numpy()
+    return raw.view(dt).reshape(-1)

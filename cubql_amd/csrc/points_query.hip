@@ -1,0 +1,272 @@
+// points_query.hip -- batched closest-point and kNN queries over a point BVH.
+//
+// Replaces user kernels that call cuBQL::points::findClosest / findKNN once
+// per thread (queries/points/findClosest.h:181-226, queries/points/knn.h:
+// 75-116; sample launch: samples/s01_closestPoint_points_gpu/
+// closestPoint.cu:41-66).  One query per lane of a wave64; traversal from
+// include/cubql_amd/traversal.hip.h.  Result rule: among equal float32
+// distances the lowest primID wins (kNN lists are ordered by
+// (sqrDist, primID)).
+#include "common.h"
+#include "cubql_amd/traversal.hip.h"
+
+namespace cubql {
+namespace pq {
+
+constexpr int kBlock = 256;
+constexpr int kFastStack = 64;
+constexpr int kDeepStack = 1024;
+
+__device__ __forceinline__ float3 load_pt(const float *p, uint32_t i) {
+  return make_float3(p[3 * (size_t)i + 0], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2]);
+}
+
+struct Overflow {
+  uint32_t count;
+  uint32_t deepFailed;
+};
+
+template <int STACK>
+__device__ __forceinline__ bool fcp_one(const cubql_bvh3f &bvh, const float *points, float3 q,
+                                        float sqrMax, int32_t &best, float &bestD) {
+  best = -1;
+  bestD = sqrMax;
+  auto prim = [&](uint32_t pid) -> float {
+    float3 p = load_pt(points, pid);
+    float d = cubql_dev::sqr_dist_pt(p.x, p.y, p.z, q.x, q.y, q.z);
+    if (d < bestD || (d == bestD && best >= 0 && (int32_t)pid < best)) {
+      bestD = d;
+      best = (int32_t)pid;
+    }
+    return bestD;
+  };
+  return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, sqrMax, prim);
+}
+
+// fast path: 64-entry stack; queries that overflow go to the deep kernel.
+__global__ __launch_bounds__(kBlock) void k_fcp(cubql_bvh3f bvh, const float *points,
+                                                const float *queries, uint32_t nq, float sqrMax,
+                                                int32_t *outIds, float *outD, Overflow *ovf,
+                                                uint32_t *ovfList) {
+  const uint32_t qi = blockIdx.x * kBlock + threadIdx.x;
+  if (qi >= nq) return;
+  float3 q = load_pt(queries, qi);
+  int32_t best;
+  float bestD;
+  if (!fcp_one<kFastStack>(bvh, points, q, sqrMax, best, bestD)) {
+    uint32_t slot = atomicAdd(&ovf->count, 1u);
+    ovfList[slot] = qi;
+    return;
+  }
+  outIds[qi] = best;
+  if (outD) outD[qi] = best >= 0 ? bestD : INFINITY;
+}
+
+__global__ __launch_bounds__(64) void k_fcp_deep(cubql_bvh3f bvh, const float *points,
+                                                 const float *queries, float sqrMax,
+                                                 int32_t *outIds, float *outD, Overflow *ovf,
+                                                 const uint32_t *ovfList) {
+  const uint32_t n = ovf->count;
+  for (uint32_t i = blockIdx.x * 64 + threadIdx.x; i < n; i += gridDim.x * 64) {
+    const uint32_t qi = ovfList[i];
+    float3 q = load_pt(queries, qi);
+    int32_t best;
+    float bestD;
+    if (!fcp_one<kDeepStack>(bvh, points, q, sqrMax, best, bestD)) {
+      atomicAdd(&ovf->deepFailed, 1u);
+      best = -1;
+    }
+    outIds[qi] = best;
+    if (outD) outD[qi] = best >= 0 ? bestD : INFINITY;
+  }
+}
+
+// kNN with a register-resident candidate list of K entries (K >= k).
+template <int K, int STACK>
+__device__ __forceinline__ bool knn_one(const cubql_bvh3f &bvh, const float *points, float3 q,
+                                        int k, float sqrMax, int32_t (&ids)[K], float (&ds)[K],
+                                        int &found) {
+  found = 0;
+#pragma unroll
+  for (int j = 0; j < K; j++) { ids[j] = -1; ds[j] = INFINITY; }
+  float cull = sqrMax;
+  auto prim = [&](uint32_t pid) -> float {
+    float3 p = load_pt(points, pid);
+    float d = cubql_dev::sqr_dist_pt(p.x, p.y, p.z, q.x, q.y, q.z);
+    int32_t id = (int32_t)pid;
+    if (!(d < sqrMax)) return cull;
+    if (found == k) {
+      // reject unless (d,id) < (ds[k-1], ids[k-1])
+      float dl = ds[0];
+      int32_t il = ids[0];
+#pragma unroll
+      for (int j = 0; j < K; j++) if (j == k - 1) { dl = ds[j]; il = ids[j]; }
+      if (d > dl || (d == dl && id > il)) return cull;
+    } else {
+      found++;
+    }
+    // insertion: shift entries greater than (d,id) one slot to the right,
+    // over the first `found` slots (register-unrolled)
+    int32_t carryI = id;
+    float carryD = d;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      if (j < found) {
+        bool greater = ds[j] > carryD || (ds[j] == carryD && ids[j] > carryI);
+        if (greater) {
+          float td = ds[j];
+          int32_t ti = ids[j];
+          ds[j] = carryD;
+          ids[j] = carryI;
+          carryD = td;
+          carryI = ti;
+        }
+      }
+    }
+    if (found == k) {
+#pragma unroll
+      for (int j = 0; j < K; j++) if (j == k - 1) cull = ds[j];
+    }
+    return cull;
+  };
+  return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, sqrMax, prim);
+}
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_knn(cubql_bvh3f bvh, const float *points,
+                                                const float *queries, uint32_t nq, int k,
+                                                float sqrMax, int32_t *outIds, float *outD,
+                                                int32_t *outFound, Overflow *ovf,
+                                                uint32_t *ovfList) {
+  const uint32_t qi = blockIdx.x * kBlock + threadIdx.x;
+  if (qi >= nq) return;
+  float3 q = load_pt(queries, qi);
+  int32_t ids[K];
+  float ds[K];
+  int found;
+  if (!knn_one<K, kFastStack>(bvh, points, q, k, sqrMax, ids, ds, found)) {
+    uint32_t slot = atomicAdd(&ovf->count, 1u);
+    ovfList[slot] = qi;
+    return;
+  }
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    if (j < k) {
+      outIds[(size_t)qi * k + j] = ids[j];
+      outD[(size_t)qi * k + j] = ds[j];
+    }
+  }
+  if (outFound) outFound[qi] = found;
+}
+
+template <int K>
+__global__ __launch_bounds__(64) void k_knn_deep(cubql_bvh3f bvh, const float *points,
+                                                 const float *queries, int k, float sqrMax,
+                                                 int32_t *outIds, float *outD, int32_t *outFound,
+                                                 Overflow *ovf, const uint32_t *ovfList) {
+  const uint32_t n = ovf->count;
+  for (uint32_t i = blockIdx.x * 64 + threadIdx.x; i < n; i += gridDim.x * 64) {
+    const uint32_t qi = ovfList[i];
+    float3 q = load_pt(queries, qi);
+    int32_t ids[K];
+    float ds[K];
+    int found;
+    if (!knn_one<K, kDeepStack>(bvh, points, q, k, sqrMax, ids, ds, found)) {
+      atomicAdd(&ovf->deepFailed, 1u);
+      found = 0;
+      for (int j = 0; j < K; j++) { ids[j] = -1; ds[j] = INFINITY; }
+    }
+    for (int j = 0; j < K; j++) {
+      if (j < k) {
+        outIds[(size_t)qi * k + j] = ids[j];
+        outD[(size_t)qi * k + j] = ds[j];
+      }
+    }
+    if (outFound) outFound[qi] = found;
+  }
+}
+
+__global__ void k_fill_notfound(uint32_t nq, int k, int32_t *ids, float *d, int32_t *found) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t total = (size_t)nq * k;
+  for (; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    ids[i] = -1;
+    if (d) d[i] = INFINITY;
+    if (found && i < nq) found[i] = 0;
+  }
+}
+
+template <int K>
+int launch_knn(const cubql_bvh3f &bvh, const float *points, const float *queries, uint32_t nq,
+               int k, float sqrMax, int32_t *ids, float *d, int32_t *found, Overflow *ovf,
+               uint32_t *ovfList, hipStream_t s) {
+  k_knn<K><<<div_up(nq, kBlock), kBlock, 0, s>>>(bvh, points, queries, nq, k, sqrMax, ids, d, found,
+                                                  ovf, ovfList);
+  k_knn_deep<K><<<64, 64, 0, s>>>(bvh, points, queries, k, sqrMax, ids, d, found, ovf, ovfList);
+  CUBQL_LAUNCH_CHECK();
+  return CUBQL_OK;
+}
+
+}  // namespace pq
+}  // namespace cubql
+
+using namespace cubql;
+
+extern "C" int cubql_fcp_points_f3(const cubql_bvh3f *bvh, const float *d_points,
+                                   const float *d_queries, uint32_t numQueries,
+                                   float sqrMaxSearchDist, int32_t *d_outIDs, float *d_outSqrDist,
+                                   cubql_stream_t stream) {
+  clear_error();
+  if (!bvh || (!d_queries && numQueries) || (!d_outIDs && numQueries)) {
+    set_error("cubql_fcp_points_f3: null argument");
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  if (numQueries == 0) return CUBQL_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (bvh->numNodes == 0) {
+    pq::k_fill_notfound<<<256, 256, 0, s>>>(numQueries, 1, d_outIDs, d_outSqrDist, nullptr);
+    CUBQL_LAUNCH_CHECK();
+    return CUBQL_OK;
+  }
+  Scratch scr(nullptr, s);
+  pq::Overflow *ovf = nullptr;
+  uint32_t *ovfList = nullptr;
+  CUBQL_CHECK(scr.alloc(&ovf, 1));
+  CUBQL_CHECK(scr.alloc(&ovfList, numQueries));
+  CUBQL_CHECK(fill_u32((uint32_t *)ovf, 0u, 2, s));
+  pq::k_fcp<<<div_up(numQueries, pq::kBlock), pq::kBlock, 0, s>>>(
+      *bvh, d_points, d_queries, numQueries, sqrMaxSearchDist, d_outIDs, d_outSqrDist, ovf, ovfList);
+  pq::k_fcp_deep<<<64, 64, 0, s>>>(*bvh, d_points, d_queries, sqrMaxSearchDist, d_outIDs,
+                                   d_outSqrDist, ovf, ovfList);
+  CUBQL_LAUNCH_CHECK();
+  return CUBQL_OK;
+}
+
+extern "C" int cubql_knn_points_f3(const cubql_bvh3f *bvh, const float *d_points,
+                                   const float *d_queries, uint32_t numQueries, int k,
+                                   float sqrMaxSearchDist, int32_t *d_outIDs, float *d_outSqrDist,
+                                   int32_t *d_numFound, cubql_stream_t stream) {
+  clear_error();
+  if (!bvh || k < 1 || k > 64 || (numQueries && (!d_queries || !d_outIDs || !d_outSqrDist))) {
+    set_error("cubql_knn_points_f3: invalid argument (k=%d)", k);
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  if (numQueries == 0) return CUBQL_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (bvh->numNodes == 0) {
+    pq::k_fill_notfound<<<256, 256, 0, s>>>(numQueries, k, d_outIDs, d_outSqrDist, d_numFound);
+    CUBQL_LAUNCH_CHECK();
+    return CUBQL_OK;
+  }
+  Scratch scr(nullptr, s);
+  pq::Overflow *ovf = nullptr;
+  uint32_t *ovfList = nullptr;
+  CUBQL_CHECK(scr.alloc(&ovf, 1));
+  CUBQL_CHECK(scr.alloc(&ovfList, numQueries));
+  CUBQL_CHECK(fill_u32((uint32_t *)ovf, 0u, 2, s));
+  if (k <= 4) return pq::launch_knn<4>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, s);
+  if (k <= 8) return pq::launch_knn<8>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, s);
+  if (k <= 16) return pq::launch_knn<16>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, s);
+  if (k <= 32) return pq::launch_knn<32>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, s);
+  return pq::launch_knn<64>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, s);
+}

@@ -1,0 +1,12 @@
+// refit.h -- internal entry of the generic bottom-up refit (refit.hip).
+#pragma once
+#include "common.h"
+
+namespace cubql {
+// Recomputes every reachable node box of *bvh from the prim boxes.
+int refit_tree(cubql_bvh3f *bvh, const cubql_box3f *boxes, hipStream_t s,
+               const cubql_mem_resource *mem);
+// Spatial-median builder (sm_builder.hip); T = leaf threshold (>= 1).
+int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, hipStream_t s,
+             const cubql_mem_resource *mem);
+}  // namespace cubql

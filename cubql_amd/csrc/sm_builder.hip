@@ -1,0 +1,807 @@
+// sm_builder.hip -- adaptive spatial-median BVH builder for gfx950.
+//
+// Replaces cuBQL::gpuBuilder's spatial-median path (builder/cuda/
+// sm_builder.h:469-614, gpu_builder.h:31-67).  Split rules are those shared
+// by the reference GPU and CPU builders (sm_builder.h:185-220, 361-363;
+// cpu/spatialMedian.h:81-107): a node with count <= T is a leaf; otherwise
+// split the widest centroid-bounds axis (ties to the lower axis) at the
+// midpoint; a prim goes right iff centre[dim] >= pos.  Degenerate splits
+// follow the CPU builder exactly (one empty side -> split the node's range
+// in the middle), so the GPU tree equals cpu::spatialMedian's tree node for
+// node, up to node numbering.
+//
+// MI355X design (not the reference's prim-state + global sort scheme):
+//   * prims are kept physically partitioned: each open node owns a
+//     contiguous range of 16-byte items {cx,cy,cz,primID} in HBM;
+//   * one "partition" pass per tree level moves every prim of a large node
+//     to its child range (left stable, right reversed -- the CPU order),
+//     using a single-pass decoupled look-back scan over 2048-item tiles and
+//     accumulating child centroid bounds with one atomic per tile;
+//   * nodes of <= kSmall prims are finished by one wavefront each, entirely
+//     in LDS, which also computes all their boxes bottom-up;
+//   * the level loop runs on the device: kernels read their work counts from
+//     device memory, the host only checks for completion every few levels.
+#include "block_ops.h"
+#include "common.h"
+
+namespace cubql {
+namespace sm {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 8;
+constexpr int kTile = kBlock * kItems;  // 2048 prims per partition tile
+constexpr int kSmall = 256;             // max prims of a wave-finished node
+constexpr int kLevelsPerSync = 8;
+
+struct Counters {
+  uint32_t numValid;
+  int32_t rootCb[6];  // encoded centroid bounds of all valid prims
+  uint32_t allPoints; // every valid box is a point equal to its centre
+  uint32_t nodeCount;
+  uint32_t numLarge[2];
+  uint32_t numSmall;
+  uint32_t numFixup;
+  uint32_t ticket;
+  uint32_t numTiles;
+  uint32_t numTop;    // entries in the top-node list
+  uint32_t pad[3];
+};
+
+struct Seg {          // an open node with > kSmall prims
+  uint32_t begin, end, node, pad;
+  int32_t cb[6];      // encoded centroid bounds
+  uint32_t pad2[2];
+};
+static_assert(sizeof(Seg) == 48, "Seg");
+
+struct SmallSeg {     // a node finished by one wave
+  uint32_t begin, end, node, buf;
+};
+
+struct Ctx {
+  const cubql_box3f *boxes;
+  uint32_t n;              // number of input boxes
+  uint32_t T;              // leaf threshold
+  float4 *items[2];
+  uint32_t *primIDs;       // output primIDs (n entries)
+  Node *nodes;             // temp nodes (capacity 2*numValid+2)
+  Counters *ctr;
+  uint32_t *blockCount;    // init pass
+  Seg *segs[2];            // per-level large segments (ping-pong)
+  int32_t *childCb;        // [2*maxLarge][6]
+  uint32_t *segLeft;       // [maxLarge]
+  uint32_t *tileStart;     // [maxLarge+1]
+  int32_t *segDim;         // [maxLarge]
+  float *segPos;           // [maxLarge]
+  uint64_t *tileStatus;    // [maxTiles]
+  SmallSeg *small;         // [numValid]
+  uint32_t *fixup;         // [maxLarge] indices into next-level segs
+  uint32_t *topNodes;      // [numValid] inner nodes of the level phase
+  uint32_t *levelStart;    // [maxLevels+1] start of each level in topNodes
+  uint32_t maxLarge;
+};
+
+__device__ inline bool box_is_valid(const cubql_box3f &b) { return b.lower[0] <= b.upper[0]; }
+
+__device__ inline cubql_box3f load_box(const cubql_box3f *p, uint32_t i) {
+  const float2 *f = reinterpret_cast<const float2 *>(p + i);
+  float2 a = f[0], b = f[1], c = f[2];
+  cubql_box3f r;
+  r.lower[0] = a.x; r.lower[1] = a.y; r.lower[2] = b.x;
+  r.upper[0] = b.y; r.upper[1] = c.x; r.upper[2] = c.y;
+  return r;
+}
+
+__device__ inline float coord(const float4 &v, int d) {
+  return d == 0 ? v.x : (d == 1 ? v.y : v.z);
+}
+
+// Split choice (sm_builder.h:194-220 / spatialMedian.h:90-91): widest axis
+// by strict '>' (ties to the lower axis), pos = midpoint.
+__device__ inline void choose_split(const float lo[3], const float hi[3], int &dim, float &pos) {
+  float s0 = hi[0] - lo[0], s1 = hi[1] - lo[1], s2 = hi[2] - lo[2];
+  dim = s2 > fmaxf(s0, s1) ? 2 : (s1 > s0 ? 1 : 0);
+  float l = dim == 0 ? lo[0] : (dim == 1 ? lo[1] : lo[2]);
+  float h = dim == 0 ? hi[0] : (dim == 1 ? hi[1] : hi[2]);
+  pos = (l + h) / 2.f;
+}
+
+// block-wide min/max reduction of 12 floats (left lo/hi, right lo/hi)
+__device__ inline void block_reduce_bounds(float (&v)[12], float *smem /*12*NW*/) {
+  constexpr int NW = kBlock / kWave;
+  const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int j = 0; j < 12; j++) {
+    bool isMin = (j % 6) < 3;
+    v[j] = isMin ? wave_min(v[j]) : wave_max(v[j]);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < 12; j++) smem[j * NW + wid] = v[j];
+  }
+  __syncthreads();
+  if (threadIdx.x < 12) {
+    int j = threadIdx.x;
+    bool isMin = (j % 6) < 3;
+    float r = smem[j * NW];
+    for (int w = 1; w < NW; w++) r = isMin ? fminf(r, smem[j * NW + w]) : fmaxf(r, smem[j * NW + w]);
+    smem[j * NW] = r;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 12; j++) v[j] = smem[j * NW];
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------- init pass
+// Pass 1: per-tile valid count, root centroid bounds, point-data flag.
+__global__ __launch_bounds__(kBlock) void k_count(Ctx c) {
+  __shared__ float sb[12 * (kBlock / kWave)];
+  __shared__ uint32_t ss[kBlock / kWave + 1];
+  const uint32_t base = blockIdx.x * kTile;
+  uint32_t cnt = 0;
+  bool pts = true;
+  float v[12];
+  for (int j = 0; j < 3; j++) { v[j] = INFINITY; v[j + 3] = -INFINITY; v[j + 6] = INFINITY; v[j + 9] = -INFINITY; }
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    uint32_t i = base + k * kBlock + threadIdx.x;
+    if (i < c.n) {
+      cubql_box3f b = load_box(c.boxes, i);
+      if (box_is_valid(b)) {
+        cnt++;
+        for (int d = 0; d < 3; d++) {
+          float ce = (b.lower[d] + b.upper[d]) / 2.f;
+          v[d] = fminf(v[d], ce);
+          v[d + 3] = fmaxf(v[d + 3], ce);
+          pts = pts && (b.lower[d] == b.upper[d]) && (ce == b.lower[d]);
+        }
+      }
+    }
+  }
+  block_reduce_bounds(v, sb);
+  uint32_t tot;
+  block_excl_scan<kBlock>(cnt, ss, &tot);
+  bool allPts = __syncthreads_and(pts);
+  if (threadIdx.x == 0) {
+    c.blockCount[blockIdx.x] = tot;
+    if (tot) {
+      for (int d = 0; d < 3; d++) {
+        atomicMin(&c.ctr->rootCb[d], enc_f(v[d]));
+        atomicMax(&c.ctr->rootCb[d + 3], enc_f(v[d + 3]));
+      }
+    }
+    if (!allPts) atomicAnd(&c.ctr->allPoints, 0u);
+  }
+}
+
+// Pass 2 (single workgroup): exclusive scan of tile counts, root setup.
+__global__ __launch_bounds__(1024) void k_count_scan(Ctx c, uint32_t numBlocks) {
+  __shared__ uint32_t ss[1024 / kWave + 1];
+  uint32_t run = 0;
+  for (uint32_t b0 = 0; b0 < numBlocks; b0 += 1024) {
+    uint32_t i = b0 + threadIdx.x;
+    uint32_t v = i < numBlocks ? c.blockCount[i] : 0;
+    uint32_t tot;
+    uint32_t ex = block_excl_scan<1024>(v, ss, &tot);
+    if (i < numBlocks) c.blockCount[i] = run + ex;
+    run += tot;
+  }
+  if (threadIdx.x == 0) {
+    Counters *k = c.ctr;
+    k->numValid = run;
+    k->nodeCount = 2;
+    k->numLarge[0] = k->numLarge[1] = 0;
+    k->numSmall = 0;
+    k->numFixup = 0;
+    k->numTop = 0;
+    c.levelStart[0] = 0;
+    Node z;
+    for (int d = 0; d < 3; d++) { z.lo[d] = 0.f; z.hi[d] = 0.f; }
+    z.oc = 0;
+    c.nodes[1] = z;  // node 1 is never used (bvh.h:66-70)
+    if (run > kSmall && run > c.T) {
+      Seg s;
+      s.begin = 0; s.end = run; s.node = 0; s.pad = 0; s.pad2[0] = s.pad2[1] = 0;
+      for (int j = 0; j < 6; j++) s.cb[j] = k->rootCb[j];
+      c.segs[0][0] = s;
+      k->numLarge[0] = 1;
+    } else if (run > 0) {
+      SmallSeg s{0, run, 0, 0};
+      c.small[0] = s;
+      k->numSmall = 1;
+    }
+  }
+}
+
+// Pass 3: write compacted items (ascending primID) and the invalid tail.
+__global__ __launch_bounds__(kBlock) void k_write_items(Ctx c) {
+  __shared__ uint32_t sr[kItems * (kBlock / kWave) + 1];
+  const uint32_t base = blockIdx.x * kTile;
+  bool valid[kItems];
+  float4 it[kItems];
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    uint32_t i = base + k * kBlock + threadIdx.x;
+    valid[k] = false;
+    if (i < c.n) {
+      cubql_box3f b = load_box(c.boxes, i);
+      valid[k] = box_is_valid(b);
+      it[k] = make_float4((b.lower[0] + b.upper[0]) / 2.f, (b.lower[1] + b.upper[1]) / 2.f,
+                          (b.lower[2] + b.upper[2]) / 2.f, __uint_as_float(i));
+    }
+  }
+  uint32_t rank[kItems];
+  block_rank_striped<kBlock, kItems>(valid, rank, sr);
+  const uint32_t off = c.blockCount[blockIdx.x];
+  const uint32_t numValid = c.ctr->numValid;
+  const uint32_t invBefore = base - off;
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    uint32_t i = base + k * kBlock + threadIdx.x;
+    if (i >= c.n) continue;
+    uint32_t local = k * kBlock + threadIdx.x;
+    if (valid[k]) c.items[0][off + rank[k]] = it[k];
+    else c.primIDs[numValid + invBefore + (local - rank[k])] = i;
+  }
+}
+
+// ------------------------------------------------------------- level pass
+// Plan (single workgroup): split planes, tile offsets, child-bound reset.
+__global__ __launch_bounds__(1024) void k_plan(Ctx c, int level) {
+  __shared__ uint32_t ss[1024 / kWave + 1];
+  const int par = level & 1;
+  const uint32_t n = c.ctr->numLarge[par];
+  const Seg *segs = c.segs[par];
+  uint32_t run = 0;
+  for (uint32_t b0 = 0; b0 < n; b0 += 1024) {
+    uint32_t s = b0 + threadIdx.x;
+    uint32_t tiles = 0;
+    if (s < n) {
+      Seg g = segs[s];
+      tiles = div_up(g.end - g.begin, kTile);
+      float lo[3], hi[3];
+      for (int d = 0; d < 3; d++) { lo[d] = dec_f(g.cb[d]); hi[d] = dec_f(g.cb[d + 3]); }
+      int dim;
+      float pos;
+      choose_split(lo, hi, dim, pos);
+      c.segDim[s] = dim;
+      c.segPos[s] = pos;
+      c.segLeft[s] = 0;
+      for (int j = 0; j < 3; j++) {
+        c.childCb[(2 * s) * 6 + j] = enc_f(INFINITY);
+        c.childCb[(2 * s) * 6 + j + 3] = enc_f(-INFINITY);
+        c.childCb[(2 * s + 1) * 6 + j] = enc_f(INFINITY);
+        c.childCb[(2 * s + 1) * 6 + j + 3] = enc_f(-INFINITY);
+      }
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_scan<1024>(tiles, ss, &tot);
+    if (s < n) c.tileStart[s] = run + ex;
+    run += tot;
+  }
+  if (threadIdx.x == 0) {
+    c.tileStart[n] = run;
+    c.ctr->numTiles = run;
+    c.ctr->ticket = 0;
+  }
+}
+
+__device__ inline uint64_t status_word(uint32_t tag, uint32_t flag, uint32_t val) {
+  return ((uint64_t)tag << 34) | ((uint64_t)flag << 32) | val;
+}
+
+// Partition: each workgroup repeatedly claims a tile (ticket order, so a
+// tile only ever waits on tiles already claimed by running workgroups),
+// computes left/right ranks, resolves its prefix by decoupled look-back
+// among the tiles of its node and scatters the items.
+__global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
+  __shared__ uint32_t sr[kItems * (kBlock / kWave) + 1];
+  __shared__ float sb[12 * (kBlock / kWave)];
+  __shared__ uint32_t sTile, sSeg, sPrefix;
+  const int par = level & 1;
+  const uint32_t numTiles = c.ctr->numTiles;
+  const uint32_t nSeg = c.ctr->numLarge[par];
+  const Seg *segs = c.segs[par];
+  const float4 *src = c.items[par];
+  float4 *dst = c.items[par ^ 1];
+  const uint32_t tag = (uint32_t)level + 1;
+  while (true) {
+    if (threadIdx.x == 0) {
+      uint32_t t = atomicAdd(&c.ctr->ticket, 1u);
+      sTile = t;
+      if (t < numTiles) {
+        // segment = last s with tileStart[s] <= t
+        uint32_t lo = 0, hi = nSeg;
+        while (hi - lo > 1) {
+          uint32_t mid = (lo + hi) / 2;
+          if (c.tileStart[mid] <= t) lo = mid;
+          else hi = mid;
+        }
+        sSeg = lo;
+      }
+    }
+    __syncthreads();
+    const uint32_t t = sTile;
+    if (t >= numTiles) break;
+    const uint32_t s = sSeg;
+    const Seg g = segs[s];
+    const int dim = c.segDim[s];
+    const float pos = c.segPos[s];
+    const uint32_t k = t - c.tileStart[s];
+    const uint32_t nTilesSeg = c.tileStart[s + 1] - c.tileStart[s];
+    const uint32_t tb = g.begin + k * kTile;
+    const uint32_t te = min(tb + (uint32_t)kTile, g.end);
+
+    float4 it[kItems];
+    bool left[kItems];
+    bool in[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; j++) {
+      uint32_t i = tb + j * kBlock + threadIdx.x;
+      in[j] = i < te;
+      left[j] = false;
+      if (in[j]) {
+        it[j] = src[i];
+        left[j] = coord(it[j], dim) < pos;
+      }
+    }
+    uint32_t rank[kItems];
+    const uint32_t nLeft = block_rank_striped<kBlock, kItems>(left, rank, sr);
+
+    // decoupled look-back over the tiles of this node
+    if (threadIdx.x == 0) {
+      uint32_t prefix = 0;
+      if (k == 0) {
+        __hip_atomic_store(&c.tileStatus[t], status_word(tag, 2, nLeft), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        __hip_atomic_store(&c.tileStatus[t], status_word(tag, 1, nLeft), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t j = t - 1;
+        while (true) {
+          uint64_t w = __hip_atomic_load(&c.tileStatus[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          uint32_t wt = (uint32_t)(w >> 34);
+          uint32_t wf = (uint32_t)(w >> 32) & 3u;
+          if (wt != tag || wf == 0) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+          }
+          prefix += (uint32_t)w;
+          if (wf == 2) break;
+          j--;
+        }
+        __hip_atomic_store(&c.tileStatus[t], status_word(tag, 2, prefix + nLeft), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
+      sPrefix = prefix;
+      if (k == nTilesSeg - 1) c.segLeft[s] = prefix + nLeft;
+    }
+    __syncthreads();
+    const uint32_t P = sPrefix;
+    const uint32_t rightsBefore = k * kTile - P;
+
+    float v[12];
+    for (int j = 0; j < 3; j++) { v[j] = INFINITY; v[j + 3] = -INFINITY; v[j + 6] = INFINITY; v[j + 9] = -INFINITY; }
+#pragma unroll
+    for (int j = 0; j < kItems; j++) {
+      if (!in[j]) continue;
+      uint32_t local = j * kBlock + threadIdx.x;
+      uint32_t d;
+      int o = left[j] ? 0 : 6;
+      if (left[j]) d = g.begin + P + rank[j];
+      else d = g.end - 1 - (rightsBefore + (local - rank[j]));
+      dst[d] = it[j];
+      v[o + 0] = fminf(v[o + 0], it[j].x);
+      v[o + 1] = fminf(v[o + 1], it[j].y);
+      v[o + 2] = fminf(v[o + 2], it[j].z);
+      v[o + 3] = fmaxf(v[o + 3], it[j].x);
+      v[o + 4] = fmaxf(v[o + 4], it[j].y);
+      v[o + 5] = fmaxf(v[o + 5], it[j].z);
+    }
+    block_reduce_bounds(v, sb);
+    if (threadIdx.x < 12) {
+      int side = threadIdx.x / 6;
+      int j = threadIdx.x % 6;
+      float val = v[threadIdx.x];
+      bool nonEmpty = side == 0 ? (nLeft > 0) : (nLeft < te - tb);
+      if (nonEmpty) {
+        int32_t *dstCb = &c.childCb[(2 * s + side) * 6 + j];
+        if (j < 3) atomicMin(dstCb, enc_f(val));
+        else atomicMax(dstCb, enc_f(val));
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Emit (single workgroup): node records for the split nodes, deterministic
+// next-level / small lists, fixups for one-sided (degenerate) splits.
+__global__ __launch_bounds__(1024) void k_emit(Ctx c, int level) {
+  __shared__ uint32_t ss[1024 / kWave + 1];
+  __shared__ uint32_t sBase, sSmall, sTop;
+  const int par = level & 1;
+  const uint32_t n = c.ctr->numLarge[par];
+  const Seg *segs = c.segs[par];
+  Seg *next = c.segs[par ^ 1];
+  if (threadIdx.x == 0) {
+    sBase = c.ctr->nodeCount;
+    sSmall = c.ctr->numSmall;
+    sTop = c.ctr->numTop;
+  }
+  __syncthreads();
+  const uint32_t nodeBase = sBase, smallBase = sSmall, topBase = sTop;
+  uint32_t runLarge = 0, runSmall = 0, runFix = 0;
+  for (uint32_t b0 = 0; b0 < n; b0 += 1024) {
+    uint32_t s = b0 + threadIdx.x;
+    uint32_t nLarge = 0, nSmallC = 0, nFix = 0;
+    Seg g;
+    uint32_t cb[2], ce[2];
+    bool large[2] = {false, false};
+    bool oneSided = false;
+    if (s < n) {
+      g = segs[s];
+      uint32_t size = g.end - g.begin;
+      uint32_t nl = c.segLeft[s];
+      oneSided = (nl == 0 || nl == size);
+      uint32_t mid = oneSided ? (uint32_t)(((uint64_t)g.begin + g.end) / 2) : g.begin + nl;
+      cb[0] = g.begin; ce[0] = mid;
+      cb[1] = mid; ce[1] = g.end;
+      for (int side = 0; side < 2; side++) {
+        uint32_t sz = ce[side] - cb[side];
+        large[side] = sz > (uint32_t)kSmall && sz > c.T;
+        if (large[side]) { nLarge++; if (oneSided) nFix++; }
+        else nSmallC++;
+      }
+    }
+    uint32_t totL, totS, totF;
+    uint32_t exL = block_excl_scan<1024>(nLarge, ss, &totL);
+    uint32_t exS = block_excl_scan<1024>(nSmallC, ss, &totS);
+    uint32_t exF = block_excl_scan<1024>(nFix, ss, &totF);
+    if (s < n) {
+      uint32_t child = nodeBase + 2 * s;
+      Node *nd = &c.nodes[g.node];
+      nd->oc = pack_oc(child, 0);
+      c.topNodes[topBase + s] = g.node;
+      uint32_t li = runLarge + exL, si = runSmall + exS, fi = runFix + exF;
+      for (int side = 0; side < 2; side++) {
+        if (large[side]) {
+          Seg ns;
+          ns.begin = cb[side]; ns.end = ce[side]; ns.node = child + side; ns.pad = 0;
+          ns.pad2[0] = ns.pad2[1] = 0;
+          for (int j = 0; j < 6; j++) ns.cb[j] = c.childCb[(2 * s + side) * 6 + j];
+          next[li] = ns;
+          if (oneSided) c.fixup[fi++] = li;
+          li++;
+        } else {
+          SmallSeg ss2{cb[side], ce[side], child + side, (uint32_t)(par ^ 1)};
+          c.small[smallBase + si] = ss2;
+          si++;
+        }
+      }
+    }
+    runLarge += totL;
+    runSmall += totS;
+    runFix += totF;
+  }
+  if (threadIdx.x == 0) {
+    c.ctr->nodeCount = nodeBase + 2 * n;
+    c.ctr->numSmall = smallBase + runSmall;
+    c.ctr->numLarge[par ^ 1] = runLarge;
+    c.ctr->numLarge[par] = 0;
+    c.ctr->numFixup = runFix;
+    c.ctr->numTop = topBase + n;
+    c.levelStart[level + 1] = topBase + n;
+  }
+}
+
+// Fixup: exact centroid bounds of large children of one-sided splits.
+__global__ __launch_bounds__(kBlock) void k_fixup(Ctx c, int level) {
+  __shared__ float sb[12 * (kBlock / kWave)];
+  const int par = level & 1;
+  const uint32_t nf = c.ctr->numFixup;
+  Seg *next = c.segs[par ^ 1];
+  const float4 *items = c.items[par ^ 1];
+  for (uint32_t f = blockIdx.x; f < nf; f += gridDim.x) {
+    Seg &g = next[c.fixup[f]];
+    float v[12];
+    for (int j = 0; j < 3; j++) { v[j] = INFINITY; v[j + 3] = -INFINITY; v[j + 6] = INFINITY; v[j + 9] = -INFINITY; }
+    for (uint32_t i = g.begin + threadIdx.x; i < g.end; i += kBlock) {
+      float4 it = items[i];
+      v[0] = fminf(v[0], it.x); v[1] = fminf(v[1], it.y); v[2] = fminf(v[2], it.z);
+      v[3] = fmaxf(v[3], it.x); v[4] = fmaxf(v[4], it.y); v[5] = fmaxf(v[5], it.z);
+    }
+    block_reduce_bounds(v, sb);
+    if (threadIdx.x < 6) g.cb[threadIdx.x] = enc_f(v[threadIdx.x]);
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ small nodes
+struct Frame { uint32_t node, child, be, mp; };  // be = b | e<<16, mp = mid | phase<<16
+
+__device__ inline void leaf_bounds_items(const float4 *it, uint32_t b, uint32_t e, const Ctx &c,
+                                         bool pts, float (&bb)[6]) {
+  for (int j = 0; j < 3; j++) { bb[j] = INFINITY; bb[j + 3] = -INFINITY; }
+  for (uint32_t i = b + lane_id(); i < e; i += kWave) {
+    float4 v = it[i];
+    if (pts) {
+      bb[0] = fminf(bb[0], v.x); bb[1] = fminf(bb[1], v.y); bb[2] = fminf(bb[2], v.z);
+      bb[3] = fmaxf(bb[3], v.x); bb[4] = fmaxf(bb[4], v.y); bb[5] = fmaxf(bb[5], v.z);
+    } else {
+      cubql_box3f box = load_box(c.boxes, __float_as_uint(v.w));
+      for (int d = 0; d < 3; d++) {
+        bb[d] = fminf(bb[d], box.lower[d]);
+        bb[d + 3] = fmaxf(bb[d + 3], box.upper[d]);
+      }
+    }
+  }
+  for (int j = 0; j < 3; j++) { bb[j] = wave_min(bb[j]); bb[j + 3] = wave_max(bb[j + 3]); }
+}
+
+__device__ inline void write_node_bounds(Node *nd, const float (&bb)[6]) {
+  uint32_t l = lane_id();
+  float v = bb[0];
+  for (int j = 1; j < 6; j++) if ((int)l == j) v = bb[j];
+  if (l < 6) (&nd->lo[0])[l] = v;
+}
+
+// One wavefront per node of <= kSmall prims (or any leaf-sized node):
+// builds its whole subtree in LDS with the CPU builder's recursion order
+// (cpu/spatialMedian.h:73-115), writes primIDs in leaf order and all boxes
+// bottom-up.
+__global__ __launch_bounds__(kWave) void k_small(Ctx c) {
+  __shared__ float4 sIt[kSmall];
+  __shared__ float4 sAlt[kSmall];
+  __shared__ Frame sFr[kSmall + 1];
+  __shared__ float sBs[(kSmall + 1) * 6];
+  const uint32_t lane = lane_id();
+  const uint32_t T = c.T;
+  const bool pts = c.ctr->allPoints != 0;
+  const uint32_t nSmall = c.ctr->numSmall;
+  for (uint32_t w = blockIdx.x; w < nSmall; w += gridDim.x) {
+    const SmallSeg sg = c.small[w];
+    const float4 *gItems = c.items[sg.buf];
+    const uint32_t gb = sg.begin;
+    const uint32_t m = sg.end - sg.begin;
+    if (m <= T) {
+      // whole node is one leaf (any size up to T)
+      for (uint32_t i = lane; i < m; i += kWave) c.primIDs[gb + i] = __float_as_uint(gItems[gb + i].w);
+      float bb[6];
+      leaf_bounds_items(gItems, gb, gb + m, c, pts, bb);
+      Node *nd = &c.nodes[sg.node];
+      write_node_bounds(nd, bb);
+      if (lane == 0) nd->oc = pack_oc(gb, m);
+      continue;
+    }
+    for (uint32_t i = lane; i < m; i += kWave) sIt[i] = gItems[gb + i];
+    if (lane == 0) sFr[0] = Frame{sg.node, 0u, 0u | (m << 16), 0u};
+    __syncthreads();
+    int sp = 1, bsp = 0;
+    while (sp > 0) {
+      Frame f = sFr[sp - 1];
+      const uint32_t b = f.be & 0xffffu, e = f.be >> 16;
+      const uint32_t phase = f.mp >> 16, mid0 = f.mp & 0xffffu;
+      Node *nd = &c.nodes[f.node];
+      if (phase == 0) {
+        const uint32_t cnt = e - b;
+        if (cnt <= T) {
+          for (uint32_t i = b + lane; i < e; i += kWave) c.primIDs[gb + i] = __float_as_uint(sIt[i].w);
+          float bb[6];
+          leaf_bounds_items(sIt, b, e, c, pts, bb);
+          write_node_bounds(nd, bb);
+          if (lane == 0) nd->oc = pack_oc(gb + b, cnt);
+          {
+            float v = bb[0];
+            for (int j = 1; j < 6; j++) if ((int)lane == j) v = bb[j];
+            if (lane < 6) sBs[bsp * 6 + lane] = v;
+          }
+          bsp++;
+          sp--;
+          __syncthreads();
+          continue;
+        }
+        // centroid bounds of this range
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t i = b + lane; i < e; i += kWave) {
+          float4 v = sIt[i];
+          lo[0] = fminf(lo[0], v.x); lo[1] = fminf(lo[1], v.y); lo[2] = fminf(lo[2], v.z);
+          hi[0] = fmaxf(hi[0], v.x); hi[1] = fmaxf(hi[1], v.y); hi[2] = fmaxf(hi[2], v.z);
+        }
+        for (int d = 0; d < 3; d++) { lo[d] = wave_min(lo[d]); hi[d] = wave_max(hi[d]); }
+        int dim;
+        float pos;
+        choose_split(lo, hi, dim, pos);
+        uint32_t nL = 0, nR = 0;
+        for (uint32_t c0 = b; c0 < e; c0 += kWave) {
+          uint32_t i = c0 + lane;
+          bool act = i < e;
+          float4 v = act ? sIt[i] : make_float4(0, 0, 0, 0);
+          bool isL = act && (coord(v, dim) < pos);
+          uint64_t mL = __ballot(isL);
+          uint64_t mR = __ballot(act && !isL);
+          if (isL) sAlt[b + nL + mbcnt(mL)] = v;
+          else if (act) sAlt[e - 1 - (nR + mbcnt(mR))] = v;
+          nL += (uint32_t)__popcll(mL);
+          nR += (uint32_t)__popcll(mR);
+        }
+        uint32_t mid;
+        if (nL != 0 && nR != 0) mid = b + nL;
+        else mid = (uint32_t)(((uint64_t)gb + b + gb + e) / 2 - gb);  // spatialMedian.h:107, global indices
+        __syncthreads();
+        for (uint32_t i = b + lane; i < e; i += kWave) sIt[i] = sAlt[i];
+        uint32_t child = 0;
+        if (lane == 0) child = atomicAdd(&c.ctr->nodeCount, 2u);
+        child = __shfl(child, 0, kWave);
+        if (lane == 0) {
+          nd->oc = pack_oc(child, 0);
+          sFr[sp - 1] = Frame{f.node, child, f.be, mid | (1u << 16)};
+          sFr[sp] = Frame{child, 0u, b | (mid << 16), 0u};
+        }
+        sp++;
+        __syncthreads();
+      } else if (phase == 1) {
+        if (lane == 0) {
+          sFr[sp - 1].mp = mid0 | (2u << 16);
+          sFr[sp] = Frame{f.child + 1, 0u, mid0 | (e << 16), 0u};
+        }
+        sp++;
+        __syncthreads();
+      } else {
+        // both children done: union of the two top bounds
+        float v = 0.f;
+        if (lane < 6) {
+          float r = sBs[(bsp - 1) * 6 + lane], l = sBs[(bsp - 2) * 6 + lane];
+          v = lane < 3 ? fminf(l, r) : fmaxf(l, r);
+        }
+        __syncthreads();
+        if (lane < 6) {
+          sBs[(bsp - 2) * 6 + lane] = v;
+          (&nd->lo[0])[lane] = v;
+        }
+        bsp--;
+        sp--;
+        __syncthreads();
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Bounds of the level-phase inner nodes, deepest level first.
+__global__ void k_refit_top(Ctx c, int level) {
+  const uint32_t b = c.levelStart[level], e = c.levelStart[level + 1];
+  for (uint32_t i = b + blockIdx.x * blockDim.x + threadIdx.x; i < e; i += gridDim.x * blockDim.x) {
+    Node *nd = &c.nodes[c.topNodes[i]];
+    uint64_t off = oc_offset(nd->oc);
+    const Node l = c.nodes[off], r = c.nodes[off + 1];
+    for (int d = 0; d < 3; d++) {
+      nd->lo[d] = fminf(l.lo[d], r.lo[d]);
+      nd->hi[d] = fmaxf(l.hi[d], r.hi[d]);
+    }
+  }
+}
+
+__global__ void k_init_counters(Counters *k) {
+  if (threadIdx.x == 0) {
+    for (int d = 0; d < 3; d++) {
+      k->rootCb[d] = enc_f(INFINITY);
+      k->rootCb[d + 3] = enc_f(-INFINITY);
+    }
+    k->allPoints = 1;
+  }
+}
+
+}  // namespace sm
+
+// pinned host mirror for the few completion checks
+static uint32_t *host_words() {
+  static thread_local uint32_t *p = nullptr;
+  if (!p) {
+    if (hipHostMalloc((void **)&p, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) p = nullptr;
+  }
+  return p;
+}
+
+int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, hipStream_t s,
+             const cubql_mem_resource *mem) {
+  using namespace sm;
+  Scratch scr(mem, s);
+  Ctx c{};
+  c.boxes = boxes;
+  c.n = n;
+  c.T = T;
+  const uint32_t numBlocks = div_up(n, kTile);
+  c.maxLarge = n / (kSmall + 1) + 2;
+  const uint32_t maxTiles = div_up(n, kTile) + c.maxLarge + 1;
+  const uint32_t maxLevels = 4096;
+  CUBQL_CHECK(scr.alloc(&c.items[0], n));
+  CUBQL_CHECK(scr.alloc(&c.items[1], n));
+  CUBQL_CHECK(scr.alloc(&c.nodes, 2 * (size_t)n + 2));
+  CUBQL_CHECK(scr.alloc(&c.ctr, 1));
+  CUBQL_CHECK(scr.alloc(&c.blockCount, numBlocks + 1));
+  CUBQL_CHECK(scr.alloc(&c.segs[0], c.maxLarge));
+  CUBQL_CHECK(scr.alloc(&c.segs[1], c.maxLarge));
+  CUBQL_CHECK(scr.alloc(&c.childCb, 12 * (size_t)c.maxLarge));
+  CUBQL_CHECK(scr.alloc(&c.segLeft, c.maxLarge));
+  CUBQL_CHECK(scr.alloc(&c.tileStart, c.maxLarge + 1));
+  CUBQL_CHECK(scr.alloc(&c.segDim, c.maxLarge));
+  CUBQL_CHECK(scr.alloc(&c.segPos, c.maxLarge));
+  CUBQL_CHECK(scr.alloc(&c.tileStatus, maxTiles));
+  CUBQL_CHECK(scr.alloc(&c.small, n + 1));
+  CUBQL_CHECK(scr.alloc(&c.fixup, c.maxLarge));
+  CUBQL_CHECK(scr.alloc(&c.topNodes, n + 1));
+  CUBQL_CHECK(scr.alloc(&c.levelStart, maxLevels + 2));
+
+  uint32_t *primIDs = nullptr;
+  CUBQL_CHECK(mem_alloc(mem, (void **)&primIDs, (size_t)n * sizeof(uint32_t), s));
+  c.primIDs = primIDs;
+  auto fail = [&](int st) {
+    mem_free(mem, primIDs, s);
+    return st;
+  };
+
+  k_init_counters<<<1, 64, 0, s>>>(c.ctr);
+  int st = fill_u32((uint32_t *)c.tileStatus, 0u, 2 * (size_t)maxTiles, s);
+  if (st) return fail(st);
+  k_count<<<numBlocks, kBlock, 0, s>>>(c);
+  k_count_scan<<<1, 1024, 0, s>>>(c, numBlocks);
+  k_write_items<<<numBlocks, kBlock, 0, s>>>(c);
+  if (hipGetLastError() != hipSuccess) return fail((set_error("init launch failed"), CUBQL_ERR_HIP));
+
+  uint32_t *hw = host_words();
+  if (!hw) return fail((set_error("hipHostMalloc failed"), CUBQL_ERR_HIP));
+  int dev = 0, numCU = 256;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&numCU, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    numCU = 256;
+  const uint32_t partGrid = std::min<uint32_t>(maxTiles, (uint32_t)numCU * 8);
+  int level = 0;
+  while (true) {
+    for (int i = 0; i < kLevelsPerSync && level < (int)maxLevels; i++, level++) {
+      k_plan<<<1, 1024, 0, s>>>(c, level);
+      k_partition<<<partGrid, kBlock, 0, s>>>(c, level);
+      k_emit<<<1, 1024, 0, s>>>(c, level);
+      k_fixup<<<64, kBlock, 0, s>>>(c, level);
+    }
+    if (hipGetLastError() != hipSuccess) return fail((set_error("level launch failed"), CUBQL_ERR_HIP));
+    if (hipMemcpyAsync(hw, c.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return fail((set_error("level sync failed"), CUBQL_ERR_HIP));
+    const Counters *hc = (const Counters *)hw;
+    if (hc->numLarge[0] == 0 && hc->numLarge[1] == 0) break;
+    if (level >= (int)maxLevels) return fail((set_error("tree deeper than %u levels", maxLevels), CUBQL_ERR_INTERNAL));
+  }
+  const uint32_t numSmall = ((const Counters *)hw)->numSmall;
+  const uint32_t numValid = ((const Counters *)hw)->numValid;
+  // number of levels that actually had work
+  const uint32_t numTop = ((const Counters *)hw)->numTop;
+  if (numSmall) {
+    uint32_t g = std::min<uint32_t>(numSmall, (uint32_t)numCU * 16);
+    k_small<<<g, kWave, 0, s>>>(c);
+  }
+  if (numTop) {
+    for (int l = level - 1; l >= 0; l--) k_refit_top<<<64, 256, 0, s>>>(c, l);
+  }
+  if (hipGetLastError() != hipSuccess) return fail((set_error("finish launch failed"), CUBQL_ERR_HIP));
+  if (hipMemcpyAsync(hw, c.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return fail((set_error("final sync failed"), CUBQL_ERR_HIP));
+  const uint32_t numNodes = numValid ? ((const Counters *)hw)->nodeCount : 0;
+  Node *nodes = nullptr;
+  st = mem_alloc(mem, (void **)&nodes, std::max<size_t>(numNodes, 2) * sizeof(Node), s);
+  if (st) return fail(st);
+  if (numNodes) {
+    if (hipMemcpyAsync(nodes, c.nodes, (size_t)numNodes * sizeof(Node), hipMemcpyDeviceToDevice, s) != hipSuccess) {
+      mem_free(mem, nodes, s);
+      return fail((set_error("node copy failed"), CUBQL_ERR_HIP));
+    }
+  }
+  bvh->nodes = (cubql_node3f *)nodes;
+  bvh->numNodes = numNodes;
+  bvh->primIDs = primIDs;
+  bvh->numPrims = n;
+  return CUBQL_OK;
+}
+
+}  // namespace cubql

@@ -1,0 +1,146 @@
+/* cubql_capi.h -- C ABI of the MI355X (gfx950) BVH build-and-query library.
+ *
+ * Plain C: pointers, sizes and status codes; no C++ or torch types.  Every
+ * entry point names the reference interface it replaces (file:line in the
+ * cuBQL sources).  All device pointers are HIP device (or managed) memory.
+ * Work is enqueued on the given stream; no entry point blocks the host
+ * unless stated.  Return value: 0 on success, a negative cubql_status on
+ * failure (cubql_last_error_string() gives the message for the calling
+ * thread).  No exception or signal crosses this boundary.
+ */
+#ifndef CUBQL_AMD_CAPI_H
+#define CUBQL_AMD_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *cubql_stream_t; /* == hipStream_t */
+
+/* ---------------------------------------------------------------- types */
+
+/* box_t<float,3> (math/box.h:60-196): lower xyz then upper xyz, 24 bytes.
+   A box with lower[0] > upper[0] marks an invalid (ignored) primitive. */
+typedef struct { float lower[3]; float upper[3]; } cubql_box3f;
+
+/* BinaryBVH<float,3>::Node (bvh.h:78-102): bounds plus one 64-bit word with
+   offset in bits [0,48) and count in bits [48,64).  count==0: inner node,
+   children at offset and offset+1; count>0: leaf with primIDs
+   [offset, offset+count).  32 bytes, 16-byte aligned. */
+typedef struct {
+  cubql_box3f bounds;
+  uint64_t offsetAndCountBits;
+} cubql_node3f;
+
+/* BinaryBVH<float,3> (bvh.h:71-111): byte-identical, 32 bytes.  Node 0 is
+   the root; node 1 is unused by the GPU builders. */
+typedef struct {
+  cubql_node3f *nodes;
+  uint32_t numNodes;
+  uint32_t *primIDs;
+  uint32_t numPrims;
+} cubql_bvh3f;
+
+/* BuildConfig (bvh.h:31-64), byte-identical: maxAllowedLeafSize,
+   makeLeafThreshold, buildMethod (0 spatial median, 1 SAH, 2 ELH). */
+typedef struct {
+  int maxAllowedLeafSize;
+  int makeLeafThreshold;
+  int buildMethod;
+} cubql_build_config;
+
+enum { CUBQL_SPATIAL_MEDIAN = 0, CUBQL_SAH = 1, CUBQL_ELH = 2 };
+
+/* GpuMemoryResource (builder/cuda.h:31-34) as a callback table.  A NULL
+   resource means hipMallocAsync/hipFreeAsync on the device's default pool
+   with an unbounded release threshold (builder/cuda.h:52-67).  Callbacks
+   return 0 on success. */
+typedef struct {
+  int (*malloc_fn)(void **ptr, size_t size, cubql_stream_t s, void *user);
+  int (*free_fn)(void *ptr, cubql_stream_t s, void *user);
+  void *user;
+} cubql_mem_resource;
+
+typedef enum {
+  CUBQL_OK = 0,
+  CUBQL_ERR_INVALID_ARG = -1,
+  CUBQL_ERR_HIP = -2,
+  CUBQL_ERR_ALLOC = -3,
+  CUBQL_ERR_UNSUPPORTED = -4,
+  CUBQL_ERR_INTERNAL = -5
+} cubql_status;
+
+/* Message for the last failing call on this thread ("" if none). */
+const char *cubql_last_error_string(void);
+/* Library version string. */
+const char *cubql_version(void);
+
+/* ------------------------------------------------------------- builders */
+
+/* cuBQL::gpuBuilder(BinaryBVH<float,3>&, const box3f*, uint32_t, BuildConfig,
+   cudaStream_t, GpuMemoryResource&)  -- builder/cuda.h:91-99,
+   dispatch gpu_builder.h:31-67.  buildMethod SPATIAL_MEDIAN runs the
+   adaptive spatial-median builder (sm_builder.h:469-614) and SAH the binned
+   SAH builder (sah_builder.h:396-537); makeLeafThreshold 0 means 1.
+   numBoxes == 0 leaves *bvh untouched.  Spatial median: bvh->numPrims =
+   numBoxes, invalid prims listed after all valid ones (sm_builder.h:590).
+   Blocks the host once at the end to read back the node count. */
+int cubql_gpu_builder_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes,
+                         uint32_t numBoxes, const cubql_build_config *cfg,
+                         cubql_stream_t s, const cubql_mem_resource *mem);
+
+/* cuBQL::cuda::radixBuilder -- builder/cuda.h:134-140, radix.h:591-771.
+   63-bit Morton keys (21 bits per axis, radix.h:30-33).  bvh->numPrims =
+   number of valid prims; primIDs in Morton order, ties by primID. */
+int cubql_radix_builder_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes,
+                           uint32_t numPrims, const cubql_build_config *cfg,
+                           cubql_stream_t s, const cubql_mem_resource *mem);
+
+/* Same as cubql_radix_builder_f3 with 30-bit keys (10 bits per axis): the
+   reference's radix.h:34-38 variant and the key width of
+   rebinMortonBuilder.h:33-36. */
+int cubql_radix30_builder_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes,
+                             uint32_t numPrims, const cubql_build_config *cfg,
+                             cubql_stream_t s, const cubql_mem_resource *mem);
+
+/* cuBQL::cuda::free(BinaryBVH&, stream, GpuMemoryResource&) --
+   builder/cuda.h:155-161, gpu_builder.h:251-261.  Frees nodes and primIDs
+   with the same resource used to build, then synchronises the stream. */
+int cubql_free_f3(cubql_bvh3f *bvh, cubql_stream_t s,
+                  const cubql_mem_resource *mem);
+
+/* Bottom-up refit of all node boxes from prim boxes (sm_builder.h:616-693). */
+int cubql_refit_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes,
+                   cubql_stream_t s, const cubql_mem_resource *mem);
+
+/* -------------------------------------------------------------- queries */
+
+/* Batched cuBQL::points::findClosest (queries/points/findClosest.h:49-62,
+   181-226 over shrinkingRadiusQuery::forEachPrim, traversal/
+   shrinkingRadiusQuery.h:409-508, 766-801).  For each query i: outIDs[i] =
+   closest data point within sqrt(sqrMaxSearchDist) or -1; outSqrDist[i]
+   = its squared distance (x*x+y*y)+z*z in float32 (or +inf).  Among equal
+   distances the lowest primID is returned.  outSqrDist may be NULL.
+   points/queries are float3 arrays (12-byte stride). */
+int cubql_fcp_points_f3(const cubql_bvh3f *bvh, const float *d_points,
+                        const float *d_queries, uint32_t numQueries,
+                        float sqrMaxSearchDist, int32_t *d_outIDs,
+                        float *d_outSqrDist, cubql_stream_t s);
+
+/* Batched cuBQL::points::findKNN (queries/points/knn.h:49-116 with
+   knn::insert_linear, queries/common/knn.h:57-78).  Output rows of k
+   entries sorted by (sqrDist, primID); unfilled entries are -1/+inf.
+   d_numFound may be NULL.  1 <= k <= 64. */
+int cubql_knn_points_f3(const cubql_bvh3f *bvh, const float *d_points,
+                        const float *d_queries, uint32_t numQueries, int k,
+                        float sqrMaxSearchDist, int32_t *d_outIDs,
+                        float *d_outSqrDist, int32_t *d_numFound,
+                        cubql_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUBQL_AMD_CAPI_H */

@@ -1,0 +1,286 @@
+"""oracle -- TEST INFRASTRUCTURE ONLY.
+
+numpy/ctypes front end for the CPU checker libraries built by oracle/Makefile:
+
+* ``liboracle.so``  -- our C++ restatement of the reference semantics
+  (oracle.cpp; each routine cites the reference file:line it follows);
+* ``_ref/libcubql_ref.so`` -- the reference's own host path (cpu::spatialMedian,
+  host points::findClosest / findKNN, UniformPointGenerator), compiled from the
+  reference sources in place.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+import this package, and only as the checker.  The product (``cubql_amd``)
+never imports it.
+
+Parity status per function (what pins it) is listed in the module docstrings
+of tests/test_oracle.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+NODE_DTYPE = np.dtype([("lo", "<f4", 3), ("hi", "<f4", 3), ("oc", "<u8")])
+assert NODE_DTYPE.itemsize == 32
+
+_lib = None
+_ref = None
+
+
+def build(quiet: bool = True) -> None:
+    """Compile the oracle (and the reference shim when the reference is present)."""
+    out = subprocess.run(["make", "-C", HERE, "-j4"], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+    if not quiet:
+        print(out.stdout)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _lib = C.CDLL(path)
+        V, I, I64, F, U32 = C.c_void_p, C.c_int, C.c_int64, C.c_float, C.c_uint32
+        sig = {
+            "orc_lcg_uniform": (None, [U32, I64, I, V, V, V]),
+            "orc_sm_build": (I, [V, I, I, V, V, V, V]),
+            "orc_refit": (None, [V, I, V, V]),
+            "orc_check_tree": (I, [V, I, V, I, V, I, I, V, V]),
+            "orc_canonical": (I64, [V, V, V, I64]),
+            "orc_fcp": (None, [V, V, V, V, I64, F, I, V, V, V]),
+            "orc_knn": (None, [V, V, V, V, I64, I, F, I, V, V, V]),
+            "orc_fcp_brute": (None, [V, I64, V, I64, V, V]),
+            "orc_fcp_brute_f64": (None, [V, I64, V, I64, V, V]),
+            "orc_knn_brute": (None, [V, I64, V, I64, I, V, V]),
+            "orc_morton_keys": (I, [V, I, I, V, V]),
+            "orc_sort_pairs": (None, [V, V, I]),
+            "orc_radix_hierarchy": (I, [V, I, I, V]),
+            "orc_atomicbox_encode": (C.c_int32, [F]),
+            "orc_atomicbox_decode": (F, [C.c_int32]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(_lib, name)
+            fn.restype = res
+            fn.argtypes = args
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(os.path.join(HERE, "_ref", "libcubql_ref.so"))
+
+
+def ref():
+    """The compiled reference host path (raises if it was never built)."""
+    global _ref
+    if _ref is None:
+        path = os.path.join(HERE, "_ref", "libcubql_ref.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        _ref = C.CDLL(path)
+        V, I, F, D = C.c_void_p, C.c_int, C.c_float, C.c_double
+        sig = {
+            "ref_uniform_points": (I, [I, I, D, D, V]),
+            "ref_cpu_build": (I, [V, I, I, V, V, V, V]),
+            "ref_fcp": (None, [V, I, V, I, V, V, I, F, V, V]),
+            "ref_knn": (None, [V, I, V, I, V, V, I, I, F, V, V, V]),
+            "ref_time_cpu_build": (D, [V, I, I]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(_ref, name)
+            fn.restype = res
+            fn.argtypes = args
+    return _ref
+
+
+# ----------------------------------------------------------------------------
+# helpers
+def points_to_boxes(points: np.ndarray) -> np.ndarray:
+    """box3f().including(p) for each point (closestPoint.cu:32-39): [n,6] f32."""
+    p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    return np.ascontiguousarray(np.concatenate([p, p], axis=1))
+
+
+def lcg_uniform(seed: int, n: int, lo=0.0, hi=1.0, dims: int = 3) -> np.ndarray:
+    lo_a = np.full(dims, lo, dtype=np.float64)
+    hi_a = np.full(dims, hi, dtype=np.float64)
+    out = np.empty((n, dims), dtype=np.float32)
+    lib().orc_lcg_uniform(seed & 0xFFFFFFFF, n, dims, _p(lo_a), _p(hi_a), _p(out))
+    return out
+
+
+class Tree:
+    """A BVH in host memory: nodes (NODE_DTYPE) + primIDs (uint32)."""
+
+    def __init__(self, nodes: np.ndarray, prim_ids: np.ndarray):
+        self.nodes = np.ascontiguousarray(nodes.view(NODE_DTYPE).reshape(-1))
+        self.prim_ids = np.ascontiguousarray(prim_ids, dtype=np.uint32)
+
+    @property
+    def counts(self):
+        return (self.nodes["oc"] >> np.uint64(48)).astype(np.int64)
+
+    @property
+    def offsets(self):
+        return (self.nodes["oc"] & np.uint64((1 << 48) - 1)).astype(np.int64)
+
+
+def sm_build(boxes: np.ndarray, leaf_threshold: int) -> Tree:
+    b = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    n = b.shape[0]
+    nodes = np.zeros(2 * n + 2, dtype=NODE_DTYPE)
+    ids = np.zeros(max(n, 1), dtype=np.uint32)
+    nn = C.c_int()
+    npr = C.c_int()
+    lib().orc_sm_build(_p(b), n, leaf_threshold, _p(nodes), _p(ids), C.byref(nn), C.byref(npr))
+    return Tree(nodes[: nn.value], ids[: npr.value])
+
+
+def ref_sm_build(boxes: np.ndarray, leaf_threshold: int) -> Tree:
+    b = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    n = b.shape[0]
+    nodes = np.zeros(2 * n + 2, dtype=NODE_DTYPE)
+    ids = np.zeros(max(n, 1), dtype=np.uint32)
+    nn = C.c_int()
+    npr = C.c_int()
+    ref().ref_cpu_build(_p(b), n, leaf_threshold, _p(nodes), _p(ids), C.byref(nn), C.byref(npr))
+    return Tree(nodes[: nn.value], ids[: npr.value])
+
+
+def check_tree(tree: Tree, boxes: np.ndarray, max_leaf: int = 0):
+    """Returns (code, max_depth, num_leaves); code 0 means valid."""
+    b = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    md = C.c_int()
+    nl = C.c_int()
+    code = lib().orc_check_tree(_p(tree.nodes), len(tree.nodes), _p(tree.prim_ids),
+                                len(tree.prim_ids), _p(b), b.shape[0], max_leaf,
+                                C.byref(md), C.byref(nl))
+    return code, md.value, nl.value
+
+
+def canonical(tree: Tree) -> np.ndarray:
+    cap = 2 * len(tree.prim_ids) + 2 * len(tree.nodes) + 16
+    out = np.empty(cap, dtype=np.int64)
+    w = lib().orc_canonical(_p(tree.nodes), _p(tree.prim_ids), _p(out), cap)
+    if w < 0:
+        raise RuntimeError("canonical buffer too small")
+    return out[:w].copy()
+
+
+def fcp(tree: Tree, points, queries, sqr_max=np.inf, lowest_id=True, stats=False):
+    P = np.ascontiguousarray(points, dtype=np.float32)
+    Q = np.ascontiguousarray(queries, dtype=np.float32)
+    nq = Q.shape[0]
+    ids = np.empty(nq, dtype=np.int32)
+    d2 = np.empty(nq, dtype=np.float32)
+    st = np.zeros(2, dtype=np.int64)
+    lib().orc_fcp(_p(tree.nodes), _p(tree.prim_ids), _p(P), _p(Q), nq, float(sqr_max),
+                  1 if lowest_id else 0, _p(ids), _p(d2), _p(st))
+    return (ids, d2, st) if stats else (ids, d2)
+
+
+def knn(tree: Tree, points, queries, k, sqr_max=np.inf, lowest_id=True):
+    P = np.ascontiguousarray(points, dtype=np.float32)
+    Q = np.ascontiguousarray(queries, dtype=np.float32)
+    nq = Q.shape[0]
+    ids = np.empty((nq, k), dtype=np.int32)
+    d2 = np.empty((nq, k), dtype=np.float32)
+    nf = np.empty(nq, dtype=np.int32)
+    lib().orc_knn(_p(tree.nodes), _p(tree.prim_ids), _p(P), _p(Q), nq, k, float(sqr_max),
+                  1 if lowest_id else 0, _p(ids), _p(d2), _p(nf))
+    return ids, d2, nf
+
+
+def ref_fcp(tree: Tree, points, queries, sqr_max=np.inf):
+    P = np.ascontiguousarray(points, dtype=np.float32)
+    Q = np.ascontiguousarray(queries, dtype=np.float32)
+    nq = Q.shape[0]
+    ids = np.empty(nq, dtype=np.int32)
+    d2 = np.empty(nq, dtype=np.float32)
+    ref().ref_fcp(_p(tree.nodes), len(tree.nodes), _p(tree.prim_ids), len(tree.prim_ids),
+                  _p(P), _p(Q), nq, float(sqr_max), _p(ids), _p(d2))
+    return ids, d2
+
+
+def ref_knn(tree: Tree, points, queries, k, sqr_max=np.inf):
+    P = np.ascontiguousarray(points, dtype=np.float32)
+    Q = np.ascontiguousarray(queries, dtype=np.float32)
+    nq = Q.shape[0]
+    ids = np.empty((nq, k), dtype=np.int32)
+    d2 = np.empty((nq, k), dtype=np.float32)
+    nf = np.empty(nq, dtype=np.int32)
+    ref().ref_knn(_p(tree.nodes), len(tree.nodes), _p(tree.prim_ids), len(tree.prim_ids),
+                  _p(P), _p(Q), nq, k, float(sqr_max), _p(ids), _p(d2), _p(nf))
+    return ids, d2, nf
+
+
+def ref_uniform(seed: int, n: int, lo: float, hi: float) -> np.ndarray:
+    out = np.empty((n, 3), dtype=np.float32)
+    ref().ref_uniform_points(seed, n, lo, hi, _p(out))
+    return out
+
+
+def fcp_brute(points, queries):
+    P = np.ascontiguousarray(points, dtype=np.float32)
+    Q = np.ascontiguousarray(queries, dtype=np.float32)
+    ids = np.empty(Q.shape[0], dtype=np.int32)
+    d2 = np.empty(Q.shape[0], dtype=np.float32)
+    lib().orc_fcp_brute(_p(P), P.shape[0], _p(Q), Q.shape[0], _p(ids), _p(d2))
+    return ids, d2
+
+
+def fcp_brute_f64(points, queries):
+    P = np.ascontiguousarray(points, dtype=np.float32)
+    Q = np.ascontiguousarray(queries, dtype=np.float32)
+    ids = np.empty(Q.shape[0], dtype=np.int32)
+    d2 = np.empty(Q.shape[0], dtype=np.float64)
+    lib().orc_fcp_brute_f64(_p(P), P.shape[0], _p(Q), Q.shape[0], _p(ids), _p(d2))
+    return ids, d2
+
+
+def knn_brute(points, queries, k):
+    P = np.ascontiguousarray(points, dtype=np.float32)
+    Q = np.ascontiguousarray(queries, dtype=np.float32)
+    ids = np.empty((Q.shape[0], k), dtype=np.int32)
+    d2 = np.empty((Q.shape[0], k), dtype=np.float32)
+    lib().orc_knn_brute(_p(P), P.shape[0], _p(Q), Q.shape[0], k, _p(ids), _p(d2))
+    return ids, d2
+
+
+def morton_keys(boxes, bits_per_dim: int):
+    b = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    n = b.shape[0]
+    keys = np.empty(n, dtype=np.uint64)
+    ids = np.empty(n, dtype=np.uint32)
+    nv = lib().orc_morton_keys(_p(b), n, bits_per_dim, _p(keys), _p(ids))
+    return keys[:nv].copy(), ids[:nv].copy()
+
+
+def sort_pairs(keys, ids):
+    k = np.ascontiguousarray(keys, dtype=np.uint64).copy()
+    v = np.ascontiguousarray(ids, dtype=np.uint32).copy()
+    lib().orc_sort_pairs(_p(k), _p(v), len(k))
+    return k, v
+
+
+def radix_build(boxes, bits_per_dim: int, leaf_threshold: int) -> Tree:
+    """Morton keys -> stable sort -> top-down hierarchy -> exact refit."""
+    keys, ids = morton_keys(boxes, bits_per_dim)
+    keys, ids = sort_pairs(keys, ids)
+    n = len(keys)
+    nodes = np.zeros(2 * n + 2, dtype=NODE_DTYPE)
+    nn = lib().orc_radix_hierarchy(_p(keys), n, leaf_threshold, _p(nodes))
+    tree = Tree(nodes[:nn], ids)
+    b = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    lib().orc_refit(_p(tree.nodes), nn, _p(tree.prim_ids), _p(b))
+    return tree

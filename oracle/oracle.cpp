@@ -1,0 +1,603 @@
+// oracle/oracle.cpp -- TEST INFRASTRUCTURE ONLY (see oracle.h).
+//
+// CPU restatement of the reference semantics, used as the checker for the
+// HIP library.  Compiled with -ffp-contract=off so float expressions round
+// exactly as written.
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <vector>
+
+namespace {
+
+constexpr uint64_t kOffMask = (1ull << 48) - 1;
+inline uint64_t pack_oc(uint64_t off, uint64_t cnt) { return (off & kOffMask) | (cnt << 48); }
+inline uint64_t oc_off(uint64_t oc) { return oc & kOffMask; }
+inline uint32_t oc_cnt(uint64_t oc) { return (uint32_t)(oc >> 48); }
+
+inline bool box_valid(const orc_box3f &b) { return b.lo[0] <= b.hi[0]; }
+
+// box_t::center(): (lower+upper)/2 per dim (box.h:146-147).
+inline void box_center(const orc_box3f &b, float c[3]) {
+  for (int d = 0; d < 3; d++) c[d] = (b.lo[d] + b.hi[d]) / 2.f;
+}
+
+inline orc_box3f empty_box() {
+  orc_box3f b;
+  for (int d = 0; d < 3; d++) {
+    b.lo[d] = std::numeric_limits<float>::infinity();
+    b.hi[d] = -std::numeric_limits<float>::infinity();
+  }
+  return b;
+}
+inline void grow(orc_box3f &b, const orc_box3f &o) {
+  for (int d = 0; d < 3; d++) {
+    b.lo[d] = std::min(b.lo[d], o.lo[d]);
+    b.hi[d] = std::max(b.hi[d], o.hi[d]);
+  }
+}
+inline bool box_eq(const orc_box3f &a, const orc_box3f &b) {
+  for (int d = 0; d < 3; d++)
+    if (a.lo[d] != b.lo[d] || a.hi[d] != b.hi[d]) return false;
+  return true;
+}
+
+// fSqrDistance_rd(vec3f,vec3f) = fSquare(x)+fSquare(y)+fSquare(z), evaluated
+// left to right without contraction (conservativeDistances.h:61-90).
+inline float sqr_dist_pt(const float *p, const float *q) {
+  float dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+  return dx * dx + dy * dy + dz * dz;
+}
+// fSqrDistance_rd(vec3f, box3f): clamp then dot (shrinkingRadiusQuery.h:253-259).
+inline float sqr_dist_box(const float *q, const orc_box3f &b) {
+  float v[3];
+  for (int d = 0; d < 3; d++) {
+    float p = std::min(std::max(q[d], b.lo[d]), b.hi[d]);
+    v[d] = p - q[d];
+  }
+  return v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+}
+
+// ---------------------------------------------------------------------------
+// spatial median (cpu/spatialMedian.h:73-115)
+struct Topo { int offset, count; };
+
+void sm_rec(int nodeID, int begin, int end, std::vector<Topo> &topo,
+            std::vector<int> &ids, std::vector<int> &alt,
+            const orc_box3f *boxes, int T) {
+  if (end - begin <= T) { topo[nodeID] = {begin, end - begin}; return; }
+  float clo[3], chi[3];
+  for (int d = 0; d < 3; d++) { clo[d] = INFINITY; chi[d] = -INFINITY; }
+  for (int i = begin; i < end; i++) {
+    float c[3];
+    box_center(boxes[ids[i]], c);
+    for (int d = 0; d < 3; d++) { clo[d] = std::min(clo[d], c[d]); chi[d] = std::max(chi[d], c[d]); }
+  }
+  float sz[3] = {chi[0] - clo[0], chi[1] - clo[1], chi[2] - clo[2]};
+  // arg_max for 3 dims (vec.h:639-643): ties go to the lower dimension
+  int dim = sz[2] > std::max(sz[0], sz[1]) ? 2 : (sz[1] > sz[0] ? 1 : 0);
+  float pos = (clo[dim] + chi[dim]) / 2.f;
+  int nl = 0, nr = 0;
+  for (int i = begin; i < end; i++) {
+    int pid = ids[i];
+    float c[3];
+    box_center(boxes[pid], c);
+    if (c[dim] < pos) alt[begin + nl++] = pid;
+    else alt[end - ++nr] = pid;
+  }
+  int mid;
+  if (nl && nr) mid = begin + nl;
+  else if (end - begin <= std::max(1, T)) { topo[nodeID] = {begin, end - begin}; return; }
+  else mid = (begin + end) / 2;
+  for (int i = begin; i < end; i++) ids[i] = alt[i];
+  int child = (int)topo.size();
+  topo.push_back({0, 0});
+  topo.push_back({0, 0});
+  topo[nodeID] = {child, 0};
+  sm_rec(child + 0, begin, mid, topo, ids, alt, boxes, T);
+  sm_rec(child + 1, mid, end, topo, ids, alt, boxes, T);
+}
+
+orc_box3f refit_rec(orc_node3f *nodes, uint64_t id, const uint32_t *primIDs,
+                    const orc_box3f *boxes) {
+  orc_node3f &n = nodes[id];
+  orc_box3f b = empty_box();
+  uint32_t cnt = oc_cnt(n.oc);
+  uint64_t off = oc_off(n.oc);
+  if (cnt == 0) {
+    orc_box3f l = refit_rec(nodes, off + 0, primIDs, boxes);
+    orc_box3f r = refit_rec(nodes, off + 1, primIDs, boxes);
+    grow(b, l);
+    grow(b, r);
+  } else {
+    for (uint32_t i = 0; i < cnt; i++) grow(b, boxes[primIDs[off + i]]);
+  }
+  n.b = b;
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+// shrinking-radius traversal (shrinkingRadiusQuery.h:409-508).  `lowestId`
+// switches the cull tests from the reference's `>=`/`<` to `>`/`<=` so that
+// nodes at exactly the current radius are still visited (needed for the
+// lowest-primID tie rule), and lifts the 64-entry stack limit.
+template <typename LeafFn>
+void for_each_leaf(const orc_node3f *nodes, const uint32_t *primIDs,
+                   const float *q, float sqrMax, bool lowestId, LeafFn &&leaf,
+                   int64_t *stats) {
+  struct Entry { uint32_t idx; float dist; };
+  std::vector<Entry> stack;
+  stack.reserve(64);
+  const size_t stackCap = lowestId ? (size_t)-1 : 64;
+  float cull = sqrMax;
+  uint64_t node = nodes[0].oc;
+  while (true) {
+    while (true) {
+      if (oc_cnt(node) != 0) break;
+      uint32_t i0 = (uint32_t)oc_off(node) + 0, i1 = (uint32_t)oc_off(node) + 1;
+      const orc_node3f &n0 = nodes[i0];
+      const orc_node3f &n1 = nodes[i1];
+      if (stats) stats[0]++;
+      float d0 = sqr_dist_box(q, n0.b);
+      float d1 = sqr_dist_box(q, n1.b);
+      float dmin = std::min(d0, d1), dmax = std::max(d0, d1);
+      bool dead = lowestId ? (dmin > cull) : (dmin >= cull);
+      if (dead) { node = 0; break; }  // count 0 + dead flag via offset 0 below
+      uint32_t farID;
+      if (d0 < d1) { node = n0.oc; farID = i1; }
+      else { node = n1.oc; farID = i0; }
+      bool both = lowestId ? (dmax <= cull) : (dmax < cull);
+      if (both) {
+        if (stack.size() >= stackCap) return;  // reference: silent truncation
+        stack.push_back({farID, dmax});
+      }
+    }
+    if (oc_cnt(node) != 0) {
+      float r = leaf(primIDs + oc_off(node), oc_cnt(node));
+      if (r < 0.f) return;
+      cull = std::min(cull, r);
+    }
+    while (true) {
+      if (stack.empty()) return;
+      Entry e = stack.back();
+      stack.pop_back();
+      if (e.dist <= cull) { node = nodes[e.idx].oc; break; }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_lcg_uniform(uint32_t seed, int64_t n, int dims, const double *lo,
+                     const double *hi, float *out) {
+  for (int64_t tid = 0; tid < n; tid++) {
+    // LCG<8>::init (random.h:31-44)
+    uint32_t v0 = seed, v1 = (uint32_t)tid, s0 = 0;
+    for (int r = 0; r < 8; r++) {
+      s0 += 0x9e3779b9u;
+      v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
+      v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7e95761eu);
+    }
+    uint32_t state = v0;
+    for (int d = 0; d < dims; d++) {
+      state = 1664525u * state + 1013904223u;       // random.h:59-65
+      float r = (float)(state & 0x00FFFFFFu) / (float)0x01000000;
+      double x = lo[d] + (double)r * (hi[d] - lo[d]);  // Generator.cpp:216
+      out[tid * dims + d] = (float)x;                   // IO.h:61-66 convert
+    }
+  }
+}
+
+int orc_sm_build(const orc_box3f *boxes, int n, int leafThreshold,
+                 orc_node3f *nodes, uint32_t *primIDs, int *numNodes,
+                 int *numPrims) {
+  std::vector<int> ids;
+  ids.reserve(n);
+  for (int i = 0; i < n; i++)
+    if (box_valid(boxes[i])) ids.push_back(i);
+  std::vector<int> alt(ids.size());
+  std::vector<Topo> topo(1, Topo{0, 0});
+  sm_rec(0, 0, (int)ids.size(), topo, ids, alt, boxes, leafThreshold);
+  for (size_t i = 0; i < ids.size(); i++) primIDs[i] = (uint32_t)ids[i];
+  for (size_t i = 0; i < topo.size(); i++) {
+    nodes[i].oc = pack_oc((uint64_t)topo[i].offset, (uint64_t)topo[i].count);
+    nodes[i].b = empty_box();
+  }
+  *numNodes = (int)topo.size();
+  *numPrims = (int)ids.size();
+  if (!ids.empty()) refit_rec(nodes, 0, primIDs, boxes);
+  return 0;
+}
+
+void orc_refit(orc_node3f *nodes, int numNodes, const uint32_t *primIDs,
+               const orc_box3f *boxes) {
+  (void)numNodes;
+  refit_rec(nodes, 0, primIDs, boxes);
+}
+
+int orc_check_tree(const orc_node3f *nodes, int numNodes,
+                   const uint32_t *primIDs, int numPrims,
+                   const orc_box3f *boxes, int n, int maxLeaf, int *maxDepth,
+                   int *numLeaves) {
+  std::vector<char> seen(n, 0);
+  int depthMax = 0, leaves = 0;
+  struct Item { uint64_t id; int depth; };
+  // iterative post-order to compute exact bounds and compare
+  struct Frame { uint64_t id; int depth; int state; orc_box3f acc; };
+  std::vector<Frame> st;
+  st.push_back({0, 1, 0, empty_box()});
+  std::vector<orc_box3f> ret;  // returned child boxes
+  int64_t visited = 0;
+  while (!st.empty()) {
+    Frame &f = st.back();
+    if (f.id >= (uint64_t)numNodes) return -1;  // node index out of range
+    const orc_node3f &nd = nodes[f.id];
+    uint32_t cnt = oc_cnt(nd.oc);
+    uint64_t off = oc_off(nd.oc);
+    if (cnt) {
+      if (++visited > (int64_t)numNodes) return -2;  // cycle
+      if (maxLeaf > 0 && (int)cnt > maxLeaf) return -3;  // leaf too large
+      if (off + cnt > (uint64_t)numPrims) return -4;      // prim range out of range
+      orc_box3f b = empty_box();
+      for (uint32_t i = 0; i < cnt; i++) {
+        uint32_t p = primIDs[off + i];
+        if (p >= (uint32_t)n) return -5;
+        if (!box_valid(boxes[p])) return -6;  // invalid prim referenced
+        if (seen[p]) return -7;               // prim referenced twice
+        seen[p] = 1;
+        grow(b, boxes[p]);
+      }
+      if (!box_eq(b, nd.b)) return -8;  // leaf box mismatch
+      leaves++;
+      depthMax = std::max(depthMax, f.depth);
+      ret.push_back(b);
+      st.pop_back();
+      continue;
+    }
+    if (f.state == 0) {
+      if (++visited > (int64_t)numNodes) return -2;
+      if (off + 1 >= (uint64_t)numNodes || off == 0) return -1;
+      f.state = 1;
+      int d = f.depth;
+      st.push_back({off + 0, d + 1, 0, empty_box()});
+      continue;
+    }
+    if (f.state == 1) {
+      f.state = 2;
+      int d = f.depth;
+      st.push_back({off + 1, d + 1, 0, empty_box()});
+      continue;
+    }
+    // both children done: last two entries of ret
+    orc_box3f r = ret.back(); ret.pop_back();
+    orc_box3f l = ret.back(); ret.pop_back();
+    orc_box3f b = empty_box();
+    grow(b, l);
+    grow(b, r);
+    if (!box_eq(b, nd.b)) return -9;  // inner box mismatch
+    ret.push_back(b);
+    st.pop_back();
+  }
+  for (int i = 0; i < n; i++)
+    if (box_valid(boxes[i]) && !seen[i]) return -10;  // valid prim missing
+  if (maxDepth) *maxDepth = depthMax;
+  if (numLeaves) *numLeaves = leaves;
+  return 0;
+}
+
+int64_t orc_canonical(const orc_node3f *nodes, const uint32_t *primIDs,
+                      int64_t *out, int64_t cap) {
+  // compute min primID per node (post-order), then emit pre-order
+  struct Rec {
+    static uint32_t minId(const orc_node3f *nodes, const uint32_t *primIDs,
+                          uint64_t id, std::vector<std::pair<uint64_t, uint32_t>> &memo) {
+      const orc_node3f &nd = nodes[id];
+      uint32_t cnt = oc_cnt(nd.oc);
+      uint64_t off = oc_off(nd.oc);
+      uint32_t m = 0xffffffffu;
+      if (cnt) {
+        for (uint32_t i = 0; i < cnt; i++) m = std::min(m, primIDs[off + i]);
+      } else {
+        m = std::min(minId(nodes, primIDs, off, memo), minId(nodes, primIDs, off + 1, memo));
+      }
+      memo.push_back({id, m});
+      return m;
+    }
+  };
+  // iterative emit with on-demand min computation (memoised in a hash-free
+  // way: recompute is O(subtree), so precompute all mins via explicit map)
+  std::vector<std::pair<uint64_t, uint32_t>> memo;
+  Rec::minId(nodes, primIDs, 0, memo);
+  std::sort(memo.begin(), memo.end());
+  auto getMin = [&](uint64_t id) {
+    auto it = std::lower_bound(memo.begin(), memo.end(), std::make_pair(id, 0u));
+    return it->second;
+  };
+  int64_t w = 0;
+  std::vector<uint64_t> st{0};
+  std::vector<uint32_t> tmp;
+  while (!st.empty()) {
+    uint64_t id = st.back();
+    st.pop_back();
+    const orc_node3f &nd = nodes[id];
+    uint32_t cnt = oc_cnt(nd.oc);
+    uint64_t off = oc_off(nd.oc);
+    if (cnt) {
+      if (w + 1 + cnt > cap) return -1;
+      out[w++] = cnt;
+      tmp.assign(primIDs + off, primIDs + off + cnt);
+      std::sort(tmp.begin(), tmp.end());
+      for (uint32_t v : tmp) out[w++] = v;
+    } else {
+      if (w + 1 > cap) return -1;
+      out[w++] = -1;
+      uint64_t a = off, b = off + 1;
+      if (getMin(b) < getMin(a)) std::swap(a, b);
+      st.push_back(b);  // emitted second
+      st.push_back(a);  // emitted first
+    }
+  }
+  return w;
+}
+
+void orc_fcp(const orc_node3f *nodes, const uint32_t *primIDs,
+             const float *points, const float *queries, int64_t nq,
+             float sqrMax, int mode, int32_t *ids, float *sqrDist,
+             int64_t *stats) {
+  const bool lowest = mode != 0;
+  for (int64_t qi = 0; qi < nq; qi++) {
+    const float *q = queries + 3 * qi;
+    int32_t best = -1;
+    float bestD = sqrMax;
+    auto leaf = [&](const uint32_t *prims, uint32_t cnt) -> float {
+      float r = INFINITY;
+      for (uint32_t i = 0; i < cnt; i++) {
+        int32_t pid = (int32_t)prims[i];
+        if (stats) stats[1]++;
+        float d = sqr_dist_pt(points + 3 * (int64_t)pid, q);
+        bool take = lowest ? (d < bestD || (d == bestD && best >= 0 && pid < best))
+                           : !(d >= bestD);  // findClosest.h:211-218
+        if (take) { bestD = d; best = pid; }
+        r = std::min(r, bestD);
+      }
+      return r;
+    };
+    for_each_leaf(nodes, primIDs, q, sqrMax, lowest, leaf, stats);
+    ids[qi] = best;
+    sqrDist[qi] = best >= 0 ? bestD : INFINITY;
+  }
+}
+
+void orc_knn(const orc_node3f *nodes, const uint32_t *primIDs,
+             const float *points, const float *queries, int64_t nq, int k,
+             float sqrMax, int mode, int32_t *ids, float *sqrDist,
+             int32_t *numFound) {
+  const bool lowest = mode != 0;
+  std::vector<int32_t> cid(k);
+  std::vector<float> cd(k);
+  for (int64_t qi = 0; qi < nq; qi++) {
+    const float *q = queries + 3 * qi;
+    int found = 0;
+    float maxD = sqrMax;
+    auto insert = [&](int32_t pid, float d) {
+      int pos;
+      if (lowest) {
+        if (!(d < sqrMax)) return;
+        if (found < k) pos = found++;
+        else {
+          if (d > cd[k - 1] || (d == cd[k - 1] && pid > cid[k - 1])) return;
+          pos = k - 1;
+        }
+        while (pos > 0 && (cd[pos - 1] > d || (cd[pos - 1] == d && cid[pos - 1] > pid))) {
+          cd[pos] = cd[pos - 1];
+          cid[pos] = cid[pos - 1];
+          --pos;
+        }
+      } else {
+        // knn::insert_linear (common/knn.h:57-78)
+        if (found < k) pos = found++;
+        else {
+          if (d >= maxD) return;
+          pos = k - 1;
+        }
+        while (pos > 0 && cd[pos - 1] > d) {
+          cd[pos] = cd[pos - 1];
+          cid[pos] = cid[pos - 1];
+          --pos;
+        }
+      }
+      cd[pos] = d;
+      cid[pos] = pid;
+      if (found == k) maxD = cd[k - 1];
+    };
+    auto leaf = [&](const uint32_t *prims, uint32_t cnt) -> float {
+      float r = INFINITY;
+      for (uint32_t i = 0; i < cnt; i++) {
+        int32_t pid = (int32_t)prims[i];
+        float d = sqr_dist_pt(points + 3 * (int64_t)pid, q);
+        insert(pid, d);
+        r = std::min(r, maxD);
+      }
+      return r;
+    };
+    for_each_leaf(nodes, primIDs, q, sqrMax, lowest, leaf, nullptr);
+    for (int j = 0; j < k; j++) {
+      ids[qi * k + j] = j < found ? cid[j] : -1;
+      sqrDist[qi * k + j] = j < found ? cd[j] : INFINITY;
+    }
+    if (numFound) numFound[qi] = found;
+  }
+}
+
+void orc_fcp_brute(const float *points, int64_t n, const float *queries,
+                   int64_t nq, int32_t *ids, float *sqrDist) {
+  for (int64_t qi = 0; qi < nq; qi++) {
+    const float *q = queries + 3 * qi;
+    int32_t best = -1;
+    float bd = INFINITY;
+    for (int64_t i = 0; i < n; i++) {
+      float d = sqr_dist_pt(points + 3 * i, q);
+      if (d < bd) { bd = d; best = (int32_t)i; }
+    }
+    ids[qi] = best;
+    sqrDist[qi] = bd;
+  }
+}
+
+void orc_fcp_brute_f64(const float *points, int64_t n, const float *queries,
+                       int64_t nq, int32_t *ids, double *sqrDist) {
+  for (int64_t qi = 0; qi < nq; qi++) {
+    const float *q = queries + 3 * qi;
+    int32_t best = -1;
+    double bd = INFINITY;
+    for (int64_t i = 0; i < n; i++) {
+      double dx = (double)points[3 * i + 0] - (double)q[0];
+      double dy = (double)points[3 * i + 1] - (double)q[1];
+      double dz = (double)points[3 * i + 2] - (double)q[2];
+      double d = dx * dx + dy * dy + dz * dz;
+      if (d < bd) { bd = d; best = (int32_t)i; }
+    }
+    ids[qi] = best;
+    sqrDist[qi] = bd;
+  }
+}
+
+void orc_knn_brute(const float *points, int64_t n, const float *queries,
+                   int64_t nq, int k, int32_t *ids, float *sqrDist) {
+  std::vector<std::pair<float, int32_t>> all(n);
+  for (int64_t qi = 0; qi < nq; qi++) {
+    const float *q = queries + 3 * qi;
+    for (int64_t i = 0; i < n; i++) all[i] = {sqr_dist_pt(points + 3 * i, q), (int32_t)i};
+    int64_t kk = std::min<int64_t>(k, n);
+    std::partial_sort(all.begin(), all.begin() + kk, all.end());
+    for (int j = 0; j < k; j++) {
+      ids[qi * k + j] = j < kk ? all[j].second : -1;
+      sqrDist[qi * k + j] = j < kk ? all[j].first : INFINITY;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Morton (radix.h)
+static inline uint64_t spread3(uint32_t v, int bits) {
+  uint64_t r = 0;
+  for (int i = 0; i < bits; i++) r |= (uint64_t)((v >> i) & 1u) << (3 * i);
+  return r;
+}
+
+int orc_morton_keys(const orc_box3f *boxes, int n, int bitsPerDim,
+                    uint64_t *keys, uint32_t *ids) {
+  // centroid bounds of valid prims (radix.h:199-227); AtomicBox is exact
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = 0; i < n; i++) {
+    if (!box_valid(boxes[i])) continue;
+    float c[3];
+    for (int d = 0; d < 3; d++) c[d] = 0.5f * (boxes[i].lo[d] + boxes[i].hi[d]);
+    for (int d = 0; d < 3; d++) { lo[d] = std::min(lo[d], c[d]); hi[d] = std::max(hi[d], c[d]); }
+  }
+  // Quantizer::init (radix.h:46-53): uniform scale from the largest extent
+  float ext = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+  float inv = 1.f / std::max(ext, 1e-20f);
+  float scale = (float)(1u << bitsPerDim) * inv;
+  const uint32_t cmax = (1u << bitsPerDim) - 1;
+  int w = 0;
+  for (int i = 0; i < n; i++) {
+    if (!box_valid(boxes[i])) continue;
+    uint32_t cell[3];
+    for (int d = 0; d < 3; d++) {
+      float c = 0.5f * (boxes[i].lo[d] + boxes[i].hi[d]);
+      float f = (c - lo[d]) * scale;            // radix.h:59
+      uint32_t u = (uint32_t)f;
+      cell[d] = std::min(u, cmax);              // radix.h:60
+    }
+    keys[w] = spread3(cell[0], bitsPerDim) | (spread3(cell[1], bitsPerDim) << 1) |
+              (spread3(cell[2], bitsPerDim) << 2);
+    ids[w] = (uint32_t)i;
+    w++;
+  }
+  return w;
+}
+
+void orc_sort_pairs(uint64_t *keys, uint32_t *ids, int n) {
+  std::vector<int> perm(n);
+  std::iota(perm.begin(), perm.end(), 0);
+  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return keys[a] < keys[b]; });
+  std::vector<uint64_t> k2(n);
+  std::vector<uint32_t> i2(n);
+  for (int i = 0; i < n; i++) { k2[i] = keys[perm[i]]; i2[i] = ids[perm[i]]; }
+  std::copy(k2.begin(), k2.end(), keys);
+  std::copy(i2.begin(), i2.end(), ids);
+}
+
+int orc_radix_hierarchy(const uint64_t *keys, int n, int T, orc_node3f *nodes) {
+  // level-by-level like the reference's createNodes loop (radix.h:501-570),
+  // with deterministic breadth-first numbering
+  struct Range { uint32_t b, e; };
+  std::vector<Range> open{{0u, (uint32_t)n}};
+  std::vector<uint64_t> openIds{0};
+  nodes[1].oc = 0;
+  nodes[1].b = empty_box();
+  int numNodes = 2;
+  if (T <= 0) T = 1;
+  while (!open.empty()) {
+    std::vector<Range> next;
+    std::vector<uint64_t> nextIds;
+    for (size_t i = 0; i < open.size(); i++) {
+      uint32_t b = open[i].b, e = open[i].e;
+      uint64_t id = openIds[i];
+      nodes[id].b = empty_box();
+      int size = (int)(e - b);
+      int split = -1;
+      if (size > T) {
+        uint64_t fk = keys[b], lk = keys[e - 1];
+        if (fk == lk) split = (int)((b + e) / 2);  // radix.h:455-465
+        else {
+          int nmatch = __builtin_clzll(fk ^ lk);
+          uint64_t search = lk & (~0ull << (63 - nmatch));
+          uint32_t lo = b, hi = e;
+          while (hi > lo) {
+            uint32_t mid = (lo + hi) / 2;
+            if (keys[mid] < search) lo = mid + 1;
+            else hi = mid;
+          }
+          split = (int)lo;
+        }
+      }
+      if (split < 0) {
+        nodes[id].oc = pack_oc(b, (uint64_t)size);
+      } else {
+        uint64_t c = (uint64_t)numNodes;
+        numNodes += 2;
+        nodes[id].oc = pack_oc(c, 0);
+        next.push_back({b, (uint32_t)split});
+        nextIds.push_back(c);
+        next.push_back({(uint32_t)split, e});
+        nextIds.push_back(c + 1);
+      }
+    }
+    open.swap(next);
+    openIds.swap(nextIds);
+  }
+  return numNodes;
+}
+
+int32_t orc_atomicbox_encode(float f) {
+  int32_t bits;
+  std::memcpy(&bits, &f, 4);
+  if (bits & (int32_t)0x80000000) bits ^= 0x7fffffff;
+  return bits;
+}
+float orc_atomicbox_decode(int32_t bits) {
+  if (bits & (int32_t)0x80000000) bits ^= 0x7fffffff;
+  float f;
+  std::memcpy(&f, &bits, 4);
+  return f;
+}
+
+}  // extern "C"

@@ -1,0 +1,102 @@
+// oracle/oracle.h -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of the reference's BVH build-and-query semantics. It is the
+// checker for the HIP library: tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg are the only callers. The product library never links,
+// loads or calls anything in this directory.
+//
+// Every routine cites the reference file:line it restates (paths relative to
+// the reference repository root). Arithmetic is float32 with FP contraction
+// OFF (the Makefile passes -ffp-contract=off), matching the reference host
+// path (g++ without FMA).
+#pragma once
+#include <cstdint>
+#include <cstddef>
+
+extern "C" {
+
+/* Layout shared with the product: 24-byte box, 32-byte node whose last 8
+   bytes hold offset (low 48 bits) and count (high 16 bits); bvh.h:78-102. */
+typedef struct { float lo[3]; float hi[3]; } orc_box3f;
+typedef struct { orc_box3f b; uint64_t oc; } orc_node3f;
+
+/* LCG<8>(seed,index) uniform generator; random.h:24-68, Generator.cpp:202-218.
+   out[i*dims+d] = float(lo[d] + rng()*(hi[d]-lo[d])) computed in double. */
+void orc_lcg_uniform(uint32_t seed, int64_t n, int dims, const double *lo,
+                     const double *hi, float *out);
+
+/* Serial recursive spatial-median builder; cpu/spatialMedian.h:73-168.
+   nodes capacity must be >= 2*n+2.  Output numbering: root-first DFS, no
+   unused node 1, primIDs holds valid prims only (numPrims = #valid). */
+int orc_sm_build(const orc_box3f *boxes, int n, int leafThreshold,
+                 orc_node3f *nodes, uint32_t *primIDs, int *numNodes,
+                 int *numPrims);
+
+/* Bottom-up exact refit of node bounds from prim boxes (sm_builder.h:633-675,
+   spatialMedian.h:117-134). Only nodes reachable from node 0 are written. */
+void orc_refit(orc_node3f *nodes, int numNodes, const uint32_t *primIDs,
+               const orc_box3f *boxes);
+
+/* TreeChecker (testing/bvh/checkBuilders.cu:64-108) plus structural limits:
+   returns 0 if every reachable node's box equals exactly the union of its
+   children / leaf prim boxes, every valid prim appears exactly once, invalid
+   prims appear nowhere, leaf counts are in [1, maxLeaf] (maxLeaf<=0: no cap),
+   and no index is out of range.  Negative codes name the first failure. */
+int orc_check_tree(const orc_node3f *nodes, int numNodes,
+                   const uint32_t *primIDs, int numPrims,
+                   const orc_box3f *boxes, int n, int maxLeaf,
+                   int *maxDepth, int *numLeaves);
+
+/* Canonical serialisation of a tree (node indices and leaf-internal order
+   ignored): pre-order, a leaf emits [count, sorted ids...], an inner node
+   emits [-1] then its children ordered by their minimum primID.  Returns the
+   number of int64 written (cap = capacity), or -1 if cap is too small. */
+int64_t orc_canonical(const orc_node3f *nodes, const uint32_t *primIDs,
+                      int64_t *out, int64_t cap);
+
+/* Closest point over a point BVH.  mode 0 = reference semantics (first found
+   wins ties; shrinkingRadiusQuery.h:409-508 with the 64-entry stack and
+   findClosest.h:181-226); mode 1 = lowest primID among equal distances (no
+   stack limit).  stats[0] += node pairs visited, stats[1] += prims visited. */
+void orc_fcp(const orc_node3f *nodes, const uint32_t *primIDs,
+             const float *points, const float *queries, int64_t nq,
+             float sqrMax, int mode, int32_t *ids, float *sqrDist,
+             int64_t *stats);
+
+/* kNN over a point BVH; mode 0 = reference insert_linear (common/knn.h:57-78,
+   points/knn.h:75-116); mode 1 = (sqrDist, primID) lexicographic order.
+   ids/sqrDist are [nq*k]; unfilled slots get -1 / +inf. */
+void orc_knn(const orc_node3f *nodes, const uint32_t *primIDs,
+             const float *points, const float *queries, int64_t nq, int k,
+             float sqrMax, int mode, int32_t *ids, float *sqrDist,
+             int32_t *numFound);
+
+/* Brute force closest point, float32 reference formula (dx*dx+dy*dy)+dz*dz
+   (conservativeDistances.h:61-90) with lowest-id ties. */
+void orc_fcp_brute(const float *points, int64_t n, const float *queries,
+                   int64_t nq, int32_t *ids, float *sqrDist);
+/* Brute force in float64: exact closest distance (lowest id on ties). */
+void orc_fcp_brute_f64(const float *points, int64_t n, const float *queries,
+                       int64_t nq, int32_t *ids, double *sqrDist);
+/* Brute force kNN, float32 formula, (d,id) order. */
+void orc_knn_brute(const float *points, int64_t n, const float *queries,
+                   int64_t nq, int k, int32_t *ids, float *sqrDist);
+
+/* Morton keys; radix.h:41-70 (Quantizer), 199-241 (centroid bounds),
+   283-421 (interleave, key kernel).  bitsPerDim = 21 (63-bit) or 10
+   (30-bit).  Valid prims are compacted in ascending primID order.
+   Returns the number of valid prims. */
+int orc_morton_keys(const orc_box3f *boxes, int n, int bitsPerDim,
+                    uint64_t *keys, uint32_t *ids);
+/* Stable sort of (key,id) pairs by key (cub DeviceRadixSort semantics). */
+void orc_sort_pairs(uint64_t *keys, uint32_t *ids, int n);
+/* Top-down hierarchy over sorted keys; radix.h:446-570.  Breadth-first
+   numbering, node 1 unused.  Returns numNodes. */
+int orc_radix_hierarchy(const uint64_t *sortedKeys, int n, int leafThreshold,
+                        orc_node3f *nodes);
+
+/* AtomicBox order-preserving encode/decode; builder_common.h:102-117. */
+int32_t orc_atomicbox_encode(float f);
+float orc_atomicbox_decode(int32_t i);
+
+}  // extern "C"

@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""bench.py -- BVH build-and-query benchmark on one MI355X.
+
+Workload (the large GPU configuration the headline metric is quoted on):
+10,000,000 float32 points uniform in [0,1]^3 (reference LCG<8> generator,
+seed 2), each a degenerate box; gpuBuilder spatial median with leaf size <= 8.
+A step is one complete build (allocation and free included, as the
+reference's buildBench does, testing/bvh/buildBench.h:115-143) with the
+boxes already resident in HBM.  After the timed steps the same run measures
+10M closest-point queries (seed 3) and 1M kNN queries (k=8) on the built
+tree, the per-kernel roofline of the dominant build kernel (HIP events
+inside the library, on the build stream), and the reference's own CPU path
+on a bounded sample.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0           # MI355X HBM3E
+PUBLISHED_MPRIMS = 1e7 / 13e-3 / 1e6  # "10M points in under 13ms" (README.md:107-110)
+
+
+def cpu_info():
+    cores = os.cpu_count()
+    model = platform.processor() or ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return cores, model
+
+
+def cpu_baseline(n_sample: int, nq_sample: int, budget_s: float):
+    """Reference CPU path (cpu::spatialMedian + host findClosest), 1 core."""
+    import numpy as np
+
+    import oracle
+    from cubql_amd import synth
+
+    P = synth.lcg_uniform(2, n_sample)
+    Q = synth.lcg_uniform(3, nq_sample)
+    boxes = oracle.points_to_boxes(P)
+    kind = "reference" if oracle.ref_available() else "port"
+    if kind == "reference":
+        R = oracle.ref()
+        build_once = lambda: R.ref_time_cpu_build(oracle._p(boxes), n_sample, 8)  # noqa: E731
+    else:
+        def build_once():
+            t = time.perf_counter()
+            oracle.sm_build(boxes, 8)
+            return time.perf_counter() - t
+    build_once()  # warm-up (buildBench.h:115-117)
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 3 or (time.perf_counter() - t_start < budget_s and len(times) < 10):
+        times.append(build_once())
+    tree = oracle.ref_sm_build(boxes, 8) if kind == "reference" else oracle.sm_build(boxes, 8)
+    t = time.perf_counter()
+    if kind == "reference":
+        oracle.ref_fcp(tree, P, Q)
+    else:
+        oracle.fcp(tree, P, Q, lowest_id=False)
+    tq = time.perf_counter() - t
+    cores, model = cpu_info()
+    mean = float(np.mean(times))
+    return {
+        "value": n_sample / mean / 1e6,
+        "unit": "Mprims/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": (f"cpu::spatialMedian (serial, leaf<=8) over the first {n_sample:,} points of the "
+                   f"same seed-2 stream, {len(times)} timed builds after 1 warm-up "
+                   f"(mean {mean*1e3:.1f} ms, median {float(np.median(times))*1e3:.1f} ms); "
+                   f"host findClosest over {nq_sample:,} seed-3 queries on that tree"),
+        "fcp_mqueries_s": nq_sample / tq / 1e6,
+        "compiler": "g++ -O3 -std=c++17 -ffp-contract=off (x86-64 default -march)",
+        "host_cpus": cores,
+        "host_model": model,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--points", type=int, default=10_000_000)
+    ap.add_argument("--fcp-queries", type=int, default=10_000_000)
+    ap.add_argument("--knn-queries", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "sharding plan is described in DESIGN.md and is not part of this build.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import cubql_amd as cq
+    from cubql_amd import synth
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    N = args.points
+    P = synth.lcg_uniform_torch(2, N, device=dev)
+    boxes = cq.points_to_boxes(P)
+    Q = synth.lcg_uniform_torch(3, max(args.fcp_queries, args.knn_queries), device=dev)
+    cfg = cq.BuildConfig(makeLeafThreshold=8)
+    mem = cq.TorchMemory(dev)
+    torch.cuda.synchronize()
+    print(f"[bench] data ready: {N:,} points", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        cq.gpuBuilder(boxes, cfg, mem=mem).free()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bvh = cq.gpuBuilder(boxes, cfg, mem=mem)
+        bvh.free()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) / args.steps * 1e3
+    value = N / (ms_per_step * 1e-3) / 1e6
+    print(f"[bench] build: {ms_per_step:.3f} ms/build, {value:.1f} Mprims/s", file=sys.stderr,
+          flush=True)
+
+    # ---- per-kernel roofline of the dominant build kernel (events in the library)
+    cq.profile.reset()
+    cq.profile.enable(True)
+    for _ in range(3):
+        cq.gpuBuilder(boxes, cfg, mem=mem).free()
+    torch.cuda.synchronize()
+    cq.profile.enable(False)
+    part = cq.profile.query("sm_partition")
+    small = cq.profile.query("sm_small")
+    kern = {}
+    for name, st in [("sm_partition", part), ("sm_small", small)]:
+        if st and st["launches"]:
+            kern[name] = {"launches_per_build": st["launches"] / 3,
+                          "avg_launch_ms": st["ms"] / st["launches"],
+                          "ms_per_build": st["ms"] / 3,
+                          "bytes_per_launch": st["bytes"] / st["launches"],
+                          "gbs": st["bytes"] / (st["ms"] * 1e-3) / 1e9}
+    dom = kern.get("sm_partition")
+    roofline = None
+    if dom:
+        roofline = {"bound": "hbm", "kernel": "sm_partition",
+                    "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                    "bytes_model": "32 B per prim of the level (16-byte item read + written)",
+                    "avg_launch_ms": dom["avg_launch_ms"],
+                    "bytes_per_launch": dom["bytes_per_launch"]}
+
+    # ---- queries on one tree
+    bvh = cq.gpuBuilder(boxes, cfg, mem=mem)
+    nq = args.fcp_queries
+    Qf = Q[:nq]
+    cq.points.findClosest(bvh, P, Qf)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 3
+    e0.record()
+    for _ in range(reps):
+        cq.points.findClosest(bvh, P, Qf)
+    e1.record()
+    torch.cuda.synchronize()
+    fcp_ms = e0.elapsed_time(e1) / reps
+    nk = args.knn_queries
+    Qk = Q[:nk]
+    cq.points.findKNN(bvh, P, Qk, 8)
+    e0.record()
+    for _ in range(reps):
+        cq.points.findKNN(bvh, P, Qk, 8)
+    e1.record()
+    torch.cuda.synchronize()
+    knn_ms = e0.elapsed_time(e1) / reps
+    num_nodes = bvh.numNodes
+    bvh.free()
+    print(f"[bench] fcp {nq/fcp_ms/1e3:.1f} Mq/s, knn {nk/knn_ms/1e3:.1f} Mq/s", file=sys.stderr,
+          flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(min(args.cpu_sample, N), 200_000, budget_s=12.0)
+        print(f"[bench] cpu baseline {cpu['value']:.2f} Mprims/s", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "build Mprims/s @10M pts; fcp/kNN Mqueries/s; batched fwd+bwd vs plain",
+        "value": value,
+        "unit": "Mprims/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / PUBLISHED_MPRIMS,
+        "dtype": "f32",
+        "data": "synthetic: reference LCG<8> uniform generator, points seed 2, queries seed 3",
+        "config": {"workload": f"gpuBuilder (spatial median, leaf<=8) over {N:,} uniform float3 "
+                               f"points in [0,1]^3; {nq:,} closest-point and {nk:,} kNN (k=8) "
+                               "queries uniform in [0,1]^3",
+                   "points": N, "leaf_threshold": 8, "num_nodes": num_nodes},
+        "fcp_mqueries_s": nq / fcp_ms / 1e3,
+        "knn_mqueries_s": nk / knn_ms / 1e3,
+        "kernels": kern,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -94,6 +94,10 @@ def lib():
         "cubql_refit_f3": (I, [P(BVH3f), V, V, P(MemResource)]),
         "cubql_fcp_points_f3": (I, [P(BVH3f), V, V, U32, F, V, V, V]),
         "cubql_knn_points_f3": (I, [P(BVH3f), V, V, U32, I, F, V, V, V, V]),
+        "cubql_profile_enable": (None, [I]),
+        "cubql_profile_reset": (None, []),
+        "cubql_profile_query": (I, [C.c_char_p, P(C.c_uint64), P(C.c_double), P(C.c_double),
+                                    P(C.c_uint64)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name, None)
@@ -288,6 +292,27 @@ class points:  # noqa: N801 -- mirrors cuBQL::points
                                          float(sqrMaxSearchDist), _ptr(ids), _ptr(d2), _ptr(nf),
                                          C.c_void_p(_stream_handle(stream))), "findKNN")
         return ids, d2, nf
+
+
+class profile:  # noqa: N801
+    """Event-based per-kernel timing inside the library (cubql_profile_*)."""
+
+    @staticmethod
+    def enable(on: bool = True):
+        lib().cubql_profile_enable(1 if on else 0)
+
+    @staticmethod
+    def reset():
+        lib().cubql_profile_reset()
+
+    @staticmethod
+    def query(kernel: str):
+        """{'launches', 'ms', 'bytes', 'units'} for a kernel name, or None."""
+        n, ms, by, un = C.c_uint64(), C.c_double(), C.c_double(), C.c_uint64()
+        if lib().cubql_profile_query(kernel.encode(), C.byref(n), C.byref(ms), C.byref(by),
+                                     C.byref(un)) != 0:
+            return None
+        return {"launches": n.value, "ms": ms.value, "bytes": by.value, "units": un.value}
 
 
 def points_to_boxes(p: torch.Tensor) -> torch.Tensor:

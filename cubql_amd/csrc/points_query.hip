@@ -9,6 +9,7 @@
 // (sqrDist, primID)).
 #include "common.h"
 #include "cubql_amd/traversal.hip.h"
+#include "profile.h"
 
 namespace cubql {
 namespace pq {
@@ -200,8 +201,11 @@ template <int K>
 int launch_knn(const cubql_bvh3f &bvh, const float *points, const float *queries, uint32_t nq,
                int k, float sqrMax, int32_t *ids, float *d, int32_t *found, Overflow *ovf,
                uint32_t *ovfList, hipStream_t s) {
+  int ps = prof_begin(s);
   k_knn<K><<<div_up(nq, kBlock), kBlock, 0, s>>>(bvh, points, queries, nq, k, sqrMax, ids, d, found,
                                                   ovf, ovfList);
+  prof_end(ps, s, "knn_points");
+  prof_set(ps, (12.0 + 8.0 * k + 4.0) * nq, nq);
   k_knn_deep<K><<<64, 64, 0, s>>>(bvh, points, queries, k, sqrMax, ids, d, found, ovf, ovfList);
   CUBQL_LAUNCH_CHECK();
   return CUBQL_OK;
@@ -234,8 +238,12 @@ extern "C" int cubql_fcp_points_f3(const cubql_bvh3f *bvh, const float *d_points
   CUBQL_CHECK(scr.alloc(&ovf, 1));
   CUBQL_CHECK(scr.alloc(&ovfList, numQueries));
   CUBQL_CHECK(fill_u32((uint32_t *)ovf, 0u, 2, s));
+  int ps = prof_begin(s);
   pq::k_fcp<<<div_up(numQueries, pq::kBlock), pq::kBlock, 0, s>>>(
       *bvh, d_points, d_queries, numQueries, sqrMaxSearchDist, d_outIDs, d_outSqrDist, ovf, ovfList);
+  prof_end(ps, s, "fcp_points");
+  // compulsory bytes only: query in, id + distance out
+  prof_set(ps, 20.0 * numQueries, numQueries);
   pq::k_fcp_deep<<<64, 64, 0, s>>>(*bvh, d_points, d_queries, sqrMaxSearchDist, d_outIDs,
                                    d_outSqrDist, ovf, ovfList);
   CUBQL_LAUNCH_CHECK();

@@ -23,6 +23,10 @@
 //     device memory, the host only checks for completion every few levels.
 #include "block_ops.h"
 #include "common.h"
+#include "wave_ops.h"
+#include "profile.h"
+
+#include <vector>
 
 namespace cubql {
 namespace sm {
@@ -67,6 +71,10 @@ struct Ctx {
   Node *nodes;             // temp nodes (capacity 2*numValid+2)
   Counters *ctr;
   uint32_t *blockCount;    // init pass
+  float *blockBounds;      // init pass: per-workgroup centroid bounds + point flag (8 floats)
+  Node *loc;               // small phase: local nodes of segment s at loc[2*begin_s + j]
+  uint32_t *smallCount;    // small phase: local node count per small segment
+  uint32_t *smallBase;     // small phase: first global id of local node 1
   Seg *segs[2];            // per-level large segments (ping-pong)
   int32_t *childCb;        // [2*maxLarge][6]
   uint32_t *segLeft;       // [maxLarge]
@@ -78,6 +86,7 @@ struct Ctx {
   uint32_t *fixup;         // [maxLarge] indices into next-level segs
   uint32_t *topNodes;      // [numValid] inner nodes of the level phase
   uint32_t *levelStart;    // [maxLevels+1] start of each level in topNodes
+  uint32_t *levelItems;    // [maxLevels] prims moved by each level's partition
   uint32_t maxLarge;
 };
 
@@ -163,32 +172,49 @@ __global__ __launch_bounds__(kBlock) void k_count(Ctx c) {
   uint32_t tot;
   block_excl_scan<kBlock>(cnt, ss, &tot);
   bool allPts = __syncthreads_and(pts);
+  // per-workgroup partials, reduced by k_count_scan (no same-address atomics)
+  if (threadIdx.x < 6) c.blockBounds[blockIdx.x * 8 + threadIdx.x] = v[threadIdx.x];
   if (threadIdx.x == 0) {
     c.blockCount[blockIdx.x] = tot;
-    if (tot) {
-      for (int d = 0; d < 3; d++) {
-        atomicMin(&c.ctr->rootCb[d], enc_f(v[d]));
-        atomicMax(&c.ctr->rootCb[d + 3], enc_f(v[d + 3]));
-      }
-    }
-    if (!allPts) atomicAnd(&c.ctr->allPoints, 0u);
+    c.blockBounds[blockIdx.x * 8 + 6] = allPts ? 1.f : 0.f;
   }
 }
 
 // Pass 2 (single workgroup): exclusive scan of tile counts, root setup.
 __global__ __launch_bounds__(1024) void k_count_scan(Ctx c, uint32_t numBlocks) {
   __shared__ uint32_t ss[1024 / kWave + 1];
+  __shared__ float sr[7][1024 / kWave];
   uint32_t run = 0;
+  float red[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 1.f};
   for (uint32_t b0 = 0; b0 < numBlocks; b0 += 1024) {
     uint32_t i = b0 + threadIdx.x;
     uint32_t v = i < numBlocks ? c.blockCount[i] : 0;
+    if (i < numBlocks) {
+      const float *bb = c.blockBounds + (size_t)i * 8;
+      for (int j = 0; j < 3; j++) {
+        red[j] = fminf(red[j], bb[j]);
+        red[j + 3] = fmaxf(red[j + 3], bb[j + 3]);
+      }
+      red[6] = fminf(red[6], bb[6]);
+    }
     uint32_t tot;
     uint32_t ex = block_excl_scan<1024>(v, ss, &tot);
     if (i < numBlocks) c.blockCount[i] = run + ex;
     run += tot;
   }
+  for (int j = 0; j < 7; j++) {
+    float r = (j >= 3 && j < 6) ? wave_max(red[j]) : wave_min(red[j]);
+    if (lane_id() == 0) sr[j][threadIdx.x / kWave] = r;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
     Counters *k = c.ctr;
+    for (int j = 0; j < 7; j++) {
+      float r = sr[j][0];
+      for (int w = 1; w < 1024 / kWave; w++) r = (j >= 3 && j < 6) ? fmaxf(r, sr[j][w]) : fminf(r, sr[j][w]);
+      if (j < 6) k->rootCb[j] = enc_f(r);
+      else k->allPoints = r > 0.5f ? 1u : 0u;
+    }
     k->numValid = run;
     k->nodeCount = 2;
     k->numLarge[0] = k->numLarge[1] = 0;
@@ -253,13 +279,14 @@ __global__ __launch_bounds__(1024) void k_plan(Ctx c, int level) {
   const int par = level & 1;
   const uint32_t n = c.ctr->numLarge[par];
   const Seg *segs = c.segs[par];
-  uint32_t run = 0;
+  uint32_t run = 0, items = 0;
   for (uint32_t b0 = 0; b0 < n; b0 += 1024) {
     uint32_t s = b0 + threadIdx.x;
-    uint32_t tiles = 0;
+    uint32_t tiles = 0, size = 0;
     if (s < n) {
       Seg g = segs[s];
-      tiles = div_up(g.end - g.begin, kTile);
+      size = g.end - g.begin;
+      tiles = div_up(size, kTile);
       float lo[3], hi[3];
       for (int d = 0; d < 3; d++) { lo[d] = dec_f(g.cb[d]); hi[d] = dec_f(g.cb[d + 3]); }
       int dim;
@@ -275,15 +302,18 @@ __global__ __launch_bounds__(1024) void k_plan(Ctx c, int level) {
         c.childCb[(2 * s + 1) * 6 + j + 3] = enc_f(-INFINITY);
       }
     }
-    uint32_t tot;
+    uint32_t tot, totItems;
     uint32_t ex = block_excl_scan<1024>(tiles, ss, &tot);
+    block_excl_scan<1024>(size, ss, &totItems);
     if (s < n) c.tileStart[s] = run + ex;
     run += tot;
+    items += totItems;
   }
   if (threadIdx.x == 0) {
     c.tileStart[n] = run;
     c.ctr->numTiles = run;
     c.ctr->ticket = 0;
+    c.levelItems[level] = items;
   }
 }
 
@@ -518,163 +548,285 @@ __global__ __launch_bounds__(kBlock) void k_fixup(Ctx c, int level) {
 }
 
 // ------------------------------------------------------------ small nodes
-struct Frame { uint32_t node, child, be, mp; };  // be = b | e<<16, mp = mid | phase<<16
+// A small segment (<= kSmall prims, or any leaf-sized node) is finished by
+// one wavefront.  Its nodes are numbered locally (local 0 = the segment's
+// root, whose global id is already fixed) and written to loc[2*begin + j];
+// inner nodes hold the LOCAL index of their first child.  k_small_scan /
+// k_small_emit then assign contiguous global ids in segment order and copy
+// the nodes to the final array, so no global counter is touched per node and
+// the node numbering is deterministic.
 
-__device__ inline void leaf_bounds_items(const float4 *it, uint32_t b, uint32_t e, const Ctx &c,
-                                         bool pts, float (&bb)[6]) {
-  for (int j = 0; j < 3; j++) { bb[j] = INFINITY; bb[j + 3] = -INFINITY; }
-  for (uint32_t i = b + lane_id(); i < e; i += kWave) {
-    float4 v = it[i];
-    if (pts) {
-      bb[0] = fminf(bb[0], v.x); bb[1] = fminf(bb[1], v.y); bb[2] = fminf(bb[2], v.z);
-      bb[3] = fmaxf(bb[3], v.x); bb[4] = fmaxf(bb[4], v.y); bb[5] = fmaxf(bb[5], v.z);
+struct SmallNodeOut { Node *loc; };
+
+__device__ __forceinline__ void store_node_wave(Node *dst, const float (&lo)[3], const float (&hi)[3],
+                                                uint64_t oc) {
+  const uint32_t l = lane_id();
+  uint32_t w = 0;
+  w = (l == 0) ? __float_as_uint(lo[0]) : w;
+  w = (l == 1) ? __float_as_uint(lo[1]) : w;
+  w = (l == 2) ? __float_as_uint(lo[2]) : w;
+  w = (l == 3) ? __float_as_uint(hi[0]) : w;
+  w = (l == 4) ? __float_as_uint(hi[1]) : w;
+  w = (l == 5) ? __float_as_uint(hi[2]) : w;
+  w = (l == 6) ? (uint32_t)oc : w;
+  w = (l == 7) ? (uint32_t)(oc >> 32) : w;
+  if (l < 8) reinterpret_cast<uint32_t *>(dst)[l] = w;
+}
+
+__device__ __forceinline__ uint32_t fr_pack(uint32_t b, uint32_t e, uint32_t me) {
+  return b | (e << 8) | (me << 16);
+}
+
+// Builds the subtree of a node with cnt <= 64 prims held one per lane
+// (lane j = prim at global position pos0 + j).  Same rules and order as the
+// CPU recursion (cpu/spatialMedian.h:81-114).  Items move between lanes with
+// ds_permute; bounds are DPP wave reductions.
+template <bool PTS>
+__device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, float y, float z,
+                            uint32_t id, float bx0, float by0, float bz0, float bx1, float by1,
+                            float bz1, uint32_t rootLocal, uint32_t &nLocal, Node *loc,
+                            uint32_t *sFr) {
+  const uint32_t lane = lane_id();
+  const uint64_t lt = lanemask_lt();
+  const uint32_t T = c.T;
+  int sp = 1;
+  if (lane == 0) sFr[0] = fr_pack(0, cnt, rootLocal);
+  __syncthreads();
+  while (sp > 0) {
+    const uint32_t f = sFr[--sp];
+    const uint32_t b = f & 0xffu, e = (f >> 8) & 0xffu, me = f >> 16;
+    const bool in = lane >= b && lane < e;
+    float clo[3], chi[3], nlo[3], nhi[3];
+    clo[0] = wave_min_dpp(in ? x : INFINITY);
+    clo[1] = wave_min_dpp(in ? y : INFINITY);
+    clo[2] = wave_min_dpp(in ? z : INFINITY);
+    chi[0] = wave_max_dpp(in ? x : -INFINITY);
+    chi[1] = wave_max_dpp(in ? y : -INFINITY);
+    chi[2] = wave_max_dpp(in ? z : -INFINITY);
+    if (PTS) {
+      for (int d = 0; d < 3; d++) { nlo[d] = clo[d]; nhi[d] = chi[d]; }
     } else {
-      cubql_box3f box = load_box(c.boxes, __float_as_uint(v.w));
-      for (int d = 0; d < 3; d++) {
-        bb[d] = fminf(bb[d], box.lower[d]);
-        bb[d + 3] = fmaxf(bb[d + 3], box.upper[d]);
+      nlo[0] = wave_min_dpp(in ? bx0 : INFINITY);
+      nlo[1] = wave_min_dpp(in ? by0 : INFINITY);
+      nlo[2] = wave_min_dpp(in ? bz0 : INFINITY);
+      nhi[0] = wave_max_dpp(in ? bx1 : -INFINITY);
+      nhi[1] = wave_max_dpp(in ? by1 : -INFINITY);
+      nhi[2] = wave_max_dpp(in ? bz1 : -INFINITY);
+    }
+    const uint32_t n = e - b;
+    if (n <= T) {
+      store_node_wave(loc + me, nlo, nhi, pack_oc(pos0 + b, n));
+      if (in) c.primIDs[pos0 + lane] = id;
+      continue;
+    }
+    int dim;
+    float pos;
+    choose_split(clo, chi, dim, pos);
+    const float v = dim == 0 ? x : (dim == 1 ? y : z);
+    const bool left = in && v < pos;
+    const uint64_t mL = __ballot(left);
+    const uint64_t mR = __ballot(in && !left);
+    const uint32_t nl = (uint32_t)__popcll(mL);
+    const uint32_t dst = left ? b + (uint32_t)__popcll(mL & lt)
+                              : (in ? e - 1 - (uint32_t)__popcll(mR & lt) : lane);
+    x = lane_push_f(dst, x);
+    y = lane_push_f(dst, y);
+    z = lane_push_f(dst, z);
+    id = lane_push_u(dst, id);
+    if (!PTS) {
+      bx0 = lane_push_f(dst, bx0); by0 = lane_push_f(dst, by0); bz0 = lane_push_f(dst, bz0);
+      bx1 = lane_push_f(dst, bx1); by1 = lane_push_f(dst, by1); bz1 = lane_push_f(dst, bz1);
+    }
+    uint32_t mid;
+    if (nl != 0 && nl != n) mid = b + nl;
+    else mid = (uint32_t)(((uint64_t)pos0 + b + pos0 + e) / 2 - pos0);  // spatialMedian.h:107
+    const uint32_t child = nLocal;
+    nLocal += 2;
+    store_node_wave(loc + me, nlo, nhi, pack_oc(child, 0));
+    if (lane == 0) {
+      sFr[sp] = fr_pack(mid, e, child + 1);
+      sFr[sp + 1] = fr_pack(b, mid, child);
+    }
+    sp += 2;
+    __syncthreads();
+  }
+}
+
+template <bool PTS>
+__device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx, float4 *sIt,
+                              float4 *sAlt, uint32_t *sFr, uint32_t *sFr1, uint32_t *sMe) {
+  const uint32_t lane = lane_id();
+  const uint32_t T = c.T;
+  const float4 *gItems = c.items[sg.buf];
+  const uint32_t gb = sg.begin;
+  const uint32_t m = sg.end - sg.begin;
+  Node *loc = c.loc + 2 * (size_t)gb;
+  uint32_t nLocal = 1;
+  if (m <= T && m > 64) {
+    // leaf of any size up to T
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = lane; i < m; i += kWave) {
+      float4 v = gItems[gb + i];
+      c.primIDs[gb + i] = __float_as_uint(v.w);
+      if (PTS) {
+        lo[0] = fminf(lo[0], v.x); lo[1] = fminf(lo[1], v.y); lo[2] = fminf(lo[2], v.z);
+        hi[0] = fmaxf(hi[0], v.x); hi[1] = fmaxf(hi[1], v.y); hi[2] = fmaxf(hi[2], v.z);
+      } else {
+        cubql_box3f bx = load_box(c.boxes, __float_as_uint(v.w));
+        for (int d = 0; d < 3; d++) { lo[d] = fminf(lo[d], bx.lower[d]); hi[d] = fmaxf(hi[d], bx.upper[d]); }
       }
     }
+    for (int d = 0; d < 3; d++) { lo[d] = wave_min_dpp(lo[d]); hi[d] = wave_max_dpp(hi[d]); }
+    store_node_wave(loc, lo, hi, pack_oc(gb, m));
+  } else if (m <= 64) {
+    float4 v = lane < m ? gItems[gb + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+    cubql_box3f bx{};
+    if (!PTS && lane < m) bx = load_box(c.boxes, __float_as_uint(v.w));
+    reg_subtree<PTS>(c, gb, m, v.x, v.y, v.z, __float_as_uint(v.w), bx.lower[0], bx.lower[1],
+                     bx.lower[2], bx.upper[0], bx.upper[1], bx.upper[2], 0u, nLocal, loc, sFr);
+  } else {
+    // more than 64 prims: split in LDS until children fit one wave
+    for (uint32_t i = lane; i < m; i += kWave) sIt[i] = gItems[gb + i];
+    int sp = 1;
+    if (lane == 0) sFr1[0] = (0u) | (m << 16);
+    if (lane == 0) sMe[0] = 0;
+    __syncthreads();
+    while (sp > 0) {
+      --sp;
+      const uint32_t f = sFr1[sp];
+      const uint32_t me = sMe[sp];
+      const uint32_t b = f & 0xffffu, e = f >> 16;
+      const uint32_t n = e - b;
+      if (n <= 64) {
+        float4 v = lane < n ? sIt[b + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+        cubql_box3f bx{};
+        if (!PTS && lane < n) bx = load_box(c.boxes, __float_as_uint(v.w));
+        reg_subtree<PTS>(c, gb + b, n, v.x, v.y, v.z, __float_as_uint(v.w), bx.lower[0],
+                         bx.lower[1], bx.lower[2], bx.upper[0], bx.upper[1], bx.upper[2], me,
+                         nLocal, loc, sFr);
+        continue;
+      }
+      float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (uint32_t i = b + lane; i < e; i += kWave) {
+        float4 v = sIt[i];
+        clo[0] = fminf(clo[0], v.x); clo[1] = fminf(clo[1], v.y); clo[2] = fminf(clo[2], v.z);
+        chi[0] = fmaxf(chi[0], v.x); chi[1] = fmaxf(chi[1], v.y); chi[2] = fmaxf(chi[2], v.z);
+        if (!PTS) {
+          cubql_box3f bx = load_box(c.boxes, __float_as_uint(v.w));
+          for (int d = 0; d < 3; d++) { blo[d] = fminf(blo[d], bx.lower[d]); bhi[d] = fmaxf(bhi[d], bx.upper[d]); }
+        }
+      }
+      for (int d = 0; d < 3; d++) { clo[d] = wave_min_dpp(clo[d]); chi[d] = wave_max_dpp(chi[d]); }
+      if (!PTS)
+        for (int d = 0; d < 3; d++) { blo[d] = wave_min_dpp(blo[d]); bhi[d] = wave_max_dpp(bhi[d]); }
+      const float *nlo = PTS ? clo : blo;
+      const float *nhi = PTS ? chi : bhi;
+      float olo[3] = {nlo[0], nlo[1], nlo[2]}, ohi[3] = {nhi[0], nhi[1], nhi[2]};
+      if (n <= T) {
+        for (uint32_t i = b + lane; i < e; i += kWave) c.primIDs[gb + i] = __float_as_uint(sIt[i].w);
+        store_node_wave(loc + me, olo, ohi, pack_oc(gb + b, n));
+        continue;
+      }
+      int dim;
+      float pos;
+      choose_split(clo, chi, dim, pos);
+      uint32_t nL = 0, nR = 0;
+      for (uint32_t c0 = b; c0 < e; c0 += kWave) {
+        const uint32_t i = c0 + lane;
+        const bool act = i < e;
+        const float4 v = act ? sIt[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const bool isL = act && (coord(v, dim) < pos);
+        const uint64_t mL = __ballot(isL);
+        const uint64_t mR = __ballot(act && !isL);
+        if (isL) sAlt[b + nL + mbcnt(mL)] = v;
+        else if (act) sAlt[e - 1 - (nR + mbcnt(mR))] = v;
+        nL += (uint32_t)__popcll(mL);
+        nR += (uint32_t)__popcll(mR);
+      }
+      uint32_t mid;
+      if (nL != 0 && nR != 0) mid = b + nL;
+      else mid = (uint32_t)(((uint64_t)gb + b + gb + e) / 2 - gb);  // spatialMedian.h:107
+      __syncthreads();
+      for (uint32_t i = b + lane; i < e; i += kWave) sIt[i] = sAlt[i];
+      const uint32_t child = nLocal;
+      nLocal += 2;
+      store_node_wave(loc + me, olo, ohi, pack_oc(child, 0));
+      if (lane == 0) {
+        sFr1[sp] = mid | (e << 16);
+        sMe[sp] = child + 1;
+        sFr1[sp + 1] = b | (mid << 16);
+        sMe[sp + 1] = child;
+      }
+      sp += 2;
+      __syncthreads();
+    }
   }
-  for (int j = 0; j < 3; j++) { bb[j] = wave_min(bb[j]); bb[j + 3] = wave_max(bb[j + 3]); }
+  if (lane == 0) c.smallCount[segIdx] = nLocal;
 }
 
-__device__ inline void write_node_bounds(Node *nd, const float (&bb)[6]) {
-  uint32_t l = lane_id();
-  float v = bb[0];
-  for (int j = 1; j < 6; j++) if ((int)l == j) v = bb[j];
-  if (l < 6) (&nd->lo[0])[l] = v;
-}
-
-// One wavefront per node of <= kSmall prims (or any leaf-sized node):
-// builds its whole subtree in LDS with the CPU builder's recursion order
-// (cpu/spatialMedian.h:73-115), writes primIDs in leaf order and all boxes
-// bottom-up.
 __global__ __launch_bounds__(kWave) void k_small(Ctx c) {
   __shared__ float4 sIt[kSmall];
   __shared__ float4 sAlt[kSmall];
-  __shared__ Frame sFr[kSmall + 1];
-  __shared__ float sBs[(kSmall + 1) * 6];
-  const uint32_t lane = lane_id();
-  const uint32_t T = c.T;
+  __shared__ uint32_t sFr[66];
+  __shared__ uint32_t sFr1[kSmall + 2];
+  __shared__ uint32_t sMe[kSmall + 2];
   const bool pts = c.ctr->allPoints != 0;
   const uint32_t nSmall = c.ctr->numSmall;
   for (uint32_t w = blockIdx.x; w < nSmall; w += gridDim.x) {
     const SmallSeg sg = c.small[w];
-    const float4 *gItems = c.items[sg.buf];
-    const uint32_t gb = sg.begin;
-    const uint32_t m = sg.end - sg.begin;
-    if (m <= T) {
-      // whole node is one leaf (any size up to T)
-      for (uint32_t i = lane; i < m; i += kWave) c.primIDs[gb + i] = __float_as_uint(gItems[gb + i].w);
-      float bb[6];
-      leaf_bounds_items(gItems, gb, gb + m, c, pts, bb);
-      Node *nd = &c.nodes[sg.node];
-      write_node_bounds(nd, bb);
-      if (lane == 0) nd->oc = pack_oc(gb, m);
-      continue;
-    }
-    for (uint32_t i = lane; i < m; i += kWave) sIt[i] = gItems[gb + i];
-    if (lane == 0) sFr[0] = Frame{sg.node, 0u, 0u | (m << 16), 0u};
-    __syncthreads();
-    int sp = 1, bsp = 0;
-    while (sp > 0) {
-      Frame f = sFr[sp - 1];
-      const uint32_t b = f.be & 0xffffu, e = f.be >> 16;
-      const uint32_t phase = f.mp >> 16, mid0 = f.mp & 0xffffu;
-      Node *nd = &c.nodes[f.node];
-      if (phase == 0) {
-        const uint32_t cnt = e - b;
-        if (cnt <= T) {
-          for (uint32_t i = b + lane; i < e; i += kWave) c.primIDs[gb + i] = __float_as_uint(sIt[i].w);
-          float bb[6];
-          leaf_bounds_items(sIt, b, e, c, pts, bb);
-          write_node_bounds(nd, bb);
-          if (lane == 0) nd->oc = pack_oc(gb + b, cnt);
-          {
-            float v = bb[0];
-            for (int j = 1; j < 6; j++) if ((int)lane == j) v = bb[j];
-            if (lane < 6) sBs[bsp * 6 + lane] = v;
-          }
-          bsp++;
-          sp--;
-          __syncthreads();
-          continue;
-        }
-        // centroid bounds of this range
-        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (uint32_t i = b + lane; i < e; i += kWave) {
-          float4 v = sIt[i];
-          lo[0] = fminf(lo[0], v.x); lo[1] = fminf(lo[1], v.y); lo[2] = fminf(lo[2], v.z);
-          hi[0] = fmaxf(hi[0], v.x); hi[1] = fmaxf(hi[1], v.y); hi[2] = fmaxf(hi[2], v.z);
-        }
-        for (int d = 0; d < 3; d++) { lo[d] = wave_min(lo[d]); hi[d] = wave_max(hi[d]); }
-        int dim;
-        float pos;
-        choose_split(lo, hi, dim, pos);
-        uint32_t nL = 0, nR = 0;
-        for (uint32_t c0 = b; c0 < e; c0 += kWave) {
-          uint32_t i = c0 + lane;
-          bool act = i < e;
-          float4 v = act ? sIt[i] : make_float4(0, 0, 0, 0);
-          bool isL = act && (coord(v, dim) < pos);
-          uint64_t mL = __ballot(isL);
-          uint64_t mR = __ballot(act && !isL);
-          if (isL) sAlt[b + nL + mbcnt(mL)] = v;
-          else if (act) sAlt[e - 1 - (nR + mbcnt(mR))] = v;
-          nL += (uint32_t)__popcll(mL);
-          nR += (uint32_t)__popcll(mR);
-        }
-        uint32_t mid;
-        if (nL != 0 && nR != 0) mid = b + nL;
-        else mid = (uint32_t)(((uint64_t)gb + b + gb + e) / 2 - gb);  // spatialMedian.h:107, global indices
-        __syncthreads();
-        for (uint32_t i = b + lane; i < e; i += kWave) sIt[i] = sAlt[i];
-        uint32_t child = 0;
-        if (lane == 0) child = atomicAdd(&c.ctr->nodeCount, 2u);
-        child = __shfl(child, 0, kWave);
-        if (lane == 0) {
-          nd->oc = pack_oc(child, 0);
-          sFr[sp - 1] = Frame{f.node, child, f.be, mid | (1u << 16)};
-          sFr[sp] = Frame{child, 0u, b | (mid << 16), 0u};
-        }
-        sp++;
-        __syncthreads();
-      } else if (phase == 1) {
-        if (lane == 0) {
-          sFr[sp - 1].mp = mid0 | (2u << 16);
-          sFr[sp] = Frame{f.child + 1, 0u, mid0 | (e << 16), 0u};
-        }
-        sp++;
-        __syncthreads();
-      } else {
-        // both children done: union of the two top bounds
-        float v = 0.f;
-        if (lane < 6) {
-          float r = sBs[(bsp - 1) * 6 + lane], l = sBs[(bsp - 2) * 6 + lane];
-          v = lane < 3 ? fminf(l, r) : fmaxf(l, r);
-        }
-        __syncthreads();
-        if (lane < 6) {
-          sBs[(bsp - 2) * 6 + lane] = v;
-          (&nd->lo[0])[lane] = v;
-        }
-        bsp--;
-        sp--;
-        __syncthreads();
-      }
-    }
+    if (pts) small_segment<true>(c, sg, w, sIt, sAlt, sFr, sFr1, sMe);
+    else small_segment<false>(c, sg, w, sIt, sAlt, sFr, sFr1, sMe);
     __syncthreads();
   }
 }
 
+// Exclusive scan of (count-1) over small segments: global id of local node
+// j >= 1 of segment s is smallBase[s] + j - 1.
+__global__ __launch_bounds__(1024) void k_small_scan(Ctx c) {
+  __shared__ uint32_t ss[1024 / kWave + 1];
+  const uint32_t nSmall = c.ctr->numSmall;
+  const uint32_t base0 = c.ctr->nodeCount;
+  uint32_t run = 0;
+  for (uint32_t b0 = 0; b0 < nSmall; b0 += 1024) {
+    uint32_t i = b0 + threadIdx.x;
+    uint32_t v = i < nSmall ? c.smallCount[i] - 1 : 0;
+    uint32_t tot;
+    uint32_t ex = block_excl_scan<1024>(v, ss, &tot);
+    if (i < nSmall) c.smallBase[i] = base0 + run + ex;
+    run += tot;
+  }
+  if (threadIdx.x == 0) c.ctr->nodeCount = base0 + run;
+}
+
+// Copy local nodes to their global ids, renumbering inner-node children.
+__global__ __launch_bounds__(kWave) void k_small_emit(Ctx c, Node *out) {
+  const uint32_t nSmall = c.ctr->numSmall;
+  for (uint32_t w = blockIdx.x; w < nSmall; w += gridDim.x) {
+    const SmallSeg sg = c.small[w];
+    const uint32_t cnt = c.smallCount[w];
+    const uint32_t base = c.smallBase[w];
+    const Node *loc = c.loc + 2 * (size_t)sg.begin;
+    for (uint32_t j = lane_id(); j < cnt; j += kWave) {
+      Node nd = loc[j];
+      if (oc_count(nd.oc) == 0) nd.oc = pack_oc(base + (uint32_t)oc_offset(nd.oc) - 1, 0);
+      out[j == 0 ? sg.node : base + j - 1] = nd;
+    }
+  }
+}
+
+__global__ void k_copy_top(const Node *src, Node *dst, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // Bounds of the level-phase inner nodes, deepest level first.
-__global__ void k_refit_top(Ctx c, int level) {
+__global__ void k_refit_top(Ctx c, Node *nodes, int level) {
   const uint32_t b = c.levelStart[level], e = c.levelStart[level + 1];
   for (uint32_t i = b + blockIdx.x * blockDim.x + threadIdx.x; i < e; i += gridDim.x * blockDim.x) {
-    Node *nd = &c.nodes[c.topNodes[i]];
+    Node *nd = &nodes[c.topNodes[i]];
     uint64_t off = oc_offset(nd->oc);
-    const Node l = c.nodes[off], r = c.nodes[off + 1];
+    const Node l = nodes[off], r = nodes[off + 1];
     for (int d = 0; d < 3; d++) {
       nd->lo[d] = fminf(l.lo[d], r.lo[d]);
       nd->hi[d] = fmaxf(l.hi[d], r.hi[d]);
@@ -732,6 +884,11 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
   CUBQL_CHECK(scr.alloc(&c.fixup, c.maxLarge));
   CUBQL_CHECK(scr.alloc(&c.topNodes, n + 1));
   CUBQL_CHECK(scr.alloc(&c.levelStart, maxLevels + 2));
+  CUBQL_CHECK(scr.alloc(&c.levelItems, maxLevels + 2));
+  CUBQL_CHECK(scr.alloc(&c.blockBounds, 8 * (size_t)numBlocks + 8));
+  CUBQL_CHECK(scr.alloc(&c.loc, 2 * (size_t)n + 64));
+  CUBQL_CHECK(scr.alloc(&c.smallCount, (size_t)n + 1));
+  CUBQL_CHECK(scr.alloc(&c.smallBase, (size_t)n + 1));
 
   uint32_t *primIDs = nullptr;
   CUBQL_CHECK(mem_alloc(mem, (void **)&primIDs, (size_t)n * sizeof(uint32_t), s));
@@ -757,10 +914,14 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
     numCU = 256;
   const uint32_t partGrid = std::min<uint32_t>(maxTiles, (uint32_t)numCU * 8);
   int level = 0;
+  std::vector<std::pair<int, int>> partSlots;  // (profile slot, level)
   while (true) {
     for (int i = 0; i < kLevelsPerSync && level < (int)maxLevels; i++, level++) {
       k_plan<<<1, 1024, 0, s>>>(c, level);
+      int ps = prof_begin(s);
       k_partition<<<partGrid, kBlock, 0, s>>>(c, level);
+      prof_end(ps, s, "sm_partition");
+      if (ps >= 0) partSlots.push_back({ps, level});
       k_emit<<<1, 1024, 0, s>>>(c, level);
       k_fixup<<<64, kBlock, 0, s>>>(c, level);
     }
@@ -772,29 +933,49 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
     if (hc->numLarge[0] == 0 && hc->numLarge[1] == 0) break;
     if (level >= (int)maxLevels) return fail((set_error("tree deeper than %u levels", maxLevels), CUBQL_ERR_INTERNAL));
   }
-  const uint32_t numSmall = ((const Counters *)hw)->numSmall;
-  const uint32_t numValid = ((const Counters *)hw)->numValid;
-  // number of levels that actually had work
-  const uint32_t numTop = ((const Counters *)hw)->numTop;
+  const Counters hc0 = *(const Counters *)hw;
+  const uint32_t numSmall = hc0.numSmall;
+  const uint32_t numValid = hc0.numValid;
+  const uint32_t numTop = hc0.numTop;
+  const uint32_t topCount = hc0.nodeCount;  // ids [0, topCount) belong to the level phase
+  int smallSlot = -1;
+  const uint32_t smallGrid = std::max<uint32_t>(1u, std::min<uint32_t>(numSmall, (uint32_t)numCU * 16));
   if (numSmall) {
-    uint32_t g = std::min<uint32_t>(numSmall, (uint32_t)numCU * 16);
-    k_small<<<g, kWave, 0, s>>>(c);
+    smallSlot = prof_begin(s);
+    k_small<<<smallGrid, kWave, 0, s>>>(c);
+    prof_end(smallSlot, s, "sm_small");
+    k_small_scan<<<1, 1024, 0, s>>>(c);
   }
-  if (numTop) {
-    for (int l = level - 1; l >= 0; l--) k_refit_top<<<64, 256, 0, s>>>(c, l);
-  }
-  if (hipGetLastError() != hipSuccess) return fail((set_error("finish launch failed"), CUBQL_ERR_HIP));
+  if (hipGetLastError() != hipSuccess) return fail((set_error("small-phase launch failed"), CUBQL_ERR_HIP));
   if (hipMemcpyAsync(hw, c.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return fail((set_error("final sync failed"), CUBQL_ERR_HIP));
   const uint32_t numNodes = numValid ? ((const Counters *)hw)->nodeCount : 0;
+  if (prof_on()) {
+    // algorithmic bytes: partition reads and writes one 16-byte item per
+    // prim of the level; the small phase reads each item once, writes its
+    // primID and the nodes it creates (gathering boxes for non-point input)
+    std::vector<uint32_t> li(level + 1, 0);
+    if (hipMemcpy(li.data(), c.levelItems, sizeof(uint32_t) * level, hipMemcpyDeviceToHost) == hipSuccess) {
+      for (auto &ps : partSlots) prof_set(ps.first, 32.0 * li[ps.second], li[ps.second]);
+    }
+    const bool pts = hc0.allPoints != 0;
+    const double smallNodes = (double)numNodes - (double)topCount + (double)numSmall;
+    prof_set(smallSlot, (16.0 + 4.0 + (pts ? 0.0 : 24.0)) * numValid + 32.0 * smallNodes, numValid);
+    prof_flush();
+  }
   Node *nodes = nullptr;
   st = mem_alloc(mem, (void **)&nodes, std::max<size_t>(numNodes, 2) * sizeof(Node), s);
   if (st) return fail(st);
   if (numNodes) {
-    if (hipMemcpyAsync(nodes, c.nodes, (size_t)numNodes * sizeof(Node), hipMemcpyDeviceToDevice, s) != hipSuccess) {
+    k_copy_top<<<std::max<uint32_t>(1u, std::min<uint32_t>(div_up(topCount, 256), 1024)), 256, 0, s>>>(
+        c.nodes, nodes, topCount);
+    if (numSmall) k_small_emit<<<smallGrid, kWave, 0, s>>>(c, nodes);
+    if (numTop)
+      for (int l = level - 1; l >= 0; l--) k_refit_top<<<64, 256, 0, s>>>(c, nodes, l);
+    if (hipGetLastError() != hipSuccess) {
       mem_free(mem, nodes, s);
-      return fail((set_error("node copy failed"), CUBQL_ERR_HIP));
+      return fail((set_error("emit launch failed"), CUBQL_ERR_HIP));
     }
   }
   bvh->nodes = (cubql_node3f *)nodes;

@@ -50,3 +50,28 @@ def lcg_uniform(seed: int, n: int, lo: float = 0.0, hi: float = 1.0, dims: int =
 def points_to_boxes(p: np.ndarray) -> np.ndarray:
     p = np.ascontiguousarray(p, dtype=np.float32).reshape(-1, 3)
     return np.ascontiguousarray(np.concatenate([p, p], axis=1))
+
+
+def lcg_uniform_torch(seed: int, n: int, lo: float = 0.0, hi: float = 1.0, dims: int = 3,
+                      device="cuda"):
+    """Same values as lcg_uniform, computed on the device with int64 tensor ops."""
+    import torch
+    M = 0xFFFFFFFF
+    idx = torch.arange(n, dtype=torch.int64, device=device)
+    v0 = torch.full((n,), seed & M, dtype=torch.int64, device=device)
+    v1 = idx & M
+    s0 = 0
+    for _ in range(8):
+        s0 = (s0 + 0x9E3779B9) & M
+        v0 = (v0 + ((((v1 << 4) & M) + 0xA341316C) & M ^ ((v1 + s0) & M)
+                    ^ (((v1 >> 5) + 0xC8013EA4) & M))) & M
+        v1 = (v1 + ((((v0 << 4) & M) + 0xAD90777D) & M ^ ((v0 + s0) & M)
+                    ^ (((v0 >> 5) + 0x7E95761E) & M))) & M
+    state = v0
+    out = torch.empty((n, dims), dtype=torch.float32, device=device)
+    span = float(hi) - float(lo)
+    for d in range(dims):
+        state = (state * 1664525 + 1013904223) & M
+        r = (state & 0x00FFFFFF).to(torch.float32) / 16777216.0
+        out[:, d] = (float(lo) + r.to(torch.float64) * span).to(torch.float32)
+    return out

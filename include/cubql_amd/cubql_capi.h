@@ -140,6 +140,21 @@ int cubql_knn_points_f3(const cubql_bvh3f *bvh, const float *d_points,
                         float *d_outSqrDist, int32_t *d_numFound,
                         cubql_stream_t s);
 
+/* ------------------------------------------------------------ profiling */
+
+/* Event-based per-kernel timing, the counterpart of the reference's
+   compile-time CUBQL_PROFILE timers (builder/cuda/sm_builder.h:24-66).  Off
+   by default.  When on, builders and queries bracket their dominant kernels
+   with HIP events on the caller's stream and accumulate, per kernel name,
+   the number of (non-empty) launches, total milliseconds, algorithmic bytes
+   and work units (prims or queries).  Kernel names: "sm_partition",
+   "sm_small", "fcp_points", "knn_points", "radix_onesweep", ... */
+void cubql_profile_enable(int on);
+void cubql_profile_reset(void);
+/* 0 on success, -1 if the kernel has no record. */
+int cubql_profile_query(const char *kernel, uint64_t *launches, double *totalMs,
+                        double *bytes, uint64_t *units);
+
 #ifdef __cplusplus
 }
 #endif

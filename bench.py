@@ -167,6 +167,34 @@ def main():
                     "avg_launch_ms": dom["avg_launch_ms"],
                     "bytes_per_launch": dom["bytes_per_launch"]}
 
+    # ---- Morton builders (30-bit keys per the workload, 63-bit reference default)
+    radix = {}
+    for name, fn in [("radix30", cq.cuda.radix30Builder), ("radix63", cq.cuda.radixBuilder)]:
+        fn(boxes, cfg, mem=mem).free()
+        torch.cuda.synchronize()
+        r0 = time.perf_counter()
+        reps = max(3, args.steps // 2)
+        for _ in range(reps):
+            fn(boxes, cfg, mem=mem).free()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - r0) / reps * 1e3
+        radix[name] = {"ms_per_build": ms, "mprims_s": N / (ms * 1e-3) / 1e6}
+    print(f"[bench] radix30 {radix['radix30']['mprims_s']:.1f} Mprims/s, "
+          f"radix63 {radix['radix63']['mprims_s']:.1f} Mprims/s", file=sys.stderr, flush=True)
+    cq.profile.reset()
+    cq.profile.enable(True)
+    for _ in range(2):
+        cq.cuda.radix30Builder(boxes, cfg, mem=mem).free()
+    torch.cuda.synchronize()
+    cq.profile.enable(False)
+    for name in ["radix_keys", "radix_onesweep", "radix_hierarchy", "refit"]:
+        st = cq.profile.query(name)
+        if st and st["launches"]:
+            kern["radix30:" + name] = {"launches_per_build": st["launches"] / 2,
+                                       "avg_launch_ms": st["ms"] / st["launches"],
+                                       "ms_per_build": st["ms"] / 2,
+                                       "gbs": st["bytes"] / (st["ms"] * 1e-3) / 1e9}
+
     # ---- queries on one tree
     bvh = cq.gpuBuilder(boxes, cfg, mem=mem)
     nq = args.fcp_queries
@@ -217,6 +245,7 @@ def main():
                                f"points in [0,1]^3; {nq:,} closest-point and {nk:,} kNN (k=8) "
                                "queries uniform in [0,1]^3",
                    "points": N, "leaf_threshold": 8, "num_nodes": num_nodes},
+        "radix_builds": radix,
         "fcp_mqueries_s": nq / fcp_ms / 1e3,
         "knn_mqueries_s": nk / knn_ms / 1e3,
         "kernels": kern,

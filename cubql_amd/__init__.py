@@ -94,6 +94,8 @@ def lib():
         "cubql_refit_f3": (I, [P(BVH3f), V, V, P(MemResource)]),
         "cubql_fcp_points_f3": (I, [P(BVH3f), V, V, U32, F, V, V, V]),
         "cubql_knn_points_f3": (I, [P(BVH3f), V, V, U32, I, F, V, V, V, V]),
+        "cubql_radix_sort_pairs_u32": (I, [V, V, V, V, U32, I, V]),
+        "cubql_radix_sort_pairs_u64": (I, [V, V, V, V, U32, I, V]),
         "cubql_profile_enable": (None, [I]),
         "cubql_profile_reset": (None, []),
         "cubql_profile_query": (I, [C.c_char_p, P(C.c_uint64), P(C.c_double), P(C.c_double),
@@ -313,6 +315,20 @@ class profile:  # noqa: N801
                                      C.byref(un)) != 0:
             return None
         return {"launches": n.value, "ms": ms.value, "bytes": by.value, "units": un.value}
+
+
+def radix_sort_pairs(keys: torch.Tensor, vals: torch.Tensor, key_bits: int, stream=None):
+    """Stable sort of (keys, vals) in place on key bits [0, key_bits).
+
+    keys: int32 (read as uint32) or int64 (read as uint64); vals: int32."""
+    if keys.dtype not in (torch.int32, torch.int64) or vals.dtype != torch.int32:
+        raise CubqlError("keys must be int32/int64 and vals int32")
+    n = keys.numel()
+    kt, vt = torch.empty_like(keys), torch.empty_like(vals)
+    fn = lib().cubql_radix_sort_pairs_u32 if keys.dtype == torch.int32 else lib().cubql_radix_sort_pairs_u64
+    _check(fn(_ptr(keys), _ptr(vals), _ptr(kt), _ptr(vt), n, int(key_bits),
+              C.c_void_p(_stream_handle(stream))), "radix_sort_pairs")
+    return keys, vals
 
 
 def points_to_boxes(p: torch.Tensor) -> torch.Tensor:

@@ -1,0 +1,397 @@
+// radix_builder.hip -- Morton-code (radix) BVH builder for gfx950.
+//
+// Replaces cuBQL::cuda::radixBuilder (builder/cuda/radix.h:591-771):
+//   1. centroid bounds of the valid prims (radix.h:199-241) and the uniform
+//      quantizer scale = 2^b * (1/max(largest extent, 1e-20)) (radix.h:41-53);
+//   2. Morton key per valid prim: cell = min(u32((c-lo)*scale), 2^b-1),
+//      bit i of x,y,z -> bits 3i,3i+1,3i+2 (radix.h:54-62, 283-394);
+//      valid prims are compacted in ascending primID order (the reference's
+//      atomic tail swap, radix.h:412-416, is order-nondeterministic);
+//   3. stable key/value radix sort (radix_sort.hip, replacing cub SortPairs);
+//   4. top-down hierarchy: a range of <= T prims is a leaf, a range of equal
+//      keys is split in the middle, otherwise at the first key that has the
+//      highest differing bit of its first and last key set (findSplit,
+//      radix.h:446-482; createNodes radix.h:501-570), one launch per level,
+//      children allocated with one atomic per workgroup;
+//   5. exact bottom-up refit (refit.hip).
+// 63-bit keys (21 bits per axis) are the reference default; 30-bit keys
+// (10 bits per axis) are its radix.h:34-38 variant.
+#include "block_ops.h"
+#include "build_common.h"
+#include "common.h"
+#include "profile.h"
+#include "radix_sort.h"
+#include "refit.h"
+
+#include <vector>
+
+namespace cubql {
+namespace rb {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 8;
+constexpr int kTile = kBlock * kItems;
+constexpr int kLevelsPerSync = 8;
+constexpr int kMaxLevels = 160;  // > 63 key bits + log2(2^31) mid splits
+
+enum { C_NUMVALID = 0, C_NODECOUNT = 1, C_LEVEL0 = 8 };  // counter slots
+
+struct RCtx {
+  const cubql_box3f *boxes;
+  uint32_t n, T;
+  int bitsPerDim;
+  uint32_t *blockCount;
+  float *blockBounds;
+  uint32_t *ctr;      // [C_LEVEL0 + level]: open-node count of each level
+  float *quant;       // lo[3], scale
+  uint32_t *ids;      // compacted primIDs (sorted in place by the sort)
+  Node *nodes;        // temp nodes, capacity 2*n+2
+  uint2 *ranges;      // [b,e) per node
+  uint32_t *open[2];  // open node ids per level (ping-pong)
+};
+
+__global__ __launch_bounds__(kBlock) void k_rcount(RCtx c) {
+  __shared__ float sb[12 * (kBlock / kWave)];
+  __shared__ uint32_t ss[kBlock / kWave + 1];
+  const uint32_t base = blockIdx.x * kTile;
+  uint32_t cnt = 0;
+  float v[12];
+  for (int j = 0; j < 3; j++) { v[j] = INFINITY; v[j + 3] = -INFINITY; v[j + 6] = INFINITY; v[j + 9] = -INFINITY; }
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    uint32_t i = base + k * kBlock + threadIdx.x;
+    if (i < c.n) {
+      cubql_box3f b = load_box(c.boxes, i);
+      if (box_is_valid(b)) {
+        cnt++;
+        for (int d = 0; d < 3; d++) {
+          float ce = 0.5f * (b.lower[d] + b.upper[d]);
+          v[d] = fminf(v[d], ce);
+          v[d + 3] = fmaxf(v[d + 3], ce);
+        }
+      }
+    }
+  }
+  block_reduce_bounds<kBlock>(v, sb);
+  uint32_t tot;
+  block_excl_scan<kBlock>(cnt, ss, &tot);
+  if (threadIdx.x < 6) c.blockBounds[blockIdx.x * 8 + threadIdx.x] = v[threadIdx.x];
+  if (threadIdx.x == 0) c.blockCount[blockIdx.x] = tot;
+}
+
+// single workgroup: tile offsets, centroid bounds, quantizer, root node
+__global__ __launch_bounds__(1024) void k_rscan(RCtx c, uint32_t numBlocks) {
+  __shared__ uint32_t ss[1024 / kWave + 1];
+  __shared__ float sr[6][1024 / kWave];
+  uint32_t run = 0;
+  float red[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t b0 = 0; b0 < numBlocks; b0 += 1024) {
+    uint32_t i = b0 + threadIdx.x;
+    uint32_t v = i < numBlocks ? c.blockCount[i] : 0;
+    if (i < numBlocks) {
+      const float *bb = c.blockBounds + (size_t)i * 8;
+      for (int j = 0; j < 3; j++) { red[j] = fminf(red[j], bb[j]); red[j + 3] = fmaxf(red[j + 3], bb[j + 3]); }
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_scan<1024>(v, ss, &tot);
+    if (i < numBlocks) c.blockCount[i] = run + ex;
+    run += tot;
+  }
+  for (int j = 0; j < 6; j++) {
+    float r = j < 3 ? wave_min(red[j]) : wave_max(red[j]);
+    if (lane_id() == 0) sr[j][threadIdx.x / kWave] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float lo[3], hi[3];
+    for (int j = 0; j < 6; j++) {
+      float r = sr[j][0];
+      for (int w = 1; w < 1024 / kWave; w++) r = j < 3 ? fminf(r, sr[j][w]) : fmaxf(r, sr[j][w]);
+      (j < 3 ? lo[j] : hi[j - 3]) = r;
+    }
+    // Quantizer::init (radix.h:46-53)
+    float ext = fmaxf(fmaxf(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+    float inv = 1.f / fmaxf(ext, 1e-20f);
+    float scale = (float)(1u << c.bitsPerDim) * inv;
+    c.quant[0] = lo[0]; c.quant[1] = lo[1]; c.quant[2] = lo[2]; c.quant[3] = scale;
+    c.ctr[C_NUMVALID] = run;
+    c.ctr[C_NODECOUNT] = 2;
+    Node z;
+    for (int d = 0; d < 3; d++) { z.lo[d] = 0.f; z.hi[d] = 0.f; }
+    z.oc = 0;
+    c.nodes[1] = z;  // radix.h:585-586
+    if (run > 0) {
+      c.ranges[0] = make_uint2(0u, run);
+      c.open[0][0] = 0;
+      c.ctr[C_LEVEL0] = 1;
+    } else {
+      c.ctr[C_LEVEL0] = 0;
+    }
+  }
+}
+
+__device__ __forceinline__ uint64_t spread21(uint32_t v) {
+  uint64_t x = v & 0x1fffffu;
+  x = (x | (x << 32)) & 0x1f00000000ffffull;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+  uint32_t x = v & 0x3ffu;
+  x = (x | (x << 16)) & 0x030000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+template <typename K>
+__device__ __forceinline__ K morton(const float c[3], const float *q, int bits) {
+  const uint32_t cmax = (1u << bits) - 1;
+  uint32_t cell[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    float f = (c[d] - q[d]) * q[3];  // radix.h:59, (P-bias)*scale
+    uint32_t u = (uint32_t)f;
+    cell[d] = min(u, cmax);
+  }
+  if (sizeof(K) == 8)
+    return (K)(spread21(cell[0]) | (spread21(cell[1]) << 1) | (spread21(cell[2]) << 2));
+  return (K)(spread10(cell[0]) | (spread10(cell[1]) << 1) | (spread10(cell[2]) << 2));
+}
+
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_rkeys(RCtx c, K *keys) {
+  __shared__ uint32_t sr[kItems * (kBlock / kWave) + 1];
+  __shared__ float sq[4];
+  if (threadIdx.x < 4) sq[threadIdx.x] = c.quant[threadIdx.x];
+  __syncthreads();
+  const uint32_t base = blockIdx.x * kTile;
+  bool valid[kItems];
+  K key[kItems];
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    uint32_t i = base + k * kBlock + threadIdx.x;
+    valid[k] = false;
+    key[k] = 0;
+    if (i < c.n) {
+      cubql_box3f b = load_box(c.boxes, i);
+      valid[k] = box_is_valid(b);
+      float ce[3] = {0.5f * (b.lower[0] + b.upper[0]), 0.5f * (b.lower[1] + b.upper[1]),
+                     0.5f * (b.lower[2] + b.upper[2])};
+      key[k] = morton<K>(ce, sq, c.bitsPerDim);
+    }
+  }
+  uint32_t rank[kItems];
+  block_rank_striped<kBlock, kItems>(valid, rank, sr);
+  const uint32_t off = c.blockCount[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    if (!valid[k]) continue;
+    const uint32_t i = base + k * kBlock + threadIdx.x;
+    keys[off + rank[k]] = key[k];
+    c.ids[off + rank[k]] = i;
+  }
+}
+
+template <typename K>
+__device__ __forceinline__ uint32_t find_split(const K *keys, uint32_t b, uint32_t e) {
+  const K fk = keys[b], lk = keys[e - 1];
+  if (fk == lk) return (uint32_t)(((uint64_t)b + e) / 2);  // radix.h:455-465
+  const int nbits = 8 * (int)sizeof(K);
+  const int nm = sizeof(K) == 8 ? __clzll((unsigned long long)(fk ^ lk)) : __clz((uint32_t)(fk ^ lk));
+  const K search = lk & (~(K)0 << (nbits - 1 - nm));
+  uint32_t lo = b, hi = e;
+  while (hi > lo) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (keys[mid] < search) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// One tree level: each open node becomes a leaf or splits in two.
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_rlevel(RCtx c, const K *keys, int level) {
+  __shared__ uint32_t ss[kBlock / kWave + 1];
+  __shared__ uint32_t sNode, sOpen;
+  const uint32_t nOpen = c.ctr[C_LEVEL0 + level];
+  const uint32_t *open = c.open[level & 1];
+  uint32_t *next = c.open[(level & 1) ^ 1];
+  for (uint32_t b0 = blockIdx.x * kBlock; b0 < nOpen; b0 += gridDim.x * kBlock) {
+    const uint32_t i = b0 + threadIdx.x;
+    uint32_t node = 0, split = 0;
+    uint2 r = make_uint2(0, 0);
+    bool doSplit = false;
+    if (i < nOpen) {
+      node = open[i];
+      r = c.ranges[node];
+      const uint32_t size = r.y - r.x;
+      if (size > c.T) {
+        split = find_split<K>(keys, r.x, r.y);
+        doSplit = true;
+      } else {
+        c.nodes[node].oc = pack_oc(r.x, size);
+      }
+    }
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<kBlock>(doSplit ? 1u : 0u, ss, &tot);
+    if (threadIdx.x == 0 && tot) {
+      sNode = atomicAdd(&c.ctr[C_NODECOUNT], 2 * tot);
+      sOpen = atomicAdd(&c.ctr[C_LEVEL0 + level + 1], 2 * tot);
+    }
+    __syncthreads();
+    if (doSplit) {
+      const uint32_t child = sNode + 2 * ex;
+      c.nodes[node].oc = pack_oc(child, 0);
+      c.ranges[child] = make_uint2(r.x, split);
+      c.ranges[child + 1] = make_uint2(split, r.y);
+      next[sOpen + 2 * ex] = child;
+      next[sOpen + 2 * ex + 1] = child + 1;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_rinit_levels(uint32_t *ctr) {
+  for (int i = threadIdx.x; i < kMaxLevels + 2; i += blockDim.x) ctr[C_LEVEL0 + 1 + i] = 0;
+}
+
+template <typename K>
+int build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, int bitsPerDim,
+          hipStream_t s, const cubql_mem_resource *mem) {
+  Scratch scr(mem, s);
+  RCtx c{};
+  c.boxes = boxes;
+  c.n = n;
+  c.T = T;
+  c.bitsPerDim = bitsPerDim;
+  const uint32_t numBlocks = div_up(n, kTile);
+  K *keysA = nullptr, *keysB = nullptr;
+  uint32_t *idsB = nullptr;
+  CUBQL_CHECK(scr.alloc(&c.blockCount, numBlocks + 1));
+  CUBQL_CHECK(scr.alloc(&c.blockBounds, 8 * (size_t)numBlocks + 8));
+  CUBQL_CHECK(scr.alloc(&c.ctr, C_LEVEL0 + kMaxLevels + 8));
+  CUBQL_CHECK(scr.alloc(&c.quant, 8));
+  CUBQL_CHECK(scr.alloc(&keysA, n));
+  CUBQL_CHECK(scr.alloc(&keysB, n));
+  CUBQL_CHECK(scr.alloc(&idsB, n));
+  CUBQL_CHECK(scr.alloc(&c.nodes, 2 * (size_t)n + 2));
+  CUBQL_CHECK(scr.alloc(&c.ranges, 2 * (size_t)n + 2));
+  CUBQL_CHECK(scr.alloc(&c.open[0], (size_t)n + 2));
+  CUBQL_CHECK(scr.alloc(&c.open[1], (size_t)n + 2));
+  uint32_t *primIDs = nullptr;
+  CUBQL_CHECK(mem_alloc(mem, (void **)&primIDs, (size_t)n * sizeof(uint32_t), s));
+  c.ids = primIDs;
+  auto fail = [&](int st) {
+    mem_free(mem, primIDs, s);
+    return st;
+  };
+  k_rinit_levels<<<1, 256, 0, s>>>(c.ctr);
+  int ks = prof_begin(s);
+  k_rcount<<<numBlocks, kBlock, 0, s>>>(c);
+  k_rscan<<<1, 1024, 0, s>>>(c, numBlocks);
+  k_rkeys<K><<<numBlocks, kBlock, 0, s>>>(c, keysA);
+  prof_end(ks, s, "radix_keys");
+  prof_set(ks, 2.0 * 24.0 * n + (sizeof(K) + 4.0) * n, n);
+  if (hipGetLastError() != hipSuccess) return fail((set_error("radix key launch failed"), CUBQL_ERR_HIP));
+  const int keyBits = 3 * bitsPerDim;
+  K *kRes = nullptr;
+  uint32_t *vRes = nullptr;
+  int st;
+  if (sizeof(K) == 8)
+    st = radix_sort_pairs_dev_u64((uint64_t *)keysA, primIDs, (uint64_t *)keysB, idsB, &c.ctr[C_NUMVALID],
+                                  n, keyBits, s, mem, (uint64_t **)&kRes, &vRes);
+  else
+    st = radix_sort_pairs_dev_u32((uint32_t *)keysA, primIDs, (uint32_t *)keysB, idsB, &c.ctr[C_NUMVALID],
+                                  n, keyBits, s, mem, (uint32_t **)&kRes, &vRes);
+  if (st) return fail(st);
+  if (vRes != primIDs) {
+    if (hipMemcpyAsync(primIDs, vRes, (size_t)n * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return fail((set_error("id copy failed"), CUBQL_ERR_HIP));
+  }
+  // hierarchy, level by level, with a completion check every few levels
+  int dev = 0, numCU = 256;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&numCU, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    numCU = 256;
+  const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(div_up(n, kBlock), (uint32_t)numCU * 8));
+  std::vector<uint32_t> hc(C_LEVEL0 + kMaxLevels + 8);
+  int level = 0;
+  int hs = prof_begin(s);
+  while (true) {
+    for (int i = 0; i < kLevelsPerSync && level < kMaxLevels; i++, level++)
+      k_rlevel<K><<<grid, kBlock, 0, s>>>(c, kRes, level);
+    if (hipGetLastError() != hipSuccess) return fail((set_error("level launch failed"), CUBQL_ERR_HIP));
+    if (hipMemcpyAsync(hc.data(), c.ctr, sizeof(uint32_t) * hc.size(), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return fail((set_error("level sync failed"), CUBQL_ERR_HIP));
+    if (hc[C_LEVEL0 + level] == 0) break;
+    if (level >= kMaxLevels) return fail((set_error("radix hierarchy deeper than %d levels", kMaxLevels), CUBQL_ERR_INTERNAL));
+  }
+  prof_end(hs, s, "radix_hierarchy");
+  const uint32_t numValid = hc[C_NUMVALID];
+  const uint32_t numNodes = numValid ? hc[C_NODECOUNT] : 0;
+  prof_set(hs, 8.0 * numNodes, numNodes);
+  Node *nodes = nullptr;
+  st = mem_alloc(mem, (void **)&nodes, std::max<size_t>(numNodes, 2) * sizeof(Node), s);
+  if (st) return fail(st);
+  if (numNodes &&
+      hipMemcpyAsync(nodes, c.nodes, (size_t)numNodes * sizeof(Node), hipMemcpyDeviceToDevice, s) != hipSuccess) {
+    mem_free(mem, nodes, s);
+    return fail((set_error("node copy failed"), CUBQL_ERR_HIP));
+  }
+  bvh->nodes = (cubql_node3f *)nodes;
+  bvh->numNodes = numNodes;
+  bvh->primIDs = primIDs;
+  bvh->numPrims = numValid;
+  int rs = prof_begin(s);
+  st = refit_tree(bvh, boxes, s, mem);
+  prof_end(rs, s, "refit");
+  prof_set(rs, 24.0 * numValid + 64.0 * numNodes, numNodes);
+  if (prof_on()) {
+    if (hipStreamSynchronize(s) == hipSuccess) prof_flush();
+  }
+  return st;
+}
+
+}  // namespace rb
+}  // namespace cubql
+
+using namespace cubql;
+
+static int radix_entry(const char *fn, cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n,
+                       const cubql_build_config *cfg, cubql_stream_t stream,
+                       const cubql_mem_resource *mem, int bits) {
+  clear_error();
+  if (!bvh || !cfg || (n && !boxes)) {
+    set_error("%s: null argument", fn);
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  if (n > 0x7fffffffu) {
+    set_error("%s: too many prims", fn);
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  if (n == 0) return CUBQL_OK;
+  // radix.h:596-600: threshold = min(makeLeafThreshold, maxAllowedLeafSize), 0 -> 1
+  int T = cfg->makeLeafThreshold > 0 ? std::min(cfg->makeLeafThreshold, cfg->maxAllowedLeafSize) : 1;
+  T = std::max(1, std::min(T, 65535));
+  hipStream_t s = (hipStream_t)stream;
+  if (bits == 21) return rb::build<uint64_t>(bvh, boxes, n, (uint32_t)T, 21, s, mem);
+  return rb::build<uint32_t>(bvh, boxes, n, (uint32_t)T, 10, s, mem);
+}
+
+extern "C" int cubql_radix_builder_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes, uint32_t numPrims,
+                                      const cubql_build_config *cfg, cubql_stream_t s,
+                                      const cubql_mem_resource *mem) {
+  return radix_entry("cubql_radix_builder_f3", bvh, d_boxes, numPrims, cfg, s, mem, 21);
+}
+
+extern "C" int cubql_radix30_builder_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes, uint32_t numPrims,
+                                        const cubql_build_config *cfg, cubql_stream_t s,
+                                        const cubql_mem_resource *mem) {
+  return radix_entry("cubql_radix30_builder_f3", bvh, d_boxes, numPrims, cfg, s, mem, 10);
+}

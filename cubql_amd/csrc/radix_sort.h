@@ -5,24 +5,26 @@
 // keys (uint32 or uint64) with uint32 values, ascending, stable.
 //
 // Design: 8-bit digits.  One histogram pass computes the digit counts of
-// every sort pass at once (per-workgroup histograms, reduced without global
+// every sort pass at once (per-tile histograms, summed without global
 // atomics).  Each sort pass is then a single read + single write of the
-// data: a workgroup claims a 4096-key tile (ticket order), ranks keys per
-// digit with wave64 ballots, resolves each digit's global offset by a
-// decoupled look-back over earlier tiles, stages the tile in LDS in digit
-// order and writes it out so that consecutive lanes write consecutive
-// addresses.
+// data: a workgroup claims a 4096-key tile (ticket order, so it only waits
+// on tiles that running workgroups hold), ranks keys per digit with wave64
+// ballots, resolves each digit's global offset by a decoupled look-back over
+// earlier tiles, stages the tile in LDS in digit order and writes it out so
+// that consecutive lanes write consecutive addresses.
 #pragma once
 #include "common.h"
 
 namespace cubql {
 
-// Sort (keys, vals) of length n.  Scratch comes from `mem`.  The sorted
-// result is written to (keysOut, valsOut); keysIn/valsIn are clobbered.
-// Only bits [0, keyBits) are sorted on.
-int radix_sort_pairs_u32(uint32_t *keysIn, uint32_t *valsIn, uint32_t *keysOut, uint32_t *valsOut,
-                         uint32_t n, int keyBits, hipStream_t s, const cubql_mem_resource *mem);
-int radix_sort_pairs_u64(uint64_t *keysIn, uint32_t *valsIn, uint64_t *keysOut, uint32_t *valsOut,
-                         uint32_t n, int keyBits, hipStream_t s, const cubql_mem_resource *mem);
+// Sorts (kA, vA) of *nDev (device-side count, <= nMax) pairs on key bits
+// [0, keyBits).  (kB, vB) is the ping-pong buffer; *kRes / *vRes receive the
+// buffers holding the result (kA/vA when the pass count is even).
+int radix_sort_pairs_dev_u32(uint32_t *kA, uint32_t *vA, uint32_t *kB, uint32_t *vB,
+                             const uint32_t *nDev, uint32_t nMax, int keyBits, hipStream_t s,
+                             const cubql_mem_resource *mem, uint32_t **kRes, uint32_t **vRes);
+int radix_sort_pairs_dev_u64(uint64_t *kA, uint32_t *vA, uint64_t *kB, uint32_t *vB,
+                             const uint32_t *nDev, uint32_t nMax, int keyBits, hipStream_t s,
+                             const cubql_mem_resource *mem, uint64_t **kRes, uint32_t **vRes);
 
 }  // namespace cubql

@@ -24,6 +24,7 @@
 #include "block_ops.h"
 #include "common.h"
 #include "wave_ops.h"
+#include "build_common.h"
 #include "profile.h"
 
 #include <vector>
@@ -90,58 +91,6 @@ struct Ctx {
   uint32_t maxLarge;
 };
 
-__device__ inline bool box_is_valid(const cubql_box3f &b) { return b.lower[0] <= b.upper[0]; }
-
-__device__ inline cubql_box3f load_box(const cubql_box3f *p, uint32_t i) {
-  const float2 *f = reinterpret_cast<const float2 *>(p + i);
-  float2 a = f[0], b = f[1], c = f[2];
-  cubql_box3f r;
-  r.lower[0] = a.x; r.lower[1] = a.y; r.lower[2] = b.x;
-  r.upper[0] = b.y; r.upper[1] = c.x; r.upper[2] = c.y;
-  return r;
-}
-
-__device__ inline float coord(const float4 &v, int d) {
-  return d == 0 ? v.x : (d == 1 ? v.y : v.z);
-}
-
-// Split choice (sm_builder.h:194-220 / spatialMedian.h:90-91): widest axis
-// by strict '>' (ties to the lower axis), pos = midpoint.
-__device__ inline void choose_split(const float lo[3], const float hi[3], int &dim, float &pos) {
-  float s0 = hi[0] - lo[0], s1 = hi[1] - lo[1], s2 = hi[2] - lo[2];
-  dim = s2 > fmaxf(s0, s1) ? 2 : (s1 > s0 ? 1 : 0);
-  float l = dim == 0 ? lo[0] : (dim == 1 ? lo[1] : lo[2]);
-  float h = dim == 0 ? hi[0] : (dim == 1 ? hi[1] : hi[2]);
-  pos = (l + h) / 2.f;
-}
-
-// block-wide min/max reduction of 12 floats (left lo/hi, right lo/hi)
-__device__ inline void block_reduce_bounds(float (&v)[12], float *smem /*12*NW*/) {
-  constexpr int NW = kBlock / kWave;
-  const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
-#pragma unroll
-  for (int j = 0; j < 12; j++) {
-    bool isMin = (j % 6) < 3;
-    v[j] = isMin ? wave_min(v[j]) : wave_max(v[j]);
-  }
-  if (lane == 0) {
-#pragma unroll
-    for (int j = 0; j < 12; j++) smem[j * NW + wid] = v[j];
-  }
-  __syncthreads();
-  if (threadIdx.x < 12) {
-    int j = threadIdx.x;
-    bool isMin = (j % 6) < 3;
-    float r = smem[j * NW];
-    for (int w = 1; w < NW; w++) r = isMin ? fminf(r, smem[j * NW + w]) : fmaxf(r, smem[j * NW + w]);
-    smem[j * NW] = r;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < 12; j++) v[j] = smem[j * NW];
-  __syncthreads();
-}
-
 // ---------------------------------------------------------------- init pass
 // Pass 1: per-tile valid count, root centroid bounds, point-data flag.
 __global__ __launch_bounds__(kBlock) void k_count(Ctx c) {
@@ -168,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_count(Ctx c) {
       }
     }
   }
-  block_reduce_bounds(v, sb);
+  block_reduce_bounds<kBlock>(v, sb);
   uint32_t tot;
   block_excl_scan<kBlock>(cnt, ss, &tot);
   bool allPts = __syncthreads_and(pts);
@@ -429,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
       v[o + 4] = fmaxf(v[o + 4], it[j].y);
       v[o + 5] = fmaxf(v[o + 5], it[j].z);
     }
-    block_reduce_bounds(v, sb);
+    block_reduce_bounds<kBlock>(v, sb);
     if (threadIdx.x < 12) {
       int side = threadIdx.x / 6;
       int j = threadIdx.x % 6;
@@ -541,7 +490,7 @@ __global__ __launch_bounds__(kBlock) void k_fixup(Ctx c, int level) {
       v[0] = fminf(v[0], it.x); v[1] = fminf(v[1], it.y); v[2] = fminf(v[2], it.z);
       v[3] = fmaxf(v[3], it.x); v[4] = fmaxf(v[4], it.y); v[5] = fmaxf(v[5], it.z);
     }
-    block_reduce_bounds(v, sb);
+    block_reduce_bounds<kBlock>(v, sb);
     if (threadIdx.x < 6) g.cb[threadIdx.x] = enc_f(v[threadIdx.x]);
     __syncthreads();
   }

@@ -116,6 +116,18 @@ int cubql_free_f3(cubql_bvh3f *bvh, cubql_stream_t s,
 int cubql_refit_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes,
                    cubql_stream_t s, const cubql_mem_resource *mem);
 
+/* Stable ascending sort of (key, value) pairs on key bits [0, keyBits) --
+   the operation the reference builders obtain from
+   cub::DeviceRadixSort::SortPairs (builder/cuda/radix.h:666-687,
+   rebinMortonBuilder.h:1242-1263).  Result in (d_keys, d_vals); the Tmp
+   arrays (n entries each) are scratch. */
+int cubql_radix_sort_pairs_u32(uint32_t *d_keys, uint32_t *d_vals,
+                               uint32_t *d_keysTmp, uint32_t *d_valsTmp,
+                               uint32_t n, int keyBits, cubql_stream_t s);
+int cubql_radix_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals,
+                               uint64_t *d_keysTmp, uint32_t *d_valsTmp,
+                               uint32_t n, int keyBits, cubql_stream_t s);
+
 /* -------------------------------------------------------------- queries */
 
 /* Batched cuBQL::points::findClosest (queries/points/findClosest.h:49-62,

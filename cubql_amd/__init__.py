@@ -96,6 +96,10 @@ def lib():
         "cubql_knn_points_f3": (I, [P(BVH3f), V, V, U32, I, F, V, V, V, V]),
         "cubql_radix_sort_pairs_u32": (I, [V, V, V, V, U32, I, V]),
         "cubql_radix_sort_pairs_u64": (I, [V, V, V, V, U32, I, V]),
+        "cubql_triangle_boxes_f3": (I, [V, V, U32, V, V]),
+        "cubql_fcp_triangles_f3": (I, [P(BVH3f), V, V, V, U32, F, V, V, V, V]),
+        "cubql_fcp_triangles_backward_f3": (I, [V, V, V, U32, V, V, V, V, V]),
+        "cubql_points_distance_backward_f3": (I, [V, V, U32, I, V, V, V, V, V]),
         "cubql_profile_enable": (None, [I]),
         "cubql_profile_reset": (None, []),
         "cubql_profile_query": (I, [C.c_char_p, P(C.c_uint64), P(C.c_double), P(C.c_double),
@@ -294,6 +298,70 @@ class points:  # noqa: N801 -- mirrors cuBQL::points
                                          float(sqrMaxSearchDist), _ptr(ids), _ptr(d2), _ptr(nf),
                                          C.c_void_p(_stream_handle(stream))), "findKNN")
         return ids, d2, nf
+
+
+class triangles:  # noqa: N801 -- mirrors cuBQL::triangles
+    @staticmethod
+    def boxes(tris: Optional[torch.Tensor], verts: torch.Tensor, stream=None) -> torch.Tensor:
+        """Triangle::bounds() per triangle: [numTris, 6] float32."""
+        verts = _as_points(verts, "verts")
+        n = (tris.shape[0] if tris is not None else verts.shape[0] // 3)
+        out = torch.empty((n, 6), dtype=torch.float32, device=verts.device)
+        _check(lib().cubql_triangle_boxes_f3(_ptr(_as_tris(tris)), _ptr(verts), n, _ptr(out),
+                                             C.c_void_p(_stream_handle(stream))), "triangle_boxes")
+        return out
+
+    @staticmethod
+    def findClosest(bvh: BinaryBVH, tris: Optional[torch.Tensor], verts: torch.Tensor,
+                    queries: torch.Tensor, sqrMaxSearchDist: float = math.inf,
+                    want_points: bool = True, stream=None):
+        """Batched triangles::findClosest: (int32 triIDs, float32 sqrDist, closest points)."""
+        verts = _as_points(verts, "verts")
+        queries = _as_points(queries, "queries")
+        nq = queries.shape[0]
+        ids = torch.empty(nq, dtype=torch.int32, device=queries.device)
+        d2 = torch.empty(nq, dtype=torch.float32, device=queries.device)
+        cp = torch.empty((nq, 3), dtype=torch.float32, device=queries.device) if want_points else None
+        _check(lib().cubql_fcp_triangles_f3(C.byref(bvh.c), _ptr(_as_tris(tris)), _ptr(verts),
+                                            _ptr(queries), nq, float(sqrMaxSearchDist), _ptr(ids),
+                                            _ptr(d2), _ptr(cp), C.c_void_p(_stream_handle(stream))),
+               "triangles.findClosest")
+        return ids, d2, cp
+
+    @staticmethod
+    def backward(tris: Optional[torch.Tensor], verts: torch.Tensor, queries: torch.Tensor,
+                 tri_ids: torch.Tensor, grad_dist: torch.Tensor, grad_queries=None,
+                 grad_verts=None, stream=None):
+        """Adjoint of the unsigned distances (cubql_fcp_triangles_backward_f3)."""
+        verts = _as_points(verts, "verts")
+        queries = _as_points(queries, "queries")
+        _check(lib().cubql_fcp_triangles_backward_f3(
+            _ptr(_as_tris(tris)), _ptr(verts), _ptr(queries), queries.shape[0],
+            _ptr(tri_ids.contiguous()), _ptr(grad_dist.contiguous()), _ptr(grad_queries),
+            _ptr(grad_verts), C.c_void_p(_stream_handle(stream))), "triangles.backward")
+        return grad_queries, grad_verts
+
+
+def points_backward(data: torch.Tensor, queries: torch.Tensor, ids: torch.Tensor,
+                    grad_dist: torch.Tensor, grad_queries=None, grad_points=None, stream=None):
+    """Adjoint of point distances for fcp (ids [nq]) or kNN (ids [nq,k])."""
+    data = _as_points(data, "data")
+    queries = _as_points(queries, "queries")
+    nq = queries.shape[0]
+    k = 1 if ids.dim() == 1 else ids.shape[1]
+    _check(lib().cubql_points_distance_backward_f3(
+        _ptr(data), _ptr(queries), nq, k, _ptr(ids.contiguous()), _ptr(grad_dist.contiguous()),
+        _ptr(grad_queries), _ptr(grad_points), C.c_void_p(_stream_handle(stream))),
+        "points_backward")
+    return grad_queries, grad_points
+
+
+def _as_tris(t: Optional[torch.Tensor]):
+    if t is None:
+        return None
+    if t.dtype != torch.int32 or not t.is_cuda:
+        raise CubqlError("tris must be an int32 CUDA tensor of shape [n,3]")
+    return t.contiguous()
 
 
 class profile:  # noqa: N801

@@ -1,0 +1,357 @@
+// triangles.hip -- closest point on a triangle mesh, and distance adjoints.
+//
+// Forward: replaces cuBQL::triangles::findClosest / closestPoint
+// (queries/triangles/findClosest.h:56-152) as a batched kernel, one query
+// per lane, over an indexed mesh (tris[3i..3i+2] index verts) or, when tris
+// is NULL, the reference's non-indexed Triangle array (verts[9i..9i+8]).
+// The per-triangle test is the reference's: N = (b-a)x(c-a); outside if
+// N == 0 or q lies beyond any edge plane, then the closest of the three
+// clamped edge projections; else the plane distance along
+// normalize(N) = N*(1/sqrt(N.N)).  Float32 with contraction off; among equal
+// distances the lowest triangle id wins.
+//
+// Backward (no reference counterpart): recomputes the closest feature of
+// the saved triangle id -- no traversal state is stored -- and applies the
+// envelope theorem: for d = |q - P|, P = sum_j w_j v_j,
+//   dd/dq = (q-P)/d,  dd/dv_j = -w_j (q-P)/d,   (0 at d == 0)
+// with w the barycentric weights of P (face: projection barycentrics,
+// edge: (1-t, t), vertex: one-hot).  Vertex gradients are scatter-added.
+#include "common.h"
+#include "cubql_amd/traversal.hip.h"
+#include "profile.h"
+
+namespace cubql {
+namespace tri {
+
+constexpr int kBlock = 256;
+constexpr int kFastStack = 64;
+constexpr int kDeepStack = 1024;
+
+struct F3 { float x, y, z; };
+__device__ __forceinline__ F3 ld3(const float *p, size_t i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+__device__ __forceinline__ F3 sub(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 add(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ F3 mul(float s, F3 a) { return {s * a.x, s * a.y, s * a.z}; }
+// dot (vec.h:458-466) and cross (vec.h:232-237), no contraction
+__device__ __forceinline__ float dot(F3 a, F3 b) {
+#pragma clang fp contract(off)
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+__device__ __forceinline__ F3 cross(F3 a, F3 b) {
+#pragma clang fp contract(off)
+  return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+__device__ __forceinline__ float clamp01(float t) { return fminf(fmaxf(t, 0.f), 1.f); }
+
+__device__ __forceinline__ void tri_verts(const int32_t *tris, const float *verts, uint32_t t, F3 &a,
+                                          F3 &b, F3 &c) {
+  if (tris) {
+    a = ld3(verts, (size_t)tris[3 * (size_t)t + 0]);
+    b = ld3(verts, (size_t)tris[3 * (size_t)t + 1]);
+    c = ld3(verts, (size_t)tris[3 * (size_t)t + 2]);
+  } else {
+    a = ld3(verts, 3 * (size_t)t);
+    b = ld3(verts, 3 * (size_t)t + 1);
+    c = ld3(verts, 3 * (size_t)t + 2);
+  }
+}
+
+__device__ __forceinline__ float edge_closest(F3 a, F3 b, F3 p, float &t, F3 &cp) {
+  F3 ab = sub(b, a);
+  t = clamp01(dot(sub(p, a), ab) / dot(ab, ab));
+  cp = add(a, mul(t, ab));
+  F3 d = sub(cp, p);
+  return dot(d, d);
+}
+
+// closestPoint(Triangle) (findClosest.h:93-122); returns squared distance,
+// closest point P and (when WANT_W) barycentric weights of P.
+template <bool WANT_W>
+__device__ __forceinline__ float tri_closest(F3 a, F3 b, F3 c, F3 q, F3 &P, float (&w)[3]) {
+  const F3 N = cross(sub(b, a), sub(c, a));
+  const bool outside = (N.x == 0.f && N.y == 0.f && N.z == 0.f) ||
+                       (dot(sub(q, a), cross(sub(b, a), N)) >= 0.f) ||
+                       (dot(sub(q, b), cross(sub(c, b), N)) >= 0.f) ||
+                       (dot(sub(q, c), cross(sub(a, c), N)) >= 0.f);
+  if (outside) {
+    float t0, t1, t2;
+    F3 c0, c1, c2;
+    const float d0 = edge_closest(a, b, q, t0, c0);
+    const float d1 = edge_closest(b, c, q, t1, c1);
+    const float d2 = edge_closest(c, a, q, t2, c2);
+    // sequential strictly-closer acceptance: first minimal edge wins
+    float best = d0;
+    P = c0;
+    int e = 0;
+    if (d1 < best) { best = d1; P = c1; e = 1; }
+    if (d2 < best) { best = d2; P = c2; e = 2; }
+    if (WANT_W) {
+      if (e == 0) { w[0] = 1.f - t0; w[1] = t0; w[2] = 0.f; }
+      else if (e == 1) { w[0] = 0.f; w[1] = 1.f - t1; w[2] = t1; }
+      else { w[0] = t2; w[1] = 0.f; w[2] = 1.f - t2; }
+    }
+    return best;
+  }
+  const float nn = dot(N, N);
+  const F3 Nn = mul(1.f / sqrtf(nn), N);
+  const float sd = dot(sub(q, a), Nn);
+  P = sub(q, mul(sd, Nn));
+  if (WANT_W) {
+    const float wa = dot(cross(sub(c, b), sub(P, b)), N) / nn;
+    const float wb = dot(cross(sub(a, c), sub(P, c)), N) / nn;
+    w[0] = wa;
+    w[1] = wb;
+    w[2] = 1.f - wa - wb;
+  }
+  return sd * sd;
+}
+
+struct Overflow { uint32_t count, deepFailed; };
+
+template <int STACK>
+__device__ __forceinline__ bool fcp_tri_one(const cubql_bvh3f &bvh, const int32_t *tris, const float *verts,
+                                            F3 q, float sqrMax, int32_t &best, float &bestD, F3 &bestP) {
+  best = -1;
+  bestD = sqrMax;
+  bestP = {0.f, 0.f, 0.f};
+  auto prim = [&](uint32_t t) -> float {
+    F3 a, b, c, P;
+    tri_verts(tris, verts, t, a, b, c);
+    float w[3];
+    const float d = tri_closest<false>(a, b, c, q, P, w);
+    if (d < bestD || (d == bestD && best >= 0 && (int32_t)t < best)) {
+      bestD = d;
+      best = (int32_t)t;
+      bestP = P;
+    }
+    return bestD;
+  };
+  return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, sqrMax, prim);
+}
+
+__device__ __forceinline__ void write_result(uint32_t qi, int32_t best, float bestD, F3 P, int32_t *ids,
+                                             float *d2, float *cp) {
+  ids[qi] = best;
+  if (d2) d2[qi] = best >= 0 ? bestD : INFINITY;
+  if (cp) {
+    cp[3 * (size_t)qi] = P.x;
+    cp[3 * (size_t)qi + 1] = P.y;
+    cp[3 * (size_t)qi + 2] = P.z;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fcp_tris(cubql_bvh3f bvh, const int32_t *tris, const float *verts,
+                                                     const float *queries, uint32_t nq, float sqrMax,
+                                                     int32_t *ids, float *d2, float *cp, Overflow *ovf,
+                                                     uint32_t *ovfList) {
+  const uint32_t qi = blockIdx.x * kBlock + threadIdx.x;
+  if (qi >= nq) return;
+  const F3 q = ld3(queries, qi);
+  int32_t best;
+  float bestD;
+  F3 P;
+  if (!fcp_tri_one<kFastStack>(bvh, tris, verts, q, sqrMax, best, bestD, P)) {
+    ovfList[atomicAdd(&ovf->count, 1u)] = qi;
+    return;
+  }
+  write_result(qi, best, bestD, P, ids, d2, cp);
+}
+
+__global__ __launch_bounds__(64) void k_fcp_tris_deep(cubql_bvh3f bvh, const int32_t *tris, const float *verts,
+                                                      const float *queries, float sqrMax, int32_t *ids,
+                                                      float *d2, float *cp, Overflow *ovf,
+                                                      const uint32_t *ovfList) {
+  const uint32_t n = ovf->count;
+  for (uint32_t i = blockIdx.x * 64 + threadIdx.x; i < n; i += gridDim.x * 64) {
+    const uint32_t qi = ovfList[i];
+    const F3 q = ld3(queries, qi);
+    int32_t best;
+    float bestD;
+    F3 P;
+    if (!fcp_tri_one<kDeepStack>(bvh, tris, verts, q, sqrMax, best, bestD, P)) {
+      atomicAdd(&ovf->deepFailed, 1u);
+      best = -1;
+    }
+    write_result(qi, best, bestD, P, ids, d2, cp);
+  }
+}
+
+__global__ void k_tri_boxes(const int32_t *tris, const float *verts, uint32_t n, cubql_box3f *boxes) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    F3 a, b, c;
+    tri_verts(tris, verts, i, a, b, c);
+    cubql_box3f bx;  // Triangle::bounds() (Triangle.h:35-36)
+    bx.lower[0] = fminf(fminf(a.x, b.x), c.x);
+    bx.lower[1] = fminf(fminf(a.y, b.y), c.y);
+    bx.lower[2] = fminf(fminf(a.z, b.z), c.z);
+    bx.upper[0] = fmaxf(fmaxf(a.x, b.x), c.x);
+    bx.upper[1] = fmaxf(fmaxf(a.y, b.y), c.y);
+    bx.upper[2] = fmaxf(fmaxf(a.z, b.z), c.z);
+    boxes[i] = bx;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_tri_backward(const int32_t *tris, const float *verts,
+                                                         const float *queries, uint32_t nq,
+                                                         const int32_t *ids, const float *gradDist,
+                                                         float *gradQ, float *gradV) {
+  const uint32_t qi = blockIdx.x * kBlock + threadIdx.x;
+  if (qi >= nq) return;
+  const int32_t t = ids[qi];
+  const F3 q = ld3(queries, qi);
+  F3 gq = {0.f, 0.f, 0.f};
+  if (t >= 0) {
+    F3 a, b, c, P;
+    tri_verts(tris, verts, (uint32_t)t, a, b, c);
+    float w[3];
+    const float s = tri_closest<true>(a, b, c, q, P, w);
+    const float d = sqrtf(s);
+    if (d > 0.f) {
+      const float g = gradDist[qi] / d;
+      const F3 dir = sub(q, P);
+      gq = mul(g, dir);
+      if (gradV) {
+        for (int j = 0; j < 3; j++) {
+          const size_t v = tris ? (size_t)tris[3 * (size_t)t + j] : 3 * (size_t)t + j;
+          atomicAdd(&gradV[3 * v + 0], -w[j] * gq.x);
+          atomicAdd(&gradV[3 * v + 1], -w[j] * gq.y);
+          atomicAdd(&gradV[3 * v + 2], -w[j] * gq.z);
+        }
+      }
+    }
+  }
+  if (gradQ) {
+    gradQ[3 * (size_t)qi] = gq.x;
+    gradQ[3 * (size_t)qi + 1] = gq.y;
+    gradQ[3 * (size_t)qi + 2] = gq.z;
+  }
+}
+
+// points: d = |q-p| for each (query, neighbour) entry; k entries per query.
+__global__ __launch_bounds__(kBlock) void k_point_backward(const float *points, const float *queries,
+                                                           uint32_t nq, int k, const int32_t *ids,
+                                                           const float *gradDist, float *gradQ,
+                                                           float *gradP) {
+  const uint32_t qi = blockIdx.x * kBlock + threadIdx.x;
+  if (qi >= nq) return;
+  const F3 q = ld3(queries, qi);
+  F3 gq = {0.f, 0.f, 0.f};
+  for (int j = 0; j < k; j++) {
+    const int32_t p = ids[(size_t)qi * k + j];
+    if (p < 0) continue;
+    const F3 diff = sub(q, ld3(points, (size_t)p));
+    const float d = sqrtf(dot(diff, diff));
+    if (!(d > 0.f)) continue;
+    const F3 g = mul(gradDist[(size_t)qi * k + j] / d, diff);
+    gq = add(gq, g);
+    if (gradP) {
+      atomicAdd(&gradP[3 * (size_t)p + 0], -g.x);
+      atomicAdd(&gradP[3 * (size_t)p + 1], -g.y);
+      atomicAdd(&gradP[3 * (size_t)p + 2], -g.z);
+    }
+  }
+  if (gradQ) {
+    gradQ[3 * (size_t)qi] = gq.x;
+    gradQ[3 * (size_t)qi + 1] = gq.y;
+    gradQ[3 * (size_t)qi + 2] = gq.z;
+  }
+}
+
+__global__ void k_fill_nf(uint32_t nq, int32_t *ids, float *d2) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
+    ids[i] = -1;
+    if (d2) d2[i] = INFINITY;
+  }
+}
+
+}  // namespace tri
+}  // namespace cubql
+
+using namespace cubql;
+
+extern "C" int cubql_triangle_boxes_f3(const int32_t *d_tris, const float *d_verts, uint32_t numTris,
+                                       cubql_box3f *d_boxes, cubql_stream_t s) {
+  clear_error();
+  if (numTris == 0) return CUBQL_OK;
+  if (!d_verts || !d_boxes) {
+    set_error("cubql_triangle_boxes_f3: null argument");
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  tri::k_tri_boxes<<<std::min<uint32_t>(div_up(numTris, 256), 8192), 256, 0, (hipStream_t)s>>>(
+      d_tris, d_verts, numTris, d_boxes);
+  CUBQL_LAUNCH_CHECK();
+  return CUBQL_OK;
+}
+
+extern "C" int cubql_fcp_triangles_f3(const cubql_bvh3f *bvh, const int32_t *d_tris, const float *d_verts,
+                                      const float *d_queries, uint32_t numQueries, float sqrMax,
+                                      int32_t *d_outTriIDs, float *d_outSqrDist, float *d_outClosest,
+                                      cubql_stream_t stream) {
+  clear_error();
+  if (!bvh || (numQueries && (!d_verts || !d_queries || !d_outTriIDs))) {
+    set_error("cubql_fcp_triangles_f3: null argument");
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  if (numQueries == 0) return CUBQL_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (bvh->numNodes == 0) {
+    tri::k_fill_nf<<<256, 256, 0, s>>>(numQueries, d_outTriIDs, d_outSqrDist);
+    CUBQL_LAUNCH_CHECK();
+    return CUBQL_OK;
+  }
+  Scratch scr(nullptr, s);
+  tri::Overflow *ovf = nullptr;
+  uint32_t *ovfList = nullptr;
+  CUBQL_CHECK(scr.alloc(&ovf, 1));
+  CUBQL_CHECK(scr.alloc(&ovfList, numQueries));
+  CUBQL_CHECK(fill_u32((uint32_t *)ovf, 0u, 2, s));
+  int ps = prof_begin(s);
+  tri::k_fcp_tris<<<div_up(numQueries, tri::kBlock), tri::kBlock, 0, s>>>(
+      *bvh, d_tris, d_verts, d_queries, numQueries, sqrMax, d_outTriIDs, d_outSqrDist, d_outClosest, ovf,
+      ovfList);
+  prof_end(ps, s, "fcp_triangles");
+  prof_set(ps, 20.0 * numQueries, numQueries);
+  tri::k_fcp_tris_deep<<<64, 64, 0, s>>>(*bvh, d_tris, d_verts, d_queries, sqrMax, d_outTriIDs,
+                                         d_outSqrDist, d_outClosest, ovf, ovfList);
+  CUBQL_LAUNCH_CHECK();
+  return CUBQL_OK;
+}
+
+extern "C" int cubql_fcp_triangles_backward_f3(const int32_t *d_tris, const float *d_verts,
+                                               const float *d_queries, uint32_t numQueries,
+                                               const int32_t *d_triIDs, const float *d_gradDist,
+                                               float *d_gradQueries, float *d_gradVerts,
+                                               cubql_stream_t s) {
+  clear_error();
+  if (numQueries == 0) return CUBQL_OK;
+  if (!d_verts || !d_queries || !d_triIDs || !d_gradDist) {
+    set_error("cubql_fcp_triangles_backward_f3: null argument");
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  int ps = prof_begin((hipStream_t)s);
+  tri::k_tri_backward<<<div_up(numQueries, tri::kBlock), tri::kBlock, 0, (hipStream_t)s>>>(
+      d_tris, d_verts, d_queries, numQueries, d_triIDs, d_gradDist, d_gradQueries, d_gradVerts);
+  prof_end(ps, (hipStream_t)s, "triangles_backward");
+  prof_set(ps, (12.0 + 4.0 + 4.0 + 12.0 + 3 * 12.0 + 3 * 24.0) * numQueries, numQueries);
+  CUBQL_LAUNCH_CHECK();
+  return CUBQL_OK;
+}
+
+extern "C" int cubql_points_distance_backward_f3(const float *d_points, const float *d_queries,
+                                                 uint32_t numQueries, int k, const int32_t *d_ids,
+                                                 const float *d_gradDist, float *d_gradQueries,
+                                                 float *d_gradPoints, cubql_stream_t s) {
+  clear_error();
+  if (numQueries == 0) return CUBQL_OK;
+  if (!d_points || !d_queries || !d_ids || !d_gradDist || k < 1) {
+    set_error("cubql_points_distance_backward_f3: invalid argument");
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  int ps = prof_begin((hipStream_t)s);
+  tri::k_point_backward<<<div_up(numQueries, tri::kBlock), tri::kBlock, 0, (hipStream_t)s>>>(
+      d_points, d_queries, numQueries, k, d_ids, d_gradDist, d_gradQueries, d_gradPoints);
+  prof_end(ps, (hipStream_t)s, "points_backward");
+  prof_set(ps, (12.0 + 12.0 + k * (4.0 + 4.0 + 12.0 + 24.0)) * numQueries, numQueries);
+  CUBQL_LAUNCH_CHECK();
+  return CUBQL_OK;
+}

@@ -75,3 +75,88 @@ def lcg_uniform_torch(seed: int, n: int, lo: float = 0.0, hi: float = 1.0, dims:
         r = (state & 0x00FFFFFF).to(torch.float32) / 16777216.0
         out[:, d] = (float(lo) + r.to(torch.float64) * span).to(torch.float32)
     return out
+
+
+def box_muller(u1: np.ndarray, u2: np.ndarray) -> np.ndarray:
+    """Standard normal deviates from two uniform [0,1) streams (float64)."""
+    u1 = 1.0 - u1.astype(np.float64)  # (0,1]: log is finite
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2.astype(np.float64))
+
+
+def lcg_draws(seed: int, n: int, k: int) -> np.ndarray:
+    """k successive LCG<8>(seed, i) draws in [0,1) per element i: [n, k]."""
+    return lcg_uniform(seed, n, 0.0, 1.0, dims=k)
+
+
+def icosphere(subdiv: int):
+    """Unit icosphere: 20*4^subdiv triangles, 10*4^subdiv+2 vertices (float64)."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    V = np.array([(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t),
+                  (0, -1, -t), (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)],
+                 dtype=np.float64)
+    F = np.array([(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9),
+                  (5, 11, 4), (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2),
+                  (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+                  (8, 6, 7), (9, 8, 1)], dtype=np.int64)
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    for _ in range(subdiv):
+        a, b, c = F[:, 0], F[:, 1], F[:, 2]
+        e = np.concatenate([np.stack([a, b], 1), np.stack([b, c], 1), np.stack([c, a], 1)])
+        e.sort(axis=1)
+        key = e[:, 0] * (len(V) + 1) + e[:, 1]
+        uk, inv = np.unique(key, return_inverse=True)
+        ea, eb = uk // (len(V) + 1), uk % (len(V) + 1)
+        mid = V[ea] + V[eb]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = len(V)
+        V = np.concatenate([V, mid])
+        m = len(F)
+        ab, bc, ca = base + inv[:m], base + inv[m:2 * m], base + inv[2 * m:]
+        F = np.concatenate([np.stack([a, ab, ca], 1), np.stack([b, bc, ab], 1),
+                            np.stack([c, ca, bc], 1), np.stack([ab, bc, ca], 1)])
+    return V, F
+
+
+def noisy_icosphere(subdiv: int = 8, seed: int = 4, sigma: float = 0.01):
+    """Icosphere of radius 1 with vertices displaced radially by N(0, sigma^2)
+    (Box-Muller over LCG<8>(seed, vertex) draws).  Returns (verts f32 [V,3],
+    tris i32 [F,3])."""
+    V, F = icosphere(subdiv)
+    u = lcg_draws(seed, len(V), 2)
+    r = 1.0 + sigma * box_muller(u[:, 0], u[:, 1])
+    return (V * r[:, None]).astype(np.float32), F.astype(np.int32)
+
+
+def mesh_surface_queries(verts: np.ndarray, tris: np.ndarray, n: int, seed: int = 5,
+                         sigma: float = 0.05) -> np.ndarray:
+    """n queries: a triangle chosen with probability proportional to its area,
+    a uniform point on it (Triangle::sample folding, Triangle.h:41-45), plus
+    an N(0, sigma^2) offset per axis.  All draws from LCG<8>(seed, i)."""
+    V = verts.astype(np.float64)
+    a, b, c = V[tris[:, 0]], V[tris[:, 1]], V[tris[:, 2]]
+    area = np.linalg.norm(np.cross(b - a, c - a), axis=1)
+    cdf = np.cumsum(area)
+    cdf /= cdf[-1]
+    out = np.empty((n, 3), dtype=np.float32)
+    chunk = 1 << 21
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        u = _lcg_range(seed, s, e, 9)
+        t = np.minimum(np.searchsorted(cdf, u[:, 0], side="right"), len(tris) - 1)
+        uu, vv = u[:, 1].copy(), u[:, 2].copy()
+        fold = uu + vv >= 1.0
+        uu[fold], vv[fold] = 1.0 - uu[fold], 1.0 - vv[fold]
+        p = (1.0 - uu - vv)[:, None] * a[t] + uu[:, None] * b[t] + vv[:, None] * c[t]
+        g = np.stack([box_muller(u[:, 3 + 2 * j], u[:, 4 + 2 * j]) for j in range(3)], 1)
+        out[s:e] = (p + sigma * g).astype(np.float32)
+    return out
+
+
+def _lcg_range(seed: int, b: int, e: int, dims: int) -> np.ndarray:
+    """LCG<8>(seed, i) draws for i in [b, e)."""
+    state = lcg_init(seed, np.arange(b, e, dtype=np.uint64))
+    out = np.empty((e - b, dims), dtype=np.float64)
+    for d in range(dims):
+        state = (state * np.uint64(1664525) + np.uint64(1013904223)) & _M32
+        out[:, d] = ((state & np.uint64(0x00FFFFFF)).astype(np.float32) / np.float32(16777216.0))
+    return out

@@ -152,6 +152,57 @@ int cubql_knn_points_f3(const cubql_bvh3f *bvh, const float *d_points,
                         float *d_outSqrDist, int32_t *d_numFound,
                         cubql_stream_t s);
 
+/* Triangle i = (verts[tris[3i]], verts[tris[3i+1]], verts[tris[3i+2]]); with
+   d_tris == NULL it is (verts[3*(3i)], ...): the reference's non-indexed
+   Triangle array (queries/triangles/Triangle.h:26-32). */
+
+/* Builder input boxes for triangles: Triangle::bounds() per triangle
+   (Triangle.h:35-36), the generateBoxes kernel of
+   samples/s02_distanceToTriangleMesh/distanceToTriangleMesh.cu:55-64. */
+int cubql_triangle_boxes_f3(const int32_t *d_tris, const float *d_verts,
+                            uint32_t numTris, cubql_box3f *d_boxes,
+                            cubql_stream_t s);
+
+/* Batched cuBQL::triangles::findClosest (queries/triangles/findClosest.h:
+   132-152; per-triangle test closestPoint :93-122).  For each query:
+   outTriIDs[i] = closest triangle within sqrt(sqrMax) or -1 (lowest id on
+   equal distances), outSqrDist[i] its squared distance (float32, no FMA
+   contraction), outClosest[3i..3i+2] the closest point FCPResult::P.
+   outSqrDist and outClosest may be NULL. */
+int cubql_fcp_triangles_f3(const cubql_bvh3f *bvh, const int32_t *d_tris,
+                           const float *d_verts, const float *d_queries,
+                           uint32_t numQueries, float sqrMaxSearchDist,
+                           int32_t *d_outTriIDs, float *d_outSqrDist,
+                           float *d_outClosest, cubql_stream_t s);
+
+/* Adjoint of the unsigned distance d_i = |q_i - P_i| of a forward
+   cubql_fcp_triangles_f3 call (no reference counterpart: the reference has
+   no differentiation).  Recomputes the closest feature from the saved
+   triangle ids; with upstream gradients g_i = dL/dd_i:
+     gradQueries[i] = g_i (q_i - P_i)/d_i                 (overwritten)
+     gradVerts[v]  += -g_i w_ij (q_i - P_i)/d_i           (accumulated)
+   where w are the barycentric weights of P on the triangle; 0 at d = 0 and
+   for ids < 0.  Either output may be NULL. */
+int cubql_fcp_triangles_backward_f3(const int32_t *d_tris, const float *d_verts,
+                                    const float *d_queries, uint32_t numQueries,
+                                    const int32_t *d_triIDs,
+                                    const float *d_gradDist,
+                                    float *d_gradQueries, float *d_gradVerts,
+                                    cubql_stream_t s);
+
+/* Adjoint of point distances d_ij = |q_i - p_{ids[i*k+j]}| from a forward
+   fcp (k = 1) or kNN call (no reference counterpart):
+     gradQueries[i]  = sum_j g_ij (q_i - p)/d_ij          (overwritten)
+     gradPoints[p]  += -g_ij (q_i - p)/d_ij               (accumulated)
+   ids < 0 and d = 0 contribute nothing.  Either output may be NULL. */
+int cubql_points_distance_backward_f3(const float *d_points,
+                                      const float *d_queries,
+                                      uint32_t numQueries, int k,
+                                      const int32_t *d_ids,
+                                      const float *d_gradDist,
+                                      float *d_gradQueries,
+                                      float *d_gradPoints, cubql_stream_t s);
+
 /* ------------------------------------------------------------ profiling */
 
 /* Event-based per-kernel timing, the counterpart of the reference's

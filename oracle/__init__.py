@@ -284,3 +284,89 @@ def radix_build(boxes, bits_per_dim: int, leaf_threshold: int) -> Tree:
     b = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
     lib().orc_refit(_p(tree.nodes), nn, _p(tree.prim_ids), _p(b))
     return tree
+
+
+# ----------------------------------------------------------------------------
+# triangles and gradients
+def _tri_sigs():
+    L = lib()
+    V, I64, F = C.c_void_p, C.c_int64, C.c_float
+    for name, (res, args) in {
+        "orc_tri_closest": (F, [V, V, V, V, V, V]),
+        "orc_tri_boxes": (None, [V, V, I64, V]),
+        "orc_fcp_tris": (None, [V, V, V, V, V, I64, F, V, V, V]),
+        "orc_fcp_tris_brute_f64": (None, [V, V, I64, V, I64, V, V]),
+        "orc_tri_grad_f64": (None, [V, V, V, I64, V, V, V, V]),
+        "orc_point_grad_f64": (None, [V, V, I64, C.c_int, V, V, V, V]),
+    }.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+def _tris_arg(tris):
+    return None if tris is None else np.ascontiguousarray(tris, dtype=np.int32)
+
+
+def tri_boxes(tris, verts):
+    t, v = _tris_arg(tris), np.ascontiguousarray(verts, dtype=np.float32)
+    n = t.shape[0] if t is not None else v.shape[0] // 3
+    out = np.empty((n, 6), dtype=np.float32)
+    _tri_sigs().orc_tri_boxes(_p(t), _p(v), n, _p(out))
+    return out
+
+
+def fcp_tris(tree: Tree, tris, verts, queries, sqr_max=np.inf):
+    t, v = _tris_arg(tris), np.ascontiguousarray(verts, dtype=np.float32)
+    Q = np.ascontiguousarray(queries, dtype=np.float32)
+    nq = Q.shape[0]
+    ids = np.empty(nq, dtype=np.int32)
+    d2 = np.empty(nq, dtype=np.float32)
+    P = np.empty((nq, 3), dtype=np.float32)
+    _tri_sigs().orc_fcp_tris(_p(tree.nodes), _p(tree.prim_ids), _p(t), _p(v), _p(Q), nq,
+                             float(sqr_max), _p(ids), _p(d2), _p(P))
+    return ids, d2, P
+
+
+def fcp_tris_brute_f64(tris, verts, queries):
+    t, v = _tris_arg(tris), np.ascontiguousarray(verts, dtype=np.float32)
+    Q = np.ascontiguousarray(queries, dtype=np.float32)
+    n = t.shape[0] if t is not None else v.shape[0] // 3
+    ids = np.empty(Q.shape[0], dtype=np.int32)
+    d2 = np.empty(Q.shape[0], dtype=np.float64)
+    _tri_sigs().orc_fcp_tris_brute_f64(_p(t), _p(v), n, _p(Q), Q.shape[0], _p(ids), _p(d2))
+    return ids, d2
+
+
+def tri_grad_f64(tris, verts, queries, ids, g):
+    t, v = _tris_arg(tris), np.ascontiguousarray(verts, dtype=np.float32)
+    Q = np.ascontiguousarray(queries, dtype=np.float32)
+    gq = np.zeros((Q.shape[0], 3), dtype=np.float64)
+    gv = np.zeros((v.shape[0], 3), dtype=np.float64)
+    _tri_sigs().orc_tri_grad_f64(_p(t), _p(v), _p(Q), Q.shape[0],
+                                 _p(np.ascontiguousarray(ids, dtype=np.int32)),
+                                 _p(np.ascontiguousarray(g, dtype=np.float64)), _p(gq), _p(gv))
+    return gq, gv
+
+
+def point_grad_f64(points, queries, ids, g):
+    P = np.ascontiguousarray(points, dtype=np.float32)
+    Q = np.ascontiguousarray(queries, dtype=np.float32)
+    ids2 = np.ascontiguousarray(ids, dtype=np.int32)
+    k = 1 if ids2.ndim == 1 else ids2.shape[1]
+    gq = np.zeros((Q.shape[0], 3), dtype=np.float64)
+    gp = np.zeros((P.shape[0], 3), dtype=np.float64)
+    _tri_sigs().orc_point_grad_f64(_p(P), _p(Q), Q.shape[0], k, _p(ids2),
+                                   _p(np.ascontiguousarray(g, dtype=np.float64)), _p(gq), _p(gp))
+    return gq, gp
+
+
+def tri_closest(a, b, c, q):
+    """(sqrDist, P[3], w[3]) of the float32 restatement for one triangle."""
+    f = lambda x: np.ascontiguousarray(x, dtype=np.float32)  # noqa: E731
+    a, b, c, q = f(a), f(b), f(c), f(q)
+    P = np.empty(3, dtype=np.float32)
+    w = np.empty(3, dtype=np.float32)
+    d = _tri_sigs().orc_tri_closest(_p(a), _p(b), _p(c), _p(q), _p(P), _p(w))
+    return d, P, w

@@ -587,6 +587,232 @@ int orc_radix_hierarchy(const uint64_t *keys, int n, int T, orc_node3f *nodes) {
   return numNodes;
 }
 
+// ---------------------------------------------------------------------------
+// triangles (queries/triangles/findClosest.h:56-152)
+namespace {
+struct V3 { float x, y, z; };
+inline V3 v3(const float *p) { return {p[0], p[1], p[2]}; }
+inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 mul(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+// dot: ((0 + x*x') + y*y') + z*z' (vec.h:458-466); the leading 0 is exact
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// cross (vec.h:232-237)
+inline V3 cross(V3 a, V3 b) {
+  return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+// clamp(t) = min(max(t,0),1) with the device's NaN-ignoring fminf/fmaxf
+inline float clamp01(float t) { return std::fmin(std::fmax(t, 0.f), 1.f); }
+
+// Edge::closestPoint (findClosest.h:74-88): distance and parameter t
+inline float edge_closest(V3 a, V3 b, V3 p, float &t, V3 &cp) {
+  V3 ab = sub(b, a);
+  t = dot(sub(p, a), ab) / dot(ab, ab);
+  t = clamp01(t);
+  cp = add(a, mul(t, ab));
+  V3 d = sub(cp, p);
+  return dot(d, d);
+}
+
+void tri_verts(const int32_t *tris, const float *verts, int64_t i, const float *&a,
+               const float *&b, const float *&c) {
+  if (tris) {
+    a = verts + 3 * (int64_t)tris[3 * i + 0];
+    b = verts + 3 * (int64_t)tris[3 * i + 1];
+    c = verts + 3 * (int64_t)tris[3 * i + 2];
+  } else {
+    a = verts + 9 * i;
+    b = verts + 9 * i + 3;
+    c = verts + 9 * i + 6;
+  }
+}
+
+// exact (FP64) closest point on a triangle, textbook region method
+// (Ericson, Real-Time Collision Detection, 5.1.5); returns squared distance
+double tri_closest_f64(const float *af, const float *bf, const float *cf, const float *qf,
+                       double P[3], double w[3]) {
+  double a[3], b[3], c[3], p[3];
+  for (int i = 0; i < 3; i++) { a[i] = af[i]; b[i] = bf[i]; c[i] = cf[i]; p[i] = qf[i]; }
+  auto d3 = [](const double *x, const double *y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+  double ab[3], ac[3], ap[3], bp[3], cp[3];
+  for (int i = 0; i < 3; i++) {
+    ab[i] = b[i] - a[i]; ac[i] = c[i] - a[i]; ap[i] = p[i] - a[i];
+    bp[i] = p[i] - b[i]; cp[i] = p[i] - c[i];
+  }
+  double d1 = d3(ab, ap), d2 = d3(ac, ap), d3v = d3(ab, bp), d4 = d3(ac, bp);
+  double d5 = d3(ab, cp), d6 = d3(ac, cp);
+  double u = 1, v = 0, ww = 0;  // weights of a, b, c
+  if (d1 <= 0 && d2 <= 0) { u = 1; v = 0; ww = 0; }
+  else if (d3v >= 0 && d4 <= d3v) { u = 0; v = 1; ww = 0; }
+  else {
+    double vc = d1 * d4 - d3v * d2;
+    if (vc <= 0 && d1 >= 0 && d3v <= 0) { double t = d1 / (d1 - d3v); u = 1 - t; v = t; ww = 0; }
+    else if (d6 >= 0 && d5 <= d6) { u = 0; v = 0; ww = 1; }
+    else {
+      double vb = d5 * d2 - d1 * d6;
+      if (vb <= 0 && d2 >= 0 && d6 <= 0) { double t = d2 / (d2 - d6); u = 1 - t; v = 0; ww = t; }
+      else {
+        double va = d3v * d6 - d5 * d4;
+        if (va <= 0 && (d4 - d3v) >= 0 && (d5 - d6) >= 0) {
+          double t = (d4 - d3v) / ((d4 - d3v) + (d5 - d6)); u = 0; v = 1 - t; ww = t;
+        } else {
+          double den = 1.0 / (va + vb + vc);
+          v = vb * den; ww = vc * den; u = 1 - v - ww;
+        }
+      }
+    }
+  }
+  double s = 0;
+  for (int i = 0; i < 3; i++) {
+    P[i] = u * a[i] + v * b[i] + ww * c[i];
+    double d = p[i] - P[i];
+    s += d * d;
+  }
+  w[0] = u; w[1] = v; w[2] = ww;
+  return s;
+}
+}  // namespace
+
+float orc_tri_closest(const float *af, const float *bf, const float *cf, const float *qf, float *P,
+                      float *w) {
+  V3 a = v3(af), b = v3(bf), c = v3(cf), q = v3(qf);
+  V3 N = cross(sub(b, a), sub(c, a));
+  bool outside = (N.x == 0.f && N.y == 0.f && N.z == 0.f) ||
+                 (dot(sub(q, a), cross(sub(b, a), N)) >= 0.f) ||
+                 (dot(sub(q, b), cross(sub(c, b), N)) >= 0.f) ||
+                 (dot(sub(q, c), cross(sub(a, c), N)) >= 0.f);
+  if (outside) {
+    // three edge tests in order, strictly-closer acceptance (findClosest.h:109-112)
+    float best = INFINITY, t;
+    V3 cp, bestP{0, 0, 0};
+    float bw[3] = {0, 0, 0};
+    float d = edge_closest(a, b, q, t, cp);
+    if (d < best) { best = d; bestP = cp; bw[0] = 1.f - t; bw[1] = t; bw[2] = 0.f; }
+    d = edge_closest(b, c, q, t, cp);
+    if (d < best) { best = d; bestP = cp; bw[0] = 0.f; bw[1] = 1.f - t; bw[2] = t; }
+    d = edge_closest(c, a, q, t, cp);
+    if (d < best) { best = d; bestP = cp; bw[0] = t; bw[1] = 0.f; bw[2] = 1.f - t; }
+    if (P) { P[0] = bestP.x; P[1] = bestP.y; P[2] = bestP.z; }
+    if (w) { w[0] = bw[0]; w[1] = bw[1]; w[2] = bw[2]; }
+    return best;
+  }
+  // normalize(N) = N * (1/sqrt(dot(N,N))) (vec.h:580-582)
+  float nn = dot(N, N);
+  V3 Nn = mul(1.f / std::sqrt(nn), N);
+  float sd = dot(sub(q, a), Nn);
+  V3 Pp = sub(q, mul(sd, Nn));
+  if (P) { P[0] = Pp.x; P[1] = Pp.y; P[2] = Pp.z; }
+  if (w) {
+    float wa = dot(cross(sub(c, b), sub(Pp, b)), N) / nn;
+    float wb = dot(cross(sub(a, c), sub(Pp, c)), N) / nn;
+    w[0] = wa; w[1] = wb; w[2] = 1.f - wa - wb;
+  }
+  return sd * sd;
+}
+
+void orc_tri_boxes(const int32_t *tris, const float *verts, int64_t numTris, orc_box3f *boxes) {
+  for (int64_t i = 0; i < numTris; i++) {
+    const float *a, *b, *c;
+    tri_verts(tris, verts, i, a, b, c);
+    for (int d = 0; d < 3; d++) {
+      boxes[i].lo[d] = std::min(std::min(a[d], b[d]), c[d]);
+      boxes[i].hi[d] = std::max(std::max(a[d], b[d]), c[d]);
+    }
+  }
+}
+
+void orc_fcp_tris(const orc_node3f *nodes, const uint32_t *primIDs, const int32_t *tris,
+                  const float *verts, const float *queries, int64_t nq, float sqrMax,
+                  int32_t *ids, float *sqrDist, float *Pout) {
+  for (int64_t qi = 0; qi < nq; qi++) {
+    const float *q = queries + 3 * qi;
+    int32_t best = -1;
+    float bestD = sqrMax;
+    float bestP[3] = {0, 0, 0};
+    auto leaf = [&](const uint32_t *prims, uint32_t cnt) -> float {
+      for (uint32_t i = 0; i < cnt; i++) {
+        int32_t pid = (int32_t)prims[i];
+        const float *a, *b, *c;
+        tri_verts(tris, verts, pid, a, b, c);
+        float P[3];
+        float d = orc_tri_closest(a, b, c, q, P, nullptr);
+        if (d < bestD || (d == bestD && best >= 0 && pid < best)) {
+          bestD = d;
+          best = pid;
+          bestP[0] = P[0]; bestP[1] = P[1]; bestP[2] = P[2];
+        }
+      }
+      return bestD;
+    };
+    for_each_leaf(nodes, primIDs, q, sqrMax, true, leaf, nullptr);
+    ids[qi] = best;
+    sqrDist[qi] = best >= 0 ? bestD : INFINITY;
+    if (Pout) for (int d = 0; d < 3; d++) Pout[3 * qi + d] = bestP[d];
+  }
+}
+
+void orc_fcp_tris_brute_f64(const int32_t *tris, const float *verts, int64_t numTris,
+                            const float *queries, int64_t nq, int32_t *ids, double *sqrDist) {
+  for (int64_t qi = 0; qi < nq; qi++) {
+    double best = INFINITY;
+    int32_t bi = -1;
+    for (int64_t i = 0; i < numTris; i++) {
+      const float *a, *b, *c;
+      tri_verts(tris, verts, i, a, b, c);
+      double P[3], w[3];
+      double d = tri_closest_f64(a, b, c, queries + 3 * qi, P, w);
+      if (d < best) { best = d; bi = (int32_t)i; }
+    }
+    ids[qi] = bi;
+    sqrDist[qi] = best;
+  }
+}
+
+void orc_tri_grad_f64(const int32_t *tris, const float *verts, const float *queries, int64_t nq,
+                      const int32_t *ids, const double *g, double *gradQ, double *gradV) {
+  for (int64_t qi = 0; qi < nq; qi++) {
+    for (int d = 0; d < 3; d++) gradQ[3 * qi + d] = 0.0;
+    const int32_t t = ids[qi];
+    if (t < 0) continue;
+    const float *a, *b, *c;
+    tri_verts(tris, verts, t, a, b, c);
+    double P[3], w[3];
+    double s = tri_closest_f64(a, b, c, queries + 3 * qi, P, w);
+    double dist = std::sqrt(s);
+    if (!(dist > 0)) continue;
+    double dir[3];
+    for (int d = 0; d < 3; d++) dir[d] = ((double)queries[3 * qi + d] - P[d]) / dist;
+    for (int d = 0; d < 3; d++) gradQ[3 * qi + d] = g[qi] * dir[d];
+    int64_t vi[3];
+    for (int j = 0; j < 3; j++) vi[j] = tris ? (int64_t)tris[3 * t + j] : 3 * (int64_t)t + j;
+    for (int j = 0; j < 3; j++)
+      for (int d = 0; d < 3; d++) gradV[3 * vi[j] + d] -= g[qi] * w[j] * dir[d];
+  }
+}
+
+void orc_point_grad_f64(const float *points, const float *queries, int64_t nq, int k,
+                        const int32_t *ids, const double *g, double *gradQ, double *gradP) {
+  for (int64_t qi = 0; qi < nq; qi++) {
+    for (int d = 0; d < 3; d++) gradQ[3 * qi + d] = 0.0;
+    for (int j = 0; j < k; j++) {
+      const int32_t p = ids[qi * k + j];
+      if (p < 0) continue;
+      double diff[3], s = 0;
+      for (int d = 0; d < 3; d++) {
+        diff[d] = (double)queries[3 * qi + d] - (double)points[3 * (int64_t)p + d];
+        s += diff[d] * diff[d];
+      }
+      double dist = std::sqrt(s);
+      if (!(dist > 0)) continue;
+      for (int d = 0; d < 3; d++) {
+        double gg = g[qi * k + j] * diff[d] / dist;
+        gradQ[3 * qi + d] += gg;
+        gradP[3 * (int64_t)p + d] -= gg;
+      }
+    }
+  }
+}
+
 int32_t orc_atomicbox_encode(float f) {
   int32_t bits;
   std::memcpy(&bits, &f, 4);

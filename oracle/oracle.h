@@ -95,6 +95,49 @@ void orc_sort_pairs(uint64_t *keys, uint32_t *ids, int n);
 int orc_radix_hierarchy(const uint64_t *sortedKeys, int n, int leafThreshold,
                         orc_node3f *nodes);
 
+/* ---------------------------------------------------------- triangles
+   Triangle i is (verts[tris[3i]], verts[tris[3i+1]], verts[tris[3i+2]]);
+   with tris == NULL it is verts[3i..3i+2] (the reference's non-indexed
+   Triangle array).  */
+
+/* Closest point on one triangle, float32, contraction off
+   (queries/triangles/findClosest.h:74-122): returns the squared distance,
+   writes the closest point P and the barycentric weights of P on (a,b,c)
+   (face: projection barycentrics; edge: (1-t, t); vertex: one-hot). */
+float orc_tri_closest(const float *a, const float *b, const float *c,
+                      const float *q, float *P, float *w);
+
+/* Triangle boxes = Triangle::bounds() (queries/triangles/Triangle.h:35-36). */
+void orc_tri_boxes(const int32_t *tris, const float *verts, int64_t numTris,
+                   orc_box3f *boxes);
+
+/* Closest triangle over a BVH (queries/triangles/findClosest.h:132-152)
+   with the lowest-id tie rule; P may be NULL. */
+void orc_fcp_tris(const orc_node3f *nodes, const uint32_t *primIDs,
+                  const int32_t *tris, const float *verts, const float *queries,
+                  int64_t nq, float sqrMax, int32_t *ids, float *sqrDist,
+                  float *P);
+
+/* FP64 brute force: exact closest distance (textbook region method) over
+   all triangles; id = lowest triangle with the minimal FP64 distance. */
+void orc_fcp_tris_brute_f64(const int32_t *tris, const float *verts,
+                            int64_t numTris, const float *queries, int64_t nq,
+                            int32_t *ids, double *sqrDist);
+
+/* FP64 gradient of sum_i g_i * dist(q_i, tri_{id_i}) w.r.t. queries and
+   vertices (envelope theorem: d/dq = (q-P)/d, d/dv_j = -w_j (q-P)/d, with
+   P and w from the FP64 closest point); zero at d == 0.  gradQ [nq*3] is
+   overwritten, gradV [nv*3] accumulated. */
+void orc_tri_grad_f64(const int32_t *tris, const float *verts,
+                      const float *queries, int64_t nq, const int32_t *ids,
+                      const double *g, double *gradQ, double *gradV);
+
+/* FP64 gradient for points: d/dq = (q-p)/d, d/dp = -(q-p)/d (k entries per
+   query, ids < 0 skipped); gradQ overwritten, gradP accumulated. */
+void orc_point_grad_f64(const float *points, const float *queries, int64_t nq,
+                        int k, const int32_t *ids, const double *g,
+                        double *gradQ, double *gradP);
+
 /* AtomicBox order-preserving encode/decode; builder_common.h:102-117. */
 int32_t orc_atomicbox_encode(float f);
 float orc_atomicbox_decode(int32_t i);

@@ -66,6 +66,19 @@ _MALLOC_CB = C.CFUNCTYPE(C.c_int, C.POINTER(C.c_void_p), C.c_size_t, C.c_void_p,
 _FREE_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p)
 
 
+class BatchParams(C.Structure):
+    """cubql_batch_params: leafSize, blockSize (64/128/256), stackDepth (32/64), maxCloudSize."""
+    _fields_ = [("leafSize", C.c_int), ("blockSize", C.c_int), ("stackDepth", C.c_int),
+                ("maxCloudSize", C.c_uint32)]
+
+    def __init__(self, leafSize: int = 8, blockSize: int = 256, stackDepth: int = 64,
+                 maxCloudSize: int = 0):
+        super().__init__(leafSize, blockSize, stackDepth, maxCloudSize)
+
+    def as_tuple(self):
+        return (self.leafSize, self.blockSize, self.stackDepth, self.maxCloudSize)
+
+
 class MemResource(C.Structure):
     _fields_ = [("malloc_fn", _MALLOC_CB), ("free_fn", _FREE_CB), ("user", C.c_void_p)]
 
@@ -100,6 +113,13 @@ def lib():
         "cubql_fcp_triangles_f3": (I, [P(BVH3f), V, V, V, U32, F, V, V, V, V]),
         "cubql_fcp_triangles_backward_f3": (I, [V, V, V, U32, V, V, V, V, V]),
         "cubql_points_distance_backward_f3": (I, [V, V, U32, I, V, V, V, V, V]),
+        "cubql_batch_chamfer_workspace_f3": (C.c_size_t, [U32, U32, U32]),
+        "cubql_batch_chamfer_f3": (I, [V, V, U32, V, V, U32, U32, P(BatchParams), V, C.c_size_t,
+                                       V, V, V, V, V, V, V, V]),
+        "cubql_batch_chamfer_graph_f3": (I, [V, V, U32, V, V, U32, U32, P(BatchParams), V,
+                                             C.c_size_t, V, V, V, V, V, V, V, P(V)]),
+        "cubql_graph_launch": (I, [V, V]),
+        "cubql_graph_destroy": (I, [V]),
         "cubql_profile_enable": (None, [I]),
         "cubql_profile_reset": (None, []),
         "cubql_profile_query": (I, [C.c_char_p, P(C.c_uint64), P(C.c_double), P(C.c_double),
@@ -362,6 +382,63 @@ def _as_tris(t: Optional[torch.Tensor]):
     if t.dtype != torch.int32 or not t.is_cuda:
         raise CubqlError("tris must be an int32 CUDA tensor of shape [n,3]")
     return t.contiguous()
+
+
+class ChamferBatch:
+    """Batched bidirectional closest point + Chamfer adjoint over many cloud
+    pairs (cubql_batch_chamfer_f3).  Buffers are allocated once; run() is
+    eager, capture() records the launch sequence into a HIP graph and
+    replay() launches it."""
+
+    def __init__(self, src: torch.Tensor, src_off: torch.Tensor, tgt: torch.Tensor,
+                 tgt_off: torch.Tensor, params: Optional[BatchParams] = None, want_grad=True):
+        self.src = _as_points(src, "src")
+        self.tgt = _as_points(tgt, "tgt")
+        self.src_off = src_off.to(torch.int32).contiguous()
+        self.tgt_off = tgt_off.to(torch.int32).contiguous()
+        self.num_clouds = self.src_off.numel() - 1
+        self.params = params or BatchParams()
+        dev = self.src.device
+        ns, nt = self.src.shape[0], self.tgt.shape[0]
+        wsb = lib().cubql_batch_chamfer_workspace_f3(self.num_clouds, ns, nt)
+        self.ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+        self.src_nn = torch.empty(ns, dtype=torch.int32, device=dev)
+        self.tgt_nn = torch.empty(nt, dtype=torch.int32, device=dev)
+        self.src_d2 = torch.empty(ns, dtype=torch.float32, device=dev)
+        self.tgt_d2 = torch.empty(nt, dtype=torch.float32, device=dev)
+        self.loss = torch.empty(self.num_clouds, dtype=torch.float32, device=dev)
+        self.grad_src = torch.empty_like(self.src) if want_grad else None
+        self.grad_tgt = torch.empty_like(self.tgt) if want_grad else None
+        self.graph = None
+
+    def _args(self):
+        return (_ptr(self.src), _ptr(self.src_off), self.src.shape[0], _ptr(self.tgt),
+                _ptr(self.tgt_off), self.tgt.shape[0], self.num_clouds, C.byref(self.params),
+                _ptr(self.ws), self.ws.numel(), _ptr(self.src_nn), _ptr(self.src_d2),
+                _ptr(self.tgt_nn), _ptr(self.tgt_d2), _ptr(self.loss), _ptr(self.grad_src),
+                _ptr(self.grad_tgt))
+
+    def run(self, stream=None):
+        _check(lib().cubql_batch_chamfer_f3(*self._args(), C.c_void_p(_stream_handle(stream))),
+               "batch_chamfer")
+        return self
+
+    def capture(self):
+        g = C.c_void_p()
+        _check(lib().cubql_batch_chamfer_graph_f3(*self._args(), C.byref(g)), "batch_chamfer_graph")
+        self.graph = g
+        return self
+
+    def replay(self, stream=None):
+        _check(lib().cubql_graph_launch(self.graph, C.c_void_p(_stream_handle(stream))), "graph_launch")
+        return self
+
+    def __del__(self):
+        try:
+            if self.graph:
+                lib().cubql_graph_destroy(self.graph)
+        except Exception:  # noqa: BLE001
+            pass
 
 
 class profile:  # noqa: N801

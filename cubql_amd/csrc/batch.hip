@@ -1,0 +1,657 @@
+// batch.hip -- batched build + bidirectional closest point + Chamfer adjoint
+// over thousands of independent point-cloud pairs (no reference counterpart:
+// the reference builds one BVH per call and syncs with the host every level).
+//
+// Build, for each side (source / target) at once:
+//   * one workgroup per cloud computes the cloud's centroid bounds and its
+//     30-bit Morton keys with that cloud's own quantizer (radix.h:41-70); the
+//     sort key is (cloud << 30) | morton, the value the global point index;
+//   * one stable onesweep radix sort over all keys (segmented Morton sort);
+//   * one level-synchronous top-down hierarchy over the sorted keys where a
+//     range that spans several clouds is always split (its highest differing
+//     key bit is a cloud bit, so the split lands on a cloud boundary) and a
+//     single-cloud range follows radix.h:446-570 (leaf if <= T, equal keys
+//     split in the middle).  The node of each single-cloud range is that
+//     cloud's root, and its subtree is exactly the tree radix30Builder builds
+//     for the cloud alone;
+//   * exact bottom-up refit.
+// Queries run one per lane over all clouds, in Morton order of the query
+// side, each from its cloud's root in the other side's forest; the adjoint of
+// loss[c] = mean_i d(s_i, T_c) + mean_j d(t_j, S_c) is scatter-added.  The
+// whole sequence is a fixed number of launches on one stream with no host
+// synchronisation and no allocation (caller workspace), so it can be
+// captured in a HIP graph and replayed.
+#include "block_ops.h"
+#include "build_common.h"
+#include "common.h"
+#include "cubql_amd/traversal.hip.h"
+#include "profile.h"
+#include "radix_sort.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace cubql {
+namespace bt {
+
+constexpr int kCloudShift = 30;
+constexpr uint32_t kNoRoot = 0xffffffffu;
+
+struct Side {
+  const float *pts;
+  const uint32_t *off;  // numClouds + 1 offsets (device)
+  uint32_t n;
+  // persistent (kept for the queries)
+  uint64_t *keys;       // sorted keys
+  uint32_t *ids;        // sorted global point indices = primIDs of the forest
+  Node *nodes;          // capacity 2n + 2
+  uint32_t *rootOf;     // numClouds
+  uint32_t *ctr;        // [0] node count, [1] n (device), [8 + level] open count
+};
+
+struct Temp {
+  uint64_t *keysB;
+  uint32_t *valsB;
+  uint2 *ranges;
+  uint32_t *open[2];
+  uint32_t *parent;
+  uint32_t *arrive;
+  void *sortWs;
+  uint32_t *ovf;        // [0] count, [1] deep failures, then the list
+};
+
+enum { C_NODES = 0, C_N = 1, C_LEVEL0 = 8 };
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+  uint32_t x = v & 0x3ffu;
+  x = (x | (x << 16)) & 0x030000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+// one workgroup per cloud: centroid bounds, quantizer, keys
+__global__ __launch_bounds__(256) void k_bkeys(Side sd, uint32_t numClouds) {
+  __shared__ float sb[12 * 4];
+  const uint32_t c = blockIdx.x;
+  const uint32_t b = sd.off[c], e = sd.off[c + 1];
+  float v[12];
+  for (int j = 0; j < 3; j++) { v[j] = INFINITY; v[j + 3] = -INFINITY; v[j + 6] = INFINITY; v[j + 9] = -INFINITY; }
+  for (uint32_t i = b + threadIdx.x; i < e; i += 256) {
+    for (int d = 0; d < 3; d++) {
+      const float p = sd.pts[3 * (size_t)i + d];
+      const float ce = 0.5f * (p + p);  // centre of the degenerate box (p, p)
+      v[d] = fminf(v[d], ce);
+      v[d + 3] = fmaxf(v[d + 3], ce);
+    }
+  }
+  block_reduce_bounds<256>(v, sb);
+  if (b == e) return;
+  // Quantizer::init (radix.h:46-53), 10 bits per axis
+  const float ext = fmaxf(fmaxf(v[3] - v[0], v[4] - v[1]), v[5] - v[2]);
+  const float scale = 1024.f * (1.f / fmaxf(ext, 1e-20f));
+  for (uint32_t i = b + threadIdx.x; i < e; i += 256) {
+    uint32_t cell[3];
+    for (int d = 0; d < 3; d++) {
+      const float p = sd.pts[3 * (size_t)i + d];
+      const float f = (0.5f * (p + p) - v[d]) * scale;
+      cell[d] = min((uint32_t)f, 1023u);
+    }
+    const uint32_t m = spread10(cell[0]) | (spread10(cell[1]) << 1) | (spread10(cell[2]) << 2);
+    sd.keys[i] = ((uint64_t)c << kCloudShift) | m;
+    sd.ids[i] = i;
+  }
+}
+
+__global__ void k_binit(Side sd, Temp tp, uint32_t numClouds) {
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < numClouds; c += gridDim.x * blockDim.x)
+    sd.rootOf[c] = kNoRoot;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sd.ctr[C_NODES] = 2;
+    Node z;
+    for (int d = 0; d < 3; d++) { z.lo[d] = 0.f; z.hi[d] = 0.f; }
+    z.oc = 0;
+    sd.nodes[1] = z;
+    if (sd.n > 0) {
+      tp.ranges[0] = make_uint2(0u, sd.n);
+      tp.open[0][0] = 0;
+      sd.ctr[C_LEVEL0] = 1;
+    }
+  }
+}
+
+// the root of a forest that holds a single cloud is node 0
+__global__ void k_bsingle_root(Side sd) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && sd.n > 0) {
+    const uint64_t c0 = sd.keys[0] >> kCloudShift, c1 = sd.keys[sd.n - 1] >> kCloudShift;
+    if (c0 == c1) sd.rootOf[c0] = 0;
+  }
+}
+
+__device__ __forceinline__ uint32_t find_split64(const uint64_t *keys, uint32_t b, uint32_t e) {
+  const uint64_t fk = keys[b], lk = keys[e - 1];
+  if (fk == lk) return (uint32_t)(((uint64_t)b + e) / 2);
+  const int nm = __clzll((unsigned long long)(fk ^ lk));
+  const uint64_t search = lk & (~0ull << (63 - nm));
+  uint32_t lo = b, hi = e;
+  while (hi > lo) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (keys[mid] < search) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_blevel(Side sd, Temp tp, uint32_t T, int level) {
+  __shared__ uint32_t ss[256 / kWave + 1];
+  __shared__ uint32_t sNode, sOpen;
+  const uint32_t nOpen = sd.ctr[C_LEVEL0 + level];
+  const uint32_t *open = tp.open[level & 1];
+  uint32_t *next = tp.open[(level & 1) ^ 1];
+  for (uint32_t b0 = blockIdx.x * 256; b0 < nOpen; b0 += gridDim.x * 256) {
+    const uint32_t i = b0 + threadIdx.x;
+    uint32_t node = 0, split = 0;
+    uint2 r = make_uint2(0, 0);
+    bool doSplit = false, multi = false;
+    if (i < nOpen) {
+      node = open[i];
+      r = tp.ranges[node];
+      const uint32_t size = r.y - r.x;
+      multi = (sd.keys[r.x] >> kCloudShift) != (sd.keys[r.y - 1] >> kCloudShift);
+      if (multi || size > T) {
+        split = find_split64(sd.keys, r.x, r.y);
+        doSplit = true;
+      } else {
+        sd.nodes[node].oc = pack_oc(r.x, size);
+      }
+    }
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<256>(doSplit ? 1u : 0u, ss, &tot);
+    if (threadIdx.x == 0 && tot) {
+      sNode = atomicAdd(&sd.ctr[C_NODES], 2 * tot);
+      sOpen = atomicAdd(&sd.ctr[C_LEVEL0 + level + 1], 2 * tot);
+    }
+    __syncthreads();
+    if (doSplit) {
+      const uint32_t child = sNode + 2 * ex;
+      sd.nodes[node].oc = pack_oc(child, 0);
+      tp.ranges[child] = make_uint2(r.x, split);
+      tp.ranges[child + 1] = make_uint2(split, r.y);
+      next[sOpen + 2 * ex] = child;
+      next[sOpen + 2 * ex + 1] = child + 1;
+      if (multi) {
+        const uint64_t cl = sd.keys[r.x] >> kCloudShift, cm = sd.keys[split - 1] >> kCloudShift;
+        if (cl == cm) sd.rootOf[cl] = child;
+        const uint64_t cr = sd.keys[split] >> kCloudShift, ce = sd.keys[r.y - 1] >> kCloudShift;
+        if (cr == ce) sd.rootOf[cr] = child + 1;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_bparents(Side sd, Temp tp) {
+  const uint32_t numNodes = sd.ctr[C_NODES];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < numNodes; i += gridDim.x * blockDim.x) {
+    tp.arrive[i] = 0;
+    if (i == 0) tp.parent[0] = 0xffffffffu;
+    if (i == 1) continue;
+    const uint64_t oc = sd.nodes[i].oc;
+    if (oc_count(oc) != 0) continue;
+    const uint32_t o = (uint32_t)oc_offset(oc);
+    tp.parent[o] = i;
+    tp.parent[o + 1] = i;
+  }
+}
+
+// leaves start, the second child to arrive finishes the parent
+__global__ void k_brefit(Side sd, Temp tp) {
+  const uint32_t numNodes = sd.ctr[C_NODES];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < numNodes; i += gridDim.x * blockDim.x) {
+    if (i == 1) continue;
+    const uint64_t oc = sd.nodes[i].oc;
+    const uint32_t cnt = oc_count(oc);
+    if (cnt == 0) continue;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    const uint32_t o = (uint32_t)oc_offset(oc);
+    for (uint32_t j = 0; j < cnt; j++) {
+      const size_t p = sd.ids[o + j];
+      for (int d = 0; d < 3; d++) {
+        const float x = sd.pts[3 * p + d];
+        lo[d] = fminf(lo[d], x);
+        hi[d] = fmaxf(hi[d], x);
+      }
+    }
+    uint32_t node = i;
+    while (true) {
+      Node *nd = &sd.nodes[node];
+      for (int d = 0; d < 3; d++) { nd->lo[d] = lo[d]; nd->hi[d] = hi[d]; }
+      const uint32_t par = tp.parent[node];
+      if (par == 0xffffffffu) break;
+      __threadfence();
+      if (atomicAdd(&tp.arrive[par], 1u) == 0) break;
+      __threadfence();
+      const uint32_t o2 = (uint32_t)oc_offset(sd.nodes[par].oc);
+      const volatile Node *l = &sd.nodes[o2];
+      const volatile Node *r = &sd.nodes[o2 + 1];
+      for (int d = 0; d < 3; d++) {
+        lo[d] = fminf(l->lo[d], r->lo[d]);
+        hi[d] = fmaxf(l->hi[d], r->hi[d]);
+      }
+      node = par;
+    }
+  }
+}
+
+template <int STACK>
+__device__ __forceinline__ bool fcp_forest(const Side &tgt, uint32_t root, float3 q, int32_t &best,
+                                           float &bestD) {
+  best = -1;
+  bestD = INFINITY;
+  cubql_bvh3f bvh;
+  bvh.nodes = (cubql_node3f *)tgt.nodes;
+  bvh.numNodes = 2;
+  bvh.primIDs = tgt.ids;
+  bvh.numPrims = tgt.n;
+  const float *pts = tgt.pts;
+  auto prim = [&](uint32_t pid) -> float {
+    const float px = pts[3 * (size_t)pid], py = pts[3 * (size_t)pid + 1], pz = pts[3 * (size_t)pid + 2];
+    const float d = cubql_dev::sqr_dist_pt(px, py, pz, q.x, q.y, q.z);
+    if (d < bestD || (d == bestD && best >= 0 && (int32_t)pid < best)) {
+      bestD = d;
+      best = (int32_t)pid;
+    }
+    return bestD;
+  };
+  return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, INFINITY, prim, root);
+}
+
+// queries of side `qs` against the forest of side `ts`, in Morton order of qs
+template <int BLOCK, int STACK>
+__global__ __launch_bounds__(BLOCK) void k_bquery(Side qs, Side ts, int32_t *nn, float *d2, uint32_t *ovf) {
+  const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= qs.n) return;
+  const uint32_t qi = qs.ids[t];
+  const uint32_t cloud = (uint32_t)(qs.keys[t] >> kCloudShift);
+  const uint32_t root = ts.rootOf[cloud];
+  int32_t best = -1;
+  float bestD = INFINITY;
+  if (root != kNoRoot) {
+    const float3 q = make_float3(qs.pts[3 * (size_t)qi], qs.pts[3 * (size_t)qi + 1], qs.pts[3 * (size_t)qi + 2]);
+    if (!fcp_forest<STACK>(ts, root, q, best, bestD)) {
+      ovf[2 + atomicAdd(&ovf[0], 1u)] = t;
+      return;
+    }
+  }
+  nn[qi] = best;
+  d2[qi] = bestD;
+}
+
+__global__ __launch_bounds__(64) void k_bquery_deep(Side qs, Side ts, int32_t *nn, float *d2, uint32_t *ovf) {
+  const uint32_t n = ovf[0];
+  for (uint32_t i = blockIdx.x * 64 + threadIdx.x; i < n; i += gridDim.x * 64) {
+    const uint32_t t = ovf[2 + i];
+    const uint32_t qi = qs.ids[t];
+    const uint32_t cloud = (uint32_t)(qs.keys[t] >> kCloudShift);
+    const float3 q = make_float3(qs.pts[3 * (size_t)qi], qs.pts[3 * (size_t)qi + 1], qs.pts[3 * (size_t)qi + 2]);
+    int32_t best;
+    float bestD;
+    if (!fcp_forest<1024>(ts, ts.rootOf[cloud], q, best, bestD)) {
+      atomicAdd(&ovf[1], 1u);
+      best = -1;
+      bestD = INFINITY;
+    }
+    nn[qi] = best;
+    d2[qi] = bestD;
+  }
+}
+
+// adjoint of loss[c] = mean over the query cloud of d(q, nearest): the
+// query's own term and its neighbour's term are both scatter-added
+__global__ __launch_bounds__(256) void k_bback(Side qs, Side ts, const int32_t *nn, const float *d2,
+                                               float *loss, float *gradQ, float *gradT) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  const bool act = t < qs.n;
+  uint32_t qi = 0, cloud = 0;
+  float contrib = 0.f;
+  if (act) {
+    qi = qs.ids[t];
+    cloud = (uint32_t)(qs.keys[t] >> kCloudShift);
+    const int32_t p = nn[qi];
+    if (p >= 0) {
+      const float d = sqrtf(d2[qi]);
+      const float g = 1.f / (float)(qs.off[cloud + 1] - qs.off[cloud]);
+      contrib = g * d;
+      if (d > 0.f) {
+        const float s = g / d;
+        for (int k = 0; k < 3; k++) {
+          const float diff = qs.pts[3 * (size_t)qi + k] - ts.pts[3 * (size_t)p + k];
+          if (gradQ) atomicAdd(&gradQ[3 * (size_t)qi + k], s * diff);
+          if (gradT) atomicAdd(&gradT[3 * (size_t)p + k], -s * diff);
+        }
+      }
+    }
+  }
+  if (loss) {
+    // one atomic per wave when all active lanes share the cloud
+    const uint32_t c0 = __shfl(cloud, 0, kWave);
+    const bool uniform = __all(!act || cloud == c0);
+    if (uniform) {
+      float sum = contrib;
+      for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, kWave);
+      if (lane_id() == 0 && __any(act)) atomicAdd(&loss[c0], sum);
+    } else if (act && contrib != 0.f) {
+      atomicAdd(&loss[cloud], contrib);
+    }
+  }
+}
+
+__global__ void k_fill_f32(float *p, float v, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// ----------------------------------------------------------- workspace
+struct Layout {
+  size_t bytes = 0;
+  size_t take(size_t b) {
+    size_t o = bytes;
+    bytes += (b + 255) & ~size_t(255);
+    return o;
+  }
+};
+
+struct Plan {
+  size_t keys[2], ids[2], nodes[2], roots[2], ctr[2];
+  size_t keysB, valsB, ranges, open0, open1, parent, arrive, sortWs, ovf;
+  size_t total;
+  uint32_t maxLevels;
+};
+
+Plan make_plan(uint32_t numClouds, uint32_t nSrc, uint32_t nTgt) {
+  Layout L;
+  Plan p{};
+  const uint32_t nMax = std::max(std::max(nSrc, nTgt), 1u);
+  const uint32_t ns[2] = {nSrc, nTgt};
+  for (int s = 0; s < 2; s++) {
+    const size_t n = std::max(ns[s], 1u);
+    p.keys[s] = L.take(8 * n);
+    p.ids[s] = L.take(4 * n);
+    p.nodes[s] = L.take(32 * (2 * n + 2));
+    p.roots[s] = L.take(4 * (size_t)std::max(numClouds, 1u));
+    p.ctr[s] = L.take(4 * 256);
+  }
+  p.keysB = L.take(8 * (size_t)nMax);
+  p.valsB = L.take(4 * (size_t)nMax);
+  p.ranges = L.take(8 * (2 * (size_t)nMax + 2));
+  p.open0 = L.take(4 * ((size_t)nMax + 2));
+  p.open1 = L.take(4 * ((size_t)nMax + 2));
+  p.parent = L.take(4 * (2 * (size_t)nMax + 2));
+  p.arrive = L.take(4 * (2 * (size_t)nMax + 2));
+  int cloudBits = 0;
+  while ((1u << cloudBits) < std::max(numClouds, 1u)) cloudBits++;
+  p.sortWs = L.take(radix_sort_ws_bytes(nMax, kCloudShift + cloudBits));
+  p.ovf = L.take(4 * ((size_t)nMax + 2));
+  p.total = L.bytes;
+  return p;
+}
+
+int cloud_bits(uint32_t numClouds) {
+  int b = 0;
+  while ((1u << b) < std::max(numClouds, 1u)) b++;
+  return b;
+}
+
+struct Args {
+  const float *src;
+  const uint32_t *srcOff;
+  uint32_t nSrc;
+  const float *tgt;
+  const uint32_t *tgtOff;
+  uint32_t nTgt;
+  uint32_t numClouds;
+  cubql_batch_params prm;
+  char *ws;
+  int32_t *srcNN, *tgtNN;
+  float *srcD2, *tgtD2, *loss, *gradSrc, *gradTgt;
+};
+
+int build_side(Side &sd, Temp &tp, const Args &a, const Plan &pl, hipStream_t s, int numCU) {
+  const int cbits = cloud_bits(a.numClouds);
+  CUBQL_CHECK(fill_u32(sd.ctr, 0u, 256, s));
+  CUBQL_CHECK(fill_u32(&sd.ctr[C_N], sd.n, 1, s));
+  if (a.numClouds) k_bkeys<<<a.numClouds, 256, 0, s>>>(sd, a.numClouds);
+  uint64_t *kr = nullptr;
+  uint32_t *vr = nullptr;
+  CUBQL_CHECK(radix_sort_pairs_ws_u64(sd.keys, sd.ids, tp.keysB, tp.valsB, &sd.ctr[C_N], std::max(sd.n, 1u),
+                                      kCloudShift + cbits, s, tp.sortWs, &kr, &vr));
+  if (kr != sd.keys) {
+    CUBQL_HIP(hipMemcpyAsync(sd.keys, kr, 8 * (size_t)sd.n, hipMemcpyDeviceToDevice, s));
+    CUBQL_HIP(hipMemcpyAsync(sd.ids, vr, 4 * (size_t)sd.n, hipMemcpyDeviceToDevice, s));
+  }
+  k_binit<<<std::max(1u, std::min<uint32_t>(div_up(a.numClouds, 256), 64)), 256, 0, s>>>(sd, tp, a.numClouds);
+  k_bsingle_root<<<1, 64, 0, s>>>(sd);
+  // depth bound: cloud bits + 30 Morton bits + mid splits of equal keys
+  int midBits = 0;
+  const uint32_t maxCloud = a.prm.maxCloudSize ? a.prm.maxCloudSize : std::max(sd.n, 1u);
+  while ((1ull << midBits) < maxCloud) midBits++;
+  const int levels = std::min(cbits + kCloudShift + midBits + 2, 256 - C_LEVEL0 - 2);
+  const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(div_up(sd.n, 256), (uint32_t)numCU * 4));
+  const uint32_t T = (uint32_t)std::max(1, std::min(a.prm.leafSize, 65535));
+  for (int l = 0; l < levels; l++) k_blevel<<<grid, 256, 0, s>>>(sd, tp, T, l);
+  const uint32_t rg = std::max<uint32_t>(1u, std::min<uint32_t>(div_up(2 * (size_t)sd.n, 256), (uint32_t)numCU * 8));
+  k_bparents<<<rg, 256, 0, s>>>(sd, tp);
+  k_brefit<<<rg, 256, 0, s>>>(sd, tp);
+  CUBQL_LAUNCH_CHECK();
+  return CUBQL_OK;
+}
+
+template <int BLOCK, int STACK>
+void launch_query(const Side &q, const Side &t, int32_t *nn, float *d2, uint32_t *ovf, hipStream_t s) {
+  k_bquery<BLOCK, STACK><<<div_up(std::max(q.n, 1u), BLOCK), BLOCK, 0, s>>>(q, t, nn, d2, ovf);
+}
+
+int query_dir(const Side &q, const Side &t, int32_t *nn, float *d2, uint32_t *ovf, const cubql_batch_params &p,
+              hipStream_t s) {
+  if (q.n == 0) return CUBQL_OK;
+  CUBQL_CHECK(fill_u32(ovf, 0u, 2, s));
+  const int b = p.blockSize, st = p.stackDepth;
+  if (b == 64 && st <= 32) launch_query<64, 32>(q, t, nn, d2, ovf, s);
+  else if (b == 64) launch_query<64, 64>(q, t, nn, d2, ovf, s);
+  else if (b == 128 && st <= 32) launch_query<128, 32>(q, t, nn, d2, ovf, s);
+  else if (b == 128) launch_query<128, 64>(q, t, nn, d2, ovf, s);
+  else if (st <= 32) launch_query<256, 32>(q, t, nn, d2, ovf, s);
+  else launch_query<256, 64>(q, t, nn, d2, ovf, s);
+  k_bquery_deep<<<64, 64, 0, s>>>(q, t, nn, d2, ovf);
+  CUBQL_LAUNCH_CHECK();
+  return CUBQL_OK;
+}
+
+int run(const Args &a, hipStream_t s) {
+  const Plan pl = make_plan(a.numClouds, a.nSrc, a.nTgt);
+  int dev = 0, numCU = 256;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&numCU, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    numCU = 256;
+  Side side[2];
+  const float *pts[2] = {a.src, a.tgt};
+  const uint32_t *offs[2] = {a.srcOff, a.tgtOff};
+  const uint32_t ns[2] = {a.nSrc, a.nTgt};
+  for (int i = 0; i < 2; i++) {
+    side[i].pts = pts[i];
+    side[i].off = offs[i];
+    side[i].n = ns[i];
+    side[i].keys = (uint64_t *)(a.ws + pl.keys[i]);
+    side[i].ids = (uint32_t *)(a.ws + pl.ids[i]);
+    side[i].nodes = (Node *)(a.ws + pl.nodes[i]);
+    side[i].rootOf = (uint32_t *)(a.ws + pl.roots[i]);
+    side[i].ctr = (uint32_t *)(a.ws + pl.ctr[i]);
+  }
+  Temp tp;
+  tp.keysB = (uint64_t *)(a.ws + pl.keysB);
+  tp.valsB = (uint32_t *)(a.ws + pl.valsB);
+  tp.ranges = (uint2 *)(a.ws + pl.ranges);
+  tp.open[0] = (uint32_t *)(a.ws + pl.open0);
+  tp.open[1] = (uint32_t *)(a.ws + pl.open1);
+  tp.parent = (uint32_t *)(a.ws + pl.parent);
+  tp.arrive = (uint32_t *)(a.ws + pl.arrive);
+  tp.sortWs = a.ws + pl.sortWs;
+  tp.ovf = (uint32_t *)(a.ws + pl.ovf);
+
+  int ps = prof_begin(s);
+  CUBQL_CHECK(build_side(side[0], tp, a, pl, s, numCU));
+  CUBQL_CHECK(build_side(side[1], tp, a, pl, s, numCU));
+  prof_end(ps, s, "batch_build");
+  prof_set(ps, (12.0 + 12.0 + 8.0 * 2 * 6 + 8.0 + 24.0) * (a.nSrc + a.nTgt), a.nSrc + a.nTgt);
+  const uint32_t fg = 1024;
+  if (a.loss) k_fill_f32<<<std::max(1u, std::min(div_up(a.numClouds, 256), fg)), 256, 0, s>>>(a.loss, 0.f, a.numClouds);
+  if (a.gradSrc) k_fill_f32<<<fg, 256, 0, s>>>(a.gradSrc, 0.f, 3 * (size_t)a.nSrc);
+  if (a.gradTgt) k_fill_f32<<<fg, 256, 0, s>>>(a.gradTgt, 0.f, 3 * (size_t)a.nTgt);
+  ps = prof_begin(s);
+  CUBQL_CHECK(query_dir(side[0], side[1], a.srcNN, a.srcD2, tp.ovf, a.prm, s));
+  CUBQL_CHECK(query_dir(side[1], side[0], a.tgtNN, a.tgtD2, tp.ovf, a.prm, s));
+  prof_end(ps, s, "batch_query");
+  prof_set(ps, 20.0 * (a.nSrc + a.nTgt), a.nSrc + a.nTgt);
+  ps = prof_begin(s);
+  if (a.loss || a.gradSrc || a.gradTgt) {
+    if (a.nSrc) k_bback<<<div_up(a.nSrc, 256), 256, 0, s>>>(side[0], side[1], a.srcNN, a.srcD2, a.loss, a.gradSrc, a.gradTgt);
+    if (a.nTgt) k_bback<<<div_up(a.nTgt, 256), 256, 0, s>>>(side[1], side[0], a.tgtNN, a.tgtD2, a.loss, a.gradTgt, a.gradSrc);
+  }
+  prof_end(ps, s, "batch_backward");
+  prof_set(ps, 68.0 * (a.nSrc + a.nTgt), a.nSrc + a.nTgt);
+  CUBQL_LAUNCH_CHECK();
+  return CUBQL_OK;
+}
+
+int check_args(const Args &a, size_t wsBytes) {
+  if (a.numClouds == 0) return CUBQL_OK;
+  if (!a.src || !a.srcOff || !a.tgt || !a.tgtOff || !a.ws || !a.srcNN || !a.tgtNN || !a.srcD2 || !a.tgtD2) {
+    set_error("cubql_batch_chamfer_f3: null argument");
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  const Plan pl = make_plan(a.numClouds, a.nSrc, a.nTgt);
+  if (wsBytes < pl.total) {
+    set_error("cubql_batch_chamfer_f3: workspace of %zu bytes is below the %zu needed", wsBytes, pl.total);
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  const int b = a.prm.blockSize;
+  if (!(b == 64 || b == 128 || b == 256) || !(a.prm.stackDepth == 32 || a.prm.stackDepth == 64) ||
+      a.prm.leafSize < 1) {
+    set_error("cubql_batch_chamfer_f3: params must have blockSize 64/128/256, stackDepth 32/64, leafSize >= 1");
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  return CUBQL_OK;
+}
+
+}  // namespace bt
+}  // namespace cubql
+
+using namespace cubql;
+
+struct cubql_graph_s {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" size_t cubql_batch_chamfer_workspace_f3(uint32_t numClouds, uint32_t numSrc, uint32_t numTgt) {
+  return bt::make_plan(numClouds, numSrc, numTgt).total;
+}
+
+static bt::Args make_args(const float *src, const uint32_t *srcOff, uint32_t nSrc, const float *tgt,
+                          const uint32_t *tgtOff, uint32_t nTgt, uint32_t numClouds,
+                          const cubql_batch_params *prm, void *ws, int32_t *srcNN, float *srcD2,
+                          int32_t *tgtNN, float *tgtD2, float *loss, float *gradSrc, float *gradTgt) {
+  bt::Args a{};
+  a.src = src; a.srcOff = srcOff; a.nSrc = nSrc;
+  a.tgt = tgt; a.tgtOff = tgtOff; a.nTgt = nTgt;
+  a.numClouds = numClouds;
+  if (prm) a.prm = *prm;
+  else a.prm = cubql_batch_params{8, 256, 64, 0};
+  a.ws = (char *)ws;
+  a.srcNN = srcNN; a.srcD2 = srcD2; a.tgtNN = tgtNN; a.tgtD2 = tgtD2;
+  a.loss = loss; a.gradSrc = gradSrc; a.gradTgt = gradTgt;
+  return a;
+}
+
+extern "C" int cubql_batch_chamfer_f3(const float *d_src, const uint32_t *d_srcOffsets, uint32_t numSrc,
+                                      const float *d_tgt, const uint32_t *d_tgtOffsets, uint32_t numTgt,
+                                      uint32_t numClouds, const cubql_batch_params *params, void *d_workspace,
+                                      size_t workspaceBytes, int32_t *d_srcNN, float *d_srcSqrDist,
+                                      int32_t *d_tgtNN, float *d_tgtSqrDist, float *d_loss,
+                                      float *d_gradSrc, float *d_gradTgt, cubql_stream_t s) {
+  clear_error();
+  bt::Args a = make_args(d_src, d_srcOffsets, numSrc, d_tgt, d_tgtOffsets, numTgt, numClouds, params,
+                         d_workspace, d_srcNN, d_srcSqrDist, d_tgtNN, d_tgtSqrDist, d_loss, d_gradSrc, d_gradTgt);
+  CUBQL_CHECK(bt::check_args(a, workspaceBytes));
+  if (numClouds == 0) return CUBQL_OK;
+  return bt::run(a, (hipStream_t)s);
+}
+
+extern "C" int cubql_batch_chamfer_graph_f3(const float *d_src, const uint32_t *d_srcOffsets, uint32_t numSrc,
+                                            const float *d_tgt, const uint32_t *d_tgtOffsets, uint32_t numTgt,
+                                            uint32_t numClouds, const cubql_batch_params *params,
+                                            void *d_workspace, size_t workspaceBytes, int32_t *d_srcNN,
+                                            float *d_srcSqrDist, int32_t *d_tgtNN, float *d_tgtSqrDist,
+                                            float *d_loss, float *d_gradSrc, float *d_gradTgt,
+                                            cubql_graph *out) {
+  clear_error();
+  if (!out) {
+    set_error("cubql_batch_chamfer_graph_f3: out is NULL");
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  *out = nullptr;
+  bt::Args a = make_args(d_src, d_srcOffsets, numSrc, d_tgt, d_tgtOffsets, numTgt, numClouds, params,
+                         d_workspace, d_srcNN, d_srcSqrDist, d_tgtNN, d_tgtSqrDist, d_loss, d_gradSrc, d_gradTgt);
+  CUBQL_CHECK(bt::check_args(a, workspaceBytes));
+  if (numClouds == 0) {
+    set_error("cubql_batch_chamfer_graph_f3: empty batch");
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  hipStream_t cs;
+  CUBQL_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  cubql_graph g = new cubql_graph_s();
+  const bool profWas = prof_on();
+  if (profWas) cubql_profile_enable(0);
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  int st = e == hipSuccess ? bt::run(a, cs) : CUBQL_ERR_HIP;
+  hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
+  if (profWas) cubql_profile_enable(1);
+  if (st == CUBQL_OK && (e != hipSuccess || e2 != hipSuccess)) {
+    set_error("graph capture failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    st = CUBQL_ERR_HIP;
+  }
+  if (st == CUBQL_OK) {
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      set_error("hipGraphInstantiate failed: %s", hipGetErrorString(e));
+      st = CUBQL_ERR_HIP;
+    }
+  }
+  (void)hipStreamDestroy(cs);
+  if (st != CUBQL_OK) {
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return st;
+  }
+  *out = g;
+  return CUBQL_OK;
+}
+
+extern "C" int cubql_graph_launch(cubql_graph g, cubql_stream_t s) {
+  clear_error();
+  if (!g || !g->exec) {
+    set_error("cubql_graph_launch: invalid graph");
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  CUBQL_HIP(hipGraphLaunch(g->exec, (hipStream_t)s));
+  return CUBQL_OK;
+}
+
+extern "C" int cubql_graph_destroy(cubql_graph g) {
+  if (!g) return CUBQL_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return CUBQL_OK;
+}

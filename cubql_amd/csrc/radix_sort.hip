@@ -167,18 +167,26 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const K *__restrict__ kin,
   }
 }
 
+// Scratch layout (bytes): status | tileHist | digitBase | tickets.
+size_t sort_ws_bytes(uint32_t nMax, int keyBits) {
+  const int passes = (keyBits + 7) / 8;
+  const size_t maxTiles = std::max<uint32_t>(1u, div_up(nMax, kTile));
+  return maxTiles * 256 * 8 + maxTiles * passes * 256 * 4 + passes * 256 * 4 + 64 + 256;
+}
+
 template <typename K>
-int sort_pairs(K *kA, uint32_t *vA, K *kB, uint32_t *vB, const uint32_t *nDev, uint32_t nMax,
-               int keyBits, hipStream_t s, const cubql_mem_resource *mem, K **kRes, uint32_t **vRes) {
+int sort_pairs_ws(K *kA, uint32_t *vA, K *kB, uint32_t *vB, const uint32_t *nDev, uint32_t nMax,
+                  int keyBits, hipStream_t s, void *ws, K **kRes, uint32_t **vRes) {
   const int passes = (keyBits + 7) / 8;
   const uint32_t maxTiles = std::max<uint32_t>(1u, div_up(nMax, kTile));
-  Scratch scr(mem, s);
-  uint32_t *tileHist = nullptr, *digitBase = nullptr, *tickets = nullptr;
-  uint64_t *status = nullptr;
-  CUBQL_CHECK(scr.alloc(&tileHist, (size_t)maxTiles * passes * 256));
-  CUBQL_CHECK(scr.alloc(&digitBase, (size_t)passes * 256));
-  CUBQL_CHECK(scr.alloc(&tickets, 16));
-  CUBQL_CHECK(scr.alloc(&status, (size_t)maxTiles * 256));
+  char *p = (char *)ws;
+  uint64_t *status = (uint64_t *)p;
+  p += (size_t)maxTiles * 256 * 8;
+  uint32_t *tileHist = (uint32_t *)p;
+  p += (size_t)maxTiles * passes * 256 * 4;
+  uint32_t *digitBase = (uint32_t *)p;
+  p += (size_t)passes * 256 * 4;
+  uint32_t *tickets = (uint32_t *)p;
   CUBQL_CHECK(fill_u32((uint32_t *)status, 0u, (size_t)maxTiles * 512, s));
   CUBQL_CHECK(fill_u32(tickets, 0u, 16, s));
   k_hist<K><<<maxTiles, kBlock, 0, s>>>(kA, nDev, passes, tileHist);
@@ -205,7 +213,23 @@ int sort_pairs(K *kA, uint32_t *vA, K *kB, uint32_t *vB, const uint32_t *nDev, u
   return CUBQL_OK;
 }
 
+template <typename K>
+int sort_pairs(K *kA, uint32_t *vA, K *kB, uint32_t *vB, const uint32_t *nDev, uint32_t nMax,
+               int keyBits, hipStream_t s, const cubql_mem_resource *mem, K **kRes, uint32_t **vRes) {
+  Scratch scr(mem, s);
+  char *ws = nullptr;
+  CUBQL_CHECK(scr.alloc(&ws, sort_ws_bytes(nMax, keyBits)));
+  return sort_pairs_ws<K>(kA, vA, kB, vB, nDev, nMax, keyBits, s, ws, kRes, vRes);
+}
+
 }  // namespace rs
+
+size_t radix_sort_ws_bytes(uint32_t nMax, int keyBits) { return rs::sort_ws_bytes(nMax, keyBits); }
+int radix_sort_pairs_ws_u64(uint64_t *kA, uint32_t *vA, uint64_t *kB, uint32_t *vB, const uint32_t *nDev,
+                            uint32_t nMax, int keyBits, hipStream_t s, void *ws, uint64_t **kRes,
+                            uint32_t **vRes) {
+  return rs::sort_pairs_ws<uint64_t>(kA, vA, kB, vB, nDev, nMax, keyBits, s, ws, kRes, vRes);
+}
 
 int radix_sort_pairs_dev_u32(uint32_t *kA, uint32_t *vA, uint32_t *kB, uint32_t *vB, const uint32_t *nDev,
                              uint32_t nMax, int keyBits, hipStream_t s, const cubql_mem_resource *mem,

@@ -160,3 +160,30 @@ def _lcg_range(seed: int, b: int, e: int, dims: int) -> np.ndarray:
         state = (state * np.uint64(1664525) + np.uint64(1013904223)) & _M32
         out[:, d] = ((state & np.uint64(0x00FFFFFF)).astype(np.float32) / np.float32(16777216.0))
     return out
+
+
+def chamfer_cloud_sizes(num_clouds: int, seed: int, lo: int = 64, hi: int = 65536):
+    """Cloud sizes log-uniform in [lo, hi]: one LCG<8>(seed, cloud) draw each."""
+    u = lcg_uniform(seed, num_clouds, 0.0, 1.0, dims=1)[:, 0].astype(np.float64)
+    return np.floor(np.exp(np.log(lo) + u * (np.log(hi + 1) - np.log(lo)))).astype(np.int64).clip(lo, hi)
+
+
+def chamfer_clouds_torch(sizes: np.ndarray, seed: int, scale_seed: int, device="cuda"):
+    """Concatenated float32 clouds: point i of the batch is N(0, I) from four
+    LCG<8>(seed, i) draws (Box-Muller), scaled by its cloud's factor
+    U[0.1, 10] (LCG<8>(scale_seed, cloud)).  Returns (points [n,3], offsets
+    [numClouds+1] int32), both on `device`."""
+    import torch
+    n = int(sizes.sum())
+    off = np.zeros(len(sizes) + 1, dtype=np.int64)
+    off[1:] = np.cumsum(sizes)
+    s = 0.1 + lcg_uniform(scale_seed, len(sizes), 0.0, 1.0, dims=1)[:, 0].astype(np.float64) * 9.9
+    u = lcg_uniform_torch(seed, n, 0.0, 1.0, dims=4, device=device).to(torch.float64)
+    r1 = torch.sqrt(-2.0 * torch.log(1.0 - u[:, 0]))
+    r2 = torch.sqrt(-2.0 * torch.log(1.0 - u[:, 2]))
+    th1, th2 = 2.0 * np.pi * u[:, 1], 2.0 * np.pi * u[:, 3]
+    g = torch.stack([r1 * torch.cos(th1), r1 * torch.sin(th1), r2 * torch.cos(th2)], 1)
+    cloud = torch.repeat_interleave(torch.arange(len(sizes), device=device),
+                                    torch.from_numpy(sizes).to(device))
+    pts = (g * torch.from_numpy(s).to(device)[cloud][:, None]).to(torch.float32).contiguous()
+    return pts, torch.from_numpy(off.astype(np.int32)).to(device)

@@ -203,6 +203,63 @@ int cubql_points_distance_backward_f3(const float *d_points,
                                       float *d_gradQueries,
                                       float *d_gradPoints, cubql_stream_t s);
 
+/* ------------------------------------------------------------- batched */
+
+/* Tunables of the batched path (chosen by cubql_amd.autotune):
+   leafSize      leaf threshold T of every per-cloud tree (>= 1);
+   blockSize     query workgroup size: 64, 128 or 256;
+   stackDepth    traversal stack entries before the deep fallback: 32 or 64;
+   maxCloudSize  optional upper bound on any cloud's size (0 = unknown); it
+                 only bounds the number of hierarchy launches. */
+typedef struct {
+  int leafSize;
+  int blockSize;
+  int stackDepth;
+  uint32_t maxCloudSize;
+} cubql_batch_params;
+
+/* Workspace bytes for cubql_batch_chamfer_f3 (no reference counterpart). */
+size_t cubql_batch_chamfer_workspace_f3(uint32_t numClouds, uint32_t numSrc,
+                                        uint32_t numTgt);
+
+/* Batched bidirectional closest point + Chamfer adjoint over numClouds
+   independent cloud pairs (no reference counterpart).  Cloud c holds source
+   points [srcOffsets[c], srcOffsets[c+1]) and target points likewise
+   (device arrays of numClouds+1 entries).  Builds one 30-bit Morton tree per
+   cloud and side (each identical to cubql_radix30_builder_f3 on that cloud
+   with makeLeafThreshold = leafSize), then for every source point the
+   closest target point of the same cloud (global index, -1 if the cloud has
+   no target points) and vice versa, with the lowest index winning ties.
+   If any of loss/gradSrc/gradTgt is given:
+     loss[c] = mean_{i in S_c} |s_i - nn(s_i)| + mean_{j in T_c} |t_j - nn(t_j)|
+   and gradSrc/gradTgt = d(sum_c loss[c]) / d(points) (overwritten).
+   No host synchronisation and no allocation: the call can be captured in a
+   HIP graph (see cubql_batch_chamfer_graph_f3). */
+int cubql_batch_chamfer_f3(const float *d_src, const uint32_t *d_srcOffsets,
+                           uint32_t numSrc, const float *d_tgt,
+                           const uint32_t *d_tgtOffsets, uint32_t numTgt,
+                           uint32_t numClouds, const cubql_batch_params *params,
+                           void *d_workspace, size_t workspaceBytes,
+                           int32_t *d_srcNN, float *d_srcSqrDist,
+                           int32_t *d_tgtNN, float *d_tgtSqrDist, float *d_loss,
+                           float *d_gradSrc, float *d_gradTgt, cubql_stream_t s);
+
+/* The same launch sequence captured once into an executable HIP graph bound
+   to these buffers; replay it with cubql_graph_launch. */
+typedef struct cubql_graph_s *cubql_graph;
+int cubql_batch_chamfer_graph_f3(const float *d_src, const uint32_t *d_srcOffsets,
+                                 uint32_t numSrc, const float *d_tgt,
+                                 const uint32_t *d_tgtOffsets, uint32_t numTgt,
+                                 uint32_t numClouds,
+                                 const cubql_batch_params *params,
+                                 void *d_workspace, size_t workspaceBytes,
+                                 int32_t *d_srcNN, float *d_srcSqrDist,
+                                 int32_t *d_tgtNN, float *d_tgtSqrDist,
+                                 float *d_loss, float *d_gradSrc,
+                                 float *d_gradTgt, cubql_graph *out);
+int cubql_graph_launch(cubql_graph g, cubql_stream_t s);
+int cubql_graph_destroy(cubql_graph g);
+
 /* ------------------------------------------------------------ profiling */
 
 /* Event-based per-kernel timing, the counterpart of the reference's

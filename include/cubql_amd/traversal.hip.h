@@ -67,14 +67,16 @@ __device__ __forceinline__ float sqr_dist_pt(float ax, float ay, float az, float
 
 // forEachLeaf.  leaf(const uint32_t *prims, uint32_t count) -> float.
 // Returns false if the stack overflowed (query incomplete).
+// `root` selects the subtree to walk (0 for a whole BinaryBVH; a cloud's
+// root node in a batched forest).
 template <int STACK, typename LeafFn>
 __device__ __forceinline__ bool for_each_leaf(const cubql_bvh3f &bvh, float qx, float qy, float qz,
-                                              float sqrMax, LeafFn &&leaf) {
+                                              float sqrMax, LeafFn &&leaf, uint32_t root = 0) {
   if (bvh.numNodes == 0) return true;
   uint2 stack[STACK];
   int sp = 0;
   float cull = sqrMax;
-  uint64_t node = node_oc(load_node(bvh.nodes, 0));
+  uint64_t node = node_oc(load_node(bvh.nodes, root));
   while (true) {
     bool haveLeaf = true;
     while (oc_cnt(node) == 0) {
@@ -113,13 +115,13 @@ __device__ __forceinline__ bool for_each_leaf(const cubql_bvh3f &bvh, float qx, 
 // forEachPrim: prim(uint32_t primID) -> float (new squared cull radius).
 template <int STACK, typename PrimFn>
 __device__ __forceinline__ bool for_each_prim(const cubql_bvh3f &bvh, float qx, float qy, float qz,
-                                              float sqrMax, PrimFn &&prim) {
+                                              float sqrMax, PrimFn &&prim, uint32_t root = 0) {
   auto leaf = [&](const uint32_t *ids, uint32_t cnt) -> float {
     float r = INFINITY;
     for (uint32_t i = 0; i < cnt; i++) r = fminf(r, prim(ids[i]));
     return r;
   };
-  return for_each_leaf<STACK>(bvh, qx, qy, qz, sqrMax, leaf);
+  return for_each_leaf<STACK>(bvh, qx, qy, qz, sqrMax, leaf, root);
 }
 
 }  // namespace cubql_dev

@@ -93,6 +93,77 @@ def cpu_baseline(n_sample: int, nq_sample: int, budget_s: float):
     }
 
 
+def bench_batched(cq, dev, num_clouds: int, plain_budget_s: float, tune: bool):
+    """C5: batched uneven Chamfer fwd+bwd vs the plain per-cloud path."""
+    import numpy as np
+    import torch
+
+    from cubql_amd import synth
+    from cubql_amd.autotune import tune_batch
+
+    ss = synth.chamfer_cloud_sizes(num_clouds, 101)
+    ts = synth.chamfer_cloud_sizes(num_clouds, 102)
+    src, so = synth.chamfer_clouds_torch(ss, 103, 104, device=dev)
+    tgt, to = synth.chamfer_clouds_torch(ts, 105, 106, device=dev)
+    torch.cuda.synchronize()
+    maxc = int(max(ss.max(), ts.max()))
+    if tune:
+        cands = [(T, b, st) for T in (4, 8, 16) for b in (128, 256) for st in (32, 64)]
+        params, times = tune_batch(src, so, tgt, to, cands, reps=2, max_cloud_size=maxc)
+    else:
+        params, times = cq.BatchParams(8, 256, 64, maxc), {}
+    batch = cq.ChamferBatch(src, so, tgt, to, params).capture()
+    for _ in range(2):
+        batch.replay()
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.replay()
+    torch.cuda.synchronize()
+    batched_ms = (time.perf_counter() - t0) / reps * 1e3
+    # plain: per cloud, radix30Builder on both sides + fcp both ways + adjoint
+    so_h, to_h = so.cpu().numpy(), to.cpu().numpy()
+    mem = cq.TorchMemory(dev)
+    cfg = cq.BuildConfig(makeLeafThreshold=params.leafSize)
+    gs = torch.empty_like(src)
+    gt = torch.empty_like(tgt)
+    done = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c in range(num_clouds):
+        S, T = src[so_h[c]:so_h[c + 1]], tgt[to_h[c]:to_h[c + 1]]
+        bt = cq.cuda.radix30Builder(cq.points_to_boxes(T), cfg, mem=mem)
+        bs = cq.cuda.radix30Builder(cq.points_to_boxes(S), cfg, mem=mem)
+        nn_s, d_s = cq.points.findClosest(bt, T, S)
+        nn_t, d_t = cq.points.findClosest(bs, S, T)
+        g_s = torch.full((S.shape[0],), 1.0 / S.shape[0], device=dev)
+        g_t = torch.full((T.shape[0],), 1.0 / T.shape[0], device=dev)
+        cq.points_backward(T, S, nn_s, g_s, gs[so_h[c]:so_h[c + 1]], gt[to_h[c]:to_h[c + 1]])
+        cq.points_backward(S, T, nn_t, g_t, gt[to_h[c]:to_h[c + 1]], gs[so_h[c]:so_h[c + 1]])
+        bt.free()
+        bs.free()
+        done += 1
+        if time.perf_counter() - t0 > plain_budget_s:
+            break
+    torch.cuda.synchronize()
+    plain_ms_done = (time.perf_counter() - t0) * 1e3
+    frac_pts = (so_h[done] + to_h[done]) / (so_h[-1] + to_h[-1])
+    plain_ms = plain_ms_done if done == num_clouds else plain_ms_done / max(done / num_clouds, 1e-9)
+    return {
+        "clouds": num_clouds, "src_points": int(so_h[-1]), "tgt_points": int(to_h[-1]),
+        "params": {"leafSize": params.leafSize, "blockSize": params.blockSize,
+                   "stackDepth": params.stackDepth},
+        "tuning_ms": times,
+        "batched_ms": batched_ms, "batched_mpoints_s": (so_h[-1] + to_h[-1]) / batched_ms / 1e3,
+        "plain_ms": plain_ms, "plain_clouds_timed": done,
+        "plain_note": ("all clouds timed" if done == num_clouds else
+                       f"first {done} clouds timed ({frac_pts:.1%} of the points), scaled by cloud count"),
+        "speedup_vs_plain": plain_ms / batched_ms,
+        "batched_is_graph_replay": True,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +174,10 @@ def main():
     ap.add_argument("--knn-queries", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--batch-clouds", type=int, default=4096)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--plain-budget-s", type=float, default=40.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -223,6 +298,12 @@ def main():
     print(f"[bench] fcp {nq/fcp_ms/1e3:.1f} Mq/s, knn {nk/knn_ms/1e3:.1f} Mq/s", file=sys.stderr,
           flush=True)
 
+    batched = None
+    if not args.no_batch:
+        batched = bench_batched(cq, dev, args.batch_clouds, args.plain_budget_s, not args.no_tune)
+        print(f"[bench] batched chamfer: {batched['batched_ms']:.2f} ms, plain {batched['plain_ms']:.1f} ms"
+              f" ({batched['speedup_vs_plain']:.1f}x)", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(min(args.cpu_sample, N), 200_000, budget_s=12.0)
@@ -248,6 +329,7 @@ def main():
         "radix_builds": radix,
         "fcp_mqueries_s": nq / fcp_ms / 1e3,
         "knn_mqueries_s": nk / knn_ms / 1e3,
+        "batched_chamfer": batched,
         "kernels": kern,
         "roofline": roofline,
         "cpu_baseline": cpu,

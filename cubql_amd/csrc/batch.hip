@@ -46,21 +46,22 @@ struct Side {
   uint32_t *ids;        // sorted global point indices = primIDs of the forest
   Node *nodes;          // capacity 2n + 2
   uint32_t *rootOf;     // numClouds
-  uint32_t *ctr;        // [0] node count, [1] n (device), [8 + level] open count
+  uint32_t *ctr;        // counters, see C_* (C_WORDS words)
 };
 
 struct Temp {
   uint64_t *keysB;
   uint32_t *valsB;
   uint2 *ranges;
-  uint32_t *open[2];
-  uint32_t *parent;
-  uint32_t *arrive;
+  uint32_t *levels;     // node ids level by level (capacity 2n+2)
   void *sortWs;
   uint32_t *ovf;        // [0] count, [1] deep failures, then the list
 };
 
-enum { C_NODES = 0, C_N = 1, C_LEVEL0 = 8 };
+// [C_LEVEL0 + l] node count of level l; [C_LSTART0 + l] its start in the
+// all-levels node list
+enum { C_NODES = 0, C_N = 1, C_LEVEL0 = 8, C_LSTART0 = 256, C_WORDS = 512 };
+constexpr int kMaxLevels = C_LSTART0 - C_LEVEL0 - 4;
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
   uint32_t x = v & 0x3ffu;
@@ -113,9 +114,10 @@ __global__ void k_binit(Side sd, Temp tp, uint32_t numClouds) {
     for (int d = 0; d < 3; d++) { z.lo[d] = 0.f; z.hi[d] = 0.f; }
     z.oc = 0;
     sd.nodes[1] = z;
+    sd.ctr[C_LSTART0] = 0;
     if (sd.n > 0) {
       tp.ranges[0] = make_uint2(0u, sd.n);
-      tp.open[0][0] = 0;
+      tp.levels[0] = 0;
       sd.ctr[C_LEVEL0] = 1;
     }
   }
@@ -147,8 +149,10 @@ __global__ __launch_bounds__(256) void k_blevel(Side sd, Temp tp, uint32_t T, in
   __shared__ uint32_t ss[256 / kWave + 1];
   __shared__ uint32_t sNode, sOpen;
   const uint32_t nOpen = sd.ctr[C_LEVEL0 + level];
-  const uint32_t *open = tp.open[level & 1];
-  uint32_t *next = tp.open[(level & 1) ^ 1];
+  const uint32_t start = sd.ctr[C_LSTART0 + level];
+  const uint32_t *open = tp.levels + start;
+  uint32_t *next = tp.levels + start + nOpen;
+  if (blockIdx.x == 0 && threadIdx.x == 0) sd.ctr[C_LSTART0 + level + 1] = start + nOpen;
   for (uint32_t b0 = blockIdx.x * 256; b0 < nOpen; b0 += gridDim.x * 256) {
     const uint32_t i = b0 + threadIdx.x;
     uint32_t node = 0, split = 0;
@@ -191,56 +195,29 @@ __global__ __launch_bounds__(256) void k_blevel(Side sd, Temp tp, uint32_t T, in
   }
 }
 
-__global__ void k_bparents(Side sd, Temp tp) {
-  const uint32_t numNodes = sd.ctr[C_NODES];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < numNodes; i += gridDim.x * blockDim.x) {
-    tp.arrive[i] = 0;
-    if (i == 0) tp.parent[0] = 0xffffffffu;
-    if (i == 1) continue;
-    const uint64_t oc = sd.nodes[i].oc;
-    if (oc_count(oc) != 0) continue;
-    const uint32_t o = (uint32_t)oc_offset(oc);
-    tp.parent[o] = i;
-    tp.parent[o + 1] = i;
-  }
-}
-
-// leaves start, the second child to arrive finishes the parent
-__global__ void k_brefit(Side sd, Temp tp) {
-  const uint32_t numNodes = sd.ctr[C_NODES];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < numNodes; i += gridDim.x * blockDim.x) {
-    if (i == 1) continue;
-    const uint64_t oc = sd.nodes[i].oc;
-    const uint32_t cnt = oc_count(oc);
-    if (cnt == 0) continue;
+// bottom-up refit, one launch per level from the deepest (no atomics)
+__global__ void k_brefit_level(Side sd, Temp tp, int level) {
+  const uint32_t n = sd.ctr[C_LEVEL0 + level];
+  const uint32_t *ids = tp.levels + sd.ctr[C_LSTART0 + level];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    Node *nd = &sd.nodes[ids[i]];
+    const uint64_t oc = nd->oc;
+    const uint32_t cnt = oc_count(oc), o = (uint32_t)oc_offset(oc);
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    const uint32_t o = (uint32_t)oc_offset(oc);
-    for (uint32_t j = 0; j < cnt; j++) {
-      const size_t p = sd.ids[o + j];
-      for (int d = 0; d < 3; d++) {
-        const float x = sd.pts[3 * p + d];
-        lo[d] = fminf(lo[d], x);
-        hi[d] = fmaxf(hi[d], x);
+    if (cnt) {
+      for (uint32_t j = 0; j < cnt; j++) {
+        const size_t p = sd.ids[o + j];
+        for (int d = 0; d < 3; d++) {
+          const float x = sd.pts[3 * p + d];
+          lo[d] = fminf(lo[d], x);
+          hi[d] = fmaxf(hi[d], x);
+        }
       }
+    } else {
+      const Node l = sd.nodes[o], r = sd.nodes[o + 1];
+      for (int d = 0; d < 3; d++) { lo[d] = fminf(l.lo[d], r.lo[d]); hi[d] = fmaxf(l.hi[d], r.hi[d]); }
     }
-    uint32_t node = i;
-    while (true) {
-      Node *nd = &sd.nodes[node];
-      for (int d = 0; d < 3; d++) { nd->lo[d] = lo[d]; nd->hi[d] = hi[d]; }
-      const uint32_t par = tp.parent[node];
-      if (par == 0xffffffffu) break;
-      __threadfence();
-      if (atomicAdd(&tp.arrive[par], 1u) == 0) break;
-      __threadfence();
-      const uint32_t o2 = (uint32_t)oc_offset(sd.nodes[par].oc);
-      const volatile Node *l = &sd.nodes[o2];
-      const volatile Node *r = &sd.nodes[o2 + 1];
-      for (int d = 0; d < 3; d++) {
-        lo[d] = fminf(l->lo[d], r->lo[d]);
-        hi[d] = fmaxf(l->hi[d], r->hi[d]);
-      }
-      node = par;
-    }
+    for (int d = 0; d < 3; d++) { nd->lo[d] = lo[d]; nd->hi[d] = hi[d]; }
   }
 }
 
@@ -364,7 +341,7 @@ struct Layout {
 
 struct Plan {
   size_t keys[2], ids[2], nodes[2], roots[2], ctr[2];
-  size_t keysB, valsB, ranges, open0, open1, parent, arrive, sortWs, ovf;
+  size_t keysB, valsB, ranges, levels, sortWs, ovf;
   size_t total;
   uint32_t maxLevels;
 };
@@ -380,15 +357,12 @@ Plan make_plan(uint32_t numClouds, uint32_t nSrc, uint32_t nTgt) {
     p.ids[s] = L.take(4 * n);
     p.nodes[s] = L.take(32 * (2 * n + 2));
     p.roots[s] = L.take(4 * (size_t)std::max(numClouds, 1u));
-    p.ctr[s] = L.take(4 * 256);
+    p.ctr[s] = L.take(4 * C_WORDS);
   }
   p.keysB = L.take(8 * (size_t)nMax);
   p.valsB = L.take(4 * (size_t)nMax);
   p.ranges = L.take(8 * (2 * (size_t)nMax + 2));
-  p.open0 = L.take(4 * ((size_t)nMax + 2));
-  p.open1 = L.take(4 * ((size_t)nMax + 2));
-  p.parent = L.take(4 * (2 * (size_t)nMax + 2));
-  p.arrive = L.take(4 * (2 * (size_t)nMax + 2));
+  p.levels = L.take(4 * (2 * (size_t)nMax + 2));
   int cloudBits = 0;
   while ((1u << cloudBits) < std::max(numClouds, 1u)) cloudBits++;
   p.sortWs = L.take(radix_sort_ws_bytes(nMax, kCloudShift + cloudBits));
@@ -419,7 +393,7 @@ struct Args {
 
 int build_side(Side &sd, Temp &tp, const Args &a, const Plan &pl, hipStream_t s, int numCU) {
   const int cbits = cloud_bits(a.numClouds);
-  CUBQL_CHECK(fill_u32(sd.ctr, 0u, 256, s));
+  CUBQL_CHECK(fill_u32(sd.ctr, 0u, C_WORDS, s));
   CUBQL_CHECK(fill_u32(&sd.ctr[C_N], sd.n, 1, s));
   if (a.numClouds) k_bkeys<<<a.numClouds, 256, 0, s>>>(sd, a.numClouds);
   uint64_t *kr = nullptr;
@@ -436,13 +410,11 @@ int build_side(Side &sd, Temp &tp, const Args &a, const Plan &pl, hipStream_t s,
   int midBits = 0;
   const uint32_t maxCloud = a.prm.maxCloudSize ? a.prm.maxCloudSize : std::max(sd.n, 1u);
   while ((1ull << midBits) < maxCloud) midBits++;
-  const int levels = std::min(cbits + kCloudShift + midBits + 2, 256 - C_LEVEL0 - 2);
+  const int levels = std::min(cbits + kCloudShift + midBits + 2, kMaxLevels);
   const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(div_up(sd.n, 256), (uint32_t)numCU * 4));
   const uint32_t T = (uint32_t)std::max(1, std::min(a.prm.leafSize, 65535));
   for (int l = 0; l < levels; l++) k_blevel<<<grid, 256, 0, s>>>(sd, tp, T, l);
-  const uint32_t rg = std::max<uint32_t>(1u, std::min<uint32_t>(div_up(2 * (size_t)sd.n, 256), (uint32_t)numCU * 8));
-  k_bparents<<<rg, 256, 0, s>>>(sd, tp);
-  k_brefit<<<rg, 256, 0, s>>>(sd, tp);
+  for (int l = levels - 1; l >= 0; l--) k_brefit_level<<<grid, 256, 0, s>>>(sd, tp, l);
   CUBQL_LAUNCH_CHECK();
   return CUBQL_OK;
 }
@@ -492,10 +464,7 @@ int run(const Args &a, hipStream_t s) {
   tp.keysB = (uint64_t *)(a.ws + pl.keysB);
   tp.valsB = (uint32_t *)(a.ws + pl.valsB);
   tp.ranges = (uint2 *)(a.ws + pl.ranges);
-  tp.open[0] = (uint32_t *)(a.ws + pl.open0);
-  tp.open[1] = (uint32_t *)(a.ws + pl.open1);
-  tp.parent = (uint32_t *)(a.ws + pl.parent);
-  tp.arrive = (uint32_t *)(a.ws + pl.arrive);
+  tp.levels = (uint32_t *)(a.ws + pl.levels);
   tp.sortWs = a.ws + pl.sortWs;
   tp.ovf = (uint32_t *)(a.ws + pl.ovf);
 

@@ -34,7 +34,9 @@ constexpr int kTile = kBlock * kItems;
 constexpr int kLevelsPerSync = 8;
 constexpr int kMaxLevels = 160;  // > 63 key bits + log2(2^31) mid splits
 
-enum { C_NUMVALID = 0, C_NODECOUNT = 1, C_LEVEL0 = 8 };  // counter slots
+// counter slots: [C_LEVEL0 + l] = node count of level l, [C_LSTART0 + l] =
+// start of level l in the all-levels node list
+enum { C_NUMVALID = 0, C_NODECOUNT = 1, C_LEVEL0 = 8, C_LSTART0 = C_LEVEL0 + kMaxLevels + 4 };
 
 struct RCtx {
   const cubql_box3f *boxes;
@@ -47,7 +49,7 @@ struct RCtx {
   uint32_t *ids;      // compacted primIDs (sorted in place by the sort)
   Node *nodes;        // temp nodes, capacity 2*n+2
   uint2 *ranges;      // [b,e) per node
-  uint32_t *open[2];  // open node ids per level (ping-pong)
+  uint32_t *levels;   // node ids level by level (capacity 2n+2)
 };
 
 __global__ __launch_bounds__(kBlock) void k_rcount(RCtx c) {
@@ -122,7 +124,7 @@ __global__ __launch_bounds__(1024) void k_rscan(RCtx c, uint32_t numBlocks) {
     c.nodes[1] = z;  // radix.h:585-586
     if (run > 0) {
       c.ranges[0] = make_uint2(0u, run);
-      c.open[0][0] = 0;
+      c.levels[0] = 0;
       c.ctr[C_LEVEL0] = 1;
     } else {
       c.ctr[C_LEVEL0] = 0;
@@ -219,8 +221,10 @@ __global__ __launch_bounds__(kBlock) void k_rlevel(RCtx c, const K *keys, int le
   __shared__ uint32_t ss[kBlock / kWave + 1];
   __shared__ uint32_t sNode, sOpen;
   const uint32_t nOpen = c.ctr[C_LEVEL0 + level];
-  const uint32_t *open = c.open[level & 1];
-  uint32_t *next = c.open[(level & 1) ^ 1];
+  const uint32_t start = c.ctr[C_LSTART0 + level];
+  const uint32_t *open = c.levels + start;
+  uint32_t *next = c.levels + start + nOpen;  // the next level follows this one
+  if (blockIdx.x == 0 && threadIdx.x == 0) c.ctr[C_LSTART0 + level + 1] = start + nOpen;
   for (uint32_t b0 = blockIdx.x * kBlock; b0 < nOpen; b0 += gridDim.x * kBlock) {
     const uint32_t i = b0 + threadIdx.x;
     uint32_t node = 0, split = 0;
@@ -258,6 +262,31 @@ __global__ __launch_bounds__(kBlock) void k_rlevel(RCtx c, const K *keys, int le
 
 __global__ void k_rinit_levels(uint32_t *ctr) {
   for (int i = threadIdx.x; i < kMaxLevels + 2; i += blockDim.x) ctr[C_LEVEL0 + 1 + i] = 0;
+  if (threadIdx.x == 0) ctr[C_LSTART0] = 0;
+}
+
+// Bottom-up refit, one launch per level from the deepest: a leaf takes the
+// union of its prim boxes, an inner node the union of its children (written
+// by the previous launch).  No atomics or fences.
+__global__ void k_rrefit_level(RCtx c, Node *nodes, const uint32_t *primIDs, int level) {
+  const uint32_t n = c.ctr[C_LEVEL0 + level];
+  const uint32_t *ids = c.levels + c.ctr[C_LSTART0 + level];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    Node *nd = &nodes[ids[i]];
+    const uint64_t oc = nd->oc;
+    const uint32_t cnt = oc_count(oc), off = (uint32_t)oc_offset(oc);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (cnt) {
+      for (uint32_t j = 0; j < cnt; j++) {
+        const cubql_box3f b = load_box(c.boxes, primIDs[off + j]);
+        for (int d = 0; d < 3; d++) { lo[d] = fminf(lo[d], b.lower[d]); hi[d] = fmaxf(hi[d], b.upper[d]); }
+      }
+    } else {
+      const Node l = nodes[off], r = nodes[off + 1];
+      for (int d = 0; d < 3; d++) { lo[d] = fminf(l.lo[d], r.lo[d]); hi[d] = fmaxf(l.hi[d], r.hi[d]); }
+    }
+    for (int d = 0; d < 3; d++) { nd->lo[d] = lo[d]; nd->hi[d] = hi[d]; }
+  }
 }
 
 template <typename K>
@@ -274,15 +303,14 @@ int build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, in
   uint32_t *idsB = nullptr;
   CUBQL_CHECK(scr.alloc(&c.blockCount, numBlocks + 1));
   CUBQL_CHECK(scr.alloc(&c.blockBounds, 8 * (size_t)numBlocks + 8));
-  CUBQL_CHECK(scr.alloc(&c.ctr, C_LEVEL0 + kMaxLevels + 8));
+  CUBQL_CHECK(scr.alloc(&c.ctr, C_LSTART0 + kMaxLevels + 8));
   CUBQL_CHECK(scr.alloc(&c.quant, 8));
   CUBQL_CHECK(scr.alloc(&keysA, n));
   CUBQL_CHECK(scr.alloc(&keysB, n));
   CUBQL_CHECK(scr.alloc(&idsB, n));
   CUBQL_CHECK(scr.alloc(&c.nodes, 2 * (size_t)n + 2));
   CUBQL_CHECK(scr.alloc(&c.ranges, 2 * (size_t)n + 2));
-  CUBQL_CHECK(scr.alloc(&c.open[0], (size_t)n + 2));
-  CUBQL_CHECK(scr.alloc(&c.open[1], (size_t)n + 2));
+  CUBQL_CHECK(scr.alloc(&c.levels, 2 * (size_t)n + 2));
   uint32_t *primIDs = nullptr;
   CUBQL_CHECK(mem_alloc(mem, (void **)&primIDs, (size_t)n * sizeof(uint32_t), s));
   c.ids = primIDs;
@@ -319,7 +347,7 @@ int build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, in
       hipDeviceGetAttribute(&numCU, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     numCU = 256;
   const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(div_up(n, kBlock), (uint32_t)numCU * 8));
-  std::vector<uint32_t> hc(C_LEVEL0 + kMaxLevels + 8);
+  std::vector<uint32_t> hc(C_LSTART0 + kMaxLevels + 8);
   int level = 0;
   int hs = prof_begin(s);
   while (true) {
@@ -349,7 +377,10 @@ int build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, in
   bvh->primIDs = primIDs;
   bvh->numPrims = numValid;
   int rs = prof_begin(s);
-  st = refit_tree(bvh, boxes, s, mem);
+  for (int l = level - 1; l >= 0; l--)
+    k_rrefit_level<<<std::max<uint32_t>(1u, std::min<uint32_t>(div_up(n, kBlock), (uint32_t)numCU * 8)), kBlock, 0, s>>>(
+        c, nodes, primIDs, l);
+  st = hipGetLastError() == hipSuccess ? CUBQL_OK : (set_error("refit launch failed"), CUBQL_ERR_HIP);
   prof_end(rs, s, "refit");
   prof_set(rs, 24.0 * numValid + 64.0 * numNodes, numNodes);
   if (prof_on()) {

@@ -103,6 +103,7 @@ def lib():
         "cubql_gpu_builder_f3": (I, [P(BVH3f), V, U32, P(BuildConfig), V, P(MemResource)]),
         "cubql_radix_builder_f3": (I, [P(BVH3f), V, U32, P(BuildConfig), V, P(MemResource)]),
         "cubql_radix30_builder_f3": (I, [P(BVH3f), V, U32, P(BuildConfig), V, P(MemResource)]),
+        "cubql_sah_builder_f3": (I, [P(BVH3f), V, U32, P(BuildConfig), C.c_int, V, P(MemResource)]),
         "cubql_free_f3": (I, [P(BVH3f), V, P(MemResource)]),
         "cubql_refit_f3": (I, [P(BVH3f), V, V, P(MemResource)]),
         "cubql_fcp_points_f3": (I, [P(BVH3f), V, V, U32, F, V, V, V]),
@@ -244,13 +245,14 @@ def _as_boxes(boxes: torch.Tensor) -> torch.Tensor:
     return boxes.contiguous().view(-1, 6)
 
 
-def _build(fn_name: str, boxes: torch.Tensor, config: Optional[BuildConfig], stream, mem):
+def _build(fn_name: str, boxes: torch.Tensor, config: Optional[BuildConfig], stream, mem,
+           *extra):
     boxes = _as_boxes(boxes)
     config = config or BuildConfig()
     mem = mem or TorchMemory(boxes.device)
     bvh = BinaryBVH(mem)
     fn = getattr(lib(), fn_name)
-    _check(fn(C.byref(bvh.c), _ptr(boxes), boxes.shape[0], C.byref(config),
+    _check(fn(C.byref(bvh.c), _ptr(boxes), boxes.shape[0], C.byref(config), *extra,
               C.c_void_p(_stream_handle(stream)), C.byref(mem.resource)), fn_name)
     return bvh
 
@@ -272,6 +274,12 @@ class cuda:  # noqa: N801 -- mirrors the reference namespace cuBQL::cuda
     def radixBuilder(boxes, config=None, stream=None, mem=None) -> BinaryBVH:
         """cuBQL::cuda::radixBuilder (builder/cuda.h:134-140), 63-bit keys."""
         return _build("cubql_radix_builder_f3", boxes, config, stream, mem)
+
+    @staticmethod
+    def sahBuilder(boxes, config=None, stream=None, mem=None, numBins: int = 16) -> BinaryBVH:
+        """cuBQL::cuda::sahBuilder (builder/cuda.h:143-153) with a bin count
+        parameter (the reference fixes 16); leaves <= config.maxAllowedLeafSize."""
+        return _build("cubql_sah_builder_f3", boxes, config, stream, mem, C.c_int(numBins))
 
     @staticmethod
     def radix30Builder(boxes, config=None, stream=None, mem=None) -> BinaryBVH:

@@ -29,6 +29,58 @@ __device__ inline void choose_split(const float lo[3], const float hi[3], int &d
   pos = (l + h) / 2.f;
 }
 
+// One wave writes a 32-byte node: lanes 0-7 store one word each.
+__device__ __forceinline__ void store_node_wave(Node *dst, const float (&lo)[3], const float (&hi)[3],
+                                                uint64_t oc) {
+  const uint32_t l = lane_id();
+  uint32_t w = 0;
+  w = (l == 0) ? __float_as_uint(lo[0]) : w;
+  w = (l == 1) ? __float_as_uint(lo[1]) : w;
+  w = (l == 2) ? __float_as_uint(lo[2]) : w;
+  w = (l == 3) ? __float_as_uint(hi[0]) : w;
+  w = (l == 4) ? __float_as_uint(hi[1]) : w;
+  w = (l == 5) ? __float_as_uint(hi[2]) : w;
+  w = (l == 6) ? (uint32_t)oc : w;
+  w = (l == 7) ? (uint32_t)(oc >> 32) : w;
+  if (l < 8) reinterpret_cast<uint32_t *>(dst)[l] = w;
+}
+
+// Tile status word for decoupled look-back: level tag (bits 34+), flag
+// (bits 32-33: 0 empty, 1 local count, 2 inclusive prefix), value.
+__device__ inline uint64_t status_word(uint32_t tag, uint32_t flag, uint32_t val) {
+  return ((uint64_t)tag << 34) | ((uint64_t)flag << 32) | val;
+}
+
+// Decoupled look-back (one thread per tile).  Tile t is the k-th tile of its
+// node; tiles are claimed in ticket order, so every tile this one waits on is
+// held by a running workgroup that publishes without waiting on later tiles.
+// Publishes `local` and returns the sum over the node's earlier tiles.
+__device__ inline uint32_t lookback_exclusive(uint64_t *status, uint32_t t, uint32_t k, uint32_t tag,
+                                              uint32_t local) {
+  if (k == 0) {
+    __hip_atomic_store(&status[t], status_word(tag, 2, local), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return 0;
+  }
+  __hip_atomic_store(&status[t], status_word(tag, 1, local), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t prefix = 0;
+  uint32_t j = t - 1;
+  while (true) {
+    const uint64_t w = __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t wt = (uint32_t)(w >> 34);
+    const uint32_t wf = (uint32_t)(w >> 32) & 3u;
+    if (wt != tag || wf == 0) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    prefix += (uint32_t)w;
+    if (wf == 2) break;
+    j--;
+  }
+  __hip_atomic_store(&status[t], status_word(tag, 2, prefix + local), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  return prefix;
+}
+
 // block-wide min/max reduction of 12 floats (left lo/hi, right lo/hi)
 template <int BLOCK>
 __device__ inline void block_reduce_bounds(float (&v)[12], float *smem /*12*BLOCK/64*/) {

@@ -24,6 +24,12 @@ static uint32_t leaf_threshold(const cubql_build_config *cfg) {
   return (uint32_t)std::min(t, 65535);
 }
 
+// BuildConfig::maxAllowedLeafSize (bvh.h:50-53); <= 0 means only the 16-bit
+// count limit applies.
+static uint32_t leaf_cap(const cubql_build_config *cfg) {
+  return cfg->maxAllowedLeafSize > 0 ? (uint32_t)std::min(cfg->maxAllowedLeafSize, 65535) : 65535u;
+}
+
 extern "C" int cubql_gpu_builder_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes, uint32_t numBoxes,
                                     const cubql_build_config *cfg, cubql_stream_t stream,
                                     const cubql_mem_resource *mem) {
@@ -34,8 +40,20 @@ extern "C" int cubql_gpu_builder_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes
   switch (cfg->buildMethod) {
     case CUBQL_SPATIAL_MEDIAN:
       return sm_build(bvh, d_boxes, numBoxes, leaf_threshold(cfg), s, mem);
+    case CUBQL_SAH:
+      return sah_build(bvh, d_boxes, numBoxes, leaf_threshold(cfg), leaf_cap(cfg), 16, s, mem);
     default:
       set_error("cubql_gpu_builder_f3: buildMethod %d not supported", cfg->buildMethod);
       return CUBQL_ERR_UNSUPPORTED;
   }
+}
+
+extern "C" int cubql_sah_builder_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes, uint32_t numPrims,
+                                    const cubql_build_config *cfg, int numBins, cubql_stream_t stream,
+                                    const cubql_mem_resource *mem) {
+  clear_error();
+  CUBQL_CHECK(check_build_args("cubql_sah_builder_f3", bvh, d_boxes, numPrims, cfg));
+  if (numPrims == 0) return CUBQL_OK;
+  return sah_build(bvh, d_boxes, numPrims, leaf_threshold(cfg), leaf_cap(cfg), numBins,
+                   (hipStream_t)stream, mem);
 }

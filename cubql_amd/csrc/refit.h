@@ -9,4 +9,8 @@ int refit_tree(cubql_bvh3f *bvh, const cubql_box3f *boxes, hipStream_t s,
 // Spatial-median builder (sm_builder.hip); T = leaf threshold (>= 1).
 int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, hipStream_t s,
              const cubql_mem_resource *mem);
+// Binned SAH builder (sah_builder.hip); 1 <= T <= cap <= 65535 after
+// clamping, numBins in [2, 64].
+int sah_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, uint32_t cap,
+              int numBins, hipStream_t s, const cubql_mem_resource *mem);
 }  // namespace cubql

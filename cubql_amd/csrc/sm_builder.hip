@@ -266,9 +266,6 @@ __global__ __launch_bounds__(1024) void k_plan(Ctx c, int level) {
   }
 }
 
-__device__ inline uint64_t status_word(uint32_t tag, uint32_t flag, uint32_t val) {
-  return ((uint64_t)tag << 34) | ((uint64_t)flag << 32) | val;
-}
 
 // Partition: each workgroup repeatedly claims a tile (ticket order, so a
 // tile only ever waits on tiles already claimed by running workgroups),
@@ -330,29 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
 
     // decoupled look-back over the tiles of this node
     if (threadIdx.x == 0) {
-      uint32_t prefix = 0;
-      if (k == 0) {
-        __hip_atomic_store(&c.tileStatus[t], status_word(tag, 2, nLeft), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      } else {
-        __hip_atomic_store(&c.tileStatus[t], status_word(tag, 1, nLeft), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t j = t - 1;
-        while (true) {
-          uint64_t w = __hip_atomic_load(&c.tileStatus[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          uint32_t wt = (uint32_t)(w >> 34);
-          uint32_t wf = (uint32_t)(w >> 32) & 3u;
-          if (wt != tag || wf == 0) {
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-          }
-          prefix += (uint32_t)w;
-          if (wf == 2) break;
-          j--;
-        }
-        __hip_atomic_store(&c.tileStatus[t], status_word(tag, 2, prefix + nLeft), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      }
+      const uint32_t prefix = lookback_exclusive(c.tileStatus, t, k, tag, nLeft);
       sPrefix = prefix;
       if (k == nTilesSeg - 1) c.segLeft[s] = prefix + nLeft;
     }
@@ -506,21 +481,6 @@ __global__ __launch_bounds__(kBlock) void k_fixup(Ctx c, int level) {
 // the node numbering is deterministic.
 
 struct SmallNodeOut { Node *loc; };
-
-__device__ __forceinline__ void store_node_wave(Node *dst, const float (&lo)[3], const float (&hi)[3],
-                                                uint64_t oc) {
-  const uint32_t l = lane_id();
-  uint32_t w = 0;
-  w = (l == 0) ? __float_as_uint(lo[0]) : w;
-  w = (l == 1) ? __float_as_uint(lo[1]) : w;
-  w = (l == 2) ? __float_as_uint(lo[2]) : w;
-  w = (l == 3) ? __float_as_uint(hi[0]) : w;
-  w = (l == 4) ? __float_as_uint(hi[1]) : w;
-  w = (l == 5) ? __float_as_uint(hi[2]) : w;
-  w = (l == 6) ? (uint32_t)oc : w;
-  w = (l == 7) ? (uint32_t)(oc >> 32) : w;
-  if (l < 8) reinterpret_cast<uint32_t *>(dst)[l] = w;
-}
 
 __device__ __forceinline__ uint32_t fr_pack(uint32_t b, uint32_t e, uint32_t me) {
   return b | (e << 8) | (me << 16);

@@ -92,6 +92,19 @@ int cubql_gpu_builder_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes,
                          uint32_t numBoxes, const cubql_build_config *cfg,
                          cubql_stream_t s, const cubql_mem_resource *mem);
 
+/* cuBQL::cuda::sahBuilder -- builder/cuda.h:143-153, sah_builder.h:396-552,
+   with the bin count as a parameter (the reference fixes SAHBins::numBins
+   = 16, sah_builder.h:53-61; gpuBuilder with buildMethod SAH uses 16).
+   numBins in [2, 64].  Leaves never exceed cfg->maxAllowedLeafSize
+   (bvh.h:50-53; <= 0 means 65535): where the SAH prefers a larger leaf the
+   best split is taken, and a node no plane can split is cut in the middle
+   of its prim range.  bvh->numPrims = numPrims, invalid prims (lower.x >
+   upper.x, sah_builder.h:140) listed after all valid ones. */
+int cubql_sah_builder_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes,
+                         uint32_t numPrims, const cubql_build_config *cfg,
+                         int numBins, cubql_stream_t s,
+                         const cubql_mem_resource *mem);
+
 /* cuBQL::cuda::radixBuilder -- builder/cuda.h:134-140, radix.h:591-771.
    63-bit Morton keys (21 bits per axis, radix.h:30-33).  bvh->numPrims =
    number of valid prims; primIDs in Morton order, ties by primID. */

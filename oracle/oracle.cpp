@@ -102,6 +102,141 @@ void sm_rec(int nodeID, int begin, int end, std::vector<Topo> &topo,
   sm_rec(child + 1, mid, end, topo, ids, alt, boxes, T);
 }
 
+// ---------------------------------------------------------------------------
+// binned SAH (cuda/sah_builder.h)
+
+// surfaceArea(box3f) (box.h:199-205)
+inline float surface_area(const orc_box3f &b) {
+  const float sx = b.hi[0] - b.lo[0];
+  const float sy = b.hi[1] - b.lo[1];
+  const float sz = b.hi[2] - b.lo[2];
+  return sx * sy + sx * sz + sy * sz;
+}
+
+// bin of a centre coordinate (sah_builder.h:181-187, 301-308).  The float
+// clamp equals max(0,min(NB-1,int(rel*NB))) for every finite in-range value
+// and gives the saturating GPU conversion's answer for NaN and +-inf.
+inline int sah_bin(float c, float lo, float hi, int nb) {
+  if (!(hi > lo)) return 0;
+  const float rel = (c - lo) / (hi - lo + 1e-20f);
+  const float f = rel * (float)nb;
+  if (f >= (float)(nb - 1)) return nb - 1;
+  if (f > 0.f) return (int)f;
+  return 0;
+}
+
+struct SahBin { orc_box3f b; int count; };
+
+// evaluateSAH (sah_builder.h:62-106); allowLeaf=false drops the leaf
+// candidate.  Returns the split dim or -1.
+int sah_evaluate(const std::vector<SahBin> &bins, int nb, bool allowLeaf, int &splitBin) {
+  float bestCost = INFINITY;
+  int splitDim = -1;
+  std::vector<float> rAreas(nb);
+  for (int d = 0; d < 3; d++) {
+    const SahBin *db = bins.data() + d * nb;
+    orc_box3f box = empty_box();
+    int rCount = 0;
+    for (int b = nb - 1; b >= 0; --b) {
+      grow(box, db[b].b);
+      rCount += db[b].count;
+      rAreas[b] = surface_area(box);
+    }
+    const float leafCost = rAreas[0] * (float)rCount;
+    if (allowLeaf && leafCost < bestCost) {
+      bestCost = leafCost;
+      splitDim = -1;
+    }
+    box = empty_box();
+    int lCount = 0;
+    for (int b = 0; b < nb; b++) {
+      const float rArea = rAreas[b];
+      const float lArea = surface_area(box);
+      if (lCount > 0 && rCount > 0) {
+        const float cost = lArea * (float)lCount + rArea * (float)rCount;
+        if (cost < bestCost) {
+          bestCost = cost;
+          splitDim = d;
+          splitBin = b;
+        }
+      }
+      grow(box, db[b].b);
+      lCount += db[b].count;
+      rCount -= db[b].count;
+    }
+  }
+  return splitDim;
+}
+
+struct SahCtx {
+  const orc_box3f *boxes;
+  int T, nb, cap;
+  std::vector<int> ids, alt;
+  std::vector<Topo> topo;
+};
+
+// centroid bounds, bins, decision for the prims ids[begin,end)
+// returns dim (-1 leaf, -2 middle split)
+int sah_decide_range(const SahCtx &c, const int *ids, int begin, int end, float clo[3],
+                     float chi[3], int &splitBin) {
+  const int n = end - begin;
+  for (int d = 0; d < 3; d++) { clo[d] = INFINITY; chi[d] = -INFINITY; }
+  for (int i = begin; i < end; i++) {
+    float ce[3];
+    box_center(c.boxes[ids[i]], ce);
+    for (int d = 0; d < 3; d++) { clo[d] = std::min(clo[d], ce[d]); chi[d] = std::max(chi[d], ce[d]); }
+  }
+  std::vector<SahBin> bins(3 * c.nb, SahBin{empty_box(), 0});
+  for (int i = begin; i < end; i++) {
+    const orc_box3f &pb = c.boxes[ids[i]];
+    for (int d = 0; d < 3; d++) {
+      // binPrims uses 0.5f*(lo+hi), equal to center() for every input
+      const float pd = 0.5f * (pb.lo[d] + pb.hi[d]);
+      SahBin &bn = bins[d * c.nb + sah_bin(pd, clo[d], chi[d], c.nb)];
+      grow(bn.b, pb);
+      bn.count++;
+    }
+  }
+  int dim = sah_evaluate(bins, c.nb, true, splitBin);
+  if (dim < 0 && n > c.cap) {
+    dim = sah_evaluate(bins, c.nb, false, splitBin);
+    if (dim < 0) dim = -2;
+  }
+  return dim;
+}
+
+void sah_rec(SahCtx &c, int nodeID, int begin, int end) {
+  const int n = end - begin;
+  if (n <= std::min(c.T, c.cap)) { c.topo[nodeID] = {begin, n}; return; }
+  float clo[3], chi[3];
+  int bin = 0;
+  const int dim = sah_decide_range(c, c.ids.data(), begin, end, clo, chi, bin);
+  if (dim == -1) { c.topo[nodeID] = {begin, n}; return; }
+  const int mid0 = begin + n / 2;
+  int nl = 0, nr = 0;
+  for (int i = begin; i < end; i++) {
+    const int pid = c.ids[i];
+    bool left;
+    if (dim == -2) {
+      left = i < mid0;
+    } else {
+      const orc_box3f &pb = c.boxes[pid];
+      const float pd = 0.5f * (pb.lo[dim] + pb.hi[dim]);
+      left = sah_bin(pd, clo[dim], chi[dim], c.nb) < bin;  // side = (bin >= splitBin)
+    }
+    if (left) c.alt[begin + nl++] = pid;
+    else c.alt[end - ++nr] = pid;
+  }
+  for (int i = begin; i < end; i++) c.ids[i] = c.alt[i];
+  const int mid = begin + nl;
+  const int child = (int)c.topo.size();
+  c.topo.push_back({0, 0});
+  c.topo.push_back({0, 0});
+  c.topo[nodeID] = {child, 0};
+  sah_rec(c, child + 0, begin, mid);
+  sah_rec(c, child + 1, mid, end);
+}
+
 orc_box3f refit_rec(orc_node3f *nodes, uint64_t id, const uint32_t *primIDs,
                     const orc_box3f *boxes) {
   orc_node3f &n = nodes[id];
@@ -213,6 +348,42 @@ int orc_sm_build(const orc_box3f *boxes, int n, int leafThreshold,
   *numPrims = (int)ids.size();
   if (!ids.empty()) refit_rec(nodes, 0, primIDs, boxes);
   return 0;
+}
+
+int orc_sah_build(const orc_box3f *boxes, int n, int leafThreshold, int numBins,
+                  int leafCap, orc_node3f *nodes, uint32_t *primIDs, int *numNodes,
+                  int *numPrims) {
+  SahCtx c;
+  c.boxes = boxes;
+  c.T = std::max(1, leafThreshold);
+  c.nb = numBins;
+  c.cap = leafCap;
+  for (int i = 0; i < n; i++)
+    if (box_valid(boxes[i])) c.ids.push_back(i);
+  c.alt.resize(c.ids.size());
+  c.topo.assign(1, Topo{0, 0});
+  if (!c.ids.empty()) sah_rec(c, 0, 0, (int)c.ids.size());
+  for (size_t i = 0; i < c.ids.size(); i++) primIDs[i] = (uint32_t)c.ids[i];
+  for (size_t i = 0; i < c.topo.size(); i++) {
+    nodes[i].oc = pack_oc((uint64_t)c.topo[i].offset, (uint64_t)c.topo[i].count);
+    nodes[i].b = empty_box();
+  }
+  *numNodes = c.ids.empty() ? 0 : (int)c.topo.size();
+  *numPrims = (int)c.ids.size();
+  if (!c.ids.empty()) refit_rec(nodes, 0, primIDs, boxes);
+  return 0;
+}
+
+int orc_sah_decide(const orc_box3f *boxes, int n, int numBins, int allowLeaf, int *bin) {
+  SahCtx c;
+  c.boxes = boxes;
+  c.nb = numBins;
+  c.cap = allowLeaf ? n : 0;
+  c.T = 0;
+  std::vector<int> ids(n);
+  std::iota(ids.begin(), ids.end(), 0);
+  float clo[3], chi[3];
+  return sah_decide_range(c, ids.data(), 0, n, clo, chi, *bin);
 }
 
 void orc_refit(orc_node3f *nodes, int numNodes, const uint32_t *primIDs,

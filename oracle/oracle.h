@@ -32,6 +32,30 @@ int orc_sm_build(const orc_box3f *boxes, int n, int leafThreshold,
                  orc_node3f *nodes, uint32_t *primIDs, int *numNodes,
                  int *numPrims);
 
+/* Binned SAH builder; restates cuda/sah_builder.h:55-106 (evaluateSAH),
+   152-205 (binPrims), 261-330 (selectSplits/updatePrims) as a recursion.
+   Bins: bin = clamp(int(rel*numBins)) with rel = (c-lo)/(hi-lo+1e-20f), bin 0
+   when hi <= lo; a bin accumulates the prim BOXES and a count.  A node with
+   count <= min(leafThreshold,leafCap) is a leaf; otherwise the first minimum
+   of [leaf cost A*N, then for dim 0..2, bin 0..numBins-1 the split cost
+   A_L*N_L + A_R*N_R over splits with both sides non-empty] decides, with
+   strict '<' as in the reference.  Leaf cap (no counterpart in the
+   reference's SAH builder, which ignores BuildConfig::maxAllowedLeafSize): a
+   leaf choice over more than leafCap prims takes the best split instead;
+   with no valid split the node's current order is cut in the middle.
+   Partition order: left side stable, right side reversed (the GPU
+   builder's order), so primIDs match position for position.  Node
+   numbering: root-first DFS without an unused node 1. */
+int orc_sah_build(const orc_box3f *boxes, int n, int leafThreshold, int numBins,
+                  int leafCap, orc_node3f *nodes, uint32_t *primIDs, int *numNodes,
+                  int *numPrims);
+
+/* One SAH split decision over an explicit set of prim boxes (the root step
+   of orc_sah_build) with centroid bounds taken over all n boxes.  Returns
+   the split dim and *bin, -1 for a leaf, or -2 when allowLeaf is 0 and no
+   split has both sides non-empty (the middle-split fallback). */
+int orc_sah_decide(const orc_box3f *boxes, int n, int numBins, int allowLeaf, int *bin);
+
 /* Bottom-up exact refit of node bounds from prim boxes (sm_builder.h:633-675,
    spatialMedian.h:117-134). Only nodes reachable from node 0 are written. */
 void orc_refit(orc_node3f *nodes, int numNodes, const uint32_t *primIDs,

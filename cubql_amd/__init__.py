@@ -103,6 +103,7 @@ def lib():
         "cubql_gpu_builder_f3": (I, [P(BVH3f), V, U32, P(BuildConfig), V, P(MemResource)]),
         "cubql_radix_builder_f3": (I, [P(BVH3f), V, U32, P(BuildConfig), V, P(MemResource)]),
         "cubql_radix30_builder_f3": (I, [P(BVH3f), V, U32, P(BuildConfig), V, P(MemResource)]),
+        "cubql_rebin_radix_builder_f3": (I, [P(BVH3f), V, U32, P(BuildConfig), V, P(MemResource)]),
         "cubql_sah_builder_f3": (I, [P(BVH3f), V, U32, P(BuildConfig), C.c_int, V, P(MemResource)]),
         "cubql_free_f3": (I, [P(BVH3f), V, P(MemResource)]),
         "cubql_refit_f3": (I, [P(BVH3f), V, V, P(MemResource)]),
@@ -276,8 +277,14 @@ class cuda:  # noqa: N801 -- mirrors the reference namespace cuBQL::cuda
         return _build("cubql_radix_builder_f3", boxes, config, stream, mem)
 
     @staticmethod
+    def rebinRadixBuilder(boxes, config=None, stream=None, mem=None) -> BinaryBVH:
+        """cuBQL::cuda::rebinRadixBuilder (builder/cuda.h:148-154): robust
+        30-bit Morton builder that re-keys ranges of equal keys."""
+        return _build("cubql_rebin_radix_builder_f3", boxes, config, stream, mem)
+
+    @staticmethod
     def sahBuilder(boxes, config=None, stream=None, mem=None, numBins: int = 16) -> BinaryBVH:
-        """cuBQL::cuda::sahBuilder (builder/cuda.h:143-153) with a bin count
+        """cuBQL::cuda::sahBuilder (builder/cuda.h:122-130) with a bin count
         parameter (the reference fixes 16); leaves <= config.maxAllowedLeafSize."""
         return _build("cubql_sah_builder_f3", boxes, config, stream, mem, C.c_int(numBins))
 

@@ -32,11 +32,17 @@ constexpr int kBlock = 256;
 constexpr int kItems = 8;
 constexpr int kTile = kBlock * kItems;
 constexpr int kLevelsPerSync = 8;
-constexpr int kMaxLevels = 160;  // > 63 key bits + log2(2^31) mid splits
+// > 63 key bits + log2(2^31) mid splits; the rebin builder restarts the
+// level sequence after each rebin round, so it gets more room
+constexpr int kMaxLevels = 1024;
 
 // counter slots: [C_LEVEL0 + l] = node count of level l, [C_LSTART0 + l] =
-// start of level l in the all-levels node list
-enum { C_NUMVALID = 0, C_NODECOUNT = 1, C_LEVEL0 = 8, C_LSTART0 = C_LEVEL0 + kMaxLevels + 4 };
+// start of level l in the all-levels node list; C_JOBS.. are used by the
+// rebin builder only
+enum {
+  C_NUMVALID = 0, C_NODECOUNT = 1, C_JOBS = 2, C_JOBPRIMS = 3, C_SMALL = 4, C_CURJOBS = 5,
+  C_LEVEL0 = 8, C_LSTART0 = C_LEVEL0 + kMaxLevels + 4
+};
 
 struct RCtx {
   const cubql_box3f *boxes;
@@ -49,7 +55,15 @@ struct RCtx {
   uint32_t *ids;      // compacted primIDs (sorted in place by the sort)
   Node *nodes;        // temp nodes, capacity 2*n+2
   uint2 *ranges;      // [b,e) per node
-  uint32_t *levels;   // node ids level by level (capacity 2n+2)
+  uint32_t *levels;   // node ids level by level (capacity 2n+2, more with rebinning)
+  // rebin builder
+  uint32_t *jobs;      // nodes whose keys are all equal (> 32 prims)
+  uint32_t *jobStart;  // exclusive prefix of job sizes
+  float *jobQuant;     // per job: lo xyz, scale
+  uint32_t *jobLocal;  // per job: 1 = key by local index
+  uint32_t *small;     // thread-phase nodes (<= 32 prims)
+  uint32_t *smallCount, *smallBase;
+  Node *loc;           // thread-phase local nodes at loc[2*begin + j]
 };
 
 __global__ __launch_bounds__(kBlock) void k_rcount(RCtx c) {
@@ -262,6 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_rlevel(RCtx c, const K *keys, int le
 
 __global__ void k_rinit_levels(uint32_t *ctr) {
   for (int i = threadIdx.x; i < kMaxLevels + 2; i += blockDim.x) ctr[C_LEVEL0 + 1 + i] = 0;
+  if (threadIdx.x < C_LEVEL0 - C_JOBS) ctr[C_JOBS + threadIdx.x] = 0;
   if (threadIdx.x == 0) ctr[C_LSTART0] = 0;
 }
 
@@ -389,6 +404,443 @@ int build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, in
   return st;
 }
 
+
+// ------------------------------------------------------------------ rebin
+// Robust rebinning builder (rebinMortonBuilder.h:595-1476, NO_BLOCK_PHASE
+// as in the reference, :22).
+constexpr uint32_t kThreadPhase = 32;  // smallEnoughToLeaveBroadPhase (:624-636)
+
+// One broad-phase level: each open node becomes a leaf (size <= T), a
+// thread-phase node (size <= 32), splits at findSplit, or -- all keys
+// equal -- a rebin job (broadPhaseCreateNodes, :664-749).
+__global__ __launch_bounds__(kBlock) void k_rlevel_rebin(RCtx c, const uint32_t *keys, int level) {
+  __shared__ uint32_t ss[kBlock / kWave + 1];
+  __shared__ uint32_t sNode, sOpen, sSmall, sJob;
+  const uint32_t nOpen = c.ctr[C_LEVEL0 + level];
+  const uint32_t start = c.ctr[C_LSTART0 + level];
+  const uint32_t *open = c.levels + start;
+  uint32_t *next = c.levels + start + nOpen;
+  if (blockIdx.x == 0 && threadIdx.x == 0) c.ctr[C_LSTART0 + level + 1] = start + nOpen;
+  for (uint32_t b0 = blockIdx.x * kBlock; b0 < nOpen; b0 += gridDim.x * kBlock) {
+    const uint32_t i = b0 + threadIdx.x;
+    uint32_t node = 0, split = 0;
+    uint2 r = make_uint2(0, 0);
+    int cat = -1;  // 0 split, 1 thread phase, 2 rebin job
+    if (i < nOpen) {
+      node = open[i];
+      r = c.ranges[node];
+      const uint32_t size = r.y - r.x;
+      if (size <= c.T) {
+        c.nodes[node].oc = pack_oc(r.x, size);
+      } else if (size <= kThreadPhase) {
+        cat = 1;
+      } else if (keys[r.x] != keys[r.y - 1]) {
+        split = find_split<uint32_t>(keys, r.x, r.y);
+        cat = 0;
+      } else {
+        cat = 2;
+      }
+    }
+    uint32_t totSp, totSm, totJ;
+    const uint32_t exSp = block_excl_scan<kBlock>(cat == 0 ? 1u : 0u, ss, &totSp);
+    const uint32_t exSm = block_excl_scan<kBlock>(cat == 1 ? 1u : 0u, ss, &totSm);
+    const uint32_t exJ = block_excl_scan<kBlock>(cat == 2 ? 1u : 0u, ss, &totJ);
+    if (threadIdx.x == 0) {
+      if (totSp) {
+        sNode = atomicAdd(&c.ctr[C_NODECOUNT], 2 * totSp);
+        sOpen = atomicAdd(&c.ctr[C_LEVEL0 + level + 1], 2 * totSp);
+      }
+      if (totSm) sSmall = atomicAdd(&c.ctr[C_SMALL], totSm);
+      if (totJ) sJob = atomicAdd(&c.ctr[C_JOBS], totJ);
+    }
+    __syncthreads();
+    if (cat == 0) {
+      const uint32_t child = sNode + 2 * exSp;
+      c.nodes[node].oc = pack_oc(child, 0);
+      c.ranges[child] = make_uint2(r.x, split);
+      c.ranges[child + 1] = make_uint2(split, r.y);
+      next[sOpen + 2 * exSp] = child;
+      next[sOpen + 2 * exSp + 1] = child + 1;
+    } else if (cat == 1) {
+      c.small[sSmall + exSm] = node;
+    } else if (cat == 2) {
+      c.jobs[sJob + exJ] = node;
+    }
+    __syncthreads();
+  }
+}
+
+// Single workgroup: job prefix sums; the jobs become the open nodes of
+// `level` (the level loop resumes there once their keys are rebuilt).
+__global__ __launch_bounds__(1024) void k_rebin_setup(RCtx c, int level) {
+  __shared__ uint32_t ss[1024 / kWave + 1];
+  const uint32_t J = c.ctr[C_JOBS];
+  const uint32_t start = c.ctr[C_LSTART0 + level];
+  uint32_t run = 0;
+  for (uint32_t b0 = 0; b0 < J; b0 += 1024) {
+    const uint32_t j = b0 + threadIdx.x;
+    uint32_t size = 0;
+    if (j < J) {
+      const uint2 r = c.ranges[c.jobs[j]];
+      size = r.y - r.x;
+      c.levels[start + j] = c.jobs[j];
+    }
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<1024>(size, ss, &tot);
+    if (j < J) c.jobStart[j] = run + ex;
+    run += tot;
+  }
+  if (threadIdx.x == 0) {
+    c.jobStart[J] = run;
+    c.ctr[C_JOBPRIMS] = run;
+    c.ctr[C_CURJOBS] = J;
+    c.ctr[C_JOBS] = 0;
+    c.ctr[C_LEVEL0 + level] = J;
+  }
+}
+
+__device__ __forceinline__ void center3(const cubql_box3f &b, float (&ce)[3]) {
+  for (int d = 0; d < 3; d++) ce[d] = 0.5f * (b.lower[d] + b.upper[d]);
+}
+
+// Quantizer<float,3>::init over a centroid domain (:38-60) into q[4].
+__device__ __forceinline__ void quant_init(const float (&lo)[3], const float (&hi)[3], float *q) {
+  const float ext = fmaxf(fmaxf(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+  q[0] = lo[0]; q[1] = lo[1]; q[2] = lo[2];
+  q[3] = 1024.f * (1.f / fmaxf(ext, 1e-20f));
+}
+
+// Per job (one workgroup each): centroid domain, quantizer, and whether to
+// key by local index: the domain is one point (rebinCreateNewKeys, :1052-
+// 1055), or all its prims fall in one cell (the cells of the domain's lower
+// and upper corner agree on every axis), which the reference would rebin
+// forever.
+__global__ __launch_bounds__(kBlock) void k_rebin_domain(RCtx c) {
+  __shared__ float sb[12 * (kBlock / kWave)];
+  const uint32_t J = c.ctr[C_CURJOBS];
+  for (uint32_t j = blockIdx.x; j < J; j += gridDim.x) {
+    const uint2 r = c.ranges[c.jobs[j]];
+    float v[12];
+    for (int d = 0; d < 3; d++) { v[d] = INFINITY; v[d + 3] = -INFINITY; v[d + 6] = INFINITY; v[d + 9] = -INFINITY; }
+    for (uint32_t i = r.x + threadIdx.x; i < r.y; i += kBlock) {
+      float ce[3];
+      center3(load_box(c.boxes, c.ids[i]), ce);
+      for (int d = 0; d < 3; d++) { v[d] = fminf(v[d], ce[d]); v[d + 3] = fmaxf(v[d + 3], ce[d]); }
+    }
+    block_reduce_bounds<kBlock>(v, sb);
+    if (threadIdx.x == 0) {
+      const float lo[3] = {v[0], v[1], v[2]}, hi[3] = {v[3], v[4], v[5]};
+      float q[4];
+      quant_init(lo, hi, q);
+      const bool point = lo[0] == hi[0] && lo[1] == hi[1] && lo[2] == hi[2];
+      const bool oneCell = morton<uint32_t>(lo, q, 10) == morton<uint32_t>(hi, q, 10);
+      for (int k = 0; k < 4; k++) c.jobQuant[4 * j + k] = q[k];
+      c.jobLocal[j] = (point || oneCell) ? 1u : 0u;
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ uint32_t find_job(const uint32_t *start, uint32_t J, uint32_t t) {
+  uint32_t lo = 0, hi = J;  // last j with start[j] <= t
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (start[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// New (job:key) pairs, one per prim of every job, in position order.
+__global__ __launch_bounds__(kBlock) void k_rebin_keys(RCtx c, uint64_t *rk, uint32_t *rv) {
+  const uint32_t J = c.ctr[C_CURJOBS];
+  const uint32_t P = c.ctr[C_JOBPRIMS];
+  for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < P; t += gridDim.x * kBlock) {
+    const uint32_t j = find_job(c.jobStart, J, t);
+    const uint32_t local = t - c.jobStart[j];
+    const uint32_t id = c.ids[c.ranges[c.jobs[j]].x + local];
+    uint32_t key = local;
+    if (!c.jobLocal[j]) {
+      float ce[3];
+      center3(load_box(c.boxes, id), ce);
+      key = morton<uint32_t>(ce, c.jobQuant + 4 * j, 10);
+    }
+    rk[t] = ((uint64_t)j << 32) | key;
+    rv[t] = id;
+  }
+}
+
+// Sorted pairs back into each job's range (rebinWriteBack, :1002-1023).
+__global__ __launch_bounds__(kBlock) void k_rebin_writeback(RCtx c, uint32_t *keys, const uint64_t *rk,
+                                                            const uint32_t *rv) {
+  const uint32_t P = c.ctr[C_JOBPRIMS];
+  for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < P; t += gridDim.x * kBlock) {
+    const uint64_t k = rk[t];
+    const uint32_t j = (uint32_t)(k >> 32);
+    const uint32_t pos = c.ranges[c.jobs[j]].x + (t - c.jobStart[j]);
+    keys[pos] = (uint32_t)k;
+    c.ids[pos] = rv[t];
+  }
+}
+
+// Thread phase (threadPhaseCreateNodes, :773-880): one thread finishes a
+// range of <= 32 prims.  A range that cannot split is re-keyed in its own
+// domain (or by local index) and sorted with the reference's exchange sort
+// (bubbleSort, :758-771), whose swap sequence fixes the order of equal
+// keys; if it still cannot split it becomes a leaf.  Local nodes (local 0 =
+// the range's node) go to loc[2*begin + j]; boxes are computed bottom-up.
+__global__ __launch_bounds__(kBlock) void k_thread_phase(RCtx c, uint32_t *keys) {
+  const uint32_t nSmall = c.ctr[C_SMALL];
+  for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < nSmall; s += gridDim.x * kBlock) {
+    const uint2 r = c.ranges[c.small[s]];
+    Node *loc = c.loc + 2 * (size_t)r.x;
+    uint32_t stB[kThreadPhase + 1], stE[kThreadPhase + 1], stM[kThreadPhase + 1];
+    int sp = 0;
+    stB[0] = r.x; stE[0] = r.y; stM[0] = 0; sp = 1;
+    uint32_t nLocal = 1;
+    while (sp > 0) {
+      --sp;
+      const uint32_t b = stB[sp], e = stE[sp], me = stM[sp];
+      const uint32_t n = e - b;
+      bool leaf = n <= c.T;
+      uint32_t split = 0;
+      if (!leaf) {
+        if (keys[b] != keys[e - 1]) {
+          split = find_split<uint32_t>(keys, b, e);
+        } else {
+          float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+          for (uint32_t i = b; i < e; i++) {
+            float ce[3];
+            center3(load_box(c.boxes, c.ids[i]), ce);
+            for (int d = 0; d < 3; d++) { lo[d] = fminf(lo[d], ce[d]); hi[d] = fmaxf(hi[d], ce[d]); }
+          }
+          const bool point = lo[0] == hi[0] && lo[1] == hi[1] && lo[2] == hi[2];
+          float q[4];
+          quant_init(lo, hi, q);
+          for (uint32_t i = b; i < e; i++) {
+            if (point) {
+              keys[i] = i - b;
+            } else {
+              float ce[3];
+              center3(load_box(c.boxes, c.ids[i]), ce);
+              keys[i] = morton<uint32_t>(ce, q, 10);
+            }
+          }
+          for (uint32_t i = b; i < e; i++)
+            for (uint32_t j = i + 1; j < e; j++) {
+              const uint32_t ki = keys[i], kj = keys[j];
+              if (kj < ki) {
+                keys[i] = kj; keys[j] = ki;
+                const uint32_t t = c.ids[i];
+                c.ids[i] = c.ids[j];
+                c.ids[j] = t;
+              }
+            }
+          if (keys[b] != keys[e - 1]) split = find_split<uint32_t>(keys, b, e);
+          else leaf = true;
+        }
+      }
+      if (leaf) {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t i = b; i < e; i++) {
+          const cubql_box3f bx = load_box(c.boxes, c.ids[i]);
+          for (int d = 0; d < 3; d++) { lo[d] = fminf(lo[d], bx.lower[d]); hi[d] = fmaxf(hi[d], bx.upper[d]); }
+        }
+        Node nd;
+        for (int d = 0; d < 3; d++) { nd.lo[d] = lo[d]; nd.hi[d] = hi[d]; }
+        nd.oc = pack_oc(b, n);
+        loc[me] = nd;
+        continue;
+      }
+      const uint32_t child = nLocal;
+      nLocal += 2;
+      loc[me].oc = pack_oc(child, 0);
+      stB[sp] = split; stE[sp] = e; stM[sp] = child + 1;
+      stB[sp + 1] = b; stE[sp + 1] = split; stM[sp + 1] = child;
+      sp += 2;
+    }
+    // children have larger local indices than their parent
+    for (int j = (int)nLocal - 1; j >= 0; j--) {
+      Node nd = loc[j];
+      if (oc_count(nd.oc) == 0) {
+        const Node l = loc[oc_offset(nd.oc)], rr = loc[oc_offset(nd.oc) + 1];
+        for (int d = 0; d < 3; d++) { nd.lo[d] = fminf(l.lo[d], rr.lo[d]); nd.hi[d] = fmaxf(l.hi[d], rr.hi[d]); }
+        loc[j] = nd;
+      }
+    }
+    c.smallCount[s] = nLocal;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_thread_scan(RCtx c) {
+  __shared__ uint32_t ss[1024 / kWave + 1];
+  const uint32_t nSmall = c.ctr[C_SMALL];
+  const uint32_t base0 = c.ctr[C_NODECOUNT];
+  uint32_t run = 0;
+  for (uint32_t b0 = 0; b0 < nSmall; b0 += 1024) {
+    const uint32_t i = b0 + threadIdx.x;
+    const uint32_t v = i < nSmall ? c.smallCount[i] - 1 : 0;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<1024>(v, ss, &tot);
+    if (i < nSmall) c.smallBase[i] = base0 + run + ex;
+    run += tot;
+  }
+  if (threadIdx.x == 0) c.ctr[C_NODECOUNT] = base0 + run;
+}
+
+__global__ __launch_bounds__(kWave) void k_thread_emit(RCtx c, Node *out) {
+  const uint32_t nSmall = c.ctr[C_SMALL];
+  for (uint32_t s = blockIdx.x; s < nSmall; s += gridDim.x) {
+    const uint32_t node = c.small[s];
+    const uint32_t cnt = c.smallCount[s];
+    const uint32_t base = c.smallBase[s];
+    const Node *loc = c.loc + 2 * (size_t)c.ranges[node].x;
+    for (uint32_t j = lane_id(); j < cnt; j += kWave) {
+      Node nd = loc[j];
+      if (oc_count(nd.oc) == 0) nd.oc = pack_oc(base + (uint32_t)oc_offset(nd.oc) - 1, 0);
+      out[j == 0 ? node : base + j - 1] = nd;
+    }
+  }
+}
+
+int rebin_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, hipStream_t s,
+                const cubql_mem_resource *mem) {
+  Scratch scr(mem, s);
+  RCtx c{};
+  c.boxes = boxes;
+  c.n = n;
+  c.T = T;
+  c.bitsPerDim = 10;
+  const uint32_t numBlocks = div_up(n, kTile);
+  const uint32_t maxJobs = n / (kThreadPhase + 1) + 2;
+  uint32_t *keysA = nullptr, *keysB = nullptr, *idsB = nullptr;
+  uint64_t *rkA = nullptr, *rkB = nullptr;
+  uint32_t *rvA = nullptr, *rvB = nullptr;
+  CUBQL_CHECK(scr.alloc(&c.blockCount, numBlocks + 1));
+  CUBQL_CHECK(scr.alloc(&c.blockBounds, 8 * (size_t)numBlocks + 8));
+  CUBQL_CHECK(scr.alloc(&c.ctr, C_LSTART0 + kMaxLevels + 8));
+  CUBQL_CHECK(scr.alloc(&c.quant, 8));
+  CUBQL_CHECK(scr.alloc(&keysA, n));
+  CUBQL_CHECK(scr.alloc(&keysB, n));
+  CUBQL_CHECK(scr.alloc(&idsB, n));
+  CUBQL_CHECK(scr.alloc(&c.nodes, 2 * (size_t)n + 2));
+  CUBQL_CHECK(scr.alloc(&c.ranges, 2 * (size_t)n + 2));
+  // a job node is listed again in the level after its rebin round
+  CUBQL_CHECK(scr.alloc(&c.levels, 2 * (size_t)n + 2 + 64 * (size_t)maxJobs));
+  CUBQL_CHECK(scr.alloc(&c.jobs, maxJobs));
+  CUBQL_CHECK(scr.alloc(&c.jobStart, maxJobs + 1));
+  CUBQL_CHECK(scr.alloc(&c.jobQuant, 4 * (size_t)maxJobs));
+  CUBQL_CHECK(scr.alloc(&c.jobLocal, maxJobs));
+  CUBQL_CHECK(scr.alloc(&c.small, (size_t)n + 1));
+  CUBQL_CHECK(scr.alloc(&c.smallCount, (size_t)n + 1));
+  CUBQL_CHECK(scr.alloc(&c.smallBase, (size_t)n + 1));
+  CUBQL_CHECK(scr.alloc(&c.loc, 2 * (size_t)n + 64));
+  CUBQL_CHECK(scr.alloc(&rkA, n));
+  CUBQL_CHECK(scr.alloc(&rkB, n));
+  CUBQL_CHECK(scr.alloc(&rvA, n));
+  CUBQL_CHECK(scr.alloc(&rvB, n));
+  uint32_t *primIDs = nullptr;
+  CUBQL_CHECK(mem_alloc(mem, (void **)&primIDs, (size_t)n * sizeof(uint32_t), s));
+  c.ids = primIDs;
+  auto fail = [&](int st) {
+    mem_free(mem, primIDs, s);
+    return st;
+  };
+  k_rinit_levels<<<1, 256, 0, s>>>(c.ctr);
+  int ks = prof_begin(s);
+  k_rcount<<<numBlocks, kBlock, 0, s>>>(c);
+  k_rscan<<<1, 1024, 0, s>>>(c, numBlocks);
+  k_rkeys<uint32_t><<<numBlocks, kBlock, 0, s>>>(c, keysA);
+  prof_end(ks, s, "rebin_keys");
+  prof_set(ks, 2.0 * 24.0 * n + 8.0 * n, n);
+  if (hipGetLastError() != hipSuccess) return fail((set_error("rebin key launch failed"), CUBQL_ERR_HIP));
+  uint32_t *keys = nullptr, *vRes = nullptr;
+  int st = radix_sort_pairs_dev_u32(keysA, primIDs, keysB, idsB, &c.ctr[C_NUMVALID], n, 30, s, mem, &keys, &vRes);
+  if (st) return fail(st);
+  if (vRes != primIDs &&
+      hipMemcpyAsync(primIDs, vRes, (size_t)n * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return fail((set_error("id copy failed"), CUBQL_ERR_HIP));
+  int dev = 0, numCU = 256;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&numCU, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    numCU = 256;
+  const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(div_up(n, kBlock), (uint32_t)numCU * 8));
+  int jobBits = 1;
+  while ((1ull << jobBits) < (uint64_t)maxJobs) jobBits++;
+  std::vector<uint32_t> hc(C_LSTART0 + kMaxLevels + 8);
+  int level = 0, rounds = 0;
+  int hs = prof_begin(s);
+  while (true) {
+    const int first = level;
+    for (int i = 0; i < kLevelsPerSync && level < kMaxLevels; i++, level++)
+      k_rlevel_rebin<<<grid, kBlock, 0, s>>>(c, keys, level);
+    if (hipGetLastError() != hipSuccess) return fail((set_error("rebin level launch failed"), CUBQL_ERR_HIP));
+    if (hipMemcpyAsync(hc.data(), c.ctr, sizeof(uint32_t) * hc.size(), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return fail((set_error("rebin level sync failed"), CUBQL_ERR_HIP));
+    int L = -1;
+    for (int l = first; l <= level; l++)
+      if (hc[C_LEVEL0 + l] == 0) { L = l; break; }
+    if (L < 0) {
+      if (level >= kMaxLevels) return fail((set_error("rebin hierarchy deeper than %d levels", kMaxLevels), CUBQL_ERR_INTERNAL));
+      continue;
+    }
+    if (hc[C_JOBS] == 0) { level = L; break; }
+    if (L >= kMaxLevels - 1) return fail((set_error("rebin hierarchy deeper than %d levels", kMaxLevels), CUBQL_ERR_INTERNAL));
+    // rebin round: the jobs become level L's open nodes with rebuilt keys
+    rounds++;
+    const uint32_t J = hc[C_JOBS];
+    k_rebin_setup<<<1, 1024, 0, s>>>(c, L);
+    k_rebin_domain<<<std::min<uint32_t>(J, (uint32_t)numCU * 8), kBlock, 0, s>>>(c);
+    k_rebin_keys<<<grid, kBlock, 0, s>>>(c, rkA, rvA);
+    uint64_t *rkRes = nullptr;
+    uint32_t *rvRes = nullptr;
+    st = radix_sort_pairs_dev_u64(rkA, rvA, rkB, rvB, &c.ctr[C_JOBPRIMS], n, 32 + jobBits, s, mem, &rkRes, &rvRes);
+    if (st) return fail(st);
+    k_rebin_writeback<<<grid, kBlock, 0, s>>>(c, keys, rkRes, rvRes);
+    if (hipGetLastError() != hipSuccess) return fail((set_error("rebin round launch failed"), CUBQL_ERR_HIP));
+    level = L;
+  }
+  prof_end(hs, s, "rebin_hierarchy");
+  const uint32_t numSmall = hc[C_SMALL];
+  if (numSmall) {
+    k_thread_phase<<<std::max<uint32_t>(1u, std::min<uint32_t>(div_up(numSmall, kBlock), (uint32_t)numCU * 8)), kBlock, 0, s>>>(c, keys);
+    k_thread_scan<<<1, 1024, 0, s>>>(c);
+  }
+  if (hipGetLastError() != hipSuccess) return fail((set_error("thread phase launch failed"), CUBQL_ERR_HIP));
+  if (hipMemcpyAsync(hc.data(), c.ctr, sizeof(uint32_t) * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return fail((set_error("rebin sync failed"), CUBQL_ERR_HIP));
+  const uint32_t numValid = hc[C_NUMVALID];
+  const uint32_t numNodes = numValid ? hc[C_NODECOUNT] : 0;
+  prof_set(hs, 8.0 * numNodes, numNodes);
+  Node *nodes = nullptr;
+  st = mem_alloc(mem, (void **)&nodes, std::max<size_t>(numNodes, 2) * sizeof(Node), s);
+  if (st) return fail(st);
+  if (numNodes) {
+    if (hipMemcpyAsync(nodes, c.nodes, (size_t)numNodes * sizeof(Node), hipMemcpyDeviceToDevice, s) != hipSuccess) {
+      mem_free(mem, nodes, s);
+      return fail((set_error("node copy failed"), CUBQL_ERR_HIP));
+    }
+    if (numSmall) k_thread_emit<<<std::min<uint32_t>(numSmall, (uint32_t)numCU * 16), kWave, 0, s>>>(c, nodes);
+  }
+  bvh->nodes = (cubql_node3f *)nodes;
+  bvh->numNodes = numNodes;
+  bvh->primIDs = primIDs;
+  bvh->numPrims = numValid;
+  int rs = prof_begin(s);
+  for (int l = level - 1; l >= 0; l--)
+    k_rrefit_level<<<grid, kBlock, 0, s>>>(c, nodes, primIDs, l);
+  st = hipGetLastError() == hipSuccess ? CUBQL_OK : (set_error("refit launch failed"), CUBQL_ERR_HIP);
+  prof_end(rs, s, "refit");
+  prof_set(rs, 24.0 * numValid + 64.0 * numNodes, numNodes);
+  if (prof_on()) {
+    if (hipStreamSynchronize(s) == hipSuccess) prof_flush();
+  }
+  (void)rounds;
+  return st;
+}
+
 }  // namespace rb
 }  // namespace cubql
 
@@ -419,6 +871,25 @@ extern "C" int cubql_radix_builder_f3(cubql_bvh3f *bvh, const cubql_box3f *d_box
                                       const cubql_build_config *cfg, cubql_stream_t s,
                                       const cubql_mem_resource *mem) {
   return radix_entry("cubql_radix_builder_f3", bvh, d_boxes, numPrims, cfg, s, mem, 21);
+}
+
+extern "C" int cubql_rebin_radix_builder_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes,
+                                            uint32_t numPrims, const cubql_build_config *cfg,
+                                            cubql_stream_t stream, const cubql_mem_resource *mem) {
+  clear_error();
+  if (!bvh || !cfg || (numPrims && !d_boxes)) {
+    set_error("cubql_rebin_radix_builder_f3: null argument");
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  if (numPrims > 0x7fffffffu) {
+    set_error("cubql_rebin_radix_builder_f3: too many prims");
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  if (numPrims == 0) return CUBQL_OK;
+  // rebinMortonBuilder.h:1175-1178
+  int T = cfg->makeLeafThreshold > 0 ? std::min(cfg->makeLeafThreshold, cfg->maxAllowedLeafSize) : 1;
+  T = std::max(1, std::min(T, 65535));
+  return rb::rebin_build(bvh, d_boxes, numPrims, (uint32_t)T, (hipStream_t)stream, mem);
 }
 
 extern "C" int cubql_radix30_builder_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes, uint32_t numPrims,

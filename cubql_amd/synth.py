@@ -187,3 +187,32 @@ def chamfer_clouds_torch(sizes: np.ndarray, seed: int, scale_seed: int, device="
                                     torch.from_numpy(sizes).to(device))
     pts = (g * torch.from_numpy(s).to(device)[cloud][:, None]).to(torch.float32).contiguous()
     return pts, torch.from_numpy(off.astype(np.int32)).to(device)
+
+
+def gaussian_mixture(n: int, seed: int, mixture_seed: int = 40, components: int = 64,
+                     outlier_frac: float = 0.01, outlier_radius: float = 1e6) -> np.ndarray:
+    """[n,3] float32 points from a fixed Gaussian mixture with outliers.
+
+    The mixture (from `mixture_seed`): `components` centres uniform in
+    [-1e3, 1e3]^3 and per-component sigma log-uniform in [1e-4, 1e-1].
+    Point i uses eight LCG<8>(seed, i) draws: u0 picks the component, u1 < 
+    outlier_frac makes it an outlier at distance outlier_radius from the
+    origin in a direction from three normal deviates (u2..u5), otherwise it
+    is centre + sigma * N(0, I) (u2..u7, Box-Muller).  Points and queries of
+    one workload share `mixture_seed` and differ in `seed`.
+    """
+    m = lcg_draws(mixture_seed, components, 4).astype(np.float64)
+    centres = -1e3 + m[:, :3] * 2e3
+    sigma = np.exp(np.log(1e-4) + m[:, 3] * (np.log(1e-1) - np.log(1e-4)))
+    u = lcg_draws(seed, n, 8).astype(np.float64)
+    comp = np.minimum((u[:, 0] * components).astype(np.int64), components - 1)
+    g = np.stack([box_muller(u[:, 2], u[:, 3]), box_muller(u[:, 4], u[:, 5]),
+                  box_muller(u[:, 6], u[:, 7])], axis=1)
+    pts = centres[comp] + sigma[comp, None] * g
+    out = u[:, 1] < outlier_frac
+    if out.any():
+        d = g[out]
+        norm = np.linalg.norm(d, axis=1, keepdims=True)
+        d = np.where(norm > 0, d / np.maximum(norm, 1e-300), np.array([1.0, 0.0, 0.0]))
+        pts[out] = outlier_radius * d
+    return pts.astype(np.float32)

@@ -92,7 +92,7 @@ int cubql_gpu_builder_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes,
                          uint32_t numBoxes, const cubql_build_config *cfg,
                          cubql_stream_t s, const cubql_mem_resource *mem);
 
-/* cuBQL::cuda::sahBuilder -- builder/cuda.h:143-153, sah_builder.h:396-552,
+/* cuBQL::cuda::sahBuilder -- builder/cuda.h:122-130, sah_builder.h:396-552,
    with the bin count as a parameter (the reference fixes SAHBins::numBins
    = 16, sah_builder.h:53-61; gpuBuilder with buildMethod SAH uses 16).
    numBins in [2, 64].  Leaves never exceed cfg->maxAllowedLeafSize
@@ -118,6 +118,20 @@ int cubql_radix_builder_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes,
 int cubql_radix30_builder_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes,
                              uint32_t numPrims, const cubql_build_config *cfg,
                              cubql_stream_t s, const cubql_mem_resource *mem);
+
+/* cuBQL::cuda::rebinRadixBuilder -- builder/cuda.h:148-154,
+   rebinMortonBuilder.h:1166-1487 (NO_BLOCK_PHASE, its default).  30-bit
+   Morton keys; a range of more than 32 prims whose keys are all equal is
+   re-keyed in its own centroid domain and re-sorted (broad phase); ranges
+   of <= 32 prims re-key in place with the reference's exchange sort
+   (thread phase).  Leaf threshold = min(makeLeafThreshold,
+   maxAllowedLeafSize), 0 -> 1.  bvh->numPrims = number of valid prims.  A
+   range whose prims all fall in one cell of their own domain (extent far
+   below 1e-20) is keyed by position, where the reference would not
+   terminate. */
+int cubql_rebin_radix_builder_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes,
+                                 uint32_t numPrims, const cubql_build_config *cfg,
+                                 cubql_stream_t s, const cubql_mem_resource *mem);
 
 /* cuBQL::cuda::free(BinaryBVH&, stream, GpuMemoryResource&) --
    builder/cuda.h:155-161, gpu_builder.h:251-261.  Frees nodes and primIDs

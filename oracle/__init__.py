@@ -57,6 +57,7 @@ def lib():
             "orc_sm_build": (I, [V, I, I, V, V, V, V]),
             "orc_sah_build": (I, [V, I, I, I, I, V, V, V, V]),
             "orc_sah_decide": (I, [V, I, I, I, V]),
+            "orc_rebin_build": (I, [V, I, I, V, V, V, V]),
             "orc_refit": (None, [V, I, V, V]),
             "orc_check_tree": (I, [V, I, V, I, V, I, I, V, V]),
             "orc_canonical": (I64, [V, V, V, I64]),
@@ -159,6 +160,18 @@ def sah_build(boxes: np.ndarray, leaf_threshold: int, num_bins: int = 16,
     npr = C.c_int()
     lib().orc_sah_build(_p(b), n, leaf_threshold, num_bins, leaf_cap, _p(nodes), _p(ids),
                         C.byref(nn), C.byref(npr))
+    return Tree(nodes[: nn.value], ids[: npr.value])
+
+
+def rebin_build(boxes: np.ndarray, leaf_threshold: int) -> Tree:
+    """Robust rebinning Morton tree (restates cuda/rebinMortonBuilder.h)."""
+    b = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    n = b.shape[0]
+    nodes = np.zeros(2 * n + 2, dtype=NODE_DTYPE)
+    ids = np.zeros(max(n, 1), dtype=np.uint32)
+    nn = C.c_int()
+    npr = C.c_int()
+    lib().orc_rebin_build(_p(b), n, leaf_threshold, _p(nodes), _p(ids), C.byref(nn), C.byref(npr))
     return Tree(nodes[: nn.value], ids[: npr.value])
 
 

@@ -759,6 +759,166 @@ int orc_radix_hierarchy(const uint64_t *keys, int n, int T, orc_node3f *nodes) {
 }
 
 // ---------------------------------------------------------------------------
+// robust rebinning Morton builder (cuda/rebinMortonBuilder.h)
+namespace {
+
+struct RebinCtx {
+  const orc_box3f *boxes;
+  int T;
+  std::vector<uint32_t> keys, ids;
+  std::vector<Topo> topo;
+};
+
+inline void center_of(const orc_box3f &b, float c[3]) {
+  for (int d = 0; d < 3; d++) c[d] = (b.lo[d] + b.hi[d]) / 2.f;  // box.h:146-147
+}
+
+// Quantizer<float,3> (rebinMortonBuilder.h:38-60) + interleaveBits
+// (:416-437): 10 bits per axis, bit i of x,y,z -> bits 3i,3i+1,3i+2.
+struct Quant30 {
+  float lo[3], scale;
+  void init(const float dlo[3], const float dhi[3]) {
+    for (int d = 0; d < 3; d++) lo[d] = dlo[d];
+    const float ext = std::max(std::max(dhi[0] - dlo[0], dhi[1] - dlo[1]), dhi[2] - dlo[2]);
+    scale = (float)(1u << 10) * (1.f / std::max(ext, 1e-20f));
+  }
+  uint32_t key(const float c[3]) const {
+    uint32_t cell[3];
+    for (int d = 0; d < 3; d++) {
+      const float f = (c[d] - lo[d]) * scale;
+      cell[d] = std::min((uint32_t)f, 1023u);
+    }
+    return (uint32_t)(spread3(cell[0], 10) | (spread3(cell[1], 10) << 1) | (spread3(cell[2], 10) << 2));
+  }
+};
+
+// centroid domain of ids[b,e); returns true when lower == upper (all axes)
+bool rebin_domain(const RebinCtx &c, uint32_t b, uint32_t e, Quant30 &q) {
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = b; i < e; i++) {
+    float ce[3];
+    center_of(c.boxes[c.ids[i]], ce);
+    for (int d = 0; d < 3; d++) { lo[d] = std::min(lo[d], ce[d]); hi[d] = std::max(hi[d], ce[d]); }
+  }
+  q.init(lo, hi);
+  return lo[0] == hi[0] && lo[1] == hi[1] && lo[2] == hi[2];
+}
+
+// findSplit (rebinMortonBuilder.h:595-622): no equal-keys fallback
+bool rebin_find_split(const std::vector<uint32_t> &keys, uint32_t b, uint32_t e, uint32_t &split) {
+  const uint32_t fk = keys[b], lk = keys[e - 1];
+  if (fk == lk) return false;
+  const int nm = __builtin_clz(fk ^ lk);
+  const uint32_t search = lk & (uint32_t)(0xffffffffull << (31 - nm));
+  uint32_t lo = b, hi = e;
+  while (hi > lo) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (keys[mid] < search) lo = mid + 1;
+    else hi = mid;
+  }
+  split = lo;
+  return true;
+}
+
+// Broad-phase rebin of one range (rebinMortonBuilder.h:1027-1063, 1305-1405):
+// new key = Morton code in the range's own centroid domain (local index if
+// that domain is a single point), then a stable sort by the new key.  If a
+// non-degenerate domain still gives one key for every prim (extent far
+// below 1e-20), the local index is used instead -- the reference would
+// rebin that range forever.
+void broad_rebin(RebinCtx &c, uint32_t b, uint32_t e) {
+  Quant30 q;
+  const bool degenerate = rebin_domain(c, b, e, q);
+  std::vector<std::pair<uint32_t, uint32_t>> kv(e - b);
+  for (uint32_t i = b; i < e; i++) {
+    float ce[3];
+    center_of(c.boxes[c.ids[i]], ce);
+    kv[i - b] = {degenerate ? i - b : q.key(ce), c.ids[i]};
+  }
+  bool allEqual = true;
+  for (auto &p : kv) allEqual = allEqual && p.first == kv[0].first;
+  if (allEqual)
+    for (uint32_t i = 0; i < e - b; i++) kv[i].first = i;
+  std::stable_sort(kv.begin(), kv.end(),
+                   [](const std::pair<uint32_t, uint32_t> &x, const std::pair<uint32_t, uint32_t> &y) {
+                     return x.first < y.first;
+                   });
+  for (uint32_t i = b; i < e; i++) { c.keys[i] = kv[i - b].first; c.ids[i] = kv[i - b].second; }
+}
+
+// Thread-phase rebin (rebinMortonBuilder.h:820-844): re-key in the range's
+// own domain (or key = i-begin), then bubbleSort (:758-771), an unstable
+// exchange sort whose exact swap sequence fixes the order of equal keys.
+void thread_rebin(RebinCtx &c, uint32_t b, uint32_t e) {
+  Quant30 q;
+  const bool degenerate = rebin_domain(c, b, e, q);
+  for (uint32_t i = b; i < e; i++) {
+    if (degenerate) {
+      c.keys[i] = i - b;
+    } else {
+      float ce[3];
+      center_of(c.boxes[c.ids[i]], ce);
+      c.keys[i] = q.key(ce);
+    }
+  }
+  for (uint32_t i = b; i < e; i++)
+    for (uint32_t j = i + 1; j < e; j++)
+      if (c.keys[j] < c.keys[i]) {
+        std::swap(c.keys[i], c.keys[j]);
+        std::swap(c.ids[i], c.ids[j]);
+      }
+}
+
+// A node's fate (broadPhaseCreateNodes :664-749, threadPhaseCreateNodes
+// :773-880): leaf if size <= T; ranges of <= 32 prims are finished in the
+// thread phase; larger ranges split by findSplit or are rebinned and
+// examined again.
+void rebin_rec(RebinCtx &c, int nodeID, uint32_t b, uint32_t e) {
+  const uint32_t n = e - b;
+  if ((int)n <= c.T) { c.topo[nodeID] = {(int)b, (int)n}; return; }
+  uint32_t split = 0;
+  if (n > 32) {
+    while (!rebin_find_split(c.keys, b, e, split)) broad_rebin(c, b, e);
+  } else if (!rebin_find_split(c.keys, b, e, split)) {
+    thread_rebin(c, b, e);
+    if (!rebin_find_split(c.keys, b, e, split)) { c.topo[nodeID] = {(int)b, (int)n}; return; }
+  }
+  const int child = (int)c.topo.size();
+  c.topo.push_back({0, 0});
+  c.topo.push_back({0, 0});
+  c.topo[nodeID] = {child, 0};
+  rebin_rec(c, child + 0, b, split);
+  rebin_rec(c, child + 1, split, e);
+}
+
+}  // namespace
+
+int orc_rebin_build(const orc_box3f *boxes, int n, int leafThreshold, orc_node3f *nodes,
+                    uint32_t *primIDs, int *numNodes, int *numPrims) {
+  RebinCtx c;
+  c.boxes = boxes;
+  c.T = std::max(1, leafThreshold);
+  std::vector<uint64_t> k64(std::max(n, 1));
+  std::vector<uint32_t> ids(std::max(n, 1));
+  const int nv = orc_morton_keys(boxes, n, 10, k64.data(), ids.data());
+  orc_sort_pairs(k64.data(), ids.data(), nv);
+  c.keys.resize(nv);
+  c.ids.assign(ids.begin(), ids.begin() + nv);
+  for (int i = 0; i < nv; i++) c.keys[i] = (uint32_t)k64[i];
+  c.topo.assign(1, Topo{0, 0});
+  if (nv > 0) rebin_rec(c, 0, 0, (uint32_t)nv);
+  for (int i = 0; i < nv; i++) primIDs[i] = c.ids[i];
+  for (size_t i = 0; i < c.topo.size(); i++) {
+    nodes[i].oc = pack_oc((uint64_t)c.topo[i].offset, (uint64_t)c.topo[i].count);
+    nodes[i].b = empty_box();
+  }
+  *numNodes = nv ? (int)c.topo.size() : 0;
+  *numPrims = nv;
+  if (nv) refit_rec(nodes, 0, primIDs, boxes);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
 // triangles (queries/triangles/findClosest.h:56-152)
 namespace {
 struct V3 { float x, y, z; };

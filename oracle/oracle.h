@@ -56,6 +56,19 @@ int orc_sah_build(const orc_box3f *boxes, int n, int leafThreshold, int numBins,
    split has both sides non-empty (the middle-split fallback). */
 int orc_sah_decide(const orc_box3f *boxes, int n, int numBins, int allowLeaf, int *bin);
 
+/* Robust rebinning Morton builder; cuda/rebinMortonBuilder.h:595-1476 with
+   NO_BLOCK_PHASE (its default, :22).  30-bit keys over the valid prims
+   (ascending primID before a stable sort), then per node: leaf if size <=
+   leafThreshold; a range of > 32 prims splits at findSplit (no equal-keys
+   fallback) or, when all its keys are equal, is re-keyed in its own
+   centroid domain (the local index if that domain is one point) and stably
+   re-sorted, then examined again; a range of <= 32 prims (thread phase)
+   re-keys in place and applies the reference's exchange sort, and becomes a
+   leaf if it still cannot split.  Numbering: root-first DFS, no unused
+   node 1. */
+int orc_rebin_build(const orc_box3f *boxes, int n, int leafThreshold, orc_node3f *nodes,
+                    uint32_t *primIDs, int *numNodes, int *numPrims);
+
 /* Bottom-up exact refit of node bounds from prim boxes (sm_builder.h:633-675,
    spatialMedian.h:117-134). Only nodes reachable from node 0 are written. */
 void orc_refit(orc_node3f *nodes, int numNodes, const uint32_t *primIDs,

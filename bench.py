@@ -164,6 +164,95 @@ def bench_batched(cq, dev, num_clouds: int, plain_budget_s: float, tune: bool):
     }
 
 
+def _time_cuda(fn, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_triangles(cq, dev, reps: int):
+    """1,310,720-triangle noisy icosphere: SAH build (32 bins, leaves <= 4),
+    4M closest-triangle queries forward and backward."""
+    import torch
+    from cubql_amd import synth
+    V, F = synth.noisy_icosphere(8, seed=4)
+    Q = synth.mesh_surface_queries(V, F, 4_000_000, seed=5)
+    dV, dF, dQ = (torch.from_numpy(V).to(dev), torch.from_numpy(F).to(dev),
+                  torch.from_numpy(Q).to(dev))
+    mem = cq.TorchMemory(dev)
+    cfg = cq.BuildConfig(makeLeafThreshold=4, maxAllowedLeafSize=4)
+    boxes = cq.triangles.boxes(dF, dV)
+    build_ms = _time_cuda(lambda: cq.cuda.sahBuilder(boxes, cfg, mem=mem, numBins=32).free(), reps)
+    cq.profile.reset()
+    cq.profile.enable(True)
+    cq.cuda.sahBuilder(boxes, cfg, mem=mem, numBins=32).free()
+    torch.cuda.synchronize()
+    cq.profile.enable(False)
+    kern = {}
+    for name in ["sah_bin", "sah_partition", "sah_small"]:
+        st = cq.profile.query(name)
+        if st and st["launches"]:
+            kern[name] = {"launches_per_build": st["launches"], "ms_per_build": st["ms"],
+                          "gbs": st["bytes"] / (st["ms"] * 1e-3) / 1e9}
+    bvh = cq.cuda.sahBuilder(boxes, cfg, mem=mem, numBins=32)
+    out = {}
+
+    def fwd():
+        out["r"] = cq.triangles.findClosest(bvh, dF, dV, dQ, want_points=False)
+    fwd_ms = _time_cuda(fwd, reps)
+    ids = out["r"][0]
+    g = torch.ones(len(Q), device=dev)
+    gq = torch.empty_like(dQ)
+    gv = torch.zeros_like(dV)
+    bwd_ms = _time_cuda(lambda: cq.triangles.backward(dF, dV, dQ, ids, g, gq, gv), reps)
+    res = {"triangles": int(len(F)), "queries": int(len(Q)), "sah_bins": 32, "leaf": 4,
+           "sah_build_ms": build_ms, "sah_build_mprims_s": len(F) / build_ms / 1e3,
+           "num_nodes": int(bvh.numNodes),
+           "fcp_ms": fwd_ms, "fcp_mqueries_s": len(Q) / fwd_ms / 1e3,
+           "backward_ms": bwd_ms, "fwd_bwd_mqueries_s": len(Q) / (fwd_ms + bwd_ms) / 1e3,
+           "sah_kernels": kern}
+    bvh.free()
+    return res
+
+
+def bench_mixture(cq, dev, reps: int):
+    """2M-point Gaussian mixture with 1% outliers at 1e6: rebinning Morton
+    build (leaves <= 8), kNN k=16 over 2M queries, mean-kNN backward."""
+    import torch
+    from cubql_amd import synth
+    P = torch.from_numpy(synth.gaussian_mixture(2_000_000, seed=41)).to(dev)
+    Q = torch.from_numpy(synth.gaussian_mixture(2_000_000, seed=42)).to(dev)
+    mem = cq.TorchMemory(dev)
+    cfg = cq.BuildConfig(makeLeafThreshold=8)
+    boxes = cq.points_to_boxes(P)
+    build_ms = _time_cuda(lambda: cq.cuda.rebinRadixBuilder(boxes, cfg, mem=mem).free(), reps)
+    bvh = cq.cuda.rebinRadixBuilder(boxes, cfg, mem=mem)
+    out = {}
+
+    def fwd():
+        out["r"] = cq.points.findKNN(bvh, P, Q, 16)
+    knn_ms = _time_cuda(fwd, reps)
+    ids = out["r"][0]
+    g = torch.full((len(Q), 16), 1.0 / (len(Q) * 16), device=dev)
+    gq = torch.empty_like(Q)
+    gp = torch.zeros_like(P)
+    bwd_ms = _time_cuda(lambda: cq.points_backward(P, Q, ids, g, gq, gp), reps)
+    res = {"points": int(len(P)), "queries": int(len(Q)), "k": 16, "leaf": 8,
+           "rebin_build_ms": build_ms, "rebin_build_mprims_s": len(P) / build_ms / 1e3,
+           "num_nodes": int(bvh.numNodes),
+           "knn_ms": knn_ms, "knn_mqueries_s": len(Q) / knn_ms / 1e3,
+           "backward_ms": bwd_ms, "fwd_bwd_mqueries_s": len(Q) / (knn_ms + bwd_ms) / 1e3}
+    bvh.free()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +267,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--plain-budget-s", type=float, default=40.0)
+    ap.add_argument("--no-extra", action="store_true", help="skip the triangle and mixture legs")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 5 device copies of 1 GiB (reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -192,6 +284,15 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.calibrate_copy:
+        # a plain copy of known size: each dispatch reads and writes 1 GiB
+        x = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_dispatch": x.numel() * 4, "dispatches": 5}))
+        return
     N = args.points
     P = synth.lcg_uniform_torch(2, N, device=dev)
     boxes = cq.points_to_boxes(P)
@@ -298,6 +399,17 @@ def main():
     print(f"[bench] fcp {nq/fcp_ms/1e3:.1f} Mq/s, knn {nk/knn_ms/1e3:.1f} Mq/s", file=sys.stderr,
           flush=True)
 
+    tris = mixture = None
+    if not args.no_extra:
+        tris = bench_triangles(cq, dev, reps=5)
+        print(f"[bench] triangles: SAH build {tris['sah_build_ms']:.2f} ms, fcp "
+              f"{tris['fcp_mqueries_s']:.1f} Mq/s, fwd+bwd {tris['fwd_bwd_mqueries_s']:.1f} Mq/s",
+              file=sys.stderr, flush=True)
+        mixture = bench_mixture(cq, dev, reps=5)
+        print(f"[bench] mixture: rebin build {mixture['rebin_build_ms']:.2f} ms, knn16 "
+              f"{mixture['knn_mqueries_s']:.1f} Mq/s, fwd+bwd {mixture['fwd_bwd_mqueries_s']:.1f} Mq/s",
+              file=sys.stderr, flush=True)
+
     batched = None
     if not args.no_batch:
         batched = bench_batched(cq, dev, args.batch_clouds, args.plain_budget_s, not args.no_tune)
@@ -329,6 +441,8 @@ def main():
         "radix_builds": radix,
         "fcp_mqueries_s": nq / fcp_ms / 1e3,
         "knn_mqueries_s": nk / knn_ms / 1e3,
+        "triangle_mesh": tris,
+        "gaussian_mixture": mixture,
         "batched_chamfer": batched,
         "kernels": kern,
         "roofline": roofline,

@@ -164,6 +164,20 @@ def bench_batched(cq, dev, num_clouds: int, plain_budget_s: float, tune: bool):
     }
 
 
+def partition_traffic():
+    """HBM bytes per sm::k_partition launch measured with rocprofv3 PMC
+    counters (FETCH_SIZE + WRITE_SIZE, scaled by a 1 GiB copy); the summary
+    with its commands lives in profiles/sm_partition_10M_pmc.md."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "sm_partition_10M_pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def _time_cuda(fn, reps):
     import torch
     fn()
@@ -338,7 +352,8 @@ def main():
     if dom:
         roofline = {"bound": "hbm", "kernel": "sm_partition",
                     "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                    "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": partition_traffic(),
+                    "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
                     "bytes_model": "32 B per prim of the level (16-byte item read + written)",
                     "avg_launch_ms": dom["avg_launch_ms"],
                     "bytes_per_launch": dom["bytes_per_launch"]}

@@ -15,15 +15,22 @@ __device__ inline uint32_t wave_incl_scan(uint32_t v) {
   return v;
 }
 
+template <bool LDS_ONLY>
+__device__ __forceinline__ void block_sync() {
+  if (LDS_ONLY) sync_block_lds();
+  else __syncthreads();
+}
+
 // Exclusive scan across a workgroup of BLOCK threads.  `smem` needs
 // BLOCK/64 + 1 words.  Returns the exclusive prefix; *total gets the sum.
-template <int BLOCK>
+// LDS_ONLY: barriers order LDS only (global stores stay in flight).
+template <int BLOCK, bool LDS_ONLY = false>
 __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *smem, uint32_t *total) {
   constexpr int NW = BLOCK / kWave;
   const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
   uint32_t incl = wave_incl_scan(v);
   if (lane == kWave - 1) smem[wid] = incl;
-  __syncthreads();
+  block_sync<LDS_ONLY>();
   if (threadIdx.x == 0) {
     uint32_t run = 0;
     for (int w = 0; w < NW; w++) {
@@ -33,10 +40,10 @@ __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *smem, uint32_t 
     }
     smem[NW] = run;
   }
-  __syncthreads();
+  block_sync<LDS_ONLY>();
   uint32_t r = smem[wid] + incl - v;
   *total = smem[NW];
-  __syncthreads();
+  block_sync<LDS_ONLY>();
   return r;
 }
 
@@ -44,7 +51,7 @@ __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *smem, uint32_t 
 // index k*BLOCK + t.  rank[k] = number of flagged items with a smaller tile
 // index.  smem needs ITEMS*BLOCK/64 words.  Returns the number of flagged
 // items in the tile.
-template <int BLOCK, int ITEMS>
+template <int BLOCK, int ITEMS, bool LDS_ONLY = false>
 __device__ inline uint32_t block_rank_striped(const bool (&flag)[ITEMS],
                                               uint32_t (&rank)[ITEMS],
                                               uint32_t *smem) {
@@ -58,7 +65,7 @@ __device__ inline uint32_t block_rank_striped(const bool (&flag)[ITEMS],
     m[k] = __ballot(flag[k]);
     if (lane == 0) smem[k * NW + wid] = (uint32_t)__popcll(m[k]);
   }
-  __syncthreads();
+  block_sync<LDS_ONLY>();
   if (wid == 0) {
     // each lane rewrites only the slot it read; smem[NC] is read after the
     // barrier below
@@ -67,11 +74,11 @@ __device__ inline uint32_t block_rank_striped(const bool (&flag)[ITEMS],
     if (lane < (uint32_t)NC) smem[lane] = incl - c;
     if (lane == (uint32_t)NC - 1) smem[NC] = incl;
   }
-  __syncthreads();
+  block_sync<LDS_ONLY>();
 #pragma unroll
   for (int k = 0; k < ITEMS; k++) rank[k] = smem[k * NW + wid] + mbcnt(m[k]);
   uint32_t tot = smem[NC];
-  __syncthreads();
+  block_sync<LDS_ONLY>();
   return tot;
 }
 

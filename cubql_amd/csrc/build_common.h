@@ -51,38 +51,56 @@ __device__ inline uint64_t status_word(uint32_t tag, uint32_t flag, uint32_t val
   return ((uint64_t)tag << 34) | ((uint64_t)flag << 32) | val;
 }
 
-// Decoupled look-back (one thread per tile).  Tile t is the k-th tile of its
-// node; tiles are claimed in ticket order, so every tile this one waits on is
-// held by a running workgroup that publishes without waiting on later tiles.
-// Publishes `local` and returns the sum over the node's earlier tiles.
-__device__ inline uint32_t lookback_exclusive(uint64_t *status, uint32_t t, uint32_t k, uint32_t tag,
-                                              uint32_t local) {
+// Decoupled look-back, wave-cooperative: all 64 lanes of one wave call it
+// for tile t, the k-th tile of its node.  Publishes `local`, then reads the
+// status words of up to 64 preceding tiles at once (lane i reads tile
+// t-1-i), sums them up to the nearest tile whose inclusive prefix is known
+// and repeats further back if none is.  Tiles are claimed in ticket order,
+// so every tile waited on is held by a running workgroup that publishes
+// without waiting on later tiles.  Returns the node's sum over its earlier
+// tiles (in every lane).
+__device__ inline uint32_t lookback_exclusive_wave(uint64_t *status, uint32_t t, uint32_t k, uint32_t tag,
+                                                   uint32_t local) {
+  const uint32_t lane = lane_id();
   if (k == 0) {
-    __hip_atomic_store(&status[t], status_word(tag, 2, local), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane == 0)
+      __hip_atomic_store(&status[t], status_word(tag, 2, local), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return 0;
   }
-  __hip_atomic_store(&status[t], status_word(tag, 1, local), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (lane == 0)
+    __hip_atomic_store(&status[t], status_word(tag, 1, local), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   uint32_t prefix = 0;
-  uint32_t j = t - 1;
+  uint32_t first = t - 1;  // tile read by lane 0
+  uint32_t remaining = k;  // earlier tiles of this node: t-k .. t-1
   while (true) {
-    const uint64_t w = __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t wt = (uint32_t)(w >> 34);
-    const uint32_t wf = (uint32_t)(w >> 32) & 3u;
-    if (wt != tag || wf == 0) {
+    const bool inRange = lane < remaining;
+    uint64_t w = 0;
+    if (inRange) w = __hip_atomic_load(&status[first - lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t flag = ((uint32_t)(w >> 34) == tag) ? (uint32_t)(w >> 32) & 3u : 0u;
+    const uint64_t incl = __ballot(inRange && flag == 2);
+    const uint64_t notReady = __ballot(inRange && flag == 0);
+    const uint32_t stop = incl ? (uint32_t)__builtin_ctzll(incl) : 63u;  // lanes 0..stop count
+    const uint64_t upto = stop >= 63 ? ~0ull : ((1ull << (stop + 1)) - 1);
+    if (notReady & upto) {
       __builtin_amdgcn_s_sleep(1);
       continue;
     }
-    prefix += (uint32_t)w;
-    if (wf == 2) break;
-    j--;
+    uint32_t v = (inRange && lane <= stop) ? (uint32_t)w : 0u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    prefix += v;
+    if (incl) break;
+    first -= 64;
+    remaining -= 64;
   }
-  __hip_atomic_store(&status[t], status_word(tag, 2, prefix + local), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
+  if (lane == 0)
+    __hip_atomic_store(&status[t], status_word(tag, 2, prefix + local), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   return prefix;
 }
 
 // block-wide min/max reduction of 12 floats (left lo/hi, right lo/hi)
-template <int BLOCK>
+template <int BLOCK, bool LDS_ONLY = false>
 __device__ inline void block_reduce_bounds(float (&v)[12], float *smem /*12*BLOCK/64*/) {
   constexpr int NW = BLOCK / kWave;
   const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
@@ -95,7 +113,7 @@ __device__ inline void block_reduce_bounds(float (&v)[12], float *smem /*12*BLOC
 #pragma unroll
     for (int j = 0; j < 12; j++) smem[j * NW + wid] = v[j];
   }
-  __syncthreads();
+  block_sync<LDS_ONLY>();
   if (threadIdx.x < 12) {
     int j = threadIdx.x;
     bool isMin = (j % 6) < 3;
@@ -103,10 +121,10 @@ __device__ inline void block_reduce_bounds(float (&v)[12], float *smem /*12*BLOC
     for (int w = 1; w < NW; w++) r = isMin ? fminf(r, smem[j * NW + w]) : fmaxf(r, smem[j * NW + w]);
     smem[j * NW] = r;
   }
-  __syncthreads();
+  block_sync<LDS_ONLY>();
 #pragma unroll
   for (int j = 0; j < 12; j++) v[j] = smem[j * NW];
-  __syncthreads();
+  block_sync<LDS_ONLY>();
 }
 
 }  // namespace cubql

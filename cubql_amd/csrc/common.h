@@ -125,6 +125,23 @@ __device__ inline uint32_t mbcnt(uint64_t m) {
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Workgroup barrier that orders LDS only.  __syncthreads() also waits for
+// every outstanding global access of the calling wave (s_waitcnt vmcnt(0)
+// before the barrier), which stalls loops that stream stores; use this where
+// threads communicate through LDS alone.
+__device__ __forceinline__ void sync_block_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+// The same for a one-wave workgroup: a wave's LDS accesses execute in
+// program order, so only the compiler must not reorder them.
+__device__ __forceinline__ void sync_wave_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __host__ __device__ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 // device-side fill (used instead of hipMemsetAsync so the sequence stays

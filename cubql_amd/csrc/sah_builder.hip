@@ -438,7 +438,7 @@ __global__ __launch_bounds__(kBlock) void k_bin(Ctx c, int level) {
   for (uint32_t bt = blockIdx.x; bt < nbt; bt += gridDim.x) {
     if (threadIdx.x == 0) sSeg = find_seg(c.binStart, nSeg, bt);
     for (int w = threadIdx.x; w < W; w += kBlock) sBins[w] = bin_init(w);
-    __syncthreads();
+    sync_block_lds();
     const uint32_t s = sSeg;
     const Seg g = c.segs[par][s];
     float clo[3], chi[3];
@@ -446,10 +446,10 @@ __global__ __launch_bounds__(kBlock) void k_bin(Ctx c, int level) {
     const uint32_t b0 = g.begin + (bt - c.binStart[s]) * kBinTile;
     const uint32_t e0 = min(b0 + (uint32_t)kBinTile, g.end);
     for (uint32_t i = b0 + threadIdx.x; i < e0; i += kBlock) bin_add(sBins, nb, src[i], clo, chi);
-    __syncthreads();
+    sync_block_lds();
     int32_t *out = c.partial + (size_t)bt * W;
     for (int w = threadIdx.x; w < W; w += kBlock) out[w] = sBins[w];
-    __syncthreads();
+    sync_block_lds();
   }
 }
 
@@ -473,7 +473,7 @@ __global__ __launch_bounds__(kBlock) void k_select(Ctx c, int level) {
       }
       sBins[w] = acc;
     }
-    __syncthreads();
+    sync_block_lds();
     if (threadIdx.x < kWave) {
       const Seg g = c.segs[par][s];
       const uint32_t n = g.end - g.begin;
@@ -497,7 +497,7 @@ __global__ __launch_bounds__(kBlock) void k_select(Ctx c, int level) {
       }
       store_node_wave(&c.nodes[g.node], lo, hi, mode == kModeLeaf ? pack_oc(g.begin, n) : 0ull);
     }
-    __syncthreads();
+    sync_block_lds();
   }
 }
 
@@ -521,7 +521,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
       sTile = t;
       if (t < numTiles) sSeg = find_seg(c.tileStart, nSeg, t);
     }
-    __syncthreads();
+    sync_block_lds();
     const uint32_t t = sTile;
     if (t >= numTiles) break;
     const uint32_t s = sSeg;
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
     const uint32_t te = min(tb + (uint32_t)kTile, g.end);
     if (mode == kModeLeaf) {
       for (uint32_t i = tb + threadIdx.x; i < te; i += kBlock) c.primIDs[i] = item_id(src[i]);
-      __syncthreads();
+      sync_block_lds();
       continue;
     }
     const int bin = c.segBin[s];
@@ -553,13 +553,15 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
       }
     }
     uint32_t rank[kItems];
-    const uint32_t nLeft = block_rank_striped<kBlock, kItems>(left, rank, sr);
-    if (threadIdx.x == 0) {
-      const uint32_t prefix = lookback_exclusive(c.tileStatus, t, k, tag, nLeft);
-      sPrefix = prefix;
-      if (k == nTilesSeg - 1) c.segLeft[s] = prefix + nLeft;
+    const uint32_t nLeft = block_rank_striped<kBlock, kItems, true>(left, rank, sr);
+    if (threadIdx.x < kWave) {
+      const uint32_t prefix = lookback_exclusive_wave(c.tileStatus, t, k, tag, nLeft);
+      if (threadIdx.x == 0) {
+        sPrefix = prefix;
+        if (k == nTilesSeg - 1) c.segLeft[s] = prefix + nLeft;
+      }
     }
-    __syncthreads();
+    sync_block_lds();
     const uint32_t P = sPrefix;
     const uint32_t rightsBefore = k * kTile - P;
     float v[12];
@@ -577,7 +579,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
         v[o + 3 + q] = fmaxf(v[o + 3 + q], ce);
       }
     }
-    block_reduce_bounds<kBlock>(v, sb);
+    block_reduce_bounds<kBlock, true>(v, sb);
     if (threadIdx.x < 12) {
       const int side = threadIdx.x / 6;
       const int j = threadIdx.x % 6;
@@ -588,7 +590,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
         else atomicMax(dcb, enc_f(v[threadIdx.x]));
       }
     }
-    __syncthreads();
+    sync_block_lds();
   }
 }
 
@@ -691,7 +693,7 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
   for (uint32_t i = lane; i < m; i += kWave) sIt[i] = gItems[gb + i];
   int sp = 1;
   if (lane == 0) { sFr[0] = m << 16; sMe[0] = 0; }
-  __syncthreads();
+  sync_wave_lds();
   while (sp > 0) {
     --sp;
     const uint32_t f = sFr[sp];
@@ -717,15 +719,15 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
     int mode = kModeLeaf, bin = 0;
     if (n > c.T) {
       for (int w = lane; w < 3 * nb * kBinWords; w += kWave) sBins[w] = bin_init(w);
-      __syncthreads();
+      sync_wave_lds();
       for (uint32_t i = b + lane; i < e; i += kWave) bin_add(sBins, nb, sIt[i], clo, chi);
-      __syncthreads();
+      sync_wave_lds();
       mode = sah_decide_wave(sBins, nb, n, c.cap, bin);
     }
     if (mode == kModeLeaf) {
       for (uint32_t i = b + lane; i < e; i += kWave) c.primIDs[gb + i] = item_id(sIt[i]);
       store_node_wave(loc + me, blo, bhi, pack_oc(gb + b, n));
-      __syncthreads();
+      sync_wave_lds();
       continue;
     }
     const float lo = mode >= 0 ? clo[mode] : 0.f, hi = mode >= 0 ? chi[mode] : 0.f;
@@ -745,7 +747,7 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
       nL += (uint32_t)__popcll(mL);
       nR += (uint32_t)__popcll(mR);
     }
-    __syncthreads();
+    sync_wave_lds();
     for (uint32_t i = b + lane; i < e; i += kWave) sIt[i] = sAlt[i];
     const uint32_t mid = b + nL;
     const uint32_t child = nLocal;
@@ -758,7 +760,7 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
       sMe[sp + 1] = child;
     }
     sp += 2;
-    __syncthreads();
+    sync_wave_lds();
   }
   if (lane == 0) c.smallCount[segIdx] = nLocal;
 }
@@ -772,7 +774,7 @@ __global__ __launch_bounds__(kWave) void k_small(Ctx c) {
   const uint32_t nSmall = c.ctr->numSmall;
   for (uint32_t w = blockIdx.x; w < nSmall; w += gridDim.x) {
     small_segment(c, c.small[w], w, sIt, sAlt, sBins, sFr, sMe);
-    __syncthreads();
+    sync_wave_lds();
   }
 }
 

@@ -297,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
         sSeg = lo;
       }
     }
-    __syncthreads();
+    sync_block_lds();
     const uint32_t t = sTile;
     if (t >= numTiles) break;
     const uint32_t s = sSeg;
@@ -323,15 +323,17 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
       }
     }
     uint32_t rank[kItems];
-    const uint32_t nLeft = block_rank_striped<kBlock, kItems>(left, rank, sr);
+    const uint32_t nLeft = block_rank_striped<kBlock, kItems, true>(left, rank, sr);
 
     // decoupled look-back over the tiles of this node
-    if (threadIdx.x == 0) {
-      const uint32_t prefix = lookback_exclusive(c.tileStatus, t, k, tag, nLeft);
-      sPrefix = prefix;
-      if (k == nTilesSeg - 1) c.segLeft[s] = prefix + nLeft;
+    if (threadIdx.x < kWave) {
+      const uint32_t prefix = lookback_exclusive_wave(c.tileStatus, t, k, tag, nLeft);
+      if (threadIdx.x == 0) {
+        sPrefix = prefix;
+        if (k == nTilesSeg - 1) c.segLeft[s] = prefix + nLeft;
+      }
     }
-    __syncthreads();
+    sync_block_lds();
     const uint32_t P = sPrefix;
     const uint32_t rightsBefore = k * kTile - P;
 
@@ -353,7 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
       v[o + 4] = fmaxf(v[o + 4], it[j].y);
       v[o + 5] = fmaxf(v[o + 5], it[j].z);
     }
-    block_reduce_bounds<kBlock>(v, sb);
+    block_reduce_bounds<kBlock, true>(v, sb);
     if (threadIdx.x < 12) {
       int side = threadIdx.x / 6;
       int j = threadIdx.x % 6;
@@ -365,7 +367,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
         else atomicMax(dstCb, enc_f(val));
       }
     }
-    __syncthreads();
+    sync_block_lds();
   }
 }
 
@@ -500,7 +502,7 @@ __device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, 
   const uint32_t T = c.T;
   int sp = 1;
   if (lane == 0) sFr[0] = fr_pack(0, cnt, rootLocal);
-  __syncthreads();
+  sync_wave_lds();
   while (sp > 0) {
     const uint32_t f = sFr[--sp];
     const uint32_t b = f & 0xffu, e = (f >> 8) & 0xffu, me = f >> 16;
@@ -557,7 +559,7 @@ __device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, 
       sFr[sp + 1] = fr_pack(b, mid, child);
     }
     sp += 2;
-    __syncthreads();
+    sync_wave_lds();
   }
 }
 
@@ -599,7 +601,7 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
     int sp = 1;
     if (lane == 0) sFr1[0] = (0u) | (m << 16);
     if (lane == 0) sMe[0] = 0;
-    __syncthreads();
+    sync_wave_lds();
     while (sp > 0) {
       --sp;
       const uint32_t f = sFr1[sp];
@@ -656,7 +658,7 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
       uint32_t mid;
       if (nL != 0 && nR != 0) mid = b + nL;
       else mid = (uint32_t)(((uint64_t)gb + b + gb + e) / 2 - gb);  // spatialMedian.h:107
-      __syncthreads();
+      sync_wave_lds();
       for (uint32_t i = b + lane; i < e; i += kWave) sIt[i] = sAlt[i];
       const uint32_t child = nLocal;
       nLocal += 2;
@@ -668,7 +670,7 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
         sMe[sp + 1] = child;
       }
       sp += 2;
-      __syncthreads();
+      sync_wave_lds();
     }
   }
   if (lane == 0) c.smallCount[segIdx] = nLocal;
@@ -686,7 +688,7 @@ __global__ __launch_bounds__(kWave) void k_small(Ctx c) {
     const SmallSeg sg = c.small[w];
     if (pts) small_segment<true>(c, sg, w, sIt, sAlt, sFr, sFr1, sMe);
     else small_segment<false>(c, sg, w, sIt, sAlt, sFr, sFr1, sMe);
-    __syncthreads();
+    sync_wave_lds();
   }
 }
 

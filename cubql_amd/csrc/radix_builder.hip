@@ -18,6 +18,7 @@
 // (10 bits per axis) are its radix.h:34-38 variant.
 #include "block_ops.h"
 #include "build_common.h"
+#include "wave_ops.h"
 #include "common.h"
 #include "profile.h"
 #include "radix_sort.h"
@@ -59,6 +60,7 @@ struct RCtx {
   // rebin builder
   uint32_t *jobs;      // nodes whose keys are all equal (> 32 prims)
   uint32_t *jobStart;  // exclusive prefix of job sizes
+  int32_t *jobDom;     // per job: encoded centroid bounds
   float *jobQuant;     // per job: lo xyz, scale
   uint32_t *jobLocal;  // per job: 1 = key by local index
   uint32_t *small;     // thread-phase nodes (<= 32 prims)
@@ -484,6 +486,10 @@ __global__ __launch_bounds__(1024) void k_rebin_setup(RCtx c, int level) {
       const uint2 r = c.ranges[c.jobs[j]];
       size = r.y - r.x;
       c.levels[start + j] = c.jobs[j];
+      for (int d = 0; d < 3; d++) {
+        c.jobDom[6 * (size_t)j + d] = enc_f(INFINITY);
+        c.jobDom[6 * (size_t)j + 3 + d] = enc_f(-INFINITY);
+      }
     }
     uint32_t tot;
     const uint32_t ex = block_excl_scan<1024>(size, ss, &tot);
@@ -510,37 +516,6 @@ __device__ __forceinline__ void quant_init(const float (&lo)[3], const float (&h
   q[3] = 1024.f * (1.f / fmaxf(ext, 1e-20f));
 }
 
-// Per job (one workgroup each): centroid domain, quantizer, and whether to
-// key by local index: the domain is one point (rebinCreateNewKeys, :1052-
-// 1055), or all its prims fall in one cell (the cells of the domain's lower
-// and upper corner agree on every axis), which the reference would rebin
-// forever.
-__global__ __launch_bounds__(kBlock) void k_rebin_domain(RCtx c) {
-  __shared__ float sb[12 * (kBlock / kWave)];
-  const uint32_t J = c.ctr[C_CURJOBS];
-  for (uint32_t j = blockIdx.x; j < J; j += gridDim.x) {
-    const uint2 r = c.ranges[c.jobs[j]];
-    float v[12];
-    for (int d = 0; d < 3; d++) { v[d] = INFINITY; v[d + 3] = -INFINITY; v[d + 6] = INFINITY; v[d + 9] = -INFINITY; }
-    for (uint32_t i = r.x + threadIdx.x; i < r.y; i += kBlock) {
-      float ce[3];
-      center3(load_box(c.boxes, c.ids[i]), ce);
-      for (int d = 0; d < 3; d++) { v[d] = fminf(v[d], ce[d]); v[d + 3] = fmaxf(v[d + 3], ce[d]); }
-    }
-    block_reduce_bounds<kBlock>(v, sb);
-    if (threadIdx.x == 0) {
-      const float lo[3] = {v[0], v[1], v[2]}, hi[3] = {v[3], v[4], v[5]};
-      float q[4];
-      quant_init(lo, hi, q);
-      const bool point = lo[0] == hi[0] && lo[1] == hi[1] && lo[2] == hi[2];
-      const bool oneCell = morton<uint32_t>(lo, q, 10) == morton<uint32_t>(hi, q, 10);
-      for (int k = 0; k < 4; k++) c.jobQuant[4 * j + k] = q[k];
-      c.jobLocal[j] = (point || oneCell) ? 1u : 0u;
-    }
-    __syncthreads();
-  }
-}
-
 __device__ __forceinline__ uint32_t find_job(const uint32_t *start, uint32_t J, uint32_t t) {
   uint32_t lo = 0, hi = J;  // last j with start[j] <= t
   while (hi - lo > 1) {
@@ -549,6 +524,67 @@ __device__ __forceinline__ uint32_t find_job(const uint32_t *start, uint32_t J, 
     else hi = mid;
   }
   return lo;
+}
+
+// Centroid domain of every job, over all rebin prims at once: lanes of a
+// wave whose prims all belong to one job (the common case: jobs are
+// contiguous and large) reduce in registers and issue one atomic per bound;
+// mixed waves fall back to per-lane atomics.  jobDom holds encoded bounds.
+__global__ __launch_bounds__(kBlock) void k_rebin_grow(RCtx c) {
+  const uint32_t J = c.ctr[C_CURJOBS];
+  const uint32_t P = c.ctr[C_JOBPRIMS];
+  const uint32_t lane = lane_id();
+  const uint32_t stride = gridDim.x * kBlock;
+  for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~(uint32_t)(kWave - 1)); base < P; base += stride) {
+    const uint32_t t = base + lane;
+    const bool active = t < P;
+    uint32_t j = 0;
+    float ce[3] = {0.f, 0.f, 0.f};
+    if (active) {
+      j = find_job(c.jobStart, J, t);
+      center3(load_box(c.boxes, c.ids[c.ranges[c.jobs[j]].x + (t - c.jobStart[j])]), ce);
+    }
+    const uint32_t j0 = __builtin_amdgcn_readfirstlane(j);  // lane 0 is active when base < P
+    int32_t *dom = c.jobDom + 6 * (size_t)j;
+    if (__ballot(active && j != j0) == 0) {
+      float v[6];
+      for (int d = 0; d < 3; d++) {
+        v[d] = wave_min_dpp(active ? ce[d] : INFINITY);
+        v[d + 3] = wave_max_dpp(active ? ce[d] : -INFINITY);
+      }
+      if (lane < 6) {
+        const float x = lane == 0 ? v[0] : lane == 1 ? v[1] : lane == 2 ? v[2] : lane == 3 ? v[3] : lane == 4 ? v[4] : v[5];
+        if (lane < 3) atomicMin(&c.jobDom[6 * (size_t)j0 + lane], enc_f(x));
+        else atomicMax(&c.jobDom[6 * (size_t)j0 + lane], enc_f(x));
+      }
+    } else if (active) {
+      for (int d = 0; d < 3; d++) {
+        atomicMin(&dom[d], enc_f(ce[d]));
+        atomicMax(&dom[d + 3], enc_f(ce[d]));
+      }
+    }
+  }
+}
+
+// Per job: quantizer of its domain and whether to key by local index: the
+// domain is one point (rebinCreateNewKeys, :1052-1055), or all its prims fall
+// in one cell (the cells of the domain's lower and upper corner agree on
+// every axis), which the reference would rebin forever.
+__global__ void k_rebin_finalize(RCtx c) {
+  const uint32_t J = c.ctr[C_CURJOBS];
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < J; j += gridDim.x * blockDim.x) {
+    float lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+      lo[d] = dec_f(c.jobDom[6 * (size_t)j + d]);
+      hi[d] = dec_f(c.jobDom[6 * (size_t)j + 3 + d]);
+    }
+    float q[4];
+    quant_init(lo, hi, q);
+    const bool point = lo[0] == hi[0] && lo[1] == hi[1] && lo[2] == hi[2];
+    const bool oneCell = morton<uint32_t>(lo, q, 10) == morton<uint32_t>(hi, q, 10);
+    for (int k = 0; k < 4; k++) c.jobQuant[4 * j + k] = q[k];
+    c.jobLocal[j] = (point || oneCell) ? 1u : 0u;
+  }
 }
 
 // New (job:key) pairs, one per prim of every job, in position order.
@@ -729,6 +765,7 @@ int rebin_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t
   CUBQL_CHECK(scr.alloc(&c.levels, 2 * (size_t)n + 2 + 64 * (size_t)maxJobs));
   CUBQL_CHECK(scr.alloc(&c.jobs, maxJobs));
   CUBQL_CHECK(scr.alloc(&c.jobStart, maxJobs + 1));
+  CUBQL_CHECK(scr.alloc(&c.jobDom, 6 * (size_t)maxJobs));
   CUBQL_CHECK(scr.alloc(&c.jobQuant, 4 * (size_t)maxJobs));
   CUBQL_CHECK(scr.alloc(&c.jobLocal, maxJobs));
   CUBQL_CHECK(scr.alloc(&c.small, (size_t)n + 1));
@@ -791,7 +828,8 @@ int rebin_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t
     rounds++;
     const uint32_t J = hc[C_JOBS];
     k_rebin_setup<<<1, 1024, 0, s>>>(c, L);
-    k_rebin_domain<<<std::min<uint32_t>(J, (uint32_t)numCU * 8), kBlock, 0, s>>>(c);
+    k_rebin_grow<<<grid, kBlock, 0, s>>>(c);
+    k_rebin_finalize<<<std::max<uint32_t>(1u, std::min<uint32_t>(div_up(J, kBlock), 1024u)), kBlock, 0, s>>>(c);
     k_rebin_keys<<<grid, kBlock, 0, s>>>(c, rkA, rvA);
     uint64_t *rkRes = nullptr;
     uint32_t *rvRes = nullptr;

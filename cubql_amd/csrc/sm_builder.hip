@@ -49,7 +49,7 @@ struct Counters {
   uint32_t ticket;
   uint32_t numTiles;
   uint32_t numTop;    // entries in the top-node list
-  uint32_t pad[3];
+  uint32_t baseNode, baseSmall, baseTop;  // this level's bases (emit)
 };
 
 struct Seg {          // an open node with > kSmall prims
@@ -76,12 +76,18 @@ struct Ctx {
   Node *loc;               // small phase: local nodes of segment s at loc[2*begin_s + j]
   uint32_t *smallCount;    // small phase: local node count per small segment
   uint32_t *smallBase;     // small phase: first global id of local node 1
-  Seg *segs[2];            // per-level large segments (ping-pong)
-  int32_t *childCb;        // [2*maxLarge][6]
-  uint32_t *segLeft;       // [maxLarge]
-  uint32_t *tileStart;     // [maxLarge+1]
-  int32_t *segDim;         // [maxLarge]
-  float *segPos;           // [maxLarge]
+  // per-level arrays, double-buffered by level parity: the emit of level l
+  // reads the [l&1] copies and prepares the [(l+1)&1] ones
+  Seg *segs[2];            // large segments
+  int32_t *childCb[2];     // [2*maxLarge][6] child centroid bounds
+  uint32_t *segLeft[2];    // [maxLarge] left-side count after the partition
+  uint32_t *tileStart[2];  // [maxLarge+1] first partition tile of each segment
+  uint32_t *tileSeg[2];    // [maxTiles] segment of each partition tile
+  int32_t *segDim[2];      // [maxLarge] split plane
+  float *segPos[2];
+  uint4 *emitOff;          // [maxLarge] in-block offsets (large|small|fix, tiles, -, -)
+  uint4 *blockSum;         // [emit blocks] block totals (large|small|fix, tiles, items, -)
+  uint4 *blockOff;         // [emit blocks] exclusive prefix of blockSum
   uint64_t *tileStatus;    // [maxTiles]
   SmallSeg *small;         // [numValid]
   uint32_t *fixup;         // [maxLarge] indices into next-level segs
@@ -175,18 +181,38 @@ __global__ __launch_bounds__(1024) void k_count_scan(Ctx c, uint32_t numBlocks) 
     for (int d = 0; d < 3; d++) { z.lo[d] = 0.f; z.hi[d] = 0.f; }
     z.oc = 0;
     c.nodes[1] = z;  // node 1 is never used (bvh.h:66-70)
+    k->numTiles = 0;
+    k->ticket = 0;
     if (run > kSmall && run > c.T) {
       Seg s;
       s.begin = 0; s.end = run; s.node = 0; s.pad = 0; s.pad2[0] = s.pad2[1] = 0;
       for (int j = 0; j < 6; j++) s.cb[j] = k->rootCb[j];
       c.segs[0][0] = s;
       k->numLarge[0] = 1;
+      // plan of level 0
+      float lo[3], hi[3];
+      for (int d = 0; d < 3; d++) { lo[d] = dec_f(s.cb[d]); hi[d] = dec_f(s.cb[d + 3]); }
+      int dim;
+      float pos;
+      choose_split(lo, hi, dim, pos);
+      c.segDim[0][0] = dim;
+      c.segPos[0][0] = pos;
+      c.segLeft[0][0] = 0;
+      for (int j = 0; j < 12; j++) c.childCb[0][j] = enc_f((j % 6) < 3 ? INFINITY : -INFINITY);
+      const uint32_t tiles = div_up(run, kTile);
+      c.tileStart[0][0] = 0;
+      c.tileStart[0][1] = tiles;
+      k->numTiles = tiles;
+      c.levelItems[0] = run;
     } else if (run > 0) {
       SmallSeg s{0, run, 0, 0};
       c.small[0] = s;
       k->numSmall = 1;
     }
   }
+  __syncthreads();
+  const uint32_t tiles = c.ctr->numTiles;
+  for (uint32_t t = threadIdx.x; t < tiles; t += 1024) c.tileSeg[0][t] = 0;
 }
 
 // Pass 3: write compacted items (ascending primID) and the invalid tail.
@@ -222,51 +248,6 @@ __global__ __launch_bounds__(kBlock) void k_write_items(Ctx c) {
 }
 
 // ------------------------------------------------------------- level pass
-// Plan (single workgroup): split planes, tile offsets, child-bound reset.
-__global__ __launch_bounds__(1024) void k_plan(Ctx c, int level) {
-  __shared__ uint32_t ss[1024 / kWave + 1];
-  const int par = level & 1;
-  const uint32_t n = c.ctr->numLarge[par];
-  const Seg *segs = c.segs[par];
-  uint32_t run = 0, items = 0;
-  for (uint32_t b0 = 0; b0 < n; b0 += 1024) {
-    uint32_t s = b0 + threadIdx.x;
-    uint32_t tiles = 0, size = 0;
-    if (s < n) {
-      Seg g = segs[s];
-      size = g.end - g.begin;
-      tiles = div_up(size, kTile);
-      float lo[3], hi[3];
-      for (int d = 0; d < 3; d++) { lo[d] = dec_f(g.cb[d]); hi[d] = dec_f(g.cb[d + 3]); }
-      int dim;
-      float pos;
-      choose_split(lo, hi, dim, pos);
-      c.segDim[s] = dim;
-      c.segPos[s] = pos;
-      c.segLeft[s] = 0;
-      for (int j = 0; j < 3; j++) {
-        c.childCb[(2 * s) * 6 + j] = enc_f(INFINITY);
-        c.childCb[(2 * s) * 6 + j + 3] = enc_f(-INFINITY);
-        c.childCb[(2 * s + 1) * 6 + j] = enc_f(INFINITY);
-        c.childCb[(2 * s + 1) * 6 + j + 3] = enc_f(-INFINITY);
-      }
-    }
-    uint32_t tot, totItems;
-    uint32_t ex = block_excl_scan<1024>(tiles, ss, &tot);
-    block_excl_scan<1024>(size, ss, &totItems);
-    if (s < n) c.tileStart[s] = run + ex;
-    run += tot;
-    items += totItems;
-  }
-  if (threadIdx.x == 0) {
-    c.tileStart[n] = run;
-    c.ctr->numTiles = run;
-    c.ctr->ticket = 0;
-    c.levelItems[level] = items;
-  }
-}
-
-
 // Partition: each workgroup repeatedly claims a tile (ticket order, so a
 // tile only ever waits on tiles already claimed by running workgroups),
 // computes left/right ranks, resolves its prefix by decoupled look-back
@@ -277,7 +258,6 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
   __shared__ uint32_t sTile, sSeg, sPrefix;
   const int par = level & 1;
   const uint32_t numTiles = c.ctr->numTiles;
-  const uint32_t nSeg = c.ctr->numLarge[par];
   const Seg *segs = c.segs[par];
   const float4 *src = c.items[par];
   float4 *dst = c.items[par ^ 1];
@@ -286,26 +266,18 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
     if (threadIdx.x == 0) {
       uint32_t t = atomicAdd(&c.ctr->ticket, 1u);
       sTile = t;
-      if (t < numTiles) {
-        // segment = last s with tileStart[s] <= t
-        uint32_t lo = 0, hi = nSeg;
-        while (hi - lo > 1) {
-          uint32_t mid = (lo + hi) / 2;
-          if (c.tileStart[mid] <= t) lo = mid;
-          else hi = mid;
-        }
-        sSeg = lo;
-      }
+      if (t < numTiles) sSeg = c.tileSeg[par][t];
     }
     sync_block_lds();
     const uint32_t t = sTile;
     if (t >= numTiles) break;
     const uint32_t s = sSeg;
     const Seg g = segs[s];
-    const int dim = c.segDim[s];
-    const float pos = c.segPos[s];
-    const uint32_t k = t - c.tileStart[s];
-    const uint32_t nTilesSeg = c.tileStart[s + 1] - c.tileStart[s];
+    const int dim = c.segDim[par][s];
+    const float pos = c.segPos[par][s];
+    const uint32_t ts = c.tileStart[par][s];
+    const uint32_t k = t - ts;
+    const uint32_t nTilesSeg = c.tileStart[par][s + 1] - ts;
     const uint32_t tb = g.begin + k * kTile;
     const uint32_t te = min(tb + (uint32_t)kTile, g.end);
 
@@ -330,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
       const uint32_t prefix = lookback_exclusive_wave(c.tileStatus, t, k, tag, nLeft);
       if (threadIdx.x == 0) {
         sPrefix = prefix;
-        if (k == nTilesSeg - 1) c.segLeft[s] = prefix + nLeft;
+        if (k == nTilesSeg - 1) c.segLeft[par][s] = prefix + nLeft;
       }
     }
     sync_block_lds();
@@ -362,7 +334,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
       float val = v[threadIdx.x];
       bool nonEmpty = side == 0 ? (nLeft > 0) : (nLeft < te - tb);
       if (nonEmpty) {
-        int32_t *dstCb = &c.childCb[(2 * s + side) * 6 + j];
+        int32_t *dstCb = &c.childCb[par][(2 * s + side) * 6 + j];
         if (j < 3) atomicMin(dstCb, enc_f(val));
         else atomicMax(dstCb, enc_f(val));
       }
@@ -371,91 +343,159 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
   }
 }
 
-// Emit (single workgroup): node records for the split nodes, deterministic
-// next-level / small lists, fixups for one-sided (degenerate) splits.
-__global__ __launch_bounds__(1024) void k_emit(Ctx c, int level) {
-  __shared__ uint32_t ss[1024 / kWave + 1];
-  __shared__ uint32_t sBase, sSmall, sTop;
-  const int par = level & 1;
-  const uint32_t n = c.ctr->numLarge[par];
-  const Seg *segs = c.segs[par];
-  Seg *next = c.segs[par ^ 1];
-  if (threadIdx.x == 0) {
-    sBase = c.ctr->nodeCount;
-    sSmall = c.ctr->numSmall;
-    sTop = c.ctr->numTop;
-  }
-  __syncthreads();
-  const uint32_t nodeBase = sBase, smallBase = sSmall, topBase = sTop;
-  uint32_t runLarge = 0, runSmall = 0, runFix = 0;
-  for (uint32_t b0 = 0; b0 < n; b0 += 1024) {
-    uint32_t s = b0 + threadIdx.x;
-    uint32_t nLarge = 0, nSmallC = 0, nFix = 0;
-    Seg g;
-    uint32_t cb[2], ce[2];
-    bool large[2] = {false, false};
-    bool oneSided = false;
-    if (s < n) {
-      g = segs[s];
-      uint32_t size = g.end - g.begin;
-      uint32_t nl = c.segLeft[s];
-      oneSided = (nl == 0 || nl == size);
-      uint32_t mid = oneSided ? (uint32_t)(((uint64_t)g.begin + g.end) / 2) : g.begin + nl;
-      cb[0] = g.begin; ce[0] = mid;
-      cb[1] = mid; ce[1] = g.end;
-      for (int side = 0; side < 2; side++) {
-        uint32_t sz = ce[side] - cb[side];
-        large[side] = sz > (uint32_t)kSmall && sz > c.T;
-        if (large[side]) { nLarge++; if (oneSided) nFix++; }
-        else nSmallC++;
-      }
-    }
-    uint32_t totL, totS, totF;
-    uint32_t exL = block_excl_scan<1024>(nLarge, ss, &totL);
-    uint32_t exS = block_excl_scan<1024>(nSmallC, ss, &totS);
-    uint32_t exF = block_excl_scan<1024>(nFix, ss, &totF);
-    if (s < n) {
-      uint32_t child = nodeBase + 2 * s;
-      Node *nd = &c.nodes[g.node];
-      nd->oc = pack_oc(child, 0);
-      c.topNodes[topBase + s] = g.node;
-      uint32_t li = runLarge + exL, si = runSmall + exS, fi = runFix + exF;
-      for (int side = 0; side < 2; side++) {
-        if (large[side]) {
-          Seg ns;
-          ns.begin = cb[side]; ns.end = ce[side]; ns.node = child + side; ns.pad = 0;
-          ns.pad2[0] = ns.pad2[1] = 0;
-          for (int j = 0; j < 6; j++) ns.cb[j] = c.childCb[(2 * s + side) * 6 + j];
-          next[li] = ns;
-          if (oneSided) c.fixup[fi++] = li;
-          li++;
-        } else {
-          SmallSeg ss2{cb[side], ce[side], child + side, (uint32_t)(par ^ 1)};
-          c.small[smallBase + si] = ss2;
-          si++;
-        }
-      }
-    }
-    runLarge += totL;
-    runSmall += totS;
-    runFix += totF;
-  }
-  if (threadIdx.x == 0) {
-    c.ctr->nodeCount = nodeBase + 2 * n;
-    c.ctr->numSmall = smallBase + runSmall;
-    c.ctr->numLarge[par ^ 1] = runLarge;
-    c.ctr->numLarge[par] = 0;
-    c.ctr->numFixup = runFix;
-    c.ctr->numTop = topBase + n;
-    c.levelStart[level + 1] = topBase + n;
+// Emit, in three passes over the level's segments (count, scan, write), so
+// levels with tens of thousands of segments run on the whole chip: node
+// records for the split nodes, the next level's segments (with their split
+// planes, tile offsets and tile->segment map: the next level's plan), the
+// small-segment list and fixups for one-sided (degenerate) splits.  Output
+// order is segment order, so numbering is deterministic.
+constexpr int kEmitBlock = 256;
+
+struct EmitSeg {
+  Seg g;
+  uint32_t cb[2], ce[2];
+  bool large[2];
+  bool oneSided;
+};
+
+__device__ __forceinline__ void emit_classify(const Ctx &c, int par, uint32_t s, EmitSeg &e) {
+  e.g = c.segs[par][s];
+  const uint32_t size = e.g.end - e.g.begin;
+  const uint32_t nl = c.segLeft[par][s];
+  e.oneSided = (nl == 0 || nl == size);
+  const uint32_t mid = e.oneSided ? (uint32_t)(((uint64_t)e.g.begin + e.g.end) / 2) : e.g.begin + nl;
+  e.cb[0] = e.g.begin; e.ce[0] = mid;
+  e.cb[1] = mid; e.ce[1] = e.g.end;
+  for (int side = 0; side < 2; side++) {
+    const uint32_t sz = e.ce[side] - e.cb[side];
+    e.large[side] = sz > (uint32_t)kSmall && sz > c.T;
   }
 }
 
-// Fixup: exact centroid bounds of large children of one-sided splits.
+// packed per-segment counts: large children | small children << 10 | fixups << 20
+__device__ __forceinline__ void emit_counts(const EmitSeg &e, uint32_t &lsf, uint32_t &tiles, uint32_t &items) {
+  lsf = 0; tiles = 0; items = 0;
+  for (int side = 0; side < 2; side++) {
+    if (e.large[side]) {
+      lsf += 1u + (e.oneSided ? (1u << 20) : 0u);
+      tiles += div_up(e.ce[side] - e.cb[side], kTile);
+      items += e.ce[side] - e.cb[side];
+    } else {
+      lsf += 1u << 10;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kEmitBlock) void k_emit_count(Ctx c, int level) {
+  __shared__ uint32_t ss[kEmitBlock / kWave + 1];
+  const int par = level & 1;
+  const uint32_t n = c.ctr->numLarge[par];
+  const uint32_t s = blockIdx.x * kEmitBlock + threadIdx.x;
+  if (blockIdx.x * kEmitBlock >= n) return;
+  uint32_t lsf = 0, tiles = 0, items = 0;
+  if (s < n) {
+    EmitSeg e;
+    emit_classify(c, par, s, e);
+    emit_counts(e, lsf, tiles, items);
+  }
+  uint32_t totLsf, totT, totI;
+  const uint32_t exLsf = block_excl_scan<kEmitBlock>(lsf, ss, &totLsf);
+  const uint32_t exT = block_excl_scan<kEmitBlock>(tiles, ss, &totT);
+  block_excl_scan<kEmitBlock>(items, ss, &totI);
+  if (s < n) c.emitOff[s] = make_uint4(exLsf, exT, 0u, 0u);
+  if (threadIdx.x == 0) c.blockSum[blockIdx.x] = make_uint4(totLsf, totT, totI, 0u);
+}
+
+__global__ __launch_bounds__(1024) void k_emit_scan(Ctx c, int level) {
+  __shared__ uint32_t ss[1024 / kWave + 1];
+  const int par = level & 1;
+  const uint32_t n = c.ctr->numLarge[par];
+  const uint32_t nb = div_up(n, kEmitBlock);
+  uint32_t runL = 0, runS = 0, runF = 0, runT = 0, runI = 0;
+  for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
+    const uint32_t b = b0 + threadIdx.x;
+    uint4 v = b < nb ? c.blockSum[b] : make_uint4(0u, 0u, 0u, 0u);
+    uint32_t tL, tS, tF, tT, tI;
+    const uint32_t eL = block_excl_scan<1024>(v.x & 1023u, ss, &tL);
+    const uint32_t eS = block_excl_scan<1024>((v.x >> 10) & 1023u, ss, &tS);
+    const uint32_t eF = block_excl_scan<1024>(v.x >> 20, ss, &tF);
+    const uint32_t eT = block_excl_scan<1024>(v.y, ss, &tT);
+    block_excl_scan<1024>(v.z, ss, &tI);
+    if (b < nb) c.blockOff[b] = make_uint4(runL + eL, runS + eS, runF + eF, runT + eT);
+    runL += tL; runS += tS; runF += tF; runT += tT; runI += tI;
+  }
+  if (threadIdx.x == 0) {
+    Counters *k = c.ctr;
+    k->baseNode = k->nodeCount;
+    k->baseSmall = k->numSmall;
+    k->baseTop = k->numTop;
+    k->nodeCount += 2 * n;
+    k->numSmall += runS;
+    k->numLarge[par ^ 1] = runL;
+    k->numFixup = runF;
+    k->numTop += n;
+    c.levelStart[level + 1] = k->numTop;
+    k->numTiles = runT;
+    k->ticket = 0;
+    c.tileStart[par ^ 1][runL] = runT;
+    c.levelItems[level + 1] = runI;
+  }
+}
+
+__global__ __launch_bounds__(kEmitBlock) void k_emit_write(Ctx c, int level) {
+  const int par = level & 1, np = par ^ 1;
+  const uint32_t n = c.ctr->numLarge[par];
+  const uint32_t s = blockIdx.x * kEmitBlock + threadIdx.x;
+  if (s >= n) return;
+  const Counters *k = c.ctr;
+  const uint32_t nodeBase = k->baseNode, smallBase = k->baseSmall, topBase = k->baseTop;
+  EmitSeg e;
+  emit_classify(c, par, s, e);
+  const uint4 bo = c.blockOff[blockIdx.x];
+  const uint4 eo = c.emitOff[s];
+  uint32_t li = bo.x + (eo.x & 1023u), si = bo.y + ((eo.x >> 10) & 1023u), fi = bo.z + (eo.x >> 20);
+  uint32_t ti = bo.w + eo.y;
+  const uint32_t child = nodeBase + 2 * s;
+  c.nodes[e.g.node].oc = pack_oc(child, 0);
+  c.topNodes[topBase + s] = e.g.node;
+  Seg *next = c.segs[np];
+  for (int side = 0; side < 2; side++) {
+    if (e.large[side]) {
+      Seg ns;
+      ns.begin = e.cb[side]; ns.end = e.ce[side]; ns.node = child + side; ns.pad = 0;
+      ns.pad2[0] = ns.pad2[1] = 0;
+      for (int j = 0; j < 6; j++) ns.cb[j] = c.childCb[par][(2 * s + side) * 6 + j];
+      next[li] = ns;
+      // plan of the next level (one-sided children: k_fixup redoes the plane)
+      float lo[3], hi[3];
+      for (int d = 0; d < 3; d++) { lo[d] = dec_f(ns.cb[d]); hi[d] = dec_f(ns.cb[d + 3]); }
+      int dim;
+      float pos;
+      choose_split(lo, hi, dim, pos);
+      c.segDim[np][li] = dim;
+      c.segPos[np][li] = pos;
+      c.segLeft[np][li] = 0;
+      for (int j = 0; j < 12; j++) c.childCb[np][li * 12 + j] = enc_f((j % 6) < 3 ? INFINITY : -INFINITY);
+      const uint32_t tiles = div_up(ns.end - ns.begin, kTile);
+      c.tileStart[np][li] = ti;
+      for (uint32_t t = 0; t < tiles; t++) c.tileSeg[np][ti + t] = li;
+      ti += tiles;
+      if (e.oneSided) c.fixup[fi++] = li;
+      li++;
+    } else {
+      c.small[smallBase + si] = SmallSeg{e.cb[side], e.ce[side], child + side, (uint32_t)np};
+      si++;
+    }
+  }
+}
+
+// Fixup: exact centroid bounds and split planes of large children of
+// one-sided splits.  Also closes the level (its segment count drops to 0).
 __global__ __launch_bounds__(kBlock) void k_fixup(Ctx c, int level) {
   __shared__ float sb[12 * (kBlock / kWave)];
   const int par = level & 1;
   const uint32_t nf = c.ctr->numFixup;
+  if (blockIdx.x == 0 && threadIdx.x == 0) c.ctr->numLarge[par] = 0;
   Seg *next = c.segs[par ^ 1];
   const float4 *items = c.items[par ^ 1];
   for (uint32_t f = blockIdx.x; f < nf; f += gridDim.x) {
@@ -469,6 +509,14 @@ __global__ __launch_bounds__(kBlock) void k_fixup(Ctx c, int level) {
     }
     block_reduce_bounds<kBlock>(v, sb);
     if (threadIdx.x < 6) g.cb[threadIdx.x] = enc_f(v[threadIdx.x]);
+    if (threadIdx.x == 0) {
+      const float lo[3] = {v[0], v[1], v[2]}, hi[3] = {v[3], v[4], v[5]};
+      int dim;
+      float pos;
+      choose_split(lo, hi, dim, pos);
+      c.segDim[par ^ 1][c.fixup[f]] = dim;
+      c.segPos[par ^ 1][c.fixup[f]] = pos;
+    }
     __syncthreads();
   }
 }
@@ -785,11 +833,18 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
   CUBQL_CHECK(scr.alloc(&c.blockCount, numBlocks + 1));
   CUBQL_CHECK(scr.alloc(&c.segs[0], c.maxLarge));
   CUBQL_CHECK(scr.alloc(&c.segs[1], c.maxLarge));
-  CUBQL_CHECK(scr.alloc(&c.childCb, 12 * (size_t)c.maxLarge));
-  CUBQL_CHECK(scr.alloc(&c.segLeft, c.maxLarge));
-  CUBQL_CHECK(scr.alloc(&c.tileStart, c.maxLarge + 1));
-  CUBQL_CHECK(scr.alloc(&c.segDim, c.maxLarge));
-  CUBQL_CHECK(scr.alloc(&c.segPos, c.maxLarge));
+  const uint32_t maxEmitBlocks = div_up(c.maxLarge, kEmitBlock);
+  for (int p = 0; p < 2; p++) {
+    CUBQL_CHECK(scr.alloc(&c.childCb[p], 12 * (size_t)c.maxLarge));
+    CUBQL_CHECK(scr.alloc(&c.segLeft[p], c.maxLarge));
+    CUBQL_CHECK(scr.alloc(&c.tileStart[p], c.maxLarge + 1));
+    CUBQL_CHECK(scr.alloc(&c.tileSeg[p], maxTiles));
+    CUBQL_CHECK(scr.alloc(&c.segDim[p], c.maxLarge));
+    CUBQL_CHECK(scr.alloc(&c.segPos[p], c.maxLarge));
+  }
+  CUBQL_CHECK(scr.alloc(&c.emitOff, c.maxLarge));
+  CUBQL_CHECK(scr.alloc(&c.blockSum, maxEmitBlocks + 1));
+  CUBQL_CHECK(scr.alloc(&c.blockOff, maxEmitBlocks + 1));
   CUBQL_CHECK(scr.alloc(&c.tileStatus, maxTiles));
   CUBQL_CHECK(scr.alloc(&c.small, n + 1));
   CUBQL_CHECK(scr.alloc(&c.fixup, c.maxLarge));
@@ -828,12 +883,13 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
   std::vector<std::pair<int, int>> partSlots;  // (profile slot, level)
   while (true) {
     for (int i = 0; i < kLevelsPerSync && level < (int)maxLevels; i++, level++) {
-      k_plan<<<1, 1024, 0, s>>>(c, level);
       int ps = prof_begin(s);
       k_partition<<<partGrid, kBlock, 0, s>>>(c, level);
       prof_end(ps, s, "sm_partition");
       if (ps >= 0) partSlots.push_back({ps, level});
-      k_emit<<<1, 1024, 0, s>>>(c, level);
+      k_emit_count<<<maxEmitBlocks, kEmitBlock, 0, s>>>(c, level);
+      k_emit_scan<<<1, 1024, 0, s>>>(c, level);
+      k_emit_write<<<maxEmitBlocks, kEmitBlock, 0, s>>>(c, level);
       k_fixup<<<64, kBlock, 0, s>>>(c, level);
     }
     if (hipGetLastError() != hipSuccess) return fail((set_error("level launch failed"), CUBQL_ERR_HIP));

@@ -10,6 +10,7 @@
 #include "common.h"
 #include "cubql_amd/traversal.hip.h"
 #include "profile.h"
+#include "query_order.h"
 
 namespace cubql {
 namespace pq {
@@ -48,9 +49,10 @@ __device__ __forceinline__ bool fcp_one(const cubql_bvh3f &bvh, const float *poi
 __global__ __launch_bounds__(kBlock) void k_fcp(cubql_bvh3f bvh, const float *points,
                                                 const float *queries, uint32_t nq, float sqrMax,
                                                 int32_t *outIds, float *outD, Overflow *ovf,
-                                                uint32_t *ovfList) {
-  const uint32_t qi = blockIdx.x * kBlock + threadIdx.x;
-  if (qi >= nq) return;
+                                                uint32_t *ovfList, const uint32_t *order) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nq) return;
+  const uint32_t qi = order ? order[i] : i;
   float3 q = load_pt(queries, qi);
   int32_t best;
   float bestD;
@@ -138,9 +140,10 @@ __global__ __launch_bounds__(kBlock) void k_knn(cubql_bvh3f bvh, const float *po
                                                 const float *queries, uint32_t nq, int k,
                                                 float sqrMax, int32_t *outIds, float *outD,
                                                 int32_t *outFound, Overflow *ovf,
-                                                uint32_t *ovfList) {
-  const uint32_t qi = blockIdx.x * kBlock + threadIdx.x;
-  if (qi >= nq) return;
+                                                uint32_t *ovfList, const uint32_t *order) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nq) return;
+  const uint32_t qi = order ? order[i] : i;
   float3 q = load_pt(queries, qi);
   int32_t ids[K];
   float ds[K];
@@ -200,10 +203,10 @@ __global__ void k_fill_notfound(uint32_t nq, int k, int32_t *ids, float *d, int3
 template <int K>
 int launch_knn(const cubql_bvh3f &bvh, const float *points, const float *queries, uint32_t nq,
                int k, float sqrMax, int32_t *ids, float *d, int32_t *found, Overflow *ovf,
-               uint32_t *ovfList, hipStream_t s) {
+               uint32_t *ovfList, const uint32_t *order, hipStream_t s) {
   int ps = prof_begin(s);
   k_knn<K><<<div_up(nq, kBlock), kBlock, 0, s>>>(bvh, points, queries, nq, k, sqrMax, ids, d, found,
-                                                  ovf, ovfList);
+                                                  ovf, ovfList, order);
   prof_end(ps, s, "knn_points");
   prof_set(ps, (12.0 + 8.0 * k + 4.0) * nq, nq);
   k_knn_deep<K><<<64, 64, 0, s>>>(bvh, points, queries, k, sqrMax, ids, d, found, ovf, ovfList);
@@ -238,9 +241,11 @@ extern "C" int cubql_fcp_points_f3(const cubql_bvh3f *bvh, const float *d_points
   CUBQL_CHECK(scr.alloc(&ovf, 1));
   CUBQL_CHECK(scr.alloc(&ovfList, numQueries));
   CUBQL_CHECK(fill_u32((uint32_t *)ovf, 0u, 2, s));
+  const uint32_t *order = nullptr;
+  CUBQL_CHECK(query_order(*bvh, d_queries, numQueries, s, scr, &order));
   int ps = prof_begin(s);
   pq::k_fcp<<<div_up(numQueries, pq::kBlock), pq::kBlock, 0, s>>>(
-      *bvh, d_points, d_queries, numQueries, sqrMaxSearchDist, d_outIDs, d_outSqrDist, ovf, ovfList);
+      *bvh, d_points, d_queries, numQueries, sqrMaxSearchDist, d_outIDs, d_outSqrDist, ovf, ovfList, order);
   prof_end(ps, s, "fcp_points");
   // compulsory bytes only: query in, id + distance out
   prof_set(ps, 20.0 * numQueries, numQueries);
@@ -272,9 +277,11 @@ extern "C" int cubql_knn_points_f3(const cubql_bvh3f *bvh, const float *d_points
   CUBQL_CHECK(scr.alloc(&ovf, 1));
   CUBQL_CHECK(scr.alloc(&ovfList, numQueries));
   CUBQL_CHECK(fill_u32((uint32_t *)ovf, 0u, 2, s));
-  if (k <= 4) return pq::launch_knn<4>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, s);
-  if (k <= 8) return pq::launch_knn<8>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, s);
-  if (k <= 16) return pq::launch_knn<16>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, s);
-  if (k <= 32) return pq::launch_knn<32>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, s);
-  return pq::launch_knn<64>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, s);
+  const uint32_t *order = nullptr;
+  CUBQL_CHECK(query_order(*bvh, d_queries, numQueries, s, scr, &order));
+  if (k <= 4) return pq::launch_knn<4>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
+  if (k <= 8) return pq::launch_knn<8>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
+  if (k <= 16) return pq::launch_knn<16>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
+  if (k <= 32) return pq::launch_knn<32>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
+  return pq::launch_knn<64>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
 }

@@ -536,58 +536,103 @@ __device__ __forceinline__ uint32_t fr_pack(uint32_t b, uint32_t e, uint32_t me)
   return b | (e << 8) | (me << 16);
 }
 
+// Segmented inclusive min/max scan over contiguous lane segments: lane l
+// ends with the reduction over [head(l), l].  Steps are the DPP wave scan
+// (row_shr 1,2,4,8 inside rows of 16, then row_bcast 15 and 31 across rows);
+// cond[k] says whether step k's source lane lies in the lane's segment.
+template <bool MIN>
+__device__ __forceinline__ float seg_scan(float v, const bool (&cond)[6]) {
+  const float I = MIN ? INFINITY : -INFINITY;
+  float t;
+  t = dpp_f<0x111>(I, v); v = cond[0] ? (MIN ? fminf(v, t) : fmaxf(v, t)) : v;
+  t = dpp_f<0x112>(I, v); v = cond[1] ? (MIN ? fminf(v, t) : fmaxf(v, t)) : v;
+  t = dpp_f<0x114>(I, v); v = cond[2] ? (MIN ? fminf(v, t) : fmaxf(v, t)) : v;
+  t = dpp_f<0x118>(I, v); v = cond[3] ? (MIN ? fminf(v, t) : fmaxf(v, t)) : v;
+  t = dpp_f<0x142, 0xA>(I, v); v = cond[4] ? (MIN ? fminf(v, t) : fmaxf(v, t)) : v;
+  t = dpp_f<0x143, 0xC>(I, v); v = cond[5] ? (MIN ? fminf(v, t) : fmaxf(v, t)) : v;
+  return v;
+}
+
+// Reduction of v over the lane's segment [sb, se), in every lane of it.
+template <bool MIN>
+__device__ __forceinline__ float seg_reduce(float v, const bool (&cond)[6], uint32_t last) {
+  return __shfl(seg_scan<MIN>(v, cond), (int)last, kWave);
+}
+
+__device__ __forceinline__ void store_node_lane(Node *dst, const float (&lo)[3], const float (&hi)[3],
+                                                uint64_t oc) {
+  Node nd;
+  for (int d = 0; d < 3; d++) { nd.lo[d] = lo[d]; nd.hi[d] = hi[d]; }
+  nd.oc = oc;
+  *dst = nd;
+}
+
 // Builds the subtree of a node with cnt <= 64 prims held one per lane
-// (lane j = prim at global position pos0 + j).  Same rules and order as the
-// CPU recursion (cpu/spatialMedian.h:81-114).  Items move between lanes with
-// ds_permute; bounds are DPP wave reductions.
+// (lane j = prim at position pos0 + j), splitting every open node of the
+// subtree in the same round: each lane tracks the range [sb, se) of the node
+// that holds its position; bounds are segmented wave reductions, ranks come
+// from ballots masked to the segment, items move with one ds_permute per
+// value.  Split rules and partition order (left stable, right reversed,
+// one-sided -> middle) are those of the CPU recursion
+// (cpu/spatialMedian.h:81-114), so positions and leaves are identical to a
+// node-at-a-time build; local ids are assigned breadth-first.
 template <bool PTS>
 __device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, float y, float z,
                             uint32_t id, float bx0, float by0, float bz0, float bx1, float by1,
-                            float bz1, uint32_t rootLocal, uint32_t &nLocal, Node *loc,
-                            uint32_t *sFr) {
+                            float bz1, uint32_t rootLocal, uint32_t &nLocal, Node *loc) {
   const uint32_t lane = lane_id();
   const uint64_t lt = lanemask_lt();
   const uint32_t T = c.T;
-  int sp = 1;
-  if (lane == 0) sFr[0] = fr_pack(0, cnt, rootLocal);
-  sync_wave_lds();
-  while (sp > 0) {
-    const uint32_t f = sFr[--sp];
-    const uint32_t b = f & 0xffu, e = (f >> 8) & 0xffu, me = f >> 16;
-    const bool in = lane >= b && lane < e;
+  bool active = lane < cnt;
+  uint32_t sb = active ? 0u : lane, se = active ? cnt : lane + 1u;
+  uint32_t me = rootLocal;
+  while (__ballot(active)) {
+    const int h = (int)sb, l = (int)lane;
+    const bool cond[6] = {l - 1 >= h, l - 2 >= h, l - 4 >= h, l - 8 >= h,
+                          (int)(lane & ~15u) - 1 >= h, lane >= 32u && 31 >= h};
+    const uint32_t last = se - 1;
     float clo[3], chi[3], nlo[3], nhi[3];
-    clo[0] = wave_min_dpp(in ? x : INFINITY);
-    clo[1] = wave_min_dpp(in ? y : INFINITY);
-    clo[2] = wave_min_dpp(in ? z : INFINITY);
-    chi[0] = wave_max_dpp(in ? x : -INFINITY);
-    chi[1] = wave_max_dpp(in ? y : -INFINITY);
-    chi[2] = wave_max_dpp(in ? z : -INFINITY);
+    clo[0] = seg_reduce<true>(x, cond, last);
+    clo[1] = seg_reduce<true>(y, cond, last);
+    clo[2] = seg_reduce<true>(z, cond, last);
+    chi[0] = seg_reduce<false>(x, cond, last);
+    chi[1] = seg_reduce<false>(y, cond, last);
+    chi[2] = seg_reduce<false>(z, cond, last);
     if (PTS) {
       for (int d = 0; d < 3; d++) { nlo[d] = clo[d]; nhi[d] = chi[d]; }
     } else {
-      nlo[0] = wave_min_dpp(in ? bx0 : INFINITY);
-      nlo[1] = wave_min_dpp(in ? by0 : INFINITY);
-      nlo[2] = wave_min_dpp(in ? bz0 : INFINITY);
-      nhi[0] = wave_max_dpp(in ? bx1 : -INFINITY);
-      nhi[1] = wave_max_dpp(in ? by1 : -INFINITY);
-      nhi[2] = wave_max_dpp(in ? bz1 : -INFINITY);
+      nlo[0] = seg_reduce<true>(bx0, cond, last);
+      nlo[1] = seg_reduce<true>(by0, cond, last);
+      nlo[2] = seg_reduce<true>(bz0, cond, last);
+      nhi[0] = seg_reduce<false>(bx1, cond, last);
+      nhi[1] = seg_reduce<false>(by1, cond, last);
+      nhi[2] = seg_reduce<false>(bz1, cond, last);
     }
-    const uint32_t n = e - b;
-    if (n <= T) {
-      store_node_wave(loc + me, nlo, nhi, pack_oc(pos0 + b, n));
-      if (in) c.primIDs[pos0 + lane] = id;
-      continue;
+    const uint32_t n = se - sb;
+    const bool isHead = lane == sb;
+    const bool leaf = active && n <= T;
+    const bool split = active && !leaf;
+    if (leaf) {
+      c.primIDs[pos0 + lane] = id;
+      if (isHead) store_node_lane(loc + me, nlo, nhi, pack_oc(pos0 + sb, n));
     }
     int dim;
     float pos;
     choose_split(clo, chi, dim, pos);
     const float v = dim == 0 ? x : (dim == 1 ? y : z);
-    const bool left = in && v < pos;
-    const uint64_t mL = __ballot(left);
-    const uint64_t mR = __ballot(in && !left);
+    const bool isL = split && v < pos;
+    const uint64_t segMask = (se >= 64u ? ~0ull : ((1ull << se) - 1ull)) & ~((1ull << sb) - 1ull);
+    const uint64_t mL = __ballot(isL) & segMask;
+    const uint64_t mR = __ballot(split && !isL) & segMask;
     const uint32_t nl = (uint32_t)__popcll(mL);
-    const uint32_t dst = left ? b + (uint32_t)__popcll(mL & lt)
-                              : (in ? e - 1 - (uint32_t)__popcll(mR & lt) : lane);
+    const uint32_t dst = !split ? lane
+                                : (isL ? sb + (uint32_t)__popcll(mL & lt) : se - 1u - (uint32_t)__popcll(mR & lt));
+    const uint32_t mid = (nl != 0 && nl != n) ? sb + nl
+                                              : (uint32_t)(((uint64_t)pos0 + sb + pos0 + se) / 2 - pos0);  // spatialMedian.h:107
+    const uint64_t heads = __ballot(split && isHead);
+    const uint32_t child = (uint32_t)__shfl((int)(nLocal + 2u * (uint32_t)__popcll(heads & lt)), (int)sb, kWave);
+    nLocal += 2u * (uint32_t)__popcll(heads);
+    if (split && isHead) store_node_lane(loc + me, nlo, nhi, pack_oc(child, 0));
     x = lane_push_f(dst, x);
     y = lane_push_f(dst, y);
     z = lane_push_f(dst, z);
@@ -596,18 +641,11 @@ __device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, 
       bx0 = lane_push_f(dst, bx0); by0 = lane_push_f(dst, by0); bz0 = lane_push_f(dst, bz0);
       bx1 = lane_push_f(dst, bx1); by1 = lane_push_f(dst, by1); bz1 = lane_push_f(dst, bz1);
     }
-    uint32_t mid;
-    if (nl != 0 && nl != n) mid = b + nl;
-    else mid = (uint32_t)(((uint64_t)pos0 + b + pos0 + e) / 2 - pos0);  // spatialMedian.h:107
-    const uint32_t child = nLocal;
-    nLocal += 2;
-    store_node_wave(loc + me, nlo, nhi, pack_oc(child, 0));
-    if (lane == 0) {
-      sFr[sp] = fr_pack(mid, e, child + 1);
-      sFr[sp + 1] = fr_pack(b, mid, child);
+    if (split) {
+      if (lane < mid) { se = mid; me = child; }
+      else { sb = mid; me = child + 1u; }
     }
-    sp += 2;
-    sync_wave_lds();
+    if (leaf) active = false;
   }
 }
 
@@ -642,7 +680,7 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
     cubql_box3f bx{};
     if (!PTS && lane < m) bx = load_box(c.boxes, __float_as_uint(v.w));
     reg_subtree<PTS>(c, gb, m, v.x, v.y, v.z, __float_as_uint(v.w), bx.lower[0], bx.lower[1],
-                     bx.lower[2], bx.upper[0], bx.upper[1], bx.upper[2], 0u, nLocal, loc, sFr);
+                     bx.lower[2], bx.upper[0], bx.upper[1], bx.upper[2], 0u, nLocal, loc);
   } else {
     // more than 64 prims: split in LDS until children fit one wave
     for (uint32_t i = lane; i < m; i += kWave) sIt[i] = gItems[gb + i];
@@ -662,7 +700,7 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
         if (!PTS && lane < n) bx = load_box(c.boxes, __float_as_uint(v.w));
         reg_subtree<PTS>(c, gb + b, n, v.x, v.y, v.z, __float_as_uint(v.w), bx.lower[0],
                          bx.lower[1], bx.lower[2], bx.upper[0], bx.upper[1], bx.upper[2], me,
-                         nLocal, loc, sFr);
+                         nLocal, loc);
         continue;
       }
       float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};

@@ -19,6 +19,7 @@
 #include "common.h"
 #include "cubql_amd/traversal.hip.h"
 #include "profile.h"
+#include "query_order.h"
 
 namespace cubql {
 namespace tri {
@@ -143,9 +144,10 @@ __device__ __forceinline__ void write_result(uint32_t qi, int32_t best, float be
 __global__ __launch_bounds__(kBlock) void k_fcp_tris(cubql_bvh3f bvh, const int32_t *tris, const float *verts,
                                                      const float *queries, uint32_t nq, float sqrMax,
                                                      int32_t *ids, float *d2, float *cp, Overflow *ovf,
-                                                     uint32_t *ovfList) {
-  const uint32_t qi = blockIdx.x * kBlock + threadIdx.x;
-  if (qi >= nq) return;
+                                                     uint32_t *ovfList, const uint32_t *order) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nq) return;
+  const uint32_t qi = order ? order[i] : i;
   const F3 q = ld3(queries, qi);
   int32_t best;
   float bestD;
@@ -305,10 +307,12 @@ extern "C" int cubql_fcp_triangles_f3(const cubql_bvh3f *bvh, const int32_t *d_t
   CUBQL_CHECK(scr.alloc(&ovf, 1));
   CUBQL_CHECK(scr.alloc(&ovfList, numQueries));
   CUBQL_CHECK(fill_u32((uint32_t *)ovf, 0u, 2, s));
+  const uint32_t *order = nullptr;
+  CUBQL_CHECK(query_order(*bvh, d_queries, numQueries, s, scr, &order));
   int ps = prof_begin(s);
   tri::k_fcp_tris<<<div_up(numQueries, tri::kBlock), tri::kBlock, 0, s>>>(
       *bvh, d_tris, d_verts, d_queries, numQueries, sqrMax, d_outTriIDs, d_outSqrDist, d_outClosest, ovf,
-      ovfList);
+      ovfList, order);
   prof_end(ps, s, "fcp_triangles");
   prof_set(ps, 20.0 * numQueries, numQueries);
   tri::k_fcp_tris_deep<<<64, 64, 0, s>>>(*bvh, d_tris, d_verts, d_queries, sqrMax, d_outTriIDs,

@@ -431,6 +431,16 @@ def main():
         print(f"[bench] batched chamfer: {batched['batched_ms']:.2f} ms, plain {batched['plain_ms']:.1f} ms"
               f" ({batched['speedup_vs_plain']:.1f}x)", file=sys.stderr, flush=True)
 
+    # host->device rate for callers that hold host arrays (never part of value)
+    hbox = torch.empty(boxes.shape, dtype=torch.float32, pin_memory=True)
+    hbox.copy_(boxes)
+    dbox = torch.empty_like(boxes)
+    dbox.copy_(hbox, non_blocking=True)
+    torch.cuda.synchronize()
+    h2d_ms = _time_cuda(lambda: dbox.copy_(hbox, non_blocking=True), 5)
+    pcie_h2d_gbs = hbox.numel() * 4 / (h2d_ms * 1e-3) / 1e9
+    del hbox, dbox
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(min(args.cpu_sample, N), 200_000, budget_s=12.0)
@@ -456,6 +466,7 @@ def main():
         "radix_builds": radix,
         "fcp_mqueries_s": nq / fcp_ms / 1e3,
         "knn_mqueries_s": nk / knn_ms / 1e3,
+        "pcie_h2d_gbs": pcie_h2d_gbs,
         "triangle_mesh": tris,
         "gaussian_mixture": mixture,
         "batched_chamfer": batched,

@@ -18,8 +18,10 @@ __device__ __forceinline__ uint64_t st_word(uint32_t tag, uint32_t flag, uint32_
   return ((uint64_t)tag << 34) | ((uint64_t)flag << 32) | v;
 }
 
-// Digit histograms of every pass for one tile of keys: written per tile
-// (no global atomics); k_hist_reduce sums them.
+// Digit histograms of every pass over kHistSpan tiles of keys: one partial
+// histogram per workgroup (no global atomics); k_hist_reduce sums them.
+constexpr uint32_t kHistSpan = 16;
+
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_hist(const K *keys, const uint32_t *nPtr, int passes,
                                                  uint32_t *tileHist) {
@@ -27,22 +29,19 @@ __global__ __launch_bounds__(kBlock) void k_hist(const K *keys, const uint32_t *
   const uint32_t n = *nPtr;
   for (int i = threadIdx.x; i < 8 * 256; i += kBlock) (&h[0][0])[i] = 0;
   __syncthreads();
-  const uint32_t base = blockIdx.x * kTile;
-  if (base < n) {
-    for (int k = 0; k < kTile / kBlock; k++) {
-      uint32_t i = base + k * kBlock + threadIdx.x;
-      if (i < n) {
-        K key = keys[i];
-        for (int p = 0; p < passes; p++) atomicAdd(&h[p][(uint32_t)(key >> (8 * p)) & 255u], 1u);
-      }
-    }
+  const uint32_t base = blockIdx.x * kTile * kHistSpan;
+  const uint32_t end = (uint32_t)min((uint64_t)n, (uint64_t)base + (uint64_t)kTile * kHistSpan);
+  for (uint32_t i = base + threadIdx.x; i < end; i += kBlock) {
+    const K key = keys[i];
+    for (int p = 0; p < passes; p++) atomicAdd(&h[p][(uint32_t)(key >> (8 * p)) & 255u], 1u);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < passes * 256; i += kBlock)
     tileHist[(size_t)blockIdx.x * passes * 256 + i] = (&h[0][0])[i];
 }
 
-// One workgroup per pass: sum tile histograms, exclusive scan over digits.
+// One workgroup per pass: sum the partial histograms, exclusive scan over
+// digits.
 __global__ __launch_bounds__(256) void k_hist_reduce(const uint32_t *tileHist, uint32_t numTiles,
                                                      int passes, uint32_t *digitBase) {
   __shared__ uint32_t ss[256 / kWave + 1];
@@ -189,8 +188,9 @@ int sort_pairs_ws(K *kA, uint32_t *vA, K *kB, uint32_t *vB, const uint32_t *nDev
   uint32_t *tickets = (uint32_t *)p;
   CUBQL_CHECK(fill_u32((uint32_t *)status, 0u, (size_t)maxTiles * 512, s));
   CUBQL_CHECK(fill_u32(tickets, 0u, 16, s));
-  k_hist<K><<<maxTiles, kBlock, 0, s>>>(kA, nDev, passes, tileHist);
-  k_hist_reduce<<<passes, 256, 0, s>>>(tileHist, maxTiles, passes, digitBase);
+  const uint32_t histBlocks = div_up(maxTiles, kHistSpan);
+  k_hist<K><<<histBlocks, kBlock, 0, s>>>(kA, nDev, passes, tileHist);
+  k_hist_reduce<<<passes, 256, 0, s>>>(tileHist, histBlocks, passes, digitBase);
   int dev = 0, numCU = 256;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&numCU, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)

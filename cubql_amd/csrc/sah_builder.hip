@@ -46,6 +46,11 @@ constexpr int kSmall = 256;             // max prims of a wave-finished node
 constexpr int kMaxBins = 64;            // one lane per bin
 constexpr int kBinWords = 8;            // lo xyz, hi xyz (encoded), count, pad
 constexpr int kLevelsPerSync = 8;
+// interleaved tile-ticket counters and child-bound copies spread the
+// same-address atomics of the partition (see sm_builder.hip)
+constexpr uint32_t kTicketCounters = 8;
+constexpr uint32_t kTicketStride = 32;
+constexpr uint32_t kCbCopies = 8;
 
 enum : int32_t { kModeLeaf = -1, kModeMiddle = -2 };
 
@@ -89,7 +94,8 @@ struct Ctx {
   uint32_t *blockCount;
   float *blockBounds;
   Seg *segs[2];
-  int32_t *childCb;    // [2*maxLarge][6]
+  int32_t *childCb;    // [kCbCopies][2*maxLarge][6]
+  uint32_t *tickets;   // kTicketCounters counters, kTicketStride words apart
   uint32_t *segLeft;   // [maxLarge]
   uint32_t *tileStart; // [maxLarge+1] partition tiles
   uint32_t *binStart;  // [maxLarge+1] binning tiles
@@ -154,38 +160,32 @@ __device__ __forceinline__ void bin_add(int32_t *bins, int nb, const Item &it, c
   }
 }
 
+// Inclusive wave prefix scans with DPP (row_shr 1,2,4,8 inside rows, then
+// row_bcast 15 and 31 across rows); idle lanes hold the identity.
 template <bool MIN>
-__device__ __forceinline__ float scan_up_excl(float v, float ident) {
-  const uint32_t lane = lane_id();
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    const float t = __shfl_up(v, o, kWave);
-    if (lane >= (uint32_t)o) v = MIN ? fminf(v, t) : fmaxf(v, t);
-  }
-  const float e = __shfl_up(v, 1, kWave);
-  return lane == 0 ? ident : e;
-}
-template <bool MIN>
-__device__ __forceinline__ float scan_down_incl(float v) {
-  const uint32_t lane = lane_id();
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    const float t = __shfl_down(v, o, kWave);
-    if (lane + o < (uint32_t)kWave) v = MIN ? fminf(v, t) : fmaxf(v, t);
-  }
+__device__ __forceinline__ float dpp_scan_f(float v) {
+  const float I = MIN ? INFINITY : -INFINITY;
+  v = MIN ? fminf(v, dpp_f<0x111>(I, v)) : fmaxf(v, dpp_f<0x111>(I, v));
+  v = MIN ? fminf(v, dpp_f<0x112>(I, v)) : fmaxf(v, dpp_f<0x112>(I, v));
+  v = MIN ? fminf(v, dpp_f<0x114>(I, v)) : fmaxf(v, dpp_f<0x114>(I, v));
+  v = MIN ? fminf(v, dpp_f<0x118>(I, v)) : fmaxf(v, dpp_f<0x118>(I, v));
+  v = MIN ? fminf(v, dpp_f<0x142, 0xA>(I, v)) : fmaxf(v, dpp_f<0x142, 0xA>(I, v));
+  v = MIN ? fminf(v, dpp_f<0x143, 0xC>(I, v)) : fmaxf(v, dpp_f<0x143, 0xC>(I, v));
   return v;
 }
-__device__ __forceinline__ uint32_t scan_up_excl_u(uint32_t v) {
-  const uint32_t incl = wave_incl_scan(v);
-  return incl - v;
+
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t old, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROWMASK, 0xF, false);
 }
-__device__ __forceinline__ uint32_t scan_down_incl_u(uint32_t v) {
-  const uint32_t lane = lane_id();
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    const uint32_t t = __shfl_down(v, o, kWave);
-    if (lane + o < (uint32_t)kWave) v += t;
-  }
+
+__device__ __forceinline__ uint32_t dpp_scan_u(uint32_t v) {
+  v += dpp_u<0x111>(0u, v);
+  v += dpp_u<0x112>(0u, v);
+  v += dpp_u<0x114>(0u, v);
+  v += dpp_u<0x118>(0u, v);
+  v += dpp_u<0x142, 0xA>(0u, v);
+  v += dpp_u<0x143, 0xC>(0u, v);
   return v;
 }
 
@@ -197,35 +197,38 @@ __device__ __forceinline__ uint32_t scan_down_incl_u(uint32_t v) {
 __device__ int sah_decide_wave(const int32_t *bins, int nb, uint32_t n, uint32_t cap, int &splitBin) {
   const uint32_t lane = lane_id();
   const bool inBin = lane < (uint32_t)nb;
+  // lane j: left side = bins 0..j-1 (a prefix scan over bins shifted by one
+  // lane), right side = bins j..nb-1 (a prefix scan over the reversed bins,
+  // read back from lane nb-1-j)
+  const bool hasLeft = lane >= 1u && lane <= (uint32_t)nb;
+  const uint32_t binL = hasLeft ? lane - 1u : 0u;
+  const uint32_t binR = inBin ? (uint32_t)nb - 1u - lane : 0u;
   float cost[3];
   float leafCost = INFINITY;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    float blo[3], bhi[3];
-    uint32_t cnt = 0;
-    const int32_t *w = bins + (d * nb + (inBin ? lane : 0)) * kBinWords;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      blo[k] = inBin ? dec_f(w[k]) : INFINITY;
-      bhi[k] = inBin ? dec_f(w[3 + k]) : -INFINITY;
-    }
-    cnt = inBin ? (uint32_t)w[6] : 0u;
+    const int32_t *wl = bins + (d * nb + binL) * kBinWords;
+    const int32_t *wr = bins + (d * nb + binR) * kBinWords;
     float llo[3], lhi[3], rlo[3], rhi[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      llo[k] = scan_up_excl<true>(blo[k], INFINITY);
-      lhi[k] = scan_up_excl<false>(bhi[k], -INFINITY);
-      rlo[k] = scan_down_incl<true>(blo[k]);
-      rhi[k] = scan_down_incl<false>(bhi[k]);
+      llo[k] = dpp_scan_f<true>(hasLeft ? dec_f(wl[k]) : INFINITY);
+      lhi[k] = dpp_scan_f<false>(hasLeft ? dec_f(wl[3 + k]) : -INFINITY);
+      rlo[k] = dpp_scan_f<true>(inBin ? dec_f(wr[k]) : INFINITY);
+      rhi[k] = dpp_scan_f<false>(inBin ? dec_f(wr[3 + k]) : -INFINITY);
     }
-    const uint32_t lCnt = scan_up_excl_u(cnt);
-    const uint32_t rCnt = scan_down_incl_u(cnt);
+    const uint32_t lCnt = dpp_scan_u(hasLeft ? (uint32_t)wl[6] : 0u);
+    const uint32_t rCntRev = dpp_scan_u(inBin ? (uint32_t)wr[6] : 0u);
+    // reversed lane i holds the right side of split bin nb-1-i
+    const float rAreaRev = surface_area(rlo, rhi);
+    const int src = inBin ? nb - 1 - (int)lane : (int)lane;
+    const float rArea = __shfl(rAreaRev, src, kWave);
+    const uint32_t rCnt = (uint32_t)__shfl((int)rCntRev, src, kWave);
     if (d == 0) {
       // rAreas[0] * rCount: the union of all bins is the node box
-      const float a0 = surface_area(rlo, rhi) * (float)rCnt;
-      leafCost = __shfl(a0, 0, kWave);
+      leafCost = __shfl(rArea * (float)rCnt, 0, kWave);
     }
-    float c = surface_area(llo, lhi) * (float)lCnt + surface_area(rlo, rhi) * (float)rCnt;
+    const float c = surface_area(llo, lhi) * (float)lCnt + rArea * (float)rCnt;
     const bool valid = inBin && lCnt > 0 && rCnt > 0 && c == c;
     cost[d] = valid ? c : INFINITY;
   }
@@ -383,10 +386,13 @@ __global__ __launch_bounds__(1024) void k_plan(Ctx c, int level) {
       btiles = div_up(size, kBinTile);
       c.segLeft[s] = 0;
       for (int j = 0; j < 3; j++) {
-        c.childCb[(2 * s) * 6 + j] = enc_f(INFINITY);
-        c.childCb[(2 * s) * 6 + j + 3] = enc_f(-INFINITY);
-        c.childCb[(2 * s + 1) * 6 + j] = enc_f(INFINITY);
-        c.childCb[(2 * s + 1) * 6 + j + 3] = enc_f(-INFINITY);
+        for (uint32_t cp = 0; cp < kCbCopies; cp++) {
+          int32_t *cb = c.childCb + (size_t)cp * 12 * c.maxLarge;
+          cb[(2 * s) * 6 + j] = enc_f(INFINITY);
+          cb[(2 * s) * 6 + j + 3] = enc_f(-INFINITY);
+          cb[(2 * s + 1) * 6 + j] = enc_f(INFINITY);
+          cb[(2 * s + 1) * 6 + j + 3] = enc_f(-INFINITY);
+        }
       }
     }
     uint32_t tot, totB, totItems;
@@ -407,6 +413,7 @@ __global__ __launch_bounds__(1024) void k_plan(Ctx c, int level) {
     c.ctr->numTiles = run;
     c.ctr->numBinTiles = runB;
     c.ctr->ticket = 0;
+    for (uint32_t cc = 0; cc < kTicketCounters; cc++) c.tickets[kTicketStride * cc] = 0;
     c.levelItems[level] = items;
   }
 }
@@ -517,7 +524,9 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
   const int nb = c.nb;
   while (true) {
     if (threadIdx.x == 0) {
-      const uint32_t t = atomicAdd(&c.ctr->ticket, 1u);
+      // counter cc hands out tiles cc, cc + kTicketCounters, ...
+      const uint32_t cc = blockIdx.x % kTicketCounters;
+      const uint32_t t = cc + kTicketCounters * atomicAdd(&c.tickets[kTicketStride * cc], 1u);
       sTile = t;
       if (t < numTiles) sSeg = find_seg(c.tileStart, nSeg, t);
     }
@@ -585,7 +594,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
       const int j = threadIdx.x % 6;
       const bool nonEmpty = side == 0 ? (nLeft > 0) : (nLeft < te - tb);
       if (nonEmpty) {
-        int32_t *dcb = &c.childCb[(2 * s + side) * 6 + j];
+        int32_t *dcb = &c.childCb[((size_t)(blockIdx.x % kCbCopies) * 2 * c.maxLarge + 2 * s + side) * 6 + j];
         if (j < 3) atomicMin(dcb, enc_f(v[threadIdx.x]));
         else atomicMax(dcb, enc_f(v[threadIdx.x]));
       }
@@ -644,7 +653,14 @@ __global__ __launch_bounds__(1024) void k_emit(Ctx c, int level) {
           Seg ns;
           ns.begin = cb[side]; ns.end = ce[side]; ns.node = child + side; ns.pad = 0;
           ns.pad2[0] = ns.pad2[1] = 0;
-          for (int j = 0; j < 6; j++) ns.cb[j] = c.childCb[(2 * s + side) * 6 + j];
+          for (int j = 0; j < 6; j++) {
+            int32_t v = c.childCb[(2 * s + side) * 6 + j];
+            for (uint32_t cp = 1; cp < kCbCopies; cp++) {
+              const int32_t w = c.childCb[((size_t)cp * 2 * c.maxLarge + 2 * s + side) * 6 + j];
+              v = j < 3 ? min(v, w) : max(v, w);
+            }
+            ns.cb[j] = v;
+          }
           next[li++] = ns;
         } else {
           c.small[smallBase + si] = SmallSeg{cb[side], ce[side], child + side, (uint32_t)(par ^ 1)};
@@ -851,7 +867,8 @@ int sah_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T
   CUBQL_CHECK(scr.alloc(&c.blockBounds, 8 * (size_t)numBlocks + 8));
   CUBQL_CHECK(scr.alloc(&c.segs[0], c.maxLarge));
   CUBQL_CHECK(scr.alloc(&c.segs[1], c.maxLarge));
-  CUBQL_CHECK(scr.alloc(&c.childCb, 12 * (size_t)c.maxLarge));
+  CUBQL_CHECK(scr.alloc(&c.childCb, 12 * (size_t)c.maxLarge * kCbCopies));
+  CUBQL_CHECK(scr.alloc(&c.tickets, kTicketStride * kTicketCounters));
   CUBQL_CHECK(scr.alloc(&c.segLeft, c.maxLarge));
   CUBQL_CHECK(scr.alloc(&c.tileStart, c.maxLarge + 1));
   CUBQL_CHECK(scr.alloc(&c.binStart, c.maxLarge + 1));
@@ -892,7 +909,8 @@ int sah_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&numCU, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     numCU = 256;
-  const uint32_t partGrid = std::min<uint32_t>(maxTiles, (uint32_t)numCU * 8);
+  // at least one workgroup per ticket counter, so every counter's tiles run
+  const uint32_t partGrid = std::max<uint32_t>(kTicketCounters, std::min<uint32_t>(maxTiles, (uint32_t)numCU * 8));
   const uint32_t binGrid = std::min<uint32_t>(maxBinTiles, (uint32_t)numCU * 8);
   const uint32_t selGrid = std::min<uint32_t>(c.maxLarge, (uint32_t)numCU * 8);
   int level = 0;

@@ -37,6 +37,13 @@ constexpr int kItems = 8;
 constexpr int kTile = kBlock * kItems;  // 2048 prims per partition tile
 constexpr int kSmall = 256;             // max prims of a wave-finished node
 constexpr int kLevelsPerSync = 8;
+// Tiles are claimed from kTicketCounters interleaved counters (counter c
+// hands out tiles c, c+kTicketCounters, ...) and child centroid bounds are
+// accumulated into kCbCopies copies: both spread same-address atomics, which
+// serialise at one memory-side atomic unit per address.
+constexpr uint32_t kTicketCounters = 8;
+constexpr uint32_t kTicketStride = 32;  // words between counters
+constexpr uint32_t kCbCopies = 8;
 
 struct Counters {
   uint32_t numValid;
@@ -64,6 +71,7 @@ struct SmallSeg {     // a node finished by one wave
 };
 
 struct Ctx {
+  uint32_t *tickets;       // kTicketCounters counters, kTicketStride words apart
   const cubql_box3f *boxes;
   uint32_t n;              // number of input boxes
   uint32_t T;              // leaf threshold
@@ -79,7 +87,7 @@ struct Ctx {
   // per-level arrays, double-buffered by level parity: the emit of level l
   // reads the [l&1] copies and prepares the [(l+1)&1] ones
   Seg *segs[2];            // large segments
-  int32_t *childCb[2];     // [2*maxLarge][6] child centroid bounds
+  int32_t *childCb[2];     // [kCbCopies][2*maxLarge][6] child centroid bounds
   uint32_t *segLeft[2];    // [maxLarge] left-side count after the partition
   uint32_t *tileStart[2];  // [maxLarge+1] first partition tile of each segment
   uint32_t *tileSeg[2];    // [maxTiles] segment of each partition tile
@@ -183,6 +191,7 @@ __global__ __launch_bounds__(1024) void k_count_scan(Ctx c, uint32_t numBlocks) 
     c.nodes[1] = z;  // node 1 is never used (bvh.h:66-70)
     k->numTiles = 0;
     k->ticket = 0;
+    for (uint32_t cc = 0; cc < kTicketCounters; cc++) c.tickets[kTicketStride * cc] = 0;
     if (run > kSmall && run > c.T) {
       Seg s;
       s.begin = 0; s.end = run; s.node = 0; s.pad = 0; s.pad2[0] = s.pad2[1] = 0;
@@ -198,7 +207,8 @@ __global__ __launch_bounds__(1024) void k_count_scan(Ctx c, uint32_t numBlocks) 
       c.segDim[0][0] = dim;
       c.segPos[0][0] = pos;
       c.segLeft[0][0] = 0;
-      for (int j = 0; j < 12; j++) c.childCb[0][j] = enc_f((j % 6) < 3 ? INFINITY : -INFINITY);
+      for (uint32_t cp = 0; cp < kCbCopies; cp++)
+        for (int j = 0; j < 12; j++) c.childCb[0][(size_t)cp * 12 * c.maxLarge + j] = enc_f((j % 6) < 3 ? INFINITY : -INFINITY);
       const uint32_t tiles = div_up(run, kTile);
       c.tileStart[0][0] = 0;
       c.tileStart[0][1] = tiles;
@@ -264,7 +274,11 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
   const uint32_t tag = (uint32_t)level + 1;
   while (true) {
     if (threadIdx.x == 0) {
-      uint32_t t = atomicAdd(&c.ctr->ticket, 1u);
+      // counter c hands out tiles c, c + kTicketCounters, ...: every tile a
+      // claimed tile waits on is either claimed (and held by a running
+      // workgroup) or next in its counter, which running workgroups serve
+      const uint32_t cc = blockIdx.x % kTicketCounters;
+      const uint32_t t = cc + kTicketCounters * atomicAdd(&c.tickets[kTicketStride * cc], 1u);
       sTile = t;
       if (t < numTiles) sSeg = c.tileSeg[par][t];
     }
@@ -334,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
       float val = v[threadIdx.x];
       bool nonEmpty = side == 0 ? (nLeft > 0) : (nLeft < te - tb);
       if (nonEmpty) {
-        int32_t *dstCb = &c.childCb[par][(2 * s + side) * 6 + j];
+        int32_t *dstCb = &c.childCb[par][((size_t)(blockIdx.x % kCbCopies) * 2 * c.maxLarge + 2 * s + side) * 6 + j];
         if (j < 3) atomicMin(dstCb, enc_f(val));
         else atomicMax(dstCb, enc_f(val));
       }
@@ -437,6 +451,7 @@ __global__ __launch_bounds__(1024) void k_emit_scan(Ctx c, int level) {
     c.levelStart[level + 1] = k->numTop;
     k->numTiles = runT;
     k->ticket = 0;
+    for (uint32_t cc = 0; cc < kTicketCounters; cc++) c.tickets[kTicketStride * cc] = 0;
     c.tileStart[par ^ 1][runL] = runT;
     c.levelItems[level + 1] = runI;
   }
@@ -464,7 +479,14 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_write(Ctx c, int level) {
       Seg ns;
       ns.begin = e.cb[side]; ns.end = e.ce[side]; ns.node = child + side; ns.pad = 0;
       ns.pad2[0] = ns.pad2[1] = 0;
-      for (int j = 0; j < 6; j++) ns.cb[j] = c.childCb[par][(2 * s + side) * 6 + j];
+      for (int j = 0; j < 6; j++) {
+        int32_t v = c.childCb[par][(2 * s + side) * 6 + j];
+        for (uint32_t cp = 1; cp < kCbCopies; cp++) {
+          const int32_t w = c.childCb[par][((size_t)cp * 2 * c.maxLarge + 2 * s + side) * 6 + j];
+          v = j < 3 ? min(v, w) : max(v, w);
+        }
+        ns.cb[j] = v;
+      }
       next[li] = ns;
       // plan of the next level (one-sided children: k_fixup redoes the plane)
       float lo[3], hi[3];
@@ -475,7 +497,9 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_write(Ctx c, int level) {
       c.segDim[np][li] = dim;
       c.segPos[np][li] = pos;
       c.segLeft[np][li] = 0;
-      for (int j = 0; j < 12; j++) c.childCb[np][li * 12 + j] = enc_f((j % 6) < 3 ? INFINITY : -INFINITY);
+      for (uint32_t cp = 0; cp < kCbCopies; cp++)
+        for (int j = 0; j < 12; j++)
+          c.childCb[np][(size_t)cp * 12 * c.maxLarge + li * 12 + j] = enc_f((j % 6) < 3 ? INFINITY : -INFINITY);
       const uint32_t tiles = div_up(ns.end - ns.begin, kTile);
       c.tileStart[np][li] = ti;
       for (uint32_t t = 0; t < tiles; t++) c.tileSeg[np][ti + t] = li;
@@ -873,7 +897,7 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
   CUBQL_CHECK(scr.alloc(&c.segs[1], c.maxLarge));
   const uint32_t maxEmitBlocks = div_up(c.maxLarge, kEmitBlock);
   for (int p = 0; p < 2; p++) {
-    CUBQL_CHECK(scr.alloc(&c.childCb[p], 12 * (size_t)c.maxLarge));
+    CUBQL_CHECK(scr.alloc(&c.childCb[p], 12 * (size_t)c.maxLarge * kCbCopies));
     CUBQL_CHECK(scr.alloc(&c.segLeft[p], c.maxLarge));
     CUBQL_CHECK(scr.alloc(&c.tileStart[p], c.maxLarge + 1));
     CUBQL_CHECK(scr.alloc(&c.tileSeg[p], maxTiles));
@@ -881,6 +905,7 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
     CUBQL_CHECK(scr.alloc(&c.segPos[p], c.maxLarge));
   }
   CUBQL_CHECK(scr.alloc(&c.emitOff, c.maxLarge));
+  CUBQL_CHECK(scr.alloc(&c.tickets, kTicketStride * kTicketCounters));
   CUBQL_CHECK(scr.alloc(&c.blockSum, maxEmitBlocks + 1));
   CUBQL_CHECK(scr.alloc(&c.blockOff, maxEmitBlocks + 1));
   CUBQL_CHECK(scr.alloc(&c.tileStatus, maxTiles));
@@ -916,7 +941,8 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&numCU, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     numCU = 256;
-  const uint32_t partGrid = std::min<uint32_t>(maxTiles, (uint32_t)numCU * 8);
+  // at least one workgroup per ticket counter, so every counter's tiles run
+  const uint32_t partGrid = std::max<uint32_t>(kTicketCounters, std::min<uint32_t>(maxTiles, (uint32_t)numCU * 8));
   int level = 0;
   std::vector<std::pair<int, int>> partSlots;  // (profile slot, level)
   while (true) {

@@ -685,7 +685,7 @@ __global__ __launch_bounds__(1024) void k_emit(Ctx c, int level) {
 // LDS with the same rules; nodes are numbered locally (local 0 = the node
 // itself) into loc[2*begin + j] and renumbered by k_small_scan/k_small_emit.
 __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx, Item *sIt,
-                              Item *sAlt, int32_t *sBins, uint32_t *sFr, uint32_t *sMe) {
+                              int32_t *sBins, uint32_t *sFr, uint32_t *sMe) {
   const uint32_t lane = lane_id();
   const Item *gItems = c.items[sg.buf];
   const uint32_t gb = sg.begin;
@@ -748,23 +748,34 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
     }
     const float lo = mode >= 0 ? clo[mode] : 0.f, hi = mode >= 0 ? chi[mode] : 0.f;
     const uint32_t midPos = b + n / 2;
+    // partition in place: the node's items (<= kSmall/64 per lane) are read
+    // into registers first; a wave's LDS accesses run in program order, so
+    // the writes below cannot overtake those reads
+    constexpr int kPer = kSmall / kWave;
+    Item v[kPer];
+    uint32_t dst[kPer];
+    bool act[kPer];
     uint32_t nL = 0, nR = 0;
-    for (uint32_t c0 = b; c0 < e; c0 += kWave) {
-      const uint32_t i = c0 + lane;
-      const bool act = i < e;
-      Item v;
-      if (act) v = sIt[i];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint32_t i = b + (uint32_t)k * kWave + lane;
+      act[k] = i < e;
       bool isL = false;
-      if (act) isL = mode == kModeMiddle ? i < midPos : sah_bin(item_center(v, mode), lo, hi, nb) < bin;
+      if (act[k]) {
+        v[k] = sIt[i];
+        isL = mode == kModeMiddle ? i < midPos : sah_bin(item_center(v[k], mode), lo, hi, nb) < bin;
+      }
       const uint64_t mL = __ballot(isL);
-      const uint64_t mR = __ballot(act && !isL);
-      if (isL) sAlt[b + nL + mbcnt(mL)] = v;
-      else if (act) sAlt[e - 1 - (nR + mbcnt(mR))] = v;
+      const uint64_t mR = __ballot(act[k] && !isL);
+      dst[k] = isL ? b + nL + mbcnt(mL) : e - 1 - (nR + mbcnt(mR));
       nL += (uint32_t)__popcll(mL);
       nR += (uint32_t)__popcll(mR);
     }
     sync_wave_lds();
-    for (uint32_t i = b + lane; i < e; i += kWave) sIt[i] = sAlt[i];
+#pragma unroll
+    for (int k = 0; k < kPer; k++)
+      if (act[k]) sIt[dst[k]] = v[k];
+    sync_wave_lds();
     const uint32_t mid = b + nL;
     const uint32_t child = nLocal;
     nLocal += 2;
@@ -781,15 +792,20 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
   if (lane == 0) c.smallCount[segIdx] = nLocal;
 }
 
+// LDS (dynamic, sized by the bin count): items | bins | two stacks
+__host__ __device__ constexpr size_t small_lds_bytes(int nb) {
+  return sizeof(Item) * kSmall + sizeof(int32_t) * 3 * (size_t)nb * kBinWords + 2 * sizeof(uint32_t) * (kSmall + 2);
+}
+
 __global__ __launch_bounds__(kWave) void k_small(Ctx c) {
-  __shared__ Item sIt[kSmall];
-  __shared__ Item sAlt[kSmall];
-  __shared__ int32_t sBins[3 * kMaxBins * kBinWords];
-  __shared__ uint32_t sFr[kSmall + 2];
-  __shared__ uint32_t sMe[kSmall + 2];
+  extern __shared__ __align__(16) unsigned char smem[];
+  Item *sIt = reinterpret_cast<Item *>(smem);
+  int32_t *sBins = reinterpret_cast<int32_t *>(smem + sizeof(Item) * kSmall);
+  uint32_t *sFr = reinterpret_cast<uint32_t *>(sBins + 3 * c.nb * kBinWords);
+  uint32_t *sMe = sFr + (kSmall + 2);
   const uint32_t nSmall = c.ctr->numSmall;
   for (uint32_t w = blockIdx.x; w < nSmall; w += gridDim.x) {
-    small_segment(c, c.small[w], w, sIt, sAlt, sBins, sFr, sMe);
+    small_segment(c, c.small[w], w, sIt, sBins, sFr, sMe);
     sync_wave_lds();
   }
 }
@@ -947,7 +963,7 @@ int sah_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T
   int smallSlot = -1;
   if (numSmall) {
     smallSlot = prof_begin(s);
-    k_small<<<smallGrid, kWave, 0, s>>>(c);
+    k_small<<<smallGrid, kWave, small_lds_bytes(numBins), s>>>(c);
     prof_end(smallSlot, s, "sah_small");
     k_small_scan<<<1, 1024, 0, s>>>(c);
   }

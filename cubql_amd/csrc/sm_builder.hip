@@ -675,7 +675,7 @@ __device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, 
 
 template <bool PTS>
 __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx, float4 *sIt,
-                              float4 *sAlt, uint32_t *sFr, uint32_t *sFr1, uint32_t *sMe) {
+                              uint32_t *sFr1, uint32_t *sMe) {
   const uint32_t lane = lane_id();
   const uint32_t T = c.T;
   const float4 *gItems = c.items[sg.buf];
@@ -752,16 +752,23 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
       int dim;
       float pos;
       choose_split(clo, chi, dim, pos);
+      // partition in place: the node's items (<= kSmall/64 per lane) are
+      // read into registers first; a wave's LDS accesses run in program
+      // order, so the writes below cannot overtake those reads
+      constexpr int kPer = kSmall / kWave;
+      float4 v[kPer];
+      uint32_t dst[kPer];
+      bool act[kPer];
       uint32_t nL = 0, nR = 0;
-      for (uint32_t c0 = b; c0 < e; c0 += kWave) {
-        const uint32_t i = c0 + lane;
-        const bool act = i < e;
-        const float4 v = act ? sIt[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const bool isL = act && (coord(v, dim) < pos);
+#pragma unroll
+      for (int k = 0; k < kPer; k++) {
+        const uint32_t i = b + (uint32_t)k * kWave + lane;
+        act[k] = i < e;
+        v[k] = act[k] ? sIt[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const bool isL = act[k] && (coord(v[k], dim) < pos);
         const uint64_t mL = __ballot(isL);
-        const uint64_t mR = __ballot(act && !isL);
-        if (isL) sAlt[b + nL + mbcnt(mL)] = v;
-        else if (act) sAlt[e - 1 - (nR + mbcnt(mR))] = v;
+        const uint64_t mR = __ballot(act[k] && !isL);
+        dst[k] = isL ? b + nL + mbcnt(mL) : e - 1 - (nR + mbcnt(mR));
         nL += (uint32_t)__popcll(mL);
         nR += (uint32_t)__popcll(mR);
       }
@@ -769,7 +776,9 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
       if (nL != 0 && nR != 0) mid = b + nL;
       else mid = (uint32_t)(((uint64_t)gb + b + gb + e) / 2 - gb);  // spatialMedian.h:107
       sync_wave_lds();
-      for (uint32_t i = b + lane; i < e; i += kWave) sIt[i] = sAlt[i];
+#pragma unroll
+      for (int k = 0; k < kPer; k++)
+        if (act[k]) sIt[dst[k]] = v[k];
       const uint32_t child = nLocal;
       nLocal += 2;
       store_node_wave(loc + me, olo, ohi, pack_oc(child, 0));
@@ -788,16 +797,14 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
 
 __global__ __launch_bounds__(kWave) void k_small(Ctx c) {
   __shared__ float4 sIt[kSmall];
-  __shared__ float4 sAlt[kSmall];
-  __shared__ uint32_t sFr[66];
   __shared__ uint32_t sFr1[kSmall + 2];
   __shared__ uint32_t sMe[kSmall + 2];
   const bool pts = c.ctr->allPoints != 0;
   const uint32_t nSmall = c.ctr->numSmall;
   for (uint32_t w = blockIdx.x; w < nSmall; w += gridDim.x) {
     const SmallSeg sg = c.small[w];
-    if (pts) small_segment<true>(c, sg, w, sIt, sAlt, sFr, sFr1, sMe);
-    else small_segment<false>(c, sg, w, sIt, sAlt, sFr, sFr1, sMe);
+    if (pts) small_segment<true>(c, sg, w, sIt, sFr1, sMe);
+    else small_segment<false>(c, sg, w, sIt, sFr1, sMe);
     sync_wave_lds();
   }
 }

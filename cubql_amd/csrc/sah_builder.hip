@@ -160,88 +160,106 @@ __device__ __forceinline__ void bin_add(int32_t *bins, int nb, const Item &it, c
   }
 }
 
-// Inclusive wave prefix scans with DPP (row_shr 1,2,4,8 inside rows, then
-// row_bcast 15 and 31 across rows); idle lanes hold the identity.
-template <bool MIN>
-__device__ __forceinline__ float dpp_scan_f(float v) {
-  const float I = MIN ? INFINITY : -INFINITY;
-  v = MIN ? fminf(v, dpp_f<0x111>(I, v)) : fmaxf(v, dpp_f<0x111>(I, v));
-  v = MIN ? fminf(v, dpp_f<0x112>(I, v)) : fmaxf(v, dpp_f<0x112>(I, v));
-  v = MIN ? fminf(v, dpp_f<0x114>(I, v)) : fmaxf(v, dpp_f<0x114>(I, v));
-  v = MIN ? fminf(v, dpp_f<0x118>(I, v)) : fmaxf(v, dpp_f<0x118>(I, v));
-  v = MIN ? fminf(v, dpp_f<0x142, 0xA>(I, v)) : fmaxf(v, dpp_f<0x142, 0xA>(I, v));
-  v = MIN ? fminf(v, dpp_f<0x143, 0xC>(I, v)) : fmaxf(v, dpp_f<0x143, 0xC>(I, v));
+// Inclusive prefix scans with DPP (row_shr 1,2,4,8 inside rows, then
+// row_bcast 15, and 31 unless the two 32-lane halves scan independently),
+// on encoded bounds (enc_f keeps float order, so integer min/max equal the
+// float ones) and counts: with an identity as the old value each step is
+// one fused v_min/v_max/v_add_*_dpp.
+template <int OP, bool CROSS32>  // OP: 0 min, 1 max, 2 add
+__device__ __forceinline__ int32_t dpp_scan_i(int32_t v) {
+  const int32_t I = OP == 0 ? INT32_MAX : (OP == 1 ? INT32_MIN : 0);
+#define CUBQL_DPP_STEP(CTRL, ROWMASK)                                                        \
+  {                                                                                          \
+    const int32_t t = __builtin_amdgcn_update_dpp(I, v, CTRL, ROWMASK, 0xF, false);          \
+    v = OP == 0 ? min(v, t) : (OP == 1 ? max(v, t) : v + t);                                 \
+  }
+  CUBQL_DPP_STEP(0x111, 0xF)
+  CUBQL_DPP_STEP(0x112, 0xF)
+  CUBQL_DPP_STEP(0x114, 0xF)
+  CUBQL_DPP_STEP(0x118, 0xF)
+  CUBQL_DPP_STEP(0x142, 0xA)
+  if (CROSS32) CUBQL_DPP_STEP(0x143, 0xC)
+#undef CUBQL_DPP_STEP
   return v;
 }
 
-template <int CTRL, int ROWMASK = 0xF>
-__device__ __forceinline__ uint32_t dpp_u(uint32_t old, uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROWMASK, 0xF, false);
+// Split costs for the candidates held by this lane's (dim, bin): the left
+// side is the prefix scan over bins shifted by one lane, the right side the
+// prefix scan over the reversed bins read back from lane rev.
+// Returns +inf for invalid candidates; *leafTerm = area * count of all bins.
+template <bool CROSS32>
+__device__ __forceinline__ float split_cost(const int32_t *bins, int nb, int d, uint32_t bl, bool inBin,
+                                            int rev, float *leafTerm) {
+  const bool hasLeft = bl >= 1u && bl <= (uint32_t)nb;
+  const uint32_t binL = hasLeft ? bl - 1u : 0u;
+  const uint32_t binR = inBin ? (uint32_t)nb - 1u - bl : 0u;
+  const int32_t *wl = bins + (d * nb + binL) * kBinWords;
+  const int32_t *wr = bins + (d * nb + binR) * kBinWords;
+  float llo[3], lhi[3], rlo[3], rhi[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    llo[k] = dec_f(dpp_scan_i<0, CROSS32>(hasLeft ? wl[k] : INT32_MAX));
+    lhi[k] = dec_f(dpp_scan_i<1, CROSS32>(hasLeft ? wl[3 + k] : INT32_MIN));
+    rlo[k] = dec_f(dpp_scan_i<0, CROSS32>(inBin ? wr[k] : INT32_MAX));
+    rhi[k] = dec_f(dpp_scan_i<1, CROSS32>(inBin ? wr[3 + k] : INT32_MIN));
+  }
+  const uint32_t lCnt = (uint32_t)dpp_scan_i<2, CROSS32>(hasLeft ? wl[6] : 0);
+  const uint32_t rCntRev = (uint32_t)dpp_scan_i<2, CROSS32>(inBin ? wr[6] : 0);
+  const float rArea = __shfl(surface_area(rlo, rhi), rev, kWave);
+  const uint32_t rCnt = (uint32_t)__shfl((int)rCntRev, rev, kWave);
+  *leafTerm = rArea * (float)rCnt;
+  const float c = surface_area(llo, lhi) * (float)lCnt + rArea * (float)rCnt;
+  return (inBin && lCnt > 0 && rCnt > 0 && c == c) ? c : INFINITY;
 }
 
-__device__ __forceinline__ uint32_t dpp_scan_u(uint32_t v) {
-  v += dpp_u<0x111>(0u, v);
-  v += dpp_u<0x112>(0u, v);
-  v += dpp_u<0x114>(0u, v);
-  v += dpp_u<0x118>(0u, v);
-  v += dpp_u<0x142, 0xA>(0u, v);
-  v += dpp_u<0x143, 0xC>(0u, v);
-  return v;
-}
-
-// evaluateSAH (sah_builder.h:62-106) for one node, by one full wave: lane j
-// holds split candidate "bin j" of each dimension.  Every prefix/suffix
-// union is exact (min/max), so costs equal the serial evaluation's bit for
-// bit; the first minimum in (dim, bin) order is found by wave min + ballot.
+// evaluateSAH (sah_builder.h:62-106) for one node, by one full wave: each
+// lane holds one (dim, bin) split candidate.  Every prefix/suffix union is
+// exact (min/max), so costs equal the serial evaluation's bit for bit; the
+// first minimum in (dim, bin) order is found by wave min + ballot.  With at
+// most 32 bins dims 0 and 1 share one pass (lanes 0-31, 32-63).
 // n = prims of the node.  Returns the split dim, kModeLeaf or kModeMiddle.
 __device__ int sah_decide_wave(const int32_t *bins, int nb, uint32_t n, uint32_t cap, int &splitBin) {
   const uint32_t lane = lane_id();
-  const bool inBin = lane < (uint32_t)nb;
-  // lane j: left side = bins 0..j-1 (a prefix scan over bins shifted by one
-  // lane), right side = bins j..nb-1 (a prefix scan over the reversed bins,
-  // read back from lane nb-1-j)
-  const bool hasLeft = lane >= 1u && lane <= (uint32_t)nb;
-  const uint32_t binL = hasLeft ? lane - 1u : 0u;
-  const uint32_t binR = inBin ? (uint32_t)nb - 1u - lane : 0u;
-  float cost[3];
-  float leafCost = INFINITY;
+  float cost[3];     // candidate sets in (dim, bin) order; lane order within each
+  int passDim[3];    // first dim of each candidate set (-1: unused)
+  int nSets;
+  float leafCost;
+  if (nb <= 32) {
+    const uint32_t bl = lane & 31u, half = lane >> 5;
+    float leaf0, leaf1;
+    cost[0] = split_cost<false>(bins, nb, (int)half, bl, bl < (uint32_t)nb, (int)(32 * half + nb - 1 - bl) & 63,
+                                &leaf0);
+    cost[1] = split_cost<false>(bins, nb, 2, bl, half == 0 && bl < (uint32_t)nb, (int)(nb - 1 - bl) & 63, &leaf1);
+    passDim[0] = 0; passDim[1] = 2; passDim[2] = -1;
+    nSets = 2;
+    leafCost = __shfl(leaf0, 0, kWave);
+  } else {
+    const bool inBin = lane < (uint32_t)nb;
+    float lt[3];
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
-    const int32_t *wl = bins + (d * nb + binL) * kBinWords;
-    const int32_t *wr = bins + (d * nb + binR) * kBinWords;
-    float llo[3], lhi[3], rlo[3], rhi[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      llo[k] = dpp_scan_f<true>(hasLeft ? dec_f(wl[k]) : INFINITY);
-      lhi[k] = dpp_scan_f<false>(hasLeft ? dec_f(wl[3 + k]) : -INFINITY);
-      rlo[k] = dpp_scan_f<true>(inBin ? dec_f(wr[k]) : INFINITY);
-      rhi[k] = dpp_scan_f<false>(inBin ? dec_f(wr[3 + k]) : -INFINITY);
+    for (int d = 0; d < 3; d++) {
+      cost[d] = split_cost<true>(bins, nb, d, lane, inBin, inBin ? nb - 1 - (int)lane : (int)lane, &lt[d]);
+      passDim[d] = d;
     }
-    const uint32_t lCnt = dpp_scan_u(hasLeft ? (uint32_t)wl[6] : 0u);
-    const uint32_t rCntRev = dpp_scan_u(inBin ? (uint32_t)wr[6] : 0u);
-    // reversed lane i holds the right side of split bin nb-1-i
-    const float rAreaRev = surface_area(rlo, rhi);
-    const int src = inBin ? nb - 1 - (int)lane : (int)lane;
-    const float rArea = __shfl(rAreaRev, src, kWave);
-    const uint32_t rCnt = (uint32_t)__shfl((int)rCntRev, src, kWave);
-    if (d == 0) {
-      // rAreas[0] * rCount: the union of all bins is the node box
-      leafCost = __shfl(rArea * (float)rCnt, 0, kWave);
-    }
-    const float c = surface_area(llo, lhi) * (float)lCnt + rArea * (float)rCnt;
-    const bool valid = inBin && lCnt > 0 && rCnt > 0 && c == c;
-    cost[d] = valid ? c : INFINITY;
+    nSets = 3;
+    leafCost = __shfl(lt[0], 0, kWave);
   }
+  // rAreas[0] * rCount: the union of all bins is the node box
   auto pick = [&](float thr, int &bin) -> int {
     float best = thr;
     int dim = kModeLeaf;
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      const float m = wave_min(cost[d]);
+    for (int p = 0; p < nSets; p++) {
+      const float m = wave_min(cost[p]);
       if (m < best) {
         best = m;
-        dim = d;
-        bin = (int)__builtin_ctzll(__ballot(cost[d] == m));
+        const uint32_t idx = (uint32_t)__builtin_ctzll(__ballot(cost[p] == m));
+        if (nb <= 32) {
+          dim = passDim[p] + (int)(idx >> 5);
+          bin = (int)(idx & 31u);
+        } else {
+          dim = passDim[p];
+          bin = (int)idx;
+        }
       }
     }
     return dim;

@@ -258,102 +258,149 @@ __global__ __launch_bounds__(kBlock) void k_write_items(Ctx c) {
 }
 
 // ------------------------------------------------------------- level pass
-// Partition: each workgroup repeatedly claims a tile (ticket order, so a
-// tile only ever waits on tiles already claimed by running workgroups),
-// computes left/right ranks, resolves its prefix by decoupled look-back
-// among the tiles of its node and scatters the items.
+// Partition, software-pipelined over a workgroup's tiles.  A workgroup
+// claims tiles in ticket order (so a tile only ever waits on tiles claimed
+// by running workgroups); for its current tile it ranks the items left /
+// right, stages them in LDS in output order, resolves its prefix by
+// decoupled look-back among the tiles of its node and copies the staged
+// items out in contiguous runs.  While wave 0 looks back, wave 1 claims
+// and describes the workgroup's next tile, whose items are loaded into
+// registers before the current tile's copy-out, so each tile's load
+// latency overlaps the previous tile's look-back and stores.
+//
+// A workgroup holds at most two tiles, the later one claimed after the
+// earlier; every wait is on a lower tile, so the lowest unfinished tile
+// never waits and the grid drains.
+struct TileDesc {   // the next tile, written by one thread, read after a barrier
+  uint32_t t, seg, begin, end, dim, pos, tileFirst, tileEnd;
+};
+
 __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
+  __shared__ float4 sStage[kTile];
   __shared__ uint32_t sr[kItems * (kBlock / kWave) + 1];
   __shared__ float sb[12 * (kBlock / kWave)];
-  __shared__ uint32_t sTile, sSeg, sPrefix;
+  __shared__ TileDesc sNext;
+  __shared__ uint32_t sPrefix;
   const int par = level & 1;
   const uint32_t numTiles = c.ctr->numTiles;
-  const Seg *segs = c.segs[par];
   const float4 *src = c.items[par];
   float4 *dst = c.items[par ^ 1];
   const uint32_t tag = (uint32_t)level + 1;
-  while (true) {
-    if (threadIdx.x == 0) {
-      // counter c hands out tiles c, c + kTicketCounters, ...: every tile a
-      // claimed tile waits on is either claimed (and held by a running
-      // workgroup) or next in its counter, which running workgroups serve
-      const uint32_t cc = blockIdx.x % kTicketCounters;
-      const uint32_t t = cc + kTicketCounters * atomicAdd(&c.tickets[kTicketStride * cc], 1u);
-      sTile = t;
-      if (t < numTiles) sSeg = c.tileSeg[par][t];
+  // counter cc hands out tiles cc, cc + kTicketCounters, ...: every tile a
+  // claimed tile waits on is either claimed (and held by a running
+  // workgroup) or next in its counter, which running workgroups serve
+  const uint32_t cc = blockIdx.x % kTicketCounters;
+  auto fetch = [&]() {
+    TileDesc d;
+    d.t = cc + kTicketCounters * atomicAdd(&c.tickets[kTicketStride * cc], 1u);
+    if (d.t < numTiles) {
+      d.seg = c.tileSeg[par][d.t];
+      const Seg g = c.segs[par][d.seg];
+      d.begin = g.begin;
+      d.end = g.end;
+      d.dim = (uint32_t)c.segDim[par][d.seg];
+      d.pos = __float_as_uint(c.segPos[par][d.seg]);
+      d.tileFirst = c.tileStart[par][d.seg];
+      d.tileEnd = c.tileStart[par][d.seg + 1];
     }
-    sync_block_lds();
-    const uint32_t t = sTile;
-    if (t >= numTiles) break;
-    const uint32_t s = sSeg;
-    const Seg g = segs[s];
-    const int dim = c.segDim[par][s];
-    const float pos = c.segPos[par][s];
-    const uint32_t ts = c.tileStart[par][s];
-    const uint32_t k = t - ts;
-    const uint32_t nTilesSeg = c.tileStart[par][s + 1] - ts;
-    const uint32_t tb = g.begin + k * kTile;
-    const uint32_t te = min(tb + (uint32_t)kTile, g.end);
-
-    float4 it[kItems];
-    bool left[kItems];
-    bool in[kItems];
+    sNext = d;
+  };
+  if (threadIdx.x == 0) fetch();
+  sync_block_lds();
+  TileDesc cur = sNext;
+  if (cur.t >= numTiles) return;
+  float4 it[kItems];
+  {
+    const uint32_t tb = cur.begin + (cur.t - cur.tileFirst) * kTile;
+    const uint32_t last = min(tb + (uint32_t)kTile, cur.end) - 1;
 #pragma unroll
-    for (int j = 0; j < kItems; j++) {
-      uint32_t i = tb + j * kBlock + threadIdx.x;
-      in[j] = i < te;
-      left[j] = false;
-      if (in[j]) {
-        it[j] = src[i];
-        left[j] = coord(it[j], dim) < pos;
-      }
-    }
+    for (int j = 0; j < kItems; j++) it[j] = src[min(tb + j * kBlock + threadIdx.x, last)];
+  }
+  while (true) {
+    const uint32_t k = cur.t - cur.tileFirst;
+    const uint32_t tb = cur.begin + k * kTile;
+    const uint32_t n = min(tb + (uint32_t)kTile, cur.end) - tb;
+    const int dim = (int)cur.dim;
+    const float pos = __uint_as_float(cur.pos);
+    bool left[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; j++) left[j] = j * kBlock + threadIdx.x < n && coord(it[j], dim) < pos;
     uint32_t rank[kItems];
+    // its barriers also order the previous tile's reads of sStage, sNext
+    // and sPrefix before this tile's writes
     const uint32_t nLeft = block_rank_striped<kBlock, kItems, true>(left, rank, sr);
 
-    // decoupled look-back over the tiles of this node
+    // stage in output order: lefts ascending, rights reversed (the right
+    // side fills its range from the end)
+#pragma unroll
+    for (int j = 0; j < kItems; j++) {
+      const uint32_t l = j * kBlock + threadIdx.x;
+      if (l < n) sStage[left[j] ? rank[j] : n - 1 - (l - rank[j])] = it[j];
+    }
     if (threadIdx.x < kWave) {
-      const uint32_t prefix = lookback_exclusive_wave(c.tileStatus, t, k, tag, nLeft);
+      const uint32_t prefix = lookback_exclusive_wave(c.tileStatus, cur.t, k, tag, nLeft);
       if (threadIdx.x == 0) {
         sPrefix = prefix;
-        if (k == nTilesSeg - 1) c.segLeft[par][s] = prefix + nLeft;
+        if (cur.t == cur.tileEnd - 1) c.segLeft[par][cur.seg] = prefix + nLeft;
       }
+    } else if (threadIdx.x == kWave) {
+      fetch();
     }
     sync_block_lds();
     const uint32_t P = sPrefix;
-    const uint32_t rightsBefore = k * kTile - P;
-
-    float v[12];
-    for (int j = 0; j < 3; j++) { v[j] = INFINITY; v[j + 3] = -INFINITY; v[j + 6] = INFINITY; v[j + 9] = -INFINITY; }
+    const TileDesc nxt = sNext;
+    // the next tile's loads go out before this tile's stores (the asm
+    // statement keeps the compiler from sinking them below the copy-out;
+    // lanes past a short tile re-read its last item)
+    if (nxt.t < numTiles) {
+      const uint32_t tb2 = nxt.begin + (nxt.t - nxt.tileFirst) * kTile;
+      const uint32_t last = min(tb2 + (uint32_t)kTile, nxt.end) - 1;
 #pragma unroll
-    for (int j = 0; j < kItems; j++) {
-      if (!in[j]) continue;
-      uint32_t local = j * kBlock + threadIdx.x;
-      uint32_t d;
-      int o = left[j] ? 0 : 6;
-      if (left[j]) d = g.begin + P + rank[j];
-      else d = g.end - 1 - (rightsBefore + (local - rank[j]));
-      dst[d] = it[j];
-      v[o + 0] = fminf(v[o + 0], it[j].x);
-      v[o + 1] = fminf(v[o + 1], it[j].y);
-      v[o + 2] = fminf(v[o + 2], it[j].z);
-      v[o + 3] = fmaxf(v[o + 3], it[j].x);
-      v[o + 4] = fmaxf(v[o + 4], it[j].y);
-      v[o + 5] = fmaxf(v[o + 5], it[j].z);
+      for (int j = 0; j < kItems; j++) it[j] = src[min(tb2 + j * kBlock + threadIdx.x, last)];
+      asm volatile("" ::: "memory");
+    }
+    // copy out: staged [0, nLeft) -> begin + P + i; staged [nLeft, n) ->
+    // the right run, which ends rightsBefore items before the node's end;
+    // child centroid bounds on the way (left lo/hi, right lo/hi)
+    const uint32_t leftBase = cur.begin + P;
+    const uint32_t rightBase = cur.end - (k * kTile - P) - n;
+    float v[12];
+    {
+      float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll 4
+      for (uint32_t i = threadIdx.x; i < nLeft; i += kBlock) {
+        const float4 x = sStage[i];
+        dst[leftBase + i] = x;
+        lo[0] = fminf(lo[0], x.x); lo[1] = fminf(lo[1], x.y); lo[2] = fminf(lo[2], x.z);
+        hi[0] = fmaxf(hi[0], x.x); hi[1] = fmaxf(hi[1], x.y); hi[2] = fmaxf(hi[2], x.z);
+      }
+      for (int d = 0; d < 3; d++) { v[d] = lo[d]; v[d + 3] = hi[d]; }
+    }
+    {
+      float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll 4
+      for (uint32_t i = nLeft + threadIdx.x; i < n; i += kBlock) {
+        const float4 x = sStage[i];
+        dst[rightBase + i] = x;
+        lo[0] = fminf(lo[0], x.x); lo[1] = fminf(lo[1], x.y); lo[2] = fminf(lo[2], x.z);
+        hi[0] = fmaxf(hi[0], x.x); hi[1] = fmaxf(hi[1], x.y); hi[2] = fmaxf(hi[2], x.z);
+      }
+      for (int d = 0; d < 3; d++) { v[d + 6] = lo[d]; v[d + 9] = hi[d]; }
     }
     block_reduce_bounds<kBlock, true>(v, sb);
     if (threadIdx.x < 12) {
-      int side = threadIdx.x / 6;
-      int j = threadIdx.x % 6;
-      float val = v[threadIdx.x];
-      bool nonEmpty = side == 0 ? (nLeft > 0) : (nLeft < te - tb);
+      const int side = threadIdx.x / 6;
+      const int j = threadIdx.x % 6;
+      const bool nonEmpty = side == 0 ? (nLeft > 0) : (nLeft < n);
       if (nonEmpty) {
-        int32_t *dstCb = &c.childCb[par][((size_t)(blockIdx.x % kCbCopies) * 2 * c.maxLarge + 2 * s + side) * 6 + j];
-        if (j < 3) atomicMin(dstCb, enc_f(val));
-        else atomicMax(dstCb, enc_f(val));
+        int32_t *dstCb =
+            &c.childCb[par][((size_t)(blockIdx.x % kCbCopies) * 2 * c.maxLarge + 2 * cur.seg + side) * 6 + j];
+        if (j < 3) atomicMin(dstCb, enc_f(v[threadIdx.x]));
+        else atomicMax(dstCb, enc_f(v[threadIdx.x]));
       }
     }
-    sync_block_lds();
+    if (nxt.t >= numTiles) break;
+    cur = nxt;
   }
 }
 
@@ -949,7 +996,12 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
       hipDeviceGetAttribute(&numCU, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     numCU = 256;
   // at least one workgroup per ticket counter, so every counter's tiles run
-  const uint32_t partGrid = std::max<uint32_t>(kTicketCounters, std::min<uint32_t>(maxTiles, (uint32_t)numCU * 8));
+  // (the partition is persistent: one resident wave of workgroups)
+  int partPerCU = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&partPerCU, k_partition, kBlock, 0) != hipSuccess || partPerCU < 1)
+    partPerCU = 1;
+  const uint32_t partGrid =
+      std::max<uint32_t>(kTicketCounters, std::min<uint32_t>(maxTiles, (uint32_t)numCU * (uint32_t)partPerCU));
   int level = 0;
   std::vector<std::pair<int, int>> partSlots;  // (profile slot, level)
   while (true) {

@@ -508,54 +508,77 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_write(Ctx c, int level) {
   const int par = level & 1, np = par ^ 1;
   const uint32_t n = c.ctr->numLarge[par];
   const uint32_t s = blockIdx.x * kEmitBlock + threadIdx.x;
-  if (s >= n) return;
-  const Counters *k = c.ctr;
-  const uint32_t nodeBase = k->baseNode, smallBase = k->baseSmall, topBase = k->baseTop;
-  EmitSeg e;
-  emit_classify(c, par, s, e);
-  const uint4 bo = c.blockOff[blockIdx.x];
-  const uint4 eo = c.emitOff[s];
-  uint32_t li = bo.x + (eo.x & 1023u), si = bo.y + ((eo.x >> 10) & 1023u), fi = bo.z + (eo.x >> 20);
-  uint32_t ti = bo.w + eo.y;
-  const uint32_t child = nodeBase + 2 * s;
-  c.nodes[e.g.node].oc = pack_oc(child, 0);
-  c.topNodes[topBase + s] = e.g.node;
-  Seg *next = c.segs[np];
-  for (int side = 0; side < 2; side++) {
-    if (e.large[side]) {
-      Seg ns;
-      ns.begin = e.cb[side]; ns.end = e.ce[side]; ns.node = child + side; ns.pad = 0;
-      ns.pad2[0] = ns.pad2[1] = 0;
-      for (int j = 0; j < 6; j++) {
-        int32_t v = c.childCb[par][(2 * s + side) * 6 + j];
-        for (uint32_t cp = 1; cp < kCbCopies; cp++) {
-          const int32_t w = c.childCb[par][((size_t)cp * 2 * c.maxLarge + 2 * s + side) * 6 + j];
-          v = j < 3 ? min(v, w) : max(v, w);
+  // waves with no segment leave; in the others every lane stays for the
+  // wave-cooperative tile-map fill below
+  if (blockIdx.x * kEmitBlock + (threadIdx.x & ~(uint32_t)(kWave - 1)) >= n) return;
+  uint32_t fillStart[2] = {0, 0}, fillCount[2] = {0, 0}, fillSeg[2] = {0, 0};
+  if (s < n) {
+    const Counters *k = c.ctr;
+    const uint32_t nodeBase = k->baseNode, smallBase = k->baseSmall, topBase = k->baseTop;
+    EmitSeg e;
+    emit_classify(c, par, s, e);
+    const uint4 bo = c.blockOff[blockIdx.x];
+    const uint4 eo = c.emitOff[s];
+    uint32_t li = bo.x + (eo.x & 1023u), si = bo.y + ((eo.x >> 10) & 1023u), fi = bo.z + (eo.x >> 20);
+    uint32_t ti = bo.w + eo.y;
+    const uint32_t child = nodeBase + 2 * s;
+    c.nodes[e.g.node].oc = pack_oc(child, 0);
+    c.topNodes[topBase + s] = e.g.node;
+    Seg *next = c.segs[np];
+    for (int side = 0; side < 2; side++) {
+      if (e.large[side]) {
+        Seg ns;
+        ns.begin = e.cb[side]; ns.end = e.ce[side]; ns.node = child + side; ns.pad = 0;
+        ns.pad2[0] = ns.pad2[1] = 0;
+        for (int j = 0; j < 6; j++) {
+          int32_t v = c.childCb[par][(2 * s + side) * 6 + j];
+          for (uint32_t cp = 1; cp < kCbCopies; cp++) {
+            const int32_t w = c.childCb[par][((size_t)cp * 2 * c.maxLarge + 2 * s + side) * 6 + j];
+            v = j < 3 ? min(v, w) : max(v, w);
+          }
+          ns.cb[j] = v;
         }
-        ns.cb[j] = v;
+        next[li] = ns;
+        // plan of the next level (one-sided children: k_fixup redoes the plane)
+        float lo[3], hi[3];
+        for (int d = 0; d < 3; d++) { lo[d] = dec_f(ns.cb[d]); hi[d] = dec_f(ns.cb[d + 3]); }
+        int dim;
+        float pos;
+        choose_split(lo, hi, dim, pos);
+        c.segDim[np][li] = dim;
+        c.segPos[np][li] = pos;
+        c.segLeft[np][li] = 0;
+        for (uint32_t cp = 0; cp < kCbCopies; cp++)
+          for (int j = 0; j < 12; j++)
+            c.childCb[np][(size_t)cp * 12 * c.maxLarge + li * 12 + j] = enc_f((j % 6) < 3 ? INFINITY : -INFINITY);
+        const uint32_t tiles = div_up(ns.end - ns.begin, kTile);
+        c.tileStart[np][li] = ti;
+        fillStart[side] = ti;
+        fillCount[side] = tiles;
+        fillSeg[side] = li;
+        ti += tiles;
+        if (e.oneSided) c.fixup[fi++] = li;
+        li++;
+      } else {
+        c.small[smallBase + si] = SmallSeg{e.cb[side], e.ce[side], child + side, (uint32_t)np};
+        si++;
       }
-      next[li] = ns;
-      // plan of the next level (one-sided children: k_fixup redoes the plane)
-      float lo[3], hi[3];
-      for (int d = 0; d < 3; d++) { lo[d] = dec_f(ns.cb[d]); hi[d] = dec_f(ns.cb[d + 3]); }
-      int dim;
-      float pos;
-      choose_split(lo, hi, dim, pos);
-      c.segDim[np][li] = dim;
-      c.segPos[np][li] = pos;
-      c.segLeft[np][li] = 0;
-      for (uint32_t cp = 0; cp < kCbCopies; cp++)
-        for (int j = 0; j < 12; j++)
-          c.childCb[np][(size_t)cp * 12 * c.maxLarge + li * 12 + j] = enc_f((j % 6) < 3 ? INFINITY : -INFINITY);
-      const uint32_t tiles = div_up(ns.end - ns.begin, kTile);
-      c.tileStart[np][li] = ti;
-      for (uint32_t t = 0; t < tiles; t++) c.tileSeg[np][ti + t] = li;
-      ti += tiles;
-      if (e.oneSided) c.fixup[fi++] = li;
-      li++;
-    } else {
-      c.small[smallBase + si] = SmallSeg{e.cb[side], e.ce[side], child + side, (uint32_t)np};
-      si++;
+    }
+  }
+  // tile -> segment map: runs by their own lane, except the long runs of
+  // the top levels' few big segments, which the whole wave writes
+  constexpr uint32_t kOwnFill = 256;
+  const uint32_t lane = lane_id();
+  for (int side = 0; side < 2; side++) {
+    if (fillCount[side] <= kOwnFill)
+      for (uint32_t t = 0; t < fillCount[side]; t++) c.tileSeg[np][fillStart[side] + t] = fillSeg[side];
+    uint64_t m = __ballot(fillCount[side] > kOwnFill);
+    while (m) {
+      const int src = __builtin_ctzll(m);
+      m &= m - 1;
+      const uint32_t st = __shfl(fillStart[side], src, kWave), cnt = __shfl(fillCount[side], src, kWave);
+      const uint32_t sg = __shfl(fillSeg[side], src, kWave);
+      for (uint32_t t = lane; t < cnt; t += kWave) c.tileSeg[np][st + t] = sg;
     }
   }
 }

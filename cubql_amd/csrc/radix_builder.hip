@@ -842,7 +842,7 @@ int rebin_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t
   prof_end(hs, s, "rebin_hierarchy");
   const uint32_t numSmall = hc[C_SMALL];
   if (numSmall) {
-    k_thread_phase<<<std::max<uint32_t>(1u, std::min<uint32_t>(div_up(numSmall, kBlock), (uint32_t)numCU * 8)), kBlock, 0, s>>>(c, keys);
+    k_thread_phase<<<std::max<uint32_t>(1u, div_up(numSmall, kBlock)), kBlock, 0, s>>>(c, keys);
     k_thread_scan<<<1, 1024, 0, s>>>(c);
   }
   if (hipGetLastError() != hipSuccess) return fail((set_error("thread phase launch failed"), CUBQL_ERR_HIP));
@@ -860,7 +860,7 @@ int rebin_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t
       mem_free(mem, nodes, s);
       return fail((set_error("node copy failed"), CUBQL_ERR_HIP));
     }
-    if (numSmall) k_thread_emit<<<std::min<uint32_t>(numSmall, (uint32_t)numCU * 16), kWave, 0, s>>>(c, nodes);
+    if (numSmall) k_thread_emit<<<numSmall, kWave, 0, s>>>(c, nodes);
   }
   bvh->nodes = (cubql_node3f *)nodes;
   bvh->numNodes = numNodes;

@@ -977,7 +977,9 @@ int sah_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T
   const Counters hc0 = *(const Counters *)hw;
   const uint32_t numSmall = hc0.numSmall, numValid = hc0.numValid;
   const uint32_t topCount = hc0.nodeCount;
-  const uint32_t smallGrid = std::max<uint32_t>(1u, std::min<uint32_t>(numSmall, (uint32_t)numCU * 16));
+  // one workgroup per segment: the hardware dispatcher balances the uneven
+  // segment costs far better than a fixed round-robin over resident waves
+  const uint32_t smallGrid = std::max<uint32_t>(1u, numSmall);
   int smallSlot = -1;
   if (numSmall) {
     smallSlot = prof_begin(s);

@@ -263,10 +263,12 @@ __global__ __launch_bounds__(kBlock) void k_write_items(Ctx c) {
 // by running workgroups); for its current tile it ranks the items left /
 // right, stages them in LDS in output order, resolves its prefix by
 // decoupled look-back among the tiles of its node and copies the staged
-// items out in contiguous runs.  While wave 0 looks back, wave 1 claims
-// and describes the workgroup's next tile, whose items are loaded into
-// registers before the current tile's copy-out, so each tile's load
-// latency overlaps the previous tile's look-back and stores.
+// items out in contiguous runs.  After its look-back the workgroup claims
+// and describes its next tile, whose items are loaded into registers
+// before the current tile's copy-out, so each tile's load latency overlaps
+// the previous tile's stores.  (Claiming only then keeps the claim order
+// close to the order in which tiles publish their counts, which shortens
+// every later tile's look-back.)
 //
 // A workgroup holds at most two tiles, the later one claimed after the
 // earlier; every wait is on a lower tile, so the lowest unfinished tile
@@ -342,9 +344,8 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
       if (threadIdx.x == 0) {
         sPrefix = prefix;
         if (cur.t == cur.tileEnd - 1) c.segLeft[par][cur.seg] = prefix + nLeft;
+        fetch();
       }
-    } else if (threadIdx.x == kWave) {
-      fetch();
     }
     sync_block_lds();
     const uint32_t P = sPrefix;
@@ -1052,7 +1053,9 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
   const uint32_t numTop = hc0.numTop;
   const uint32_t topCount = hc0.nodeCount;  // ids [0, topCount) belong to the level phase
   int smallSlot = -1;
-  const uint32_t smallGrid = std::max<uint32_t>(1u, std::min<uint32_t>(numSmall, (uint32_t)numCU * 16));
+  // one workgroup per segment: the hardware dispatcher balances the uneven
+  // segment costs far better than a fixed round-robin over resident waves
+  const uint32_t smallGrid = std::max<uint32_t>(1u, numSmall);
   if (numSmall) {
     smallSlot = prof_begin(s);
     k_small<<<smallGrid, kWave, 0, s>>>(c);

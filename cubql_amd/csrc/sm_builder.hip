@@ -627,8 +627,11 @@ __global__ __launch_bounds__(kBlock) void k_fixup(Ctx c, int level) {
 
 struct SmallNodeOut { Node *loc; };
 
+// a pending node of the small phase's LDS loop in one word: range [b, e)
+// (<= kSmall) and local id (< 2 * kSmall)
+static_assert(kSmall <= 256, "small-phase stack packing");
 __device__ __forceinline__ uint32_t fr_pack(uint32_t b, uint32_t e, uint32_t me) {
-  return b | (e << 8) | (me << 16);
+  return b | (e << 9) | (me << 18);
 }
 
 // Segmented inclusive min/max scan over contiguous lane segments: lane l
@@ -746,7 +749,7 @@ __device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, 
 
 template <bool PTS>
 __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx, float4 *sIt,
-                              uint32_t *sFr1, uint32_t *sMe) {
+                              uint32_t *sFr) {
   const uint32_t lane = lane_id();
   const uint32_t T = c.T;
   const float4 *gItems = c.items[sg.buf];
@@ -780,14 +783,13 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
     // more than 64 prims: split in LDS until children fit one wave
     for (uint32_t i = lane; i < m; i += kWave) sIt[i] = gItems[gb + i];
     int sp = 1;
-    if (lane == 0) sFr1[0] = (0u) | (m << 16);
-    if (lane == 0) sMe[0] = 0;
+    if (lane == 0) sFr[0] = fr_pack(0u, m, 0u);
     sync_wave_lds();
     while (sp > 0) {
       --sp;
-      const uint32_t f = sFr1[sp];
-      const uint32_t me = sMe[sp];
-      const uint32_t b = f & 0xffffu, e = f >> 16;
+      const uint32_t f = sFr[sp];
+      const uint32_t me = f >> 18;
+      const uint32_t b = f & 511u, e = (f >> 9) & 511u;
       const uint32_t n = e - b;
       if (n <= 64) {
         float4 v = lane < n ? sIt[b + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -854,10 +856,8 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
       nLocal += 2;
       store_node_wave(loc + me, olo, ohi, pack_oc(child, 0));
       if (lane == 0) {
-        sFr1[sp] = mid | (e << 16);
-        sMe[sp] = child + 1;
-        sFr1[sp + 1] = b | (mid << 16);
-        sMe[sp + 1] = child;
+        sFr[sp] = fr_pack(mid, e, child + 1);
+        sFr[sp + 1] = fr_pack(b, mid, child);
       }
       sp += 2;
       sync_wave_lds();
@@ -868,14 +868,13 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
 
 __global__ __launch_bounds__(kWave) void k_small(Ctx c) {
   __shared__ float4 sIt[kSmall];
-  __shared__ uint32_t sFr1[kSmall + 2];
-  __shared__ uint32_t sMe[kSmall + 2];
+  __shared__ uint32_t sFr[kSmall + 2];
   const bool pts = c.ctr->allPoints != 0;
   const uint32_t nSmall = c.ctr->numSmall;
   for (uint32_t w = blockIdx.x; w < nSmall; w += gridDim.x) {
     const SmallSeg sg = c.small[w];
-    if (pts) small_segment<true>(c, sg, w, sIt, sFr1, sMe);
-    else small_segment<false>(c, sg, w, sIt, sFr1, sMe);
+    if (pts) small_segment<true>(c, sg, w, sIt, sFr);
+    else small_segment<false>(c, sg, w, sIt, sFr);
     sync_wave_lds();
   }
 }

@@ -241,7 +241,10 @@ __device__ __forceinline__ bool fcp_forest(const Side &tgt, uint32_t root, float
     }
     return bestD;
   };
-  return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, INFINITY, prim, root);
+  // seed the radius with the nearest-child leaf (visited again below,
+  // changing nothing)
+  cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim, root);
+  return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, bestD, prim, root);
 }
 
 // queries of side `qs` against the forest of side `ts`, in Morton order of qs

@@ -42,24 +42,9 @@ __device__ __forceinline__ bool fcp_one(const cubql_bvh3f &bvh, const float *poi
     }
     return bestD;
   };
-  // Seed the radius: descend along the nearer child to one leaf and take
-  // its prims first.  The traversal below then starts with that radius, so
-  // on the way down it stacks only far children that can still hold a
-  // closer (or equally close, lower-id) prim; the seed leaf is visited again
-  // and changes nothing.
-  if (bvh.numNodes > 0) {
-    uint64_t node = cubql_dev::node_oc(cubql_dev::load_node(bvh.nodes, 0));
-    while (cubql_dev::oc_cnt(node) == 0) {
-      const uint32_t i0 = cubql_dev::oc_off(node);
-      const cubql_dev::QNode n0 = cubql_dev::load_node(bvh.nodes, i0);
-      const cubql_dev::QNode n1 = cubql_dev::load_node(bvh.nodes, i0 + 1);
-      const float d0 = cubql_dev::sqr_dist_box(q.x, q.y, q.z, n0);
-      const float d1 = cubql_dev::sqr_dist_box(q.x, q.y, q.z, n1);
-      node = d0 < d1 ? cubql_dev::node_oc(n0) : cubql_dev::node_oc(n1);
-    }
-    const uint32_t *ids = bvh.primIDs + cubql_dev::oc_off(node);
-    for (uint32_t j = 0; j < cubql_dev::oc_cnt(node); j++) prim(ids[j]);
-  }
+  // seed the radius with the nearest-child leaf (visited again below,
+  // changing nothing)
+  cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim);
   return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, bestD, prim);
 }
 

@@ -127,7 +127,10 @@ __device__ __forceinline__ bool fcp_tri_one(const cubql_bvh3f &bvh, const int32_
     }
     return bestD;
   };
-  return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, sqrMax, prim);
+  // seed the radius with the nearest-child leaf (visited again below,
+  // changing nothing)
+  cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim);
+  return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, bestD, prim);
 }
 
 __device__ __forceinline__ void write_result(uint32_t qi, int32_t best, float bestD, F3 P, int32_t *ids,

@@ -112,6 +112,26 @@ __device__ __forceinline__ bool for_each_leaf(const cubql_bvh3f &bvh, float qx, 
   }
 }
 
+// Descends from `root` along the nearer child (ties to the second child, as
+// the traversal orders them) to one leaf and calls prim(primID) for its
+// prims.  A closest-point query that runs this first can start the full
+// traversal with the radius it found, so that on the way down it stacks only
+// far children that can still hold a closer prim.
+template <typename PrimFn>
+__device__ __forceinline__ void visit_nearest_leaf(const cubql_bvh3f &bvh, float qx, float qy, float qz,
+                                                   PrimFn &&prim, uint32_t root = 0) {
+  if (bvh.numNodes == 0) return;
+  uint64_t node = node_oc(load_node(bvh.nodes, root));
+  while (oc_cnt(node) == 0) {
+    const uint32_t i0 = oc_off(node);
+    const QNode n0 = load_node(bvh.nodes, i0);
+    const QNode n1 = load_node(bvh.nodes, i0 + 1);
+    node = sqr_dist_box(qx, qy, qz, n0) < sqr_dist_box(qx, qy, qz, n1) ? node_oc(n0) : node_oc(n1);
+  }
+  const uint32_t *ids = bvh.primIDs + oc_off(node);
+  for (uint32_t j = 0; j < oc_cnt(node); j++) prim(ids[j]);
+}
+
 // forEachPrim: prim(uint32_t primID) -> float (new squared cull radius).
 template <int STACK, typename PrimFn>
 __device__ __forceinline__ bool for_each_prim(const cubql_bvh3f &bvh, float qx, float qy, float qz,

@@ -523,7 +523,15 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_write(Ctx c, int level) {
     uint32_t li = bo.x + (eo.x & 1023u), si = bo.y + ((eo.x >> 10) & 1023u), fi = bo.z + (eo.x >> 20);
     uint32_t ti = bo.w + eo.y;
     const uint32_t child = nodeBase + 2 * s;
-    c.nodes[e.g.node].oc = pack_oc(child, 0);
+    if (k->allPoints) {
+      // point input: the node's box is its centroid bounds, final here
+      Node nd;
+      for (int d = 0; d < 3; d++) { nd.lo[d] = dec_f(e.g.cb[d]); nd.hi[d] = dec_f(e.g.cb[d + 3]); }
+      nd.oc = pack_oc(child, 0);
+      c.nodes[e.g.node] = nd;
+    } else {
+      c.nodes[e.g.node].oc = pack_oc(child, 0);
+    }
     c.topNodes[topBase + s] = e.g.node;
     Seg *next = c.segs[np];
     for (int side = 0; side < 2; side++) {
@@ -1086,7 +1094,8 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
     k_copy_top<<<std::max<uint32_t>(1u, std::min<uint32_t>(div_up(topCount, 256), 1024)), 256, 0, s>>>(
         c.nodes, nodes, topCount);
     if (numSmall) k_small_emit<<<smallGrid, kWave, 0, s>>>(c, nodes);
-    if (numTop)
+    // (point input: the emit already wrote the level-phase boxes)
+    if (numTop && !hc0.allPoints)
       for (int l = level - 1; l >= 0; l--) k_refit_top<<<64, 256, 0, s>>>(c, nodes, l);
     if (hipGetLastError() != hipSuccess) {
       mem_free(mem, nodes, s);

@@ -50,6 +50,7 @@ struct Counters {
   int32_t rootCb[6];  // encoded centroid bounds of all valid prims
   uint32_t allPoints; // every valid box is a point equal to its centre
   uint32_t nodeCount;
+  uint32_t numNodesOut;   // node count after the small phase
   uint32_t numLarge[2];
   uint32_t numSmall;
   uint32_t numFixup;
@@ -889,23 +890,43 @@ __global__ __launch_bounds__(kWave) void k_small(Ctx c) {
 
 // Exclusive scan of (count-1) over small segments: global id of local node
 // j >= 1 of segment s is smallBase[s] + j - 1.
-__global__ __launch_bounds__(1024) void k_small_scan(Ctx c) {
+// Global ids of the small segments' local nodes: an exclusive scan of
+// (local count - 1) over the segment list, kSmallScanTile segments per
+// workgroup (8 consecutive per thread) joined by decoupled look-back.  The
+// workgroups only wait on lower ones, which the dispatcher starts first.
+// The final node count goes to numNodesOut (every workgroup reads
+// nodeCount as its base).
+constexpr uint32_t kSmallScanPer = 8;
+constexpr uint32_t kSmallScanTile = 1024 * kSmallScanPer;
+__global__ __launch_bounds__(1024) void k_small_scan(Ctx c, uint32_t tag) {
   __shared__ uint32_t ss[1024 / kWave + 1];
+  __shared__ uint32_t sPrefix;
   const uint32_t nSmall = c.ctr->numSmall;
   const uint32_t base0 = c.ctr->nodeCount;
-  uint32_t run = 0;
-  for (uint32_t b0 = 0; b0 < nSmall; b0 += 1024) {
-    uint32_t i = b0 + threadIdx.x;
-    uint32_t v = i < nSmall ? c.smallCount[i] - 1 : 0;
-    uint32_t tot;
-    uint32_t ex = block_excl_scan<1024>(v, ss, &tot);
-    if (i < nSmall) c.smallBase[i] = base0 + run + ex;
-    run += tot;
+  const uint32_t i0 = blockIdx.x * kSmallScanTile + threadIdx.x * kSmallScanPer;
+  uint32_t v[kSmallScanPer], sum = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kSmallScanPer; j++) {
+    v[j] = i0 + j < nSmall ? c.smallCount[i0 + j] - 1 : 0u;
+    sum += v[j];
   }
-  if (threadIdx.x == 0) c.ctr->nodeCount = base0 + run;
+  uint32_t tot;
+  const uint32_t ex = block_excl_scan<1024, true>(sum, ss, &tot);
+  if (threadIdx.x < kWave) {
+    const uint32_t prefix = lookback_exclusive_wave(c.tileStatus, blockIdx.x, blockIdx.x, tag, tot);
+    if (threadIdx.x == 0) {
+      sPrefix = prefix;
+      if (blockIdx.x == gridDim.x - 1) c.ctr->numNodesOut = base0 + prefix + tot;
+    }
+  }
+  sync_block_lds();
+  uint32_t o = base0 + sPrefix + ex;
+#pragma unroll
+  for (uint32_t j = 0; j < kSmallScanPer; j++) {
+    if (i0 + j < nSmall) c.smallBase[i0 + j] = o;
+    o += v[j];
+  }
 }
-
-// Copy local nodes to their global ids, renumbering inner-node children.
 __global__ __launch_bounds__(kWave) void k_small_emit(Ctx c, Node *out) {
   const uint32_t nSmall = c.ctr->numSmall;
   for (uint32_t w = blockIdx.x; w < nSmall; w += gridDim.x) {
@@ -1067,13 +1088,14 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
     smallSlot = prof_begin(s);
     k_small<<<smallGrid, kWave, 0, s>>>(c);
     prof_end(smallSlot, s, "sm_small");
-    k_small_scan<<<1, 1024, 0, s>>>(c);
+    // (status tag beyond every level's)
+    k_small_scan<<<div_up(numSmall, kSmallScanTile), 1024, 0, s>>>(c, maxLevels + 2);
   }
   if (hipGetLastError() != hipSuccess) return fail((set_error("small-phase launch failed"), CUBQL_ERR_HIP));
   if (hipMemcpyAsync(hw, c.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return fail((set_error("final sync failed"), CUBQL_ERR_HIP));
-  const uint32_t numNodes = numValid ? ((const Counters *)hw)->nodeCount : 0;
+  const uint32_t numNodes = numValid ? (numSmall ? ((const Counters *)hw)->numNodesOut : topCount) : 0;
   if (prof_on()) {
     // algorithmic bytes: partition reads and writes one 16-byte item per
     // prim of the level; the small phase reads each item once, writes its

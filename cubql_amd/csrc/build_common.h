@@ -99,6 +99,42 @@ __device__ inline uint32_t lookback_exclusive_wave(uint64_t *status, uint32_t t,
   return prefix;
 }
 
+// Global ids for the local nodes of a list of finished segments: an
+// exclusive scan of (count[i] - 1) over i < n, offset by `base`, one
+// kCountScanTile-entry tile per 1024-thread workgroup (8 consecutive entries
+// per thread), joined by decoupled look-back on status[blockIdx.x] (with a
+// tag no other use of `status` shares; workgroups only wait on lower ones,
+// which the dispatcher starts first).  The last workgroup stores base +
+// total to *total.  smem: 1024/64 + 2 words.
+constexpr uint32_t kCountScanPer = 8;
+constexpr uint32_t kCountScanTile = 1024 * kCountScanPer;
+__device__ inline void count_scan_tile(const uint32_t *count, uint32_t *out, uint32_t n, uint32_t base,
+                                       uint64_t *status, uint32_t tag, uint32_t *total, uint32_t *smem) {
+  const uint32_t i0 = blockIdx.x * kCountScanTile + threadIdx.x * kCountScanPer;
+  uint32_t v[kCountScanPer], sum = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kCountScanPer; j++) {
+    v[j] = i0 + j < n ? count[i0 + j] - 1 : 0u;
+    sum += v[j];
+  }
+  uint32_t tot;
+  const uint32_t ex = block_excl_scan<1024, true>(sum, smem, &tot);
+  if (threadIdx.x < kWave) {
+    const uint32_t prefix = lookback_exclusive_wave(status, blockIdx.x, blockIdx.x, tag, tot);
+    if (threadIdx.x == 0) {
+      smem[1024 / kWave + 1] = prefix;
+      if (blockIdx.x == gridDim.x - 1) *total = base + prefix + tot;
+    }
+  }
+  sync_block_lds();
+  uint32_t o = base + smem[1024 / kWave + 1] + ex;
+#pragma unroll
+  for (uint32_t j = 0; j < kCountScanPer; j++) {
+    if (i0 + j < n) out[i0 + j] = o;
+    o += v[j];
+  }
+}
+
 // block-wide min/max reduction of 12 floats (left lo/hi, right lo/hi)
 template <int BLOCK, bool LDS_ONLY = false>
 __device__ inline void block_reduce_bounds(float (&v)[12], float *smem /*12*BLOCK/64*/) {

@@ -42,6 +42,7 @@ constexpr int kMaxLevels = 1024;
 // rebin builder only
 enum {
   C_NUMVALID = 0, C_NODECOUNT = 1, C_JOBS = 2, C_JOBPRIMS = 3, C_SMALL = 4, C_CURJOBS = 5,
+  C_NODESOUT = 6,  // node count after the thread phase
   C_LEVEL0 = 8, C_LSTART0 = C_LEVEL0 + kMaxLevels + 4
 };
 
@@ -66,6 +67,7 @@ struct RCtx {
   uint32_t *small;     // thread-phase nodes (<= 32 prims)
   uint32_t *smallCount, *smallBase;
   Node *loc;           // thread-phase local nodes at loc[2*begin + j]
+  uint64_t *scanStatus;  // look-back status of the thread-phase id scan
 };
 
 __global__ __launch_bounds__(kBlock) void k_rcount(RCtx c) {
@@ -708,20 +710,12 @@ __global__ __launch_bounds__(kBlock) void k_thread_phase(RCtx c, uint32_t *keys)
   }
 }
 
+// global ids of the thread-phase ranges' local nodes (the final node count
+// goes to C_NODESOUT: every workgroup reads C_NODECOUNT as its base)
 __global__ __launch_bounds__(1024) void k_thread_scan(RCtx c) {
-  __shared__ uint32_t ss[1024 / kWave + 1];
-  const uint32_t nSmall = c.ctr[C_SMALL];
-  const uint32_t base0 = c.ctr[C_NODECOUNT];
-  uint32_t run = 0;
-  for (uint32_t b0 = 0; b0 < nSmall; b0 += 1024) {
-    const uint32_t i = b0 + threadIdx.x;
-    const uint32_t v = i < nSmall ? c.smallCount[i] - 1 : 0;
-    uint32_t tot;
-    const uint32_t ex = block_excl_scan<1024>(v, ss, &tot);
-    if (i < nSmall) c.smallBase[i] = base0 + run + ex;
-    run += tot;
-  }
-  if (threadIdx.x == 0) c.ctr[C_NODECOUNT] = base0 + run;
+  __shared__ uint32_t ss[1024 / kWave + 2];
+  count_scan_tile(c.smallCount, c.smallBase, c.ctr[C_SMALL], c.ctr[C_NODECOUNT], c.scanStatus, 1u,
+                  &c.ctr[C_NODESOUT], ss);
 }
 
 __global__ __launch_bounds__(kWave) void k_thread_emit(RCtx c, Node *out) {
@@ -772,6 +766,8 @@ int rebin_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t
   CUBQL_CHECK(scr.alloc(&c.smallCount, (size_t)n + 1));
   CUBQL_CHECK(scr.alloc(&c.smallBase, (size_t)n + 1));
   CUBQL_CHECK(scr.alloc(&c.loc, 2 * (size_t)n + 64));
+  const uint32_t numScanTiles = div_up(n, kCountScanTile) + 1;
+  CUBQL_CHECK(scr.alloc(&c.scanStatus, numScanTiles));
   CUBQL_CHECK(scr.alloc(&rkA, n));
   CUBQL_CHECK(scr.alloc(&rkB, n));
   CUBQL_CHECK(scr.alloc(&rvA, n));
@@ -784,6 +780,7 @@ int rebin_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t
     return st;
   };
   k_rinit_levels<<<1, 256, 0, s>>>(c.ctr);
+  if (int fs = fill_u32((uint32_t *)c.scanStatus, 0u, 2 * (size_t)numScanTiles, s)) return fail(fs);
   int ks = prof_begin(s);
   k_rcount<<<numBlocks, kBlock, 0, s>>>(c);
   k_rscan<<<1, 1024, 0, s>>>(c, numBlocks);
@@ -843,14 +840,14 @@ int rebin_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t
   const uint32_t numSmall = hc[C_SMALL];
   if (numSmall) {
     k_thread_phase<<<std::max<uint32_t>(1u, div_up(numSmall, kBlock)), kBlock, 0, s>>>(c, keys);
-    k_thread_scan<<<1, 1024, 0, s>>>(c);
+    k_thread_scan<<<div_up(numSmall, kCountScanTile), 1024, 0, s>>>(c);
   }
   if (hipGetLastError() != hipSuccess) return fail((set_error("thread phase launch failed"), CUBQL_ERR_HIP));
   if (hipMemcpyAsync(hc.data(), c.ctr, sizeof(uint32_t) * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return fail((set_error("rebin sync failed"), CUBQL_ERR_HIP));
   const uint32_t numValid = hc[C_NUMVALID];
-  const uint32_t numNodes = numValid ? hc[C_NODECOUNT] : 0;
+  const uint32_t numNodes = numValid ? (numSmall ? hc[C_NODESOUT] : hc[C_NODECOUNT]) : 0;
   prof_set(hs, 8.0 * numNodes, numNodes);
   Node *nodes = nullptr;
   st = mem_alloc(mem, (void **)&nodes, std::max<size_t>(numNodes, 2) * sizeof(Node), s);

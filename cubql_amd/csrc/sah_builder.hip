@@ -63,6 +63,7 @@ struct Counters {
   uint32_t numValid;
   int32_t rootCb[6];
   uint32_t nodeCount;
+  uint32_t numNodesOut;  // node count after the small phase
   uint32_t numLarge[2];
   uint32_t numSmall;
   uint32_t ticket;
@@ -828,20 +829,12 @@ __global__ __launch_bounds__(kWave) void k_small(Ctx c) {
   }
 }
 
-__global__ __launch_bounds__(1024) void k_small_scan(Ctx c) {
-  __shared__ uint32_t ss[1024 / kWave + 1];
-  const uint32_t nSmall = c.ctr->numSmall;
-  const uint32_t base0 = c.ctr->nodeCount;
-  uint32_t run = 0;
-  for (uint32_t b0 = 0; b0 < nSmall; b0 += 1024) {
-    const uint32_t i = b0 + threadIdx.x;
-    const uint32_t v = i < nSmall ? c.smallCount[i] - 1 : 0;
-    uint32_t tot;
-    const uint32_t ex = block_excl_scan<1024>(v, ss, &tot);
-    if (i < nSmall) c.smallBase[i] = base0 + run + ex;
-    run += tot;
-  }
-  if (threadIdx.x == 0) c.ctr->nodeCount = base0 + run;
+// global ids of the small segments' local nodes (the final node count goes
+// to numNodesOut: every workgroup reads nodeCount as its base)
+__global__ __launch_bounds__(1024) void k_small_scan(Ctx c, uint32_t tag) {
+  __shared__ uint32_t ss[1024 / kWave + 2];
+  count_scan_tile(c.smallCount, c.smallBase, c.ctr->numSmall, c.ctr->nodeCount, c.tileStatus, tag,
+                  &c.ctr->numNodesOut, ss);
 }
 
 __global__ __launch_bounds__(kWave) void k_small_emit(Ctx c, Node *out) {
@@ -866,6 +859,7 @@ __global__ void k_copy_top(const Node *src, Node *dst, uint32_t n) {
 
 }  // namespace sah
 
+static_assert(sizeof(sah::Counters) <= 64 * sizeof(uint32_t), "host counter mirror");
 static uint32_t *sah_host_words() {
   static thread_local uint32_t *p = nullptr;
   if (!p && hipHostMalloc((void **)&p, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
@@ -985,13 +979,14 @@ int sah_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T
     smallSlot = prof_begin(s);
     k_small<<<smallGrid, kWave, small_lds_bytes(numBins), s>>>(c);
     prof_end(smallSlot, s, "sah_small");
-    k_small_scan<<<1, 1024, 0, s>>>(c);
+    // (status tag beyond every level's)
+    k_small_scan<<<div_up(numSmall, kCountScanTile), 1024, 0, s>>>(c, maxLevels + 2);
   }
   if (hipGetLastError() != hipSuccess) return fail((set_error("sah small launch failed"), CUBQL_ERR_HIP));
   if (hipMemcpyAsync(hw, c.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return fail((set_error("sah final sync failed"), CUBQL_ERR_HIP));
-  const uint32_t numNodes = numValid ? ((const Counters *)hw)->nodeCount : 0;
+  const uint32_t numNodes = numValid ? (numSmall ? ((const Counters *)hw)->numNodesOut : topCount) : 0;
   if (prof_on()) {
     // algorithmic bytes: binning reads one 32-byte item per prim of the
     // level; the partition reads and writes it; the small phase reads each

@@ -890,42 +890,12 @@ __global__ __launch_bounds__(kWave) void k_small(Ctx c) {
 
 // Exclusive scan of (count-1) over small segments: global id of local node
 // j >= 1 of segment s is smallBase[s] + j - 1.
-// Global ids of the small segments' local nodes: an exclusive scan of
-// (local count - 1) over the segment list, kSmallScanTile segments per
-// workgroup (8 consecutive per thread) joined by decoupled look-back.  The
-// workgroups only wait on lower ones, which the dispatcher starts first.
-// The final node count goes to numNodesOut (every workgroup reads
-// nodeCount as its base).
-constexpr uint32_t kSmallScanPer = 8;
-constexpr uint32_t kSmallScanTile = 1024 * kSmallScanPer;
+// global ids of the small segments' local nodes (the final node count goes
+// to numNodesOut: every workgroup reads nodeCount as its base)
 __global__ __launch_bounds__(1024) void k_small_scan(Ctx c, uint32_t tag) {
-  __shared__ uint32_t ss[1024 / kWave + 1];
-  __shared__ uint32_t sPrefix;
-  const uint32_t nSmall = c.ctr->numSmall;
-  const uint32_t base0 = c.ctr->nodeCount;
-  const uint32_t i0 = blockIdx.x * kSmallScanTile + threadIdx.x * kSmallScanPer;
-  uint32_t v[kSmallScanPer], sum = 0;
-#pragma unroll
-  for (uint32_t j = 0; j < kSmallScanPer; j++) {
-    v[j] = i0 + j < nSmall ? c.smallCount[i0 + j] - 1 : 0u;
-    sum += v[j];
-  }
-  uint32_t tot;
-  const uint32_t ex = block_excl_scan<1024, true>(sum, ss, &tot);
-  if (threadIdx.x < kWave) {
-    const uint32_t prefix = lookback_exclusive_wave(c.tileStatus, blockIdx.x, blockIdx.x, tag, tot);
-    if (threadIdx.x == 0) {
-      sPrefix = prefix;
-      if (blockIdx.x == gridDim.x - 1) c.ctr->numNodesOut = base0 + prefix + tot;
-    }
-  }
-  sync_block_lds();
-  uint32_t o = base0 + sPrefix + ex;
-#pragma unroll
-  for (uint32_t j = 0; j < kSmallScanPer; j++) {
-    if (i0 + j < nSmall) c.smallBase[i0 + j] = o;
-    o += v[j];
-  }
+  __shared__ uint32_t ss[1024 / kWave + 2];
+  count_scan_tile(c.smallCount, c.smallBase, c.ctr->numSmall, c.ctr->nodeCount, c.tileStatus, tag,
+                  &c.ctr->numNodesOut, ss);
 }
 __global__ __launch_bounds__(kWave) void k_small_emit(Ctx c, Node *out) {
   const uint32_t nSmall = c.ctr->numSmall;
@@ -974,6 +944,7 @@ __global__ void k_init_counters(Counters *k) {
 }  // namespace sm
 
 // pinned host mirror for the few completion checks
+static_assert(sizeof(sm::Counters) <= 64 * sizeof(uint32_t), "host counter mirror");
 static uint32_t *host_words() {
   static thread_local uint32_t *p = nullptr;
   if (!p) {
@@ -1089,7 +1060,7 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
     k_small<<<smallGrid, kWave, 0, s>>>(c);
     prof_end(smallSlot, s, "sm_small");
     // (status tag beyond every level's)
-    k_small_scan<<<div_up(numSmall, kSmallScanTile), 1024, 0, s>>>(c, maxLevels + 2);
+    k_small_scan<<<div_up(numSmall, kCountScanTile), 1024, 0, s>>>(c, maxLevels + 2);
   }
   if (hipGetLastError() != hipSuccess) return fail((set_error("small-phase launch failed"), CUBQL_ERR_HIP));
   if (hipMemcpyAsync(hw, c.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess ||

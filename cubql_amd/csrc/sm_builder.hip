@@ -53,11 +53,10 @@ struct Counters {
   uint32_t numNodesOut;   // node count after the small phase
   uint32_t numLarge[2];
   uint32_t numSmall;
-  uint32_t numFixup;
   uint32_t ticket;
   uint32_t numTiles;
   uint32_t numTop;    // entries in the top-node list
-  uint32_t baseNode, baseSmall, baseTop;  // this level's bases (emit)
+  uint32_t baseNode[2], baseSmall[2], baseTop[2];  // each level's bases, by level parity (emit)
 };
 
 struct Seg {          // an open node with > kSmall prims
@@ -94,12 +93,10 @@ struct Ctx {
   uint32_t *tileSeg[2];    // [maxTiles] segment of each partition tile
   int32_t *segDim[2];      // [maxLarge] split plane
   float *segPos[2];
-  uint4 *emitOff;          // [maxLarge] in-block offsets (large|small|fix, tiles, -, -)
-  uint4 *blockSum;         // [emit blocks] block totals (large|small|fix, tiles, items, -)
-  uint4 *blockOff;         // [emit blocks] exclusive prefix of blockSum
+  uint64_t *emitStatus;    // [4][maxEmitBlocks] emit look-back status per count
+  uint32_t maxEmitBlocks;
   uint64_t *tileStatus;    // [maxTiles]
   SmallSeg *small;         // [numValid]
-  uint32_t *fixup;         // [maxLarge] indices into next-level segs
   uint32_t *topNodes;      // [numValid] inner nodes of the level phase
   uint32_t *levelStart;    // [maxLevels+1] start of each level in topNodes
   uint32_t *levelItems;    // [maxLevels] prims moved by each level's partition
@@ -183,7 +180,6 @@ __global__ __launch_bounds__(1024) void k_count_scan(Ctx c, uint32_t numBlocks) 
     k->nodeCount = 2;
     k->numLarge[0] = k->numLarge[1] = 0;
     k->numSmall = 0;
-    k->numFixup = 0;
     k->numTop = 0;
     c.levelStart[0] = 0;
     Node z;
@@ -220,6 +216,10 @@ __global__ __launch_bounds__(1024) void k_count_scan(Ctx c, uint32_t numBlocks) 
       c.small[0] = s;
       k->numSmall = 1;
     }
+    // level 0's bases for the emit
+    k->baseNode[0] = k->nodeCount;
+    k->baseSmall[0] = k->numSmall;
+    k->baseTop[0] = k->numTop;
   }
   __syncthreads();
   const uint32_t tiles = c.ctr->numTiles;
@@ -449,82 +449,96 @@ __device__ __forceinline__ void emit_counts(const EmitSeg &e, uint32_t &lsf, uin
   }
 }
 
-__global__ __launch_bounds__(kEmitBlock) void k_emit_count(Ctx c, int level) {
+// One emit launch per level: each workgroup classifies kEmitBlock segments,
+// scans their counts (next-level large segments, small segments, tiles,
+// prims) within the workgroup, gets its offsets from the workgroups before
+// it by decoupled look-back (one wave per count; workgroups only wait on
+// lower ones, which the dispatcher starts first), and writes the node
+// records, the next level's segments and plan, the tile->segment map and
+// the small-segment list.  Children of one-sided (degenerate) splits are
+// halves of the node's range whose centroid bounds the partition did not
+// see: the workgroup reduces them from the prims and plans them itself.
+// The last workgroup writes the level totals and the next level's bases
+// (into the other parity's slots) and clears this level's segment count;
+// every workgroup has read that count before it published, and the last
+// one has waited for all of them.  A level with no segments only carries
+// its bookkeeping forward.
+__global__ __launch_bounds__(kEmitBlock) void k_emit(Ctx c, int level) {
   __shared__ uint32_t ss[kEmitBlock / kWave + 1];
-  const int par = level & 1;
-  const uint32_t n = c.ctr->numLarge[par];
-  const uint32_t s = blockIdx.x * kEmitBlock + threadIdx.x;
+  __shared__ uint32_t sPre[4];
+  __shared__ uint3 sFix[2 * kEmitBlock];  // (segment, begin, end) of one-sided halves
+  __shared__ uint32_t sNumFix;
+  __shared__ float sb[12 * (kEmitBlock / kWave)];
+  const int par = level & 1, np = par ^ 1;
+  Counters *k = c.ctr;
+  // (read once: the last workgroup clears it)
+  const uint32_t n = __hip_atomic_load(&k->numLarge[par], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (n == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      k->numLarge[np] = 0;
+      k->numTiles = 0;
+      k->baseNode[np] = k->baseNode[par];
+      k->baseSmall[np] = k->baseSmall[par];
+      k->baseTop[np] = k->baseTop[par];
+      c.levelStart[level + 1] = k->baseTop[par];
+      c.levelItems[level + 1] = 0;
+    }
+    return;
+  }
   if (blockIdx.x * kEmitBlock >= n) return;
-  uint32_t lsf = 0, tiles = 0, items = 0;
-  if (s < n) {
-    EmitSeg e;
-    emit_classify(c, par, s, e);
-    emit_counts(e, lsf, tiles, items);
-  }
-  uint32_t totLsf, totT, totI;
-  const uint32_t exLsf = block_excl_scan<kEmitBlock>(lsf, ss, &totLsf);
-  const uint32_t exT = block_excl_scan<kEmitBlock>(tiles, ss, &totT);
-  block_excl_scan<kEmitBlock>(items, ss, &totI);
-  if (s < n) c.emitOff[s] = make_uint4(exLsf, exT, 0u, 0u);
-  if (threadIdx.x == 0) c.blockSum[blockIdx.x] = make_uint4(totLsf, totT, totI, 0u);
-}
-
-__global__ __launch_bounds__(1024) void k_emit_scan(Ctx c, int level) {
-  __shared__ uint32_t ss[1024 / kWave + 1];
-  const int par = level & 1;
-  const uint32_t n = c.ctr->numLarge[par];
   const uint32_t nb = div_up(n, kEmitBlock);
-  uint32_t runL = 0, runS = 0, runF = 0, runT = 0, runI = 0;
-  for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
-    const uint32_t b = b0 + threadIdx.x;
-    uint4 v = b < nb ? c.blockSum[b] : make_uint4(0u, 0u, 0u, 0u);
-    uint32_t tL, tS, tF, tT, tI;
-    const uint32_t eL = block_excl_scan<1024>(v.x & 1023u, ss, &tL);
-    const uint32_t eS = block_excl_scan<1024>((v.x >> 10) & 1023u, ss, &tS);
-    const uint32_t eF = block_excl_scan<1024>(v.x >> 20, ss, &tF);
-    const uint32_t eT = block_excl_scan<1024>(v.y, ss, &tT);
-    block_excl_scan<1024>(v.z, ss, &tI);
-    if (b < nb) c.blockOff[b] = make_uint4(runL + eL, runS + eS, runF + eF, runT + eT);
-    runL += tL; runS += tS; runF += tF; runT += tT; runI += tI;
+  const uint32_t s = blockIdx.x * kEmitBlock + threadIdx.x;
+  const uint32_t nodeBase = k->baseNode[par], smallBase = k->baseSmall[par], topBase = k->baseTop[par];
+  const bool pts = k->allPoints != 0;
+  const uint32_t tag = (uint32_t)level + 1;
+  EmitSeg e;
+  uint32_t nL = 0, nS = 0, nT = 0, nI = 0;
+  if (s < n) {
+    emit_classify(c, par, s, e);
+    for (int side = 0; side < 2; side++) {
+      if (e.large[side]) {
+        nL++;
+        nT += div_up(e.ce[side] - e.cb[side], kTile);
+        nI += e.ce[side] - e.cb[side];
+      } else {
+        nS++;
+      }
+    }
   }
-  if (threadIdx.x == 0) {
-    Counters *k = c.ctr;
-    k->baseNode = k->nodeCount;
-    k->baseSmall = k->numSmall;
-    k->baseTop = k->numTop;
-    k->nodeCount += 2 * n;
-    k->numSmall += runS;
-    k->numLarge[par ^ 1] = runL;
-    k->numFixup = runF;
-    k->numTop += n;
-    c.levelStart[level + 1] = k->numTop;
+  if (threadIdx.x == 0) sNumFix = 0;
+  uint32_t tL, tS, tT, tI;
+  const uint32_t exL = block_excl_scan<kEmitBlock, true>(nL, ss, &tL);
+  const uint32_t exS = block_excl_scan<kEmitBlock, true>(nS, ss, &tS);
+  const uint32_t exT = block_excl_scan<kEmitBlock, true>(nT, ss, &tT);
+  const uint32_t exI = block_excl_scan<kEmitBlock, true>(nI, ss, &tI);
+  const uint32_t w = threadIdx.x / kWave;
+  if (w < 4) {
+    const uint32_t v = w == 0 ? tL : (w == 1 ? tS : (w == 2 ? tT : tI));
+    const uint32_t pre =
+        lookback_exclusive_wave(c.emitStatus + (size_t)w * c.maxEmitBlocks, blockIdx.x, blockIdx.x, tag, v);
+    if (lane_id() == 0) sPre[w] = pre;
+  }
+  sync_block_lds();
+  const uint32_t preL = sPre[0], preS = sPre[1], preT = sPre[2], preI = sPre[3];
+  if (blockIdx.x == nb - 1 && threadIdx.x == 0) {
+    const uint32_t runL = preL + tL, runS = preS + tS, runT = preT + tT, runI = preI + tI;
+    k->nodeCount = k->baseNode[np] = nodeBase + 2 * n;
+    k->numSmall = k->baseSmall[np] = smallBase + runS;
+    k->numTop = k->baseTop[np] = topBase + n;
+    k->numLarge[np] = runL;
+    k->numLarge[par] = 0;
+    c.levelStart[level + 1] = topBase + n;
     k->numTiles = runT;
     k->ticket = 0;
     for (uint32_t cc = 0; cc < kTicketCounters; cc++) c.tickets[kTicketStride * cc] = 0;
-    c.tileStart[par ^ 1][runL] = runT;
+    c.tileStart[np][runL] = runT;
     c.levelItems[level + 1] = runI;
   }
-}
-
-__global__ __launch_bounds__(kEmitBlock) void k_emit_write(Ctx c, int level) {
-  const int par = level & 1, np = par ^ 1;
-  const uint32_t n = c.ctr->numLarge[par];
-  const uint32_t s = blockIdx.x * kEmitBlock + threadIdx.x;
-  // waves with no segment leave; in the others every lane stays for the
-  // wave-cooperative tile-map fill below
-  if (blockIdx.x * kEmitBlock + (threadIdx.x & ~(uint32_t)(kWave - 1)) >= n) return;
   uint32_t fillStart[2] = {0, 0}, fillCount[2] = {0, 0}, fillSeg[2] = {0, 0};
   if (s < n) {
-    const Counters *k = c.ctr;
-    const uint32_t nodeBase = k->baseNode, smallBase = k->baseSmall, topBase = k->baseTop;
-    EmitSeg e;
-    emit_classify(c, par, s, e);
-    const uint4 bo = c.blockOff[blockIdx.x];
-    const uint4 eo = c.emitOff[s];
-    uint32_t li = bo.x + (eo.x & 1023u), si = bo.y + ((eo.x >> 10) & 1023u), fi = bo.z + (eo.x >> 20);
-    uint32_t ti = bo.w + eo.y;
+    uint32_t li = preL + exL, si = preS + exS, ti = preT + exT;
     const uint32_t child = nodeBase + 2 * s;
-    if (k->allPoints) {
+    if (pts) {
       // point input: the node's box is its centroid bounds, final here
       Node nd;
       for (int d = 0; d < 3; d++) { nd.lo[d] = dec_f(e.g.cb[d]); nd.hi[d] = dec_f(e.g.cb[d + 3]); }
@@ -543,13 +557,13 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_write(Ctx c, int level) {
         for (int j = 0; j < 6; j++) {
           int32_t v = c.childCb[par][(2 * s + side) * 6 + j];
           for (uint32_t cp = 1; cp < kCbCopies; cp++) {
-            const int32_t w = c.childCb[par][((size_t)cp * 2 * c.maxLarge + 2 * s + side) * 6 + j];
-            v = j < 3 ? min(v, w) : max(v, w);
+            const int32_t w2 = c.childCb[par][((size_t)cp * 2 * c.maxLarge + 2 * s + side) * 6 + j];
+            v = j < 3 ? min(v, w2) : max(v, w2);
           }
           ns.cb[j] = v;
         }
         next[li] = ns;
-        // plan of the next level (one-sided children: k_fixup redoes the plane)
+        // plan of the next level (one-sided children are planned below)
         float lo[3], hi[3];
         for (int d = 0; d < 3; d++) { lo[d] = dec_f(ns.cb[d]); hi[d] = dec_f(ns.cb[d + 3]); }
         int dim;
@@ -567,7 +581,7 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_write(Ctx c, int level) {
         fillCount[side] = tiles;
         fillSeg[side] = li;
         ti += tiles;
-        if (e.oneSided) c.fixup[fi++] = li;
+        if (e.oneSided) sFix[atomicAdd(&sNumFix, 1u)] = make_uint3(li, ns.begin, ns.end);
         li++;
       } else {
         c.small[smallBase + si] = SmallSeg{e.cb[side], e.ce[side], child + side, (uint32_t)np};
@@ -591,37 +605,32 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_write(Ctx c, int level) {
       for (uint32_t t = lane; t < cnt; t += kWave) c.tileSeg[np][st + t] = sg;
     }
   }
-}
-
-// Fixup: exact centroid bounds and split planes of large children of
-// one-sided splits.  Also closes the level (its segment count drops to 0).
-__global__ __launch_bounds__(kBlock) void k_fixup(Ctx c, int level) {
-  __shared__ float sb[12 * (kBlock / kWave)];
-  const int par = level & 1;
-  const uint32_t nf = c.ctr->numFixup;
-  if (blockIdx.x == 0 && threadIdx.x == 0) c.ctr->numLarge[par] = 0;
-  Seg *next = c.segs[par ^ 1];
-  const float4 *items = c.items[par ^ 1];
-  for (uint32_t f = blockIdx.x; f < nf; f += gridDim.x) {
-    Seg &g = next[c.fixup[f]];
+  // one-sided splits: centroid bounds and plane of each half from its prims
+  // (a full barrier: the segment records above must land before these
+  // overwrite their bounds)
+  __syncthreads();
+  const uint32_t nFix = sNumFix;
+  const float4 *items = c.items[np];
+  for (uint32_t f = 0; f < nFix; f++) {
+    const uint32_t li = sFix[f].x, gb = sFix[f].y, ge = sFix[f].z;
+    Seg &g = c.segs[np][li];
     float v[12];
     for (int j = 0; j < 3; j++) { v[j] = INFINITY; v[j + 3] = -INFINITY; v[j + 6] = INFINITY; v[j + 9] = -INFINITY; }
-    for (uint32_t i = g.begin + threadIdx.x; i < g.end; i += kBlock) {
-      float4 it = items[i];
+    for (uint32_t i = gb + threadIdx.x; i < ge; i += kEmitBlock) {
+      const float4 it = items[i];
       v[0] = fminf(v[0], it.x); v[1] = fminf(v[1], it.y); v[2] = fminf(v[2], it.z);
       v[3] = fmaxf(v[3], it.x); v[4] = fmaxf(v[4], it.y); v[5] = fmaxf(v[5], it.z);
     }
-    block_reduce_bounds<kBlock>(v, sb);
+    block_reduce_bounds<kEmitBlock>(v, sb);
     if (threadIdx.x < 6) g.cb[threadIdx.x] = enc_f(v[threadIdx.x]);
     if (threadIdx.x == 0) {
       const float lo[3] = {v[0], v[1], v[2]}, hi[3] = {v[3], v[4], v[5]};
       int dim;
       float pos;
       choose_split(lo, hi, dim, pos);
-      c.segDim[par ^ 1][c.fixup[f]] = dim;
-      c.segPos[par ^ 1][c.fixup[f]] = pos;
+      c.segDim[np][li] = dim;
+      c.segPos[np][li] = pos;
     }
-    __syncthreads();
   }
 }
 
@@ -981,13 +990,11 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
     CUBQL_CHECK(scr.alloc(&c.segDim[p], c.maxLarge));
     CUBQL_CHECK(scr.alloc(&c.segPos[p], c.maxLarge));
   }
-  CUBQL_CHECK(scr.alloc(&c.emitOff, c.maxLarge));
+  c.maxEmitBlocks = maxEmitBlocks;
+  CUBQL_CHECK(scr.alloc(&c.emitStatus, 4 * (size_t)maxEmitBlocks));
   CUBQL_CHECK(scr.alloc(&c.tickets, kTicketStride * kTicketCounters));
-  CUBQL_CHECK(scr.alloc(&c.blockSum, maxEmitBlocks + 1));
-  CUBQL_CHECK(scr.alloc(&c.blockOff, maxEmitBlocks + 1));
   CUBQL_CHECK(scr.alloc(&c.tileStatus, maxTiles));
   CUBQL_CHECK(scr.alloc(&c.small, n + 1));
-  CUBQL_CHECK(scr.alloc(&c.fixup, c.maxLarge));
   CUBQL_CHECK(scr.alloc(&c.topNodes, n + 1));
   CUBQL_CHECK(scr.alloc(&c.levelStart, maxLevels + 2));
   CUBQL_CHECK(scr.alloc(&c.levelItems, maxLevels + 2));
@@ -1006,6 +1013,7 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
 
   k_init_counters<<<1, 64, 0, s>>>(c.ctr);
   int st = fill_u32((uint32_t *)c.tileStatus, 0u, 2 * (size_t)maxTiles, s);
+  if (!st) st = fill_u32((uint32_t *)c.emitStatus, 0u, 8 * (size_t)maxEmitBlocks, s);
   if (st) return fail(st);
   k_count<<<numBlocks, kBlock, 0, s>>>(c);
   k_count_scan<<<1, 1024, 0, s>>>(c, numBlocks);
@@ -1033,17 +1041,15 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
       k_partition<<<partGrid, kBlock, 0, s>>>(c, level);
       prof_end(ps, s, "sm_partition");
       if (ps >= 0) partSlots.push_back({ps, level});
-      k_emit_count<<<maxEmitBlocks, kEmitBlock, 0, s>>>(c, level);
-      k_emit_scan<<<1, 1024, 0, s>>>(c, level);
-      k_emit_write<<<maxEmitBlocks, kEmitBlock, 0, s>>>(c, level);
-      k_fixup<<<64, kBlock, 0, s>>>(c, level);
+      k_emit<<<maxEmitBlocks, kEmitBlock, 0, s>>>(c, level);
     }
     if (hipGetLastError() != hipSuccess) return fail((set_error("level launch failed"), CUBQL_ERR_HIP));
     if (hipMemcpyAsync(hw, c.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       return fail((set_error("level sync failed"), CUBQL_ERR_HIP));
     const Counters *hc = (const Counters *)hw;
-    if (hc->numLarge[0] == 0 && hc->numLarge[1] == 0) break;
+    // open segments of the next level to run
+    if (hc->numLarge[level & 1] == 0) break;
     if (level >= (int)maxLevels) return fail((set_error("tree deeper than %u levels", maxLevels), CUBQL_ERR_INTERNAL));
   }
   const Counters hc0 = *(const Counters *)hw;

@@ -286,6 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
   __shared__ uint32_t sPrefix;
   const int par = level & 1;
   const uint32_t numTiles = c.ctr->numTiles;
+  if (numTiles == 0) return;  // a level past the tree's depth
   const float4 *src = c.items[par];
   float4 *dst = c.items[par ^ 1];
   const uint32_t tag = (uint32_t)level + 1;
@@ -1035,8 +1036,14 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
       std::max<uint32_t>(kTicketCounters, std::min<uint32_t>(maxTiles, (uint32_t)numCU * (uint32_t)partPerCU));
   int level = 0;
   std::vector<std::pair<int, int>> partSlots;  // (profile slot, level)
+  // The first batch covers the depth a well-spread input needs (halving
+  // down to kSmall prims, plus one level of slack): running a level past the
+  // tree's depth costs two near-empty launches, a host check ~20 us.
+  int batch = 1;
+  for (uint64_t m = (uint64_t)kSmall; m < n; m *= 2) batch++;
+  batch = std::max(kLevelsPerSync, batch);
   while (true) {
-    for (int i = 0; i < kLevelsPerSync && level < (int)maxLevels; i++, level++) {
+    for (int i = 0; i < batch && level < (int)maxLevels; i++, level++) {
       int ps = prof_begin(s);
       k_partition<<<partGrid, kBlock, 0, s>>>(c, level);
       prof_end(ps, s, "sm_partition");
@@ -1050,6 +1057,7 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
     const Counters *hc = (const Counters *)hw;
     // open segments of the next level to run
     if (hc->numLarge[level & 1] == 0) break;
+    batch = kLevelsPerSync;
     if (level >= (int)maxLevels) return fail((set_error("tree deeper than %u levels", maxLevels), CUBQL_ERR_INTERNAL));
   }
   const Counters hc0 = *(const Counters *)hw;

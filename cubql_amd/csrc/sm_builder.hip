@@ -105,6 +105,8 @@ struct Ctx {
 
 // ---------------------------------------------------------------- init pass
 // Pass 1: per-tile valid count, root centroid bounds, point-data flag.
+// It also writes every prim's item in place, which is the compacted item
+// array whenever all boxes are valid; otherwise pass 3 rewrites it.
 __global__ __launch_bounds__(kBlock) void k_count(Ctx c) {
   __shared__ float sb[12 * (kBlock / kWave)];
   __shared__ uint32_t ss[kBlock / kWave + 1];
@@ -118,13 +120,15 @@ __global__ __launch_bounds__(kBlock) void k_count(Ctx c) {
     uint32_t i = base + k * kBlock + threadIdx.x;
     if (i < c.n) {
       cubql_box3f b = load_box(c.boxes, i);
+      float ce[3];
+      for (int d = 0; d < 3; d++) ce[d] = (b.lower[d] + b.upper[d]) / 2.f;
+      c.items[0][i] = make_float4(ce[0], ce[1], ce[2], __uint_as_float(i));
       if (box_is_valid(b)) {
         cnt++;
         for (int d = 0; d < 3; d++) {
-          float ce = (b.lower[d] + b.upper[d]) / 2.f;
-          v[d] = fminf(v[d], ce);
-          v[d + 3] = fmaxf(v[d + 3], ce);
-          pts = pts && (b.lower[d] == b.upper[d]) && (ce == b.lower[d]);
+          v[d] = fminf(v[d], ce[d]);
+          v[d + 3] = fmaxf(v[d + 3], ce[d]);
+          pts = pts && (b.lower[d] == b.upper[d]) && (ce[d] == b.lower[d]);
         }
       }
     }
@@ -226,8 +230,10 @@ __global__ __launch_bounds__(1024) void k_count_scan(Ctx c, uint32_t numBlocks) 
   for (uint32_t t = threadIdx.x; t < tiles; t += 1024) c.tileSeg[0][t] = 0;
 }
 
-// Pass 3: write compacted items (ascending primID) and the invalid tail.
+// Pass 3: write compacted items (ascending primID) and the invalid tail;
+// nothing to do when every box was valid.
 __global__ __launch_bounds__(kBlock) void k_write_items(Ctx c) {
+  if (c.ctr->numValid == c.n) return;
   __shared__ uint32_t sr[kItems * (kBlock / kWave) + 1];
   const uint32_t base = blockIdx.x * kTile;
   bool valid[kItems];

@@ -161,29 +161,6 @@ __device__ __forceinline__ void bin_add(int32_t *bins, int nb, const Item &it, c
   }
 }
 
-// Inclusive prefix scans with DPP (row_shr 1,2,4,8 inside rows, then
-// row_bcast 15, and 31 unless the two 32-lane halves scan independently),
-// on encoded bounds (enc_f keeps float order, so integer min/max equal the
-// float ones) and counts: with an identity as the old value each step is
-// one fused v_min/v_max/v_add_*_dpp.
-template <int OP, bool CROSS32>  // OP: 0 min, 1 max, 2 add
-__device__ __forceinline__ int32_t dpp_scan_i(int32_t v) {
-  const int32_t I = OP == 0 ? INT32_MAX : (OP == 1 ? INT32_MIN : 0);
-#define CUBQL_DPP_STEP(CTRL, ROWMASK)                                                        \
-  {                                                                                          \
-    const int32_t t = __builtin_amdgcn_update_dpp(I, v, CTRL, ROWMASK, 0xF, false);          \
-    v = OP == 0 ? min(v, t) : (OP == 1 ? max(v, t) : v + t);                                 \
-  }
-  CUBQL_DPP_STEP(0x111, 0xF)
-  CUBQL_DPP_STEP(0x112, 0xF)
-  CUBQL_DPP_STEP(0x114, 0xF)
-  CUBQL_DPP_STEP(0x118, 0xF)
-  CUBQL_DPP_STEP(0x142, 0xA)
-  if (CROSS32) CUBQL_DPP_STEP(0x143, 0xC)
-#undef CUBQL_DPP_STEP
-  return v;
-}
-
 // Split costs for the candidates held by this lane's (dim, bin): the left
 // side is the prefix scan over bins shifted by one lane, the right side the
 // prefix scan over the reversed bins read back from lane rev.

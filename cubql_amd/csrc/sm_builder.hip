@@ -663,22 +663,30 @@ __device__ __forceinline__ uint32_t fr_pack(uint32_t b, uint32_t e, uint32_t me)
 // ends with the reduction over [head(l), l].  Steps are the DPP wave scan
 // (row_shr 1,2,4,8 inside rows of 16, then row_bcast 15 and 31 across rows);
 // cond[k] says whether step k's source lane lies in the lane's segment.
+// (on enc_cmp-encoded values: integer min/max with an identity as the DPP
+// "old" value is one fused v_min/v_max_i32_dpp per step, plus the select)
 template <bool MIN>
-__device__ __forceinline__ float seg_scan(float v, const bool (&cond)[6]) {
-  const float I = MIN ? INFINITY : -INFINITY;
-  float t;
-  t = dpp_f<0x111>(I, v); v = cond[0] ? (MIN ? fminf(v, t) : fmaxf(v, t)) : v;
-  t = dpp_f<0x112>(I, v); v = cond[1] ? (MIN ? fminf(v, t) : fmaxf(v, t)) : v;
-  t = dpp_f<0x114>(I, v); v = cond[2] ? (MIN ? fminf(v, t) : fmaxf(v, t)) : v;
-  t = dpp_f<0x118>(I, v); v = cond[3] ? (MIN ? fminf(v, t) : fmaxf(v, t)) : v;
-  t = dpp_f<0x142, 0xA>(I, v); v = cond[4] ? (MIN ? fminf(v, t) : fmaxf(v, t)) : v;
-  t = dpp_f<0x143, 0xC>(I, v); v = cond[5] ? (MIN ? fminf(v, t) : fmaxf(v, t)) : v;
+__device__ __forceinline__ int32_t seg_scan(int32_t v, const bool (&cond)[6]) {
+  const int32_t I = MIN ? INT32_MAX : INT32_MIN;
+#define CUBQL_SEG_STEP(K, CTRL, ROWMASK)                                                     \
+  {                                                                                          \
+    const int32_t t = __builtin_amdgcn_update_dpp(I, v, CTRL, ROWMASK, 0xF, false);          \
+    const int32_t m = MIN ? min(v, t) : max(v, t);                                           \
+    v = cond[K] ? m : v;                                                                     \
+  }
+  CUBQL_SEG_STEP(0, 0x111, 0xF)
+  CUBQL_SEG_STEP(1, 0x112, 0xF)
+  CUBQL_SEG_STEP(2, 0x114, 0xF)
+  CUBQL_SEG_STEP(3, 0x118, 0xF)
+  CUBQL_SEG_STEP(4, 0x142, 0xA)
+  CUBQL_SEG_STEP(5, 0x143, 0xC)
+#undef CUBQL_SEG_STEP
   return v;
 }
 
 // Reduction of v over the lane's segment [sb, se), in every lane of it.
 template <bool MIN>
-__device__ __forceinline__ float seg_reduce(float v, const bool (&cond)[6], uint32_t last) {
+__device__ __forceinline__ int32_t seg_reduce(int32_t v, const bool (&cond)[6], uint32_t last) {
   return __shfl(seg_scan<MIN>(v, cond), (int)last, kWave);
 }
 
@@ -709,27 +717,35 @@ __device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, 
   bool active = lane < cnt;
   uint32_t sb = active ? 0u : lane, se = active ? cnt : lane + 1u;
   uint32_t me = rootLocal;
+  // coordinates as enc_cmp integers: bounds by fused integer DPP scans,
+  // and the plane test in the same order as the float compare
+  int32_t xe = enc_cmp(x), ye = enc_cmp(y), ze = enc_cmp(z);
+  int32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0, b4 = 0, b5 = 0;
+  if (!PTS) {
+    b0 = enc_cmp(bx0); b1 = enc_cmp(by0); b2 = enc_cmp(bz0);
+    b3 = enc_cmp(bx1); b4 = enc_cmp(by1); b5 = enc_cmp(bz1);
+  }
   while (__ballot(active)) {
     const int h = (int)sb, l = (int)lane;
     const bool cond[6] = {l - 1 >= h, l - 2 >= h, l - 4 >= h, l - 8 >= h,
                           (int)(lane & ~15u) - 1 >= h, lane >= 32u && 31 >= h};
     const uint32_t last = se - 1;
     float clo[3], chi[3], nlo[3], nhi[3];
-    clo[0] = seg_reduce<true>(x, cond, last);
-    clo[1] = seg_reduce<true>(y, cond, last);
-    clo[2] = seg_reduce<true>(z, cond, last);
-    chi[0] = seg_reduce<false>(x, cond, last);
-    chi[1] = seg_reduce<false>(y, cond, last);
-    chi[2] = seg_reduce<false>(z, cond, last);
+    clo[0] = dec_f(seg_reduce<true>(xe, cond, last));
+    clo[1] = dec_f(seg_reduce<true>(ye, cond, last));
+    clo[2] = dec_f(seg_reduce<true>(ze, cond, last));
+    chi[0] = dec_f(seg_reduce<false>(xe, cond, last));
+    chi[1] = dec_f(seg_reduce<false>(ye, cond, last));
+    chi[2] = dec_f(seg_reduce<false>(ze, cond, last));
     if (PTS) {
       for (int d = 0; d < 3; d++) { nlo[d] = clo[d]; nhi[d] = chi[d]; }
     } else {
-      nlo[0] = seg_reduce<true>(bx0, cond, last);
-      nlo[1] = seg_reduce<true>(by0, cond, last);
-      nlo[2] = seg_reduce<true>(bz0, cond, last);
-      nhi[0] = seg_reduce<false>(bx1, cond, last);
-      nhi[1] = seg_reduce<false>(by1, cond, last);
-      nhi[2] = seg_reduce<false>(bz1, cond, last);
+      nlo[0] = dec_f(seg_reduce<true>(b0, cond, last));
+      nlo[1] = dec_f(seg_reduce<true>(b1, cond, last));
+      nlo[2] = dec_f(seg_reduce<true>(b2, cond, last));
+      nhi[0] = dec_f(seg_reduce<false>(b3, cond, last));
+      nhi[1] = dec_f(seg_reduce<false>(b4, cond, last));
+      nhi[2] = dec_f(seg_reduce<false>(b5, cond, last));
     }
     const uint32_t n = se - sb;
     const bool isHead = lane == sb;
@@ -742,8 +758,8 @@ __device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, 
     int dim;
     float pos;
     choose_split(clo, chi, dim, pos);
-    const float v = dim == 0 ? x : (dim == 1 ? y : z);
-    const bool isL = split && v < pos;
+    const int32_t ve = dim == 0 ? xe : (dim == 1 ? ye : ze);
+    const bool isL = split && ve < enc_cmp(pos);
     const uint64_t segMask = (se >= 64u ? ~0ull : ((1ull << se) - 1ull)) & ~((1ull << sb) - 1ull);
     const uint64_t mL = __ballot(isL) & segMask;
     const uint64_t mR = __ballot(split && !isL) & segMask;
@@ -756,13 +772,14 @@ __device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, 
     const uint32_t child = (uint32_t)__shfl((int)(nLocal + 2u * (uint32_t)__popcll(heads & lt)), (int)sb, kWave);
     nLocal += 2u * (uint32_t)__popcll(heads);
     if (split && isHead) store_node_lane(loc + me, nlo, nhi, pack_oc(child, 0));
-    x = lane_push_f(dst, x);
-    y = lane_push_f(dst, y);
-    z = lane_push_f(dst, z);
+    xe = (int32_t)lane_push_u(dst, (uint32_t)xe);
+    ye = (int32_t)lane_push_u(dst, (uint32_t)ye);
+    ze = (int32_t)lane_push_u(dst, (uint32_t)ze);
     id = lane_push_u(dst, id);
     if (!PTS) {
-      bx0 = lane_push_f(dst, bx0); by0 = lane_push_f(dst, by0); bz0 = lane_push_f(dst, bz0);
-      bx1 = lane_push_f(dst, bx1); by1 = lane_push_f(dst, by1); bz1 = lane_push_f(dst, bz1);
+      b0 = (int32_t)lane_push_u(dst, (uint32_t)b0); b1 = (int32_t)lane_push_u(dst, (uint32_t)b1);
+      b2 = (int32_t)lane_push_u(dst, (uint32_t)b2); b3 = (int32_t)lane_push_u(dst, (uint32_t)b3);
+      b4 = (int32_t)lane_push_u(dst, (uint32_t)b4); b5 = (int32_t)lane_push_u(dst, (uint32_t)b5);
     }
     if (split) {
       if (lane < mid) { se = mid; me = child; }
@@ -825,20 +842,26 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
                          nLocal, loc);
         continue;
       }
-      float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
-      float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      // bounds on enc_cmp integers (fused integer DPP reductions)
+      int32_t cle[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, che[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+      int32_t ble[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, bhe[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
       for (uint32_t i = b + lane; i < e; i += kWave) {
-        float4 v = sIt[i];
-        clo[0] = fminf(clo[0], v.x); clo[1] = fminf(clo[1], v.y); clo[2] = fminf(clo[2], v.z);
-        chi[0] = fmaxf(chi[0], v.x); chi[1] = fmaxf(chi[1], v.y); chi[2] = fmaxf(chi[2], v.z);
+        const float4 v = sIt[i];
+        const int32_t ex = enc_cmp(v.x), ey = enc_cmp(v.y), ez = enc_cmp(v.z);
+        cle[0] = min(cle[0], ex); cle[1] = min(cle[1], ey); cle[2] = min(cle[2], ez);
+        che[0] = max(che[0], ex); che[1] = max(che[1], ey); che[2] = max(che[2], ez);
         if (!PTS) {
           cubql_box3f bx = load_box(c.boxes, __float_as_uint(v.w));
-          for (int d = 0; d < 3; d++) { blo[d] = fminf(blo[d], bx.lower[d]); bhi[d] = fmaxf(bhi[d], bx.upper[d]); }
+          for (int d = 0; d < 3; d++) {
+            ble[d] = min(ble[d], enc_cmp(bx.lower[d]));
+            bhe[d] = max(bhe[d], enc_cmp(bx.upper[d]));
+          }
         }
       }
-      for (int d = 0; d < 3; d++) { clo[d] = wave_min_dpp(clo[d]); chi[d] = wave_max_dpp(chi[d]); }
+      float clo[3], chi[3], blo[3] = {0.f, 0.f, 0.f}, bhi[3] = {0.f, 0.f, 0.f};
+      for (int d = 0; d < 3; d++) { clo[d] = dec_f(wave_min_i32(cle[d])); chi[d] = dec_f(wave_max_i32(che[d])); }
       if (!PTS)
-        for (int d = 0; d < 3; d++) { blo[d] = wave_min_dpp(blo[d]); bhi[d] = wave_max_dpp(bhi[d]); }
+        for (int d = 0; d < 3; d++) { blo[d] = dec_f(wave_min_i32(ble[d])); bhi[d] = dec_f(wave_max_i32(bhe[d])); }
       const float *nlo = PTS ? clo : blo;
       const float *nhi = PTS ? chi : bhi;
       float olo[3] = {nlo[0], nlo[1], nlo[2]}, ohi[3] = {nhi[0], nhi[1], nhi[2]};

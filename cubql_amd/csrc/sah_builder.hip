@@ -227,7 +227,8 @@ __device__ int sah_decide_wave(const int32_t *bins, int nb, uint32_t n, uint32_t
     float best = thr;
     int dim = kModeLeaf;
     for (int p = 0; p < nSets; p++) {
-      const float m = wave_min(cost[p]);
+      // (costs are +0 .. +inf: their bit patterns order like the values)
+      const float m = __int_as_float(wave_min_i32(__float_as_int(cost[p])));
       if (m < best) {
         best = m;
         const uint32_t idx = (uint32_t)__builtin_ctzll(__ballot(cost[p] == m));
@@ -487,8 +488,8 @@ __global__ __launch_bounds__(kBlock) void k_select(Ctx c, int level) {
       const bool inBin = lane_id() < (uint32_t)nb;
       const int32_t *w = sBins + (inBin ? lane_id() : 0) * kBinWords;
       for (int k = 0; k < 3; k++) {
-        lo[k] = wave_min(inBin ? dec_f(w[k]) : INFINITY);
-        hi[k] = wave_max(inBin ? dec_f(w[3 + k]) : -INFINITY);
+        lo[k] = dec_f(wave_min_i32(inBin ? w[k] : INT32_MAX));
+        hi[k] = dec_f(wave_max_i32(inBin ? w[3 + k] : INT32_MIN));
       }
       if (lane_id() == 0) {
         c.segMode[s] = mode;
@@ -712,21 +713,23 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
     const uint32_t me = sMe[sp];
     const uint32_t b = f & 0xffffu, e = f >> 16;
     const uint32_t n = e - b;
-    float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    // bounds on enc_cmp integers (fused integer DPP reductions)
+    int32_t ble[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, bhe[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    int32_t cle[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, che[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
     for (uint32_t i = b + lane; i < e; i += kWave) {
       const Item v = sIt[i];
-      blo[0] = fminf(blo[0], v.a.x); blo[1] = fminf(blo[1], v.a.y); blo[2] = fminf(blo[2], v.a.z);
-      bhi[0] = fmaxf(bhi[0], v.b.x); bhi[1] = fmaxf(bhi[1], v.b.y); bhi[2] = fmaxf(bhi[2], v.b.z);
+      ble[0] = min(ble[0], enc_cmp(v.a.x)); ble[1] = min(ble[1], enc_cmp(v.a.y)); ble[2] = min(ble[2], enc_cmp(v.a.z));
+      bhe[0] = max(bhe[0], enc_cmp(v.b.x)); bhe[1] = max(bhe[1], enc_cmp(v.b.y)); bhe[2] = max(bhe[2], enc_cmp(v.b.z));
       for (int d = 0; d < 3; d++) {
-        const float ce = item_center(v, d);
-        clo[d] = fminf(clo[d], ce);
-        chi[d] = fmaxf(chi[d], ce);
+        const int32_t ce = enc_cmp(item_center(v, d));
+        cle[d] = min(cle[d], ce);
+        che[d] = max(che[d], ce);
       }
     }
+    float blo[3], bhi[3], clo[3], chi[3];
     for (int d = 0; d < 3; d++) {
-      blo[d] = wave_min_dpp(blo[d]); bhi[d] = wave_max_dpp(bhi[d]);
-      clo[d] = wave_min_dpp(clo[d]); chi[d] = wave_max_dpp(chi[d]);
+      blo[d] = dec_f(wave_min_i32(ble[d])); bhi[d] = dec_f(wave_max_i32(bhe[d]));
+      clo[d] = dec_f(wave_min_i32(cle[d])); chi[d] = dec_f(wave_max_i32(che[d]));
     }
     int mode = kModeLeaf, bin = 0;
     if (n > c.T) {

@@ -284,28 +284,56 @@ __global__ void k_rinit_levels(uint32_t *ctr) {
   if (threadIdx.x == 0) ctr[C_LSTART0] = 0;
 }
 
-// Bottom-up refit, one launch per level from the deepest: a leaf takes the
-// union of its prim boxes, an inner node the union of its children (written
-// by the previous launch).  No atomics or fences.
+// Bottom-up refit from the deepest level: a leaf takes the union of its prim
+// boxes, an inner node the union of its children (refitted before it).
+__device__ __forceinline__ void refit_node(const RCtx &c, Node *nodes, const uint32_t *primIDs, uint32_t id) {
+  Node *nd = &nodes[id];
+  const uint64_t oc = nd->oc;
+  const uint32_t cnt = oc_count(oc), off = (uint32_t)oc_offset(oc);
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  if (cnt) {
+    for (uint32_t j = 0; j < cnt; j++) {
+      const cubql_box3f b = load_box(c.boxes, primIDs[off + j]);
+      for (int d = 0; d < 3; d++) { lo[d] = fminf(lo[d], b.lower[d]); hi[d] = fmaxf(hi[d], b.upper[d]); }
+    }
+  } else {
+    const Node l = nodes[off], r = nodes[off + 1];
+    for (int d = 0; d < 3; d++) { lo[d] = fminf(l.lo[d], r.lo[d]); hi[d] = fmaxf(l.hi[d], r.hi[d]); }
+  }
+  for (int d = 0; d < 3; d++) { nd->lo[d] = lo[d]; nd->hi[d] = hi[d]; }
+}
+
+// one launch per wide level (the launch boundary orders the levels)
 __global__ void k_rrefit_level(RCtx c, Node *nodes, const uint32_t *primIDs, int level) {
   const uint32_t n = c.ctr[C_LEVEL0 + level];
   const uint32_t *ids = c.levels + c.ctr[C_LSTART0 + level];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    Node *nd = &nodes[ids[i]];
-    const uint64_t oc = nd->oc;
-    const uint32_t cnt = oc_count(oc), off = (uint32_t)oc_offset(oc);
-    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    if (cnt) {
-      for (uint32_t j = 0; j < cnt; j++) {
-        const cubql_box3f b = load_box(c.boxes, primIDs[off + j]);
-        for (int d = 0; d < 3; d++) { lo[d] = fminf(lo[d], b.lower[d]); hi[d] = fmaxf(hi[d], b.upper[d]); }
-      }
-    } else {
-      const Node l = nodes[off], r = nodes[off + 1];
-      for (int d = 0; d < 3; d++) { lo[d] = fminf(l.lo[d], r.lo[d]); hi[d] = fmaxf(l.hi[d], r.hi[d]); }
-    }
-    for (int d = 0; d < 3; d++) { nd->lo[d] = lo[d]; nd->hi[d] = hi[d]; }
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    refit_node(c, nodes, primIDs, ids[i]);
+}
+
+// the narrow top levels [0, levels) in one workgroup, deepest first; its
+// barriers order the levels (one CU: the node writes are seen by the
+// workgroup's later reads)
+constexpr uint32_t kRefitTopMax = 4096;  // nodes per level for the one-workgroup pass
+__global__ __launch_bounds__(1024) void k_rrefit_top(RCtx c, Node *nodes, const uint32_t *primIDs, int levels) {
+  for (int level = levels - 1; level >= 0; level--) {
+    const uint32_t n = c.ctr[C_LEVEL0 + level];
+    const uint32_t *ids = c.levels + c.ctr[C_LSTART0 + level];
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) refit_node(c, nodes, primIDs, ids[i]);
+    __syncthreads();
   }
+}
+
+// host: refit levels [0, levels) given their node counts
+static void refit_launch(const RCtx &c, Node *nodes, const uint32_t *primIDs, int levels,
+                         const std::vector<uint32_t> &hc, int numCU, hipStream_t s) {
+  int top = 0;  // levels [0, top) are narrow
+  while (top < levels && hc[C_LEVEL0 + top] <= kRefitTopMax) top++;
+  for (int l = levels - 1; l >= top; l--) {
+    const uint32_t g = std::max<uint32_t>(1u, std::min<uint32_t>(div_up(hc[C_LEVEL0 + l], kBlock), (uint32_t)numCU * 8));
+    k_rrefit_level<<<g, kBlock, 0, s>>>(c, nodes, primIDs, l);
+  }
+  if (top > 0) k_rrefit_top<<<1, 1024, 0, s>>>(c, nodes, primIDs, top);
 }
 
 template <typename K>
@@ -396,9 +424,7 @@ int build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, in
   bvh->primIDs = primIDs;
   bvh->numPrims = numValid;
   int rs = prof_begin(s);
-  for (int l = level - 1; l >= 0; l--)
-    k_rrefit_level<<<std::max<uint32_t>(1u, std::min<uint32_t>(div_up(n, kBlock), (uint32_t)numCU * 8)), kBlock, 0, s>>>(
-        c, nodes, primIDs, l);
+  refit_launch(c, nodes, primIDs, level, hc, numCU, s);
   st = hipGetLastError() == hipSuccess ? CUBQL_OK : (set_error("refit launch failed"), CUBQL_ERR_HIP);
   prof_end(rs, s, "refit");
   prof_set(rs, 24.0 * numValid + 64.0 * numNodes, numNodes);
@@ -864,8 +890,7 @@ int rebin_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t
   bvh->primIDs = primIDs;
   bvh->numPrims = numValid;
   int rs = prof_begin(s);
-  for (int l = level - 1; l >= 0; l--)
-    k_rrefit_level<<<grid, kBlock, 0, s>>>(c, nodes, primIDs, l);
+  refit_launch(c, nodes, primIDs, level, hc, numCU, s);
   st = hipGetLastError() == hipSuccess ? CUBQL_OK : (set_error("refit launch failed"), CUBQL_ERR_HIP);
   prof_end(rs, s, "refit");
   prof_set(rs, 24.0 * numValid + 64.0 * numNodes, numNodes);

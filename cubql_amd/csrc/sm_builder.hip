@@ -35,7 +35,11 @@ namespace sm {
 constexpr int kBlock = 256;
 constexpr int kItems = 8;
 constexpr int kTile = kBlock * kItems;  // 2048 prims per partition tile
-constexpr int kSmall = 256;             // max prims of a wave-finished node
+constexpr int kSmall = 256;             // max prims of a node a wave finishes in LDS
+// Segments of up to kMedium prims leave the level loop too: their wave
+// splits them in global memory (between the two item buffers) until the
+// pieces fit kSmall.
+constexpr int kMedium = 1024;
 constexpr int kLevelsPerSync = 8;
 // Tiles are claimed from kTicketCounters interleaved counters (counter c
 // hands out tiles c, c+kTicketCounters, ...) and child centroid bounds are
@@ -59,7 +63,7 @@ struct Counters {
   uint32_t baseNode[2], baseSmall[2], baseTop[2];  // each level's bases, by level parity (emit)
 };
 
-struct Seg {          // an open node with > kSmall prims
+struct Seg {          // an open node with > kMedium prims
   uint32_t begin, end, node, pad;
   int32_t cb[6];      // encoded centroid bounds
   uint32_t pad2[2];
@@ -193,7 +197,7 @@ __global__ __launch_bounds__(1024) void k_count_scan(Ctx c, uint32_t numBlocks) 
     k->numTiles = 0;
     k->ticket = 0;
     for (uint32_t cc = 0; cc < kTicketCounters; cc++) c.tickets[kTicketStride * cc] = 0;
-    if (run > kSmall && run > c.T) {
+    if (run > kMedium && run > c.T) {
       Seg s;
       s.begin = 0; s.end = run; s.node = 0; s.pad = 0; s.pad2[0] = s.pad2[1] = 0;
       for (int j = 0; j < 6; j++) s.cb[j] = k->rootCb[j];
@@ -438,7 +442,7 @@ __device__ __forceinline__ void emit_classify(const Ctx &c, int par, uint32_t s,
   e.cb[1] = mid; e.ce[1] = e.g.end;
   for (int side = 0; side < 2; side++) {
     const uint32_t sz = e.ce[side] - e.cb[side];
-    e.large[side] = sz > (uint32_t)kSmall && sz > c.T;
+    e.large[side] = sz > (uint32_t)kMedium && sz > c.T;
   }
 }
 
@@ -654,7 +658,7 @@ struct SmallNodeOut { Node *loc; };
 
 // a pending node of the small phase's LDS loop in one word: range [b, e)
 // (<= kSmall) and local id (< 2 * kSmall)
-static_assert(kSmall <= 256, "small-phase stack packing");
+static_assert(kSmall <= 256 && 2 * kMedium <= (1 << 14), "small-phase stack packing");
 __device__ __forceinline__ uint32_t fr_pack(uint32_t b, uint32_t e, uint32_t me) {
   return b | (e << 9) | (me << 18);
 }
@@ -789,16 +793,14 @@ __device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, 
   }
 }
 
+// Finishes the node of items [gb, gb + m) of gItems (m <= kSmall, or a leaf
+// of any size up to T) whose local id is rootMe, numbering new nodes from
+// nLocal into loc.
 template <bool PTS>
-__device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx, float4 *sIt,
-                              uint32_t *sFr) {
+__device__ void small_sub(const Ctx &c, const float4 *gItems, uint32_t gb, uint32_t m, uint32_t rootMe,
+                          uint32_t &nLocal, Node *loc, float4 *sIt, uint32_t *sFr) {
   const uint32_t lane = lane_id();
   const uint32_t T = c.T;
-  const float4 *gItems = c.items[sg.buf];
-  const uint32_t gb = sg.begin;
-  const uint32_t m = sg.end - sg.begin;
-  Node *loc = c.loc + 2 * (size_t)gb;
-  uint32_t nLocal = 1;
   if (m <= T && m > 64) {
     // leaf of any size up to T
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -814,18 +816,32 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
       }
     }
     for (int d = 0; d < 3; d++) { lo[d] = wave_min_dpp(lo[d]); hi[d] = wave_max_dpp(hi[d]); }
-    store_node_wave(loc, lo, hi, pack_oc(gb, m));
+    store_node_wave(loc + rootMe, lo, hi, pack_oc(gb, m));
   } else if (m <= 64) {
     float4 v = lane < m ? gItems[gb + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
     cubql_box3f bx{};
     if (!PTS && lane < m) bx = load_box(c.boxes, __float_as_uint(v.w));
     reg_subtree<PTS>(c, gb, m, v.x, v.y, v.z, __float_as_uint(v.w), bx.lower[0], bx.lower[1],
-                     bx.lower[2], bx.upper[0], bx.upper[1], bx.upper[2], 0u, nLocal, loc);
+                     bx.lower[2], bx.upper[0], bx.upper[1], bx.upper[2], rootMe, nLocal, loc);
   } else {
     // more than 64 prims: split in LDS until children fit one wave
-    for (uint32_t i = lane; i < m; i += kWave) sIt[i] = gItems[gb + i];
+    {
+      // all loads in flight before the first LDS write
+      constexpr int kPer = kSmall / kWave;
+      float4 t[kPer];
+#pragma unroll
+      for (int k = 0; k < kPer; k++) {
+        const uint32_t i = (uint32_t)k * kWave + lane;
+        if (i < m) t[k] = gItems[gb + i];
+      }
+#pragma unroll
+      for (int k = 0; k < kPer; k++) {
+        const uint32_t i = (uint32_t)k * kWave + lane;
+        if (i < m) sIt[i] = t[k];
+      }
+    }
     int sp = 1;
-    if (lane == 0) sFr[0] = fr_pack(0u, m, 0u);
+    if (lane == 0) sFr[0] = fr_pack(0u, m, rootMe);
     sync_wave_lds();
     while (sp > 0) {
       --sp;
@@ -911,18 +927,197 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
       sync_wave_lds();
     }
   }
+}
+
+// A pending piece of a medium segment: range [b, e) relative to the
+// segment, local id, item buffer, and (when known) its centroid and box
+// bounds as enc_cmp integers (lo xyz, hi xyz).
+struct MedFrame {
+  uint32_t b, e, me, buf;
+  int32_t cb[6];
+  int32_t bb[6];
+};
+static_assert(sizeof(MedFrame) == 64, "MedFrame");
+constexpr int kMedChunk = 4;  // items per lane per round trip of the medium passes
+
+__device__ __forceinline__ void bounds_init(int32_t (&v)[6]) {
+  v[0] = v[1] = v[2] = INT32_MAX;
+  v[3] = v[4] = v[5] = INT32_MIN;
+}
+__device__ __forceinline__ void bounds_add(int32_t (&v)[6], int32_t x, int32_t y, int32_t z, int32_t X,
+                                           int32_t Y, int32_t Z) {
+  v[0] = min(v[0], x); v[1] = min(v[1], y); v[2] = min(v[2], z);
+  v[3] = max(v[3], X); v[4] = max(v[4], Y); v[5] = max(v[5], Z);
+}
+__device__ __forceinline__ void bounds_wave(int32_t (&v)[6]) {
+  for (int d = 0; d < 3; d++) { v[d] = wave_min_i32(v[d]); v[d + 3] = wave_max_i32(v[d + 3]); }
+}
+
+// Centroid and box bounds (wave-reduced) of items [lo, hi) of src.
+template <bool PTS>
+__device__ void med_bounds(const Ctx &c, const float4 *src, uint32_t lo, uint32_t hi, int32_t (&cb)[6],
+                           int32_t (&bb)[6]) {
+  const uint32_t lane = lane_id();
+  bounds_init(cb);
+  bounds_init(bb);
+  for (uint32_t c0 = lo; c0 < hi; c0 += kMedChunk * kWave) {
+    float4 v[kMedChunk];
+    bool act[kMedChunk];
+#pragma unroll
+    for (int k = 0; k < kMedChunk; k++) {
+      const uint32_t i = c0 + (uint32_t)k * kWave + lane;
+      act[k] = i < hi;
+      v[k] = act[k] ? src[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int k = 0; k < kMedChunk; k++) {
+      if (!act[k]) continue;
+      const int32_t ex = enc_cmp(v[k].x), ey = enc_cmp(v[k].y), ez = enc_cmp(v[k].z);
+      bounds_add(cb, ex, ey, ez, ex, ey, ez);
+      if (!PTS) {
+        const cubql_box3f bx = load_box(c.boxes, __float_as_uint(v[k].w));
+        bounds_add(bb, enc_cmp(bx.lower[0]), enc_cmp(bx.lower[1]), enc_cmp(bx.lower[2]),
+                   enc_cmp(bx.upper[0]), enc_cmp(bx.upper[1]), enc_cmp(bx.upper[2]));
+      }
+    }
+  }
+  bounds_wave(cb);
+  if (!PTS) bounds_wave(bb);
+}
+
+// One wave finishes a small segment.  Pieces of more than kSmall prims are
+// split in global memory: a partition pass from the piece's buffer into the
+// other one (left stable, right reversed, one-sided -> middle, as
+// everywhere) that also gathers both sides' centroid bounds, so a child piece
+// of point input needs no bounds pass of its own unless the split was
+// one-sided; pieces that fit
+// kSmall are finished at once, larger ones wait on a stack that never holds
+// more than kMedium / (kSmall + 1) disjoint pieces.  Local ids are assigned
+// in that order.
+template <bool PTS>
+__device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx, float4 *sIt, uint32_t *sFr,
+                              MedFrame *sMed) {
+  const uint32_t lane = lane_id();
+  const uint32_t T = c.T;
+  const uint32_t gb = sg.begin;
+  const uint32_t m = sg.end - sg.begin;
+  Node *loc = c.loc + 2 * (size_t)gb;
+  uint32_t nLocal = 1;
+  auto piece = [&](uint32_t buf, uint32_t b, uint32_t n, uint32_t me) {
+    small_sub<PTS>(c, c.items[buf], gb + b, n, me, nLocal, loc, sIt, sFr);
+    sync_wave_lds();
+  };
+  if (m <= (uint32_t)kSmall || m <= T) {
+    piece(sg.buf, 0u, m, 0u);
+  } else {
+    int sp = 1;
+    bool known = false;  // bounds of the popped frame
+    if (lane == 0) {
+      MedFrame f0{};
+      f0.b = 0u; f0.e = m; f0.me = 0u; f0.buf = sg.buf | 2u;  // bit 1: bounds unknown
+      sMed[0] = f0;
+    }
+    sync_wave_lds();
+    while (sp > 0) {
+      --sp;
+      const MedFrame f = sMed[sp];
+      const uint32_t b = f.b, e = f.e, me = f.me, bf = f.buf & 1u;
+      known = (f.buf & 2u) == 0;
+      const float4 *src = c.items[bf];
+      float4 *dst = c.items[bf ^ 1u];
+      int32_t cb[6], bb[6];
+      if (known) {
+        for (int j = 0; j < 6; j++) { cb[j] = f.cb[j]; bb[j] = f.bb[j]; }
+      } else {
+        med_bounds<PTS>(c, src, gb + b, gb + e, cb, bb);
+      }
+      float clo[3], chi[3], olo[3], ohi[3];
+      for (int d = 0; d < 3; d++) {
+        clo[d] = dec_f(cb[d]); chi[d] = dec_f(cb[d + 3]);
+        olo[d] = PTS ? clo[d] : dec_f(bb[d]);
+        ohi[d] = PTS ? chi[d] : dec_f(bb[d + 3]);
+      }
+      int dim;
+      float pos;
+      choose_split(clo, chi, dim, pos);
+      const int32_t posE = enc_cmp(pos);
+      // partition pass; per-lane centroid bounds of both sides (point input:
+      // the children's boxes; boxes of other input take a bounds pass, which
+      // keeps the kernel's register count at the small pieces' occupancy)
+      int32_t sc[2][6];
+      bounds_init(sc[0]); bounds_init(sc[1]);
+      uint32_t nL = 0, nR = 0;
+      for (uint32_t c0 = b; c0 < e; c0 += kMedChunk * kWave) {
+        float4 v[kMedChunk];
+        bool act[kMedChunk];
+#pragma unroll
+        for (int k = 0; k < kMedChunk; k++) {
+          const uint32_t i = c0 + (uint32_t)k * kWave + lane;
+          act[k] = i < e;
+          v[k] = act[k] ? src[gb + i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int k = 0; k < kMedChunk; k++) {
+          const int32_t ex = enc_cmp(v[k].x), ey = enc_cmp(v[k].y), ez = enc_cmp(v[k].z);
+          const int32_t ec = dim == 0 ? ex : (dim == 1 ? ey : ez);
+          const bool isL = act[k] && ec < posE;
+          const uint64_t mL = __ballot(isL);
+          const uint64_t mR = __ballot(act[k] && !isL);
+          if (act[k]) {
+            const uint32_t d = isL ? b + nL + mbcnt(mL) : e - 1 - (nR + mbcnt(mR));
+            dst[gb + d] = v[k];
+            const int s = isL ? 0 : 1;
+            // (select both sides' updates: no divergent array indexing)
+            for (int q = 0; q < 2; q++)
+              if (s == q) bounds_add(sc[q], ex, ey, ez, ex, ey, ez);
+          }
+          nL += (uint32_t)__popcll(mL);
+          nR += (uint32_t)__popcll(mR);
+        }
+      }
+      const bool oneSided = nL == 0 || nR == 0;
+      const uint32_t mid = !oneSided ? b + nL
+                                     : (uint32_t)(((uint64_t)gb + b + gb + e) / 2 - gb);  // spatialMedian.h:107
+      const uint32_t child = nLocal;
+      nLocal += 2;
+      store_node_wave(loc + me, olo, ohi, pack_oc(child, 0));
+      // the pieces are read back below, by this wave
+      __threadfence_block();
+      for (int side = 0; side < 2; side++) {
+        const uint32_t cb0 = side ? mid : b, ce0 = side ? e : mid;
+        if (ce0 - cb0 <= (uint32_t)kSmall || ce0 - cb0 <= T) {
+          piece(bf ^ 1u, cb0, ce0 - cb0, child + side);
+        } else {
+          MedFrame g{};
+          const bool carry = PTS && !oneSided;
+          g.b = cb0; g.e = ce0; g.me = child + side; g.buf = (bf ^ 1u) | (carry ? 0u : 2u);
+          if (carry) {
+            // (wave-uniform branch: every lane joins the reductions)
+            int32_t rc[6];
+            for (int j = 0; j < 6; j++) rc[j] = side ? sc[1][j] : sc[0][j];
+            bounds_wave(rc);
+            for (int j = 0; j < 6; j++) g.cb[j] = g.bb[j] = rc[j];
+          }
+          if (lane == 0) sMed[sp] = g;
+          sp++;
+          sync_wave_lds();
+        }
+      }
+    }
+  }
   if (lane == 0) c.smallCount[segIdx] = nLocal;
 }
 
 __global__ __launch_bounds__(kWave) void k_small(Ctx c) {
   __shared__ float4 sIt[kSmall];
   __shared__ uint32_t sFr[kSmall + 2];
+  __shared__ MedFrame sMed[kMedium / (kSmall + 1) + 1];
   const bool pts = c.ctr->allPoints != 0;
   const uint32_t nSmall = c.ctr->numSmall;
   for (uint32_t w = blockIdx.x; w < nSmall; w += gridDim.x) {
     const SmallSeg sg = c.small[w];
-    if (pts) small_segment<true>(c, sg, w, sIt, sFr);
-    else small_segment<false>(c, sg, w, sIt, sFr);
+    if (pts) small_segment<true>(c, sg, w, sIt, sFr, sMed);
+    else small_segment<false>(c, sg, w, sIt, sFr, sMed);
     sync_wave_lds();
   }
 }
@@ -1001,7 +1196,7 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
   c.n = n;
   c.T = T;
   const uint32_t numBlocks = div_up(n, kTile);
-  c.maxLarge = n / (kSmall + 1) + 2;
+  c.maxLarge = n / (kMedium + 1) + 2;
   const uint32_t maxTiles = div_up(n, kTile) + c.maxLarge + 1;
   const uint32_t maxLevels = 4096;
   CUBQL_CHECK(scr.alloc(&c.items[0], n));
@@ -1066,10 +1261,10 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
   int level = 0;
   std::vector<std::pair<int, int>> partSlots;  // (profile slot, level)
   // The first batch covers the depth a well-spread input needs (halving
-  // down to kSmall prims, plus one level of slack): running a level past the
-  // tree's depth costs two near-empty launches, a host check ~20 us.
+  // down to kMedium prims, plus one level of slack): running a level past
+  // the tree's depth costs two near-empty launches, a host check ~20 us.
   int batch = 1;
-  for (uint64_t m = (uint64_t)kSmall; m < n; m *= 2) batch++;
+  for (uint64_t m = (uint64_t)kMedium; m < n; m *= 2) batch++;
   batch = std::max(kLevelsPerSync, batch);
   while (true) {
     for (int i = 0; i < batch && level < (int)maxLevels; i++, level++) {

@@ -40,14 +40,21 @@ constexpr int kSmall = 256;             // max prims of a node a wave finishes i
 // splits them in global memory (between the two item buffers) until the
 // pieces fit kSmall.
 constexpr int kMedium = 1024;
-constexpr int kLevelsPerSync = 8;
+constexpr int kPassesPerSync = 4;
 // Tiles are claimed from kTicketCounters interleaved counters (counter c
-// hands out tiles c, c+kTicketCounters, ...) and child centroid bounds are
-// accumulated into kCbCopies copies: both spread same-address atomics, which
-// serialise at one memory-side atomic unit per address.
+// hands out tiles c, c+kTicketCounters, ...) and per-segment counts and
+// bounds are accumulated into kCbCopies copies: both spread same-address
+// atomics, which serialise at one memory-side atomic unit per address.
 constexpr uint32_t kTicketCounters = 8;
 constexpr uint32_t kTicketStride = 32;  // words between counters
 constexpr uint32_t kCbCopies = 8;
+// pass A accumulators per segment and copy: six encoded bound sets (left
+// side, right side, the halves of a one-sided left split, the halves of a
+// one-sided right split) and the left count
+constexpr uint32_t kAccA = 40;  // words (36 bounds, count, padding)
+// pass B accumulators per segment and copy: the bounds of the four
+// grandchild classes
+constexpr uint32_t kAccB = 24;
 
 struct Counters {
   uint32_t numValid;
@@ -63,16 +70,43 @@ struct Counters {
   uint32_t baseNode[2], baseSmall[2], baseTop[2];  // each level's bases, by level parity (emit)
 };
 
-struct Seg {          // an open node with > kMedium prims
-  uint32_t begin, end, node, pad;
+// An open node with > kMedium prims.  [begin, end) is its range in the CPU
+// recursion's order; its items sit at [pbeg, pbeg + end - begin) of the
+// pass's item buffer, in that order or (rev) in reverse.
+struct Seg {
+  uint32_t begin, end, node, pbeg;
   int32_t cb[6];      // encoded centroid bounds
-  uint32_t pad2[2];
+  uint32_t rev, pad;
 };
 static_assert(sizeof(Seg) == 48, "Seg");
 
-struct SmallSeg {     // a node finished by one wave
-  uint32_t begin, end, node, buf;
+// A pass's plan for one segment S, from its pass-A counts: S's split (mode
+// 0: by its plane, children [begin, mid) and [mid, end); mode 1 / 2: the
+// plane left every prim on the left / right side and the children are the
+// halves of the range) and, per child, whether it is split in the same pass
+// and by which plane.
+struct PairPlan {
+  uint32_t begin, end, pbeg, rev;
+  uint32_t mid, mode, dim, pos;
+  uint32_t split[2], cdim[2];
+  uint32_t cpos[2], pad[2];
+  int32_t ccb[2][6];  // children's encoded centroid bounds
+  uint32_t pad2[4];
 };
+static_assert(sizeof(PairPlan) == 128, "PairPlan");
+
+// Order of child `side` of a plan relative to S's order: the right child of
+// a split by the plane holds S's right-side prims in reverse (the CPU order),
+// the halves of a one-sided split keep the order of the partitioned range.
+__host__ __device__ __forceinline__ uint32_t child_rev(uint32_t mode, uint32_t side) {
+  return mode == 0 ? side : (mode == 2 ? 1u : 0u);
+}
+
+struct SmallSeg {     // a node finished by one wave
+  uint32_t begin, end, node, buf;  // range (CPU order), node, item buffer
+  uint32_t pbeg, rev, pad0, pad1;  // first slot of its items, reverse storage
+};
+static_assert(sizeof(SmallSeg) == 32, "SmallSeg");
 
 struct Ctx {
   uint32_t *tickets;       // kTicketCounters counters, kTicketStride words apart
@@ -90,22 +124,44 @@ struct Ctx {
   uint32_t *smallBase;     // small phase: first global id of local node 1
   // per-level arrays, double-buffered by level parity: the emit of level l
   // reads the [l&1] copies and prepares the [(l+1)&1] ones
-  Seg *segs[2];            // large segments
-  int32_t *childCb[2];     // [kCbCopies][2*maxLarge][6] child centroid bounds
-  uint32_t *segLeft[2];    // [maxLarge] left-side count after the partition
-  uint32_t *tileStart[2];  // [maxLarge+1] first partition tile of each segment
-  uint32_t *tileSeg[2];    // [maxTiles] segment of each partition tile
+  Seg *segs[2];            // open segments of a pass
+  uint32_t *tileStart[2];  // [maxLarge+1] first tile of each segment
+  uint32_t *tileSeg[2];    // [maxTiles] segment of each tile
   int32_t *segDim[2];      // [maxLarge] split plane
   float *segPos[2];
-  uint64_t *emitStatus;    // [4][maxEmitBlocks] emit look-back status per count
+  int32_t *accA;           // [maxLarge][kCbCopies][kAccA] pass A sums (k_bounds)
+  int32_t *accB;           // [maxLarge][kCbCopies][kAccB] pass B bounds (k_pair)
+  PairPlan *plan;          // [maxLarge] (k_plan)
+  uint32_t *classTot;      // [maxLarge][4] grandchild class sizes (k_pair)
+  uint64_t *emitStatus;    // [5][maxEmitBlocks] emit look-back status per count
   uint32_t maxEmitBlocks;
-  uint64_t *tileStatus;    // [maxTiles]
+  uint64_t *tileStatus;    // [4][maxTiles] look-back status per class
+  uint32_t maxTiles;
   SmallSeg *small;         // [numValid]
   uint32_t *topNodes;      // [numValid] inner nodes of the level phase
-  uint32_t *levelStart;    // [maxLevels+1] start of each level in topNodes
-  uint32_t *levelItems;    // [maxLevels] prims moved by each level's partition
+  uint32_t *levelStart;    // [2*maxPasses+1] start of each node list in topNodes
+  uint32_t *levelItems;    // [maxPasses] prims moved by each pass
   uint32_t maxLarge;
 };
+
+// Accumulator identities: min slots +inf, max slots -inf (bound sets of six
+// words, lo xyz then hi xyz); pass A's count word (36) and padding are 0.
+__device__ __forceinline__ int32_t acc_identity(uint32_t w) {
+  return (w % 6) < 3 ? enc_f(INFINITY) : enc_f(-INFINITY);
+}
+static_assert(kAccA == 40 && kAccB == 24, "accumulator layout");
+// 16-byte word group q of a segment's pass-A accumulators (kCbCopies * 10
+// groups, copies contiguous), cleared
+__device__ __forceinline__ int4 acc_a_clear(uint32_t q) {
+  const uint32_t w = 4 * (q % 10);
+  if (w >= 36) return make_int4(0, 0, 0, 0);
+  return make_int4(acc_identity(w), acc_identity(w + 1), acc_identity(w + 2), acc_identity(w + 3));
+}
+__device__ __forceinline__ int4 acc_b_clear(uint32_t q) {
+  const uint32_t w = 4 * (q % 6);
+  return make_int4(acc_identity(w), acc_identity(w + 1), acc_identity(w + 2), acc_identity(w + 3));
+}
+
 
 // ---------------------------------------------------------------- init pass
 // Pass 1: per-tile valid count, root centroid bounds, point-data flag.
@@ -199,11 +255,11 @@ __global__ __launch_bounds__(1024) void k_count_scan(Ctx c, uint32_t numBlocks) 
     for (uint32_t cc = 0; cc < kTicketCounters; cc++) c.tickets[kTicketStride * cc] = 0;
     if (run > kMedium && run > c.T) {
       Seg s;
-      s.begin = 0; s.end = run; s.node = 0; s.pad = 0; s.pad2[0] = s.pad2[1] = 0;
+      s.begin = 0; s.end = run; s.node = 0; s.pbeg = 0; s.rev = 0; s.pad = 0;
       for (int j = 0; j < 6; j++) s.cb[j] = k->rootCb[j];
       c.segs[0][0] = s;
       k->numLarge[0] = 1;
-      // plan of level 0
+      // the root's plane
       float lo[3], hi[3];
       for (int d = 0; d < 3; d++) { lo[d] = dec_f(s.cb[d]); hi[d] = dec_f(s.cb[d + 3]); }
       int dim;
@@ -211,20 +267,17 @@ __global__ __launch_bounds__(1024) void k_count_scan(Ctx c, uint32_t numBlocks) 
       choose_split(lo, hi, dim, pos);
       c.segDim[0][0] = dim;
       c.segPos[0][0] = pos;
-      c.segLeft[0][0] = 0;
-      for (uint32_t cp = 0; cp < kCbCopies; cp++)
-        for (int j = 0; j < 12; j++) c.childCb[0][(size_t)cp * 12 * c.maxLarge + j] = enc_f((j % 6) < 3 ? INFINITY : -INFINITY);
       const uint32_t tiles = div_up(run, kTile);
       c.tileStart[0][0] = 0;
       c.tileStart[0][1] = tiles;
       k->numTiles = tiles;
       c.levelItems[0] = run;
     } else if (run > 0) {
-      SmallSeg s{0, run, 0, 0};
+      SmallSeg s{0, run, 0, 0, 0, 0, 0, 0};
       c.small[0] = s;
       k->numSmall = 1;
     }
-    // level 0's bases for the emit
+    // pass 0's bases for the emit
     k->baseNode[0] = k->nodeCount;
     k->baseSmall[0] = k->numSmall;
     k->baseTop[0] = k->numTop;
@@ -268,219 +321,512 @@ __global__ __launch_bounds__(kBlock) void k_write_items(Ctx c) {
   }
 }
 
-// ------------------------------------------------------------- level pass
-// Partition, software-pipelined over a workgroup's tiles.  A workgroup
-// claims tiles in ticket order (so a tile only ever waits on tiles claimed
-// by running workgroups); for its current tile it ranks the items left /
-// right, stages them in LDS in output order, resolves its prefix by
-// decoupled look-back among the tiles of its node and copies the staged
-// items out in contiguous runs.  After its look-back the workgroup claims
-// and describes its next tile, whose items are loaded into registers
-// before the current tile's copy-out, so each tile's load latency overlaps
-// the previous tile's stores.  (Claiming only then keeps the claim order
-// close to the order in which tiles publish their counts, which shortens
-// every later tile's look-back.)
-//
-// A workgroup holds at most two tiles, the later one claimed after the
-// earlier; every wait is on a lower tile, so the lowest unfinished tile
-// never waits and the grid drains.
-struct TileDesc {   // the next tile, written by one thread, read after a barrier
-  uint32_t t, seg, begin, end, dim, pos, tileFirst, tileEnd;
+// ------------------------------------------------------------- pair passes
+// Every pass splits each open segment S two tree levels deep, with the
+// result of the CPU recursion (cpu/spatialMedian.h:81-114), which
+// partitions a node's range in place -- left-side prims first in their
+// order, right-side prims after them in reverse order -- and splits a node
+// whose plane leaves one side empty at the middle position:
+//   * k_bounds (pass A) reads S's items once and accumulates the left count
+//     and the centroid bounds of both sides of S's plane (and of the position
+//     halves a one-sided split would take);
+//   * k_plan turns them into S's children: ranges, bounds and planes;
+//   * k_pair (pass B) reads S's items again and writes each straight to the
+//     range of its grandchild;
+//   * k_emit2 writes the nodes, the next pass's segments and tile map, and
+//     the small-segment list.
+// The items cross HBM three times per two levels instead of four.  A
+// grandchild range can only be filled from the two ends of its parent's
+// range (where they meet is known when the pass ends), so the children of a
+// node whose order is the reverse of S's -- the right child -- come out
+// reversed: a segment carries that as a flag and readers walk it backwards.
+
+
+// Items [j0, j0 + cnt) of a segment's order (striped: item jj of thread t is
+// j0 + jj*kBlock + t); lanes past the end re-read the last item.  (Slots
+// are clamped to the buffer, lim items.)
+__device__ __forceinline__ void load_tile(const float4 *src, uint32_t pbeg, uint32_t n, uint32_t rev,
+                                          uint32_t j0, uint32_t cnt, uint32_t lim, float4 (&it)[kItems]) {
+#pragma unroll
+  for (int jj = 0; jj < kItems; jj++) {
+    const uint32_t j = j0 + min((uint32_t)(jj * kBlock) + threadIdx.x, cnt - 1);
+    it[jj] = src[min(rev ? pbeg + n - 1 - j : pbeg + j, lim - 1)];
+  }
+}
+
+// Pass A: each workgroup walks a contiguous run of the pass's tiles (no tile
+// waits on another).  It accumulates per thread while its tiles belong to
+// one segment and adds its sums to the segment's accumulators (one of
+// kCbCopies copies) when the segment changes, so a segment's slots take
+// about one atomic per workgroup and word, not one per tile.
+// The next tile's loads are in flight during that flush.  One thread of
+// wave 0 reads the tile descriptors (into LDS) and wave 3 issues the
+// atomics: a wave's memory counter retires in order, so a wave that waited
+// on its descriptor loads behind its own atomics would wait for them too.
+struct TileA {
+  uint32_t s, n, j0, cnt, pbeg, rev;
+  int dim;
+  float pos;
 };
 
-__global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
-  __shared__ float4 sStage[kTile];
-  __shared__ uint32_t sr[kItems * (kBlock / kWave) + 1];
-  __shared__ float sb[12 * (kBlock / kWave)];
-  __shared__ TileDesc sNext;
-  __shared__ uint32_t sPrefix;
-  const int par = level & 1;
+// bounds on enc_f integers (the accumulators' order): integer min/max, and
+// fused DPP steps for the wave reductions
+#define CUBQL_ADD6(A, O, X, Y, Z)                                               \
+  {                                                                             \
+    A[O] = min(A[O], X); A[O + 1] = min(A[O + 1], Y); A[O + 2] = min(A[O + 2], Z); \
+    A[O + 3] = max(A[O + 3], X); A[O + 4] = max(A[O + 4], Y); A[O + 5] = max(A[O + 5], Z); \
+  }
+
+__global__ __launch_bounds__(kBlock) void k_bounds(Ctx c, int pass) {
+  constexpr int NW = kBlock / kWave;
+  constexpr uint32_t kAtomicWave = NW - 1;
+  __shared__ int32_t sb[36 * NW];
+  __shared__ uint32_t sc[NW];
+  __shared__ TileA sD[2];
+  const int par = pass & 1;
   const uint32_t numTiles = c.ctr->numTiles;
-  if (numTiles == 0) return;  // a level past the tree's depth
+  const float4 *src = c.items[par];
+  const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
+  auto U = [](uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); };
+  auto readD = [&](int sl) {
+    TileA d;
+    d.s = U(sD[sl].s); d.n = U(sD[sl].n); d.j0 = U(sD[sl].j0); d.cnt = U(sD[sl].cnt);
+    d.pbeg = U(sD[sl].pbeg); d.rev = U(sD[sl].rev); d.dim = (int)U((uint32_t)sD[sl].dim);
+    d.pos = __uint_as_float(U(__float_as_uint(sD[sl].pos)));
+    return d;
+  };
+  auto desc = [&](uint32_t t) {
+    TileA d;
+    d.s = c.tileSeg[par][t];
+    const Seg g = c.segs[par][d.s];
+    d.n = g.end - g.begin;
+    d.j0 = (t - c.tileStart[par][d.s]) * kTile;
+    d.cnt = min(d.n - d.j0, (uint32_t)kTile);
+    d.pbeg = g.pbeg;
+    d.rev = g.rev;
+    d.dim = c.segDim[par][d.s];
+    d.pos = c.segPos[par][d.s];
+    return d;
+  };
+  const uint32_t per = div_up(numTiles, gridDim.x);
+  uint32_t t = blockIdx.x * per;
+  const uint32_t tEnd = min(t + per, numTiles);
+  if (t >= tEnd) return;
+  int sl = 0;
+  if (threadIdx.x == 0) sD[0] = desc(t);
+  sync_block_lds();
+  TileA d = readD(0);
+  float4 it[kItems];
+  load_tile(src, d.pbeg, d.n, d.rev, d.j0, d.cnt, c.n, it);
+  // sets: left / right of the plane (0, 6); positions < h / >= h, the
+  // halves of an all-left split (12, 18); positions >= n-h / < n-h, the
+  // halves of an all-right split (24, 30); h = n/2
+  int32_t a[36];
+  for (int i = 0; i < 36; i++) a[i] = acc_identity(i);
+  uint32_t nl = 0;
+  while (true) {
+    const uint32_t h = d.n / 2;
+#pragma unroll
+    for (int jj = 0; jj < kItems; jj++) {
+      const uint32_t q = (uint32_t)(jj * kBlock) + threadIdx.x;
+      if (q < d.cnt) {
+        const float4 p = it[jj];
+        const uint32_t j = d.j0 + q;
+        const int32_t ex = enc_f(p.x), ey = enc_f(p.y), ez = enc_f(p.z);
+        if (coord(p, d.dim) < d.pos) {
+          nl++;
+          CUBQL_ADD6(a, 0, ex, ey, ez);
+        } else {
+          CUBQL_ADD6(a, 6, ex, ey, ez);
+        }
+        if (j < h) CUBQL_ADD6(a, 12, ex, ey, ez) else CUBQL_ADD6(a, 18, ex, ey, ez);
+        if (j >= d.n - h) CUBQL_ADD6(a, 24, ex, ey, ez) else CUBQL_ADD6(a, 30, ex, ey, ez);
+      }
+    }
+    const uint32_t t2 = t + 1;
+    const bool more = t2 < tEnd;
+    TileA d2 = d;
+    if (more) {
+      // (the other slot's last reader passed the barrier before this
+      // tile's loads)
+      if (threadIdx.x == 0) sD[sl ^ 1] = desc(t2);
+      sync_block_lds();
+      sl ^= 1;
+      d2 = readD(sl);
+      load_tile(src, d2.pbeg, d2.n, d2.rev, d2.j0, d2.cnt, c.n, it);
+    }
+    if (!more || d2.s != d.s) {
+#pragma unroll
+      for (int i = 0; i < 36; i++) a[i] = (i % 6) < 3 ? wave_min_i32(a[i]) : wave_max_i32(a[i]);
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) nl += __shfl_xor(nl, o, kWave);
+      if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < 36; i++) sb[i * NW + wid] = a[i];
+        sc[wid] = nl;
+      }
+      sync_block_lds();
+      int32_t *acc = c.accA + ((size_t)d.s * kCbCopies + blockIdx.x % kCbCopies) * kAccA;
+      const uint32_t i = threadIdx.x - kAtomicWave * kWave;
+      if (wid == kAtomicWave && i < 36) {
+        int32_t r = sb[i * NW];
+        for (int w = 1; w < NW; w++) r = (i % 6) < 3 ? min(r, sb[i * NW + w]) : max(r, sb[i * NW + w]);
+        if (r != acc_identity(i)) {
+          if ((i % 6) < 3) atomicMin(&acc[i], r);
+          else atomicMax(&acc[i], r);
+        }
+      } else if (wid == kAtomicWave && i == 36) {
+        uint32_t tot = 0;
+        for (int w = 0; w < NW; w++) tot += sc[w];
+        if (tot) atomicAdd(&acc[36], (int32_t)tot);
+      }
+      sync_block_lds();
+      for (int i = 0; i < 36; i++) a[i] = acc_identity(i);
+      nl = 0;
+    }
+    if (!more) break;
+    t = t2;
+    d = d2;
+  }
+}
+
+// Plans of the pass's segments, one wave each, from the pass-A sums; the
+// wave also clears the segment's pass-B accumulators.
+__global__ __launch_bounds__(256) void k_plan(Ctx c, int pass) {
+  const int par = pass & 1;
+  const uint32_t n = c.ctr->numLarge[par];
+  const uint32_t s = blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
+  if (s >= n) return;  // (whole waves)
+  const uint32_t lane = lane_id();
+  const int32_t *acc = c.accA + (size_t)s * kCbCopies * kAccA;
+  uint32_t nl = lane < kCbCopies ? (uint32_t)acc[lane * kAccA + 36] : 0u;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) nl += __shfl_xor(nl, o, kWave);
+  const Seg g = c.segs[par][s];
+  const uint32_t size = g.end - g.begin;
+  const uint32_t mode = nl == size ? 1u : (nl == 0 ? 2u : 0u);
+  // lane j < 12: word j of the children's bound sets, over the copies
+  int32_t v = acc_identity(lane % 12);
+  if (lane < 12)
+    for (uint32_t cp = 0; cp < kCbCopies; cp++) {
+      const int32_t x = acc[cp * kAccA + 12 * mode + lane];
+      v = (lane % 6) < 3 ? min(v, x) : max(v, x);
+    }
+  int32_t cb[12];
+#pragma unroll
+  for (int j = 0; j < 12; j++) cb[j] = __shfl(v, j, kWave);
+  if (lane == 0) {
+    PairPlan P{};
+    P.begin = g.begin; P.end = g.end; P.pbeg = g.pbeg; P.rev = g.rev;
+    P.mid = mode ? g.begin + size / 2 : g.begin + nl;  // spatialMedian.h:107 for one-sided splits
+    P.mode = mode;
+    P.dim = (uint32_t)c.segDim[par][s];
+    P.pos = __float_as_uint(c.segPos[par][s]);
+    for (int side = 0; side < 2; side++) {
+      const uint32_t sz = side ? g.end - P.mid : P.mid - g.begin;
+      P.split[side] = sz > (uint32_t)kMedium && sz > c.T;
+      float lo[3], hi[3];
+      for (int d = 0; d < 3; d++) {
+        P.ccb[side][d] = cb[6 * side + d];
+        P.ccb[side][d + 3] = cb[6 * side + d + 3];
+        lo[d] = dec_f(cb[6 * side + d]);
+        hi[d] = dec_f(cb[6 * side + d + 3]);
+      }
+      if (P.split[side]) {
+        int dim;
+        float pos;
+        choose_split(lo, hi, dim, pos);
+        P.cdim[side] = (uint32_t)dim;
+        P.cpos[side] = __float_as_uint(pos);
+      }
+    }
+    c.plan[s] = P;
+  }
+  if (lane < kCbCopies * kAccB / 4)
+    reinterpret_cast<int4 *>(c.accB + (size_t)s * kCbCopies * kAccB)[lane] = acc_b_clear(lane);
+  // clean the pass-A slots for the next pass that uses segment index s
+  // (every slot starts clean: k_init_acc)
+  int4 *a4 = reinterpret_cast<int4 *>(c.accA + (size_t)s * kCbCopies * kAccA);
+  for (uint32_t q = lane; q < kCbCopies * kAccA / 4; q += kWave) a4[q] = acc_a_clear(q);
+}
+
+// Clears every segment slot's pass-A accumulators once per build.
+__global__ void k_init_acc(Ctx c) {
+  constexpr uint32_t kGroups = kCbCopies * kAccA / 4;
+  int4 *a4 = reinterpret_cast<int4 *>(c.accA);
+  const size_t total = (size_t)c.maxLarge * kGroups;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
+    a4[i] = acc_a_clear((uint32_t)(i % kGroups));
+}
+
+// Ranks of striped items in four classes (4 = none): rank[k] = number of
+// items of the same class with a smaller tile index; tot[q] = class sizes.
+// smem needs 4*ITEMS*BLOCK/64 + 4 words.
+template <int BLOCK, int ITEMS>
+__device__ inline void block_rank4_striped(const uint32_t (&cls)[ITEMS], uint32_t (&rank)[ITEMS],
+                                           uint32_t (&tot)[4], uint32_t *smem) {
+  constexpr int NW = BLOCK / kWave;
+  constexpr int NC = ITEMS * NW;  // counters per class
+  static_assert(4 * NC == 2 * kWave, "two counters per lane");
+  const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < ITEMS; k++) {
+    uint32_t r = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < 4; q++) {
+      const uint64_t m = __ballot(cls[k] == q);
+      if (lane == 0) smem[(q * ITEMS + k) * NW + wid] = (uint32_t)__popcll(m);
+      if (cls[k] == q) r = mbcnt(m);
+    }
+    rank[k] = r;
+  }
+  sync_block_lds();
+  if (wid == 0) {
+    // lane l scans counters 2l, 2l+1; class q owns lanes [16q, 16q+16)
+    const uint32_t a = smem[2 * lane], b = smem[2 * lane + 1];
+    const uint32_t incl = wave_incl_scan(a + b);
+    const uint32_t ex = incl - a - b;
+    const uint32_t base = (uint32_t)__shfl((int)ex, (int)(lane & ~15u), kWave);
+    smem[2 * lane] = ex - base;
+    smem[2 * lane + 1] = ex - base + a;
+    if ((lane & 15u) == 15u) smem[4 * NC + (lane >> 4)] = incl - base;
+  }
+  sync_block_lds();
+#pragma unroll
+  for (int k = 0; k < ITEMS; k++)
+    if (cls[k] < 4) rank[k] += smem[(cls[k] * ITEMS + k) * NW + wid];
+#pragma unroll
+  for (int q = 0; q < 4; q++) tot[q] = smem[4 * NC + q];
+  sync_block_lds();
+}
+
+// Pass B, software-pipelined over a workgroup's tiles like a one-level
+// partition: a workgroup claims tiles in ticket order (so a tile only ever
+// waits on tiles claimed by running workgroups), classifies its items into
+// the four grandchild classes (child 0 or 1 of S, then that child's left or
+// right side; an unsplit child is one class), stages them in LDS in output
+// order, resolves the four prefixes by decoupled look-back among the tiles
+// of its segment (one wave per class) and copies the staged items out in
+// four contiguous runs.  Class (child, left) fills the child's range from
+// its start, class (child, right) from its end.  After its look-back the
+// workgroup claims its next tile, whose items are loaded before this tile's
+// copy-out.  A workgroup holds at most two tiles, the later one claimed
+// after the earlier; a tile publishes its class counts before it waits on
+// anything, so every wait ends and the grid drains.
+__global__ __launch_bounds__(kBlock, 4) void k_pair(Ctx c, int pass) {
+  constexpr int NW = kBlock / kWave;
+  static_assert(NW == 4, "one look-back wave per class");
+  __shared__ float4 sStage[kTile];
+  __shared__ uint32_t sr[4 * kItems * NW + 4];
+  __shared__ int32_t sb[24 * NW];
+  __shared__ PairPlan sPlan[2];
+  __shared__ uint32_t sTile[2][4];  // tile, segment, first tile, end tile
+  __shared__ uint32_t sPrefix[4];
+  const int par = pass & 1;
+  const uint32_t numTiles = c.ctr->numTiles;
+  if (numTiles == 0) return;  // a pass past the tree's depth
   const float4 *src = c.items[par];
   float4 *dst = c.items[par ^ 1];
-  const uint32_t tag = (uint32_t)level + 1;
-  // counter cc hands out tiles cc, cc + kTicketCounters, ...: every tile a
-  // claimed tile waits on is either claimed (and held by a running
-  // workgroup) or next in its counter, which running workgroups serve
+  const uint32_t tag = (uint32_t)pass + 1;
+  const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
   const uint32_t cc = blockIdx.x % kTicketCounters;
-  auto fetch = [&]() {
-    TileDesc d;
-    d.t = cc + kTicketCounters * atomicAdd(&c.tickets[kTicketStride * cc], 1u);
-    if (d.t < numTiles) {
-      d.seg = c.tileSeg[par][d.t];
-      const Seg g = c.segs[par][d.seg];
-      d.begin = g.begin;
-      d.end = g.end;
-      d.dim = (uint32_t)c.segDim[par][d.seg];
-      d.pos = __float_as_uint(c.segPos[par][d.seg]);
-      d.tileFirst = c.tileStart[par][d.seg];
-      d.tileEnd = c.tileStart[par][d.seg + 1];
+  auto fetch = [&](int sl) {  // one thread
+    const uint32_t t = cc + kTicketCounters * atomicAdd(&c.tickets[kTicketStride * cc], 1u);
+    sTile[sl][0] = t;
+    if (t < numTiles) {
+      const uint32_t sg = c.tileSeg[par][t];
+      sTile[sl][1] = sg;
+      sTile[sl][2] = c.tileStart[par][sg];
+      sTile[sl][3] = c.tileStart[par][sg + 1];
+      sPlan[sl] = c.plan[sg];
     }
-    sNext = d;
   };
-  if (threadIdx.x == 0) fetch();
+  auto U = [](uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); };
+  int slot = 0;
+  if (threadIdx.x == 0) fetch(0);
   sync_block_lds();
-  TileDesc cur = sNext;
-  if (cur.t >= numTiles) return;
+  uint32_t t = U(sTile[0][0]);
+  if (t >= numTiles) return;
   float4 it[kItems];
+  // class bounds, per thread, over the workgroup's consecutive tiles of one
+  // segment; added to the segment's accumulators when the segment changes,
+  // by threads 0-23, after the next tile's look-back (a wave's memory
+  // counter retires in order: atomics issued just before a look-back would
+  // hold up its status loads)
+  int32_t cacc[24];
+  for (int i = 0; i < 24; i++) cacc[i] = acc_identity(i);
+  int32_t pend = 0;
+  uint32_t pendSeg = 0;
+  bool pendOn = false;
+  auto addB = [&](uint32_t sg, int32_t r) {
+    const uint32_t i = threadIdx.x;
+    if (i < 24 && r != acc_identity(i)) {
+      int32_t *acc = c.accB + ((size_t)sg * kCbCopies + blockIdx.x % kCbCopies) * kAccB;
+      if ((i % 6) < 3) atomicMin(&acc[i], r);
+      else atomicMax(&acc[i], r);
+    }
+  };
   {
-    const uint32_t tb = cur.begin + (cur.t - cur.tileFirst) * kTile;
-    const uint32_t last = min(tb + (uint32_t)kTile, cur.end) - 1;
-#pragma unroll
-    for (int j = 0; j < kItems; j++) it[j] = src[min(tb + j * kBlock + threadIdx.x, last)];
+    const uint32_t n = U(sPlan[0].end) - U(sPlan[0].begin);
+    const uint32_t j0 = (t - U(sTile[0][2])) * kTile;
+    load_tile(src, U(sPlan[0].pbeg), n, U(sPlan[0].rev), j0, min(n - j0, (uint32_t)kTile), c.n, it);
   }
   while (true) {
-    const uint32_t k = cur.t - cur.tileFirst;
-    const uint32_t tb = cur.begin + k * kTile;
-    const uint32_t n = min(tb + (uint32_t)kTile, cur.end) - tb;
-    const int dim = (int)cur.dim;
-    const float pos = __uint_as_float(cur.pos);
-    bool left[kItems];
+    const PairPlan &P = sPlan[slot];
+    const uint32_t seg = U(sTile[slot][1]), tileFirst = U(sTile[slot][2]), tileEnd = U(sTile[slot][3]);
+    const uint32_t begin = U(P.begin), end = U(P.end), mid = U(P.mid), mode = U(P.mode);
+    const int dim = (int)U(P.dim);
+    const float pos = __uint_as_float(U(P.pos));
+    const uint32_t split0 = U(P.split[0]), split1 = U(P.split[1]);
+    const int cdim0 = (int)U(P.cdim[0]), cdim1 = (int)U(P.cdim[1]);
+    const float cpos0 = __uint_as_float(U(P.cpos[0])), cpos1 = __uint_as_float(U(P.cpos[1]));
+    const uint32_t n = end - begin, h = n / 2;
+    const uint32_t k = t - tileFirst;
+    const uint32_t j0 = k * kTile;
+    const uint32_t cnt = min(n - j0, (uint32_t)kTile);
+    uint32_t cls[kItems];
 #pragma unroll
-    for (int j = 0; j < kItems; j++) left[j] = j * kBlock + threadIdx.x < n && coord(it[j], dim) < pos;
-    uint32_t rank[kItems];
-    // its barriers also order the previous tile's reads of sStage, sNext
-    // and sPrefix before this tile's writes
-    const uint32_t nLeft = block_rank_striped<kBlock, kItems, true>(left, rank, sr);
-
-    // stage in output order: lefts ascending, rights reversed (the right
-    // side fills its range from the end)
-#pragma unroll
-    for (int j = 0; j < kItems; j++) {
-      const uint32_t l = j * kBlock + threadIdx.x;
-      if (l < n) sStage[left[j] ? rank[j] : n - 1 - (l - rank[j])] = it[j];
+    for (int jj = 0; jj < kItems; jj++) {
+      const uint32_t q = (uint32_t)(jj * kBlock) + threadIdx.x;
+      const uint32_t j = j0 + q;
+      const uint32_t c1 = mode == 0 ? (coord(it[jj], dim) < pos ? 0u : 1u)
+                                    : (mode == 1 ? (j >= h ? 1u : 0u) : (j < n - h ? 1u : 0u));
+      const bool sp = c1 ? split1 != 0 : split0 != 0;
+      const int cd = c1 ? cdim1 : cdim0;
+      const float cpv = c1 ? cpos1 : cpos0;
+      const uint32_t c2 = (sp && !(coord(it[jj], cd) < cpv)) ? 1u : 0u;
+      cls[jj] = q < cnt ? 2 * c1 + c2 : 4u;
     }
-    if (threadIdx.x < kWave) {
-      const uint32_t prefix = lookback_exclusive_wave(c.tileStatus, cur.t, k, tag, nLeft);
-      if (threadIdx.x == 0) {
-        sPrefix = prefix;
-        if (cur.t == cur.tileEnd - 1) c.segLeft[par][cur.seg] = prefix + nLeft;
-        fetch();
+    uint32_t rank[kItems], tot[4];
+    // its barriers also order the previous tile's reads of sStage, sPlan,
+    // sTile and sPrefix before this tile's writes
+    block_rank4_striped<kBlock, kItems>(cls, rank, tot, sr);
+    const uint32_t off1 = tot[0], off2 = off1 + tot[1], off3 = off2 + tot[2];
+#pragma unroll
+    for (int jj = 0; jj < kItems; jj++) {
+      const uint32_t q = cls[jj], r = rank[jj];
+      if (q < 4) {
+        const uint32_t idx = q == 0 ? r : (q == 1 ? off2 - 1 - r : (q == 2 ? off2 + r : off3 + tot[3] - 1 - r));
+        sStage[idx] = it[jj];
+      }
+    }
+    {
+      const uint32_t mine = wid == 0 ? tot[0] : (wid == 1 ? tot[1] : (wid == 2 ? tot[2] : tot[3]));
+      const uint32_t pre = lookback_exclusive_wave(c.tileStatus + (size_t)wid * c.maxTiles, t, k, tag, mine);
+      if (lane == 0) {
+        sPrefix[wid] = pre;
+        if (t == tileEnd - 1) c.classTot[4 * (size_t)seg + wid] = pre + mine;
+        if (wid == 0) fetch(slot ^ 1);
       }
     }
     sync_block_lds();
-    const uint32_t P = sPrefix;
-    const TileDesc nxt = sNext;
-    // the next tile's loads go out before this tile's stores (the asm
-    // statement keeps the compiler from sinking them below the copy-out;
-    // lanes past a short tile re-read its last item)
-    if (nxt.t < numTiles) {
-      const uint32_t tb2 = nxt.begin + (nxt.t - nxt.tileFirst) * kTile;
-      const uint32_t last = min(tb2 + (uint32_t)kTile, nxt.end) - 1;
-#pragma unroll
-      for (int j = 0; j < kItems; j++) it[j] = src[min(tb2 + j * kBlock + threadIdx.x, last)];
+    const uint32_t p0 = sPrefix[0], p1 = sPrefix[1], p2 = sPrefix[2], p3 = sPrefix[3];
+    const int nslot = slot ^ 1;
+    const uint32_t nt = U(sTile[nslot][0]);
+    // the next tile's loads go out before this tile's stores
+    if (nt < numTiles) {
+      const PairPlan &Q = sPlan[nslot];
+      const uint32_t n2 = U(Q.end) - U(Q.begin);
+      const uint32_t j02 = (nt - U(sTile[nslot][2])) * kTile;
+      load_tile(src, U(Q.pbeg), n2, U(Q.rev), j02, min(n2 - j02, (uint32_t)kTile), c.n, it);
       asm volatile("" ::: "memory");
     }
-    // copy out: staged [0, nLeft) -> begin + P + i; staged [nLeft, n) ->
-    // the right run, which ends rightsBefore items before the node's end;
-    // child centroid bounds on the way (left lo/hi, right lo/hi)
-    const uint32_t leftBase = cur.begin + P;
-    const uint32_t rightBase = cur.end - (k * kTile - P) - n;
-    float v[12];
-    {
-      float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (pendOn) {
+      addB(pendSeg, pend);
+      pendOn = false;
+    }
+    // copy out: staged class q -> its run; bounds of each class on the way
+    const uint32_t runOff[4] = {0u, off1, off2, off3};
+    const uint32_t runBase[4] = {begin + p0, mid - p1 - tot[1], mid + p2, end - p3 - tot[3]};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint32_t o = runOff[q], cntq = tot[q], rb = runBase[q];
 #pragma unroll 4
-      for (uint32_t i = threadIdx.x; i < nLeft; i += kBlock) {
-        const float4 x = sStage[i];
-        dst[leftBase + i] = x;
-        lo[0] = fminf(lo[0], x.x); lo[1] = fminf(lo[1], x.y); lo[2] = fminf(lo[2], x.z);
-        hi[0] = fmaxf(hi[0], x.x); hi[1] = fmaxf(hi[1], x.y); hi[2] = fmaxf(hi[2], x.z);
-      }
-      for (int d = 0; d < 3; d++) { v[d] = lo[d]; v[d + 3] = hi[d]; }
-    }
-    {
-      float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-#pragma unroll 4
-      for (uint32_t i = nLeft + threadIdx.x; i < n; i += kBlock) {
-        const float4 x = sStage[i];
-        dst[rightBase + i] = x;
-        lo[0] = fminf(lo[0], x.x); lo[1] = fminf(lo[1], x.y); lo[2] = fminf(lo[2], x.z);
-        hi[0] = fmaxf(hi[0], x.x); hi[1] = fmaxf(hi[1], x.y); hi[2] = fmaxf(hi[2], x.z);
-      }
-      for (int d = 0; d < 3; d++) { v[d + 6] = lo[d]; v[d + 9] = hi[d]; }
-    }
-    block_reduce_bounds<kBlock, true>(v, sb);
-    if (threadIdx.x < 12) {
-      const int side = threadIdx.x / 6;
-      const int j = threadIdx.x % 6;
-      const bool nonEmpty = side == 0 ? (nLeft > 0) : (nLeft < n);
-      if (nonEmpty) {
-        int32_t *dstCb =
-            &c.childCb[par][((size_t)(blockIdx.x % kCbCopies) * 2 * c.maxLarge + 2 * cur.seg + side) * 6 + j];
-        if (j < 3) atomicMin(dstCb, enc_f(v[threadIdx.x]));
-        else atomicMax(dstCb, enc_f(v[threadIdx.x]));
+      for (uint32_t i = threadIdx.x; i < cntq; i += kBlock) {
+        const float4 x = sStage[o + i];
+        dst[min(rb + i, c.n - 1)] = x;  // (in range by construction; clamped to the buffer)
+        const int32_t ex = enc_f(x.x), ey = enc_f(x.y), ez = enc_f(x.z);
+        CUBQL_ADD6(cacc, 6 * q, ex, ey, ez);
       }
     }
-    if (nxt.t >= numTiles) break;
-    cur = nxt;
+    if (nt >= numTiles || U(sTile[nslot][1]) != seg) {
+#pragma unroll
+      for (int i = 0; i < 24; i++) cacc[i] = (i % 6) < 3 ? wave_min_i32(cacc[i]) : wave_max_i32(cacc[i]);
+      if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < 24; i++) sb[i * NW + wid] = cacc[i];
+      }
+      sync_block_lds();
+      int32_t r = 0;
+      if (threadIdx.x < 24) {
+        const uint32_t i = threadIdx.x;
+        r = sb[i * NW];
+        for (int w = 1; w < NW; w++) r = (i % 6) < 3 ? min(r, sb[i * NW + w]) : max(r, sb[i * NW + w]);
+      }
+      sync_block_lds();
+      if (nt >= numTiles) {
+        addB(seg, r);
+      } else {
+        pend = r;
+        pendSeg = seg;
+        pendOn = true;
+      }
+      for (int i = 0; i < 24; i++) cacc[i] = acc_identity(i);
+    }
+    if (nt >= numTiles) break;
+    t = nt;
+    slot = nslot;
   }
 }
 
-// Emit, in three passes over the level's segments (count, scan, write), so
-// levels with tens of thousands of segments run on the whole chip: node
-// records for the split nodes, the next level's segments (with their split
-// planes, tile offsets and tile->segment map: the next level's plan), the
-// small-segment list and fixups for one-sided (degenerate) splits.  Output
-// order is segment order, so numbering is deterministic.
-constexpr int kEmitBlock = 256;
-
-struct EmitSeg {
-  Seg g;
-  uint32_t cb[2], ce[2];
-  bool large[2];
-  bool oneSided;
+// A grandchild of S under child range [cb, ce) (child order rv relative to
+// S), side gs: its range in the CPU order, its storage direction and whether
+// it stays open.  nCL = the child's left-side count.  A one-sided child
+// split takes the halves of the child's range; a reversed child is first
+// turned around in place by the emit (otherwise each half would sit in the
+// other half's range), so its halves are stored in order.
+struct GrandKid {
+  uint32_t b, e, rev;
+  bool large, oneSided;
 };
-
-__device__ __forceinline__ void emit_classify(const Ctx &c, int par, uint32_t s, EmitSeg &e) {
-  e.g = c.segs[par][s];
-  const uint32_t size = e.g.end - e.g.begin;
-  const uint32_t nl = c.segLeft[par][s];
-  e.oneSided = (nl == 0 || nl == size);
-  const uint32_t mid = e.oneSided ? (uint32_t)(((uint64_t)e.g.begin + e.g.end) / 2) : e.g.begin + nl;
-  e.cb[0] = e.g.begin; e.ce[0] = mid;
-  e.cb[1] = mid; e.ce[1] = e.g.end;
-  for (int side = 0; side < 2; side++) {
-    const uint32_t sz = e.ce[side] - e.cb[side];
-    e.large[side] = sz > (uint32_t)kMedium && sz > c.T;
-  }
+__device__ __forceinline__ GrandKid grand_kid(uint32_t cb, uint32_t ce, uint32_t nCL, uint32_t rv, uint32_t gs,
+                                              uint32_t T) {
+  GrandKid g;
+  const uint32_t sz = ce - cb;
+  g.oneSided = nCL == 0 || nCL == sz;
+  const uint32_t mid = g.oneSided ? cb + sz / 2 : cb + nCL;  // spatialMedian.h:107
+  g.b = gs ? mid : cb;
+  g.e = gs ? ce : mid;
+  g.rev = g.oneSided ? 0u : rv;
+  g.large = g.e - g.b > (uint32_t)kMedium && g.e - g.b > T;
+  return g;
 }
 
-// packed per-segment counts: large children | small children << 10 | fixups << 20
-__device__ __forceinline__ void emit_counts(const EmitSeg &e, uint32_t &lsf, uint32_t &tiles, uint32_t &items) {
-  lsf = 0; tiles = 0; items = 0;
-  for (int side = 0; side < 2; side++) {
-    if (e.large[side]) {
-      lsf += 1u + (e.oneSided ? (1u << 20) : 0u);
-      tiles += div_up(e.ce[side] - e.cb[side], kTile);
-      items += e.ce[side] - e.cb[side];
-    } else {
-      lsf += 1u << 10;
-    }
-  }
-}
+// Emit of a pass: each workgroup takes kEmitBlock segments, counts per
+// segment the next pass's segments, small segments, tiles, prims and split
+// children, scans the counts within the workgroup, gets its offsets from
+// the workgroups before it by decoupled look-back (one wave per count;
+// workgroups only wait on lower ones, which the dispatcher starts first),
+// and writes: S's node and its split children's nodes (numbered after all
+// of the pass's children), the next pass's segments (with planes, tile
+// offsets and the tile->segment map) and the small-segment list.  Halves
+// of one-sided child splits get their bounds and planes from their prims.
+// The last workgroup writes the pass totals and the next pass's bases and
+// clears this pass's segment count (every workgroup has read it before it
+// published; the last one has waited for all of them).  A pass with no
+// segments only carries its bookkeeping forward.
+constexpr int kEmitBlock = 512;
+constexpr int kEmitCounts = 5;
 
-// One emit launch per level: each workgroup classifies kEmitBlock segments,
-// scans their counts (next-level large segments, small segments, tiles,
-// prims) within the workgroup, gets its offsets from the workgroups before
-// it by decoupled look-back (one wave per count; workgroups only wait on
-// lower ones, which the dispatcher starts first), and writes the node
-// records, the next level's segments and plan, the tile->segment map and
-// the small-segment list.  Children of one-sided (degenerate) splits are
-// halves of the node's range whose centroid bounds the partition did not
-// see: the workgroup reduces them from the prims and plans them itself.
-// The last workgroup writes the level totals and the next level's bases
-// (into the other parity's slots) and clears this level's segment count;
-// every workgroup has read that count before it published, and the last
-// one has waited for all of them.  A level with no segments only carries
-// its bookkeeping forward.
-__global__ __launch_bounds__(kEmitBlock) void k_emit(Ctx c, int level) {
-  __shared__ uint32_t ss[kEmitBlock / kWave + 1];
-  __shared__ uint32_t sPre[4];
-  __shared__ uint3 sFix[2 * kEmitBlock];  // (segment, begin, end) of one-sided halves
-  __shared__ uint32_t sNumFix;
-  __shared__ float sb[12 * (kEmitBlock / kWave)];
-  const int par = level & 1, np = par ^ 1;
+__global__ __launch_bounds__(kEmitBlock) void k_emit2(Ctx c, int pass) {
+  constexpr int NW = kEmitBlock / kWave;
+  __shared__ uint32_t ss[NW + 1];
+  __shared__ uint32_t sPre[kEmitCounts];
+  __shared__ uint3 sFix[4 * kEmitBlock];  // (segment, first slot, size) of open halves
+  __shared__ uint2 sRev[2 * kEmitBlock];  // (first slot, size) of ranges to turn around
+  __shared__ uint32_t sNumFix, sNumRev;
+  __shared__ float sb[12 * NW];
+  const int par = pass & 1, np = par ^ 1;
   Counters *k = c.ctr;
   // (read once: the last workgroup clears it)
   const uint32_t n = __hip_atomic_load(&k->numLarge[par], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -491,8 +837,9 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit(Ctx c, int level) {
       k->baseNode[np] = k->baseNode[par];
       k->baseSmall[np] = k->baseSmall[par];
       k->baseTop[np] = k->baseTop[par];
-      c.levelStart[level + 1] = k->baseTop[par];
-      c.levelItems[level + 1] = 0;
+      c.levelStart[2 * pass + 1] = k->baseTop[par];
+      c.levelStart[2 * pass + 2] = k->baseTop[par];
+      c.levelItems[pass + 1] = 0;
     }
     return;
   }
@@ -501,139 +848,197 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit(Ctx c, int level) {
   const uint32_t s = blockIdx.x * kEmitBlock + threadIdx.x;
   const uint32_t nodeBase = k->baseNode[par], smallBase = k->baseSmall[par], topBase = k->baseTop[par];
   const bool pts = k->allPoints != 0;
-  const uint32_t tag = (uint32_t)level + 1;
-  EmitSeg e;
-  uint32_t nL = 0, nS = 0, nT = 0, nI = 0;
+  const uint32_t tag = (uint32_t)pass + 1;
+  const uint32_t T = c.T;
+  uint32_t begin = 0, end = 0, mid = 0, mode = 0, split[2] = {0, 0}, nCL[2] = {0, 0};
+  uint32_t cnt[kEmitCounts] = {0, 0, 0, 0, 0};  // open, small, tiles, prims, split children
+  // everything the writes below read, loaded before the first store (which
+  // the compiler would not move loads across) and before the scans
+  int32_t ccb[12], gcb[24];
+  Seg g0{};
   if (s < n) {
-    emit_classify(c, par, s, e);
+    const PairPlan &P = c.plan[s];
+    begin = P.begin; end = P.end; mid = P.mid; mode = P.mode;
+    split[0] = P.split[0]; split[1] = P.split[1];
+    for (int j = 0; j < 12; j++) ccb[j] = P.ccb[j / 6][j % 6];
+    nCL[0] = c.classTot[4 * (size_t)s + 0];
+    nCL[1] = c.classTot[4 * (size_t)s + 2];
+    g0 = c.segs[par][s];
+    for (int j = 0; j < 24; j++) gcb[j] = acc_identity(j);
+    if (split[0] | split[1]) {
+      const int4 *a = reinterpret_cast<const int4 *>(c.accB + (size_t)s * kCbCopies * kAccB);
+#pragma unroll
+      for (uint32_t cp = 0; cp < kCbCopies; cp++) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+          const int4 x = a[cp * 6 + q];
+          const int w = 4 * q;
+          gcb[w] = (w % 6) < 3 ? min(gcb[w], x.x) : max(gcb[w], x.x);
+          gcb[w + 1] = ((w + 1) % 6) < 3 ? min(gcb[w + 1], x.y) : max(gcb[w + 1], x.y);
+          gcb[w + 2] = ((w + 2) % 6) < 3 ? min(gcb[w + 2], x.z) : max(gcb[w + 2], x.z);
+          gcb[w + 3] = ((w + 3) % 6) < 3 ? min(gcb[w + 3], x.w) : max(gcb[w + 3], x.w);
+        }
+      }
+    }
     for (int side = 0; side < 2; side++) {
-      if (e.large[side]) {
-        nL++;
-        nT += div_up(e.ce[side] - e.cb[side], kTile);
-        nI += e.ce[side] - e.cb[side];
-      } else {
-        nS++;
+      const uint32_t cb = side ? mid : begin, ce = side ? end : mid;
+      if (!split[side]) { cnt[1]++; continue; }
+      cnt[4]++;
+      for (uint32_t gs = 0; gs < 2; gs++) {
+        const GrandKid g = grand_kid(cb, ce, nCL[side], child_rev(mode, side), gs, T);
+        if (g.large) { cnt[0]++; cnt[2] += div_up(g.e - g.b, kTile); cnt[3] += g.e - g.b; }
+        else cnt[1]++;
       }
     }
   }
-  if (threadIdx.x == 0) sNumFix = 0;
-  uint32_t tL, tS, tT, tI;
-  const uint32_t exL = block_excl_scan<kEmitBlock, true>(nL, ss, &tL);
-  const uint32_t exS = block_excl_scan<kEmitBlock, true>(nS, ss, &tS);
-  const uint32_t exT = block_excl_scan<kEmitBlock, true>(nT, ss, &tT);
-  const uint32_t exI = block_excl_scan<kEmitBlock, true>(nI, ss, &tI);
+  if (threadIdx.x == 0) sNumFix = sNumRev = 0;
+  uint32_t ex[kEmitCounts], tt[kEmitCounts];
+#pragma unroll
+  for (int q = 0; q < kEmitCounts; q++) ex[q] = block_excl_scan<kEmitBlock, true>(cnt[q], ss, &tt[q]);
   const uint32_t w = threadIdx.x / kWave;
-  if (w < 4) {
-    const uint32_t v = w == 0 ? tL : (w == 1 ? tS : (w == 2 ? tT : tI));
+  if (w < (uint32_t)kEmitCounts) {
+    uint32_t v = tt[0];
+#pragma unroll
+    for (int q = 1; q < kEmitCounts; q++) v = w == (uint32_t)q ? tt[q] : v;
     const uint32_t pre =
         lookback_exclusive_wave(c.emitStatus + (size_t)w * c.maxEmitBlocks, blockIdx.x, blockIdx.x, tag, v);
     if (lane_id() == 0) sPre[w] = pre;
   }
   sync_block_lds();
-  const uint32_t preL = sPre[0], preS = sPre[1], preT = sPre[2], preI = sPre[3];
+  uint32_t pre[kEmitCounts];
+#pragma unroll
+  for (int q = 0; q < kEmitCounts; q++) pre[q] = sPre[q];
   if (blockIdx.x == nb - 1 && threadIdx.x == 0) {
-    const uint32_t runL = preL + tL, runS = preS + tS, runT = preT + tT, runI = preI + tI;
-    k->nodeCount = k->baseNode[np] = nodeBase + 2 * n;
+    const uint32_t runL = pre[0] + tt[0], runS = pre[1] + tt[1], runT = pre[2] + tt[2], runI = pre[3] + tt[3];
+    const uint32_t runX = pre[4] + tt[4];
+    k->nodeCount = k->baseNode[np] = nodeBase + 2 * n + 2 * runX;
     k->numSmall = k->baseSmall[np] = smallBase + runS;
-    k->numTop = k->baseTop[np] = topBase + n;
+    k->numTop = k->baseTop[np] = topBase + n + runX;
     k->numLarge[np] = runL;
     k->numLarge[par] = 0;
-    c.levelStart[level + 1] = topBase + n;
+    c.levelStart[2 * pass + 1] = topBase + n;
+    c.levelStart[2 * pass + 2] = topBase + n + runX;
     k->numTiles = runT;
     k->ticket = 0;
-    for (uint32_t cc = 0; cc < kTicketCounters; cc++) c.tickets[kTicketStride * cc] = 0;
+    for (uint32_t q = 0; q < kTicketCounters; q++) c.tickets[kTicketStride * q] = 0;
     c.tileStart[np][runL] = runT;
-    c.levelItems[level + 1] = runI;
+    c.levelItems[pass + 1] = runI;
   }
-  uint32_t fillStart[2] = {0, 0}, fillCount[2] = {0, 0}, fillSeg[2] = {0, 0};
+  uint32_t fillStart[4] = {0, 0, 0, 0}, fillCount[4] = {0, 0, 0, 0}, fillSeg[4] = {0, 0, 0, 0};
   if (s < n) {
-    uint32_t li = preL + exL, si = preS + exS, ti = preT + exT;
+    uint32_t li = pre[0] + ex[0], si = pre[1] + ex[1], ti = pre[2] + ex[2], xi = pre[4] + ex[4];
+    const Seg &g = g0;
     const uint32_t child = nodeBase + 2 * s;
     if (pts) {
-      // point input: the node's box is its centroid bounds, final here
+      // point input: a node's box is its centroid bounds, final here
       Node nd;
-      for (int d = 0; d < 3; d++) { nd.lo[d] = dec_f(e.g.cb[d]); nd.hi[d] = dec_f(e.g.cb[d + 3]); }
+      for (int d = 0; d < 3; d++) { nd.lo[d] = dec_f(g.cb[d]); nd.hi[d] = dec_f(g.cb[d + 3]); }
       nd.oc = pack_oc(child, 0);
-      c.nodes[e.g.node] = nd;
+      c.nodes[g.node] = nd;
     } else {
-      c.nodes[e.g.node].oc = pack_oc(child, 0);
+      c.nodes[g.node].oc = pack_oc(child, 0);
     }
-    c.topNodes[topBase + s] = e.g.node;
-    Seg *next = c.segs[np];
+    c.topNodes[topBase + s] = g.node;
+    int nf = 0;
     for (int side = 0; side < 2; side++) {
-      if (e.large[side]) {
-        Seg ns;
-        ns.begin = e.cb[side]; ns.end = e.ce[side]; ns.node = child + side; ns.pad = 0;
-        ns.pad2[0] = ns.pad2[1] = 0;
-        for (int j = 0; j < 6; j++) {
-          int32_t v = c.childCb[par][(2 * s + side) * 6 + j];
-          for (uint32_t cp = 1; cp < kCbCopies; cp++) {
-            const int32_t w2 = c.childCb[par][((size_t)cp * 2 * c.maxLarge + 2 * s + side) * 6 + j];
-            v = j < 3 ? min(v, w2) : max(v, w2);
-          }
-          ns.cb[j] = v;
-        }
-        next[li] = ns;
-        // plan of the next level (one-sided children are planned below)
-        float lo[3], hi[3];
-        for (int d = 0; d < 3; d++) { lo[d] = dec_f(ns.cb[d]); hi[d] = dec_f(ns.cb[d + 3]); }
-        int dim;
-        float pos;
-        choose_split(lo, hi, dim, pos);
-        c.segDim[np][li] = dim;
-        c.segPos[np][li] = pos;
-        c.segLeft[np][li] = 0;
-        for (uint32_t cp = 0; cp < kCbCopies; cp++)
-          for (int j = 0; j < 12; j++)
-            c.childCb[np][(size_t)cp * 12 * c.maxLarge + li * 12 + j] = enc_f((j % 6) < 3 ? INFINITY : -INFINITY);
-        const uint32_t tiles = div_up(ns.end - ns.begin, kTile);
-        c.tileStart[np][li] = ti;
-        fillStart[side] = ti;
-        fillCount[side] = tiles;
-        fillSeg[side] = li;
-        ti += tiles;
-        if (e.oneSided) sFix[atomicAdd(&sNumFix, 1u)] = make_uint3(li, ns.begin, ns.end);
-        li++;
-      } else {
-        c.small[smallBase + si] = SmallSeg{e.cb[side], e.ce[side], child + side, (uint32_t)np};
+      const uint32_t cb = side ? mid : begin, ce = side ? end : mid;
+      const uint32_t rv = child_rev(mode, side);
+      const uint32_t cid = child + side;
+      if (!split[side]) {
+        c.small[smallBase + si] = SmallSeg{cb, ce, cid, (uint32_t)np, cb, rv, 0, 0};
         si++;
+        continue;
+      }
+      const uint32_t gid = nodeBase + 2 * n + 2 * xi;
+      if (pts) {
+        Node nd;
+        for (int d = 0; d < 3; d++) { nd.lo[d] = dec_f(ccb[6 * side + d]); nd.hi[d] = dec_f(ccb[6 * side + d + 3]); }
+        nd.oc = pack_oc(gid, 0);
+        c.nodes[cid] = nd;
+      } else {
+        c.nodes[cid].oc = pack_oc(gid, 0);
+      }
+      c.topNodes[topBase + n + xi] = cid;
+      xi++;
+      if (rv && (nCL[side] == 0 || nCL[side] == ce - cb)) sRev[atomicAdd(&sNumRev, 1u)] = make_uint2(cb, ce - cb);
+      for (uint32_t gs = 0; gs < 2; gs++) {
+        const GrandKid gk = grand_kid(cb, ce, nCL[side], rv, gs, T);
+        if (!gk.large) {
+          c.small[smallBase + si] = SmallSeg{gk.b, gk.e, gid + gs, (uint32_t)np, gk.b, gk.rev, 0, 0};
+          si++;
+          continue;
+        }
+        Seg ns;
+        ns.begin = gk.b; ns.end = gk.e; ns.node = gid + gs; ns.pbeg = gk.b; ns.rev = gk.rev; ns.pad = 0;
+        for (int j = 0; j < 6; j++) ns.cb[j] = gcb[6 * (2 * side + gs) + j];
+        c.segs[np][li] = ns;
+        if (!gk.oneSided) {
+          float lo[3], hi[3];
+          for (int d = 0; d < 3; d++) { lo[d] = dec_f(ns.cb[d]); hi[d] = dec_f(ns.cb[d + 3]); }
+          int dim;
+          float pos;
+          choose_split(lo, hi, dim, pos);
+          c.segDim[np][li] = dim;
+          c.segPos[np][li] = pos;
+        } else {
+          sFix[atomicAdd(&sNumFix, 1u)] = make_uint3(li, gk.b, gk.e - gk.b);
+        }
+        const uint32_t tiles = div_up(gk.e - gk.b, kTile);
+        c.tileStart[np][li] = ti;
+        fillStart[nf] = ti;
+        fillCount[nf] = tiles;
+        fillSeg[nf] = li;
+        nf++;
+        ti += tiles;
+        li++;
       }
     }
   }
   // tile -> segment map: runs by their own lane, except the long runs of
-  // the top levels' few big segments, which the whole wave writes
+  // the top passes' few big segments, which the whole wave writes
   constexpr uint32_t kOwnFill = 256;
   const uint32_t lane = lane_id();
-  for (int side = 0; side < 2; side++) {
-    if (fillCount[side] <= kOwnFill)
-      for (uint32_t t = 0; t < fillCount[side]; t++) c.tileSeg[np][fillStart[side] + t] = fillSeg[side];
-    uint64_t m = __ballot(fillCount[side] > kOwnFill);
+  for (int f = 0; f < 4; f++) {
+    if (fillCount[f] <= kOwnFill)
+      for (uint32_t t = 0; t < fillCount[f]; t++) c.tileSeg[np][fillStart[f] + t] = fillSeg[f];
+    uint64_t m = __ballot(fillCount[f] > kOwnFill);
     while (m) {
       const int src = __builtin_ctzll(m);
       m &= m - 1;
-      const uint32_t st = __shfl(fillStart[side], src, kWave), cnt = __shfl(fillCount[side], src, kWave);
-      const uint32_t sg = __shfl(fillSeg[side], src, kWave);
-      for (uint32_t t = lane; t < cnt; t += kWave) c.tileSeg[np][st + t] = sg;
+      const uint32_t st = __shfl(fillStart[f], src, kWave), cn = __shfl(fillCount[f], src, kWave);
+      const uint32_t sg = __shfl(fillSeg[f], src, kWave);
+      for (uint32_t t = lane; t < cn; t += kWave) c.tileSeg[np][st + t] = sg;
     }
   }
-  // one-sided splits: centroid bounds and plane of each half from its prims
-  // (a full barrier: the segment records above must land before these
-  // overwrite their bounds)
+  // reversed children split one-sided: turn their ranges around (each
+  // thread swaps a pair of slots), so their halves are stored in order
+  __syncthreads();
+  float4 *items = c.items[np];
+  const uint32_t nRev = sNumRev;
+  for (uint32_t f = 0; f < nRev; f++) {
+    const uint32_t pb = sRev[f].x, sz = sRev[f].y;
+    for (uint32_t i = threadIdx.x; i < sz / 2; i += kEmitBlock) {
+      const float4 a = items[pb + i], b = items[pb + sz - 1 - i];
+      items[pb + i] = b;
+      items[pb + sz - 1 - i] = a;
+    }
+  }
+  // halves of one-sided child splits: centroid bounds and plane from their
+  // prims (a full barrier: the segment records above must land before these
+  // overwrite their bounds, and the turned ranges before these read them)
   __syncthreads();
   const uint32_t nFix = sNumFix;
-  const float4 *items = c.items[np];
   for (uint32_t f = 0; f < nFix; f++) {
-    const uint32_t li = sFix[f].x, gb = sFix[f].y, ge = sFix[f].z;
-    Seg &g = c.segs[np][li];
+    const uint32_t li = sFix[f].x, pb = sFix[f].y, sz = sFix[f].z;
     float v[12];
-    for (int j = 0; j < 3; j++) { v[j] = INFINITY; v[j + 3] = -INFINITY; v[j + 6] = INFINITY; v[j + 9] = -INFINITY; }
-    for (uint32_t i = gb + threadIdx.x; i < ge; i += kEmitBlock) {
-      const float4 it = items[i];
+    for (int j = 0; j < 12; j++) v[j] = (j % 6) < 3 ? INFINITY : -INFINITY;
+    for (uint32_t i = threadIdx.x; i < sz; i += kEmitBlock) {
+      const float4 it = items[pb + i];
       v[0] = fminf(v[0], it.x); v[1] = fminf(v[1], it.y); v[2] = fminf(v[2], it.z);
       v[3] = fmaxf(v[3], it.x); v[4] = fmaxf(v[4], it.y); v[5] = fmaxf(v[5], it.z);
     }
     block_reduce_bounds<kEmitBlock>(v, sb);
-    if (threadIdx.x < 6) g.cb[threadIdx.x] = enc_f(v[threadIdx.x]);
+    if (threadIdx.x < 6) c.segs[np][li].cb[threadIdx.x] = enc_f(v[threadIdx.x]);
     if (threadIdx.x == 0) {
       const float lo[3] = {v[0], v[1], v[2]}, hi[3] = {v[3], v[4], v[5]};
       int dim;
@@ -793,19 +1198,21 @@ __device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, 
   }
 }
 
-// Finishes the node of items [gb, gb + m) of gItems (m <= kSmall, or a leaf
-// of any size up to T) whose local id is rootMe, numbering new nodes from
-// nLocal into loc.
+// Finishes the node of range [gb, gb + m) (m <= kSmall, or a leaf of any
+// size up to T) whose local id is rootMe, numbering new nodes from nLocal
+// into loc.  Its items are gSrc[0 .. m) in the node's order, or (rev) in
+// reverse.
 template <bool PTS>
-__device__ void small_sub(const Ctx &c, const float4 *gItems, uint32_t gb, uint32_t m, uint32_t rootMe,
+__device__ void small_sub(const Ctx &c, const float4 *gSrc, bool rev, uint32_t gb, uint32_t m, uint32_t rootMe,
                           uint32_t &nLocal, Node *loc, float4 *sIt, uint32_t *sFr) {
   const uint32_t lane = lane_id();
   const uint32_t T = c.T;
+  auto ld = [&](uint32_t i) { return gSrc[rev ? m - 1 - i : i]; };
   if (m <= T && m > 64) {
     // leaf of any size up to T
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = lane; i < m; i += kWave) {
-      float4 v = gItems[gb + i];
+      float4 v = ld(i);
       c.primIDs[gb + i] = __float_as_uint(v.w);
       if (PTS) {
         lo[0] = fminf(lo[0], v.x); lo[1] = fminf(lo[1], v.y); lo[2] = fminf(lo[2], v.z);
@@ -818,7 +1225,7 @@ __device__ void small_sub(const Ctx &c, const float4 *gItems, uint32_t gb, uint3
     for (int d = 0; d < 3; d++) { lo[d] = wave_min_dpp(lo[d]); hi[d] = wave_max_dpp(hi[d]); }
     store_node_wave(loc + rootMe, lo, hi, pack_oc(gb, m));
   } else if (m <= 64) {
-    float4 v = lane < m ? gItems[gb + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 v = lane < m ? ld(lane) : make_float4(0.f, 0.f, 0.f, 0.f);
     cubql_box3f bx{};
     if (!PTS && lane < m) bx = load_box(c.boxes, __float_as_uint(v.w));
     reg_subtree<PTS>(c, gb, m, v.x, v.y, v.z, __float_as_uint(v.w), bx.lower[0], bx.lower[1],
@@ -832,7 +1239,7 @@ __device__ void small_sub(const Ctx &c, const float4 *gItems, uint32_t gb, uint3
 #pragma unroll
       for (int k = 0; k < kPer; k++) {
         const uint32_t i = (uint32_t)k * kWave + lane;
-        if (i < m) t[k] = gItems[gb + i];
+        if (i < m) t[k] = ld(i);
       }
 #pragma unroll
       for (int k = 0; k < kPer; k++) {
@@ -933,9 +1340,10 @@ __device__ void small_sub(const Ctx &c, const float4 *gItems, uint32_t gb, uint3
 // segment, local id, item buffer, and (when known) its centroid and box
 // bounds as enc_cmp integers (lo xyz, hi xyz).
 struct MedFrame {
-  uint32_t b, e, me, buf;
+  uint32_t b, e, me, buf;  // buf: bit 0 buffer, bit 1 bounds unknown, bit 2 reversed
+  uint32_t pb, pad[3];     // first slot of the piece's items
   int32_t cb[6];
-  int32_t bb[6];
+  int32_t pad2[2];
 };
 static_assert(sizeof(MedFrame) == 64, "MedFrame");
 constexpr int kMedChunk = 4;  // items per lane per round trip of the medium passes
@@ -1003,18 +1411,20 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
   const uint32_t m = sg.end - sg.begin;
   Node *loc = c.loc + 2 * (size_t)gb;
   uint32_t nLocal = 1;
+  // pieces written by this wave sit in their own range, in order
   auto piece = [&](uint32_t buf, uint32_t b, uint32_t n, uint32_t me) {
-    small_sub<PTS>(c, c.items[buf], gb + b, n, me, nLocal, loc, sIt, sFr);
+    small_sub<PTS>(c, c.items[buf] + gb + b, false, gb + b, n, me, nLocal, loc, sIt, sFr);
     sync_wave_lds();
   };
   if (m <= (uint32_t)kSmall || m <= T) {
-    piece(sg.buf, 0u, m, 0u);
+    small_sub<PTS>(c, c.items[sg.buf] + sg.pbeg, sg.rev != 0, gb, m, 0u, nLocal, loc, sIt, sFr);
   } else {
     int sp = 1;
     bool known = false;  // bounds of the popped frame
     if (lane == 0) {
       MedFrame f0{};
-      f0.b = 0u; f0.e = m; f0.me = 0u; f0.buf = sg.buf | 2u;  // bit 1: bounds unknown
+      f0.b = 0u; f0.e = m; f0.me = 0u; f0.buf = sg.buf | 2u | (sg.rev ? 4u : 0u);
+      f0.pb = sg.pbeg;
       sMed[0] = f0;
     }
     sync_wave_lds();
@@ -1023,13 +1433,16 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
       const MedFrame f = sMed[sp];
       const uint32_t b = f.b, e = f.e, me = f.me, bf = f.buf & 1u;
       known = (f.buf & 2u) == 0;
+      const bool rev = (f.buf & 4u) != 0;
+      const uint32_t pb = f.pb;
       const float4 *src = c.items[bf];
       float4 *dst = c.items[bf ^ 1u];
       int32_t cb[6], bb[6];
       if (known) {
-        for (int j = 0; j < 6; j++) { cb[j] = f.cb[j]; bb[j] = f.bb[j]; }
+        for (int j = 0; j < 6; j++) cb[j] = bb[j] = f.cb[j];
       } else {
-        med_bounds<PTS>(c, src, gb + b, gb + e, cb, bb);
+        // (bounds do not depend on the order)
+        med_bounds<PTS>(c, src, pb, pb + (e - b), cb, bb);
       }
       float clo[3], chi[3], olo[3], ohi[3];
       for (int d = 0; d < 3; d++) {
@@ -1054,7 +1467,7 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
         for (int k = 0; k < kMedChunk; k++) {
           const uint32_t i = c0 + (uint32_t)k * kWave + lane;
           act[k] = i < e;
-          v[k] = act[k] ? src[gb + i] : make_float4(0.f, 0.f, 0.f, 0.f);
+          v[k] = act[k] ? src[rev ? pb + (e - 1 - i) : pb + (i - b)] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
         for (int k = 0; k < kMedChunk; k++) {
@@ -1091,12 +1504,13 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
           MedFrame g{};
           const bool carry = PTS && !oneSided;
           g.b = cb0; g.e = ce0; g.me = child + side; g.buf = (bf ^ 1u) | (carry ? 0u : 2u);
+          g.pb = gb + cb0;
           if (carry) {
             // (wave-uniform branch: every lane joins the reductions)
             int32_t rc[6];
             for (int j = 0; j < 6; j++) rc[j] = side ? sc[1][j] : sc[0][j];
             bounds_wave(rc);
-            for (int j = 0; j < 6; j++) g.cb[j] = g.bb[j] = rc[j];
+            for (int j = 0; j < 6; j++) g.cb[j] = rc[j];
           }
           if (lane == 0) sMed[sp] = g;
           sp++;
@@ -1198,7 +1612,7 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
   const uint32_t numBlocks = div_up(n, kTile);
   c.maxLarge = n / (kMedium + 1) + 2;
   const uint32_t maxTiles = div_up(n, kTile) + c.maxLarge + 1;
-  const uint32_t maxLevels = 4096;
+  const uint32_t maxPasses = 2048;
   CUBQL_CHECK(scr.alloc(&c.items[0], n));
   CUBQL_CHECK(scr.alloc(&c.items[1], n));
   CUBQL_CHECK(scr.alloc(&c.nodes, 2 * (size_t)n + 2));
@@ -1207,22 +1621,25 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
   CUBQL_CHECK(scr.alloc(&c.segs[0], c.maxLarge));
   CUBQL_CHECK(scr.alloc(&c.segs[1], c.maxLarge));
   const uint32_t maxEmitBlocks = div_up(c.maxLarge, kEmitBlock);
+  CUBQL_CHECK(scr.alloc(&c.accA, (size_t)kAccA * c.maxLarge * kCbCopies));
+  CUBQL_CHECK(scr.alloc(&c.accB, (size_t)kAccB * c.maxLarge * kCbCopies));
+  CUBQL_CHECK(scr.alloc(&c.plan, c.maxLarge));
+  CUBQL_CHECK(scr.alloc(&c.classTot, 4 * (size_t)c.maxLarge));
   for (int p = 0; p < 2; p++) {
-    CUBQL_CHECK(scr.alloc(&c.childCb[p], 12 * (size_t)c.maxLarge * kCbCopies));
-    CUBQL_CHECK(scr.alloc(&c.segLeft[p], c.maxLarge));
     CUBQL_CHECK(scr.alloc(&c.tileStart[p], c.maxLarge + 1));
     CUBQL_CHECK(scr.alloc(&c.tileSeg[p], maxTiles));
     CUBQL_CHECK(scr.alloc(&c.segDim[p], c.maxLarge));
     CUBQL_CHECK(scr.alloc(&c.segPos[p], c.maxLarge));
   }
   c.maxEmitBlocks = maxEmitBlocks;
-  CUBQL_CHECK(scr.alloc(&c.emitStatus, 4 * (size_t)maxEmitBlocks));
+  CUBQL_CHECK(scr.alloc(&c.emitStatus, (size_t)kEmitCounts * maxEmitBlocks));
   CUBQL_CHECK(scr.alloc(&c.tickets, kTicketStride * kTicketCounters));
-  CUBQL_CHECK(scr.alloc(&c.tileStatus, maxTiles));
+  c.maxTiles = maxTiles;
+  CUBQL_CHECK(scr.alloc(&c.tileStatus, 4 * (size_t)maxTiles));
   CUBQL_CHECK(scr.alloc(&c.small, n + 1));
   CUBQL_CHECK(scr.alloc(&c.topNodes, n + 1));
-  CUBQL_CHECK(scr.alloc(&c.levelStart, maxLevels + 2));
-  CUBQL_CHECK(scr.alloc(&c.levelItems, maxLevels + 2));
+  CUBQL_CHECK(scr.alloc(&c.levelStart, 2 * (size_t)maxPasses + 2));
+  CUBQL_CHECK(scr.alloc(&c.levelItems, (size_t)maxPasses + 2));
   CUBQL_CHECK(scr.alloc(&c.blockBounds, 8 * (size_t)numBlocks + 8));
   CUBQL_CHECK(scr.alloc(&c.loc, 2 * (size_t)n + 64));
   CUBQL_CHECK(scr.alloc(&c.smallCount, (size_t)n + 1));
@@ -1237,9 +1654,10 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
   };
 
   k_init_counters<<<1, 64, 0, s>>>(c.ctr);
-  int st = fill_u32((uint32_t *)c.tileStatus, 0u, 2 * (size_t)maxTiles, s);
-  if (!st) st = fill_u32((uint32_t *)c.emitStatus, 0u, 8 * (size_t)maxEmitBlocks, s);
+  int st = fill_u32((uint32_t *)c.tileStatus, 0u, 8 * (size_t)maxTiles, s);
+  if (!st) st = fill_u32((uint32_t *)c.emitStatus, 0u, 2 * (size_t)kEmitCounts * maxEmitBlocks, s);
   if (st) return fail(st);
+  k_init_acc<<<std::max<uint32_t>(1u, std::min<uint32_t>(div_up(c.maxLarge * (kCbCopies * kAccA / 4), 256), 2048)), 256, 0, s>>>(c);
   k_count<<<numBlocks, kBlock, 0, s>>>(c);
   k_count_scan<<<1, 1024, 0, s>>>(c, numBlocks);
   k_write_items<<<numBlocks, kBlock, 0, s>>>(c);
@@ -1252,37 +1670,47 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
       hipDeviceGetAttribute(&numCU, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     numCU = 256;
   // at least one workgroup per ticket counter, so every counter's tiles run
-  // (the partition is persistent: one resident wave of workgroups)
-  int partPerCU = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&partPerCU, k_partition, kBlock, 0) != hipSuccess || partPerCU < 1)
+  // (pass B is persistent: one resident wave of workgroups)
+  int partPerCU = 0, boundsPerCU = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&partPerCU, k_pair, kBlock, 0) != hipSuccess || partPerCU < 1)
     partPerCU = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&boundsPerCU, k_bounds, kBlock, 0) != hipSuccess || boundsPerCU < 1)
+    boundsPerCU = 1;
   const uint32_t partGrid =
       std::max<uint32_t>(kTicketCounters, std::min<uint32_t>(maxTiles, (uint32_t)numCU * (uint32_t)partPerCU));
-  int level = 0;
-  std::vector<std::pair<int, int>> partSlots;  // (profile slot, level)
+  const uint32_t boundsGrid = std::min<uint32_t>(maxTiles, (uint32_t)numCU * (uint32_t)boundsPerCU);
+  const uint32_t planGrid = div_up(c.maxLarge, 256 / kWave);
+  int pass = 0;
+  std::vector<std::pair<int, int>> partSlots, boundsSlots;  // (profile slot, pass)
   // The first batch covers the depth a well-spread input needs (halving
-  // down to kMedium prims, plus one level of slack): running a level past
-  // the tree's depth costs two near-empty launches, a host check ~20 us.
-  int batch = 1;
-  for (uint64_t m = (uint64_t)kMedium; m < n; m *= 2) batch++;
-  batch = std::max(kLevelsPerSync, batch);
+  // down to kMedium prims, two levels a pass, plus one pass of slack):
+  // running a pass past the tree's depth costs four near-empty launches, a
+  // host check ~20 us.
+  int levels = 0;
+  for (uint64_t m = (uint64_t)kMedium; m < n; m *= 2) levels++;
+  int batch = std::max(kPassesPerSync, (levels + 1) / 2 + 1);
   while (true) {
-    for (int i = 0; i < batch && level < (int)maxLevels; i++, level++) {
+    for (int i = 0; i < batch && pass < (int)maxPasses; i++, pass++) {
+      int bs = prof_begin(s);
+      k_bounds<<<boundsGrid, kBlock, 0, s>>>(c, pass);
+      prof_end(bs, s, "sm_bounds");
+      if (bs >= 0) boundsSlots.push_back({bs, pass});
+      k_plan<<<planGrid, 256, 0, s>>>(c, pass);
       int ps = prof_begin(s);
-      k_partition<<<partGrid, kBlock, 0, s>>>(c, level);
+      k_pair<<<partGrid, kBlock, 0, s>>>(c, pass);
       prof_end(ps, s, "sm_partition");
-      if (ps >= 0) partSlots.push_back({ps, level});
-      k_emit<<<maxEmitBlocks, kEmitBlock, 0, s>>>(c, level);
+      if (ps >= 0) partSlots.push_back({ps, pass});
+      k_emit2<<<maxEmitBlocks, kEmitBlock, 0, s>>>(c, pass);
     }
-    if (hipGetLastError() != hipSuccess) return fail((set_error("level launch failed"), CUBQL_ERR_HIP));
+    if (hipGetLastError() != hipSuccess) return fail((set_error("pass launch failed"), CUBQL_ERR_HIP));
     if (hipMemcpyAsync(hw, c.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
-      return fail((set_error("level sync failed"), CUBQL_ERR_HIP));
+      return fail((set_error("pass sync failed"), CUBQL_ERR_HIP));
     const Counters *hc = (const Counters *)hw;
-    // open segments of the next level to run
-    if (hc->numLarge[level & 1] == 0) break;
-    batch = kLevelsPerSync;
-    if (level >= (int)maxLevels) return fail((set_error("tree deeper than %u levels", maxLevels), CUBQL_ERR_INTERNAL));
+    // open segments of the next pass to run
+    if (hc->numLarge[pass & 1] == 0) break;
+    batch = kPassesPerSync;
+    if (pass >= (int)maxPasses) return fail((set_error("tree deeper than %u levels", 2 * maxPasses), CUBQL_ERR_INTERNAL));
   }
   const Counters hc0 = *(const Counters *)hw;
   const uint32_t numSmall = hc0.numSmall;
@@ -1297,8 +1725,8 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
     smallSlot = prof_begin(s);
     k_small<<<smallGrid, kWave, 0, s>>>(c);
     prof_end(smallSlot, s, "sm_small");
-    // (status tag beyond every level's)
-    k_small_scan<<<div_up(numSmall, kCountScanTile), 1024, 0, s>>>(c, maxLevels + 2);
+    // (status tag beyond every pass's)
+    k_small_scan<<<div_up(numSmall, kCountScanTile), 1024, 0, s>>>(c, maxPasses + 2);
   }
   if (hipGetLastError() != hipSuccess) return fail((set_error("small-phase launch failed"), CUBQL_ERR_HIP));
   if (hipMemcpyAsync(hw, c.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -1306,12 +1734,14 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
     return fail((set_error("final sync failed"), CUBQL_ERR_HIP));
   const uint32_t numNodes = numValid ? (numSmall ? ((const Counters *)hw)->numNodesOut : topCount) : 0;
   if (prof_on()) {
-    // algorithmic bytes: partition reads and writes one 16-byte item per
-    // prim of the level; the small phase reads each item once, writes its
-    // primID and the nodes it creates (gathering boxes for non-point input)
-    std::vector<uint32_t> li(level + 1, 0);
-    if (hipMemcpy(li.data(), c.levelItems, sizeof(uint32_t) * level, hipMemcpyDeviceToHost) == hipSuccess) {
+    // algorithmic bytes: pass B reads and writes one 16-byte item per prim
+    // of the pass, pass A reads it; the small phase reads each item once,
+    // writes its primID and the nodes it creates (gathering boxes for
+    // non-point input)
+    std::vector<uint32_t> li(pass + 1, 0);
+    if (hipMemcpy(li.data(), c.levelItems, sizeof(uint32_t) * pass, hipMemcpyDeviceToHost) == hipSuccess) {
       for (auto &ps : partSlots) prof_set(ps.first, 32.0 * li[ps.second], li[ps.second]);
+      for (auto &bs : boundsSlots) prof_set(bs.first, 16.0 * li[bs.second], li[bs.second]);
     }
     const bool pts = hc0.allPoints != 0;
     const double smallNodes = (double)numNodes - (double)topCount + (double)numSmall;
@@ -1325,9 +1755,10 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
     k_copy_top<<<std::max<uint32_t>(1u, std::min<uint32_t>(div_up(topCount, 256), 1024)), 256, 0, s>>>(
         c.nodes, nodes, topCount);
     if (numSmall) k_small_emit<<<smallGrid, kWave, 0, s>>>(c, nodes);
-    // (point input: the emit already wrote the level-phase boxes)
+    // (point input: the emit already wrote the pass-phase boxes; two node
+    // lists a pass, each refit after the deeper ones)
     if (numTop && !hc0.allPoints)
-      for (int l = level - 1; l >= 0; l--) k_refit_top<<<64, 256, 0, s>>>(c, nodes, l);
+      for (int l = 2 * pass - 1; l >= 0; l--) k_refit_top<<<64, 256, 0, s>>>(c, nodes, l);
     if (hipGetLastError() != hipSuccess) {
       mem_free(mem, nodes, s);
       return fail((set_error("emit launch failed"), CUBQL_ERR_HIP));

@@ -100,7 +100,7 @@ __device__ inline uint32_t lookback_exclusive_wave(uint64_t *status, uint32_t t,
 }
 
 // Global ids for the local nodes of a list of finished segments: an
-// exclusive scan of (count[i] - 1) over i < n, offset by `base`, one
+// exclusive scan of (count[i] - sub) over i < n, offset by `base`, one
 // kCountScanTile-entry tile per 1024-thread workgroup (8 consecutive entries
 // per thread), joined by decoupled look-back on status[blockIdx.x] (with a
 // tag no other use of `status` shares; workgroups only wait on lower ones,
@@ -108,31 +108,42 @@ __device__ inline uint32_t lookback_exclusive_wave(uint64_t *status, uint32_t t,
 // total to *total.  smem: 1024/64 + 2 words.
 constexpr uint32_t kCountScanPer = 8;
 constexpr uint32_t kCountScanTile = 1024 * kCountScanPer;
-__device__ inline void count_scan_tile(const uint32_t *count, uint32_t *out, uint32_t n, uint32_t base,
-                                       uint64_t *status, uint32_t tag, uint32_t *total, uint32_t *smem) {
-  const uint32_t i0 = blockIdx.x * kCountScanTile + threadIdx.x * kCountScanPer;
+// One BLOCK-thread workgroup's part (tile blk of BLOCK*kCountScanPer
+// entries) of such a scan; returns the total over all tiles to the last
+// tile's thread 0 through *total when total is non-null.
+template <int BLOCK>
+__device__ inline void count_scan_part(const uint32_t *count, uint32_t *out, uint32_t n, uint32_t base,
+                                       uint64_t *status, uint32_t tag, uint32_t blk, bool last, uint32_t *total,
+                                       uint32_t *smem, uint32_t sub) {
+  const uint32_t i0 = blk * (BLOCK * kCountScanPer) + threadIdx.x * kCountScanPer;
   uint32_t v[kCountScanPer], sum = 0;
 #pragma unroll
   for (uint32_t j = 0; j < kCountScanPer; j++) {
-    v[j] = i0 + j < n ? count[i0 + j] - 1 : 0u;
+    v[j] = i0 + j < n ? count[i0 + j] - sub : 0u;
     sum += v[j];
   }
   uint32_t tot;
-  const uint32_t ex = block_excl_scan<1024, true>(sum, smem, &tot);
+  const uint32_t ex = block_excl_scan<BLOCK, true>(sum, smem, &tot);
   if (threadIdx.x < kWave) {
-    const uint32_t prefix = lookback_exclusive_wave(status, blockIdx.x, blockIdx.x, tag, tot);
+    const uint32_t prefix = lookback_exclusive_wave(status, blk, blk, tag, tot);
     if (threadIdx.x == 0) {
-      smem[1024 / kWave + 1] = prefix;
-      if (blockIdx.x == gridDim.x - 1) *total = base + prefix + tot;
+      smem[BLOCK / kWave + 1] = prefix;
+      if (last && total) *total = base + prefix + tot;
     }
   }
   sync_block_lds();
-  uint32_t o = base + smem[1024 / kWave + 1] + ex;
+  uint32_t o = base + smem[BLOCK / kWave + 1] + ex;
 #pragma unroll
   for (uint32_t j = 0; j < kCountScanPer; j++) {
     if (i0 + j < n) out[i0 + j] = o;
     o += v[j];
   }
+}
+__device__ inline void count_scan_tile(const uint32_t *count, uint32_t *out, uint32_t n, uint32_t base,
+                                       uint64_t *status, uint32_t tag, uint32_t *total, uint32_t *smem,
+                                       uint32_t sub = 1) {
+  count_scan_part<1024>(count, out, n, base, status, tag, blockIdx.x, blockIdx.x == gridDim.x - 1, total, smem,
+                        sub);
 }
 
 // block-wide min/max reduction of 12 floats (left lo/hi, right lo/hi)

@@ -133,6 +133,8 @@ struct Ctx {
   int32_t *accB;           // [maxLarge][kCbCopies][kAccB] pass B bounds (k_pair)
   PairPlan *plan;          // [maxLarge] (k_plan)
   uint32_t *classTot;      // [maxLarge][4] grandchild class sizes (k_pair)
+  uint32_t *tileLeft;      // [maxTiles] left-side prims of each tile (k_bounds)
+  uint32_t *tilePre;       // [maxTiles] left-side prims before each tile, over all tiles (k_tile_scan)
   uint64_t *emitStatus;    // [5][maxEmitBlocks] emit look-back status per count
   uint32_t maxEmitBlocks;
   uint64_t *tileStatus;    // [4][maxTiles] look-back status per class
@@ -383,6 +385,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds(Ctx c, int pass) {
   __shared__ int32_t sb[36 * NW];
   __shared__ uint32_t sc[NW];
   __shared__ TileA sD[2];
+  __shared__ uint32_t sTL[2][NW];  // per-wave left counts of a tile, by tile parity
   const int par = pass & 1;
   const uint32_t numTiles = c.ctr->numTiles;
   const float4 *src = c.items[par];
@@ -426,14 +429,17 @@ __global__ __launch_bounds__(kBlock) void k_bounds(Ctx c, int pass) {
   uint32_t nl = 0;
   while (true) {
     const uint32_t h = d.n / 2;
+    uint32_t wl = 0;  // the wave's left-side prims in this tile
 #pragma unroll
     for (int jj = 0; jj < kItems; jj++) {
       const uint32_t q = (uint32_t)(jj * kBlock) + threadIdx.x;
+      const float4 p = it[jj];
+      const bool lf = q < d.cnt && coord(p, d.dim) < d.pos;
+      wl += (uint32_t)__popcll(__ballot(lf));
       if (q < d.cnt) {
-        const float4 p = it[jj];
         const uint32_t j = d.j0 + q;
         const int32_t ex = enc_f(p.x), ey = enc_f(p.y), ez = enc_f(p.z);
-        if (coord(p, d.dim) < d.pos) {
+        if (lf) {
           nl++;
           CUBQL_ADD6(a, 0, ex, ey, ez);
         } else {
@@ -443,6 +449,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds(Ctx c, int pass) {
         if (j >= d.n - h) CUBQL_ADD6(a, 24, ex, ey, ez) else CUBQL_ADD6(a, 30, ex, ey, ez);
       }
     }
+    if (lane == 0) sTL[t & 1][wid] = wl;
     const uint32_t t2 = t + 1;
     const bool more = t2 < tEnd;
     TileA d2 = d;
@@ -484,6 +491,14 @@ __global__ __launch_bounds__(kBlock) void k_bounds(Ctx c, int pass) {
       for (int i = 0; i < 36; i++) a[i] = acc_identity(i);
       nl = 0;
     }
+    // (a barrier has passed since the counts were written: the next tile's
+    // or the flush's; the slot is rewritten two tiles later, past the next
+    // tile's barrier)
+    if (threadIdx.x == 0) {
+      uint32_t tl = 0;
+      for (int w = 0; w < NW; w++) tl += sTL[t & 1][w];
+      c.tileLeft[t] = tl;
+    }
     if (!more) break;
     t = t2;
     d = d2;
@@ -491,11 +506,21 @@ __global__ __launch_bounds__(kBlock) void k_bounds(Ctx c, int pass) {
 }
 
 // Plans of the pass's segments, one wave each, from the pass-A sums; the
-// wave also clears the segment's pass-B accumulators.
-__global__ __launch_bounds__(256) void k_plan(Ctx c, int pass) {
+// wave also clears the segment's pass-B accumulators.  The first scanBlocks
+// workgroups instead scan the tiles' left-side counts over all the pass's
+// tiles (pass B takes a tile's offset within its segment as a difference;
+// status region 2 of the tile look-back array).
+constexpr uint32_t kTileScanPart = 256 * kCountScanPer;
+__global__ __launch_bounds__(256) void k_plan(Ctx c, int pass, uint32_t scanBlocks) {
+  if (blockIdx.x < scanBlocks) {
+    __shared__ uint32_t ss[256 / kWave + 2];
+    count_scan_part<256>(c.tileLeft, c.tilePre, c.ctr->numTiles, 0u, c.tileStatus + 2 * (size_t)c.maxTiles,
+                         (uint32_t)pass + 1, blockIdx.x, false, nullptr, ss, 0u);
+    return;
+  }
   const int par = pass & 1;
   const uint32_t n = c.ctr->numLarge[par];
-  const uint32_t s = blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
+  const uint32_t s = (blockIdx.x - scanBlocks) * (256 / kWave) + threadIdx.x / kWave;
   if (s >= n) return;  // (whole waves)
   const uint32_t lane = lane_id();
   const int32_t *acc = c.accA + (size_t)s * kCbCopies * kAccA;
@@ -620,7 +645,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair(Ctx c, int pass) {
   __shared__ uint32_t sr[4 * kItems * NW + 4];
   __shared__ int32_t sb[24 * NW];
   __shared__ PairPlan sPlan[2];
-  __shared__ uint32_t sTile[2][4];  // tile, segment, first tile, end tile
+  __shared__ uint32_t sTile[2][5];  // tile, segment, first tile, end tile, left prims before it
   __shared__ uint32_t sPrefix[4];
   const int par = pass & 1;
   const uint32_t numTiles = c.ctr->numTiles;
@@ -638,6 +663,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair(Ctx c, int pass) {
       sTile[sl][1] = sg;
       sTile[sl][2] = c.tileStart[par][sg];
       sTile[sl][3] = c.tileStart[par][sg + 1];
+      sTile[sl][4] = c.tilePre[t] - c.tilePre[sTile[sl][2]];
       sPlan[sl] = c.plan[sg];
     }
   };
@@ -710,17 +736,31 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair(Ctx c, int pass) {
         sStage[idx] = it[jj];
       }
     }
+    // Offsets: the prims of child 0 before this tile are known (pass A's
+    // counts, or positions for a one-sided split); only the left classes of
+    // split children need a look-back (waves 0 and 1), the right classes
+    // follow by difference, an unsplit child is one class.
     {
-      const uint32_t mine = wid == 0 ? tot[0] : (wid == 1 ? tot[1] : (wid == 2 ? tot[2] : tot[3]));
-      const uint32_t pre = lookback_exclusive_wave(c.tileStatus + (size_t)wid * c.maxTiles, t, k, tag, mine);
+      const uint32_t q = 2 * wid;  // class (wid, left)
+      const bool look = wid < 2 && (wid == 0 ? split0 : split1) != 0;
+      uint32_t pre = 0;
+      if (look) {
+        const uint32_t mine = wid == 0 ? tot[0] : tot[2];
+        pre = lookback_exclusive_wave(c.tileStatus + (size_t)wid * c.maxTiles, t, k, tag, mine);
+        if (lane == 0 && t == tileEnd - 1) c.classTot[4 * (size_t)seg + q] = pre + mine;
+      }
       if (lane == 0) {
-        sPrefix[wid] = pre;
-        if (t == tileEnd - 1) c.classTot[4 * (size_t)seg + wid] = pre + mine;
+        if (wid < 2) sPrefix[wid] = pre;
         if (wid == 0) fetch(slot ^ 1);
       }
     }
     sync_block_lds();
-    const uint32_t p0 = sPrefix[0], p1 = sPrefix[1], p2 = sPrefix[2], p3 = sPrefix[3];
+    const uint32_t c0pre = mode == 0 ? U(sTile[slot][4]) : (mode == 1 ? min(j0, h) : (j0 > n - h ? j0 - (n - h) : 0u));
+    const uint32_t c1pre = j0 - c0pre;
+    const uint32_t p0 = split0 ? sPrefix[0] : c0pre;
+    const uint32_t p1 = split0 ? c0pre - p0 : 0u;
+    const uint32_t p2 = split1 ? sPrefix[1] : c1pre;
+    const uint32_t p3 = split1 ? c1pre - p2 : 0u;
     const int nslot = slot ^ 1;
     const uint32_t nt = U(sTile[nslot][0]);
     // the next tile's loads go out before this tile's stores
@@ -1625,6 +1665,8 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
   CUBQL_CHECK(scr.alloc(&c.accB, (size_t)kAccB * c.maxLarge * kCbCopies));
   CUBQL_CHECK(scr.alloc(&c.plan, c.maxLarge));
   CUBQL_CHECK(scr.alloc(&c.classTot, 4 * (size_t)c.maxLarge));
+  CUBQL_CHECK(scr.alloc(&c.tileLeft, maxTiles));
+  CUBQL_CHECK(scr.alloc(&c.tilePre, maxTiles));
   for (int p = 0; p < 2; p++) {
     CUBQL_CHECK(scr.alloc(&c.tileStart[p], c.maxLarge + 1));
     CUBQL_CHECK(scr.alloc(&c.tileSeg[p], maxTiles));
@@ -1680,6 +1722,7 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
       std::max<uint32_t>(kTicketCounters, std::min<uint32_t>(maxTiles, (uint32_t)numCU * (uint32_t)partPerCU));
   const uint32_t boundsGrid = std::min<uint32_t>(maxTiles, (uint32_t)numCU * (uint32_t)boundsPerCU);
   const uint32_t planGrid = div_up(c.maxLarge, 256 / kWave);
+  const uint32_t scanBlocks = div_up(maxTiles, kTileScanPart);
   int pass = 0;
   std::vector<std::pair<int, int>> partSlots, boundsSlots;  // (profile slot, pass)
   // The first batch covers the depth a well-spread input needs (halving
@@ -1695,7 +1738,7 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
       k_bounds<<<boundsGrid, kBlock, 0, s>>>(c, pass);
       prof_end(bs, s, "sm_bounds");
       if (bs >= 0) boundsSlots.push_back({bs, pass});
-      k_plan<<<planGrid, 256, 0, s>>>(c, pass);
+      k_plan<<<scanBlocks + planGrid, 256, 0, s>>>(c, pass, scanBlocks);
       int ps = prof_begin(s);
       k_pair<<<partGrid, kBlock, 0, s>>>(c, pass);
       prof_end(ps, s, "sm_partition");

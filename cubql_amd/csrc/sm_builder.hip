@@ -146,6 +146,14 @@ struct Ctx {
   uint32_t maxLarge;
 };
 
+// Accumulator copies in use for a segment of n prims: enough to keep the
+// same-address atomics of its tiles' workgroups (one flush per workgroup
+// and segment) in short queues, few enough that its readers merge little.
+__host__ __device__ __forceinline__ uint32_t acc_copies(uint32_t n) {
+  const uint32_t tiles = div_up(n, (uint32_t)kTile);
+  return tiles >= 64 * kCbCopies ? kCbCopies : (tiles < 128 ? 1u : tiles / 64);
+}
+
 // Accumulator identities: min slots +inf, max slots -inf (bound sets of six
 // words, lo xyz then hi xyz); pass A's count word (36) and padding are 0.
 __device__ __forceinline__ int32_t acc_identity(uint32_t w) {
@@ -473,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds(Ctx c, int pass) {
         sc[wid] = nl;
       }
       sync_block_lds();
-      int32_t *acc = c.accA + ((size_t)d.s * kCbCopies + blockIdx.x % kCbCopies) * kAccA;
+      int32_t *acc = c.accA + ((size_t)d.s * kCbCopies + blockIdx.x % acc_copies(d.n)) * kAccA;
       const uint32_t i = threadIdx.x - kAtomicWave * kWave;
       if (wid == kAtomicWave && i < 36) {
         int32_t r = sb[i * NW];
@@ -524,16 +532,17 @@ __global__ __launch_bounds__(256) void k_plan(Ctx c, int pass, uint32_t scanBloc
   if (s >= n) return;  // (whole waves)
   const uint32_t lane = lane_id();
   const int32_t *acc = c.accA + (size_t)s * kCbCopies * kAccA;
-  uint32_t nl = lane < kCbCopies ? (uint32_t)acc[lane * kAccA + 36] : 0u;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) nl += __shfl_xor(nl, o, kWave);
   const Seg g = c.segs[par][s];
   const uint32_t size = g.end - g.begin;
+  const uint32_t cps = acc_copies(size);
+  uint32_t nl = lane < cps ? (uint32_t)acc[lane * kAccA + 36] : 0u;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) nl += __shfl_xor(nl, o, kWave);
   const uint32_t mode = nl == size ? 1u : (nl == 0 ? 2u : 0u);
   // lane j < 12: word j of the children's bound sets, over the copies
   int32_t v = acc_identity(lane % 12);
   if (lane < 12)
-    for (uint32_t cp = 0; cp < kCbCopies; cp++) {
+    for (uint32_t cp = 0; cp < cps; cp++) {
       const int32_t x = acc[cp * kAccA + 12 * mode + lane];
       v = (lane % 6) < 3 ? min(v, x) : max(v, x);
     }
@@ -684,10 +693,11 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair(Ctx c, int pass) {
   int32_t pend = 0;
   uint32_t pendSeg = 0;
   bool pendOn = false;
-  auto addB = [&](uint32_t sg, int32_t r) {
+  uint32_t pendCps = 1;
+  auto addB = [&](uint32_t sg, uint32_t cps, int32_t r) {
     const uint32_t i = threadIdx.x;
     if (i < 24 && r != acc_identity(i)) {
-      int32_t *acc = c.accB + ((size_t)sg * kCbCopies + blockIdx.x % kCbCopies) * kAccB;
+      int32_t *acc = c.accB + ((size_t)sg * kCbCopies + blockIdx.x % cps) * kAccB;
       if ((i % 6) < 3) atomicMin(&acc[i], r);
       else atomicMax(&acc[i], r);
     }
@@ -772,7 +782,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair(Ctx c, int pass) {
       asm volatile("" ::: "memory");
     }
     if (pendOn) {
-      addB(pendSeg, pend);
+      addB(pendSeg, pendCps, pend);
       pendOn = false;
     }
     // copy out: staged class q -> its run; bounds of each class on the way
@@ -805,10 +815,11 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair(Ctx c, int pass) {
       }
       sync_block_lds();
       if (nt >= numTiles) {
-        addB(seg, r);
+        addB(seg, acc_copies(n), r);
       } else {
         pend = r;
         pendSeg = seg;
+        pendCps = acc_copies(n);
         pendOn = true;
       }
       for (int i = 0; i < 24; i++) cacc[i] = acc_identity(i);
@@ -907,8 +918,10 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit2(Ctx c, int pass) {
     for (int j = 0; j < 24; j++) gcb[j] = acc_identity(j);
     if (split[0] | split[1]) {
       const int4 *a = reinterpret_cast<const int4 *>(c.accB + (size_t)s * kCbCopies * kAccB);
+      const uint32_t cps = acc_copies(end - begin);
 #pragma unroll
       for (uint32_t cp = 0; cp < kCbCopies; cp++) {
+        if (cp >= cps) break;
 #pragma unroll
         for (int q = 0; q < 6; q++) {
           const int4 x = a[cp * 6 + q];

@@ -3,6 +3,7 @@
 // Layout types mirror the reference (bvh.h:71-111, math/box.h:60-196); the
 // device helpers are wave64 (CDNA4) primitives.
 #pragma once
+#include <string.h>
 
 #include <hip/hip_runtime.h>
 
@@ -92,6 +93,44 @@ struct Scratch {
     if (st != CUBQL_OK) return st;
     ptrs[n++] = v;
     *p = (T *)v;
+    return CUBQL_OK;
+  }
+};
+
+// Temporaries of one call carved from a single allocation, so the memory
+// resource is called once instead of once per array: reserve() records each
+// array, commit() allocates and assigns the pointers (256-byte aligned).
+struct ScratchArena {
+  const cubql_mem_resource *mem;
+  hipStream_t s;
+  void *slots[64];
+  size_t offs[64];
+  int n = 0;
+  size_t total = 0;
+  void *base = nullptr;
+  ScratchArena(const cubql_mem_resource *m, hipStream_t st) : mem(m), s(st) {}
+  ~ScratchArena() {
+    if (base) mem_free(mem, base, s);
+  }
+  template <typename T>
+  int reserve(T **p, size_t count) {
+    if (n >= 64) {
+      set_error("scratch table full");
+      return CUBQL_ERR_INTERNAL;
+    }
+    slots[n] = (void *)p;
+    offs[n] = total;
+    n++;
+    total += (count * sizeof(T) + 16 + 255) & ~(size_t)255;
+    return CUBQL_OK;
+  }
+  int commit() {
+    int st = mem_alloc(mem, &base, total, s);
+    if (st != CUBQL_OK) return st;
+    for (int i = 0; i < n; i++) {
+      void *v = (char *)base + offs[i];
+      memcpy(slots[i], &v, sizeof(void *));
+    }
     return CUBQL_OK;
   }
 };

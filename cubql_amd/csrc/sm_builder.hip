@@ -1598,9 +1598,11 @@ __global__ __launch_bounds__(1024) void k_small_scan(Ctx c, uint32_t tag) {
   count_scan_tile(c.smallCount, c.smallBase, c.ctr->numSmall, c.ctr->nodeCount, c.tileStatus, tag,
                   &c.ctr->numNodesOut, ss);
 }
-__global__ __launch_bounds__(kWave) void k_small_emit(Ctx c, Node *out) {
+// (one wave per segment, four segments a workgroup)
+__global__ __launch_bounds__(256) void k_small_emit(Ctx c, Node *out) {
   const uint32_t nSmall = c.ctr->numSmall;
-  for (uint32_t w = blockIdx.x; w < nSmall; w += gridDim.x) {
+  const uint32_t wpb = blockDim.x / kWave;
+  for (uint32_t w = blockIdx.x * wpb + threadIdx.x / kWave; w < nSmall; w += gridDim.x * wpb) {
     const SmallSeg sg = c.small[w];
     const uint32_t cnt = c.smallCount[w];
     const uint32_t base = c.smallBase[w];
@@ -1657,7 +1659,7 @@ static uint32_t *host_words() {
 int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, hipStream_t s,
              const cubql_mem_resource *mem) {
   using namespace sm;
-  Scratch scr(mem, s);
+  ScratchArena scr(mem, s);
   Ctx c{};
   c.boxes = boxes;
   c.n = n;
@@ -1666,39 +1668,40 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
   c.maxLarge = n / (kMedium + 1) + 2;
   const uint32_t maxTiles = div_up(n, kTile) + c.maxLarge + 1;
   const uint32_t maxPasses = 2048;
-  CUBQL_CHECK(scr.alloc(&c.items[0], n));
-  CUBQL_CHECK(scr.alloc(&c.items[1], n));
-  CUBQL_CHECK(scr.alloc(&c.nodes, 2 * (size_t)n + 2));
-  CUBQL_CHECK(scr.alloc(&c.ctr, 1));
-  CUBQL_CHECK(scr.alloc(&c.blockCount, numBlocks + 1));
-  CUBQL_CHECK(scr.alloc(&c.segs[0], c.maxLarge));
-  CUBQL_CHECK(scr.alloc(&c.segs[1], c.maxLarge));
+  CUBQL_CHECK(scr.reserve(&c.items[0], n));
+  CUBQL_CHECK(scr.reserve(&c.items[1], n));
+  CUBQL_CHECK(scr.reserve(&c.nodes, 2 * (size_t)n + 2));
+  CUBQL_CHECK(scr.reserve(&c.ctr, 1));
+  CUBQL_CHECK(scr.reserve(&c.blockCount, numBlocks + 1));
+  CUBQL_CHECK(scr.reserve(&c.segs[0], c.maxLarge));
+  CUBQL_CHECK(scr.reserve(&c.segs[1], c.maxLarge));
   const uint32_t maxEmitBlocks = div_up(c.maxLarge, kEmitBlock);
-  CUBQL_CHECK(scr.alloc(&c.accA, (size_t)kAccA * c.maxLarge * kCbCopies));
-  CUBQL_CHECK(scr.alloc(&c.accB, (size_t)kAccB * c.maxLarge * kCbCopies));
-  CUBQL_CHECK(scr.alloc(&c.plan, c.maxLarge));
-  CUBQL_CHECK(scr.alloc(&c.classTot, 4 * (size_t)c.maxLarge));
-  CUBQL_CHECK(scr.alloc(&c.tileLeft, maxTiles));
-  CUBQL_CHECK(scr.alloc(&c.tilePre, maxTiles));
+  CUBQL_CHECK(scr.reserve(&c.accA, (size_t)kAccA * c.maxLarge * kCbCopies));
+  CUBQL_CHECK(scr.reserve(&c.accB, (size_t)kAccB * c.maxLarge * kCbCopies));
+  CUBQL_CHECK(scr.reserve(&c.plan, c.maxLarge));
+  CUBQL_CHECK(scr.reserve(&c.classTot, 4 * (size_t)c.maxLarge));
+  CUBQL_CHECK(scr.reserve(&c.tileLeft, maxTiles));
+  CUBQL_CHECK(scr.reserve(&c.tilePre, maxTiles));
   for (int p = 0; p < 2; p++) {
-    CUBQL_CHECK(scr.alloc(&c.tileStart[p], c.maxLarge + 1));
-    CUBQL_CHECK(scr.alloc(&c.tileSeg[p], maxTiles));
-    CUBQL_CHECK(scr.alloc(&c.segDim[p], c.maxLarge));
-    CUBQL_CHECK(scr.alloc(&c.segPos[p], c.maxLarge));
+    CUBQL_CHECK(scr.reserve(&c.tileStart[p], c.maxLarge + 1));
+    CUBQL_CHECK(scr.reserve(&c.tileSeg[p], maxTiles));
+    CUBQL_CHECK(scr.reserve(&c.segDim[p], c.maxLarge));
+    CUBQL_CHECK(scr.reserve(&c.segPos[p], c.maxLarge));
   }
   c.maxEmitBlocks = maxEmitBlocks;
-  CUBQL_CHECK(scr.alloc(&c.emitStatus, (size_t)kEmitCounts * maxEmitBlocks));
-  CUBQL_CHECK(scr.alloc(&c.tickets, kTicketStride * kTicketCounters));
+  CUBQL_CHECK(scr.reserve(&c.emitStatus, (size_t)kEmitCounts * maxEmitBlocks));
+  CUBQL_CHECK(scr.reserve(&c.tickets, kTicketStride * kTicketCounters));
   c.maxTiles = maxTiles;
-  CUBQL_CHECK(scr.alloc(&c.tileStatus, 4 * (size_t)maxTiles));
-  CUBQL_CHECK(scr.alloc(&c.small, n + 1));
-  CUBQL_CHECK(scr.alloc(&c.topNodes, n + 1));
-  CUBQL_CHECK(scr.alloc(&c.levelStart, 2 * (size_t)maxPasses + 2));
-  CUBQL_CHECK(scr.alloc(&c.levelItems, (size_t)maxPasses + 2));
-  CUBQL_CHECK(scr.alloc(&c.blockBounds, 8 * (size_t)numBlocks + 8));
-  CUBQL_CHECK(scr.alloc(&c.loc, 2 * (size_t)n + 64));
-  CUBQL_CHECK(scr.alloc(&c.smallCount, (size_t)n + 1));
-  CUBQL_CHECK(scr.alloc(&c.smallBase, (size_t)n + 1));
+  CUBQL_CHECK(scr.reserve(&c.tileStatus, 4 * (size_t)maxTiles));
+  CUBQL_CHECK(scr.reserve(&c.small, n + 1));
+  CUBQL_CHECK(scr.reserve(&c.topNodes, n + 1));
+  CUBQL_CHECK(scr.reserve(&c.levelStart, 2 * (size_t)maxPasses + 2));
+  CUBQL_CHECK(scr.reserve(&c.levelItems, (size_t)maxPasses + 2));
+  CUBQL_CHECK(scr.reserve(&c.blockBounds, 8 * (size_t)numBlocks + 8));
+  CUBQL_CHECK(scr.reserve(&c.loc, 2 * (size_t)n + 64));
+  CUBQL_CHECK(scr.reserve(&c.smallCount, (size_t)n + 1));
+  CUBQL_CHECK(scr.reserve(&c.smallBase, (size_t)n + 1));
+  CUBQL_CHECK(scr.commit());
 
   uint32_t *primIDs = nullptr;
   CUBQL_CHECK(mem_alloc(mem, (void **)&primIDs, (size_t)n * sizeof(uint32_t), s));
@@ -1739,12 +1742,12 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
   int pass = 0;
   std::vector<std::pair<int, int>> partSlots, boundsSlots;  // (profile slot, pass)
   // The first batch covers the depth a well-spread input needs (halving
-  // down to kMedium prims, two levels a pass, plus one pass of slack):
-  // running a pass past the tree's depth costs four near-empty launches, a
-  // host check ~20 us.
+  // down to kMedium prims, two levels a pass): a pass past the tree's depth
+  // (four near-empty launches) costs about what a host check does, ~20 us,
+  // so deeper trees take a check and another batch instead of slack here.
   int levels = 0;
   for (uint64_t m = (uint64_t)kMedium; m < n; m *= 2) levels++;
-  int batch = std::max(kPassesPerSync, (levels + 1) / 2 + 1);
+  int batch = std::max(kPassesPerSync, (levels + 1) / 2);
   while (true) {
     for (int i = 0; i < batch && pass < (int)maxPasses; i++, pass++) {
       int bs = prof_begin(s);
@@ -1810,7 +1813,7 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
   if (numNodes) {
     k_copy_top<<<std::max<uint32_t>(1u, std::min<uint32_t>(div_up(topCount, 256), 1024)), 256, 0, s>>>(
         c.nodes, nodes, topCount);
-    if (numSmall) k_small_emit<<<smallGrid, kWave, 0, s>>>(c, nodes);
+    if (numSmall) k_small_emit<<<div_up(numSmall, 4), 256, 0, s>>>(c, nodes);
     // (point input: the emit already wrote the pass-phase boxes; two node
     // lists a pass, each refit after the deeper ones)
     if (numTop && !hc0.allPoints)

@@ -339,7 +339,7 @@ static void refit_launch(const RCtx &c, Node *nodes, const uint32_t *primIDs, in
 template <typename K>
 int build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, int bitsPerDim,
           hipStream_t s, const cubql_mem_resource *mem) {
-  Scratch scr(mem, s);
+  ScratchArena scr(mem, s);
   RCtx c{};
   c.boxes = boxes;
   c.n = n;
@@ -348,16 +348,17 @@ int build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, in
   const uint32_t numBlocks = div_up(n, kTile);
   K *keysA = nullptr, *keysB = nullptr;
   uint32_t *idsB = nullptr;
-  CUBQL_CHECK(scr.alloc(&c.blockCount, numBlocks + 1));
-  CUBQL_CHECK(scr.alloc(&c.blockBounds, 8 * (size_t)numBlocks + 8));
-  CUBQL_CHECK(scr.alloc(&c.ctr, C_LSTART0 + kMaxLevels + 8));
-  CUBQL_CHECK(scr.alloc(&c.quant, 8));
-  CUBQL_CHECK(scr.alloc(&keysA, n));
-  CUBQL_CHECK(scr.alloc(&keysB, n));
-  CUBQL_CHECK(scr.alloc(&idsB, n));
-  CUBQL_CHECK(scr.alloc(&c.nodes, 2 * (size_t)n + 2));
-  CUBQL_CHECK(scr.alloc(&c.ranges, 2 * (size_t)n + 2));
-  CUBQL_CHECK(scr.alloc(&c.levels, 2 * (size_t)n + 2));
+  CUBQL_CHECK(scr.reserve(&c.blockCount, numBlocks + 1));
+  CUBQL_CHECK(scr.reserve(&c.blockBounds, 8 * (size_t)numBlocks + 8));
+  CUBQL_CHECK(scr.reserve(&c.ctr, C_LSTART0 + kMaxLevels + 8));
+  CUBQL_CHECK(scr.reserve(&c.quant, 8));
+  CUBQL_CHECK(scr.reserve(&keysA, n));
+  CUBQL_CHECK(scr.reserve(&keysB, n));
+  CUBQL_CHECK(scr.reserve(&idsB, n));
+  CUBQL_CHECK(scr.reserve(&c.nodes, 2 * (size_t)n + 2));
+  CUBQL_CHECK(scr.reserve(&c.ranges, 2 * (size_t)n + 2));
+  CUBQL_CHECK(scr.reserve(&c.levels, 2 * (size_t)n + 2));
+  CUBQL_CHECK(scr.commit());
   uint32_t *primIDs = nullptr;
   CUBQL_CHECK(mem_alloc(mem, (void **)&primIDs, (size_t)n * sizeof(uint32_t), s));
   c.ids = primIDs;
@@ -761,7 +762,7 @@ __global__ __launch_bounds__(kWave) void k_thread_emit(RCtx c, Node *out) {
 
 int rebin_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, hipStream_t s,
                 const cubql_mem_resource *mem) {
-  Scratch scr(mem, s);
+  ScratchArena scr(mem, s);
   RCtx c{};
   c.boxes = boxes;
   c.n = n;
@@ -772,32 +773,33 @@ int rebin_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t
   uint32_t *keysA = nullptr, *keysB = nullptr, *idsB = nullptr;
   uint64_t *rkA = nullptr, *rkB = nullptr;
   uint32_t *rvA = nullptr, *rvB = nullptr;
-  CUBQL_CHECK(scr.alloc(&c.blockCount, numBlocks + 1));
-  CUBQL_CHECK(scr.alloc(&c.blockBounds, 8 * (size_t)numBlocks + 8));
-  CUBQL_CHECK(scr.alloc(&c.ctr, C_LSTART0 + kMaxLevels + 8));
-  CUBQL_CHECK(scr.alloc(&c.quant, 8));
-  CUBQL_CHECK(scr.alloc(&keysA, n));
-  CUBQL_CHECK(scr.alloc(&keysB, n));
-  CUBQL_CHECK(scr.alloc(&idsB, n));
-  CUBQL_CHECK(scr.alloc(&c.nodes, 2 * (size_t)n + 2));
-  CUBQL_CHECK(scr.alloc(&c.ranges, 2 * (size_t)n + 2));
+  CUBQL_CHECK(scr.reserve(&c.blockCount, numBlocks + 1));
+  CUBQL_CHECK(scr.reserve(&c.blockBounds, 8 * (size_t)numBlocks + 8));
+  CUBQL_CHECK(scr.reserve(&c.ctr, C_LSTART0 + kMaxLevels + 8));
+  CUBQL_CHECK(scr.reserve(&c.quant, 8));
+  CUBQL_CHECK(scr.reserve(&keysA, n));
+  CUBQL_CHECK(scr.reserve(&keysB, n));
+  CUBQL_CHECK(scr.reserve(&idsB, n));
+  CUBQL_CHECK(scr.reserve(&c.nodes, 2 * (size_t)n + 2));
+  CUBQL_CHECK(scr.reserve(&c.ranges, 2 * (size_t)n + 2));
   // a job node is listed again in the level after its rebin round
-  CUBQL_CHECK(scr.alloc(&c.levels, 2 * (size_t)n + 2 + 64 * (size_t)maxJobs));
-  CUBQL_CHECK(scr.alloc(&c.jobs, maxJobs));
-  CUBQL_CHECK(scr.alloc(&c.jobStart, maxJobs + 1));
-  CUBQL_CHECK(scr.alloc(&c.jobDom, 6 * (size_t)maxJobs));
-  CUBQL_CHECK(scr.alloc(&c.jobQuant, 4 * (size_t)maxJobs));
-  CUBQL_CHECK(scr.alloc(&c.jobLocal, maxJobs));
-  CUBQL_CHECK(scr.alloc(&c.small, (size_t)n + 1));
-  CUBQL_CHECK(scr.alloc(&c.smallCount, (size_t)n + 1));
-  CUBQL_CHECK(scr.alloc(&c.smallBase, (size_t)n + 1));
-  CUBQL_CHECK(scr.alloc(&c.loc, 2 * (size_t)n + 64));
+  CUBQL_CHECK(scr.reserve(&c.levels, 2 * (size_t)n + 2 + 64 * (size_t)maxJobs));
+  CUBQL_CHECK(scr.reserve(&c.jobs, maxJobs));
+  CUBQL_CHECK(scr.reserve(&c.jobStart, maxJobs + 1));
+  CUBQL_CHECK(scr.reserve(&c.jobDom, 6 * (size_t)maxJobs));
+  CUBQL_CHECK(scr.reserve(&c.jobQuant, 4 * (size_t)maxJobs));
+  CUBQL_CHECK(scr.reserve(&c.jobLocal, maxJobs));
+  CUBQL_CHECK(scr.reserve(&c.small, (size_t)n + 1));
+  CUBQL_CHECK(scr.reserve(&c.smallCount, (size_t)n + 1));
+  CUBQL_CHECK(scr.reserve(&c.smallBase, (size_t)n + 1));
+  CUBQL_CHECK(scr.reserve(&c.loc, 2 * (size_t)n + 64));
   const uint32_t numScanTiles = div_up(n, kCountScanTile) + 1;
-  CUBQL_CHECK(scr.alloc(&c.scanStatus, numScanTiles));
-  CUBQL_CHECK(scr.alloc(&rkA, n));
-  CUBQL_CHECK(scr.alloc(&rkB, n));
-  CUBQL_CHECK(scr.alloc(&rvA, n));
-  CUBQL_CHECK(scr.alloc(&rvB, n));
+  CUBQL_CHECK(scr.reserve(&c.scanStatus, numScanTiles));
+  CUBQL_CHECK(scr.reserve(&rkA, n));
+  CUBQL_CHECK(scr.reserve(&rkB, n));
+  CUBQL_CHECK(scr.reserve(&rvA, n));
+  CUBQL_CHECK(scr.reserve(&rvB, n));
+  CUBQL_CHECK(scr.commit());
   uint32_t *primIDs = nullptr;
   CUBQL_CHECK(mem_alloc(mem, (void **)&primIDs, (size_t)n * sizeof(uint32_t), s));
   c.ids = primIDs;

@@ -854,7 +854,7 @@ int sah_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T
     set_error("sahBuilder: numBins must be in [2, %d] (got %d)", kMaxBins, numBins);
     return CUBQL_ERR_INVALID_ARG;
   }
-  Scratch scr(mem, s);
+  ScratchArena scr(mem, s);
   Ctx c{};
   c.boxes = boxes;
   c.n = n;
@@ -867,30 +867,31 @@ int sah_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T
   const uint32_t maxBinTiles = div_up(n, kBinTile) + c.maxLarge + 1;
   const size_t binW = 3 * (size_t)numBins * kBinWords;
   const uint32_t maxLevels = 4096;
-  CUBQL_CHECK(scr.alloc(&c.items[0], n));
-  CUBQL_CHECK(scr.alloc(&c.items[1], n));
-  CUBQL_CHECK(scr.alloc(&c.nodes, 2 * (size_t)n + 2));
-  CUBQL_CHECK(scr.alloc(&c.ctr, 1));
-  CUBQL_CHECK(scr.alloc(&c.blockCount, numBlocks + 1));
-  CUBQL_CHECK(scr.alloc(&c.blockBounds, 8 * (size_t)numBlocks + 8));
-  CUBQL_CHECK(scr.alloc(&c.segs[0], c.maxLarge));
-  CUBQL_CHECK(scr.alloc(&c.segs[1], c.maxLarge));
-  CUBQL_CHECK(scr.alloc(&c.childCb, 12 * (size_t)c.maxLarge * kCbCopies));
-  CUBQL_CHECK(scr.alloc(&c.tickets, kTicketStride * kTicketCounters));
-  CUBQL_CHECK(scr.alloc(&c.segLeft, c.maxLarge));
-  CUBQL_CHECK(scr.alloc(&c.tileStart, c.maxLarge + 1));
-  CUBQL_CHECK(scr.alloc(&c.binStart, c.maxLarge + 1));
-  CUBQL_CHECK(scr.alloc(&c.segMode, c.maxLarge));
-  CUBQL_CHECK(scr.alloc(&c.segBin, c.maxLarge));
-  CUBQL_CHECK(scr.alloc(&c.segLo, c.maxLarge));
-  CUBQL_CHECK(scr.alloc(&c.segHi, c.maxLarge));
-  CUBQL_CHECK(scr.alloc(&c.tileStatus, maxTiles));
-  CUBQL_CHECK(scr.alloc(&c.partial, (size_t)maxBinTiles * binW));
-  CUBQL_CHECK(scr.alloc(&c.small, (size_t)n + 1));
-  CUBQL_CHECK(scr.alloc(&c.loc, 2 * (size_t)n + 64));
-  CUBQL_CHECK(scr.alloc(&c.smallCount, (size_t)n + 1));
-  CUBQL_CHECK(scr.alloc(&c.smallBase, (size_t)n + 1));
-  CUBQL_CHECK(scr.alloc(&c.levelItems, maxLevels + 2));
+  CUBQL_CHECK(scr.reserve(&c.items[0], n));
+  CUBQL_CHECK(scr.reserve(&c.items[1], n));
+  CUBQL_CHECK(scr.reserve(&c.nodes, 2 * (size_t)n + 2));
+  CUBQL_CHECK(scr.reserve(&c.ctr, 1));
+  CUBQL_CHECK(scr.reserve(&c.blockCount, numBlocks + 1));
+  CUBQL_CHECK(scr.reserve(&c.blockBounds, 8 * (size_t)numBlocks + 8));
+  CUBQL_CHECK(scr.reserve(&c.segs[0], c.maxLarge));
+  CUBQL_CHECK(scr.reserve(&c.segs[1], c.maxLarge));
+  CUBQL_CHECK(scr.reserve(&c.childCb, 12 * (size_t)c.maxLarge * kCbCopies));
+  CUBQL_CHECK(scr.reserve(&c.tickets, kTicketStride * kTicketCounters));
+  CUBQL_CHECK(scr.reserve(&c.segLeft, c.maxLarge));
+  CUBQL_CHECK(scr.reserve(&c.tileStart, c.maxLarge + 1));
+  CUBQL_CHECK(scr.reserve(&c.binStart, c.maxLarge + 1));
+  CUBQL_CHECK(scr.reserve(&c.segMode, c.maxLarge));
+  CUBQL_CHECK(scr.reserve(&c.segBin, c.maxLarge));
+  CUBQL_CHECK(scr.reserve(&c.segLo, c.maxLarge));
+  CUBQL_CHECK(scr.reserve(&c.segHi, c.maxLarge));
+  CUBQL_CHECK(scr.reserve(&c.tileStatus, maxTiles));
+  CUBQL_CHECK(scr.reserve(&c.partial, (size_t)maxBinTiles * binW));
+  CUBQL_CHECK(scr.reserve(&c.small, (size_t)n + 1));
+  CUBQL_CHECK(scr.reserve(&c.loc, 2 * (size_t)n + 64));
+  CUBQL_CHECK(scr.reserve(&c.smallCount, (size_t)n + 1));
+  CUBQL_CHECK(scr.reserve(&c.smallBase, (size_t)n + 1));
+  CUBQL_CHECK(scr.reserve(&c.levelItems, maxLevels + 2));
+  CUBQL_CHECK(scr.commit());
 
   uint32_t *primIDs = nullptr;
   CUBQL_CHECK(mem_alloc(mem, (void **)&primIDs, std::max<size_t>(n, 1) * sizeof(uint32_t), s));

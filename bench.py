@@ -338,9 +338,10 @@ def main():
     torch.cuda.synchronize()
     cq.profile.enable(False)
     part = cq.profile.query("sm_partition")
+    bnds = cq.profile.query("sm_bounds")
     small = cq.profile.query("sm_small")
     kern = {}
-    for name, st in [("sm_partition", part), ("sm_small", small)]:
+    for name, st in [("sm_partition", part), ("sm_bounds", bnds), ("sm_small", small)]:
         if st and st["launches"]:
             kern[name] = {"launches_per_build": st["launches"] / 3,
                           "avg_launch_ms": st["ms"] / st["launches"],
@@ -354,7 +355,8 @@ def main():
                     "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": partition_traffic(),
                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-                    "bytes_model": "32 B per prim of the level (16-byte item read + written)",
+                    "bytes_model": "32 B per prim of the pass (16-byte item read + written; "
+                                   "one pass places the prims two tree levels down)",
                     "avg_launch_ms": dom["avg_launch_ms"],
                     "bytes_per_launch": dom["bytes_per_launch"]}
 

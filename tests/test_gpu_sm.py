@@ -14,6 +14,7 @@ Every call goes through the C ABI (libcubql_amd.so via cubql_amd).  Checks:
 """
 import hashlib
 import math
+import zlib
 
 import numpy as np
 import pytest
@@ -121,6 +122,57 @@ def test_gpu_c1_workload(cq, oracle_mod):
     oi, od = oracle_mod.fcp(tree, P, Q, lowest_id=True)
     assert np.array_equal(ids, oi)
     assert np.array_equal(kd.cpu().numpy(), g["knn_d2"])
+    bvh.free()
+
+
+def _deep_case(name):
+    """Inputs that drive the two-level passes through their rarer paths: trees
+    deeper than the size predicts (several host batches), one-sided splits of
+    large nodes (position halves, reversed halves turned around), exact ties
+    with the split planes, and boxes with invalid entries."""
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    T = 8
+    if name == "skewed":            # density piles up near the origin: a deep tree
+        P = rng.random((400_000, 3), dtype=np.float32) ** np.float32(6)
+    elif name == "dupes_large":     # 40 distinct positions, ~7500 copies each
+        table = rng.random((40, 3), dtype=np.float32)
+        P = table[rng.integers(0, 40, 300_000)]
+    elif name == "halfdup":         # half the points on one position
+        P = rng.random((200_000, 3), dtype=np.float32)
+        P[::2] = np.float32(0.25)
+    elif name == "line":            # every split along x
+        P = np.zeros((250_000, 3), np.float32)
+        P[:, 0] = rng.random(250_000, dtype=np.float32)
+    elif name == "grid_ties":       # integer grid: planes fall on coordinates
+        g = np.arange(64, dtype=np.float32)
+        P = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+        P = P[rng.permutation(len(P))]
+        T = 1
+    else:                           # boxes, 1% of them invalid (lower > upper)
+        lo = rng.random((200_000, 3), dtype=np.float32)
+        ext = rng.random((200_000, 3), dtype=np.float32) * np.float32(0.01)
+        boxes = np.concatenate([lo, lo + ext], 1)
+        bad = rng.random(200_000) < 0.01
+        boxes[bad, 0] = boxes[bad, 3] + np.float32(1)
+        return boxes, T
+    return np.concatenate([P, P], 1).astype(np.float32), T
+
+
+@pytest.mark.parametrize("name", ["skewed", "dupes_large", "halfdup", "line", "grid_ties", "boxes"])
+def test_gpu_builder_deep_and_degenerate_inputs(cq, oracle_mod, name):
+    print(f"[gpu] sm deep case={name}", flush=True)
+    boxes, T = _deep_case(name)
+    bvh = cq.gpuBuilder(torch.from_numpy(boxes).cuda(), cq.BuildConfig(makeLeafThreshold=T))
+    torch.cuda.synchronize()
+    tree = _gpu_tree(cq, oracle_mod, bvh)
+    nv = _valid_count(boxes)
+    code, depth, leaves = oracle_mod.check_tree(tree, boxes, T)
+    assert code == 0, code
+    ref = oracle_mod.sm_build(boxes, T)
+    # the CPU recursion's prim order and tree
+    assert np.array_equal(tree.prim_ids[:nv], ref.prim_ids[:nv])
+    assert np.array_equal(oracle_mod.canonical(tree), oracle_mod.canonical(ref))
+    print(f"[gpu] sm deep case={name}: nodes={bvh.numNodes} depth={depth}", flush=True)
     bvh.free()
 
 

@@ -1575,10 +1575,14 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
   if (lane == 0) c.smallCount[segIdx] = nLocal;
 }
 
-__global__ __launch_bounds__(kWave) void k_small(Ctx c) {
+// (pp: parity of the pass after the last one launched; while segments are
+// still open there the small phase does nothing -- the host launches it
+// speculatively after the passes it expects the tree to need)
+__global__ __launch_bounds__(kWave) void k_small(Ctx c, int pp) {
   __shared__ float4 sIt[kSmall];
   __shared__ uint32_t sFr[kSmall + 2];
   __shared__ MedFrame sMed[kMedium / (kSmall + 1) + 1];
+  if (c.ctr->numLarge[pp] != 0) return;
   const bool pts = c.ctr->allPoints != 0;
   const uint32_t nSmall = c.ctr->numSmall;
   for (uint32_t w = blockIdx.x; w < nSmall; w += gridDim.x) {
@@ -1593,8 +1597,9 @@ __global__ __launch_bounds__(kWave) void k_small(Ctx c) {
 // j >= 1 of segment s is smallBase[s] + j - 1.
 // global ids of the small segments' local nodes (the final node count goes
 // to numNodesOut: every workgroup reads nodeCount as its base)
-__global__ __launch_bounds__(1024) void k_small_scan(Ctx c, uint32_t tag) {
+__global__ __launch_bounds__(1024) void k_small_scan(Ctx c, uint32_t tag, int pp) {
   __shared__ uint32_t ss[1024 / kWave + 2];
+  if (c.ctr->numLarge[pp] != 0) return;
   count_scan_tile(c.smallCount, c.smallBase, c.ctr->numSmall, c.ctr->nodeCount, c.tileStatus, tag,
                   &c.ctr->numNodesOut, ss);
 }
@@ -1748,6 +1753,9 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
   int levels = 0;
   for (uint64_t m = (uint64_t)kMedium; m < n; m *= 2) levels++;
   int batch = std::max(kPassesPerSync, (levels + 1) / 2);
+  bool firstBatch = true;
+  int specPass = -1, smallSlot = -1;
+  uint32_t specGrid = 0;
   while (true) {
     for (int i = 0; i < batch && pass < (int)maxPasses; i++, pass++) {
       int bs = prof_begin(s);
@@ -1760,6 +1768,26 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
       prof_end(ps, s, "sm_partition");
       if (ps >= 0) partSlots.push_back({ps, pass});
       k_emit2<<<maxEmitBlocks, kEmitBlock, 0, s>>>(c, pass);
+    }
+    if (firstBatch) {
+      // The small phase, launched before the host knows the passes are
+      // done (it does nothing otherwise): one workgroup per segment needs
+      // the segment count, and after P passes there are at most
+      // 4 (4^P - 1) / 3 + 1 small segments (each open segment of a pass
+      // makes at most 4 open and 4 small ones).
+      uint64_t bound = 1, open = 1;
+      for (int p = 0; p < pass && bound <= (uint64_t)n; p++) {
+        bound += 4 * open;
+        open *= 4;
+      }
+      specGrid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(bound, (uint64_t)n + 1));
+      smallSlot = prof_begin(s);
+      k_small<<<specGrid, kWave, 0, s>>>(c, pass & 1);
+      prof_end(smallSlot, s, "sm_small");
+      // (status tag beyond every pass's)
+      k_small_scan<<<div_up(specGrid, kCountScanTile), 1024, 0, s>>>(c, maxPasses + 2, pass & 1);
+      specPass = pass;
+      firstBatch = false;
     }
     if (hipGetLastError() != hipSuccess) return fail((set_error("pass launch failed"), CUBQL_ERR_HIP));
     if (hipMemcpyAsync(hw, c.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -1775,23 +1803,23 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
   const uint32_t numSmall = hc0.numSmall;
   const uint32_t numValid = hc0.numValid;
   const uint32_t numTop = hc0.numTop;
-  const uint32_t topCount = hc0.nodeCount;  // ids [0, topCount) belong to the level phase
-  int smallSlot = -1;
-  // one workgroup per segment: the hardware dispatcher balances the uneven
-  // segment costs far better than a fixed round-robin over resident waves
-  const uint32_t smallGrid = std::max<uint32_t>(1u, numSmall);
-  if (numSmall) {
+  const uint32_t topCount = hc0.nodeCount;  // ids [0, topCount) belong to the pass phase
+  uint32_t nodesOut = hc0.numNodesOut;
+  if (specPass != pass) {
+    // the speculative small phase found open segments: run it now, one
+    // workgroup per segment (the dispatcher balances the uneven segment
+    // costs far better than a fixed round-robin over resident waves)
     smallSlot = prof_begin(s);
-    k_small<<<smallGrid, kWave, 0, s>>>(c);
+    k_small<<<std::max<uint32_t>(1u, numSmall), kWave, 0, s>>>(c, pass & 1);
     prof_end(smallSlot, s, "sm_small");
-    // (status tag beyond every pass's)
-    k_small_scan<<<div_up(numSmall, kCountScanTile), 1024, 0, s>>>(c, maxPasses + 2);
+    k_small_scan<<<std::max<uint32_t>(1u, div_up(numSmall, kCountScanTile)), 1024, 0, s>>>(c, maxPasses + 2, pass & 1);
+    if (hipGetLastError() != hipSuccess) return fail((set_error("small-phase launch failed"), CUBQL_ERR_HIP));
+    if (hipMemcpyAsync(hw, c.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return fail((set_error("final sync failed"), CUBQL_ERR_HIP));
+    nodesOut = ((const Counters *)hw)->numNodesOut;
   }
-  if (hipGetLastError() != hipSuccess) return fail((set_error("small-phase launch failed"), CUBQL_ERR_HIP));
-  if (hipMemcpyAsync(hw, c.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess)
-    return fail((set_error("final sync failed"), CUBQL_ERR_HIP));
-  const uint32_t numNodes = numValid ? (numSmall ? ((const Counters *)hw)->numNodesOut : topCount) : 0;
+  const uint32_t numNodes = numValid ? (numSmall ? nodesOut : topCount) : 0;
   if (prof_on()) {
     // algorithmic bytes: pass B reads and writes one 16-byte item per prim
     // of the pass, pass A reads it; the small phase reads each item once,

@@ -13,14 +13,16 @@
 // MI355X design (not the reference's prim-state + global sort scheme):
 //   * prims are kept physically partitioned: each open node owns a
 //     contiguous range of 16-byte items {cx,cy,cz,primID} in HBM;
-//   * one "partition" pass per tree level moves every prim of a large node
-//     to its child range (left stable, right reversed -- the CPU order),
-//     using a single-pass decoupled look-back scan over 2048-item tiles and
-//     accumulating child centroid bounds with one atomic per tile;
-//   * nodes of <= kSmall prims are finished by one wavefront each, entirely
-//     in LDS, which also computes all their boxes bottom-up;
-//   * the level loop runs on the device: kernels read their work counts from
-//     device memory, the host only checks for completion every few levels.
+//   * one pass splits every open node two tree levels deep: a read-only
+//     bounds pass (k_bounds) and a partition (k_pair) that moves each prim
+//     straight to its grandchild's range in the CPU order (left stable,
+//     right reversed, per level), using decoupled look-back over 2048-item
+//     tiles; ranges that come out reversed carry a flag;
+//   * nodes of <= kMedium prims are finished by one wavefront each (pieces
+//     of <= kSmall prims in LDS), which also computes all their boxes;
+//   * the pass loop runs on the device: kernels read their work counts from
+//     device memory, the host checks for completion once per batch of
+//     passes.
 #include "block_ops.h"
 #include "common.h"
 #include "wave_ops.h"
@@ -36,7 +38,7 @@ constexpr int kBlock = 256;
 constexpr int kItems = 8;
 constexpr int kTile = kBlock * kItems;  // 2048 prims per partition tile
 constexpr int kSmall = 256;             // max prims of a node a wave finishes in LDS
-// Segments of up to kMedium prims leave the level loop too: their wave
+// Segments of up to kMedium prims leave the pass loop too: their wave
 // splits them in global memory (between the two item buffers) until the
 // pieces fit kSmall.
 constexpr int kMedium = 1024;
@@ -67,7 +69,7 @@ struct Counters {
   uint32_t ticket;
   uint32_t numTiles;
   uint32_t numTop;    // entries in the top-node list
-  uint32_t baseNode[2], baseSmall[2], baseTop[2];  // each level's bases, by level parity (emit)
+  uint32_t baseNode[2], baseSmall[2], baseTop[2];  // each pass's bases, by pass parity (emit)
 };
 
 // An open node with > kMedium prims.  [begin, end) is its range in the CPU
@@ -122,8 +124,8 @@ struct Ctx {
   Node *loc;               // small phase: local nodes of segment s at loc[2*begin_s + j]
   uint32_t *smallCount;    // small phase: local node count per small segment
   uint32_t *smallBase;     // small phase: first global id of local node 1
-  // per-level arrays, double-buffered by level parity: the emit of level l
-  // reads the [l&1] copies and prepares the [(l+1)&1] ones
+  // per-pass arrays, double-buffered by pass parity: the emit of pass p
+  // reads the [p&1] copies and prepares the [(p+1)&1] ones
   Seg *segs[2];            // open segments of a pass
   uint32_t *tileStart[2];  // [maxLarge+1] first tile of each segment
   uint32_t *tileSeg[2];    // [maxTiles] segment of each tile
@@ -140,7 +142,7 @@ struct Ctx {
   uint64_t *tileStatus;    // [4][maxTiles] look-back status per class
   uint32_t maxTiles;
   SmallSeg *small;         // [numValid]
-  uint32_t *topNodes;      // [numValid] inner nodes of the level phase
+  uint32_t *topNodes;      // [numValid] inner nodes of the pass phase
   uint32_t *levelStart;    // [2*maxPasses+1] start of each node list in topNodes
   uint32_t *levelItems;    // [maxPasses] prims moved by each pass
   uint32_t maxLarge;
@@ -1625,7 +1627,8 @@ __global__ void k_copy_top(const Node *src, Node *dst, uint32_t n) {
     dst[i] = src[i];
 }
 
-// Bounds of the level-phase inner nodes, deepest level first.
+// Bounds of the pass-phase inner nodes, node list by node list (two a
+// pass: a pass's split children before its segments), deepest first.
 __global__ void k_refit_top(Ctx c, Node *nodes, int level) {
   const uint32_t b = c.levelStart[level], e = c.levelStart[level + 1];
   for (uint32_t i = b + blockIdx.x * blockDim.x + threadIdx.x; i < e; i += gridDim.x * blockDim.x) {

@@ -57,6 +57,14 @@ constexpr uint32_t kAccA = 40;  // words (36 bounds, count, padding)
 // pass B accumulators per segment and copy: the bounds of the four
 // grandchild classes
 constexpr uint32_t kAccB = 24;
+// Pass B gathers the grandchild bounds of a split child only when the child
+// has more than kLazyBounds prims: a smaller child's grandchildren are
+// nearly always small (their own phase computes their boxes), and the emit
+// reduces the bounds of an open one from its prims.
+constexpr uint32_t kLazyBounds = 4 * kMedium;
+__host__ __device__ __forceinline__ bool gather_bounds(uint32_t split, uint32_t childSize) {
+  return split != 0 && childSize > kLazyBounds;
+}
 
 struct Counters {
   uint32_t numValid;
@@ -790,18 +798,24 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair(Ctx c, int pass) {
     // copy out: staged class q -> its run; bounds of each class on the way
     const uint32_t runOff[4] = {0u, off1, off2, off3};
     const uint32_t runBase[4] = {begin + p0, mid - p1 - tot[1], mid + p2, end - p3 - tot[3]};
+    const bool gb0 = gather_bounds(split0, mid - begin), gb1 = gather_bounds(split1, end - mid);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const uint32_t o = runOff[q], cntq = tot[q], rb = runBase[q];
+      if (q < 2 ? gb0 : gb1) {
 #pragma unroll 4
-      for (uint32_t i = threadIdx.x; i < cntq; i += kBlock) {
-        const float4 x = sStage[o + i];
-        dst[min(rb + i, c.n - 1)] = x;  // (in range by construction; clamped to the buffer)
-        const int32_t ex = enc_f(x.x), ey = enc_f(x.y), ez = enc_f(x.z);
-        CUBQL_ADD6(cacc, 6 * q, ex, ey, ez);
+        for (uint32_t i = threadIdx.x; i < cntq; i += kBlock) {
+          const float4 x = sStage[o + i];
+          dst[min(rb + i, c.n - 1)] = x;  // (in range by construction; clamped to the buffer)
+          const int32_t ex = enc_f(x.x), ey = enc_f(x.y), ez = enc_f(x.z);
+          CUBQL_ADD6(cacc, 6 * q, ex, ey, ez);
+        }
+      } else {
+#pragma unroll 4
+        for (uint32_t i = threadIdx.x; i < cntq; i += kBlock) dst[min(rb + i, c.n - 1)] = sStage[o + i];
       }
     }
-    if (nt >= numTiles || U(sTile[nslot][1]) != seg) {
+    if ((gb0 || gb1) && (nt >= numTiles || U(sTile[nslot][1]) != seg)) {
 #pragma unroll
       for (int i = 0; i < 24; i++) cacc[i] = (i % 6) < 3 ? wave_min_i32(cacc[i]) : wave_max_i32(cacc[i]);
       if (lane == 0) {
@@ -875,7 +889,7 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit2(Ctx c, int pass) {
   constexpr int NW = kEmitBlock / kWave;
   __shared__ uint32_t ss[NW + 1];
   __shared__ uint32_t sPre[kEmitCounts];
-  __shared__ uint3 sFix[4 * kEmitBlock];  // (segment, first slot, size) of open halves
+  __shared__ uint3 sFix[4 * kEmitBlock];  // (segment, first slot, size) of open grandchildren to reduce
   __shared__ uint2 sRev[2 * kEmitBlock];  // (first slot, size) of ranges to turn around
   __shared__ uint32_t sNumFix, sNumRev;
   __shared__ float sb[12 * NW];
@@ -1027,7 +1041,7 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit2(Ctx c, int pass) {
         ns.begin = gk.b; ns.end = gk.e; ns.node = gid + gs; ns.pbeg = gk.b; ns.rev = gk.rev; ns.pad = 0;
         for (int j = 0; j < 6; j++) ns.cb[j] = gcb[6 * (2 * side + gs) + j];
         c.segs[np][li] = ns;
-        if (!gk.oneSided) {
+        if (!gk.oneSided && gather_bounds(split[side], ce - cb)) {
           float lo[3], hi[3];
           for (int d = 0; d < 3; d++) { lo[d] = dec_f(ns.cb[d]); hi[d] = dec_f(ns.cb[d + 3]); }
           int dim;
@@ -1078,9 +1092,11 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit2(Ctx c, int pass) {
       items[pb + sz - 1 - i] = a;
     }
   }
-  // halves of one-sided child splits: centroid bounds and plane from their
-  // prims (a full barrier: the segment records above must land before these
-  // overwrite their bounds, and the turned ranges before these read them)
+  // open grandchildren whose bounds pass B did not gather (halves of
+  // one-sided child splits, children of <= kLazyBounds prims): centroid
+  // bounds and plane from their prims (a full barrier: the segment records
+  // above must land before these overwrite their bounds, and the turned
+  // ranges before these read them)
   __syncthreads();
   const uint32_t nFix = sNumFix;
   for (uint32_t f = 0; f < nFix; f++) {

@@ -1142,24 +1142,34 @@ __device__ __forceinline__ uint32_t fr_pack(uint32_t b, uint32_t e, uint32_t me)
 // Segmented inclusive min/max scan over contiguous lane segments: lane l
 // ends with the reduction over [head(l), l].  Steps are the DPP wave scan
 // (row_shr 1,2,4,8 inside rows of 16, then row_bcast 15 and 31 across rows);
-// cond[k] says whether step k's source lane lies in the lane's segment.
-// (on enc_cmp-encoded values: integer min/max with an identity as the DPP
-// "old" value is one fused v_min/v_max_i32_dpp per step, plus the select)
+// cond[k] says whether step k's source lane lies in the lane's segment (and,
+// for the broadcast steps, that the lane's row takes part in the step).
+// All scans run as unsigned max scans: a min becomes a max of complemented
+// keys, so 0 is the identity of every step, which is what a row_shr step
+// reads from a lane outside the row (bound_ctrl), and no step needs its
+// destination preloaded with an identity.
 template <bool MIN>
-__device__ __forceinline__ int32_t seg_scan(int32_t v, const bool (&cond)[6]) {
-  const int32_t I = MIN ? INT32_MAX : INT32_MIN;
-#define CUBQL_SEG_STEP(K, CTRL, ROWMASK)                                                     \
-  {                                                                                          \
-    const int32_t t = __builtin_amdgcn_update_dpp(I, v, CTRL, ROWMASK, 0xF, false);          \
-    const int32_t m = MIN ? min(v, t) : max(v, t);                                           \
-    v = cond[K] ? m : v;                                                                     \
+__device__ __forceinline__ uint32_t seg_key(int32_t v) {
+  const uint32_t u = (uint32_t)v ^ 0x80000000u;  // signed order -> unsigned order
+  return MIN ? ~u : u;
+}
+template <bool MIN>
+__device__ __forceinline__ int32_t seg_unkey(uint32_t k) {
+  return (int32_t)((MIN ? ~k : k) ^ 0x80000000u);
+}
+__device__ __forceinline__ uint32_t seg_scan_max(uint32_t v, const bool (&cond)[6]) {
+#define CUBQL_SEG_STEP(K, CTRL, ROWMASK, BOUND)                                                    \
+  {                                                                                                \
+    const uint32_t t = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, ROWMASK, 0xF, BOUND);      \
+    const uint32_t m = max(v, t);                                                                  \
+    v = cond[K] ? m : v;                                                                           \
   }
-  CUBQL_SEG_STEP(0, 0x111, 0xF)
-  CUBQL_SEG_STEP(1, 0x112, 0xF)
-  CUBQL_SEG_STEP(2, 0x114, 0xF)
-  CUBQL_SEG_STEP(3, 0x118, 0xF)
-  CUBQL_SEG_STEP(4, 0x142, 0xA)
-  CUBQL_SEG_STEP(5, 0x143, 0xC)
+  CUBQL_SEG_STEP(0, 0x111, 0xF, true)
+  CUBQL_SEG_STEP(1, 0x112, 0xF, true)
+  CUBQL_SEG_STEP(2, 0x114, 0xF, true)
+  CUBQL_SEG_STEP(3, 0x118, 0xF, true)
+  CUBQL_SEG_STEP(4, 0x142, 0xA, false)
+  CUBQL_SEG_STEP(5, 0x143, 0xC, false)
 #undef CUBQL_SEG_STEP
   return v;
 }
@@ -1167,7 +1177,7 @@ __device__ __forceinline__ int32_t seg_scan(int32_t v, const bool (&cond)[6]) {
 // Reduction of v over the lane's segment [sb, se), in every lane of it.
 template <bool MIN>
 __device__ __forceinline__ int32_t seg_reduce(int32_t v, const bool (&cond)[6], uint32_t last) {
-  return __shfl(seg_scan<MIN>(v, cond), (int)last, kWave);
+  return seg_unkey<MIN>((uint32_t)__shfl((int)seg_scan_max(seg_key<MIN>(v), cond), (int)last, kWave));
 }
 
 __device__ __forceinline__ void store_node_lane(Node *dst, const float (&lo)[3], const float (&hi)[3],
@@ -1207,8 +1217,9 @@ __device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, 
   }
   while (__ballot(active)) {
     const int h = (int)sb, l = (int)lane;
+    // (the broadcast steps write rows 1, 3 and rows 2, 3 only)
     const bool cond[6] = {l - 1 >= h, l - 2 >= h, l - 4 >= h, l - 8 >= h,
-                          (int)(lane & ~15u) - 1 >= h, lane >= 32u && 31 >= h};
+                          (lane & 16u) != 0 && (int)(lane & ~15u) - 1 >= h, lane >= 32u && 31 >= h};
     const uint32_t last = se - 1;
     float clo[3], chi[3], nlo[3], nhi[3];
     clo[0] = dec_f(seg_reduce<true>(xe, cond, last));

@@ -267,6 +267,27 @@ def bench_mixture(cq, dev, reps: int):
     return res
 
 
+def build_model_bytes(cq, bvh, n):
+    """Algorithmic bytes of one spatial-median build, counted from its tree:
+    24 B box read + 4 B primID written per prim, 32 B per node written, a 12 B
+    centroid written per prim, 20 B per prim and tree level while its node is
+    open (12 B centroid read, 4 B node id read and written) and a 24 B leaf
+    box read for the bounds.  Returns (bytes, sum over levels of open prims)."""
+    import numpy as np
+    oc = cq.nodes_to_numpy(bvh)["oc"].astype(np.uint64)
+    off = (oc & np.uint64((1 << 48) - 1)).astype(np.int64)
+    cnt = (oc >> np.uint64(48)).astype(np.int64)
+    frontier, depth, open_prims = np.array([0], np.int64), 0, 0
+    while frontier.size:
+        leaf = cnt[frontier] > 0
+        open_prims += int(cnt[frontier[leaf]].sum()) * depth
+        inner = frontier[~leaf]
+        frontier = np.concatenate([off[inner], off[inner] + 1])
+        depth += 1
+    m = int(bvh.numNodes)
+    return 64 * n + 32 * m + 20 * open_prims, open_prims
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -412,7 +433,14 @@ def main():
     torch.cuda.synchronize()
     knn_ms = e0.elapsed_time(e1) / reps
     num_nodes = bvh.numNodes
+    model_bytes, open_prims = build_model_bytes(cq, bvh, N)
     bvh.free()
+    # the whole build against HBM peak, with the per-level byte model
+    build_roofline = {"bound": "hbm", "bytes": model_bytes, "sum_open_prims": open_prims,
+                      "achieved": model_bytes / (ms_per_step * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": model_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                      "bytes_model": "64 B/prim (box read, primID, centroid, leaf box) + 32 B/node "
+                                     "+ 20 B per prim and level in an open node"}
     print(f"[bench] fcp {nq/fcp_ms/1e3:.1f} Mq/s, knn {nk/knn_ms/1e3:.1f} Mq/s", file=sys.stderr,
           flush=True)
 
@@ -474,6 +502,7 @@ def main():
         "batched_chamfer": batched,
         "kernels": kern,
         "roofline": roofline,
+        "build_roofline": build_roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -165,7 +165,8 @@ def bench_batched(cq, dev, num_clouds: int, plain_budget_s: float, tune: bool):
 
 
 def partition_traffic():
-    """HBM bytes per sm::k_partition launch measured with rocprofv3 PMC
+    """HBM bytes per sm::k_pair launch (the pass-B partition, reported as
+    sm_partition) measured with rocprofv3 PMC
     counters (FETCH_SIZE + WRITE_SIZE, scaled by a 1 GiB copy); the summary
     with its commands lives in profiles/sm_partition_10M_pmc.md."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "sm_partition_10M_pmc.md")

@@ -399,8 +399,13 @@ int build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, in
   int level = 0;
   int hs = prof_begin(s);
   while (true) {
-    for (int i = 0; i < kLevelsPerSync && level < kMaxLevels; i++, level++)
-      k_rlevel<K><<<grid, kBlock, 0, s>>>(c, kRes, level);
+    for (int i = 0; i < kLevelsPerSync && level < kMaxLevels; i++, level++) {
+      // level l has at most 2^l open nodes: the top levels launch a few
+      // workgroups instead of a chip-wide grid of idle ones
+      const uint64_t maxOpen = std::min<uint64_t>(level < 40 ? (1ull << level) : ~0ull, (uint64_t)n);
+      const uint32_t gridL = std::max<uint32_t>(1u, std::min<uint32_t>(grid, div_up(maxOpen, kBlock)));
+      k_rlevel<K><<<gridL, kBlock, 0, s>>>(c, kRes, level);
+    }
     if (hipGetLastError() != hipSuccess) return fail((set_error("level launch failed"), CUBQL_ERR_HIP));
     if (hipMemcpyAsync(hc.data(), c.ctr, sizeof(uint32_t) * hc.size(), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)

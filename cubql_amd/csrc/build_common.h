@@ -23,9 +23,11 @@ __device__ inline float coord(const float4 &v, int d) {
 // by strict '>' (ties to the lower axis), pos = midpoint.
 __device__ inline void choose_split(const float lo[3], const float hi[3], int &dim, float &pos) {
   float s0 = hi[0] - lo[0], s1 = hi[1] - lo[1], s2 = hi[2] - lo[2];
-  dim = s2 > fmaxf(s0, s1) ? 2 : (s1 > s0 ? 1 : 0);
-  float l = dim == 0 ? lo[0] : (dim == 1 ? lo[1] : lo[2]);
-  float h = dim == 0 ? hi[0] : (dim == 1 ? hi[1] : hi[2]);
+  const bool d2 = s2 > fmaxf(s0, s1), d1 = !d2 && s1 > s0;
+  dim = d2 ? 2 : (d1 ? 1 : 0);
+  // (selects, not a branch on dim)
+  const float l = d2 ? lo[2] : (d1 ? lo[1] : lo[0]);
+  const float h = d2 ? hi[2] : (d1 ? hi[1] : hi[0]);
   pos = (l + h) / 2.f;
 }
 

@@ -1180,6 +1180,31 @@ __device__ __forceinline__ int32_t seg_reduce(int32_t v, const bool (&cond)[6], 
   return seg_unkey<MIN>((uint32_t)__shfl((int)seg_scan_max(seg_key<MIN>(v), cond), (int)last, kWave));
 }
 
+// Inclusive unsigned max scan over the whole wave (0 is the identity).
+__device__ __forceinline__ uint32_t wave_scan_max_u(uint32_t v) {
+#define CUBQL_MAX_STEP(CTRL, ROWMASK)                                                               \
+  {                                                                                                 \
+    const uint32_t t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWMASK, 0xF, true);  \
+    v = max(v, t);                                                                                  \
+  }
+  CUBQL_MAX_STEP(0x111, 0xF)
+  CUBQL_MAX_STEP(0x112, 0xF)
+  CUBQL_MAX_STEP(0x114, 0xF)
+  CUBQL_MAX_STEP(0x118, 0xF)
+  CUBQL_MAX_STEP(0x142, 0xA)
+  CUBQL_MAX_STEP(0x143, 0xC)
+#undef CUBQL_MAX_STEP
+  return v;
+}
+
+// Segmented max of 26-bit values by an unsegmented scan: the key puts the
+// lane's segment head above the value, heads grow along the lanes, so the
+// scan's maximum over lanes [0, l] comes from l's own segment.
+constexpr uint32_t kRelBits = 26, kRelMask = (1u << kRelBits) - 1u;
+__device__ __forceinline__ uint32_t rel_seg_max(uint32_t sbHi, uint32_t u, uint32_t last) {
+  return (uint32_t)__shfl((int)wave_scan_max_u(sbHi | u), (int)last, kWave) & kRelMask;
+}
+
 __device__ __forceinline__ void store_node_lane(Node *dst, const float (&lo)[3], const float (&hi)[3],
                                                 uint64_t oc) {
   Node nd;
@@ -1191,9 +1216,9 @@ __device__ __forceinline__ void store_node_lane(Node *dst, const float (&lo)[3],
 // Builds the subtree of a node with cnt <= 64 prims held one per lane
 // (lane j = prim at position pos0 + j), splitting every open node of the
 // subtree in the same round: each lane tracks the range [sb, se) of the node
-// that holds its position; bounds are segmented wave reductions, ranks come
-// from ballots masked to the segment, items move with one ds_permute per
-// value.  Split rules and partition order (left stable, right reversed,
+// that holds its position; bounds are segmented wave reductions, ranks are
+// ballot prefix counts less those of the segment's head, items move with
+// one ds_permute per value.  Split rules and partition order (left stable, right reversed,
 // one-sided -> middle) are those of the CPU recursion
 // (cpu/spatialMedian.h:81-114), so positions and leaves are identical to a
 // node-at-a-time build; local ids are assigned breadth-first.
@@ -1202,7 +1227,6 @@ __device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, 
                             uint32_t id, float bx0, float by0, float bz0, float bx1, float by1,
                             float bz1, uint32_t rootLocal, uint32_t &nLocal, Node *loc) {
   const uint32_t lane = lane_id();
-  const uint64_t lt = lanemask_lt();
   const uint32_t T = c.T;
   bool active = lane < cnt;
   uint32_t sb = active ? 0u : lane, se = active ? cnt : lane + 1u;
@@ -1215,29 +1239,74 @@ __device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, 
     b0 = enc_cmp(bx0); b1 = enc_cmp(by0); b2 = enc_cmp(bz0);
     b3 = enc_cmp(bx1); b4 = enc_cmp(by1); b5 = enc_cmp(bz1);
   }
+  bool first = true;
+  // point input whose root extent spans < 2^26 encoded floats per axis:
+  // the segmented reductions of later rounds run as unsegmented scans on
+  // (segment head, value - root minimum) keys
+  bool rel = false;
+  int32_t eLo[3] = {0, 0, 0};
   while (__ballot(active)) {
-    const int h = (int)sb, l = (int)lane;
-    // (the broadcast steps write rows 1, 3 and rows 2, 3 only)
-    const bool cond[6] = {l - 1 >= h, l - 2 >= h, l - 4 >= h, l - 8 >= h,
-                          (lane & 16u) != 0 && (int)(lane & ~15u) - 1 >= h, lane >= 32u && 31 >= h};
-    const uint32_t last = se - 1;
     float clo[3], chi[3], nlo[3], nhi[3];
-    clo[0] = dec_f(seg_reduce<true>(xe, cond, last));
-    clo[1] = dec_f(seg_reduce<true>(ye, cond, last));
-    clo[2] = dec_f(seg_reduce<true>(ze, cond, last));
-    chi[0] = dec_f(seg_reduce<false>(xe, cond, last));
-    chi[1] = dec_f(seg_reduce<false>(ye, cond, last));
-    chi[2] = dec_f(seg_reduce<false>(ze, cond, last));
-    if (PTS) {
-      for (int d = 0; d < 3; d++) { nlo[d] = clo[d]; nhi[d] = chi[d]; }
+    if (PTS && rel) {
+      const uint32_t sbHi = sb << kRelBits, last = se - 1;
+      const int32_t e3[3] = {xe, ye, ze};
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const uint32_t u = (uint32_t)(e3[d] - eLo[d]);
+        clo[d] = dec_f(eLo[d] + (int32_t)(rel_seg_max(sbHi, u ^ kRelMask, last) ^ kRelMask));
+        chi[d] = dec_f(eLo[d] + (int32_t)rel_seg_max(sbHi, u, last));
+      }
+    } else if (first) {
+      // one segment, lanes [0, cnt): plain wave reductions, idle lanes
+      // holding the identity
+      first = false;
+      clo[0] = dec_f(wave_min_i32(active ? xe : INT32_MAX));
+      clo[1] = dec_f(wave_min_i32(active ? ye : INT32_MAX));
+      clo[2] = dec_f(wave_min_i32(active ? ze : INT32_MAX));
+      chi[0] = dec_f(wave_max_i32(active ? xe : INT32_MIN));
+      chi[1] = dec_f(wave_max_i32(active ? ye : INT32_MIN));
+      chi[2] = dec_f(wave_max_i32(active ? ze : INT32_MIN));
+      if (!PTS) {
+        nlo[0] = dec_f(wave_min_i32(active ? b0 : INT32_MAX));
+        nlo[1] = dec_f(wave_min_i32(active ? b1 : INT32_MAX));
+        nlo[2] = dec_f(wave_min_i32(active ? b2 : INT32_MAX));
+        nhi[0] = dec_f(wave_max_i32(active ? b3 : INT32_MIN));
+        nhi[1] = dec_f(wave_max_i32(active ? b4 : INT32_MIN));
+        nhi[2] = dec_f(wave_max_i32(active ? b5 : INT32_MIN));
+      }
+      if (PTS) {
+        rel = true;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          eLo[d] = enc_cmp(clo[d]);
+          rel = rel && (int64_t)enc_cmp(chi[d]) - eLo[d] <= (int64_t)kRelMask;
+        }
+        // lanes past the prims hold the minimum (relative value 0)
+        if (lane >= cnt) { xe = eLo[0]; ye = eLo[1]; ze = eLo[2]; }
+      }
     } else {
-      nlo[0] = dec_f(seg_reduce<true>(b0, cond, last));
-      nlo[1] = dec_f(seg_reduce<true>(b1, cond, last));
-      nlo[2] = dec_f(seg_reduce<true>(b2, cond, last));
-      nhi[0] = dec_f(seg_reduce<false>(b3, cond, last));
-      nhi[1] = dec_f(seg_reduce<false>(b4, cond, last));
-      nhi[2] = dec_f(seg_reduce<false>(b5, cond, last));
+      const int h = (int)sb, l = (int)lane;
+      // (the broadcast steps write rows 1, 3 and rows 2, 3 only)
+      const bool cond[6] = {l - 1 >= h, l - 2 >= h, l - 4 >= h, l - 8 >= h,
+                            (lane & 16u) != 0 && (int)(lane & ~15u) - 1 >= h, lane >= 32u && 31 >= h};
+      const uint32_t last = se - 1;
+      clo[0] = dec_f(seg_reduce<true>(xe, cond, last));
+      clo[1] = dec_f(seg_reduce<true>(ye, cond, last));
+      clo[2] = dec_f(seg_reduce<true>(ze, cond, last));
+      chi[0] = dec_f(seg_reduce<false>(xe, cond, last));
+      chi[1] = dec_f(seg_reduce<false>(ye, cond, last));
+      chi[2] = dec_f(seg_reduce<false>(ze, cond, last));
+      if (!PTS) {
+        nlo[0] = dec_f(seg_reduce<true>(b0, cond, last));
+        nlo[1] = dec_f(seg_reduce<true>(b1, cond, last));
+        nlo[2] = dec_f(seg_reduce<true>(b2, cond, last));
+        nhi[0] = dec_f(seg_reduce<false>(b3, cond, last));
+        nhi[1] = dec_f(seg_reduce<false>(b4, cond, last));
+        nhi[2] = dec_f(seg_reduce<false>(b5, cond, last));
+      }
     }
+    if (PTS)
+      for (int d = 0; d < 3; d++) { nlo[d] = clo[d]; nhi[d] = chi[d]; }
     const uint32_t n = se - sb;
     const bool isHead = lane == sb;
     const bool leaf = active && n <= T;
@@ -1251,16 +1320,20 @@ __device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, 
     choose_split(clo, chi, dim, pos);
     const int32_t ve = dim == 0 ? xe : (dim == 1 ? ye : ze);
     const bool isL = split && ve < enc_cmp(pos);
-    const uint64_t segMask = (se >= 64u ? ~0ull : ((1ull << se) - 1ull)) & ~((1ull << sb) - 1ull);
-    const uint64_t mL = __ballot(isL) & segMask;
-    const uint64_t mR = __ballot(split && !isL) & segMask;
-    const uint32_t nl = (uint32_t)__popcll(mL);
-    const uint32_t dst = !split ? lane
-                                : (isL ? sb + (uint32_t)__popcll(mL & lt) : se - 1u - (uint32_t)__popcll(mR & lt));
-    const uint32_t mid = (nl != 0 && nl != n) ? sb + nl
-                                              : (uint32_t)(((uint64_t)pos0 + sb + pos0 + se) / 2 - pos0);  // spatialMedian.h:107
+    // ranks inside the segment: counts below the lane minus the counts
+    // below its head, read from the head with the children's local id
+    // (7 + 7 + 18 bits); the left count of the segment from its last lane
+    const uint64_t mL = __ballot(isL), mR = __ballot(split && !isL);
     const uint64_t heads = __ballot(split && isHead);
-    const uint32_t child = (uint32_t)__shfl((int)(nLocal + 2u * (uint32_t)__popcll(heads & lt)), (int)sb, kWave);
+    const uint32_t pL = mbcnt(mL), pR = mbcnt(mR);
+    const uint32_t hw = (uint32_t)__shfl((int)(pL | (pR << 7) | ((nLocal + 2u * mbcnt(heads)) << 14)), (int)sb, kWave);
+    const uint32_t tL = (uint32_t)__shfl((int)(pL + (isL ? 1u : 0u)), (int)(se - 1u), kWave);
+    const uint32_t hL = hw & 127u, hR = (hw >> 7) & 127u, child = hw >> 14;
+    const uint32_t nl = tL - hL;
+    const uint32_t dst = !split ? lane : (isL ? sb + (pL - hL) : se - 1u - (pR - hR));
+    // (one-sided: the middle of the global range, spatialMedian.h:107;
+    // (2 pos0 + sb + se) / 2 - pos0 == (sb + se) / 2)
+    const uint32_t mid = (nl != 0 && nl != n) ? sb + nl : (sb + se) >> 1;
     nLocal += 2u * (uint32_t)__popcll(heads);
     if (split && isHead) store_node_lane(loc + me, nlo, nhi, pack_oc(child, 0));
     xe = (int32_t)lane_push_u(dst, (uint32_t)xe);
@@ -1541,6 +1614,10 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
       // keeps the kernel's register count at the small pieces' occupancy)
       int32_t sc[2][6];
       bounds_init(sc[0]); bounds_init(sc[1]);
+      // (only a piece of more than 2 kSmall prims can have a child that
+      // needs them; a child of another piece that is still larger than
+      // kSmall after an uneven split takes a bounds pass)
+      const bool gather = PTS && e - b > 2u * (uint32_t)kSmall;
       uint32_t nL = 0, nR = 0;
       for (uint32_t c0 = b; c0 < e; c0 += kMedChunk * kWave) {
         float4 v[kMedChunk];
@@ -1561,18 +1638,21 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
           if (act[k]) {
             const uint32_t d = isL ? b + nL + mbcnt(mL) : e - 1 - (nR + mbcnt(mR));
             dst[gb + d] = v[k];
-            const int s = isL ? 0 : 1;
-            // (select both sides' updates: no divergent array indexing)
-            for (int q = 0; q < 2; q++)
-              if (s == q) bounds_add(sc[q], ex, ey, ez, ex, ey, ez);
+            if (gather) {
+              const int s = isL ? 0 : 1;
+              // (select both sides' updates: no divergent array indexing)
+              for (int q = 0; q < 2; q++)
+                if (s == q) bounds_add(sc[q], ex, ey, ez, ex, ey, ez);
+            }
           }
           nL += (uint32_t)__popcll(mL);
           nR += (uint32_t)__popcll(mR);
         }
       }
       const bool oneSided = nL == 0 || nR == 0;
-      const uint32_t mid = !oneSided ? b + nL
-                                     : (uint32_t)(((uint64_t)gb + b + gb + e) / 2 - gb);  // spatialMedian.h:107
+      // (one-sided: the middle of the global range, spatialMedian.h:107;
+      // (2 gb + b + e) / 2 - gb == (b + e) / 2)
+      const uint32_t mid = !oneSided ? b + nL : (b + e) >> 1;
       const uint32_t child = nLocal;
       nLocal += 2;
       store_node_wave(loc + me, olo, ohi, pack_oc(child, 0));
@@ -1584,7 +1664,7 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
           piece(bf ^ 1u, cb0, ce0 - cb0, child + side);
         } else {
           MedFrame g{};
-          const bool carry = PTS && !oneSided;
+          const bool carry = gather && !oneSided;
           g.b = cb0; g.e = ce0; g.me = child + side; g.buf = (bf ^ 1u) | (carry ? 0u : 2u);
           g.pb = gb + cb0;
           if (carry) {

@@ -143,6 +143,10 @@ def _deep_case(name):
     elif name == "line":            # every split along x
         P = np.zeros((250_000, 3), np.float32)
         P[:, 0] = rng.random(250_000, dtype=np.float32)
+    elif name == "signed":          # both signs on every axis, -0 and +0 included
+        P = rng.random((300_000, 3), dtype=np.float32) * np.float32(2) - np.float32(1)
+        P[::1000] = np.float32(0)
+        P[1::1000] = np.float32(-0.0)
     elif name == "grid_ties":       # integer grid: planes fall on coordinates
         g = np.arange(64, dtype=np.float32)
         P = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
@@ -158,7 +162,7 @@ def _deep_case(name):
     return np.concatenate([P, P], 1).astype(np.float32), T
 
 
-@pytest.mark.parametrize("name", ["skewed", "dupes_large", "halfdup", "line", "grid_ties", "boxes"])
+@pytest.mark.parametrize("name", ["skewed", "dupes_large", "halfdup", "line", "signed", "grid_ties", "boxes"])
 def test_gpu_builder_deep_and_degenerate_inputs(cq, oracle_mod, name):
     print(f"[gpu] sm deep case={name}", flush=True)
     boxes, T = _deep_case(name)

@@ -31,6 +31,28 @@ __device__ inline void choose_split(const float lo[3], const float hi[3], int &d
   pos = (l + h) / 2.f;
 }
 
+// One wave copies a finished small segment's cnt local nodes to the final
+// array: local node 0 to `root`, local node j >= 1 to base + j - 1, with an
+// inner node's child offset (a local id) rebased the same way.  A node moves
+// as two 16-byte words; the second holds hi[1], hi[2] and oc.
+__device__ __forceinline__ void copy_local_nodes(const Node *loc, uint32_t cnt, uint32_t root, uint32_t base,
+                                                 Node *out) {
+  const uint4 *src = reinterpret_cast<const uint4 *>(loc);
+  for (uint32_t j = lane_id(); j < cnt; j += kWave) {
+    const uint4 a = src[2 * j];
+    uint4 b = src[2 * j + 1];
+    const uint64_t oc = ((uint64_t)b.w << 32) | b.z;
+    if (oc_count(oc) == 0) {
+      const uint64_t g = pack_oc(base + (uint32_t)oc_offset(oc) - 1, 0);
+      b.z = (uint32_t)g;
+      b.w = (uint32_t)(g >> 32);
+    }
+    uint4 *d = reinterpret_cast<uint4 *>(out + (j == 0 ? root : base + j - 1));
+    d[0] = a;
+    d[1] = b;
+  }
+}
+
 // One wave writes a 32-byte node: lanes 0-7 store one word each.
 __device__ __forceinline__ void store_node_wave(Node *dst, const float (&lo)[3], const float (&hi)[3],
                                                 uint64_t oc) {

@@ -754,14 +754,7 @@ __global__ __launch_bounds__(kWave) void k_thread_emit(RCtx c, Node *out) {
   const uint32_t nSmall = c.ctr[C_SMALL];
   for (uint32_t s = blockIdx.x; s < nSmall; s += gridDim.x) {
     const uint32_t node = c.small[s];
-    const uint32_t cnt = c.smallCount[s];
-    const uint32_t base = c.smallBase[s];
-    const Node *loc = c.loc + 2 * (size_t)c.ranges[node].x;
-    for (uint32_t j = lane_id(); j < cnt; j += kWave) {
-      Node nd = loc[j];
-      if (oc_count(nd.oc) == 0) nd.oc = pack_oc(base + (uint32_t)oc_offset(nd.oc) - 1, 0);
-      out[j == 0 ? node : base + j - 1] = nd;
-    }
+    copy_local_nodes(c.loc + 2 * (size_t)c.ranges[node].x, c.smallCount[s], node, c.smallBase[s], out);
   }
 }
 

@@ -821,14 +821,7 @@ __global__ __launch_bounds__(kWave) void k_small_emit(Ctx c, Node *out) {
   const uint32_t nSmall = c.ctr->numSmall;
   for (uint32_t w = blockIdx.x; w < nSmall; w += gridDim.x) {
     const SmallSeg sg = c.small[w];
-    const uint32_t cnt = c.smallCount[w];
-    const uint32_t base = c.smallBase[w];
-    const Node *loc = c.loc + 2 * (size_t)sg.begin;
-    for (uint32_t j = lane_id(); j < cnt; j += kWave) {
-      Node nd = loc[j];
-      if (oc_count(nd.oc) == 0) nd.oc = pack_oc(base + (uint32_t)oc_offset(nd.oc) - 1, 0);
-      out[j == 0 ? sg.node : base + j - 1] = nd;
-    }
+    copy_local_nodes(c.loc + 2 * (size_t)sg.begin, c.smallCount[w], sg.node, c.smallBase[w], out);
   }
 }
 

@@ -1720,23 +1720,7 @@ __global__ __launch_bounds__(256) void k_small_emit(Ctx c, Node *out) {
   const uint32_t w0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * wpb + threadIdx.x / kWave));
   for (uint32_t w = w0; w < nSmall; w += gridDim.x * wpb) {
     const SmallSeg sg = c.small[w];
-    const uint32_t cnt = c.smallCount[w];
-    const uint32_t base = c.smallBase[w];
-    // a node as two 16-byte words; the second holds hi[1], hi[2] and oc
-    const uint4 *loc = reinterpret_cast<const uint4 *>(c.loc + 2 * (size_t)sg.begin);
-    for (uint32_t j = lane_id(); j < cnt; j += kWave) {
-      const uint4 a = loc[2 * j];
-      uint4 b = loc[2 * j + 1];
-      const uint64_t oc = ((uint64_t)b.w << 32) | b.z;
-      if (oc_count(oc) == 0) {
-        const uint64_t g = pack_oc(base + (uint32_t)oc_offset(oc) - 1, 0);
-        b.z = (uint32_t)g;
-        b.w = (uint32_t)(g >> 32);
-      }
-      uint4 *d = reinterpret_cast<uint4 *>(out + (j == 0 ? sg.node : base + j - 1));
-      d[0] = a;
-      d[1] = b;
-    }
+    copy_local_nodes(c.loc + 2 * (size_t)sg.begin, c.smallCount[w], sg.node, c.smallBase[w], out);
   }
 }
 

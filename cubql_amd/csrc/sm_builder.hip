@@ -1243,18 +1243,25 @@ __device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, 
   // point input whose root extent spans < 2^26 encoded floats per axis:
   // the segmented reductions of later rounds run as unsegmented scans on
   // (segment head, value - root minimum) keys
-  bool rel = false;
+  bool rel = false, relPos = false;
   int32_t eLo[3] = {0, 0, 0};
   while (__ballot(active)) {
     float clo[3], chi[3], nlo[3], nhi[3];
     if (PTS && rel) {
       const uint32_t sbHi = sb << kRelBits, last = se - 1;
       const int32_t e3[3] = {xe, ye, ze};
+      int32_t lo[3], hi[3];
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         const uint32_t u = (uint32_t)(e3[d] - eLo[d]);
-        clo[d] = dec_f(eLo[d] + (int32_t)(rel_seg_max(sbHi, u ^ kRelMask, last) ^ kRelMask));
-        chi[d] = dec_f(eLo[d] + (int32_t)rel_seg_max(sbHi, u, last));
+        lo[d] = eLo[d] + (int32_t)(rel_seg_max(sbHi, u ^ kRelMask, last) ^ kRelMask);
+        hi[d] = eLo[d] + (int32_t)rel_seg_max(sbHi, u, last);
+      }
+      // (the encoding of a non-negative float is its bit pattern)
+      if (relPos) {
+        for (int d = 0; d < 3; d++) { clo[d] = __int_as_float(lo[d]); chi[d] = __int_as_float(hi[d]); }
+      } else {
+        for (int d = 0; d < 3; d++) { clo[d] = dec_f(lo[d]); chi[d] = dec_f(hi[d]); }
       }
     } else if (first) {
       // one segment, lanes [0, cnt): plain wave reductions, idle lanes
@@ -1276,10 +1283,12 @@ __device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, 
       }
       if (PTS) {
         rel = true;
+        relPos = true;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
           eLo[d] = enc_cmp(clo[d]);
           rel = rel && (int64_t)enc_cmp(chi[d]) - eLo[d] <= (int64_t)kRelMask;
+          relPos = relPos && eLo[d] >= 0;
         }
         // lanes past the prims hold the minimum (relative value 0)
         if (lane >= cnt) { xe = eLo[0]; ye = eLo[1]; ze = eLo[2]; }
@@ -1319,7 +1328,7 @@ __device__ void reg_subtree(const Ctx &c, uint32_t pos0, uint32_t cnt, float x, 
     float pos;
     choose_split(clo, chi, dim, pos);
     const int32_t ve = dim == 0 ? xe : (dim == 1 ? ye : ze);
-    const bool isL = split && ve < enc_cmp(pos);
+    const bool isL = split && ve < (relPos ? __float_as_int(pos) : enc_cmp(pos));
     // ranks inside the segment: counts below the lane minus the counts
     // below its head, read from the head with the children's local id
     // (7 + 7 + 18 bits); the left count of the segment from its last lane

@@ -1639,8 +1639,7 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
         }
 #pragma unroll
         for (int k = 0; k < kMedChunk; k++) {
-          const int32_t ex = enc_cmp(v[k].x), ey = enc_cmp(v[k].y), ez = enc_cmp(v[k].z);
-          const int32_t ec = dim == 0 ? ex : (dim == 1 ? ey : ez);
+          const int32_t ec = enc_cmp(dim == 0 ? v[k].x : (dim == 1 ? v[k].y : v[k].z));
           const bool isL = act[k] && ec < posE;
           const uint64_t mL = __ballot(isL);
           const uint64_t mR = __ballot(act[k] && !isL);
@@ -1648,6 +1647,7 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
             const uint32_t d = isL ? b + nL + mbcnt(mL) : e - 1 - (nR + mbcnt(mR));
             dst[gb + d] = v[k];
             if (gather) {
+              const int32_t ex = enc_cmp(v[k].x), ey = enc_cmp(v[k].y), ez = enc_cmp(v[k].z);
               const int s = isL ? 0 : 1;
               // (select both sides' updates: no divergent array indexing)
               for (int q = 0; q < 2; q++)

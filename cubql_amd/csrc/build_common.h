@@ -75,24 +75,28 @@ __device__ inline uint64_t status_word(uint32_t tag, uint32_t flag, uint32_t val
   return ((uint64_t)tag << 34) | ((uint64_t)flag << 32) | val;
 }
 
+// First half of the look-back below, which a caller may issue early: tile
+// t's own count, as its inclusive prefix if it is the node's first tile
+// (k == 0), else as an aggregate.
+__device__ inline void lookback_publish(uint64_t *status, uint32_t t, uint32_t k, uint32_t tag, uint32_t local) {
+  if (lane_id() == 0)
+    __hip_atomic_store(&status[t], status_word(tag, k == 0 ? 2u : 1u, local), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Decoupled look-back, wave-cooperative: all 64 lanes of one wave call it
-// for tile t, the k-th tile of its node.  Publishes `local`, then reads the
-// status words of up to 64 preceding tiles at once (lane i reads tile
-// t-1-i), sums them up to the nearest tile whose inclusive prefix is known
-// and repeats further back if none is.  Tiles are claimed in ticket order,
-// so every tile waited on is held by a running workgroup that publishes
-// without waiting on later tiles.  Returns the node's sum over its earlier
-// tiles (in every lane).
+// for tile t, the k-th tile of its node.  Publishes `local` (unless the
+// caller already did: published), then reads the status words of up to 64
+// preceding tiles at once (lane i reads tile t-1-i), sums them up to the
+// nearest tile whose inclusive prefix is known and repeats further back if
+// none is.  Tiles are claimed in ticket order, so every tile waited on is
+// held by a running workgroup that publishes without waiting on later
+// tiles.  Returns the node's sum over its earlier tiles (in every lane).
 __device__ inline uint32_t lookback_exclusive_wave(uint64_t *status, uint32_t t, uint32_t k, uint32_t tag,
-                                                   uint32_t local) {
+                                                   uint32_t local, bool published = false) {
   const uint32_t lane = lane_id();
-  if (k == 0) {
-    if (lane == 0)
-      __hip_atomic_store(&status[t], status_word(tag, 2, local), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return 0;
-  }
-  if (lane == 0)
-    __hip_atomic_store(&status[t], status_word(tag, 1, local), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!published) lookback_publish(status, t, k, tag, local);
+  if (k == 0) return 0;
   uint32_t prefix = 0;
   uint32_t first = t - 1;  // tile read by lane 0
   uint32_t remaining = k;  // earlier tiles of this node: t-k .. t-1

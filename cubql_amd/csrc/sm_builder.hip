@@ -747,6 +747,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair(Ctx c, int pass) {
     // its barriers also order the previous tile's reads of sStage, sPlan,
     // sTile and sPrefix before this tile's writes
     block_rank4_striped<kBlock, kItems>(cls, rank, tot, sr);
+    // this tile's counts go out before the staging, so tiles looking back
+    // over it wait less
+    const bool look = wid < 2 && (wid == 0 ? split0 : split1) != 0;
+    if (look) lookback_publish(c.tileStatus + (size_t)wid * c.maxTiles, t, k, tag, wid == 0 ? tot[0] : tot[2]);
     const uint32_t off1 = tot[0], off2 = off1 + tot[1], off3 = off2 + tot[2];
 #pragma unroll
     for (int jj = 0; jj < kItems; jj++) {
@@ -762,11 +766,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair(Ctx c, int pass) {
     // follow by difference, an unsplit child is one class.
     {
       const uint32_t q = 2 * wid;  // class (wid, left)
-      const bool look = wid < 2 && (wid == 0 ? split0 : split1) != 0;
       uint32_t pre = 0;
       if (look) {
         const uint32_t mine = wid == 0 ? tot[0] : tot[2];
-        pre = lookback_exclusive_wave(c.tileStatus + (size_t)wid * c.maxTiles, t, k, tag, mine);
+        pre = lookback_exclusive_wave(c.tileStatus + (size_t)wid * c.maxTiles, t, k, tag, mine, true);
         if (lane == 0 && t == tileEnd - 1) c.classTot[4 * (size_t)seg + q] = pre + mine;
       }
       if (lane == 0) {

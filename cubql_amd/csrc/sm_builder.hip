@@ -712,25 +712,23 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair(Ctx c, int pass) {
       else atomicMax(&acc[i], r);
     }
   };
-  {
-    const uint32_t n = U(sPlan[0].end) - U(sPlan[0].begin);
-    const uint32_t j0 = (t - U(sTile[0][2])) * kTile;
-    load_tile(src, U(sPlan[0].pbeg), n, U(sPlan[0].rev), j0, min(n - j0, (uint32_t)kTile), c.n, it);
-  }
-  while (true) {
-    const PairPlan &P = sPlan[slot];
-    const uint32_t seg = U(sTile[slot][1]), tileFirst = U(sTile[slot][2]), tileEnd = U(sTile[slot][3]);
-    const uint32_t begin = U(P.begin), end = U(P.end), mid = U(P.mid), mode = U(P.mode);
+  // A tile is classified (class 2 * child + side of each item, 4 = none;
+  // three bits an item), counted (per-wave class counts and offsets in sr,
+  // class totals) and its look-back counts published as soon as its items
+  // are in; it is ranked and staged later.  A workgroup counts its next tile
+  // before it copies out the current one, so the tiles that look back over
+  // the next one do not wait for that copy-out.
+  auto classify = [&](int sl, uint32_t tt) {
+    const PairPlan &P = sPlan[sl];
+    const uint32_t n = U(P.end) - U(P.begin), h = n / 2, mode = U(P.mode);
     const int dim = (int)U(P.dim);
     const float pos = __uint_as_float(U(P.pos));
     const uint32_t split0 = U(P.split[0]), split1 = U(P.split[1]);
     const int cdim0 = (int)U(P.cdim[0]), cdim1 = (int)U(P.cdim[1]);
     const float cpos0 = __uint_as_float(U(P.cpos[0])), cpos1 = __uint_as_float(U(P.cpos[1]));
-    const uint32_t n = end - begin, h = n / 2;
-    const uint32_t k = t - tileFirst;
-    const uint32_t j0 = k * kTile;
+    const uint32_t j0 = (tt - U(sTile[sl][2])) * kTile;
     const uint32_t cnt = min(n - j0, (uint32_t)kTile);
-    uint32_t cls[kItems];
+    uint32_t pk = 0;
 #pragma unroll
     for (int jj = 0; jj < kItems; jj++) {
       const uint32_t q = (uint32_t)(jj * kBlock) + threadIdx.x;
@@ -741,22 +739,78 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair(Ctx c, int pass) {
       const int cd = c1 ? cdim1 : cdim0;
       const float cpv = c1 ? cpos1 : cpos0;
       const uint32_t c2 = (sp && !(coord(it[jj], cd) < cpv)) ? 1u : 0u;
-      cls[jj] = q < cnt ? 2 * c1 + c2 : 4u;
+      pk |= (q < cnt ? 2 * c1 + c2 : 4u) << (3 * jj);
     }
-    uint32_t rank[kItems], tot[4];
-    // its barriers also order the previous tile's reads of sStage, sPlan,
-    // sTile and sPrefix before this tile's writes
-    block_rank4_striped<kBlock, kItems>(cls, rank, tot, sr);
-    // this tile's counts go out before the staging, so tiles looking back
-    // over it wait less
-    const bool look = wid < 2 && (wid == 0 ? split0 : split1) != 0;
-    if (look) lookback_publish(c.tileStatus + (size_t)wid * c.maxTiles, t, k, tag, wid == 0 ? tot[0] : tot[2]);
-    const uint32_t off1 = tot[0], off2 = off1 + tot[1], off3 = off2 + tot[2];
+    return pk;
+  };
+  constexpr int NC = kItems * NW;  // counters per class
+  static_assert(4 * NC == 2 * kWave, "two counters per lane");
+  auto count4 = [&](uint32_t pk, uint32_t (&tot)[4]) {
 #pragma unroll
     for (int jj = 0; jj < kItems; jj++) {
-      const uint32_t q = cls[jj], r = rank[jj];
-      if (q < 4) {
-        const uint32_t idx = q == 0 ? r : (q == 1 ? off2 - 1 - r : (q == 2 ? off2 + r : off3 + tot[3] - 1 - r));
+      const uint32_t cl = (pk >> (3 * jj)) & 7u;
+#pragma unroll
+      for (uint32_t q = 0; q < 4; q++) {
+        const uint64_t m = __ballot(cl == q);
+        if (lane == 0) sr[(q * kItems + jj) * NW + wid] = (uint32_t)__popcll(m);
+      }
+    }
+    sync_block_lds();
+    if (wid == 0) {
+      // lane l scans counters 2l, 2l+1; class q owns lanes [16q, 16q+16)
+      const uint32_t a = sr[2 * lane], b = sr[2 * lane + 1];
+      const uint32_t incl = wave_incl_scan(a + b);
+      const uint32_t ex = incl - a - b;
+      const uint32_t base = (uint32_t)__shfl((int)ex, (int)(lane & ~15u), kWave);
+      sr[2 * lane] = ex - base;
+      sr[2 * lane + 1] = ex - base + a;
+      if ((lane & 15u) == 15u) sr[4 * NC + (lane >> 4)] = incl - base;
+    }
+    sync_block_lds();
+#pragma unroll
+    for (int q = 0; q < 4; q++) tot[q] = U(sr[4 * NC + q]);
+  };
+  auto publish = [&](int sl, uint32_t tt, const uint32_t (&tot)[4]) {
+    const PairPlan &P = sPlan[sl];
+    if (wid < 2 && U(wid == 0 ? P.split[0] : P.split[1]) != 0)
+      lookback_publish(c.tileStatus + (size_t)wid * c.maxTiles, tt, tt - U(sTile[sl][2]), tag,
+                       wid == 0 ? tot[0] : tot[2]);
+  };
+  uint32_t pk, tot[4];
+  {
+    const uint32_t n = U(sPlan[0].end) - U(sPlan[0].begin);
+    const uint32_t j0 = (t - U(sTile[0][2])) * kTile;
+    load_tile(src, U(sPlan[0].pbeg), n, U(sPlan[0].rev), j0, min(n - j0, (uint32_t)kTile), c.n, it);
+    pk = classify(0, t);
+    count4(pk, tot);
+    publish(0, t, tot);
+  }
+  while (true) {
+    const PairPlan &P = sPlan[slot];
+    const uint32_t seg = U(sTile[slot][1]), tileFirst = U(sTile[slot][2]), tileEnd = U(sTile[slot][3]);
+    const uint32_t begin = U(P.begin), end = U(P.end), mid = U(P.mid), mode = U(P.mode);
+    const uint32_t split0 = U(P.split[0]), split1 = U(P.split[1]);
+    const uint32_t n = end - begin, h = n / 2;
+    const uint32_t k = t - tileFirst;
+    const uint32_t j0 = k * kTile;
+    const uint32_t ctot[4] = {tot[0], tot[1], tot[2], tot[3]};
+    const uint32_t off1 = ctot[0], off2 = off1 + ctot[1], off3 = off2 + ctot[2];
+    // (the previous tile's copy-out has read sStage)
+    sync_block_lds();
+    // rank (same-class items before each item: the wave's ballot prefix
+    // plus the wave's offset) and stage in output order
+#pragma unroll
+    for (int jj = 0; jj < kItems; jj++) {
+      const uint32_t cl = (pk >> (3 * jj)) & 7u;
+      uint32_t r = 0;
+#pragma unroll
+      for (uint32_t q = 0; q < 4; q++) {
+        const uint64_t m = __ballot(cl == q);
+        if (cl == q) r = mbcnt(m);
+      }
+      if (cl < 4) {
+        r += sr[(cl * kItems + jj) * NW + wid];
+        const uint32_t idx = cl == 0 ? r : (cl == 1 ? off2 - 1 - r : (cl == 2 ? off2 + r : off3 + ctot[3] - 1 - r));
         sStage[idx] = it[jj];
       }
     }
@@ -766,9 +820,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair(Ctx c, int pass) {
     // follow by difference, an unsplit child is one class.
     {
       const uint32_t q = 2 * wid;  // class (wid, left)
+      const bool look = wid < 2 && (wid == 0 ? split0 : split1) != 0;
       uint32_t pre = 0;
       if (look) {
-        const uint32_t mine = wid == 0 ? tot[0] : tot[2];
+        const uint32_t mine = wid == 0 ? ctot[0] : ctot[2];
         pre = lookback_exclusive_wave(c.tileStatus + (size_t)wid * c.maxTiles, t, k, tag, mine, true);
         if (lane == 0 && t == tileEnd - 1) c.classTot[4 * (size_t)seg + q] = pre + mine;
       }
@@ -786,13 +841,16 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair(Ctx c, int pass) {
     const uint32_t p3 = split1 ? c1pre - p2 : 0u;
     const int nslot = slot ^ 1;
     const uint32_t nt = U(sTile[nslot][0]);
-    // the next tile's loads go out before this tile's stores
+    // the next tile: loaded, classified, counted and published before this
+    // tile's copy-out (sr is free: this tile's ranks are done)
     if (nt < numTiles) {
       const PairPlan &Q = sPlan[nslot];
       const uint32_t n2 = U(Q.end) - U(Q.begin);
       const uint32_t j02 = (nt - U(sTile[nslot][2])) * kTile;
       load_tile(src, U(Q.pbeg), n2, U(Q.rev), j02, min(n2 - j02, (uint32_t)kTile), c.n, it);
-      asm volatile("" ::: "memory");
+      pk = classify(nslot, nt);
+      count4(pk, tot);
+      publish(nslot, nt, tot);
     }
     if (pendOn) {
       addB(pendSeg, pendCps, pend);
@@ -800,11 +858,11 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair(Ctx c, int pass) {
     }
     // copy out: staged class q -> its run; bounds of each class on the way
     const uint32_t runOff[4] = {0u, off1, off2, off3};
-    const uint32_t runBase[4] = {begin + p0, mid - p1 - tot[1], mid + p2, end - p3 - tot[3]};
+    const uint32_t runBase[4] = {begin + p0, mid - p1 - ctot[1], mid + p2, end - p3 - ctot[3]};
     const bool gb0 = gather_bounds(split0, mid - begin), gb1 = gather_bounds(split1, end - mid);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const uint32_t o = runOff[q], cntq = tot[q], rb = runBase[q];
+      const uint32_t o = runOff[q], cntq = ctot[q], rb = runBase[q];
       if (q < 2 ? gb0 : gb1) {
 #pragma unroll 4
         for (uint32_t i = threadIdx.x; i < cntq; i += kBlock) {

@@ -653,10 +653,11 @@ __device__ inline void block_rank4_striped(const uint32_t (&cls)[ITEMS], uint32_
 // of its segment (one wave per class) and copies the staged items out in
 // four contiguous runs.  Class (child, left) fills the child's range from
 // its start, class (child, right) from its end.  After its look-back the
-// workgroup claims its next tile, whose items are loaded before this tile's
-// copy-out.  A workgroup holds at most two tiles, the later one claimed
-// after the earlier; a tile publishes its class counts before it waits on
-// anything, so every wait ends and the grid drains.
+// workgroup claims its next tile, which is loaded, classified, counted and
+// published before this tile's copy-out.  A workgroup holds at most two
+// tiles, the later one claimed after the earlier; a tile publishes its class
+// counts before it waits on anything but its own loads, so every wait ends
+// and the grid drains.
 __global__ __launch_bounds__(kBlock, 4) void k_pair(Ctx c, int pass) {
   constexpr int NW = kBlock / kWave;
   static_assert(NW == 4, "one look-back wave per class");

@@ -47,6 +47,40 @@ __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *smem, uint32_t 
   return r;
 }
 
+// K exclusive block scans at once (one set of barriers): ex[q] = sum of
+// v[q] over lower threads, tot[q] = the block's sum.  smem needs K*(NW+1)
+// words.
+template <int BLOCK, int K, bool LDS_ONLY = false>
+__device__ inline void block_excl_scan_multi(const uint32_t (&v)[K], uint32_t (&ex)[K], uint32_t (&tot)[K],
+                                             uint32_t *smem) {
+  constexpr int NW = BLOCK / kWave;
+  const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
+  uint32_t incl[K];
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    incl[q] = wave_incl_scan(v[q]);
+    if (lane == kWave - 1) smem[q * (NW + 1) + wid] = incl[q];
+  }
+  block_sync<LDS_ONLY>();
+  if (threadIdx.x < (uint32_t)K) {
+    uint32_t *row = smem + threadIdx.x * (NW + 1);
+    uint32_t run = 0;
+    for (int w = 0; w < NW; w++) {
+      const uint32_t c = row[w];
+      row[w] = run;
+      run += c;
+    }
+    row[NW] = run;
+  }
+  block_sync<LDS_ONLY>();
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    ex[q] = smem[q * (NW + 1) + wid] + incl[q] - v[q];
+    tot[q] = smem[q * (NW + 1) + NW];
+  }
+  block_sync<LDS_ONLY>();
+}
+
 // Ranks of flagged items in a striped tile: item k of thread t has tile
 // index k*BLOCK + t.  rank[k] = number of flagged items with a smaller tile
 // index.  smem needs ITEMS*BLOCK/64 words.  Returns the number of flagged

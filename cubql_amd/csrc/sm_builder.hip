@@ -949,7 +949,7 @@ constexpr int kEmitCounts = 5;
 
 __global__ __launch_bounds__(kEmitBlock) void k_emit2(Ctx c, int pass) {
   constexpr int NW = kEmitBlock / kWave;
-  __shared__ uint32_t ss[NW + 1];
+  __shared__ uint32_t ss[kEmitCounts * (NW + 1)];
   __shared__ uint32_t sPre[kEmitCounts];
   __shared__ uint3 sFix[4 * kEmitBlock];  // (segment, first slot, size) of open grandchildren to reduce
   __shared__ uint2 sRev[2 * kEmitBlock];  // (first slot, size) of ranges to turn around
@@ -1024,8 +1024,7 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit2(Ctx c, int pass) {
   }
   if (threadIdx.x == 0) sNumFix = sNumRev = 0;
   uint32_t ex[kEmitCounts], tt[kEmitCounts];
-#pragma unroll
-  for (int q = 0; q < kEmitCounts; q++) ex[q] = block_excl_scan<kEmitBlock, true>(cnt[q], ss, &tt[q]);
+  block_excl_scan_multi<kEmitBlock, kEmitCounts, true>(cnt, ex, tt, ss);
   const uint32_t w = threadIdx.x / kWave;
   if (w < (uint32_t)kEmitCounts) {
     uint32_t v = tt[0];

@@ -445,6 +445,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds(Ctx c, int pass) {
   int32_t a[36];
   for (int i = 0; i < 36; i++) a[i] = acc_identity(i);
   uint32_t nl = 0;
+  bool anyR = false;  // a right-side prim seen since the last flush
   while (true) {
     const uint32_t h = d.n / 2;
     uint32_t wl = 0;  // the wave's left-side prims in this tile
@@ -461,6 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds(Ctx c, int pass) {
           nl++;
           CUBQL_ADD6(a, 0, ex, ey, ez);
         } else {
+          anyR = true;
           CUBQL_ADD6(a, 6, ex, ey, ez);
         }
         if (j < h) CUBQL_ADD6(a, 12, ex, ey, ez) else CUBQL_ADD6(a, 18, ex, ey, ez);
@@ -481,8 +483,18 @@ __global__ __launch_bounds__(kBlock) void k_bounds(Ctx c, int pass) {
       load_tile(src, d2.pbeg, d2.n, d2.rev, d2.j0, d2.cnt, c.n, it);
     }
     if (!more || d2.s != d.s) {
+      // a wave that saw prims on both sides of the plane has shown that the
+      // split is not one-sided: the position halves are not needed from it
+      const bool both = __ballot(nl != 0) != 0 && __ballot(anyR) != 0;
 #pragma unroll
-      for (int i = 0; i < 36; i++) a[i] = (i % 6) < 3 ? wave_min_i32(a[i]) : wave_max_i32(a[i]);
+      for (int i = 0; i < 12; i++) a[i] = (i % 6) < 3 ? wave_min_i32(a[i]) : wave_max_i32(a[i]);
+      if (both) {
+#pragma unroll
+        for (int i = 12; i < 36; i++) a[i] = acc_identity(i);
+      } else {
+#pragma unroll
+        for (int i = 12; i < 36; i++) a[i] = (i % 6) < 3 ? wave_min_i32(a[i]) : wave_max_i32(a[i]);
+      }
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) nl += __shfl_xor(nl, o, kWave);
       if (lane == 0) {
@@ -508,6 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds(Ctx c, int pass) {
       sync_block_lds();
       for (int i = 0; i < 36; i++) a[i] = acc_identity(i);
       nl = 0;
+      anyR = false;
     }
     // (a barrier has passed since the counts were written: the next tile's
     // or the flush's; the slot is rewritten two tiles later, past the next

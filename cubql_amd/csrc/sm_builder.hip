@@ -616,47 +616,6 @@ __global__ void k_init_acc(Ctx c) {
     a4[i] = acc_a_clear((uint32_t)(i % kGroups));
 }
 
-// Ranks of striped items in four classes (4 = none): rank[k] = number of
-// items of the same class with a smaller tile index; tot[q] = class sizes.
-// smem needs 4*ITEMS*BLOCK/64 + 4 words.
-template <int BLOCK, int ITEMS>
-__device__ inline void block_rank4_striped(const uint32_t (&cls)[ITEMS], uint32_t (&rank)[ITEMS],
-                                           uint32_t (&tot)[4], uint32_t *smem) {
-  constexpr int NW = BLOCK / kWave;
-  constexpr int NC = ITEMS * NW;  // counters per class
-  static_assert(4 * NC == 2 * kWave, "two counters per lane");
-  const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
-#pragma unroll
-  for (int k = 0; k < ITEMS; k++) {
-    uint32_t r = 0;
-#pragma unroll
-    for (uint32_t q = 0; q < 4; q++) {
-      const uint64_t m = __ballot(cls[k] == q);
-      if (lane == 0) smem[(q * ITEMS + k) * NW + wid] = (uint32_t)__popcll(m);
-      if (cls[k] == q) r = mbcnt(m);
-    }
-    rank[k] = r;
-  }
-  sync_block_lds();
-  if (wid == 0) {
-    // lane l scans counters 2l, 2l+1; class q owns lanes [16q, 16q+16)
-    const uint32_t a = smem[2 * lane], b = smem[2 * lane + 1];
-    const uint32_t incl = wave_incl_scan(a + b);
-    const uint32_t ex = incl - a - b;
-    const uint32_t base = (uint32_t)__shfl((int)ex, (int)(lane & ~15u), kWave);
-    smem[2 * lane] = ex - base;
-    smem[2 * lane + 1] = ex - base + a;
-    if ((lane & 15u) == 15u) smem[4 * NC + (lane >> 4)] = incl - base;
-  }
-  sync_block_lds();
-#pragma unroll
-  for (int k = 0; k < ITEMS; k++)
-    if (cls[k] < 4) rank[k] += smem[(cls[k] * ITEMS + k) * NW + wid];
-#pragma unroll
-  for (int q = 0; q < 4; q++) tot[q] = smem[4 * NC + q];
-  sync_block_lds();
-}
-
 // Pass B, software-pipelined over a workgroup's tiles like a one-level
 // partition: a workgroup claims tiles in ticket order (so a tile only ever
 // waits on tiles claimed by running workgroups), classifies its items into

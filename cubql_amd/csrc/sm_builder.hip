@@ -608,12 +608,27 @@ __global__ __launch_bounds__(256) void k_plan(Ctx c, int pass, uint32_t scanBloc
 }
 
 // Clears every segment slot's pass-A accumulators once per build.
-__global__ void k_init_acc(Ctx c) {
+// One launch before the build: the pass-A accumulators, the look-back
+// status words of the passes and emits (nStatus words from tileStatus,
+// nEmit from emitStatus) and the counters.
+__global__ void k_init_acc(Ctx c, size_t nStatus, size_t nEmit) {
   constexpr uint32_t kGroups = kCbCopies * kAccA / 4;
   int4 *a4 = reinterpret_cast<int4 *>(c.accA);
   const size_t total = (size_t)c.maxLarge * kGroups;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
-    a4[i] = acc_a_clear((uint32_t)(i % kGroups));
+  const size_t g0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x, gs = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = g0; i < total; i += gs) a4[i] = acc_a_clear((uint32_t)(i % kGroups));
+  uint32_t *ts = reinterpret_cast<uint32_t *>(c.tileStatus);
+  for (size_t i = g0; i < nStatus; i += gs) ts[i] = 0u;
+  uint32_t *es = reinterpret_cast<uint32_t *>(c.emitStatus);
+  for (size_t i = g0; i < nEmit; i += gs) es[i] = 0u;
+  if (g0 == 0) {
+    Counters *k = c.ctr;
+    for (int d = 0; d < 3; d++) {
+      k->rootCb[d] = enc_f(INFINITY);
+      k->rootCb[d + 3] = enc_f(-INFINITY);
+    }
+    k->allPoints = 1;
+  }
 }
 
 // Pass B, software-pipelined over a workgroup's tiles like a one-level
@@ -1786,16 +1801,6 @@ __global__ void k_refit_top(Ctx c, Node *nodes, int level) {
   }
 }
 
-__global__ void k_init_counters(Counters *k) {
-  if (threadIdx.x == 0) {
-    for (int d = 0; d < 3; d++) {
-      k->rootCb[d] = enc_f(INFINITY);
-      k->rootCb[d + 3] = enc_f(-INFINITY);
-    }
-    k->allPoints = 1;
-  }
-}
-
 }  // namespace sm
 
 // pinned host mirror for the few completion checks
@@ -1863,11 +1868,10 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
     return st;
   };
 
-  k_init_counters<<<1, 64, 0, s>>>(c.ctr);
-  int st = fill_u32((uint32_t *)c.tileStatus, 0u, 8 * (size_t)maxTiles, s);
-  if (!st) st = fill_u32((uint32_t *)c.emitStatus, 0u, 2 * (size_t)kEmitCounts * maxEmitBlocks, s);
-  if (st) return fail(st);
-  k_init_acc<<<std::max<uint32_t>(1u, std::min<uint32_t>(div_up(c.maxLarge * (kCbCopies * kAccA / 4), 256), 2048)), 256, 0, s>>>(c);
+  const size_t nStatus = 8 * (size_t)maxTiles, nEmit = 2 * (size_t)kEmitCounts * maxEmitBlocks;
+  const size_t initWork = std::max<size_t>({(size_t)c.maxLarge * (kCbCopies * kAccA / 4), nStatus, nEmit});
+  k_init_acc<<<std::max<uint32_t>(1u, std::min<uint32_t>(div_up(initWork, 256), 2048)), 256, 0, s>>>(c, nStatus,
+                                                                                                   nEmit);
   k_count<<<numBlocks, kBlock, 0, s>>>(c);
   k_count_scan<<<1, 1024, 0, s>>>(c, numBlocks);
   k_write_items<<<numBlocks, kBlock, 0, s>>>(c);
@@ -1983,7 +1987,7 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
     prof_flush();
   }
   Node *nodes = nullptr;
-  st = mem_alloc(mem, (void **)&nodes, std::max<size_t>(numNodes, 2) * sizeof(Node), s);
+  const int st = mem_alloc(mem, (void **)&nodes, std::max<size_t>(numNodes, 2) * sizeof(Node), s);
   if (st) return fail(st);
   if (numNodes) {
     k_copy_top<<<std::max<uint32_t>(1u, std::min<uint32_t>(div_up(topCount, 256), 1024)), 256, 0, s>>>(

@@ -67,7 +67,7 @@ _FREE_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p)
 
 
 class BatchParams(C.Structure):
-    """cubql_batch_params: leafSize, blockSize (64/128/256), stackDepth (32/64), maxCloudSize."""
+    """cubql_batch_params: leafSize, blockSize (64/128/256), stackDepth (8/16/32/64), maxCloudSize."""
     _fields_ = [("leafSize", C.c_int), ("blockSize", C.c_int), ("stackDepth", C.c_int),
                 ("maxCloudSize", C.c_uint32)]
 
@@ -170,7 +170,14 @@ def _ptr(t: Optional[torch.Tensor]):
 
 
 class TorchMemory:
-    """cubql_mem_resource backed by torch's caching allocator."""
+    """cubql_mem_resource backed by torch's caching allocator.
+
+    Each block is allocated with the library's stream as torch's current
+    stream, so the caching allocator ties it to that stream: once the
+    library frees it (at the end of a call, while its kernels may still be
+    queued), torch hands it out again only to allocations on the same
+    stream, which run after those kernels (stream-ordered reuse, like
+    hipMallocAsync/hipFreeAsync)."""
 
     def __init__(self, device=None):
         self.device = torch.device(device) if device is not None else torch.device(
@@ -179,7 +186,10 @@ class TorchMemory:
 
         def _malloc(pp, size, stream, user):
             try:
-                t = torch.empty(int(size), dtype=torch.uint8, device=self.device)
+                st = (torch.cuda.ExternalStream(stream, device=self.device) if stream
+                      else torch.cuda.default_stream(self.device))
+                with torch.cuda.stream(st):
+                    t = torch.empty(int(size), dtype=torch.uint8, device=self.device)
                 self.live[t.data_ptr()] = t
                 pp[0] = t.data_ptr()
                 return 0

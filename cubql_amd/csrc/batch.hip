@@ -427,17 +427,23 @@ void launch_query(const Side &q, const Side &t, int32_t *nn, float *d2, uint32_t
   k_bquery<BLOCK, STACK><<<div_up(std::max(q.n, 1u), BLOCK), BLOCK, 0, s>>>(q, t, nn, d2, ovf);
 }
 
+template <int STACK>
+void launch_query_b(int block, const Side &q, const Side &t, int32_t *nn, float *d2, uint32_t *ovf,
+                    hipStream_t s) {
+  if (block == 64) launch_query<64, STACK>(q, t, nn, d2, ovf, s);
+  else if (block == 128) launch_query<128, STACK>(q, t, nn, d2, ovf, s);
+  else launch_query<256, STACK>(q, t, nn, d2, ovf, s);
+}
+
 int query_dir(const Side &q, const Side &t, int32_t *nn, float *d2, uint32_t *ovf, const cubql_batch_params &p,
               hipStream_t s) {
   if (q.n == 0) return CUBQL_OK;
   CUBQL_CHECK(fill_u32(ovf, 0u, 2, s));
   const int b = p.blockSize, st = p.stackDepth;
-  if (b == 64 && st <= 32) launch_query<64, 32>(q, t, nn, d2, ovf, s);
-  else if (b == 64) launch_query<64, 64>(q, t, nn, d2, ovf, s);
-  else if (b == 128 && st <= 32) launch_query<128, 32>(q, t, nn, d2, ovf, s);
-  else if (b == 128) launch_query<128, 64>(q, t, nn, d2, ovf, s);
-  else if (st <= 32) launch_query<256, 32>(q, t, nn, d2, ovf, s);
-  else launch_query<256, 64>(q, t, nn, d2, ovf, s);
+  if (st == 8) launch_query_b<8>(b, q, t, nn, d2, ovf, s);
+  else if (st == 16) launch_query_b<16>(b, q, t, nn, d2, ovf, s);
+  else if (st == 32) launch_query_b<32>(b, q, t, nn, d2, ovf, s);
+  else launch_query_b<64>(b, q, t, nn, d2, ovf, s);
   k_bquery_deep<<<64, 64, 0, s>>>(q, t, nn, d2, ovf);
   CUBQL_LAUNCH_CHECK();
   return CUBQL_OK;
@@ -508,9 +514,9 @@ int check_args(const Args &a, size_t wsBytes) {
     return CUBQL_ERR_INVALID_ARG;
   }
   const int b = a.prm.blockSize;
-  if (!(b == 64 || b == 128 || b == 256) || !(a.prm.stackDepth == 32 || a.prm.stackDepth == 64) ||
+  if (!(b == 64 || b == 128 || b == 256) || !(a.prm.stackDepth == 8 || a.prm.stackDepth == 16 || a.prm.stackDepth == 32 || a.prm.stackDepth == 64) ||
       a.prm.leafSize < 1) {
-    set_error("cubql_batch_chamfer_f3: params must have blockSize 64/128/256, stackDepth 32/64, leafSize >= 1");
+    set_error("cubql_batch_chamfer_f3: params must have blockSize 64/128/256, stackDepth 8/16/32/64, leafSize >= 1");
     return CUBQL_ERR_INVALID_ARG;
   }
   return CUBQL_OK;

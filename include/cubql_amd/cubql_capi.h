@@ -235,7 +235,8 @@ int cubql_points_distance_backward_f3(const float *d_points,
 /* Tunables of the batched path (chosen by cubql_amd.autotune):
    leafSize      leaf threshold T of every per-cloud tree (>= 1);
    blockSize     query workgroup size: 64, 128 or 256;
-   stackDepth    traversal stack entries before the deep fallback: 32 or 64;
+   stackDepth    traversal stack entries before the deep fallback: 8, 16, 32
+                 or 64 (queries that need more rerun with a deeper stack);
    maxCloudSize  optional upper bound on any cloud's size (0 = unknown); it
                  only bounds the number of hierarchy launches. */
 typedef struct {

@@ -4,7 +4,15 @@
   plain per-cloud path (radix30Builder on the cloud + findClosest) bit for
   bit, in both directions;
 * loss and gradients match an FP64 evaluation (oracle point_grad_f64) within
-  the float32 tolerance of the adjoint (see tests/test_gpu_tris.py);
+  1e-5 of each point's own gradient size (rule in tests/gradtol.py);
+* the full uneven batch (4096 cloud pairs, sizes log-uniform in [64, 65536],
+  about 38M points per side, the benchmark's seeds): nearest ids and squared
+  distances of 64+ clouds (the 10 largest, the 10 smallest, random ones)
+  equal the oracle's, loss and gradients match FP64, the graph replay
+  equals the eager run and the tunables do not change the result;
+* the stack-overflow path: with an 8- or 16-entry fast stack many queries
+  rerun in the deep kernel (clouds of thousands of identical points among
+  them), and still equal the 64-entry run and brute force;
 * results do not depend on the tunables (leaf size, block size, stack
   depth);
 * a captured HIP graph replayed several times, with eager work between the
@@ -15,6 +23,7 @@
 import numpy as np
 import pytest
 import torch
+from gradtol import check_chamfer_grads
 
 pytestmark = pytest.mark.gpu
 EPS = 2.0 ** -24
@@ -66,24 +75,8 @@ def test_batch_equals_per_cloud_path(cq, oracle_mod):
     assert np.array_equal(B.src_d2.cpu().numpy(), pd_s)
     assert np.array_equal(B.tgt_nn.cpu().numpy(), pn_t)
     assert np.array_equal(B.tgt_d2.cpu().numpy(), pd_t)
-    # loss and gradients against FP64
-    S, Tg = src.cpu().numpy(), tgt.cpu().numpy()
-    so_h, to_h = so.cpu().numpy(), to.cpu().numpy()
-    cs = np.repeat(np.arange(48), np.diff(so_h))
-    ct = np.repeat(np.arange(48), np.diff(to_h))
-    gs = 1.0 / np.diff(so_h)[cs]
-    gt = 1.0 / np.diff(to_h)[ct]
-    d_s = np.linalg.norm(S.astype(np.float64) - Tg[pn_s].astype(np.float64), axis=1)
-    d_t = np.linalg.norm(Tg.astype(np.float64) - S[pn_t].astype(np.float64), axis=1)
-    loss64 = np.bincount(cs, gs * d_s, 48) + np.bincount(ct, gt * d_t, 48)
-    assert np.allclose(B.loss.cpu().numpy(), loss64, rtol=1e-5, atol=1e-6)
-    gq_s, gp_t = oracle_mod.point_grad_f64(Tg, S, pn_s, gs)
-    gq_t, gp_s = oracle_mod.point_grad_f64(S, Tg, pn_t, gt)
-    g64_src, g64_tgt = gq_s + gp_s, gq_t + gp_t
-    tol_s = 1e-5 * np.abs(g64_src) + 64 * EPS * (np.abs(g64_src).max() + 1e-3)
-    assert np.all(np.abs(B.grad_src.cpu().numpy() - g64_src) <= tol_s + 1e-7)
-    tol_t = 1e-5 * np.abs(g64_tgt) + 64 * EPS * (np.abs(g64_tgt).max() + 1e-3)
-    assert np.all(np.abs(B.grad_tgt.cpu().numpy() - g64_tgt) <= tol_t + 1e-7)
+    # loss and gradients against FP64, cloud by cloud (tests/gradtol.py)
+    _check_loss_and_grads(oracle_mod, B, src, so, tgt, to, range(48))
 
 
 @pytest.mark.parametrize("prm", [(2, 64, 32), (4, 128, 64), (16, 256, 32), (8, 256, 64)])
@@ -148,3 +141,228 @@ def test_batch_edge_cases(cq, oracle_mod):
     torch.cuda.synchronize()
     bi, _ = oracle_mod.fcp_brute(Th[:9], Sh[:5])
     assert np.array_equal(B1.src_nn.cpu().numpy(), bi)
+
+
+def _check_loss_and_grads(oracle_mod, B, src, so, tgt, to, clouds):
+    S, Tg = src.cpu().numpy(), tgt.cpu().numpy()
+    so_h, to_h = so.cpu().numpy().astype(np.int64), to.cpu().numpy().astype(np.int64)
+    nn_s, nn_t = B.src_nn.cpu().numpy(), B.tgt_nn.cpu().numpy()
+    loss = B.loss.cpu().numpy()
+    gs, gt = B.grad_src.cpu().numpy(), B.grad_tgt.cpu().numpy()
+    worst = 0.0
+    for c in clouds:
+        sb, se, tb, te = so_h[c], so_h[c + 1], to_h[c], to_h[c + 1]
+        if se == sb or te == tb:
+            continue
+        a, b = S[sb:se], Tg[tb:te]
+        ls, lt = nn_s[sb:se] - tb, nn_t[tb:te] - sb
+        d_s = np.linalg.norm(a.astype(np.float64) - b[ls].astype(np.float64), axis=1)
+        d_t = np.linalg.norm(b.astype(np.float64) - a[lt].astype(np.float64), axis=1)
+        l64 = d_s.mean() + d_t.mean()
+        assert abs(loss[c] - l64) <= 1e-5 * l64 + 1e-30, (c, loss[c], l64)
+        worst = max(worst, check_chamfer_grads(oracle_mod, a, b, ls, lt, gs[sb:se], gt[tb:te],
+                                               label=f"cloud {c}"))
+    print(f"[gpu] chamfer gradient rel err (of each point's sum|g|): max {worst:.2e}", flush=True)
+
+
+def _oracle_nn(oracle_mod, data, queries):
+    """Nearest data point per query, lowest index on ties: brute force for
+    small clouds, the oracle's tree query (same rule) for large ones."""
+    if len(data) * len(queries) <= 4e7:
+        return oracle_mod.fcp_brute(data, queries)
+    tree = oracle_mod.radix_build(oracle_mod.points_to_boxes(data), 10, 8)
+    return oracle_mod.fcp(tree, data, queries, lowest_id=True)
+
+
+def test_batch_c5_full_config(cq, oracle_mod):
+    """4096 uneven cloud pairs at the benchmark's seeds and sizes."""
+    print("[gpu] batch C5 full config", flush=True)
+    from cubql_amd import synth
+    ss = synth.chamfer_cloud_sizes(4096, 101)
+    ts = synth.chamfer_cloud_sizes(4096, 102)
+    src, so = synth.chamfer_clouds_torch(ss, 103, 104)
+    tgt, to = synth.chamfer_clouds_torch(ts, 105, 106)
+    maxc = int(max(ss.max(), ts.max()))
+    print(f"[gpu] C5: {len(src):,} source / {len(tgt):,} target points, largest cloud {maxc}",
+          flush=True)
+    B = cq.ChamferBatch(src, so, tgt, to, cq.BatchParams(8, 128, 32, maxc)).run()
+    G = cq.ChamferBatch(src, so, tgt, to, cq.BatchParams(4, 256, 64, maxc)).capture()
+    for _ in range(2):
+        G.replay()
+    torch.cuda.synchronize()
+    for a, b in [(B.src_nn, G.src_nn), (B.src_d2, G.src_d2), (B.tgt_nn, G.tgt_nn),
+                 (B.tgt_d2, G.tgt_d2)]:
+        assert torch.equal(a, b)
+    tot = ss + ts
+    order = np.argsort(tot, kind="stable")
+    rng = np.random.default_rng(7)
+    sel = set(order[-10:].tolist()) | set(order[:10].tolist())
+    sel |= set(int(x) for x in np.argsort(ss)[-3:]) | set(int(x) for x in np.argsort(ts)[-3:])
+    sel |= set(rng.choice(4096, 44, replace=False).tolist())
+    sel = sorted(sel)
+    assert len(sel) >= 64
+    S, Tg = src.cpu().numpy(), tgt.cpu().numpy()
+    so_h, to_h = so.cpu().numpy().astype(np.int64), to.cpu().numpy().astype(np.int64)
+    nn_s, d_s = B.src_nn.cpu().numpy(), B.src_d2.cpu().numpy()
+    nn_t, d_t = B.tgt_nn.cpu().numpy(), B.tgt_d2.cpu().numpy()
+    for c in sel:
+        a, b = S[so_h[c]:so_h[c + 1]], Tg[to_h[c]:to_h[c + 1]]
+        oi, od = _oracle_nn(oracle_mod, b, a)
+        assert np.array_equal(nn_s[so_h[c]:so_h[c + 1]], oi + to_h[c]), f"cloud {c} src->tgt ids"
+        assert np.array_equal(d_s[so_h[c]:so_h[c + 1]], od), f"cloud {c} src->tgt distances"
+        oi, od = _oracle_nn(oracle_mod, a, b)
+        assert np.array_equal(nn_t[to_h[c]:to_h[c + 1]], oi + so_h[c]), f"cloud {c} tgt->src ids"
+        assert np.array_equal(d_t[to_h[c]:to_h[c + 1]], od), f"cloud {c} tgt->src distances"
+    print(f"[gpu] C5: {len(sel)} clouds equal the oracle (largest {int(tot[order[-1]])} points)",
+          flush=True)
+    _check_loss_and_grads(oracle_mod, B, src, so, tgt, to, sel)
+
+
+@pytest.mark.parametrize("stack", [8, 16])
+def test_batch_stack_overflow_path(cq, oracle_mod, stack):
+    """A shallow fast-path stack (8 or 16 entries) makes many queries
+    overflow, so the deep rerun carries a large share of them; results must
+    equal the 64-entry run and brute force.  Clouds of many identical points
+    (both children of every node at the same distance, one far child stacked
+    per level) are included."""
+    print(f"[gpu] batch stack overflow path stack={stack}", flush=True)
+    src, so, tgt, to = _batch(24, 64, 4000, 41)
+    rng = np.random.default_rng(12)
+    S, T = [src.cpu().numpy()], [tgt.cpu().numpy()]
+    so_l, to_l = list(so.cpu().numpy()), list(to.cpu().numpy())
+    for n in [3000, 70, 5000]:
+        dup = np.tile(rng.random((1, 3), dtype=np.float32) * 10, (n, 1))
+        S.append(np.concatenate([dup, rng.random((5, 3), dtype=np.float32) * 10]))
+        T.append(np.concatenate([np.tile(dup[:1], (n // 2 + 1, 1)),
+                                 rng.random((n // 3 + 1, 3), dtype=np.float32) * 10]))
+        so_l.append(so_l[-1] + len(S[-1]))
+        to_l.append(to_l[-1] + len(T[-1]))
+    S, T = np.concatenate(S), np.concatenate(T)
+    so_h, to_h = np.asarray(so_l, np.int32), np.asarray(to_l, np.int32)
+    dS, dT = torch.from_numpy(S).cuda(), torch.from_numpy(T).cuda()
+    dso, dto = torch.from_numpy(so_h).cuda(), torch.from_numpy(to_h).cuda()
+    B = cq.ChamferBatch(dS, dso, dT, dto, cq.BatchParams(1, 128, stack)).run()
+    R = cq.ChamferBatch(dS, dso, dT, dto, cq.BatchParams(1, 128, 64)).run()
+    torch.cuda.synchronize()
+    for a, b in [(B.src_nn, R.src_nn), (B.src_d2, R.src_d2), (B.tgt_nn, R.tgt_nn),
+                 (B.tgt_d2, R.tgt_d2)]:
+        assert torch.equal(a, b)
+    nn_s, nn_t, d_s = B.src_nn.cpu().numpy(), B.tgt_nn.cpu().numpy(), B.src_d2.cpu().numpy()
+    for c in range(len(so_h) - 1):
+        a, b = S[so_h[c]:so_h[c + 1]], T[to_h[c]:to_h[c + 1]]
+        bi, bd = oracle_mod.fcp_brute(b, a)
+        assert np.array_equal(nn_s[so_h[c]:so_h[c + 1]], bi + to_h[c]), c
+        assert np.array_equal(d_s[so_h[c]:so_h[c + 1]], bd), c
+        bi, bd = oracle_mod.fcp_brute(a, b)
+        assert np.array_equal(nn_t[to_h[c]:to_h[c + 1]], bi + so_h[c]), c
+    _check_loss_and_grads(oracle_mod, B, dS, dso, dT, dto, range(len(so_h) - 1))
+
+
+def _check_loss_and_grads(oracle_mod, B, src, so, tgt, to, clouds):
+    S, Tg = src.cpu().numpy(), tgt.cpu().numpy()
+    so_h, to_h = so.cpu().numpy().astype(np.int64), to.cpu().numpy().astype(np.int64)
+    nn_s, nn_t = B.src_nn.cpu().numpy(), B.tgt_nn.cpu().numpy()
+    loss = B.loss.cpu().numpy()
+    gs, gt = B.grad_src.cpu().numpy(), B.grad_tgt.cpu().numpy()
+    worst = 0.0
+    for c in clouds:
+        sb, se, tb, te = so_h[c], so_h[c + 1], to_h[c], to_h[c + 1]
+        if se == sb or te == tb:
+            continue
+        a, b = S[sb:se], Tg[tb:te]
+        ls, lt = nn_s[sb:se] - tb, nn_t[tb:te] - sb
+        d_s = np.linalg.norm(a.astype(np.float64) - b[ls].astype(np.float64), axis=1)
+        d_t = np.linalg.norm(b.astype(np.float64) - a[lt].astype(np.float64), axis=1)
+        l64 = d_s.mean() + d_t.mean()
+        assert abs(loss[c] - l64) <= 1e-5 * l64 + 1e-30, (c, loss[c], l64)
+        worst = max(worst, check_chamfer_grads(oracle_mod, a, b, ls, lt, gs[sb:se], gt[tb:te],
+                                               label=f"cloud {c}"))
+    print(f"[gpu] chamfer gradient rel err (of each point's sum|g|): max {worst:.2e}", flush=True)
+
+
+def _oracle_nn(oracle_mod, data, queries):
+    """Nearest data point per query, lowest index on ties: brute force for
+    small clouds, the oracle's tree query (same rule) for large ones."""
+    if len(data) * len(queries) <= 4e7:
+        return oracle_mod.fcp_brute(data, queries)
+    tree = oracle_mod.radix_build(oracle_mod.points_to_boxes(data), 10, 8)
+    return oracle_mod.fcp(tree, data, queries, lowest_id=True)
+
+
+def test_batch_c5_full_config(cq, oracle_mod):
+    """4096 uneven cloud pairs at the benchmark's seeds and sizes."""
+    print("[gpu] batch C5 full config", flush=True)
+    from cubql_amd import synth
+    ss = synth.chamfer_cloud_sizes(4096, 101)
+    ts = synth.chamfer_cloud_sizes(4096, 102)
+    src, so = synth.chamfer_clouds_torch(ss, 103, 104)
+    tgt, to = synth.chamfer_clouds_torch(ts, 105, 106)
+    maxc = int(max(ss.max(), ts.max()))
+    print(f"[gpu] C5: {len(src):,} source / {len(tgt):,} target points, largest cloud {maxc}",
+          flush=True)
+    B = cq.ChamferBatch(src, so, tgt, to, cq.BatchParams(8, 128, 32, maxc)).run()
+    G = cq.ChamferBatch(src, so, tgt, to, cq.BatchParams(4, 256, 64, maxc)).capture()
+    for _ in range(2):
+        G.replay()
+    torch.cuda.synchronize()
+    for a, b in [(B.src_nn, G.src_nn), (B.src_d2, G.src_d2), (B.tgt_nn, G.tgt_nn),
+                 (B.tgt_d2, G.tgt_d2)]:
+        assert torch.equal(a, b)
+    tot = ss + ts
+    order = np.argsort(tot, kind="stable")
+    rng = np.random.default_rng(7)
+    sel = set(order[-10:].tolist()) | set(order[:10].tolist())
+    sel |= set(int(x) for x in np.argsort(ss)[-3:]) | set(int(x) for x in np.argsort(ts)[-3:])
+    sel |= set(rng.choice(4096, 44, replace=False).tolist())
+    sel = sorted(sel)
+    assert len(sel) >= 64
+    S, Tg = src.cpu().numpy(), tgt.cpu().numpy()
+    so_h, to_h = so.cpu().numpy().astype(np.int64), to.cpu().numpy().astype(np.int64)
+    nn_s, d_s = B.src_nn.cpu().numpy(), B.src_d2.cpu().numpy()
+    nn_t, d_t = B.tgt_nn.cpu().numpy(), B.tgt_d2.cpu().numpy()
+    for c in sel:
+        a, b = S[so_h[c]:so_h[c + 1]], Tg[to_h[c]:to_h[c + 1]]
+        oi, od = _oracle_nn(oracle_mod, b, a)
+        assert np.array_equal(nn_s[so_h[c]:so_h[c + 1]], oi + to_h[c]), f"cloud {c} src->tgt ids"
+        assert np.array_equal(d_s[so_h[c]:so_h[c + 1]], od), f"cloud {c} src->tgt distances"
+        oi, od = _oracle_nn(oracle_mod, a, b)
+        assert np.array_equal(nn_t[to_h[c]:to_h[c + 1]], oi + so_h[c]), f"cloud {c} tgt->src ids"
+        assert np.array_equal(d_t[to_h[c]:to_h[c + 1]], od), f"cloud {c} tgt->src distances"
+    print(f"[gpu] C5: {len(sel)} clouds equal the oracle (largest {int(tot[order[-1]])} points)",
+          flush=True)
+    _check_loss_and_grads(oracle_mod, B, src, so, tgt, to, sel)
+
+
+@pytest.mark.parametrize("stack", [32, 64])
+def test_batch_stack_overflow_clouds(cq, oracle_mod, stack):
+    """Clouds of many identical points: every node on a query's path has
+    both children at the same distance, so the traversal stacks one far
+    child per level and overflows any stack shallower than the tree (30
+    Morton levels are equal, then the equal-key middle splits); those
+    queries rerun in the deep kernel and must still equal brute force."""
+    print(f"[gpu] batch stack overflow clouds stack={stack}", flush=True)
+    rng = np.random.default_rng(12)
+    clouds_s, clouds_t = [], []
+    for c, n in enumerate([3000, 70, 5000, 1200]):
+        dup = np.tile(rng.random((1, 3), dtype=np.float32) * 10, (n, 1))
+        clouds_s.append(np.concatenate([dup, rng.random((5, 3), dtype=np.float32) * 10]))
+        clouds_t.append(np.concatenate([np.tile(dup[:1], (n // 2 + 1, 1)),
+                                        rng.random((n // 3 + 1, 3), dtype=np.float32) * 10]))
+    S = np.concatenate(clouds_s)
+    T = np.concatenate(clouds_t)
+    so = np.concatenate([[0], np.cumsum([len(x) for x in clouds_s])]).astype(np.int32)
+    to = np.concatenate([[0], np.cumsum([len(x) for x in clouds_t])]).astype(np.int32)
+    dS, dT = torch.from_numpy(S).cuda(), torch.from_numpy(T).cuda()
+    B = cq.ChamferBatch(dS, torch.from_numpy(so).cuda(), dT, torch.from_numpy(to).cuda(),
+                        cq.BatchParams(1, 128, stack)).run()
+    torch.cuda.synchronize()
+    nn_s, nn_t = B.src_nn.cpu().numpy(), B.tgt_nn.cpu().numpy()
+    for c in range(len(clouds_s)):
+        a, b = S[so[c]:so[c + 1]], T[to[c]:to[c + 1]]
+        bi, bd = oracle_mod.fcp_brute(b, a)
+        assert np.array_equal(nn_s[so[c]:so[c + 1]], bi + to[c]), c
+        assert np.array_equal(B.src_d2.cpu().numpy()[so[c]:so[c + 1]], bd), c
+        bi, bd = oracle_mod.fcp_brute(a, b)
+        assert np.array_equal(nn_t[to[c]:to[c + 1]], bi + so[c]), c
+    _check_loss_and_grads(oracle_mod, B, dS, torch.from_numpy(so), dT, torch.from_numpy(to),
+                          range(len(clouds_s)))

@@ -13,11 +13,12 @@ Cases: uniform points (no rebinning), tight clusters beside far outliers
 32-prim thread-phase limit, invalid boxes, and the C4 workload: 2,000,000
 points of a 64-component Gaussian mixture with 1% outliers at 1e6, kNN k=16
 on a sample against the oracle, and the mean-kNN-distance backward against
-the FP64 closed form.
+the FP64 closed form (rule in tests/gradtol.py).
 """
 import numpy as np
 import pytest
 import torch
+from gradtol import check_point_grads
 
 pytestmark = pytest.mark.gpu
 F32 = np.float32
@@ -110,10 +111,37 @@ def test_gpu_rebin_c4_workload(cq, oracle_mod):
     ids_h = ids.cpu().numpy()
     assert np.array_equal(ids_h[sel], oi)
     assert np.array_equal(d2.cpu().numpy()[sel], od)
-    g64q, _ = oracle_mod.point_grad_f64(P, Q[sel], oi, np.full((len(sel), 16), 1.0 / (len(Q) * 16)))
-    d = np.sqrt(od.astype(np.float64))
-    scale = np.abs(Q[sel]).max(axis=1, keepdims=True).astype(np.float64)
-    # per neighbour: relative 1e-5 plus the float32 rounding of q - p
-    tol = (1e-5 + 16 * EPS * scale / np.maximum(d, 1e-30)).sum(1, keepdims=True) / (len(Q) * 16) + 1e-12
-    assert np.all(np.abs(gq.cpu().numpy()[sel] - g64q) <= tol)
+    # the adjoint of every query and every data point against the FP64
+    # closed form at the kernel's ids (tolerance rule in tests/gradtol.py)
+    check_point_grads(oracle_mod, P, Q, ids_h, np.full((len(Q), 16), 1.0 / (len(Q) * 16)),
+                      gq.cpu().numpy(), gp.cpu().numpy(), label="C4 kNN16 mean")
     bvh.free()
+
+
+@pytest.mark.parametrize("n,T", [(2000, 1), (2000, 8), (33, 1), (32, 1), (20, 2)])
+def test_gpu_rebin_cluster_reduces_to_radix30(cq, oracle_mod, n, T):
+    """An isolated cluster (one global Morton cell, distinct keys in its own
+    centroid domain) is re-keyed in that domain (broad phase above 32 prims,
+    thread phase at or below): its subtree in the GPU rebin tree equals the
+    GPU radix30 tree built over the cluster alone, prim order included."""
+    print(f"[gpu] rebin cluster reduction n={n} T={T}", flush=True)
+    from test_oracle_rebin import (check_cluster_reduction, find_subtree, isolated_cluster,
+                                   subtree_prim_order, subtree_tuple)
+    P, cl = isolated_cluster(n, 100 + n + T)
+
+    def gpu_tree(boxes, t, fn):
+        bvh = fn(torch.from_numpy(boxes).cuda(), cq.BuildConfig(makeLeafThreshold=t))
+        torch.cuda.synchronize()
+        tree = oracle_mod.Tree(cq.nodes_to_numpy(bvh).view(oracle_mod.NODE_DTYPE),
+                               bvh.prim_ids_tensor().cpu().numpy().view(np.uint32))
+        bvh.free()
+        return tree
+
+    tree = check_cluster_reduction(oracle_mod, lambda b, t: gpu_tree(b, t, cq.cuda.rebinRadixBuilder),
+                                   P, cl, T)
+    alone = gpu_tree(oracle_mod.points_to_boxes(P[cl]), T, cq.cuda.radix30Builder)
+    node = find_subtree(tree, cl)
+    remap = {int(g): j for j, g in enumerate(cl)}
+    ident = {j: j for j in range(len(cl))}
+    assert subtree_prim_order(tree, node, remap) == alone.prim_ids.tolist()
+    assert subtree_tuple(tree, node, remap) == subtree_tuple(alone, 0, ident)

@@ -202,3 +202,102 @@ def test_rebin_gaussian_mixture_queries(oracle_mod):
     ids, d2, _ = oracle_mod.knn(tree, P, Q, 16)
     bi, bd = oracle_mod.knn_brute(P, Q, 16)
     assert np.array_equal(ids, bi) and np.array_equal(d2, bd)
+
+
+# ------------------------------------------------------- reduction to radix30
+def isolated_cluster(n, seed, centre=(5.0, 5.0, 5.0), extent=1e-3):
+    """n points uniform in a cube of side `extent` around `centre`, far
+    outliers at +-1e6 (so the global 30-bit quantizer puts the whole cluster
+    in one cell) and the permutation that interleaves them in the input.
+    Returns (points, cluster_ids): cluster_ids are the cluster's primIDs in
+    ascending order."""
+    rng = np.random.default_rng(seed)
+    c = np.asarray(centre) + extent * (rng.random((n, 3)) - 0.5)
+    far = np.array([[1e6, 1e6, 1e6], [-1e6, -1e6, -1e6], [1e6, -1e6, 3e5],
+                    [-7e5, 2e5, 1e6]])
+    P = np.concatenate([c, far]).astype(F32)
+    perm = rng.permutation(len(P))
+    P = P[perm]
+    cluster_ids = np.sort(np.nonzero(perm < n)[0]).astype(np.uint32)
+    return P, cluster_ids
+
+
+def find_subtree(tree, prim_set):
+    """Index of the node whose prims are exactly `prim_set`, or -1."""
+    want = frozenset(int(x) for x in prim_set)
+
+    def rec(i):
+        cnt, off = int(tree.counts[i]), int(tree.offsets[i])
+        if cnt:
+            s = frozenset(tree.prim_ids[off:off + cnt].tolist())
+            return s, (i if s == want else -1)
+        a, fa = rec(off)
+        b, fb = rec(off + 1)
+        s = a | b
+        return s, (i if s == want else max(fa, fb))
+    return rec(0)[1]
+
+
+def subtree_tuple(tree, i, remap):
+    cnt, off = int(tree.counts[i]), int(tree.offsets[i])
+    if cnt:
+        return tuple(sorted(remap[int(p)] for p in tree.prim_ids[off:off + cnt]))
+    return ("node", subtree_tuple(tree, off, remap), subtree_tuple(tree, off + 1, remap))
+
+
+def subtree_prim_order(tree, i, remap):
+    """primIDs of the subtree as stored (a Morton subtree covers one
+    contiguous primID range), mapped through `remap`."""
+    def span(j):
+        cnt, off = int(tree.counts[j]), int(tree.offsets[j])
+        if cnt:
+            return off, off + cnt
+        a, b = span(off), span(off + 1)
+        return min(a[0], b[0]), max(a[1], b[1])
+    b, e = span(i)
+    return [remap[int(p)] for p in tree.prim_ids[b:e]]
+
+
+def check_cluster_reduction(oracle_mod, build, P, cluster_ids, T):
+    """The rebin tree's cluster subtree equals radix30 over the cluster alone
+    (rebinMortonBuilder.h:1305-1405 broad phase, :772-886 thread phase: the
+    range is re-keyed with the quantizer of its own centroid bounds, the same
+    formula radix.h:41-70 applies to a whole input, then re-sorted; with
+    distinct keys in that domain no further rebin happens and every sort
+    order is the key order)."""
+    B = oracle_mod.points_to_boxes(P)
+    C = P[cluster_ids]
+    CB = oracle_mod.points_to_boxes(C)
+    # preconditions: one global cell holds exactly the cluster, and the
+    # cluster's keys in its own domain are distinct
+    gk, gi = oracle_mod.morton_keys(B, 10)
+    ck = gk[np.isin(gi, cluster_ids)]
+    assert len(np.unique(ck)) == 1 and np.sum(gk == ck[0]) == len(cluster_ids)
+    lk, _ = oracle_mod.morton_keys(CB, 10)
+    assert len(np.unique(lk)) == len(cluster_ids)
+    tree = build(B, T)
+    node = find_subtree(tree, cluster_ids)
+    assert node >= 0, "no node holds exactly the cluster"
+    remap = {int(g): j for j, g in enumerate(cluster_ids)}
+    alone = oracle_mod.radix_build(CB, 10, T)
+    ident = {j: j for j in range(len(cluster_ids))}
+    assert subtree_prim_order(tree, node, remap) == alone.prim_ids.tolist()
+    assert subtree_tuple(tree, node, remap) == subtree_tuple(alone, 0, ident)
+    return tree
+
+
+@pytest.mark.parametrize("n,T,seed", [(2000, 1, 0), (2000, 8, 1), (600, 4, 2),   # broad phase
+                                      (33, 1, 3), (32, 1, 4), (20, 2, 5), (9, 1, 6)])  # thread phase
+def test_rebin_cluster_reduces_to_radix30(oracle_mod, n, T, seed):
+    P, cl = isolated_cluster(n, seed)
+    check_cluster_reduction(oracle_mod, oracle_mod.rebin_build, P, cl, T)
+
+
+def test_rebin_cluster_reduction_detects_parent_domain(oracle_mod):
+    """A restatement that re-keyed the cluster in the quantizer of the whole
+    input (instead of the cluster's own centroid bounds) would leave every key
+    equal; the reduction check must then fail.  Here: the radix30 tree of
+    the whole input, whose cluster range is split in the middle, fails it."""
+    P, cl = isolated_cluster(300, 7)
+    with pytest.raises(AssertionError):
+        check_cluster_reduction(oracle_mod, lambda b, t: oracle_mod.radix_build(b, 10, t), P, cl, 1)

@@ -150,3 +150,23 @@ def test_icosphere_counts(s):
     e = np.sort(np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]]), axis=1)
     _, cnt = np.unique(e, axis=0, return_counts=True)
     assert np.all(cnt == 2)
+
+
+def test_autograd_binding_triangle_distance_f64(oracle_mod):
+    """The float64 closest-point routine of the test-only autograd binding
+    (tests/autograd_binding.py) against the oracle's FP64 region method, on
+    queries in every Voronoi region (vertex, edge, face) of random
+    triangles."""
+    import torch
+    from autograd_binding import closest_point_on_triangle_f64
+    rng = np.random.default_rng(17)
+    n = 600
+    V = rng.standard_normal((3 * n, 3)).astype(np.float32)
+    F = np.arange(3 * n, dtype=np.int32).reshape(n, 3)
+    Q = (V[F].mean(1) + 2.0 * rng.standard_normal((n, 3))).astype(np.float32)
+    t = torch.from_numpy(V).double()
+    P = closest_point_on_triangle_f64(torch.from_numpy(Q).double(), t[F[:, 0]], t[F[:, 1]], t[F[:, 2]])
+    d = (torch.from_numpy(Q).double() - P).norm(dim=-1).numpy()
+    for i in range(n):
+        _, d2 = oracle_mod.fcp_tris_brute_f64(F[[i]] - 3 * i, V[3 * i:3 * i + 3], Q[[i]])
+        assert abs(d[i] - np.sqrt(d2[0])) <= 1e-12 * max(1.0, d[i]), i

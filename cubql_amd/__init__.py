@@ -48,6 +48,12 @@ class BVH3f(C.Structure):
                 ("primIDs", C.c_void_p), ("numPrims", C.c_uint32)]
 
 
+class WBVH3f(C.Structure):
+    """Byte-identical to WideBVH<float,3,W> (bvh.h:113-141)."""
+    _fields_ = [("nodes", C.c_void_p), ("numNodes", C.c_uint32),
+                ("primIDs", C.c_void_p), ("numPrims", C.c_uint32)]
+
+
 class BuildConfig(C.Structure):
     """Byte-identical to cuBQL::BuildConfig (bvh.h:31-64)."""
     _fields_ = [("maxAllowedLeafSize", C.c_int), ("makeLeafThreshold", C.c_int),
@@ -109,6 +115,13 @@ def lib():
         "cubql_refit_f3": (I, [P(BVH3f), V, V, P(MemResource)]),
         "cubql_fcp_points_f3": (I, [P(BVH3f), V, V, U32, F, V, V, V]),
         "cubql_knn_points_f3": (I, [P(BVH3f), V, V, U32, I, F, V, V, V, V]),
+        "cubql_gpu_builder_wide_f3": (I, [P(WBVH3f), I, V, U32, P(BuildConfig), V, P(MemResource)]),
+        "cubql_collapse_wide_f3": (I, [P(BVH3f), I, P(WBVH3f), V, P(MemResource)]),
+        "cubql_free_wide_f3": (I, [P(WBVH3f), V, P(MemResource)]),
+        "cubql_fcp_points_wide_f3": (I, [P(WBVH3f), I, V, V, U32, F, V, V, V]),
+        "cubql_box_query_f3": (I, [P(BVH3f), V, V, U32, U32, V, V, V]),
+        "cubql_box_query_wide_f3": (I, [P(WBVH3f), I, V, V, U32, U32, V, V, V]),
+        "cubql_knn_points_wide_f3": (I, [P(WBVH3f), I, V, V, U32, I, F, V, V, V, V]),
         "cubql_radix_sort_pairs_u32": (I, [V, V, V, V, U32, I, V]),
         "cubql_radix_sort_pairs_u64": (I, [V, V, V, V, U32, I, V]),
         "cubql_triangle_boxes_f3": (I, [V, V, U32, V, V]),
@@ -250,6 +263,75 @@ class BinaryBVH:
             pass
 
 
+class WideBVH:
+    """A WideBVH<float,3,W> (W = 4 or 8) in torch-managed device memory."""
+
+    def __init__(self, mem: TorchMemory, width: int):
+        if width not in (4, 8):
+            raise CubqlError("WideBVH width must be 4 or 8")
+        self.mem = mem
+        self.width = width
+        self.c = WBVH3f()
+
+    @property
+    def numNodes(self) -> int:
+        return int(self.c.numNodes)
+
+    @property
+    def numPrims(self) -> int:
+        return int(self.c.numPrims)
+
+    def nodes_tensor(self) -> torch.Tensor:
+        """[numNodes, W, 8] float32 view of the children (lo xyz, hi xyz,
+        valid:1 | offset:45 | count:16 word as 2 x u32)."""
+        if not self.c.nodes or self.numNodes == 0:
+            return torch.empty((0, self.width, 8), dtype=torch.float32, device=self.mem.device)
+        t = self.mem.tensor(self.c.nodes)
+        n = self.numNodes * self.width
+        return t[: n * 32].view(torch.float32).view(self.numNodes, self.width, 8)
+
+    def prim_ids_tensor(self) -> torch.Tensor:
+        if not self.c.primIDs:
+            return torch.empty((0,), dtype=torch.int32, device=self.mem.device)
+        return self.mem.tensor(self.c.primIDs)[: self.numPrims * 4].view(torch.int32)
+
+    def free(self, stream=None):
+        if self.c.nodes or self.c.primIDs:
+            _check(lib().cubql_free_wide_f3(C.byref(self.c), C.c_void_p(_stream_handle(stream)),
+                                            C.byref(self.mem.resource)), "cuda::free(WideBVH)")
+
+    def __del__(self):
+        try:
+            if torch is not None and (self.c.nodes or self.c.primIDs):
+                self.mem.live.pop(self.c.nodes, None)
+                self.mem.live.pop(self.c.primIDs, None)
+        except Exception:  # noqa: BLE001
+            pass
+
+
+def wideGpuBuilder(boxes: torch.Tensor, width: int = 4, config: Optional[BuildConfig] = None,
+                   stream=None, mem: Optional[TorchMemory] = None) -> WideBVH:
+    """cuBQL::gpuBuilder(WideBVH<float,3,W>&, ...) (builder/cuda.h:107-115):
+    binary build as gpuBuilder, collapsed to width 4 or 8."""
+    boxes = _as_boxes(boxes)
+    config = config or BuildConfig()
+    mem = mem or TorchMemory(boxes.device)
+    bvh = WideBVH(mem, width)
+    _check(lib().cubql_gpu_builder_wide_f3(C.byref(bvh.c), int(width), _ptr(boxes), boxes.shape[0],
+                                           C.byref(config), C.c_void_p(_stream_handle(stream)),
+                                           C.byref(mem.resource)), "gpuBuilder(WideBVH)")
+    return bvh
+
+
+def collapse_wide(bvh: "BinaryBVH", width: int = 4, stream=None) -> WideBVH:
+    """WideBVH of width 4 or 8 collapsed from any BinaryBVH (its own primIDs copy)."""
+    w = WideBVH(bvh.mem, width)
+    _check(lib().cubql_collapse_wide_f3(C.byref(bvh.c), int(width), C.byref(w.c),
+                                        C.c_void_p(_stream_handle(stream)), C.byref(bvh.mem.resource)),
+           "collapse_wide")
+    return w
+
+
 def _as_boxes(boxes: torch.Tensor) -> torch.Tensor:
     if boxes.dtype != torch.float32 or not boxes.is_cuda:
         raise CubqlError("boxes must be a float32 CUDA tensor of shape [n,6] (lower xyz, upper xyz)")
@@ -324,9 +406,14 @@ class points:  # noqa: N801 -- mirrors cuBQL::points
         nq = queries.shape[0]
         ids = torch.empty(nq, dtype=torch.int32, device=queries.device)
         d2 = torch.empty(nq, dtype=torch.float32, device=queries.device)
-        _check(lib().cubql_fcp_points_f3(C.byref(bvh.c), _ptr(data), _ptr(queries), nq,
-                                         float(sqrMaxSearchDist), _ptr(ids), _ptr(d2),
-                                         C.c_void_p(_stream_handle(stream))), "findClosest")
+        st = C.c_void_p(_stream_handle(stream))
+        if isinstance(bvh, WideBVH):
+            _check(lib().cubql_fcp_points_wide_f3(C.byref(bvh.c), bvh.width, _ptr(data), _ptr(queries), nq,
+                                                  float(sqrMaxSearchDist), _ptr(ids), _ptr(d2), st),
+                   "findClosest(WideBVH)")
+        else:
+            _check(lib().cubql_fcp_points_f3(C.byref(bvh.c), _ptr(data), _ptr(queries), nq,
+                                             float(sqrMaxSearchDist), _ptr(ids), _ptr(d2), st), "findClosest")
         return ids, d2
 
     @staticmethod
@@ -339,10 +426,39 @@ class points:  # noqa: N801 -- mirrors cuBQL::points
         ids = torch.empty((nq, k), dtype=torch.int32, device=queries.device)
         d2 = torch.empty((nq, k), dtype=torch.float32, device=queries.device)
         nf = torch.empty(nq, dtype=torch.int32, device=queries.device)
-        _check(lib().cubql_knn_points_f3(C.byref(bvh.c), _ptr(data), _ptr(queries), nq, int(k),
-                                         float(sqrMaxSearchDist), _ptr(ids), _ptr(d2), _ptr(nf),
-                                         C.c_void_p(_stream_handle(stream))), "findKNN")
+        st = C.c_void_p(_stream_handle(stream))
+        if isinstance(bvh, WideBVH):
+            _check(lib().cubql_knn_points_wide_f3(C.byref(bvh.c), bvh.width, _ptr(data), _ptr(queries), nq,
+                                                  int(k), float(sqrMaxSearchDist), _ptr(ids), _ptr(d2),
+                                                  _ptr(nf), st), "findKNN(WideBVH)")
+        else:
+            _check(lib().cubql_knn_points_f3(C.byref(bvh.c), _ptr(data), _ptr(queries), nq, int(k),
+                                             float(sqrMaxSearchDist), _ptr(ids), _ptr(d2), _ptr(nf), st),
+                   "findKNN")
         return ids, d2, nf
+
+
+class fixedBoxQuery:  # noqa: N801 -- mirrors cuBQL::fixedBoxQuery
+    @staticmethod
+    def forEachPrim(bvh, prim_boxes: torch.Tensor, query_boxes: torch.Tensor, max_per_query: int = 0,
+                    stream=None):
+        """Batched fixed-box query: (ids [nq, max_per_query] int32 in
+        traversal order, counts [nq] int32) of the prims whose boxes overlap
+        each query box; max_per_query = 0 returns the counts only."""
+        pb, qb = _as_boxes(prim_boxes), _as_boxes(query_boxes)
+        nq = qb.shape[0]
+        counts = torch.empty(nq, dtype=torch.int32, device=qb.device)
+        ids = (torch.full((nq, max_per_query), -1, dtype=torch.int32, device=qb.device)
+               if max_per_query > 0 else None)
+        st = C.c_void_p(_stream_handle(stream))
+        if isinstance(bvh, WideBVH):
+            _check(lib().cubql_box_query_wide_f3(C.byref(bvh.c), bvh.width, _ptr(pb), _ptr(qb), nq,
+                                                 int(max_per_query), _ptr(ids), _ptr(counts), st),
+                   "fixedBoxQuery(WideBVH)")
+        else:
+            _check(lib().cubql_box_query_f3(C.byref(bvh.c), _ptr(pb), _ptr(qb), nq, int(max_per_query),
+                                            _ptr(ids), _ptr(counts), st), "fixedBoxQuery")
+        return ids, counts
 
 
 class triangles:  # noqa: N801 -- mirrors cuBQL::triangles
@@ -505,6 +621,15 @@ def points_to_boxes(p: torch.Tensor) -> torch.Tensor:
     """box3f().including(p) per point (samples/s01_closestPoint_points_gpu/closestPoint.cu:16-22)."""
     p = p.contiguous().view(-1, 3)
     return torch.cat([p, p], dim=1).contiguous()
+
+
+def wide_nodes_to_numpy(bvh: WideBVH):
+    """Host copy of wide children as [numNodes, W] structured records (lo[3],
+    hi[3], valid:1 | offset:45 | count:16 word)."""
+    import numpy as np
+    dt = np.dtype([("lo", "<f4", 3), ("hi", "<f4", 3), ("oc", "<u8")])
+    raw = bvh.nodes_tensor().contiguous().cpu().numpy()
+    return raw.view(dt).reshape(bvh.numNodes, bvh.width)
 
 
 def nodes_to_numpy(bvh: BinaryBVH):

@@ -28,8 +28,8 @@ struct Overflow {
   uint32_t deepFailed;
 };
 
-template <int STACK>
-__device__ __forceinline__ bool fcp_one(const cubql_bvh3f &bvh, const float *points, float3 q,
+template <int STACK, typename BVH>
+__device__ __forceinline__ bool fcp_one(const BVH &bvh, const float *points, float3 q,
                                         float sqrMax, int32_t &best, float &bestD) {
   best = -1;
   bestD = sqrMax;
@@ -49,7 +49,9 @@ __device__ __forceinline__ bool fcp_one(const cubql_bvh3f &bvh, const float *poi
 }
 
 // fast path: 64-entry stack; queries that overflow go to the deep kernel.
-__global__ __launch_bounds__(kBlock) void k_fcp(cubql_bvh3f bvh, const float *points,
+// BVH: cubql_bvh3f (binary) or cubql_dev::WideBVH<W>.
+template <typename BVH>
+__global__ __launch_bounds__(kBlock) void k_fcp(BVH bvh, const float *points,
                                                 const float *queries, uint32_t nq, float sqrMax,
                                                 int32_t *outIds, float *outD, Overflow *ovf,
                                                 uint32_t *ovfList, const uint32_t *order) {
@@ -68,7 +70,8 @@ __global__ __launch_bounds__(kBlock) void k_fcp(cubql_bvh3f bvh, const float *po
   if (outD) outD[qi] = best >= 0 ? bestD : INFINITY;
 }
 
-__global__ __launch_bounds__(64) void k_fcp_deep(cubql_bvh3f bvh, const float *points,
+template <typename BVH>
+__global__ __launch_bounds__(64) void k_fcp_deep(BVH bvh, const float *points,
                                                  const float *queries, float sqrMax,
                                                  int32_t *outIds, float *outD, Overflow *ovf,
                                                  const uint32_t *ovfList) {
@@ -88,8 +91,8 @@ __global__ __launch_bounds__(64) void k_fcp_deep(cubql_bvh3f bvh, const float *p
 }
 
 // kNN with a register-resident candidate list of K entries (K >= k).
-template <int K, int STACK>
-__device__ __forceinline__ bool knn_one(const cubql_bvh3f &bvh, const float *points, float3 q,
+template <int K, int STACK, typename BVH>
+__device__ __forceinline__ bool knn_one(const BVH &bvh, const float *points, float3 q,
                                         int k, float sqrMax, int32_t (&ids)[K], float (&ds)[K],
                                         int &found) {
   found = 0;
@@ -138,8 +141,8 @@ __device__ __forceinline__ bool knn_one(const cubql_bvh3f &bvh, const float *poi
   return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, sqrMax, prim);
 }
 
-template <int K>
-__global__ __launch_bounds__(kBlock) void k_knn(cubql_bvh3f bvh, const float *points,
+template <int K, typename BVH>
+__global__ __launch_bounds__(kBlock) void k_knn(BVH bvh, const float *points,
                                                 const float *queries, uint32_t nq, int k,
                                                 float sqrMax, int32_t *outIds, float *outD,
                                                 int32_t *outFound, Overflow *ovf,
@@ -166,8 +169,8 @@ __global__ __launch_bounds__(kBlock) void k_knn(cubql_bvh3f bvh, const float *po
   if (outFound) outFound[qi] = found;
 }
 
-template <int K>
-__global__ __launch_bounds__(64) void k_knn_deep(cubql_bvh3f bvh, const float *points,
+template <int K, typename BVH>
+__global__ __launch_bounds__(64) void k_knn_deep(BVH bvh, const float *points,
                                                  const float *queries, int k, float sqrMax,
                                                  int32_t *outIds, float *outD, int32_t *outFound,
                                                  Overflow *ovf, const uint32_t *ovfList) {
@@ -203,18 +206,125 @@ __global__ void k_fill_notfound(uint32_t nq, int k, int32_t *ids, float *d, int3
   }
 }
 
-template <int K>
-int launch_knn(const cubql_bvh3f &bvh, const float *points, const float *queries, uint32_t nq,
+template <int K, typename BVH>
+int launch_knn(const BVH &bvh, const float *points, const float *queries, uint32_t nq,
                int k, float sqrMax, int32_t *ids, float *d, int32_t *found, Overflow *ovf,
                uint32_t *ovfList, const uint32_t *order, hipStream_t s) {
   int ps = prof_begin(s);
-  k_knn<K><<<div_up(nq, kBlock), kBlock, 0, s>>>(bvh, points, queries, nq, k, sqrMax, ids, d, found,
+  k_knn<K, BVH><<<div_up(nq, kBlock), kBlock, 0, s>>>(bvh, points, queries, nq, k, sqrMax, ids, d, found,
                                                   ovf, ovfList, order);
   prof_end(ps, s, "knn_points");
   prof_set(ps, (12.0 + 8.0 * k + 4.0) * nq, nq);
-  k_knn_deep<K><<<64, 64, 0, s>>>(bvh, points, queries, k, sqrMax, ids, d, found, ovf, ovfList);
+  k_knn_deep<K, BVH><<<64, 64, 0, s>>>(bvh, points, queries, k, sqrMax, ids, d, found, ovf, ovfList);
   CUBQL_LAUNCH_CHECK();
   return CUBQL_OK;
+}
+
+
+// union of the valid children of wide node 0: the root box the query
+// ordering quantizes queries in
+template <int W>
+__global__ void k_wide_root_box(const cubql_wide_child3f *nodes, cubql_box3f *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  cubql_box3f b;
+  for (int d = 0; d < 3; d++) {
+    b.lower[d] = INFINITY;
+    b.upper[d] = -INFINITY;
+  }
+  for (int j = 0; j < W; j++) {
+    if (!(nodes[j].validOffsetCount & 1ull)) continue;
+    for (int d = 0; d < 3; d++) {
+      b.lower[d] = fminf(b.lower[d], nodes[j].bounds.lower[d]);
+      b.upper[d] = fmaxf(b.upper[d], nodes[j].bounds.upper[d]);
+    }
+  }
+  *out = b;
+}
+
+// Device view and root box of a tree: the binary root's box is the first 24
+// bytes of node 0; a wide tree's is reduced into scratch.
+inline int root_box(const cubql_bvh3f &bvh, hipStream_t, Scratch &, const cubql_box3f **box) {
+  *box = &bvh.nodes[0].bounds;
+  return CUBQL_OK;
+}
+template <int W>
+int root_box(const cubql_dev::WideBVH<W> &bvh, hipStream_t s, Scratch &scr, const cubql_box3f **box) {
+  cubql_box3f *b = nullptr;
+  CUBQL_CHECK(scr.alloc(&b, 1));
+  k_wide_root_box<W><<<1, 64, 0, s>>>((const cubql_wide_child3f *)bvh.nodes, b);
+  CUBQL_LAUNCH_CHECK();
+  *box = b;
+  return CUBQL_OK;
+}
+
+template <typename BVH>
+int fcp_run(const BVH &bvh, const float *d_points, const float *d_queries, uint32_t numQueries,
+            float sqrMaxSearchDist, int32_t *d_outIDs, float *d_outSqrDist, hipStream_t s, const char *profName) {
+  if (bvh.numNodes == 0) {
+    k_fill_notfound<<<256, 256, 0, s>>>(numQueries, 1, d_outIDs, d_outSqrDist, nullptr);
+    CUBQL_LAUNCH_CHECK();
+    return CUBQL_OK;
+  }
+  Scratch scr(nullptr, s);
+  Overflow *ovf = nullptr;
+  uint32_t *ovfList = nullptr;
+  CUBQL_CHECK(scr.alloc(&ovf, 1));
+  CUBQL_CHECK(scr.alloc(&ovfList, numQueries));
+  CUBQL_CHECK(fill_u32((uint32_t *)ovf, 0u, 2, s));
+  const cubql_box3f *rootBox = nullptr;
+  CUBQL_CHECK(root_box(bvh, s, scr, &rootBox));
+  const uint32_t *order = nullptr;
+  void *orderWs = nullptr;
+  if (numQueries >= kQueryOrderMin) {
+    char *w = nullptr;
+    CUBQL_CHECK(scr.alloc(&w, query_order_ws_bytes(numQueries)));
+    orderWs = w;
+    CUBQL_CHECK(query_order_box_ws(rootBox, d_queries, numQueries, s, orderWs, &order));
+  }
+  int ps = prof_begin(s);
+  k_fcp<BVH><<<div_up(numQueries, kBlock), kBlock, 0, s>>>(bvh, d_points, d_queries, numQueries, sqrMaxSearchDist,
+                                                           d_outIDs, d_outSqrDist, ovf, ovfList, order);
+  prof_end(ps, s, profName);
+  // compulsory bytes only: query in, id + distance out
+  prof_set(ps, 20.0 * numQueries, numQueries);
+  k_fcp_deep<BVH><<<64, 64, 0, s>>>(bvh, d_points, d_queries, sqrMaxSearchDist, d_outIDs, d_outSqrDist, ovf,
+                                     ovfList);
+  CUBQL_LAUNCH_CHECK();
+  return CUBQL_OK;
+}
+
+template <typename BVH>
+int knn_run(const BVH &bvh, const float *d_points, const float *d_queries, uint32_t numQueries, int k,
+            float sqrMaxSearchDist, int32_t *d_outIDs, float *d_outSqrDist, int32_t *d_numFound, hipStream_t s) {
+  if (bvh.numNodes == 0) {
+    k_fill_notfound<<<256, 256, 0, s>>>(numQueries, k, d_outIDs, d_outSqrDist, d_numFound);
+    CUBQL_LAUNCH_CHECK();
+    return CUBQL_OK;
+  }
+  Scratch scr(nullptr, s);
+  Overflow *ovf = nullptr;
+  uint32_t *ovfList = nullptr;
+  CUBQL_CHECK(scr.alloc(&ovf, 1));
+  CUBQL_CHECK(scr.alloc(&ovfList, numQueries));
+  CUBQL_CHECK(fill_u32((uint32_t *)ovf, 0u, 2, s));
+  const cubql_box3f *rootBox = nullptr;
+  CUBQL_CHECK(root_box(bvh, s, scr, &rootBox));
+  const uint32_t *order = nullptr;
+  if (numQueries >= kQueryOrderMin) {
+    char *w = nullptr;
+    CUBQL_CHECK(scr.alloc(&w, query_order_ws_bytes(numQueries)));
+    CUBQL_CHECK(query_order_box_ws(rootBox, d_queries, numQueries, s, w, &order));
+  }
+  if (k <= 4) return launch_knn<4>(bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
+  if (k <= 8) return launch_knn<8>(bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
+  if (k <= 16) return launch_knn<16>(bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
+  if (k <= 32) return launch_knn<32>(bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
+  return launch_knn<64>(bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
+}
+
+template <int W>
+cubql_dev::WideBVH<W> wide_view(const cubql_wbvh3f &w) {
+  return cubql_dev::WideBVH<W>{(const float4 *)w.nodes, w.numNodes, w.primIDs, w.numPrims};
 }
 
 }  // namespace pq
@@ -232,30 +342,8 @@ extern "C" int cubql_fcp_points_f3(const cubql_bvh3f *bvh, const float *d_points
     return CUBQL_ERR_INVALID_ARG;
   }
   if (numQueries == 0) return CUBQL_OK;
-  hipStream_t s = (hipStream_t)stream;
-  if (bvh->numNodes == 0) {
-    pq::k_fill_notfound<<<256, 256, 0, s>>>(numQueries, 1, d_outIDs, d_outSqrDist, nullptr);
-    CUBQL_LAUNCH_CHECK();
-    return CUBQL_OK;
-  }
-  Scratch scr(nullptr, s);
-  pq::Overflow *ovf = nullptr;
-  uint32_t *ovfList = nullptr;
-  CUBQL_CHECK(scr.alloc(&ovf, 1));
-  CUBQL_CHECK(scr.alloc(&ovfList, numQueries));
-  CUBQL_CHECK(fill_u32((uint32_t *)ovf, 0u, 2, s));
-  const uint32_t *order = nullptr;
-  CUBQL_CHECK(query_order(*bvh, d_queries, numQueries, s, scr, &order));
-  int ps = prof_begin(s);
-  pq::k_fcp<<<div_up(numQueries, pq::kBlock), pq::kBlock, 0, s>>>(
-      *bvh, d_points, d_queries, numQueries, sqrMaxSearchDist, d_outIDs, d_outSqrDist, ovf, ovfList, order);
-  prof_end(ps, s, "fcp_points");
-  // compulsory bytes only: query in, id + distance out
-  prof_set(ps, 20.0 * numQueries, numQueries);
-  pq::k_fcp_deep<<<64, 64, 0, s>>>(*bvh, d_points, d_queries, sqrMaxSearchDist, d_outIDs,
-                                   d_outSqrDist, ovf, ovfList);
-  CUBQL_LAUNCH_CHECK();
-  return CUBQL_OK;
+  return pq::fcp_run(*bvh, d_points, d_queries, numQueries, sqrMaxSearchDist, d_outIDs, d_outSqrDist,
+                     (hipStream_t)stream, "fcp_points");
 }
 
 extern "C" int cubql_knn_points_f3(const cubql_bvh3f *bvh, const float *d_points,
@@ -268,23 +356,42 @@ extern "C" int cubql_knn_points_f3(const cubql_bvh3f *bvh, const float *d_points
     return CUBQL_ERR_INVALID_ARG;
   }
   if (numQueries == 0) return CUBQL_OK;
-  hipStream_t s = (hipStream_t)stream;
-  if (bvh->numNodes == 0) {
-    pq::k_fill_notfound<<<256, 256, 0, s>>>(numQueries, k, d_outIDs, d_outSqrDist, d_numFound);
-    CUBQL_LAUNCH_CHECK();
-    return CUBQL_OK;
+  return pq::knn_run(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist,
+                     d_numFound, (hipStream_t)stream);
+}
+
+extern "C" int cubql_fcp_points_wide_f3(const cubql_wbvh3f *bvh, int width, const float *d_points,
+                                        const float *d_queries, uint32_t numQueries, float sqrMaxSearchDist,
+                                        int32_t *d_outIDs, float *d_outSqrDist, cubql_stream_t stream) {
+  clear_error();
+  if (!bvh || (width != 4 && width != 8) || (!d_queries && numQueries) || (!d_outIDs && numQueries)) {
+    set_error("cubql_fcp_points_wide_f3: invalid argument (width %d)", width);
+    return CUBQL_ERR_INVALID_ARG;
   }
-  Scratch scr(nullptr, s);
-  pq::Overflow *ovf = nullptr;
-  uint32_t *ovfList = nullptr;
-  CUBQL_CHECK(scr.alloc(&ovf, 1));
-  CUBQL_CHECK(scr.alloc(&ovfList, numQueries));
-  CUBQL_CHECK(fill_u32((uint32_t *)ovf, 0u, 2, s));
-  const uint32_t *order = nullptr;
-  CUBQL_CHECK(query_order(*bvh, d_queries, numQueries, s, scr, &order));
-  if (k <= 4) return pq::launch_knn<4>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
-  if (k <= 8) return pq::launch_knn<8>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
-  if (k <= 16) return pq::launch_knn<16>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
-  if (k <= 32) return pq::launch_knn<32>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
-  return pq::launch_knn<64>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
+  if (numQueries == 0) return CUBQL_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (width == 4)
+    return pq::fcp_run(pq::wide_view<4>(*bvh), d_points, d_queries, numQueries, sqrMaxSearchDist, d_outIDs,
+                       d_outSqrDist, s, "fcp_points_wide4");
+  return pq::fcp_run(pq::wide_view<8>(*bvh), d_points, d_queries, numQueries, sqrMaxSearchDist, d_outIDs,
+                     d_outSqrDist, s, "fcp_points_wide8");
+}
+
+extern "C" int cubql_knn_points_wide_f3(const cubql_wbvh3f *bvh, int width, const float *d_points,
+                                        const float *d_queries, uint32_t numQueries, int k, float sqrMaxSearchDist,
+                                        int32_t *d_outIDs, float *d_outSqrDist, int32_t *d_numFound,
+                                        cubql_stream_t stream) {
+  clear_error();
+  if (!bvh || (width != 4 && width != 8) || k < 1 || k > 64 ||
+      (numQueries && (!d_queries || !d_outIDs || !d_outSqrDist))) {
+    set_error("cubql_knn_points_wide_f3: invalid argument (width %d, k=%d)", width, k);
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  if (numQueries == 0) return CUBQL_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (width == 4)
+    return pq::knn_run(pq::wide_view<4>(*bvh), d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs,
+                       d_outSqrDist, d_numFound, s);
+  return pq::knn_run(pq::wide_view<8>(*bvh), d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs,
+                     d_outSqrDist, d_numFound, s);
 }

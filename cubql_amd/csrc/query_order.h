@@ -18,4 +18,14 @@ constexpr uint32_t kQueryOrderMin = 32768;
 int query_order(const cubql_bvh3f &bvh, const float *queries, uint32_t nq, hipStream_t s, Scratch &scr,
                 const uint32_t **order);
 
+// The same on a caller-provided workspace of query_order_ws_bytes(nq)
+// bytes (256-byte aligned); *order points into it.  Allocates nothing.
+size_t query_order_ws_bytes(uint32_t nq);
+int query_order_ws(const cubql_bvh3f &bvh, const float *queries, uint32_t nq, hipStream_t s, void *ws,
+                   const uint32_t **order);
+// The same with the cells taken over a device box (rootBox == nullptr:
+// no ordering).
+int query_order_box_ws(const cubql_box3f *rootBox, const float *queries, uint32_t nq, hipStream_t s, void *ws,
+                       const uint32_t **order);
+
 }  // namespace cubql

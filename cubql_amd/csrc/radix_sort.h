@@ -29,6 +29,9 @@ int radix_sort_pairs_dev_u64(uint64_t *kA, uint32_t *vA, uint64_t *kB, uint32_t 
 
 // Workspace-based variant (no allocation; capturable in a graph).
 size_t radix_sort_ws_bytes(uint32_t nMax, int keyBits);
+int radix_sort_pairs_ws_u32(uint32_t *kA, uint32_t *vA, uint32_t *kB, uint32_t *vB,
+                            const uint32_t *nDev, uint32_t nMax, int keyBits, hipStream_t s,
+                            void *ws, uint32_t **kRes, uint32_t **vRes);
 int radix_sort_pairs_ws_u64(uint64_t *kA, uint32_t *vA, uint64_t *kB, uint32_t *vB,
                             const uint32_t *nDev, uint32_t nMax, int keyBits, hipStream_t s,
                             void *ws, uint64_t **kRes, uint32_t **vRes);

@@ -225,6 +225,11 @@ int sort_pairs(K *kA, uint32_t *vA, K *kB, uint32_t *vB, const uint32_t *nDev, u
 }  // namespace rs
 
 size_t radix_sort_ws_bytes(uint32_t nMax, int keyBits) { return rs::sort_ws_bytes(nMax, keyBits); }
+int radix_sort_pairs_ws_u32(uint32_t *kA, uint32_t *vA, uint32_t *kB, uint32_t *vB, const uint32_t *nDev,
+                            uint32_t nMax, int keyBits, hipStream_t s, void *ws, uint32_t **kRes,
+                            uint32_t **vRes) {
+  return rs::sort_pairs_ws<uint32_t>(kA, vA, kB, vB, nDev, nMax, keyBits, s, ws, kRes, vRes);
+}
 int radix_sort_pairs_ws_u64(uint64_t *kA, uint32_t *vA, uint64_t *kB, uint32_t *vB, const uint32_t *nDev,
                             uint32_t nMax, int keyBits, hipStream_t s, void *ws, uint64_t **kRes,
                             uint32_t **vRes) {

@@ -44,6 +44,25 @@ typedef struct {
   uint32_t numPrims;
 } cubql_bvh3f;
 
+/* WideBVH<float,3,W>::Node::Child (bvh.h:117-130): bounds plus one 64-bit
+   word with valid in bit 0, offset in bits [1,46) and count in bits
+   [46,62).  A valid child with count 0 is inner (its wide node is
+   nodes[offset]); otherwise a leaf over primIDs [offset, offset+count).
+   32 bytes, 16-byte aligned. */
+typedef struct {
+  cubql_box3f bounds;
+  uint64_t validOffsetCount;
+} cubql_wide_child3f;
+
+/* WideBVH<float,3,W> (bvh.h:113-141), byte-identical for W = 4 and 8:
+   node i is children [i*W, i*W+W) of `nodes`; node 0 is the root. */
+typedef struct {
+  cubql_wide_child3f *nodes;
+  uint32_t numNodes;
+  uint32_t *primIDs;
+  uint32_t numPrims;
+} cubql_wbvh3f;
+
 /* BuildConfig (bvh.h:31-64), byte-identical: maxAllowedLeafSize,
    makeLeafThreshold, buildMethod (0 spatial median, 1 SAH, 2 ELH). */
 typedef struct {
@@ -143,6 +162,32 @@ int cubql_free_f3(cubql_bvh3f *bvh, cubql_stream_t s,
 int cubql_refit_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes,
                    cubql_stream_t s, const cubql_mem_resource *mem);
 
+/* cuBQL::gpuBuilder(WideBVH<float,3,W>&, const box3f*, uint32_t,
+   BuildConfig, cudaStream_t, GpuMemoryResource&) -- builder/cuda.h:
+   107-115, wide_gpu_builder.h:162-215: builds the binary BVH as
+   cubql_gpu_builder_f3 does (cfg->buildMethod selects spatial median or
+   SAH), then collapses it (cubql_collapse_wide_f3) and keeps its primIDs.
+   width is 4 or 8.  Blocks the host to read the node counts. */
+int cubql_gpu_builder_wide_f3(cubql_wbvh3f *bvh, int width, const cubql_box3f *d_boxes,
+                              uint32_t numBoxes, const cubql_build_config *cfg,
+                              cubql_stream_t s, const cubql_mem_resource *mem);
+
+/* Collapse of any BinaryBVH into a WideBVH of width 4 or 8, the
+   collapseInit/collapseSummarize/collapseExecute steps of
+   wide_gpu_builder.h:39-160: the binary root and every inner node whose
+   depth is a multiple of log2(width) each become one wide node, whose
+   children are the leaves and such roots met by a depth-first walk of its
+   treelet (second child first); unused child slots are invalid (empty box,
+   valid bit 0).  Wide node ids increase with the binary root's index
+   (the reference assigns them with an atomic counter).  The wide tree gets
+   its own copy of the primIDs; free it with cubql_free_wide_f3. */
+int cubql_collapse_wide_f3(const cubql_bvh3f *binary, int width, cubql_wbvh3f *wide,
+                           cubql_stream_t s, const cubql_mem_resource *mem);
+
+/* cuBQL::cuda::free(WideBVH&, ...) -- builder/cuda.h:164-170: frees nodes
+   and primIDs with the resource used to build, then synchronises. */
+int cubql_free_wide_f3(cubql_wbvh3f *bvh, cubql_stream_t s, const cubql_mem_resource *mem);
+
 /* Stable ascending sort of (key, value) pairs on key bits [0, keyBits) --
    the operation the reference builders obtain from
    cub::DeviceRadixSort::SortPairs (builder/cuda/radix.h:666-687,
@@ -178,6 +223,42 @@ int cubql_knn_points_f3(const cubql_bvh3f *bvh, const float *d_points,
                         float sqrMaxSearchDist, int32_t *d_outIDs,
                         float *d_outSqrDist, int32_t *d_numFound,
                         cubql_stream_t s);
+
+/* Batched points::findClosest / findKNN over a WideBVH<float,3,W> built
+   over points (queries/points/findClosest.h:82-90, 228-258 with the wide
+   forEachLeaf of traversal/shrinkingRadiusQuery.h:553-636; width 4 or 8).
+   Same results as cubql_fcp_points_f3 / cubql_knn_points_f3: the lowest
+   primID wins equal distances, so ids and distances do not depend on the
+   tree's width. */
+int cubql_fcp_points_wide_f3(const cubql_wbvh3f *bvh, int width, const float *d_points,
+                             const float *d_queries, uint32_t numQueries,
+                             float sqrMaxSearchDist, int32_t *d_outIDs,
+                             float *d_outSqrDist, cubql_stream_t s);
+int cubql_knn_points_wide_f3(const cubql_wbvh3f *bvh, int width, const float *d_points,
+                             const float *d_queries, uint32_t numQueries, int k,
+                             float sqrMaxSearchDist, int32_t *d_outIDs,
+                             float *d_outSqrDist, int32_t *d_numFound,
+                             cubql_stream_t s);
+
+/* Batched cuBQL::fixedBoxQuery::forEachPrim (traversal/fixedBoxQuery.h:
+   42-160, 215-230) with a per-prim box test: for each query box i,
+   d_counts[i] = number of prims whose box (d_primBoxes, the boxes the tree
+   was built over) overlaps it, closed boxes as box_t::overlaps
+   (math/box.h:208-212); the first min(count, maxPerQuery) of their primIDs
+   go to d_outIDs[i*maxPerQuery ...] in traversal order (depth first, first
+   child first).  d_outIDs may be NULL (counts only): call once for the
+   counts, then again with maxPerQuery = their maximum.  A query that would
+   need more than 1024 stack entries reports count 0xffffffff. */
+int cubql_box_query_f3(const cubql_bvh3f *bvh, const cubql_box3f *d_primBoxes,
+                       const cubql_box3f *d_queryBoxes, uint32_t numQueries,
+                       uint32_t maxPerQuery, uint32_t *d_outIDs, uint32_t *d_counts,
+                       cubql_stream_t s);
+/* The same over a WideBVH of width 4 or 8 (fixedBoxQuery.h:163-212:
+   children in slot order). */
+int cubql_box_query_wide_f3(const cubql_wbvh3f *bvh, int width, const cubql_box3f *d_primBoxes,
+                            const cubql_box3f *d_queryBoxes, uint32_t numQueries,
+                            uint32_t maxPerQuery, uint32_t *d_outIDs, uint32_t *d_counts,
+                            cubql_stream_t s);
 
 /* Triangle i = (verts[tris[3i]], verts[tris[3i+1]], verts[tris[3i+2]]); with
    d_tris == NULL it is (verts[3*(3i)], ...): the reference's non-indexed

@@ -145,3 +145,248 @@ __device__ __forceinline__ bool for_each_prim(const cubql_bvh3f &bvh, float qx, 
 }
 
 }  // namespace cubql_dev
+
+// ------------------------------------------------------------ wide BVHs
+//
+// WideBVH<float,3,W> (bvh.h:113-141): node i holds W consecutive 32-byte
+// children {box, bits}: valid in bit 0, offset in bits 1-45, count in bits
+// 46-61.  A valid child with count 0 is an inner child whose wide node is
+// nodes[offset]; otherwise a leaf over primIDs[offset, offset+count).
+namespace cubql_dev {
+
+template <int W>
+struct WideBVH {
+  const float4 *nodes;  // numNodes * W children, two float4 each
+  uint32_t numNodes;
+  const uint32_t *primIDs;
+  uint32_t numPrims;
+};
+
+__device__ __forceinline__ bool wc_valid(uint64_t w) { return (w & 1ull) != 0; }
+__device__ __forceinline__ uint32_t wc_off(uint64_t w) { return (uint32_t)((w >> 1) & ((1ull << 45) - 1)); }
+__device__ __forceinline__ uint32_t wc_cnt(uint64_t w) { return (uint32_t)((w >> 46) & 0xffffu); }
+
+template <int W> struct Log2;
+template <> struct Log2<4> { static constexpr int value = 2; };
+template <> struct Log2<8> { static constexpr int value = 3; };
+
+// Children of wide node `node` in visiting order: key = (squared distance
+// bits << 32) | payload, ascending, with ~0 for invalid children and
+// children farther than `cull`.  Payload: the wide node id of an inner
+// child, (1 << 31) | (node << log2 W) | slot for a leaf child (the
+// reference's ChildOrder, shrinkingRadiusQuery.h:511-537).
+template <int W>
+__device__ __forceinline__ void wide_child_keys(const WideBVH<W> &bvh, uint32_t node, float qx, float qy, float qz,
+                                                float cull, uint64_t (&key)[W]) {
+  const float4 *c = bvh.nodes + (size_t)node * W * 2;
+#pragma unroll
+  for (int j = 0; j < W; j++) {
+    const float4 a = c[2 * j], b = c[2 * j + 1];
+    const uint64_t bits = (uint64_t)__float_as_uint(b.z) | ((uint64_t)__float_as_uint(b.w) << 32);
+    QNode n;
+    n.a = a;
+    n.b = b;
+    const float d = sqr_dist_box(qx, qy, qz, n);
+    const uint32_t payload = wc_cnt(bits) ? (0x80000000u | (node << Log2<W>::value) | (uint32_t)j) : wc_off(bits);
+    key[j] = (wc_valid(bits) && d <= cull) ? (((uint64_t)__float_as_uint(d) << 32) | payload) : ~0ull;
+  }
+#pragma unroll
+  for (int i = W - 1; i > 0; --i) {
+#pragma unroll
+    for (int j = 0; j < i; j++) {
+      const uint64_t k0 = key[j], k1 = key[j + 1];
+      key[j] = k0 < k1 ? k0 : k1;
+      key[j + 1] = k0 < k1 ? k1 : k0;
+    }
+  }
+}
+
+template <int W>
+__device__ __forceinline__ uint64_t wide_leaf_bits(const WideBVH<W> &bvh, uint32_t payload, int log2w) {
+  const uint32_t node = (payload & 0x7fffffffu) >> log2w, slot = payload & ((1u << log2w) - 1);
+  const float4 b = bvh.nodes[((size_t)node << log2w) * 2 + 2 * slot + 1];
+  return (uint64_t)__float_as_uint(b.z) | ((uint64_t)__float_as_uint(b.w) << 32);
+}
+
+// forEachLeaf over a wide BVH (shrinkingRadiusQuery.h:553-636) with the
+// binary traversal's rules: children visited nearest first, a child is
+// visited iff its distance is <= the current radius, a negative leaf
+// result ends the query.  Returns false if the stack overflowed.
+template <int STACK, int W, typename LeafFn>
+__device__ __forceinline__ bool for_each_leaf(const WideBVH<W> &bvh, float qx, float qy, float qz, float sqrMax,
+                                              LeafFn &&leaf) {
+  if (bvh.numNodes == 0) return true;
+  uint64_t stack[STACK];
+  int sp = 0;
+  float cull = sqrMax;
+  uint32_t node = 0;
+  bool expand = true;
+  while (true) {
+    if (expand) {
+      uint64_t key[W];
+      wide_child_keys<W>(bvh, node, qx, qy, qz, cull, key);
+#pragma unroll
+      for (int j = W - 1; j > 0; --j) {
+        if (key[j] != ~0ull) {
+          if (sp >= STACK) return false;
+          stack[sp++] = key[j];
+        }
+      }
+      expand = false;
+      if (key[0] != ~0ull) {
+        const uint32_t payload = (uint32_t)key[0];
+        if (payload & 0x80000000u) {
+          const uint64_t bits = wide_leaf_bits(bvh, payload, Log2<W>::value);
+          const float r = leaf(bvh.primIDs + wc_off(bits), wc_cnt(bits));
+          if (r < 0.f) return true;
+          cull = fminf(cull, r);
+        } else {
+          node = payload;
+          expand = true;
+          continue;
+        }
+      }
+    }
+    // pop the next entry still within the radius
+    bool found = false;
+    while (sp > 0) {
+      const uint64_t e = stack[--sp];
+      if (__uint_as_float((uint32_t)(e >> 32)) > cull) continue;
+      const uint32_t payload = (uint32_t)e;
+      if (payload & 0x80000000u) {
+        const uint64_t bits = wide_leaf_bits(bvh, payload, Log2<W>::value);
+        const float r = leaf(bvh.primIDs + wc_off(bits), wc_cnt(bits));
+        if (r < 0.f) return true;
+        cull = fminf(cull, r);
+        continue;
+      }
+      node = payload;
+      found = true;
+      break;
+    }
+    if (!found) return true;
+    expand = true;
+  }
+}
+
+// Descends along the nearest child to one leaf and calls prim(primID) for
+// its prims (the wide counterpart of visit_nearest_leaf).
+template <int W, typename PrimFn>
+__device__ __forceinline__ void visit_nearest_leaf(const WideBVH<W> &bvh, float qx, float qy, float qz,
+                                                   PrimFn &&prim) {
+  if (bvh.numNodes == 0) return;
+  uint32_t node = 0;
+  for (int guard = 0; guard < (1 << 20); guard++) {
+    uint64_t key[W];
+    wide_child_keys<W>(bvh, node, qx, qy, qz, INFINITY, key);
+    if (key[0] == ~0ull) return;
+    const uint32_t payload = (uint32_t)key[0];
+    if (payload & 0x80000000u) {
+      const uint64_t bits = wide_leaf_bits(bvh, payload, Log2<W>::value);
+      const uint32_t *ids = bvh.primIDs + wc_off(bits);
+      for (uint32_t j = 0; j < wc_cnt(bits); j++) prim(ids[j]);
+      return;
+    }
+    node = payload;
+  }
+}
+
+template <int STACK, int W, typename PrimFn>
+__device__ __forceinline__ bool for_each_prim(const WideBVH<W> &bvh, float qx, float qy, float qz, float sqrMax,
+                                              PrimFn &&prim) {
+  auto leaf = [&](const uint32_t *ids, uint32_t cnt) -> float {
+    float r = INFINITY;
+    for (uint32_t i = 0; i < cnt; i++) r = fminf(r, prim(ids[i]));
+    return r;
+  };
+  return for_each_leaf<STACK>(bvh, qx, qy, qz, sqrMax, leaf);
+}
+
+}  // namespace cubql_dev
+
+// ------------------------------------------------------- fixed box queries
+//
+// cuBQL::fixedBoxQuery::forEachLeaf (traversal/fixedBoxQuery.h:74-139 for
+// binary BVHs, :163-212 for wide ones): every leaf whose box overlaps a
+// fixed query box (closed boxes: overlap unless one box's upper is below
+// the other's lower on some axis, box.h:208-212), depth first, first child
+// first.  leaf(const uint32_t *primIDs, uint32_t count) returns nonzero to
+// end the query.  Returns false if the stack overflowed.
+namespace cubql_dev {
+
+__device__ __forceinline__ bool box_overlaps(const float (&lo)[3], const float (&hi)[3], const QNode &n) {
+  return !(n.a.w < lo[0] || n.b.x < lo[1] || n.b.y < lo[2] || hi[0] < n.a.x || hi[1] < n.a.y ||
+           hi[2] < n.a.z);
+}
+
+template <int STACK, typename LeafFn>
+__device__ __forceinline__ bool for_each_leaf_in_box(const cubql_bvh3f &bvh, const float (&lo)[3],
+                                                     const float (&hi)[3], LeafFn &&leaf) {
+  if (bvh.numNodes == 0) return true;
+  uint32_t stack[STACK];
+  int sp = 0;
+  uint64_t node = node_oc(load_node(bvh.nodes, 0));
+  while (true) {
+    bool haveLeaf = true;
+    while (oc_cnt(node) == 0) {
+      const uint32_t i0 = oc_off(node);
+      const QNode n0 = load_node(bvh.nodes, i0);
+      const QNode n1 = load_node(bvh.nodes, i0 + 1);
+      const bool o0 = box_overlaps(lo, hi, n0), o1 = box_overlaps(lo, hi, n1);
+      if (!o0 && !o1) { haveLeaf = false; break; }
+      if (o0 && o1) {
+        if (sp >= STACK) return false;
+        stack[sp++] = i0 + 1;
+      }
+      node = o0 ? node_oc(n0) : node_oc(n1);
+    }
+    if (haveLeaf && leaf(bvh.primIDs + oc_off(node), oc_cnt(node))) return true;
+    if (sp == 0) return true;
+    node = node_oc(load_node(bvh.nodes, stack[--sp]));
+  }
+}
+
+// wide BVH: children of a node in slot order; a stack entry is (node, next
+// slot) as in the reference
+template <int STACK, int W, typename LeafFn>
+__device__ __forceinline__ bool for_each_leaf_in_box(const WideBVH<W> &bvh, const float (&lo)[3],
+                                                     const float (&hi)[3], LeafFn &&leaf) {
+  if (bvh.numNodes == 0) return true;
+  uint2 stack[STACK];
+  int sp = 0;
+  uint32_t node = 0, slot = 0;
+  while (true) {
+    const float4 *c = bvh.nodes + ((size_t)node * W + slot) * 2;
+    QNode n;
+    n.a = c[0];
+    n.b = c[1];
+    const uint64_t bits = node_oc(n);
+    bool advance = true;
+    if (wc_valid(bits) && box_overlaps(lo, hi, n)) {
+      if (wc_cnt(bits) == 0) {
+        if (slot + 1 < W) {
+          if (sp >= STACK) return false;
+          stack[sp++] = make_uint2(node, slot + 1);
+        }
+        node = wc_off(bits);
+        slot = 0;
+        advance = false;
+      } else if (leaf(bvh.primIDs + wc_off(bits), wc_cnt(bits))) {
+        return true;
+      }
+    }
+    if (advance) {
+      if (slot + 1 < W) {
+        slot++;
+      } else if (sp > 0) {
+        const uint2 e = stack[--sp];
+        node = e.x;
+        slot = e.y;
+      } else {
+        return true;
+      }
+    }
+  }
+}
+
+}  // namespace cubql_dev

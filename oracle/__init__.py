@@ -69,6 +69,8 @@ def lib():
             "orc_morton_keys": (I, [V, I, I, V, V]),
             "orc_sort_pairs": (None, [V, V, I]),
             "orc_radix_hierarchy": (I, [V, I, I, V]),
+            "orc_wide_collapse": (I, [V, I, I, V, I]),
+            "orc_box_query": (None, [V, I, V, I, I, V, V, V, I64, I, V, V]),
             "orc_atomicbox_encode": (C.c_int32, [F]),
             "orc_atomicbox_decode": (F, [C.c_int32]),
         }
@@ -321,6 +323,39 @@ def radix_build(boxes, bits_per_dim: int, leaf_threshold: int) -> Tree:
     b = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
     lib().orc_refit(_p(tree.nodes), nn, _p(tree.prim_ids), _p(b))
     return tree
+
+
+def wide_collapse(tree: Tree, width: int):
+    """W-wide collapse of a binary tree (restates wide_gpu_builder.h:39-160):
+    returns [numWide, width] children as NODE_DTYPE records whose 'oc' word
+    is valid:1 | offset:45 | count:16."""
+    cap = max(1, len(tree.nodes))
+    out = np.zeros(cap * width, dtype=NODE_DTYPE)
+    n = lib().orc_wide_collapse(_p(tree.nodes), len(tree.nodes), width, _p(out), cap)
+    if n < 0:
+        raise RuntimeError("wide collapse failed")
+    return out[: n * width].reshape(n, width)
+
+
+def box_query(tree: Tree, prim_boxes, query_boxes, max_out: int, wide=None, width: int = 4):
+    """(ids [nq, max_out] uint32, counts [nq] uint32) of the fixed-box query
+    in traversal order (restates traversal/fixedBoxQuery.h)."""
+    pb = np.ascontiguousarray(prim_boxes, dtype=np.float32).reshape(-1, 6)
+    qb = np.ascontiguousarray(query_boxes, dtype=np.float32).reshape(-1, 6)
+    nq = qb.shape[0]
+    out = np.zeros((nq, max(max_out, 1)), dtype=np.uint32)
+    cnt = np.zeros(nq, dtype=np.uint32)
+    w = None if wide is None else np.ascontiguousarray(wide.reshape(-1)).view(NODE_DTYPE)
+    lib().orc_box_query(_p(tree.nodes), len(tree.nodes), _p(w), 0 if wide is None else wide.shape[0], width,
+                        _p(tree.prim_ids), _p(pb), _p(qb), nq, max_out, _p(out), _p(cnt))
+    return out[:, :max_out], cnt
+
+
+def wide_fields(children):
+    """(valid, offset, count) arrays of wide child records."""
+    w = children["oc"].astype(np.uint64)
+    return ((w & np.uint64(1)).astype(bool), ((w >> np.uint64(1)) & np.uint64((1 << 45) - 1)).astype(np.int64),
+            ((w >> np.uint64(46)) & np.uint64(0xffff)).astype(np.int64))
 
 
 # ----------------------------------------------------------------------------

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <limits>
 #include <numeric>
+#include <functional>
 #include <vector>
 
 namespace {
@@ -1158,3 +1159,141 @@ float orc_atomicbox_decode(int32_t bits) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Wide BVH collapse; builder/cuda/wide_gpu_builder.h:39-160 restated as a
+// serial pass.  A binary node is a wide root if it is node 0 or an inner
+// node whose depth (root = 0) is a multiple of log2(W) (:77-98).  A wide
+// node's children come from a depth-first walk of its treelet that pushes
+// offset+0 then offset+1 and stops at leaves and at other wide roots
+// (:117-146); unused slots are invalid with empty bounds (:147-158).  Wide
+// ids: increasing binary root index (the reference takes them from an
+// atomic counter, which leaves their order open).  Writes numWide*W
+// children (box + valid:1|offset:45|count:16 word) and returns numWide,
+// or -1 if `cap` wide nodes are too few.
+int orc_wide_collapse(const orc_node3f *nodes, int numNodes, int W, orc_node3f *wide, int cap) {
+  if (numNodes <= 0) return 0;
+  int logW = W == 4 ? 2 : 3;
+  bool node1Used = (nodes[0].oc >> 48) == 0 && (nodes[0].oc & ((1ull << 48) - 1)) == 1;
+  std::vector<int> depth(numNodes, -1), wideId(numNodes, -1);
+  // depth by a walk from the root (unreachable nodes keep -1)
+  std::vector<int> stack = {0};
+  depth[0] = 0;
+  while (!stack.empty()) {
+    int i = stack.back();
+    stack.pop_back();
+    uint64_t oc = nodes[i].oc;
+    if ((oc >> 48) != 0) continue;
+    int off = (int)(oc & ((1ull << 48) - 1));
+    for (int c = 0; c < 2; c++) {
+      depth[off + c] = depth[i] + 1;
+      stack.push_back(off + c);
+    }
+  }
+  (void)node1Used;
+  int numWide = 0;
+  for (int i = 0; i < numNodes; i++) {
+    bool inner = (nodes[i].oc >> 48) == 0;
+    if (i == 0 || (depth[i] > 0 && inner && depth[i] % logW == 0)) wideId[i] = numWide++;
+  }
+  if (numWide > cap) return -1;
+  for (int i = 0; i < numNodes; i++) {
+    if (wideId[i] < 0) continue;
+    orc_node3f *target = wide + (size_t)wideId[i] * W;
+    int written = 0;
+    std::vector<int> st = {i};
+    while (!st.empty()) {
+      int x = st.back();
+      st.pop_back();
+      uint64_t oc = nodes[x].oc;
+      uint64_t cnt = oc >> 48, off = oc & ((1ull << 48) - 1);
+      if (cnt != 0 || (x != i && wideId[x] >= 0)) {
+        if (written >= W) return -1;
+        orc_node3f c;
+        c.b = nodes[x].b;
+        uint64_t o = cnt ? off : (uint64_t)wideId[x];
+        c.oc = 1ull | ((o & ((1ull << 45) - 1)) << 1) | (cnt << 46);
+        target[written++] = c;
+      } else {
+        st.push_back((int)off + 0);
+        st.push_back((int)off + 1);
+      }
+    }
+    for (; written < W; written++) {
+      orc_node3f c;
+      for (int d = 0; d < 3; d++) {
+        c.b.lo[d] = INFINITY;
+        c.b.hi[d] = -INFINITY;
+      }
+      c.oc = ((uint64_t)0xffffffffu << 1) | (0xffffull << 46);
+      target[written] = c;
+    }
+  }
+  return numWide;
+}
+
+// ---------------------------------------------------------------------------
+// Fixed-box queries; traversal/fixedBoxQuery.h:74-139 (binary: depth first,
+// the first child before the second when both overlap) and :163-212 (wide:
+// children in slot order), with the per-prim test box_t::overlaps
+// (math/box.h:208-212: closed boxes).  For each query, counts[q] = number of
+// overlapping prims and out[q*maxOut ..] the first min(count, maxOut) of
+// them in traversal order.  wide == nullptr selects the binary tree.
+static bool orc_overlaps(const float *alo, const float *ahi, const float *blo, const float *bhi) {
+  for (int d = 0; d < 3; d++)
+    if (ahi[d] < blo[d] || bhi[d] < alo[d]) return false;
+  return true;
+}
+
+static void orc_box_query_wide_rec(const orc_node3f *wide, int W, uint32_t n, const orc_box3f &Q,
+                                   const std::function<void(uint64_t, uint64_t)> &leaf) {
+  for (int s = 0; s < W; s++) {
+    const orc_node3f &c = wide[(size_t)n * W + s];
+    uint64_t w = c.oc;
+    if (!(w & 1ull) || !orc_overlaps(Q.lo, Q.hi, c.b.lo, c.b.hi)) continue;
+    uint64_t off = (w >> 1) & ((1ull << 45) - 1), cnt = (w >> 46) & 0xffff;
+    if (cnt)
+      leaf(off, cnt);
+    else
+      orc_box_query_wide_rec(wide, W, (uint32_t)off, Q, leaf);
+  }
+}
+
+void orc_box_query(const orc_node3f *nodes, int numNodes, const orc_node3f *wide, int numWide, int W,
+                   const uint32_t *primIDs, const orc_box3f *primBoxes, const orc_box3f *queries, int64_t nq,
+                   int maxOut, uint32_t *out, uint32_t *counts) {
+  for (int64_t q = 0; q < nq; q++) {
+    const orc_box3f &Q = queries[q];
+    uint32_t count = 0;
+    std::function<void(uint64_t, uint64_t)> leaf = [&](uint64_t off, uint64_t cnt) {
+      for (uint64_t j = 0; j < cnt; j++) {
+        uint32_t p = primIDs[off + j];
+        if (orc_overlaps(Q.lo, Q.hi, primBoxes[p].lo, primBoxes[p].hi)) {
+          if ((int)count < maxOut) out[q * maxOut + count] = p;
+          count++;
+        }
+      }
+    };
+    if (!wide && numNodes > 0) {
+      // depth first; when both children overlap the first is walked first
+      std::vector<uint32_t> stack = {0};
+      while (!stack.empty()) {
+        uint32_t i = stack.back();
+        stack.pop_back();
+        uint64_t oc = nodes[i].oc, cnt = oc >> 48, off = oc & ((1ull << 48) - 1);
+        if (cnt) {
+          leaf(off, cnt);
+          continue;
+        }
+        bool o0 = orc_overlaps(Q.lo, Q.hi, nodes[off].b.lo, nodes[off].b.hi);
+        bool o1 = orc_overlaps(Q.lo, Q.hi, nodes[off + 1].b.lo, nodes[off + 1].b.hi);
+        if (o1) stack.push_back((uint32_t)off + 1);
+        if (o0) stack.push_back((uint32_t)off);
+      }
+    } else if (wide && numWide > 0) {
+      // children in slot order, each inner child's subtree before the next slot
+      orc_box_query_wide_rec(wide, W, 0, Q, leaf);
+    }
+    counts[q] = count;
+  }
+}

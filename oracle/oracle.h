@@ -179,4 +179,21 @@ void orc_point_grad_f64(const float *points, const float *queries, int64_t nq,
 int32_t orc_atomicbox_encode(float f);
 float orc_atomicbox_decode(int32_t i);
 
+/* Collapse of a binary BVH into a W-wide one (W = 4 or 8);
+   wide_gpu_builder.h:39-160.  Children are 32-byte records (box + one
+   64-bit word: valid bit 0, offset bits 1-45, count bits 46-61), W per wide
+   node, wide ids in increasing order of their binary root.  Returns the
+   wide node count, or -1 if it exceeds cap. */
+int orc_wide_collapse(const orc_node3f *nodes, int numNodes, int W, orc_node3f *wide, int cap);
+
+/* Fixed-box queries; traversal/fixedBoxQuery.h:74-139 (binary, wide ==
+   NULL) and :163-212 (wide, W children per node): for each query box,
+   counts[q] = number of prims whose box overlaps it (closed boxes,
+   math/box.h:208-212) and out[q*maxOut ..] the first min(count, maxOut)
+   of them in traversal order (depth first, first child / lowest slot
+   first). */
+void orc_box_query(const orc_node3f *nodes, int numNodes, const orc_node3f *wide, int numWide, int W,
+                   const uint32_t *primIDs, const orc_box3f *primBoxes, const orc_box3f *queries, int64_t nq,
+                   int maxOut, uint32_t *out, uint32_t *counts);
+
 }  // extern "C"

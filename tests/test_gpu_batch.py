@@ -258,81 +258,6 @@ def test_batch_stack_overflow_path(cq, oracle_mod, stack):
     _check_loss_and_grads(oracle_mod, B, dS, dso, dT, dto, range(len(so_h) - 1))
 
 
-def _check_loss_and_grads(oracle_mod, B, src, so, tgt, to, clouds):
-    S, Tg = src.cpu().numpy(), tgt.cpu().numpy()
-    so_h, to_h = so.cpu().numpy().astype(np.int64), to.cpu().numpy().astype(np.int64)
-    nn_s, nn_t = B.src_nn.cpu().numpy(), B.tgt_nn.cpu().numpy()
-    loss = B.loss.cpu().numpy()
-    gs, gt = B.grad_src.cpu().numpy(), B.grad_tgt.cpu().numpy()
-    worst = 0.0
-    for c in clouds:
-        sb, se, tb, te = so_h[c], so_h[c + 1], to_h[c], to_h[c + 1]
-        if se == sb or te == tb:
-            continue
-        a, b = S[sb:se], Tg[tb:te]
-        ls, lt = nn_s[sb:se] - tb, nn_t[tb:te] - sb
-        d_s = np.linalg.norm(a.astype(np.float64) - b[ls].astype(np.float64), axis=1)
-        d_t = np.linalg.norm(b.astype(np.float64) - a[lt].astype(np.float64), axis=1)
-        l64 = d_s.mean() + d_t.mean()
-        assert abs(loss[c] - l64) <= 1e-5 * l64 + 1e-30, (c, loss[c], l64)
-        worst = max(worst, check_chamfer_grads(oracle_mod, a, b, ls, lt, gs[sb:se], gt[tb:te],
-                                               label=f"cloud {c}"))
-    print(f"[gpu] chamfer gradient rel err (of each point's sum|g|): max {worst:.2e}", flush=True)
-
-
-def _oracle_nn(oracle_mod, data, queries):
-    """Nearest data point per query, lowest index on ties: brute force for
-    small clouds, the oracle's tree query (same rule) for large ones."""
-    if len(data) * len(queries) <= 4e7:
-        return oracle_mod.fcp_brute(data, queries)
-    tree = oracle_mod.radix_build(oracle_mod.points_to_boxes(data), 10, 8)
-    return oracle_mod.fcp(tree, data, queries, lowest_id=True)
-
-
-def test_batch_c5_full_config(cq, oracle_mod):
-    """4096 uneven cloud pairs at the benchmark's seeds and sizes."""
-    print("[gpu] batch C5 full config", flush=True)
-    from cubql_amd import synth
-    ss = synth.chamfer_cloud_sizes(4096, 101)
-    ts = synth.chamfer_cloud_sizes(4096, 102)
-    src, so = synth.chamfer_clouds_torch(ss, 103, 104)
-    tgt, to = synth.chamfer_clouds_torch(ts, 105, 106)
-    maxc = int(max(ss.max(), ts.max()))
-    print(f"[gpu] C5: {len(src):,} source / {len(tgt):,} target points, largest cloud {maxc}",
-          flush=True)
-    B = cq.ChamferBatch(src, so, tgt, to, cq.BatchParams(8, 128, 32, maxc)).run()
-    G = cq.ChamferBatch(src, so, tgt, to, cq.BatchParams(4, 256, 64, maxc)).capture()
-    for _ in range(2):
-        G.replay()
-    torch.cuda.synchronize()
-    for a, b in [(B.src_nn, G.src_nn), (B.src_d2, G.src_d2), (B.tgt_nn, G.tgt_nn),
-                 (B.tgt_d2, G.tgt_d2)]:
-        assert torch.equal(a, b)
-    tot = ss + ts
-    order = np.argsort(tot, kind="stable")
-    rng = np.random.default_rng(7)
-    sel = set(order[-10:].tolist()) | set(order[:10].tolist())
-    sel |= set(int(x) for x in np.argsort(ss)[-3:]) | set(int(x) for x in np.argsort(ts)[-3:])
-    sel |= set(rng.choice(4096, 44, replace=False).tolist())
-    sel = sorted(sel)
-    assert len(sel) >= 64
-    S, Tg = src.cpu().numpy(), tgt.cpu().numpy()
-    so_h, to_h = so.cpu().numpy().astype(np.int64), to.cpu().numpy().astype(np.int64)
-    nn_s, d_s = B.src_nn.cpu().numpy(), B.src_d2.cpu().numpy()
-    nn_t, d_t = B.tgt_nn.cpu().numpy(), B.tgt_d2.cpu().numpy()
-    for c in sel:
-        a, b = S[so_h[c]:so_h[c + 1]], Tg[to_h[c]:to_h[c + 1]]
-        oi, od = _oracle_nn(oracle_mod, b, a)
-        assert np.array_equal(nn_s[so_h[c]:so_h[c + 1]], oi + to_h[c]), f"cloud {c} src->tgt ids"
-        assert np.array_equal(d_s[so_h[c]:so_h[c + 1]], od), f"cloud {c} src->tgt distances"
-        oi, od = _oracle_nn(oracle_mod, a, b)
-        assert np.array_equal(nn_t[to_h[c]:to_h[c + 1]], oi + so_h[c]), f"cloud {c} tgt->src ids"
-        assert np.array_equal(d_t[to_h[c]:to_h[c + 1]], od), f"cloud {c} tgt->src distances"
-    print(f"[gpu] C5: {len(sel)} clouds equal the oracle (largest {int(tot[order[-1]])} points)",
-          flush=True)
-    _check_loss_and_grads(oracle_mod, B, src, so, tgt, to, sel)
-
-
 @pytest.mark.parametrize("stack", [32, 64])
 def test_batch_stack_overflow_clouds(cq, oracle_mod, stack):
     """Clouds of many identical points: every node on a query's path has

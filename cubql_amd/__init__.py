@@ -114,6 +114,7 @@ def lib():
         "cubql_free_f3": (I, [P(BVH3f), V, P(MemResource)]),
         "cubql_refit_f3": (I, [P(BVH3f), V, V, P(MemResource)]),
         "cubql_fcp_points_f3": (I, [P(BVH3f), V, V, U32, F, V, V, V]),
+        "cubql_fcp_points_noalloc_f3": (I, [P(BVH3f), V, V, U32, F, V, V, V]),
         "cubql_knn_points_f3": (I, [P(BVH3f), V, V, U32, I, F, V, V, V, V]),
         "cubql_gpu_builder_wide_f3": (I, [P(WBVH3f), I, V, U32, P(BuildConfig), V, P(MemResource)]),
         "cubql_collapse_wide_f3": (I, [P(BVH3f), I, P(WBVH3f), V, P(MemResource)]),

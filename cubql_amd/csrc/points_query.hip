@@ -7,6 +7,8 @@
 // include/cubql_amd/traversal.hip.h.  Result rule: among equal float32
 // distances the lowest primID wins (kNN lists are ordered by
 // (sqrDist, primID)).
+#include <mutex>
+
 #include "common.h"
 #include "cubql_amd/traversal.hip.h"
 #include "profile.h"
@@ -257,30 +259,43 @@ int root_box(const cubql_dev::WideBVH<W> &bvh, hipStream_t s, Scratch &scr, cons
   return CUBQL_OK;
 }
 
+// Workspace of one closest-point call over nq queries (256-byte aligned
+// slots): overflow counters | root box | overflow list | query order.
+constexpr size_t kWsOvf = 0, kWsRoot = 128, kWsList = 256;
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+inline size_t fcp_ws_bytes(uint32_t nq) {
+  return kWsList + align256((size_t)nq * 4) + (nq >= kQueryOrderMin ? query_order_ws_bytes(nq) : 0);
+}
+
+inline const cubql_box3f *root_box_ws(const cubql_bvh3f &bvh, hipStream_t, char *) { return &bvh.nodes[0].bounds; }
+template <int W>
+const cubql_box3f *root_box_ws(const cubql_dev::WideBVH<W> &bvh, hipStream_t s, char *ws) {
+  cubql_box3f *b = (cubql_box3f *)(ws + kWsRoot);
+  k_wide_root_box<W><<<1, 64, 0, s>>>((const cubql_wide_child3f *)bvh.nodes, b);
+  return b;
+}
+
+// Closest-point queries on a caller-provided workspace of fcp_ws_bytes(nq)
+// bytes; allocates nothing.
 template <typename BVH>
-int fcp_run(const BVH &bvh, const float *d_points, const float *d_queries, uint32_t numQueries,
-            float sqrMaxSearchDist, int32_t *d_outIDs, float *d_outSqrDist, hipStream_t s, const char *profName) {
+int fcp_run_ws(const BVH &bvh, const float *d_points, const float *d_queries, uint32_t numQueries,
+               float sqrMaxSearchDist, int32_t *d_outIDs, float *d_outSqrDist, hipStream_t s, char *ws,
+               const char *profName) {
   if (bvh.numNodes == 0) {
     k_fill_notfound<<<256, 256, 0, s>>>(numQueries, 1, d_outIDs, d_outSqrDist, nullptr);
     CUBQL_LAUNCH_CHECK();
     return CUBQL_OK;
   }
-  Scratch scr(nullptr, s);
-  Overflow *ovf = nullptr;
-  uint32_t *ovfList = nullptr;
-  CUBQL_CHECK(scr.alloc(&ovf, 1));
-  CUBQL_CHECK(scr.alloc(&ovfList, numQueries));
+  Overflow *ovf = (Overflow *)(ws + kWsOvf);
+  uint32_t *ovfList = (uint32_t *)(ws + kWsList);
+  // a fill kernel, not a memset node: callers may capture this in a graph
   CUBQL_CHECK(fill_u32((uint32_t *)ovf, 0u, 2, s));
-  const cubql_box3f *rootBox = nullptr;
-  CUBQL_CHECK(root_box(bvh, s, scr, &rootBox));
+  const cubql_box3f *rootBox = root_box_ws(bvh, s, ws);
+  CUBQL_LAUNCH_CHECK();
   const uint32_t *order = nullptr;
-  void *orderWs = nullptr;
-  if (numQueries >= kQueryOrderMin) {
-    char *w = nullptr;
-    CUBQL_CHECK(scr.alloc(&w, query_order_ws_bytes(numQueries)));
-    orderWs = w;
-    CUBQL_CHECK(query_order_box_ws(rootBox, d_queries, numQueries, s, orderWs, &order));
-  }
+  if (numQueries >= kQueryOrderMin)
+    CUBQL_CHECK(query_order_box_ws(rootBox, d_queries, numQueries, s,
+                                   ws + kWsList + align256((size_t)numQueries * 4), &order));
   int ps = prof_begin(s);
   k_fcp<BVH><<<div_up(numQueries, kBlock), kBlock, 0, s>>>(bvh, d_points, d_queries, numQueries, sqrMaxSearchDist,
                                                            d_outIDs, d_outSqrDist, ovf, ovfList, order);
@@ -291,6 +306,16 @@ int fcp_run(const BVH &bvh, const float *d_points, const float *d_queries, uint3
                                      ovfList);
   CUBQL_LAUNCH_CHECK();
   return CUBQL_OK;
+}
+
+template <typename BVH>
+int fcp_run(const BVH &bvh, const float *d_points, const float *d_queries, uint32_t numQueries,
+            float sqrMaxSearchDist, int32_t *d_outIDs, float *d_outSqrDist, hipStream_t s, const char *profName) {
+  Scratch scr(nullptr, s);
+  char *ws = nullptr;
+  if (bvh.numNodes) CUBQL_CHECK(scr.alloc(&ws, fcp_ws_bytes(numQueries)));
+  return fcp_run_ws(bvh, d_points, d_queries, numQueries, sqrMaxSearchDist, d_outIDs, d_outSqrDist, s, ws,
+                    profName);
 }
 
 template <typename BVH>
@@ -394,4 +419,76 @@ extern "C" int cubql_knn_points_wide_f3(const cubql_wbvh3f *bvh, int width, cons
                        d_outSqrDist, d_numFound, s);
   return pq::knn_run(pq::wide_view<8>(*bvh), d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs,
                      d_outSqrDist, d_numFound, s);
+}
+
+// ------------------------------------------- closest points, no allocation
+//
+// cubql_fcp_points_noalloc_f3 runs cubql_fcp_points_f3's kernels on a
+// workspace in a file-scope device array instead of allocated scratch, for
+// callers that cannot allocate.  The array is shared by every call on a
+// device and keeps the last call's contents; the overflow counters are
+// cleared by a fill kernel at the start of each call.  Calls that overlap on
+// one device (two streams, two threads) race on it.
+namespace cubql {
+namespace pq {
+constexpr uint32_t kNoAllocMaxQueries = 1u << 20;
+constexpr size_t kNoAllocScratchBytes = (size_t)24 << 20;
+constexpr int kMaxDevices = 64;
+}  // namespace pq
+}  // namespace cubql
+
+static __device__ __align__(256) char scratch[cubql::pq::kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratchMu;
+char *g_scratchPtr[cubql::pq::kMaxDevices];
+
+// Device address of `scratch` on the current device, looked up on the
+// first call there (the host-side name is not a device address).
+int scratch_ptr(char **out) {
+  using namespace cubql;
+  int dev = -1;
+  CUBQL_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= pq::kMaxDevices) {
+    set_error("cubql_fcp_points_noalloc_f3: device %d out of range", dev);
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratchMu);
+  if (!g_scratchPtr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("cubql_fcp_points_noalloc_f3: hipGetSymbolAddress failed: %s", hipGetErrorString(e));
+      return CUBQL_ERR_HIP;
+    }
+    g_scratchPtr[dev] = (char *)p;
+  }
+  *out = g_scratchPtr[dev];
+  return CUBQL_OK;
+}
+}  // namespace
+
+extern "C" int cubql_fcp_points_noalloc_f3(const cubql_bvh3f *bvh, const float *d_points, const float *d_queries,
+                                           uint32_t numQueries, float sqrMaxSearchDist, int32_t *d_outIDs,
+                                           float *d_outSqrDist, cubql_stream_t stream) {
+  clear_error();
+  if (!bvh || (!d_queries && numQueries) || (!d_outIDs && numQueries)) {
+    set_error("cubql_fcp_points_noalloc_f3: null argument");
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  if (numQueries > pq::kNoAllocMaxQueries) {
+    set_error("cubql_fcp_points_noalloc_f3: %u queries exceed the limit of %u per call", numQueries,
+              pq::kNoAllocMaxQueries);
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  if (numQueries == 0) return CUBQL_OK;
+  if (pq::fcp_ws_bytes(numQueries) > pq::kNoAllocScratchBytes) {
+    set_error("cubql_fcp_points_noalloc_f3: workspace of %zu bytes exceeds the static %zu", pq::fcp_ws_bytes(numQueries),
+              pq::kNoAllocScratchBytes);
+    return CUBQL_ERR_INTERNAL;
+  }
+  char *ws = nullptr;
+  CUBQL_CHECK(scratch_ptr(&ws));
+  return pq::fcp_run_ws(*bvh, d_points, d_queries, numQueries, sqrMaxSearchDist, d_outIDs, d_outSqrDist,
+                        (hipStream_t)stream, ws, "fcp_points_noalloc");
 }

@@ -214,6 +214,20 @@ int cubql_fcp_points_f3(const cubql_bvh3f *bvh, const float *d_points,
                         float sqrMaxSearchDist, int32_t *d_outIDs,
                         float *d_outSqrDist, cubql_stream_t s);
 
+/* No counterpart in the reference: cubql_fcp_points_f3 for callers that
+   cannot allocate (code under HIP graph capture, serving loops, harnesses
+   that forbid hipMalloc).  Same arguments and results; the call allocates
+   nothing and keeps its temporaries (overflow list, query ordering) in a
+   static 24 MiB device array that each GPU holds until the process ends.
+   At most CUBQL_FCP_NOALLOC_MAX_QUERIES queries per call (more:
+   CUBQL_ERR_INVALID_ARG).  Every call on a GPU shares that array, so calls
+   that overlap on one GPU (two streams or threads) race. */
+#define CUBQL_FCP_NOALLOC_MAX_QUERIES 1048576u
+int cubql_fcp_points_noalloc_f3(const cubql_bvh3f *bvh, const float *d_points,
+                                const float *d_queries, uint32_t numQueries,
+                                float sqrMaxSearchDist, int32_t *d_outIDs,
+                                float *d_outSqrDist, cubql_stream_t s);
+
 /* Batched cuBQL::points::findKNN (queries/points/knn.h:49-116 with
    knn::insert_linear, queries/common/knn.h:57-78).  Output rows of k
    entries sorted by (sqrDist, primID); unfilled entries are -1/+inf.

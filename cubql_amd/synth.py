@@ -33,9 +33,12 @@ def lcg_init(seed: int, index: np.ndarray) -> np.ndarray:
 
 
 def lcg_uniform(seed: int, n: int, lo: float = 0.0, hi: float = 1.0, dims: int = 3,
-                chunk: int = 1 << 22) -> np.ndarray:
-    """[n, dims] float32 points, identical to UniformPointGenerator<dims>."""
-    out = np.empty((n, dims), dtype=np.float32)
+                chunk: int = 1 << 22, dtype=np.float32) -> np.ndarray:
+    """[n, dims] points, identical to UniformPointGenerator<dims>: the
+    generator works in double (uniformPointGenerator, Generator.cpp:202-218);
+    float32 output is its conversion (samples/common/IO.h:60-66), float64
+    output the values a generatePoints .bin file holds."""
+    out = np.empty((n, dims), dtype=dtype)
     lo_d, span = np.float64(lo), np.float64(hi) - np.float64(lo)
     for b in range(0, n, chunk):
         e = min(n, b + chunk)
@@ -43,7 +46,7 @@ def lcg_uniform(seed: int, n: int, lo: float = 0.0, hi: float = 1.0, dims: int =
         for d in range(dims):
             state = (state * np.uint64(1664525) + np.uint64(1013904223)) & _M32
             r = (state & np.uint64(0x00FFFFFF)).astype(np.float32) / np.float32(16777216.0)
-            out[b:e, d] = (lo_d + r.astype(np.float64) * span).astype(np.float32)
+            out[b:e, d] = (lo_d + r.astype(np.float64) * span).astype(dtype)
     return out
 
 

@@ -136,6 +136,8 @@ def lib():
                                              C.c_size_t, V, V, V, V, V, V, V, P(V)]),
         "cubql_graph_launch": (I, [V, V]),
         "cubql_graph_destroy": (I, [V]),
+        "cubql_query_stats_points_f3": (I, [P(BVH3f), V, V, U32, I, F, V, V]),
+        "cubql_query_stats_triangles_f3": (I, [P(BVH3f), V, V, V, U32, F, V, V]),
         "cubql_profile_enable": (None, [I]),
         "cubql_profile_reset": (None, []),
         "cubql_profile_query": (I, [C.c_char_p, P(C.c_uint64), P(C.c_double), P(C.c_double),
@@ -602,6 +604,30 @@ class profile:  # noqa: N801
                                      C.byref(un)) != 0:
             return None
         return {"launches": n.value, "ms": ms.value, "bytes": by.value, "units": un.value}
+
+
+def query_stats(bvh: BinaryBVH, queries: torch.Tensor, data: Optional[torch.Tensor] = None, k: int = 0,
+                tris: Optional[torch.Tensor] = None, verts: Optional[torch.Tensor] = None,
+                sqrMaxSearchDist: float = math.inf, stream=None):
+    """Traversal statistics of a query batch (cubql_query_stats_*): points
+    closest-point (k=0) or kNN (k>0) over `data`, or closest-triangle over
+    (tris, verts).  Returns {'node_bytes', 'prims', 'queries'} summed over
+    the batch (the reference's buildQual NUM_TRAVERSAL_STEPS counters)."""
+    queries = _as_points(queries, "queries")
+    nq = queries.shape[0]
+    out = torch.zeros(3, dtype=torch.int64, device=queries.device)
+    st = C.c_void_p(_stream_handle(stream))
+    if verts is not None:
+        v = _as_points(verts, "verts")
+        _check(lib().cubql_query_stats_triangles_f3(C.byref(bvh.c), _ptr(_as_tris(tris)), _ptr(v), _ptr(queries),
+                                                    nq, float(sqrMaxSearchDist), _ptr(out), st),
+               "query_stats(triangles)")
+    else:
+        data = _as_points(data, "data")
+        _check(lib().cubql_query_stats_points_f3(C.byref(bvh.c), _ptr(data), _ptr(queries), nq, int(k),
+                                                 float(sqrMaxSearchDist), _ptr(out), st), "query_stats(points)")
+    o = out.cpu().tolist()
+    return {"node_bytes": int(o[0]), "prims": int(o[1]), "queries": int(o[2])}
 
 
 def radix_sort_pairs(keys: torch.Tensor, vals: torch.Tensor, key_bits: int, stream=None):

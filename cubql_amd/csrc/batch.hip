@@ -278,8 +278,10 @@ __global__ __launch_bounds__(64) void k_bquery_deep(Side qs, Side ts, int32_t *n
     int32_t best;
     float bestD;
     if (!fcp_forest<1024>(ts, ts.rootOf[cloud], q, best, bestD)) {
+      // deeper than the deep stack (a cloud's Morton tree is at most 46
+      // levels deep, so this does not happen): marked incomplete
       atomicAdd(&ovf[1], 1u);
-      best = -1;
+      best = -2;
       bestD = INFINITY;
     }
     nn[qi] = best;

@@ -20,6 +20,8 @@ namespace pq {
 constexpr int kBlock = 256;
 constexpr int kFastStack = 64;
 constexpr int kDeepStack = 1024;
+// id of a query whose traversal needed more than kDeepStack entries
+constexpr int32_t kIncomplete = -2;
 
 __device__ __forceinline__ float3 load_pt(const float *p, uint32_t i) {
   return make_float3(p[3 * (size_t)i + 0], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2]);
@@ -30,12 +32,13 @@ struct Overflow {
   uint32_t deepFailed;
 };
 
-template <int STACK, typename BVH>
+template <int STACK, typename BVH, typename Stats = cubql_dev::NoStats>
 __device__ __forceinline__ bool fcp_one(const BVH &bvh, const float *points, float3 q,
-                                        float sqrMax, int32_t &best, float &bestD) {
+                                        float sqrMax, int32_t &best, float &bestD, Stats &&st = Stats()) {
   best = -1;
   bestD = sqrMax;
   auto prim = [&](uint32_t pid) -> float {
+    st.prim();
     float3 p = load_pt(points, pid);
     float d = cubql_dev::sqr_dist_pt(p.x, p.y, p.z, q.x, q.y, q.z);
     if (d < bestD || (d == bestD && best >= 0 && (int32_t)pid < best)) {
@@ -46,8 +49,8 @@ __device__ __forceinline__ bool fcp_one(const BVH &bvh, const float *points, flo
   };
   // seed the radius with the nearest-child leaf (visited again below,
   // changing nothing)
-  cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim);
-  return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, bestD, prim);
+  cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim, 0u, st);
+  return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, bestD, prim, 0u, st);
 }
 
 // fast path: 64-entry stack; queries that overflow go to the deep kernel.
@@ -84,8 +87,9 @@ __global__ __launch_bounds__(64) void k_fcp_deep(BVH bvh, const float *points,
     int32_t best;
     float bestD;
     if (!fcp_one<kDeepStack>(bvh, points, q, sqrMax, best, bestD)) {
+      // deeper than the deep stack: marked incomplete, not "none found"
       atomicAdd(&ovf->deepFailed, 1u);
-      best = -1;
+      best = kIncomplete;
     }
     outIds[qi] = best;
     if (outD) outD[qi] = best >= 0 ? bestD : INFINITY;
@@ -93,15 +97,16 @@ __global__ __launch_bounds__(64) void k_fcp_deep(BVH bvh, const float *points,
 }
 
 // kNN with a register-resident candidate list of K entries (K >= k).
-template <int K, int STACK, typename BVH>
+template <int K, int STACK, typename BVH, typename Stats = cubql_dev::NoStats>
 __device__ __forceinline__ bool knn_one(const BVH &bvh, const float *points, float3 q,
                                         int k, float sqrMax, int32_t (&ids)[K], float (&ds)[K],
-                                        int &found) {
+                                        int &found, Stats &&st = Stats()) {
   found = 0;
 #pragma unroll
   for (int j = 0; j < K; j++) { ids[j] = -1; ds[j] = INFINITY; }
   float cull = sqrMax;
   auto prim = [&](uint32_t pid) -> float {
+    st.prim();
     float3 p = load_pt(points, pid);
     float d = cubql_dev::sqr_dist_pt(p.x, p.y, p.z, q.x, q.y, q.z);
     int32_t id = (int32_t)pid;
@@ -140,7 +145,7 @@ __device__ __forceinline__ bool knn_one(const BVH &bvh, const float *points, flo
     }
     return cull;
   };
-  return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, sqrMax, prim);
+  return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, sqrMax, prim, 0u, st);
 }
 
 template <int K, typename BVH>
@@ -184,9 +189,10 @@ __global__ __launch_bounds__(64) void k_knn_deep(BVH bvh, const float *points,
     float ds[K];
     int found;
     if (!knn_one<K, kDeepStack>(bvh, points, q, k, sqrMax, ids, ds, found)) {
+      // deeper than the deep stack: marked incomplete, not "none found"
       atomicAdd(&ovf->deepFailed, 1u);
-      found = 0;
-      for (int j = 0; j < K; j++) { ids[j] = -1; ds[j] = INFINITY; }
+      found = -1;
+      for (int j = 0; j < K; j++) { ids[j] = kIncomplete; ds[j] = INFINITY; }
     }
     for (int j = 0; j < K; j++) {
       if (j < k) {
@@ -491,4 +497,76 @@ extern "C" int cubql_fcp_points_noalloc_f3(const cubql_bvh3f *bvh, const float *
   CUBQL_CHECK(scratch_ptr(&ws));
   return pq::fcp_run_ws(*bvh, d_points, d_queries, numQueries, sqrMaxSearchDist, d_outIDs, d_outSqrDist,
                         (hipStream_t)stream, ws, "fcp_points_noalloc");
+}
+
+// ------------------------------------------------- traversal statistics
+namespace cubql {
+namespace pq {
+
+__device__ __forceinline__ void stats_flush(const cubql_dev::TraversalCounts &st, uint32_t nq,
+                                            unsigned long long *out) {
+  unsigned long long nb = st.nodeBytes, np = st.prims;
+  for (int o = 32; o > 0; o >>= 1) {
+    nb += __shfl_xor(nb, o, 64);
+    np += __shfl_xor(np, o, 64);
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&out[0], nb);
+    atomicAdd(&out[1], np);
+    atomicAdd(&out[2], (unsigned long long)min(64u, nq - blockIdx.x * 64));
+  }
+}
+
+// Instrumented closest-point (k == 0) or kNN queries with the deep stack:
+// per query, bytes of node data loaded and prims tested.
+template <int K>
+__global__ __launch_bounds__(64) void k_points_stats(cubql_bvh3f bvh, const float *points, const float *queries,
+                                                     uint32_t nq, int k, float sqrMax, unsigned long long *out) {
+  const uint32_t qi = blockIdx.x * 64 + threadIdx.x;
+  cubql_dev::TraversalCounts st;
+  if (qi < nq) {
+    const float3 q = load_pt(queries, qi);
+    if (k == 0) {
+      int32_t best;
+      float bestD;
+      fcp_one<kDeepStack>(bvh, points, q, sqrMax, best, bestD, st);
+    } else {
+      int32_t ids[K];
+      float ds[K];
+      int found;
+      knn_one<K, kDeepStack>(bvh, points, q, k, sqrMax, ids, ds, found, st);
+    }
+  }
+  stats_flush(st, nq, out);
+}
+
+}  // namespace pq
+}  // namespace cubql
+
+extern "C" int cubql_query_stats_points_f3(const cubql_bvh3f *bvh, const float *d_points, const float *d_queries,
+                                           uint32_t numQueries, int k, float sqrMaxSearchDist, uint64_t *d_out,
+                                           cubql_stream_t stream) {
+  using namespace cubql;
+  clear_error();
+  if (!bvh || !d_out || k < 0 || k > 64 || (numQueries && (!d_queries || !d_points))) {
+    set_error("cubql_query_stats_points_f3: invalid argument (k=%d)", k);
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  CUBQL_CHECK(fill_u32((uint32_t *)d_out, 0u, 6, s));
+  if (numQueries == 0 || bvh->numNodes == 0) return CUBQL_OK;
+  const uint32_t g = div_up(numQueries, 64);
+  unsigned long long *o = (unsigned long long *)d_out;
+  if (k <= 4)
+    pq::k_points_stats<4><<<g, 64, 0, s>>>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, o);
+  else if (k <= 8)
+    pq::k_points_stats<8><<<g, 64, 0, s>>>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, o);
+  else if (k <= 16)
+    pq::k_points_stats<16><<<g, 64, 0, s>>>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, o);
+  else if (k <= 32)
+    pq::k_points_stats<32><<<g, 64, 0, s>>>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, o);
+  else
+    pq::k_points_stats<64><<<g, 64, 0, s>>>(*bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, o);
+  CUBQL_LAUNCH_CHECK();
+  return CUBQL_OK;
 }

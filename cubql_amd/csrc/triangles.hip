@@ -28,18 +28,22 @@ namespace tri {
 constexpr int kBlock = 256;
 constexpr int kFastStack = 64;
 constexpr int kDeepStack = 1024;
+// id of a query whose traversal needed more than kDeepStack entries
+constexpr int32_t kIncomplete = -2;
 
 using namespace cubql_dev;
 
 struct Overflow { uint32_t count, deepFailed; };
 
-template <int STACK>
+template <int STACK, typename Stats = NoStats>
 __device__ __forceinline__ bool fcp_tri_one(const cubql_bvh3f &bvh, const int32_t *tris, const float *verts,
-                                            F3 q, float sqrMax, int32_t &best, float &bestD, F3 &bestP) {
+                                            F3 q, float sqrMax, int32_t &best, float &bestD, F3 &bestP,
+                                            Stats &&st = Stats()) {
   best = -1;
   bestD = sqrMax;
   bestP = {0.f, 0.f, 0.f};
   auto prim = [&](uint32_t t) -> float {
+    st.prim();
     F3 a, b, c, P;
     tri_verts(tris, verts, t, a, b, c);
     float w[3];
@@ -53,8 +57,8 @@ __device__ __forceinline__ bool fcp_tri_one(const cubql_bvh3f &bvh, const int32_
   };
   // seed the radius with the nearest-child leaf (visited again below,
   // changing nothing)
-  cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim);
-  return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, bestD, prim);
+  cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim, 0u, st);
+  return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, bestD, prim, 0u, st);
 }
 
 __device__ __forceinline__ void write_result(uint32_t qi, int32_t best, float bestD, F3 P, int32_t *ids,
@@ -98,8 +102,9 @@ __global__ __launch_bounds__(64) void k_fcp_tris_deep(cubql_bvh3f bvh, const int
     float bestD;
     F3 P;
     if (!fcp_tri_one<kDeepStack>(bvh, tris, verts, q, sqrMax, best, bestD, P)) {
+      // deeper than the deep stack: marked incomplete, not "none found"
       atomicAdd(&ovf->deepFailed, 1u);
-      best = -1;
+      best = kIncomplete;
     }
     write_result(qi, best, bestD, P, ids, d2, cp);
   }
@@ -283,6 +288,57 @@ extern "C" int cubql_points_distance_backward_f3(const float *d_points, const fl
       d_points, d_queries, numQueries, k, d_ids, d_gradDist, d_gradQueries, d_gradPoints);
   prof_end(ps, (hipStream_t)s, "points_backward");
   prof_set(ps, (12.0 + 12.0 + k * (4.0 + 4.0 + 12.0 + 24.0)) * numQueries, numQueries);
+  CUBQL_LAUNCH_CHECK();
+  return CUBQL_OK;
+}
+
+// ------------------------------------------------- traversal statistics
+namespace cubql {
+namespace tri {
+
+// Instrumented closest-triangle queries (deep stack): per query, bytes of
+// node data loaded and triangles tested, summed into out[0], out[1];
+// out[2] counts the queries.
+__global__ __launch_bounds__(64) void k_fcp_tris_stats(cubql_bvh3f bvh, const int32_t *tris, const float *verts,
+                                                       const float *queries, uint32_t nq, float sqrMax,
+                                                       unsigned long long *out) {
+  const uint32_t qi = blockIdx.x * 64 + threadIdx.x;
+  TraversalCounts st;
+  if (qi < nq) {
+    int32_t best;
+    float bestD;
+    F3 P;
+    fcp_tri_one<kDeepStack>(bvh, tris, verts, ld3(queries, qi), sqrMax, best, bestD, P, st);
+  }
+  unsigned long long nb = st.nodeBytes, np = st.prims;
+  for (int o = 32; o > 0; o >>= 1) {
+    nb += __shfl_xor(nb, o, 64);
+    np += __shfl_xor(np, o, 64);
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&out[0], nb);
+    atomicAdd(&out[1], np);
+    atomicAdd(&out[2], (unsigned long long)min(64u, nq - blockIdx.x * 64));
+  }
+}
+
+}  // namespace tri
+}  // namespace cubql
+
+extern "C" int cubql_query_stats_triangles_f3(const cubql_bvh3f *bvh, const int32_t *d_tris, const float *d_verts,
+                                              const float *d_queries, uint32_t numQueries, float sqrMaxSearchDist,
+                                              uint64_t *d_out, cubql_stream_t stream) {
+  using namespace cubql;
+  clear_error();
+  if (!bvh || !d_verts || !d_out || (numQueries && !d_queries)) {
+    set_error("cubql_query_stats_triangles_f3: null argument");
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  CUBQL_CHECK(fill_u32((uint32_t *)d_out, 0u, 6, s));
+  if (numQueries == 0 || bvh->numNodes == 0) return CUBQL_OK;
+  tri::k_fcp_tris_stats<<<div_up(numQueries, 64), 64, 0, s>>>(*bvh, d_tris, d_verts, d_queries, numQueries,
+                                                              sqrMaxSearchDist, (unsigned long long *)d_out);
   CUBQL_LAUNCH_CHECK();
   return CUBQL_OK;
 }

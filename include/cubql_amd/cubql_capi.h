@@ -102,8 +102,11 @@ const char *cubql_version(void);
 /* cuBQL::gpuBuilder(BinaryBVH<float,3>&, const box3f*, uint32_t, BuildConfig,
    cudaStream_t, GpuMemoryResource&)  -- builder/cuda.h:91-99,
    dispatch gpu_builder.h:31-67.  buildMethod SPATIAL_MEDIAN runs the
-   adaptive spatial-median builder (sm_builder.h:469-614) and SAH the binned
-   SAH builder (sah_builder.h:396-537); makeLeafThreshold 0 means 1.
+   adaptive spatial-median builder (sm_builder.h:469-614), SAH the binned
+   SAH builder (sah_builder.h:396-537) and ELH the binned edge-length-
+   heuristic builder (elh_builder.h:437-557: 8 bins, cost = sum of box
+   extents x count); makeLeafThreshold 0 means 1.  SAH and ELH honour
+   maxAllowedLeafSize as cubql_sah_builder_f3 describes.
    numBoxes == 0 leaves *bvh untouched.  Spatial median: bvh->numPrims =
    numBoxes, invalid prims listed after all valid ones (sm_builder.h:590).
    Blocks the host once at the end to read back the node count. */
@@ -208,7 +211,12 @@ int cubql_radix_sort_pairs_u64(uint64_t *d_keys, uint32_t *d_vals,
    closest data point within sqrt(sqrMaxSearchDist) or -1; outSqrDist[i]
    = its squared distance (x*x+y*y)+z*z in float32 (or +inf).  Among equal
    distances the lowest primID is returned.  outSqrDist may be NULL.
-   points/queries are float3 arrays (12-byte stride). */
+   points/queries are float3 arrays (12-byte stride).  A query whose
+   traversal needs more than 1024 stack entries (a tree deeper than 1024
+   levels) is incomplete and gets id CUBQL_INCOMPLETE (-2), distance +inf;
+   the reference truncates such a traversal silently
+   (shrinkingRadiusQuery.h:475-478). */
+#define CUBQL_INCOMPLETE (-2)
 int cubql_fcp_points_f3(const cubql_bvh3f *bvh, const float *d_points,
                         const float *d_queries, uint32_t numQueries,
                         float sqrMaxSearchDist, int32_t *d_outIDs,
@@ -231,7 +239,8 @@ int cubql_fcp_points_noalloc_f3(const cubql_bvh3f *bvh, const float *d_points,
 /* Batched cuBQL::points::findKNN (queries/points/knn.h:49-116 with
    knn::insert_linear, queries/common/knn.h:57-78).  Output rows of k
    entries sorted by (sqrDist, primID); unfilled entries are -1/+inf.
-   d_numFound may be NULL.  1 <= k <= 64. */
+   d_numFound may be NULL.  1 <= k <= 64.  An incomplete query (see
+   cubql_fcp_points_f3) gets numFound -1 and ids CUBQL_INCOMPLETE. */
 int cubql_knn_points_f3(const cubql_bvh3f *bvh, const float *d_points,
                         const float *d_queries, uint32_t numQueries, int k,
                         float sqrMaxSearchDist, int32_t *d_outIDs,
@@ -290,7 +299,8 @@ int cubql_triangle_boxes_f3(const int32_t *d_tris, const float *d_verts,
    outTriIDs[i] = closest triangle within sqrt(sqrMax) or -1 (lowest id on
    equal distances), outSqrDist[i] its squared distance (float32, no FMA
    contraction), outClosest[3i..3i+2] the closest point FCPResult::P.
-   outSqrDist and outClosest may be NULL. */
+   outSqrDist and outClosest may be NULL.  An incomplete query (see
+   cubql_fcp_points_f3) gets id CUBQL_INCOMPLETE. */
 int cubql_fcp_triangles_f3(const cubql_bvh3f *bvh, const int32_t *d_tris,
                            const float *d_verts, const float *d_queries,
                            uint32_t numQueries, float sqrMaxSearchDist,
@@ -397,6 +407,23 @@ void cubql_profile_reset(void);
 /* 0 on success, -1 if the kernel has no record. */
 int cubql_profile_query(const char *kernel, uint64_t *launches, double *totalMs,
                         double *bytes, uint64_t *units);
+
+/* Traversal statistics, the counterpart of the nodeDist / primCode counters
+   of the reference's traversal-quality test (testing/bvh/buildQual.h:65-98,
+   NUM_TRAVERSAL_STEPS): runs the same queries as cubql_fcp_points_f3
+   (k == 0), cubql_knn_points_f3 (1 <= k <= 64) or cubql_fcp_triangles_f3
+   with per-query counters and writes three device uint64 sums to d_out:
+   bytes of node data loaded (64 per child pair expanded, 32 per node popped
+   from the stack), prims (points or triangles) tested, and queries.  A
+   measurement aid: slower than the plain queries, results discarded. */
+int cubql_query_stats_points_f3(const cubql_bvh3f *bvh, const float *d_points,
+                                const float *d_queries, uint32_t numQueries, int k,
+                                float sqrMaxSearchDist, uint64_t *d_out,
+                                cubql_stream_t s);
+int cubql_query_stats_triangles_f3(const cubql_bvh3f *bvh, const int32_t *d_tris,
+                                   const float *d_verts, const float *d_queries,
+                                   uint32_t numQueries, float sqrMaxSearchDist,
+                                   uint64_t *d_out, cubql_stream_t s);
 
 #ifdef __cplusplus
 }

@@ -65,13 +65,29 @@ __device__ __forceinline__ float sqr_dist_pt(float ax, float ay, float az, float
   return dx * dx + dy * dy + dz * dz;
 }
 
+// Traversal counters for the instrumented query kernels (the counterpart of
+// the nodeDist / primCode counters of testing/bvh/buildQual.h:65-98): bytes
+// of node data loaded (a 64-byte child pair per inner node expanded, a
+// 32-byte node per stack entry popped, W 32-byte children per wide node)
+// and prims tested (counted by the query's prim code).  NoStats compiles to
+// nothing.
+struct NoStats {
+  __device__ __forceinline__ void nodes(uint32_t) {}
+  __device__ __forceinline__ void prim() {}
+};
+struct TraversalCounts {
+  uint32_t nodeBytes = 0, prims = 0;
+  __device__ __forceinline__ void nodes(uint32_t b) { nodeBytes += b; }
+  __device__ __forceinline__ void prim() { prims++; }
+};
+
 // forEachLeaf.  leaf(const uint32_t *prims, uint32_t count) -> float.
 // Returns false if the stack overflowed (query incomplete).
 // `root` selects the subtree to walk (0 for a whole BinaryBVH; a cloud's
 // root node in a batched forest).
-template <int STACK, typename LeafFn>
+template <int STACK, typename LeafFn, typename Stats>
 __device__ __forceinline__ bool for_each_leaf(const cubql_bvh3f &bvh, float qx, float qy, float qz,
-                                              float sqrMax, LeafFn &&leaf, uint32_t root = 0) {
+                                              float sqrMax, LeafFn &&leaf, uint32_t root, Stats &st) {
   if (bvh.numNodes == 0) return true;
   uint2 stack[STACK];
   int sp = 0;
@@ -83,6 +99,7 @@ __device__ __forceinline__ bool for_each_leaf(const cubql_bvh3f &bvh, float qx, 
       const uint32_t i0 = oc_off(node);
       const QNode n0 = load_node(bvh.nodes, i0);
       const QNode n1 = load_node(bvh.nodes, i0 + 1);
+      st.nodes(64);
       const float d0 = sqr_dist_box(qx, qy, qz, n0);
       const float d1 = sqr_dist_box(qx, qy, qz, n1);
       if (fminf(d0, d1) > cull) { haveLeaf = false; break; }
@@ -104,6 +121,7 @@ __device__ __forceinline__ bool for_each_leaf(const cubql_bvh3f &bvh, float qx, 
       uint2 e = stack[--sp];
       if (__uint_as_float(e.y) <= cull) {
         node = node_oc(load_node(bvh.nodes, e.x));
+        st.nodes(32);
         found = true;
         break;
       }
@@ -111,37 +129,56 @@ __device__ __forceinline__ bool for_each_leaf(const cubql_bvh3f &bvh, float qx, 
     if (!found) return true;
   }
 }
+template <int STACK, typename LeafFn>
+__device__ __forceinline__ bool for_each_leaf(const cubql_bvh3f &bvh, float qx, float qy, float qz,
+                                              float sqrMax, LeafFn &&leaf, uint32_t root = 0) {
+  NoStats st;
+  return for_each_leaf<STACK>(bvh, qx, qy, qz, sqrMax, leaf, root, st);
+}
 
 // Descends from `root` along the nearer child (ties to the second child, as
 // the traversal orders them) to one leaf and calls prim(primID) for its
 // prims.  A closest-point query that runs this first can start the full
 // traversal with the radius it found, so that on the way down it stacks only
 // far children that can still hold a closer prim.
-template <typename PrimFn>
+template <typename PrimFn, typename Stats>
 __device__ __forceinline__ void visit_nearest_leaf(const cubql_bvh3f &bvh, float qx, float qy, float qz,
-                                                   PrimFn &&prim, uint32_t root = 0) {
+                                                   PrimFn &&prim, uint32_t root, Stats &st) {
   if (bvh.numNodes == 0) return;
   uint64_t node = node_oc(load_node(bvh.nodes, root));
   while (oc_cnt(node) == 0) {
     const uint32_t i0 = oc_off(node);
     const QNode n0 = load_node(bvh.nodes, i0);
     const QNode n1 = load_node(bvh.nodes, i0 + 1);
+    st.nodes(64);
     node = sqr_dist_box(qx, qy, qz, n0) < sqr_dist_box(qx, qy, qz, n1) ? node_oc(n0) : node_oc(n1);
   }
   const uint32_t *ids = bvh.primIDs + oc_off(node);
   for (uint32_t j = 0; j < oc_cnt(node); j++) prim(ids[j]);
 }
+template <typename PrimFn>
+__device__ __forceinline__ void visit_nearest_leaf(const cubql_bvh3f &bvh, float qx, float qy, float qz,
+                                                   PrimFn &&prim, uint32_t root = 0) {
+  NoStats st;
+  visit_nearest_leaf(bvh, qx, qy, qz, prim, root, st);
+}
 
 // forEachPrim: prim(uint32_t primID) -> float (new squared cull radius).
-template <int STACK, typename PrimFn>
+template <int STACK, typename PrimFn, typename Stats>
 __device__ __forceinline__ bool for_each_prim(const cubql_bvh3f &bvh, float qx, float qy, float qz,
-                                              float sqrMax, PrimFn &&prim, uint32_t root = 0) {
+                                              float sqrMax, PrimFn &&prim, uint32_t root, Stats &st) {
   auto leaf = [&](const uint32_t *ids, uint32_t cnt) -> float {
     float r = INFINITY;
     for (uint32_t i = 0; i < cnt; i++) r = fminf(r, prim(ids[i]));
     return r;
   };
-  return for_each_leaf<STACK>(bvh, qx, qy, qz, sqrMax, leaf, root);
+  return for_each_leaf<STACK>(bvh, qx, qy, qz, sqrMax, leaf, root, st);
+}
+template <int STACK, typename PrimFn>
+__device__ __forceinline__ bool for_each_prim(const cubql_bvh3f &bvh, float qx, float qy, float qz,
+                                              float sqrMax, PrimFn &&prim, uint32_t root = 0) {
+  NoStats st;
+  return for_each_prim<STACK>(bvh, qx, qy, qz, sqrMax, prim, root, st);
 }
 
 }  // namespace cubql_dev
@@ -212,19 +249,20 @@ __device__ __forceinline__ uint64_t wide_leaf_bits(const WideBVH<W> &bvh, uint32
 // binary traversal's rules: children visited nearest first, a child is
 // visited iff its distance is <= the current radius, a negative leaf
 // result ends the query.  Returns false if the stack overflowed.
-template <int STACK, int W, typename LeafFn>
+template <int STACK, int W, typename LeafFn, typename Stats>
 __device__ __forceinline__ bool for_each_leaf(const WideBVH<W> &bvh, float qx, float qy, float qz, float sqrMax,
-                                              LeafFn &&leaf) {
+                                              LeafFn &&leaf, uint32_t root, Stats &st) {
   if (bvh.numNodes == 0) return true;
   uint64_t stack[STACK];
   int sp = 0;
   float cull = sqrMax;
-  uint32_t node = 0;
+  uint32_t node = root;
   bool expand = true;
   while (true) {
     if (expand) {
       uint64_t key[W];
       wide_child_keys<W>(bvh, node, qx, qy, qz, cull, key);
+      st.nodes(32 * W);
 #pragma unroll
       for (int j = W - 1; j > 0; --j) {
         if (key[j] != ~0ull) {
@@ -268,17 +306,24 @@ __device__ __forceinline__ bool for_each_leaf(const WideBVH<W> &bvh, float qx, f
     expand = true;
   }
 }
+template <int STACK, int W, typename LeafFn>
+__device__ __forceinline__ bool for_each_leaf(const WideBVH<W> &bvh, float qx, float qy, float qz, float sqrMax,
+                                              LeafFn &&leaf, uint32_t root = 0) {
+  NoStats st;
+  return for_each_leaf<STACK>(bvh, qx, qy, qz, sqrMax, leaf, root, st);
+}
 
 // Descends along the nearest child to one leaf and calls prim(primID) for
 // its prims (the wide counterpart of visit_nearest_leaf).
-template <int W, typename PrimFn>
+template <int W, typename PrimFn, typename Stats>
 __device__ __forceinline__ void visit_nearest_leaf(const WideBVH<W> &bvh, float qx, float qy, float qz,
-                                                   PrimFn &&prim) {
+                                                   PrimFn &&prim, uint32_t root, Stats &st) {
   if (bvh.numNodes == 0) return;
-  uint32_t node = 0;
+  uint32_t node = root;
   for (int guard = 0; guard < (1 << 20); guard++) {
     uint64_t key[W];
     wide_child_keys<W>(bvh, node, qx, qy, qz, INFINITY, key);
+    st.nodes(32 * W);
     if (key[0] == ~0ull) return;
     const uint32_t payload = (uint32_t)key[0];
     if (payload & 0x80000000u) {
@@ -290,16 +335,28 @@ __device__ __forceinline__ void visit_nearest_leaf(const WideBVH<W> &bvh, float 
     node = payload;
   }
 }
+template <int W, typename PrimFn>
+__device__ __forceinline__ void visit_nearest_leaf(const WideBVH<W> &bvh, float qx, float qy, float qz,
+                                                   PrimFn &&prim, uint32_t root = 0) {
+  NoStats st;
+  visit_nearest_leaf(bvh, qx, qy, qz, prim, root, st);
+}
 
-template <int STACK, int W, typename PrimFn>
+template <int STACK, int W, typename PrimFn, typename Stats>
 __device__ __forceinline__ bool for_each_prim(const WideBVH<W> &bvh, float qx, float qy, float qz, float sqrMax,
-                                              PrimFn &&prim) {
+                                              PrimFn &&prim, uint32_t root, Stats &st) {
   auto leaf = [&](const uint32_t *ids, uint32_t cnt) -> float {
     float r = INFINITY;
     for (uint32_t i = 0; i < cnt; i++) r = fminf(r, prim(ids[i]));
     return r;
   };
-  return for_each_leaf<STACK>(bvh, qx, qy, qz, sqrMax, leaf);
+  return for_each_leaf<STACK>(bvh, qx, qy, qz, sqrMax, leaf, root, st);
+}
+template <int STACK, int W, typename PrimFn>
+__device__ __forceinline__ bool for_each_prim(const WideBVH<W> &bvh, float qx, float qy, float qz, float sqrMax,
+                                              PrimFn &&prim, uint32_t root = 0) {
+  NoStats st;
+  return for_each_prim<STACK>(bvh, qx, qy, qz, sqrMax, prim, root, st);
 }
 
 }  // namespace cubql_dev

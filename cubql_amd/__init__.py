@@ -67,6 +67,11 @@ class BuildConfig(C.Structure):
         self.buildMethod = SAH
         return self
 
+    def enableELH(self):
+        """BuildConfig::enableELH (bvh.h:33): the edge-length-heuristic builder."""
+        self.buildMethod = ELH
+        return self
+
 
 _MALLOC_CB = C.CFUNCTYPE(C.c_int, C.POINTER(C.c_void_p), C.c_size_t, C.c_void_p, C.c_void_p)
 _FREE_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p)

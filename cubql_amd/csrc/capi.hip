@@ -42,6 +42,11 @@ extern "C" int cubql_gpu_builder_f3(cubql_bvh3f *bvh, const cubql_box3f *d_boxes
       return sm_build(bvh, d_boxes, numBoxes, leaf_threshold(cfg), s, mem);
     case CUBQL_SAH:
       return sah_build(bvh, d_boxes, numBoxes, leaf_threshold(cfg), leaf_cap(cfg), 16, s, mem);
+    case CUBQL_ELH:
+      // elhBuilder (elh_builder.h:437-557): 8 bins, edge-length cost; a
+      // makeLeafThreshold of 0 and of 1 give the same tree there (a 1-prim
+      // node has no split with two non-empty sides)
+      return sah_build(bvh, d_boxes, numBoxes, leaf_threshold(cfg), leaf_cap(cfg), 8, s, mem, true);
     default:
       set_error("cubql_gpu_builder_f3: buildMethod %d not supported", cfg->buildMethod);
       return CUBQL_ERR_UNSUPPORTED;

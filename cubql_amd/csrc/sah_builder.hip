@@ -88,6 +88,8 @@ struct Ctx {
   uint32_t T;    // effective leaf threshold (<= cap)
   uint32_t cap;  // leaf size cap
   int nb;        // bins per dimension
+  float binEps;  // bin denominator epsilon: 1e-20 (SAH) or 0 (ELH)
+  int elh;       // box measure: surface area (0) or edge-length sum (1)
   Item *items[2];
   uint32_t *primIDs;
   Node *nodes;   // level-phase nodes (capacity 2*n+2)
@@ -126,19 +128,21 @@ __device__ __forceinline__ uint32_t item_id(const Item &it) { return __float_as_
 // bin of a centre coordinate (sah_builder.h:181-187, 301-308).  Clamping in
 // float equals max(0,min(nb-1,int(rel*nb))) for in-range values and gives
 // the saturating conversion's answer for NaN and infinities.
-__device__ __forceinline__ int sah_bin(float c, float lo, float hi, int nb) {
+__device__ __forceinline__ int sah_bin(float c, float lo, float hi, int nb, float eps) {
   if (!(hi > lo)) return 0;
-  const float rel = (c - lo) / (hi - lo + 1e-20f);
+  const float rel = (c - lo) / (hi - lo + eps);
   const float f = rel * (float)nb;
   if (f >= (float)(nb - 1)) return nb - 1;
   if (f > 0.f) return (int)f;
   return 0;
 }
 
-// surfaceArea(box3f) (math/box.h:199-205)
-__device__ __forceinline__ float surface_area(const float (&lo)[3], const float (&hi)[3]) {
+// surfaceArea(box3f) (math/box.h:199-205), or for the edge-length
+// heuristic edgeLengths(box) (elh_builder.h:66-72): the extents summed from
+// 0 in dimension order
+__device__ __forceinline__ float box_measure(const float (&lo)[3], const float (&hi)[3], bool elh) {
   const float sx = hi[0] - lo[0], sy = hi[1] - lo[1], sz = hi[2] - lo[2];
-  return sx * sy + sx * sz + sy * sz;
+  return elh ? (sx + sy) + sz : sx * sy + sx * sz + sy * sz;
 }
 
 __device__ __forceinline__ int32_t bin_init(int w) {
@@ -146,11 +150,11 @@ __device__ __forceinline__ int32_t bin_init(int w) {
   return k < 3 ? enc_f(INFINITY) : (k < 6 ? enc_f(-INFINITY) : 0);
 }
 
-__device__ __forceinline__ void bin_add(int32_t *bins, int nb, const Item &it, const float (&clo)[3],
+__device__ __forceinline__ void bin_add(int32_t *bins, int nb, float eps, const Item &it, const float (&clo)[3],
                                         const float (&chi)[3]) {
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    int32_t *w = bins + (d * nb + sah_bin(item_center(it, d), clo[d], chi[d], nb)) * kBinWords;
+    int32_t *w = bins + (d * nb + sah_bin(item_center(it, d), clo[d], chi[d], nb, eps)) * kBinWords;
     atomicMin(&w[0], enc_f(it.a.x));
     atomicMin(&w[1], enc_f(it.a.y));
     atomicMin(&w[2], enc_f(it.a.z));
@@ -167,7 +171,7 @@ __device__ __forceinline__ void bin_add(int32_t *bins, int nb, const Item &it, c
 // Returns +inf for invalid candidates; *leafTerm = area * count of all bins.
 template <bool CROSS32>
 __device__ __forceinline__ float split_cost(const int32_t *bins, int nb, int d, uint32_t bl, bool inBin,
-                                            int rev, float *leafTerm) {
+                                            int rev, bool elh, float *leafTerm) {
   const bool hasLeft = bl >= 1u && bl <= (uint32_t)nb;
   const uint32_t binL = hasLeft ? bl - 1u : 0u;
   const uint32_t binR = inBin ? (uint32_t)nb - 1u - bl : 0u;
@@ -183,10 +187,10 @@ __device__ __forceinline__ float split_cost(const int32_t *bins, int nb, int d, 
   }
   const uint32_t lCnt = (uint32_t)dpp_scan_i<2, CROSS32>(hasLeft ? wl[6] : 0);
   const uint32_t rCntRev = (uint32_t)dpp_scan_i<2, CROSS32>(inBin ? wr[6] : 0);
-  const float rArea = __shfl(surface_area(rlo, rhi), rev, kWave);
+  const float rArea = __shfl(box_measure(rlo, rhi, elh), rev, kWave);
   const uint32_t rCnt = (uint32_t)__shfl((int)rCntRev, rev, kWave);
   *leafTerm = rArea * (float)rCnt;
-  const float c = surface_area(llo, lhi) * (float)lCnt + rArea * (float)rCnt;
+  const float c = box_measure(llo, lhi, elh) * (float)lCnt + rArea * (float)rCnt;
   return (inBin && lCnt > 0 && rCnt > 0 && c == c) ? c : INFINITY;
 }
 
@@ -196,7 +200,7 @@ __device__ __forceinline__ float split_cost(const int32_t *bins, int nb, int d, 
 // first minimum in (dim, bin) order is found by wave min + ballot.  With at
 // most 32 bins dims 0 and 1 share one pass (lanes 0-31, 32-63).
 // n = prims of the node.  Returns the split dim, kModeLeaf or kModeMiddle.
-__device__ int sah_decide_wave(const int32_t *bins, int nb, uint32_t n, uint32_t cap, int &splitBin) {
+__device__ int sah_decide_wave(const int32_t *bins, int nb, uint32_t n, uint32_t cap, bool elh, int &splitBin) {
   const uint32_t lane = lane_id();
   float cost[3];     // candidate sets in (dim, bin) order; lane order within each
   int passDim[3];    // first dim of each candidate set (-1: unused)
@@ -206,8 +210,9 @@ __device__ int sah_decide_wave(const int32_t *bins, int nb, uint32_t n, uint32_t
     const uint32_t bl = lane & 31u, half = lane >> 5;
     float leaf0, leaf1;
     cost[0] = split_cost<false>(bins, nb, (int)half, bl, bl < (uint32_t)nb, (int)(32 * half + nb - 1 - bl) & 63,
-                                &leaf0);
-    cost[1] = split_cost<false>(bins, nb, 2, bl, half == 0 && bl < (uint32_t)nb, (int)(nb - 1 - bl) & 63, &leaf1);
+                                elh, &leaf0);
+    cost[1] = split_cost<false>(bins, nb, 2, bl, half == 0 && bl < (uint32_t)nb, (int)(nb - 1 - bl) & 63, elh,
+                                &leaf1);
     passDim[0] = 0; passDim[1] = 2; passDim[2] = -1;
     nSets = 2;
     leafCost = __shfl(leaf0, 0, kWave);
@@ -216,7 +221,7 @@ __device__ int sah_decide_wave(const int32_t *bins, int nb, uint32_t n, uint32_t
     float lt[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      cost[d] = split_cost<true>(bins, nb, d, lane, inBin, inBin ? nb - 1 - (int)lane : (int)lane, &lt[d]);
+      cost[d] = split_cost<true>(bins, nb, d, lane, inBin, inBin ? nb - 1 - (int)lane : (int)lane, elh, &lt[d]);
       passDim[d] = d;
     }
     nSets = 3;
@@ -449,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void k_bin(Ctx c, int level) {
     seg_cb(g, clo, chi);
     const uint32_t b0 = g.begin + (bt - c.binStart[s]) * kBinTile;
     const uint32_t e0 = min(b0 + (uint32_t)kBinTile, g.end);
-    for (uint32_t i = b0 + threadIdx.x; i < e0; i += kBlock) bin_add(sBins, nb, src[i], clo, chi);
+    for (uint32_t i = b0 + threadIdx.x; i < e0; i += kBlock) bin_add(sBins, nb, c.binEps, src[i], clo, chi);
     sync_block_lds();
     int32_t *out = c.partial + (size_t)bt * W;
     for (int w = threadIdx.x; w < W; w += kBlock) out[w] = sBins[w];
@@ -482,7 +487,7 @@ __global__ __launch_bounds__(kBlock) void k_select(Ctx c, int level) {
       const Seg g = c.segs[par][s];
       const uint32_t n = g.end - g.begin;
       int bin = 0;
-      const int mode = sah_decide_wave(sBins, nb, n, c.cap, bin);
+      const int mode = sah_decide_wave(sBins, nb, n, c.cap, c.elh != 0, bin);
       // node box = union of the dimension-0 bins
       float lo[3], hi[3];
       const bool inBin = lane_id() < (uint32_t)nb;
@@ -555,7 +560,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
       left[j] = false;
       if (in[j]) {
         it[j] = src[i];
-        left[j] = mode == kModeMiddle ? i < mid : sah_bin(item_center(it[j], mode), lo, hi, nb) < bin;
+        left[j] = mode == kModeMiddle ? i < mid : sah_bin(item_center(it[j], mode), lo, hi, nb, c.binEps) < bin;
       }
     }
     uint32_t rank[kItems];
@@ -735,9 +740,9 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
     if (n > c.T) {
       for (int w = lane; w < 3 * nb * kBinWords; w += kWave) sBins[w] = bin_init(w);
       sync_wave_lds();
-      for (uint32_t i = b + lane; i < e; i += kWave) bin_add(sBins, nb, sIt[i], clo, chi);
+      for (uint32_t i = b + lane; i < e; i += kWave) bin_add(sBins, nb, c.binEps, sIt[i], clo, chi);
       sync_wave_lds();
-      mode = sah_decide_wave(sBins, nb, n, c.cap, bin);
+      mode = sah_decide_wave(sBins, nb, n, c.cap, c.elh != 0, bin);
     }
     if (mode == kModeLeaf) {
       for (uint32_t i = b + lane; i < e; i += kWave) c.primIDs[gb + i] = item_id(sIt[i]);
@@ -762,7 +767,7 @@ __device__ void small_segment(const Ctx &c, const SmallSeg &sg, uint32_t segIdx,
       bool isL = false;
       if (act[k]) {
         v[k] = sIt[i];
-        isL = mode == kModeMiddle ? i < midPos : sah_bin(item_center(v[k], mode), lo, hi, nb) < bin;
+        isL = mode == kModeMiddle ? i < midPos : sah_bin(item_center(v[k], mode), lo, hi, nb, c.binEps) < bin;
       }
       const uint64_t mL = __ballot(isL);
       const uint64_t mR = __ballot(act[k] && !isL);
@@ -841,7 +846,7 @@ static uint32_t *sah_host_words() {
 }
 
 int sah_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, uint32_t cap,
-              int numBins, hipStream_t s, const cubql_mem_resource *mem) {
+              int numBins, hipStream_t s, const cubql_mem_resource *mem, bool elh) {
   using namespace sah;
   if (numBins < 2 || numBins > kMaxBins) {
     set_error("sahBuilder: numBins must be in [2, %d] (got %d)", kMaxBins, numBins);
@@ -854,6 +859,8 @@ int sah_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T
   c.cap = std::max<uint32_t>(1u, std::min<uint32_t>(cap, 65535u));
   c.T = std::min(std::max<uint32_t>(T, 1u), c.cap);
   c.nb = numBins;
+  c.binEps = elh ? 0.f : 1e-20f;
+  c.elh = elh ? 1 : 0;
   const uint32_t numBlocks = div_up(n, kTile);
   c.maxLarge = n / (kSmall + 1) + 2;
   const uint32_t maxTiles = div_up(n, kTile) + c.maxLarge + 1;

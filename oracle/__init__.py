@@ -57,6 +57,8 @@ def lib():
             "orc_sm_build": (I, [V, I, I, V, V, V, V]),
             "orc_sah_build": (I, [V, I, I, I, I, V, V, V, V]),
             "orc_sah_decide": (I, [V, I, I, I, V]),
+            "orc_elh_build": (I, [V, I, I, I, V, V, V, V]),
+            "orc_elh_decide": (I, [V, I, I, V]),
             "orc_rebin_build": (I, [V, I, I, V, V, V, V]),
             "orc_refit": (None, [V, I, V, V]),
             "orc_check_tree": (I, [V, I, V, I, V, I, I, V, V]),
@@ -163,6 +165,26 @@ def sah_build(boxes: np.ndarray, leaf_threshold: int, num_bins: int = 16,
     lib().orc_sah_build(_p(b), n, leaf_threshold, num_bins, leaf_cap, _p(nodes), _p(ids),
                         C.byref(nn), C.byref(npr))
     return Tree(nodes[: nn.value], ids[: npr.value])
+
+
+def elh_build(boxes: np.ndarray, leaf_threshold: int, leaf_cap: int = 1 << 15) -> Tree:
+    """Binned edge-length-heuristic tree (restates cuda/elh_builder.h; see oracle.h)."""
+    b = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    n = b.shape[0]
+    nodes = np.zeros(2 * n + 2, dtype=NODE_DTYPE)
+    ids = np.zeros(max(n, 1), dtype=np.uint32)
+    nn = C.c_int()
+    npr = C.c_int()
+    lib().orc_elh_build(_p(b), n, leaf_threshold, leaf_cap, _p(nodes), _p(ids), C.byref(nn), C.byref(npr))
+    return Tree(nodes[: nn.value], ids[: npr.value])
+
+
+def elh_decide(boxes: np.ndarray, allow_leaf: bool = True):
+    """(dim, bin) of one ELH decision over all boxes; dim -1 = leaf, -2 = no split."""
+    b = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    bn = C.c_int(0)
+    d = lib().orc_elh_decide(_p(b), b.shape[0], 1 if allow_leaf else 0, C.byref(bn))
+    return d, bn.value
 
 
 def rebin_build(boxes: np.ndarray, leaf_threshold: int) -> Tree:

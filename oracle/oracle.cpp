@@ -114,12 +114,30 @@ inline float surface_area(const orc_box3f &b) {
   return sx * sy + sx * sz + sy * sz;
 }
 
-// bin of a centre coordinate (sah_builder.h:181-187, 301-308).  The float
-// clamp equals max(0,min(NB-1,int(rel*NB))) for every finite in-range value
-// and gives the saturating GPU conversion's answer for NaN and +-inf.
-inline int sah_bin(float c, float lo, float hi, int nb) {
+// edgeLengths(box) (elh_builder.h:66-72, its active #if branch): the sum of
+// the box's extents, accumulated from 0 in dimension order.
+inline float edge_lengths(const orc_box3f &b) {
+  float sum = 0.f;
+  for (int d = 0; d < 3; d++) sum += b.hi[d] - b.lo[d];
+  return sum;
+}
+
+// The two binned heuristics share everything but the bin denominator's
+// epsilon and the box measure: SAH (sah_builder.h) uses (hi-lo+1e-20f) and
+// surfaceArea, ELH (elh_builder.h) (hi-lo) and edgeLengths.
+enum { kHeurSAH = 0, kHeurELH = 1 };
+inline float heur_eps(int h) { return h == kHeurELH ? 0.f : 1e-20f; }
+inline float heur_measure(int h, const orc_box3f &b) {
+  return h == kHeurELH ? edge_lengths(b) : surface_area(b);
+}
+
+// bin of a centre coordinate (sah_builder.h:181-187, 301-308; elh_builder.h:
+// 202-213, 329-336).  The float clamp equals max(0,min(NB-1,int(rel*NB)))
+// for every finite in-range value and gives the saturating GPU conversion's
+// answer for NaN and +-inf.  (hi-lo)+0 is hi-lo exactly.
+inline int sah_bin(float c, float lo, float hi, int nb, float eps = 1e-20f) {
   if (!(hi > lo)) return 0;
-  const float rel = (c - lo) / (hi - lo + 1e-20f);
+  const float rel = (c - lo) / (hi - lo + eps);
   const float f = rel * (float)nb;
   if (f >= (float)(nb - 1)) return nb - 1;
   if (f > 0.f) return (int)f;
@@ -128,9 +146,10 @@ inline int sah_bin(float c, float lo, float hi, int nb) {
 
 struct SahBin { orc_box3f b; int count; };
 
-// evaluateSAH (sah_builder.h:62-106); allowLeaf=false drops the leaf
-// candidate.  Returns the split dim or -1.
-int sah_evaluate(const std::vector<SahBin> &bins, int nb, bool allowLeaf, int &splitBin) {
+// evaluateSAH (sah_builder.h:62-106) or evaluateELH (elh_builder.h:89-135),
+// the same loop over a different box measure; allowLeaf=false drops the
+// leaf candidate.  Returns the split dim or -1.
+int sah_evaluate(const std::vector<SahBin> &bins, int nb, bool allowLeaf, int &splitBin, int heur = kHeurSAH) {
   float bestCost = INFINITY;
   int splitDim = -1;
   std::vector<float> rAreas(nb);
@@ -141,7 +160,7 @@ int sah_evaluate(const std::vector<SahBin> &bins, int nb, bool allowLeaf, int &s
     for (int b = nb - 1; b >= 0; --b) {
       grow(box, db[b].b);
       rCount += db[b].count;
-      rAreas[b] = surface_area(box);
+      rAreas[b] = heur_measure(heur, box);
     }
     const float leafCost = rAreas[0] * (float)rCount;
     if (allowLeaf && leafCost < bestCost) {
@@ -152,7 +171,7 @@ int sah_evaluate(const std::vector<SahBin> &bins, int nb, bool allowLeaf, int &s
     int lCount = 0;
     for (int b = 0; b < nb; b++) {
       const float rArea = rAreas[b];
-      const float lArea = surface_area(box);
+      const float lArea = heur_measure(heur, box);
       if (lCount > 0 && rCount > 0) {
         const float cost = lArea * (float)lCount + rArea * (float)rCount;
         if (cost < bestCost) {
@@ -172,6 +191,7 @@ int sah_evaluate(const std::vector<SahBin> &bins, int nb, bool allowLeaf, int &s
 struct SahCtx {
   const orc_box3f *boxes;
   int T, nb, cap;
+  int heur = kHeurSAH;
   std::vector<int> ids, alt;
   std::vector<Topo> topo;
 };
@@ -193,14 +213,14 @@ int sah_decide_range(const SahCtx &c, const int *ids, int begin, int end, float 
     for (int d = 0; d < 3; d++) {
       // binPrims uses 0.5f*(lo+hi), equal to center() for every input
       const float pd = 0.5f * (pb.lo[d] + pb.hi[d]);
-      SahBin &bn = bins[d * c.nb + sah_bin(pd, clo[d], chi[d], c.nb)];
+      SahBin &bn = bins[d * c.nb + sah_bin(pd, clo[d], chi[d], c.nb, heur_eps(c.heur))];
       grow(bn.b, pb);
       bn.count++;
     }
   }
-  int dim = sah_evaluate(bins, c.nb, true, splitBin);
+  int dim = sah_evaluate(bins, c.nb, true, splitBin, c.heur);
   if (dim < 0 && n > c.cap) {
-    dim = sah_evaluate(bins, c.nb, false, splitBin);
+    dim = sah_evaluate(bins, c.nb, false, splitBin, c.heur);
     if (dim < 0) dim = -2;
   }
   return dim;
@@ -223,7 +243,7 @@ void sah_rec(SahCtx &c, int nodeID, int begin, int end) {
     } else {
       const orc_box3f &pb = c.boxes[pid];
       const float pd = 0.5f * (pb.lo[dim] + pb.hi[dim]);
-      left = sah_bin(pd, clo[dim], chi[dim], c.nb) < bin;  // side = (bin >= splitBin)
+      left = sah_bin(pd, clo[dim], chi[dim], c.nb, heur_eps(c.heur)) < bin;  // side = (bin >= splitBin)
     }
     if (left) c.alt[begin + nl++] = pid;
     else c.alt[end - ++nr] = pid;
@@ -351,10 +371,10 @@ int orc_sm_build(const orc_box3f *boxes, int n, int leafThreshold,
   return 0;
 }
 
-int orc_sah_build(const orc_box3f *boxes, int n, int leafThreshold, int numBins,
-                  int leafCap, orc_node3f *nodes, uint32_t *primIDs, int *numNodes,
-                  int *numPrims) {
+static int binned_build(int heur, const orc_box3f *boxes, int n, int leafThreshold, int numBins, int leafCap,
+                        orc_node3f *nodes, uint32_t *primIDs, int *numNodes, int *numPrims) {
   SahCtx c;
+  c.heur = heur;
   c.boxes = boxes;
   c.T = std::max(1, leafThreshold);
   c.nb = numBins;
@@ -375,8 +395,20 @@ int orc_sah_build(const orc_box3f *boxes, int n, int leafThreshold, int numBins,
   return 0;
 }
 
-int orc_sah_decide(const orc_box3f *boxes, int n, int numBins, int allowLeaf, int *bin) {
+int orc_sah_build(const orc_box3f *boxes, int n, int leafThreshold, int numBins,
+                  int leafCap, orc_node3f *nodes, uint32_t *primIDs, int *numNodes,
+                  int *numPrims) {
+  return binned_build(kHeurSAH, boxes, n, leafThreshold, numBins, leafCap, nodes, primIDs, numNodes, numPrims);
+}
+
+int orc_elh_build(const orc_box3f *boxes, int n, int leafThreshold, int leafCap, orc_node3f *nodes,
+                  uint32_t *primIDs, int *numNodes, int *numPrims) {
+  return binned_build(kHeurELH, boxes, n, leafThreshold, 8, leafCap, nodes, primIDs, numNodes, numPrims);
+}
+
+static int binned_decide(int heur, const orc_box3f *boxes, int n, int numBins, int allowLeaf, int *bin) {
   SahCtx c;
+  c.heur = heur;
   c.boxes = boxes;
   c.nb = numBins;
   c.cap = allowLeaf ? n : 0;
@@ -385,6 +417,14 @@ int orc_sah_decide(const orc_box3f *boxes, int n, int numBins, int allowLeaf, in
   std::iota(ids.begin(), ids.end(), 0);
   float clo[3], chi[3];
   return sah_decide_range(c, ids.data(), 0, n, clo, chi, *bin);
+}
+
+int orc_sah_decide(const orc_box3f *boxes, int n, int numBins, int allowLeaf, int *bin) {
+  return binned_decide(kHeurSAH, boxes, n, numBins, allowLeaf, bin);
+}
+
+int orc_elh_decide(const orc_box3f *boxes, int n, int allowLeaf, int *bin) {
+  return binned_decide(kHeurELH, boxes, n, 8, allowLeaf, bin);
 }
 
 void orc_refit(orc_node3f *nodes, int numNodes, const uint32_t *primIDs,

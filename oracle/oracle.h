@@ -56,6 +56,16 @@ int orc_sah_build(const orc_box3f *boxes, int n, int leafThreshold, int numBins,
    split has both sides non-empty (the middle-split fallback). */
 int orc_sah_decide(const orc_box3f *boxes, int n, int numBins, int allowLeaf, int *bin);
 
+/* Binned edge-length-heuristic builder; restates cuda/elh_builder.h:
+   56-135 (ELHBins, edgeLengths, evaluateELH), 184-230 (binPrims), 258-350
+   (selectSplits/updatePrims) as orc_sah_build does for SAH: 8 bins,
+   rel = (c-lo)/(hi-lo) (no epsilon), the box measure is the sum of the
+   extents, the same leaf-first strict-'<' evaluation and the same leaf-cap
+   rule (the reference's ELH builder has no cap either). */
+int orc_elh_build(const orc_box3f *boxes, int n, int leafThreshold, int leafCap, orc_node3f *nodes,
+                  uint32_t *primIDs, int *numNodes, int *numPrims);
+int orc_elh_decide(const orc_box3f *boxes, int n, int allowLeaf, int *bin);
+
 /* Robust rebinning Morton builder; cuda/rebinMortonBuilder.h:595-1476 with
    NO_BLOCK_PHASE (its default, :22).  30-bit keys over the valid prims
    (ascending primID before a stable sort), then per node: leaf if size <=

@@ -44,37 +44,90 @@ def cpu_info():
     return cores, model
 
 
-def cpu_baseline(n_sample: int, nq_sample: int, budget_s: float):
-    """Reference CPU path (cpu::spatialMedian + host findClosest), 1 core."""
+def _ref_method(fn, min_reps, min_s):
+    """The reference's timing method (testing/bvh/buildBench.h:115-133): one
+    warm-up, then repeat until at least `min_reps` runs and `min_s` seconds."""
+    fn()
+    times = []
+    t0 = time.perf_counter()
+    while len(times) < min_reps or time.perf_counter() - t0 < min_s:
+        times.append(fn())
+    return times
+
+
+def cpu_baseline(n_sample: int, nq_sample: int, min_s: float):
+    """The reference's own CPU path (cpu::spatialMedian + host findClosest /
+    findKNN, compiled from its sources into oracle/_ref), 1 core, on bounded
+    samples of the benchmark workloads; plus the same per-query function on
+    all host cores from a thread pool of ours (ctypes drops the GIL)."""
+    import concurrent.futures as cf
+
     import numpy as np
 
     import oracle
     from cubql_amd import synth
 
-    P = synth.lcg_uniform(2, n_sample)
-    Q = synth.lcg_uniform(3, nq_sample)
-    boxes = oracle.points_to_boxes(P)
     kind = "reference" if oracle.ref_available() else "port"
-    if kind == "reference":
-        R = oracle.ref()
-        build_once = lambda: R.ref_time_cpu_build(oracle._p(boxes), n_sample, 8)  # noqa: E731
-    else:
-        def build_once():
+
+    def timed_build(boxes, n):
+        if kind == "reference":
+            return lambda: oracle.ref().ref_time_cpu_build(oracle._p(boxes), n, 8)
+
+        def once():
             t = time.perf_counter()
             oracle.sm_build(boxes, 8)
             return time.perf_counter() - t
-    build_once()  # warm-up (buildBench.h:115-117)
-    times = []
-    t_start = time.perf_counter()
-    while len(times) < 3 or (time.perf_counter() - t_start < budget_s and len(times) < 10):
-        times.append(build_once())
-    tree = oracle.ref_sm_build(boxes, 8) if kind == "reference" else oracle.sm_build(boxes, 8)
+        return once
+
+    def tree_of(boxes):
+        return oracle.ref_sm_build(boxes, 8) if kind == "reference" else oracle.sm_build(boxes, 8)
+
+    def fcp(tree, P, Q):
+        return oracle.ref_fcp(tree, P, Q) if kind == "reference" else oracle.fcp(tree, P, Q, lowest_id=False)
+
+    def knn(tree, P, Q, k):
+        return oracle.ref_knn(tree, P, Q, k) if kind == "reference" else oracle.knn(tree, P, Q, k)
+
+    def rate(fn, units):
+        t = time.perf_counter()
+        fn()
+        return units / (time.perf_counter() - t) / 1e6
+
+    # 10M uniform points (seed 2): the first n_sample of the same stream
+    P = synth.lcg_uniform(2, n_sample)
+    Q = synth.lcg_uniform(3, nq_sample)
+    boxes = oracle.points_to_boxes(P)
+    times = _ref_method(timed_build(boxes, n_sample), 10, min_s)
+    tree = tree_of(boxes)
+    fcp_rate = rate(lambda: fcp(tree, P, Q), nq_sample)
+    nk = nq_sample // 4
+    knn_rate = rate(lambda: knn(tree, P, Q[:nk], 8), nk)
+    # all host cores: our thread pool over the unchanged per-query function
+    threads = min(os.cpu_count() or 1, 16)
+    Qa = synth.lcg_uniform(3, nq_sample * 4)
+    chunks = np.array_split(Qa, threads * 4)
     t = time.perf_counter()
-    if kind == "reference":
-        oracle.ref_fcp(tree, P, Q)
-    else:
-        oracle.fcp(tree, P, Q, lowest_id=False)
-    tq = time.perf_counter() - t
+    with cf.ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda q: fcp(tree, P, q), chunks))
+    fcp_all = len(Qa) / (time.perf_counter() - t) / 1e6
+
+    # 100K points (seed 0), 100K queries in [-0.1, 1.1]^3 (seed 1): full size
+    P1 = synth.lcg_uniform(0, 100_000)
+    Q1 = synth.lcg_uniform(1, 100_000, -0.1, 1.1)
+    b1 = oracle.points_to_boxes(P1)
+    t1 = _ref_method(timed_build(b1, len(P1)), 10, min(min_s, 3.0))
+    tree1 = tree_of(b1)
+    c1_fcp = rate(lambda: fcp(tree1, P1, Q1), len(Q1))
+
+    # 2M-point Gaussian mixture: spatial median (the reference has no CPU
+    # Morton builder) + findKNN k=16 over a query subset
+    P4 = synth.gaussian_mixture(2_000_000, seed=41)
+    Q4 = synth.gaussian_mixture(2_000_000, seed=42)[: nq_sample // 4]
+    b4 = oracle.points_to_boxes(P4)
+    t4 = _ref_method(timed_build(b4, len(P4)), 3, min(min_s, 3.0))
+    tree4 = tree_of(b4)
+    c4_knn = rate(lambda: knn(tree4, P4, Q4, 16), len(Q4))
+
     cores, model = cpu_info()
     mean = float(np.mean(times))
     return {
@@ -83,10 +136,25 @@ def cpu_baseline(n_sample: int, nq_sample: int, budget_s: float):
         "cores": 1,
         "kind": kind,
         "sample": (f"cpu::spatialMedian (serial, leaf<=8) over the first {n_sample:,} points of the "
-                   f"same seed-2 stream, {len(times)} timed builds after 1 warm-up "
-                   f"(mean {mean*1e3:.1f} ms, median {float(np.median(times))*1e3:.1f} ms); "
-                   f"host findClosest over {nq_sample:,} seed-3 queries on that tree"),
-        "fcp_mqueries_s": nq_sample / tq / 1e6,
+                   f"10M seed-2 stream, {len(times)} timed builds after 1 warm-up, "
+                   f"{sum(times):.1f} s (mean {mean*1e3:.1f} ms, median "
+                   f"{float(np.median(times))*1e3:.1f} ms); host findClosest over {nq_sample:,} and "
+                   f"findKNN k=8 over {nk:,} seed-3 queries on that tree"),
+        "method": "1 warm-up, then >= 10 runs and >= %.0f s (testing/bvh/buildBench.h:115-133)" % min_s,
+        "fcp_mqueries_s": fcp_rate,
+        "knn8_mqueries_s": knn_rate,
+        "fcp_mqueries_s_all_cores": fcp_all,
+        "all_cores_threads": threads,
+        "all_cores_note": (f"the reference's host findClosest over {len(Qa):,} queries from a "
+                           f"{threads}-thread pool of ours"),
+        "c1": {"build_ms_mean": float(np.mean(t1)) * 1e3, "build_ms_median": float(np.median(t1)) * 1e3,
+               "builds": len(t1), "fcp_mqueries_s": c1_fcp,
+               "sample": ("full size: 100,000 points (seed 0), 100,000 queries (seed 1); >= 10 builds, "
+                          ">= 3 s")},
+        "c4": {"build_ms_mean": float(np.mean(t4)) * 1e3, "builds": len(t4), "knn16_mqueries_s": c4_knn,
+               "sample": (f"cpu::spatialMedian over the full 2M-point mixture (>= 3 builds, >= 3 s); "
+                          f"findKNN k=16 over "
+                          f"{len(Q4):,} mixture queries")},
         "compiler": "g++ -O3 -std=c++17 -ffp-contract=off (x86-64 default -march)",
         "host_cpus": cores,
         "host_model": model,
@@ -192,6 +260,35 @@ def _time_cuda(fn, reps):
     return e0.elapsed_time(e1) / reps
 
 
+def _kernel_ms(cq, name, fn, reps=3):
+    """Average duration of one launch of kernel `name`, from the library's
+    HIP events on the launch stream (cubql_profile_*)."""
+    import torch
+    cq.profile.reset()
+    cq.profile.enable(True)
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    cq.profile.enable(False)
+    st = cq.profile.query(name)
+    return st["ms"] / st["launches"] if st and st["launches"] else None
+
+
+def query_roofline(cq, stats, kernel_ms, io_bytes, prim_bytes, kernel):
+    """Visited-bytes roofline of a query kernel: per query, `io_bytes` of
+    query in and result out, the node bytes the traversal loaded (64 per
+    child pair expanded, 32 per node popped) and `prim_bytes` per prim
+    tested, all counted by an instrumented run of the same queries."""
+    nq = stats["queries"]
+    total = nq * io_bytes + stats["node_bytes"] + stats["prims"] * prim_bytes
+    gbs = total / (kernel_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": kernel, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS, "avg_launch_ms": kernel_ms, "bytes_per_launch": total,
+            "bytes_per_query": total / nq, "node_bytes_per_query": stats["node_bytes"] / nq,
+            "prims_per_query": stats["prims"] / nq,
+            "bytes_model": f"{io_bytes} B query+result, node bytes loaded, {prim_bytes} B per prim tested"}
+
+
 def bench_triangles(cq, dev, reps: int):
     """1,310,720-triangle noisy icosphere: SAH build (32 bins, leaves <= 4),
     4M closest-triangle queries forward and backward."""
@@ -222,6 +319,11 @@ def bench_triangles(cq, dev, reps: int):
     def fwd():
         out["r"] = cq.triangles.findClosest(bvh, dF, dV, dQ, want_points=False)
     fwd_ms = _time_cuda(fwd, reps)
+    kms = _kernel_ms(cq, "fcp_triangles", fwd)
+    stats = cq.query_stats(bvh, dQ, tris=dF, verts=dV)
+    # query 12 B in, id + distance 8 B out; per triangle 4 B primID,
+    # 12 B index triple and 3 x 12 B vertices
+    roof = query_roofline(cq, stats, kms, 20, 52, "fcp_triangles") if kms else None
     ids = out["r"][0]
     g = torch.ones(len(Q), device=dev)
     gq = torch.empty_like(dQ)
@@ -232,7 +334,7 @@ def bench_triangles(cq, dev, reps: int):
            "num_nodes": int(bvh.numNodes),
            "fcp_ms": fwd_ms, "fcp_mqueries_s": len(Q) / fwd_ms / 1e3,
            "backward_ms": bwd_ms, "fwd_bwd_mqueries_s": len(Q) / (fwd_ms + bwd_ms) / 1e3,
-           "sah_kernels": kern}
+           "sah_kernels": kern, "fcp_roofline": roof}
     bvh.free()
     return res
 
@@ -254,16 +356,24 @@ def bench_mixture(cq, dev, reps: int):
     def fwd():
         out["r"] = cq.points.findKNN(bvh, P, Q, 16)
     knn_ms = _time_cuda(fwd, reps)
+    kms = _kernel_ms(cq, "knn_points", fwd)
+    stats = cq.query_stats(bvh, Q, data=P, k=16)
+    # query 12 B in, 16 x (id + distance) + count out; 16 B per point tested
+    roof = query_roofline(cq, stats, kms, 12 + 8 * 16 + 4, 16, "knn_points<16>") if kms else None
     ids = out["r"][0]
     g = torch.full((len(Q), 16), 1.0 / (len(Q) * 16), device=dev)
     gq = torch.empty_like(Q)
     gp = torch.zeros_like(P)
     bwd_ms = _time_cuda(lambda: cq.points_backward(P, Q, ids, g, gq, gp), reps)
+    # the same without the scattered point gradient: what the scatter costs
+    bwd_q_ms = _time_cuda(lambda: cq.points_backward(P, Q, ids, g, gq, None), reps)
     res = {"points": int(len(P)), "queries": int(len(Q)), "k": 16, "leaf": 8,
+           "backward_query_grad_only_ms": bwd_q_ms,
            "rebin_build_ms": build_ms, "rebin_build_mprims_s": len(P) / build_ms / 1e3,
            "num_nodes": int(bvh.numNodes),
            "knn_ms": knn_ms, "knn_mqueries_s": len(Q) / knn_ms / 1e3,
-           "backward_ms": bwd_ms, "fwd_bwd_mqueries_s": len(Q) / (knn_ms + bwd_ms) / 1e3}
+           "backward_ms": bwd_ms, "fwd_bwd_mqueries_s": len(Q) / (knn_ms + bwd_ms) / 1e3,
+           "knn_roofline": roof}
     bvh.free()
     return res
 
@@ -289,6 +399,100 @@ def build_model_bytes(cq, bvh, n):
     return 64 * n + 32 * m + 20 * open_prims, open_prims
 
 
+def build_kernel_roofline(cq, boxes, cfg, mem, kern):
+    """Per-kernel timing of the spatial-median build (library HIP events on
+    the build stream) and the roofline of its dominant kernel, sm::k_pair."""
+    import torch
+    cq.profile.reset()
+    cq.profile.enable(True)
+    for _ in range(3):
+        cq.gpuBuilder(boxes, cfg, mem=mem).free()
+    torch.cuda.synchronize()
+    cq.profile.enable(False)
+    for name in ["sm_partition", "sm_bounds", "sm_small"]:
+        st = cq.profile.query(name)
+        if st and st["launches"]:
+            kern[name] = {"launches_per_build": st["launches"] / 3,
+                          "avg_launch_ms": st["ms"] / st["launches"],
+                          "ms_per_build": st["ms"] / 3,
+                          "bytes_per_launch": st["bytes"] / st["launches"],
+                          "gbs": st["bytes"] / (st["ms"] * 1e-3) / 1e9}
+    dom = kern.get("sm_partition")
+    if not dom:
+        return None
+    return {"bound": "hbm", "kernel": "sm_partition",
+            "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": partition_traffic(),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "bytes_model": "32 B per prim of the pass (16-byte item read + written; "
+                           "one pass places the prims two tree levels down)",
+            "avg_launch_ms": dom["avg_launch_ms"],
+            "bytes_per_launch": dom["bytes_per_launch"]}
+
+
+def bench_radix(cq, boxes, cfg, mem, N, steps, kern):
+    """Morton builders: 30-bit keys per the workload, 63-bit reference default."""
+    import torch
+    radix = {}
+    for name, fn in [("radix30", cq.cuda.radix30Builder), ("radix63", cq.cuda.radixBuilder)]:
+        fn(boxes, cfg, mem=mem).free()
+        torch.cuda.synchronize()
+        r0 = time.perf_counter()
+        reps = max(3, steps // 2)
+        for _ in range(reps):
+            fn(boxes, cfg, mem=mem).free()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - r0) / reps * 1e3
+        radix[name] = {"ms_per_build": ms, "mprims_s": N / (ms * 1e-3) / 1e6}
+    print(f"[bench] radix30 {radix['radix30']['mprims_s']:.1f} Mprims/s, "
+          f"radix63 {radix['radix63']['mprims_s']:.1f} Mprims/s", file=sys.stderr, flush=True)
+    cq.profile.reset()
+    cq.profile.enable(True)
+    for _ in range(2):
+        cq.cuda.radix30Builder(boxes, cfg, mem=mem).free()
+    torch.cuda.synchronize()
+    cq.profile.enable(False)
+    for name in ["radix_keys", "radix_onesweep", "radix_hierarchy", "refit"]:
+        st = cq.profile.query(name)
+        if st and st["launches"]:
+            kern["radix30:" + name] = {"launches_per_build": st["launches"] / 2,
+                                       "avg_launch_ms": st["ms"] / st["launches"],
+                                       "ms_per_build": st["ms"] / 2,
+                                       "gbs": st["bytes"] / (st["ms"] * 1e-3) / 1e9}
+    return radix
+
+
+def bench_queries(cq, P, Q, boxes, cfg, mem, N, nq, nk, build_ms):
+    """10M closest-point and 1M kNN (k=8) queries on one tree, their
+    visited-bytes rooflines, and the whole build against HBM peak."""
+    import torch
+    bvh = cq.gpuBuilder(boxes, cfg, mem=mem)
+    Qf, Qk = Q[:nq], Q[:nk]
+    fcp_ms = _time_cuda(lambda: cq.points.findClosest(bvh, P, Qf), 3)
+    knn_ms = _time_cuda(lambda: cq.points.findKNN(bvh, P, Qk, 8), 3)
+    fcp_kms = _kernel_ms(cq, "fcp_points", lambda: cq.points.findClosest(bvh, P, Qf))
+    knn_kms = _kernel_ms(cq, "knn_points", lambda: cq.points.findKNN(bvh, P, Qk, 8))
+    query_rooflines = {
+        # query 12 B in, id + distance 8 B out; 4 B primID + 12 B point per prim
+        "fcp": query_roofline(cq, cq.query_stats(bvh, Qf, data=P, k=0), fcp_kms, 20, 16, "fcp_points"),
+        "knn8": query_roofline(cq, cq.query_stats(bvh, Qk, data=P, k=8), knn_kms, 12 + 8 * 8 + 4, 16,
+                               "knn_points<8>"),
+    }
+    num_nodes = bvh.numNodes
+    model_bytes, open_prims = build_model_bytes(cq, bvh, N)
+    bvh.free()
+    torch.cuda.synchronize()
+    build_roofline = None
+    if build_ms:
+        # the whole build against HBM peak, with the per-level byte model
+        gbs = model_bytes / (build_ms * 1e-3) / 1e9
+        build_roofline = {"bound": "hbm", "bytes": model_bytes, "sum_open_prims": open_prims,
+                          "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                          "bytes_model": "64 B/prim (box read, primID, centroid, leaf box) + 32 B/node "
+                                         "+ 20 B per prim and level in an open node"}
+    return fcp_ms, knn_ms, query_rooflines, num_nodes, build_roofline
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -299,11 +503,15 @@ def main():
     ap.add_argument("--knn-queries", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--cpu-min-s", type=float, default=10.0)
     ap.add_argument("--batch-clouds", type=int, default=4096)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--plain-budget-s", type=float, default=40.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the triangle and mixture legs")
+    ap.add_argument("--legs", default="build,radix,queries,triangles,mixture,batch,cpu",
+                    help="comma list of legs to run (profiling runs pick a few); value and "
+                         "ms_per_step come from the build leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 device copies of 1 GiB (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -338,126 +546,53 @@ def main():
     torch.cuda.synchronize()
     print(f"[bench] data ready: {N:,} points", file=sys.stderr, flush=True)
 
-    for _ in range(args.warmup):
-        cq.gpuBuilder(boxes, cfg, mem=mem).free()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        bvh = cq.gpuBuilder(boxes, cfg, mem=mem)
-        bvh.free()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) / args.steps * 1e3
-    value = N / (ms_per_step * 1e-3) / 1e6
-    print(f"[bench] build: {ms_per_step:.3f} ms/build, {value:.1f} Mprims/s", file=sys.stderr,
-          flush=True)
-
-    # ---- per-kernel roofline of the dominant build kernel (events in the library)
-    cq.profile.reset()
-    cq.profile.enable(True)
-    for _ in range(3):
-        cq.gpuBuilder(boxes, cfg, mem=mem).free()
-    torch.cuda.synchronize()
-    cq.profile.enable(False)
-    part = cq.profile.query("sm_partition")
-    bnds = cq.profile.query("sm_bounds")
-    small = cq.profile.query("sm_small")
-    kern = {}
-    for name, st in [("sm_partition", part), ("sm_bounds", bnds), ("sm_small", small)]:
-        if st and st["launches"]:
-            kern[name] = {"launches_per_build": st["launches"] / 3,
-                          "avg_launch_ms": st["ms"] / st["launches"],
-                          "ms_per_build": st["ms"] / 3,
-                          "bytes_per_launch": st["bytes"] / st["launches"],
-                          "gbs": st["bytes"] / (st["ms"] * 1e-3) / 1e9}
-    dom = kern.get("sm_partition")
-    roofline = None
-    if dom:
-        roofline = {"bound": "hbm", "kernel": "sm_partition",
-                    "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": partition_traffic(),
-                    "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-                    "bytes_model": "32 B per prim of the pass (16-byte item read + written; "
-                                   "one pass places the prims two tree levels down)",
-                    "avg_launch_ms": dom["avg_launch_ms"],
-                    "bytes_per_launch": dom["bytes_per_launch"]}
-
-    # ---- Morton builders (30-bit keys per the workload, 63-bit reference default)
-    radix = {}
-    for name, fn in [("radix30", cq.cuda.radix30Builder), ("radix63", cq.cuda.radixBuilder)]:
-        fn(boxes, cfg, mem=mem).free()
+    legs = set(args.legs.split(","))
+    if args.no_extra:
+        legs -= {"triangles", "mixture"}
+    if args.no_batch:
+        legs.discard("batch")
+    if args.no_cpu_baseline:
+        legs.discard("cpu")
+    value = ms_per_step = roofline = None
+    kern, radix = {}, {}
+    if "build" in legs:
+        for _ in range(args.warmup):
+            cq.gpuBuilder(boxes, cfg, mem=mem).free()
         torch.cuda.synchronize()
-        r0 = time.perf_counter()
-        reps = max(3, args.steps // 2)
-        for _ in range(reps):
-            fn(boxes, cfg, mem=mem).free()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            bvh = cq.gpuBuilder(boxes, cfg, mem=mem)
+            bvh.free()
         torch.cuda.synchronize()
-        ms = (time.perf_counter() - r0) / reps * 1e3
-        radix[name] = {"ms_per_build": ms, "mprims_s": N / (ms * 1e-3) / 1e6}
-    print(f"[bench] radix30 {radix['radix30']['mprims_s']:.1f} Mprims/s, "
-          f"radix63 {radix['radix63']['mprims_s']:.1f} Mprims/s", file=sys.stderr, flush=True)
-    cq.profile.reset()
-    cq.profile.enable(True)
-    for _ in range(2):
-        cq.cuda.radix30Builder(boxes, cfg, mem=mem).free()
-    torch.cuda.synchronize()
-    cq.profile.enable(False)
-    for name in ["radix_keys", "radix_onesweep", "radix_hierarchy", "refit"]:
-        st = cq.profile.query(name)
-        if st and st["launches"]:
-            kern["radix30:" + name] = {"launches_per_build": st["launches"] / 2,
-                                       "avg_launch_ms": st["ms"] / st["launches"],
-                                       "ms_per_build": st["ms"] / 2,
-                                       "gbs": st["bytes"] / (st["ms"] * 1e-3) / 1e9}
-
-    # ---- queries on one tree
-    bvh = cq.gpuBuilder(boxes, cfg, mem=mem)
-    nq = args.fcp_queries
-    Qf = Q[:nq]
-    cq.points.findClosest(bvh, P, Qf)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    reps = 3
-    e0.record()
-    for _ in range(reps):
-        cq.points.findClosest(bvh, P, Qf)
-    e1.record()
-    torch.cuda.synchronize()
-    fcp_ms = e0.elapsed_time(e1) / reps
-    nk = args.knn_queries
-    Qk = Q[:nk]
-    cq.points.findKNN(bvh, P, Qk, 8)
-    e0.record()
-    for _ in range(reps):
-        cq.points.findKNN(bvh, P, Qk, 8)
-    e1.record()
-    torch.cuda.synchronize()
-    knn_ms = e0.elapsed_time(e1) / reps
-    num_nodes = bvh.numNodes
-    model_bytes, open_prims = build_model_bytes(cq, bvh, N)
-    bvh.free()
-    # the whole build against HBM peak, with the per-level byte model
-    build_roofline = {"bound": "hbm", "bytes": model_bytes, "sum_open_prims": open_prims,
-                      "achieved": model_bytes / (ms_per_step * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                      "unit": "GB/s", "frac": model_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                      "bytes_model": "64 B/prim (box read, primID, centroid, leaf box) + 32 B/node "
-                                     "+ 20 B per prim and level in an open node"}
-    print(f"[bench] fcp {nq/fcp_ms/1e3:.1f} Mq/s, knn {nk/knn_ms/1e3:.1f} Mq/s", file=sys.stderr,
-          flush=True)
-
-    tris = mixture = None
-    if not args.no_extra:
+        t1 = time.perf_counter()
+        ms_per_step = (t1 - t0) / args.steps * 1e3
+        value = N / (ms_per_step * 1e-3) / 1e6
+        print(f"[bench] build: {ms_per_step:.3f} ms/build, {value:.1f} Mprims/s", file=sys.stderr,
+              flush=True)
+        roofline = build_kernel_roofline(cq, boxes, cfg, mem, kern)
+    if "radix" in legs:
+        radix = bench_radix(cq, boxes, cfg, mem, N, args.steps, kern)
+    tris = mixture = batched = cpu = None
+    query_rooflines = None
+    fcp_ms = knn_ms = None
+    nq, nk = args.fcp_queries, args.knn_queries
+    num_nodes = build_roofline = None
+    if "queries" in legs:
+        (fcp_ms, knn_ms, query_rooflines, num_nodes,
+         build_roofline) = bench_queries(cq, P, Q, boxes, cfg, mem, N, nq, nk, ms_per_step)
+        print(f"[bench] fcp {nq/fcp_ms/1e3:.1f} Mq/s, knn {nk/knn_ms/1e3:.1f} Mq/s", file=sys.stderr,
+              flush=True)
+    if "triangles" in legs:
         tris = bench_triangles(cq, dev, reps=5)
         print(f"[bench] triangles: SAH build {tris['sah_build_ms']:.2f} ms, fcp "
               f"{tris['fcp_mqueries_s']:.1f} Mq/s, fwd+bwd {tris['fwd_bwd_mqueries_s']:.1f} Mq/s",
               file=sys.stderr, flush=True)
+    if "mixture" in legs:
         mixture = bench_mixture(cq, dev, reps=5)
         print(f"[bench] mixture: rebin build {mixture['rebin_build_ms']:.2f} ms, knn16 "
               f"{mixture['knn_mqueries_s']:.1f} Mq/s, fwd+bwd {mixture['fwd_bwd_mqueries_s']:.1f} Mq/s",
               file=sys.stderr, flush=True)
-
-    batched = None
-    if not args.no_batch:
+    if "batch" in legs:
         batched = bench_batched(cq, dev, args.batch_clouds, args.plain_budget_s, not args.no_tune)
         print(f"[bench] batched chamfer: {batched['batched_ms']:.2f} ms, plain {batched['plain_ms']:.1f} ms"
               f" ({batched['speedup_vs_plain']:.1f}x)", file=sys.stderr, flush=True)
@@ -472,9 +607,8 @@ def main():
     pcie_h2d_gbs = hbox.numel() * 4 / (h2d_ms * 1e-3) / 1e9
     del hbox, dbox
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(min(args.cpu_sample, N), 200_000, budget_s=12.0)
+    if "cpu" in legs:
+        cpu = cpu_baseline(min(args.cpu_sample, N), 200_000, min_s=args.cpu_min_s)
         print(f"[bench] cpu baseline {cpu['value']:.2f} Mprims/s", file=sys.stderr, flush=True)
 
     line = {
@@ -487,7 +621,7 @@ def main():
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": value / PUBLISHED_MPRIMS,
+        "vs_baseline": value / PUBLISHED_MPRIMS if value else None,
         "dtype": "f32",
         "data": "synthetic: reference LCG<8> uniform generator, points seed 2, queries seed 3",
         "config": {"workload": f"gpuBuilder (spatial median, leaf<=8) over {N:,} uniform float3 "
@@ -495,8 +629,9 @@ def main():
                                "queries uniform in [0,1]^3",
                    "points": N, "leaf_threshold": 8, "num_nodes": num_nodes},
         "radix_builds": radix,
-        "fcp_mqueries_s": nq / fcp_ms / 1e3,
-        "knn_mqueries_s": nk / knn_ms / 1e3,
+        "fcp_mqueries_s": nq / fcp_ms / 1e3 if fcp_ms else None,
+        "knn_mqueries_s": nk / knn_ms / 1e3 if knn_ms else None,
+        "query_rooflines": query_rooflines,
         "pcie_h2d_gbs": pcie_h2d_gbs,
         "triangle_mesh": tris,
         "gaussian_mixture": mixture,

@@ -36,6 +36,18 @@ __host__ __device__ inline uint64_t pack_oc(uint64_t off, uint64_t cnt) {
 __host__ __device__ inline uint64_t oc_offset(uint64_t oc) { return oc & kOffsetMask; }
 __host__ __device__ inline uint32_t oc_count(uint64_t oc) { return (uint32_t)(oc >> 48); }
 
+// The dispatcher hands workgroup b of a launch to XCD b % 8, and each of
+// the 8 XCDs has its own L2.  For nb workgroups over work stored in a
+// locality order (Morton-sorted queries), xcd_chunk(b, nb) is the index of
+// the piece of work workgroup b takes such that each XCD gets one contiguous
+// run of pieces (a bijection on [0, nb)), so an XCD's L2 serves one region
+// of the tree instead of all of it.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_chunk(uint32_t b, uint32_t nb) {
+  const uint32_t q = nb / kXcds, r = nb % kXcds, x = b % kXcds, j = b / kXcds;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + j;
+}
+
 // ---------------------------------------------------------------- errors
 void set_error(const char *fmt, ...);
 void clear_error();

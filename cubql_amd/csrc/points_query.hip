@@ -60,7 +60,8 @@ __global__ __launch_bounds__(kBlock) void k_fcp(BVH bvh, const float *points,
                                                 const float *queries, uint32_t nq, float sqrMax,
                                                 int32_t *outIds, float *outD, Overflow *ovf,
                                                 uint32_t *ovfList, const uint32_t *order) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  // Morton-ordered queries: one contiguous run of them per XCD
+  const uint32_t i = (order ? xcd_chunk(blockIdx.x, gridDim.x) : blockIdx.x) * kBlock + threadIdx.x;
   if (i >= nq) return;
   const uint32_t qi = order ? order[i] : i;
   float3 q = load_pt(queries, qi);
@@ -105,6 +106,11 @@ __device__ __forceinline__ bool knn_one(const BVH &bvh, const float *points, flo
 #pragma unroll
   for (int j = 0; j < K; j++) { ids[j] = -1; ds[j] = INFINITY; }
   float cull = sqrMax;
+  // (ds[k-1], ids[k-1]) once the list is full, kept in registers so that a
+  // rejected candidate (most of them) costs two compares, not a K-wide
+  // select to read the runtime-indexed k-th entry
+  float kthD = INFINITY;
+  int32_t kthId = 0x7fffffff;
   auto prim = [&](uint32_t pid) -> float {
     st.prim();
     float3 p = load_pt(points, pid);
@@ -113,11 +119,7 @@ __device__ __forceinline__ bool knn_one(const BVH &bvh, const float *points, flo
     if (!(d < sqrMax)) return cull;
     if (found == k) {
       // reject unless (d,id) < (ds[k-1], ids[k-1])
-      float dl = ds[0];
-      int32_t il = ids[0];
-#pragma unroll
-      for (int j = 0; j < K; j++) if (j == k - 1) { dl = ds[j]; il = ids[j]; }
-      if (d > dl || (d == dl && id > il)) return cull;
+      if (d > kthD || (d == kthD && id > kthId)) return cull;
     } else {
       found++;
     }
@@ -141,7 +143,12 @@ __device__ __forceinline__ bool knn_one(const BVH &bvh, const float *points, flo
     }
     if (found == k) {
 #pragma unroll
-      for (int j = 0; j < K; j++) if (j == k - 1) cull = ds[j];
+      for (int j = 0; j < K; j++)
+        if (j == k - 1) {
+          kthD = ds[j];
+          kthId = ids[j];
+        }
+      cull = kthD;
     }
     return cull;
   };
@@ -154,7 +161,8 @@ __global__ __launch_bounds__(kBlock) void k_knn(BVH bvh, const float *points,
                                                 float sqrMax, int32_t *outIds, float *outD,
                                                 int32_t *outFound, Overflow *ovf,
                                                 uint32_t *ovfList, const uint32_t *order) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  // Morton-ordered queries: one contiguous run of them per XCD
+  const uint32_t i = (order ? xcd_chunk(blockIdx.x, gridDim.x) : blockIdx.x) * kBlock + threadIdx.x;
   if (i >= nq) return;
   const uint32_t qi = order ? order[i] : i;
   float3 q = load_pt(queries, qi);

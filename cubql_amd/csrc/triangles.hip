@@ -76,7 +76,8 @@ __global__ __launch_bounds__(kBlock) void k_fcp_tris(cubql_bvh3f bvh, const int3
                                                      const float *queries, uint32_t nq, float sqrMax,
                                                      int32_t *ids, float *d2, float *cp, Overflow *ovf,
                                                      uint32_t *ovfList, const uint32_t *order) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  // Morton-ordered queries: one contiguous run of them per XCD
+  const uint32_t i = (order ? xcd_chunk(blockIdx.x, gridDim.x) : blockIdx.x) * kBlock + threadIdx.x;
   if (i >= nq) return;
   const uint32_t qi = order ? order[i] : i;
   const F3 q = ld3(queries, qi);

@@ -452,7 +452,7 @@ def bench_radix(cq, boxes, cfg, mem, N, steps, kern):
         cq.cuda.radix30Builder(boxes, cfg, mem=mem).free()
     torch.cuda.synchronize()
     cq.profile.enable(False)
-    for name in ["radix_keys", "radix_onesweep", "radix_hierarchy", "refit"]:
+    for name in ["radix_keys", "radix_onesweep", "radix_hierarchy", "radix_emit", "refit"]:
         st = cq.profile.query(name)
         if st and st["launches"]:
             kern["radix30:" + name] = {"launches_per_build": st["launches"] / 2,

@@ -92,14 +92,26 @@ __device__ inline void lookback_publish(uint64_t *status, uint32_t t, uint32_t k
 // none is.  Tiles are claimed in ticket order, so every tile waited on is
 // held by a running workgroup that publishes without waiting on later
 // tiles.  Returns the node's sum over its earlier tiles (in every lane).
+// Bound on the polls of one look-back (each one a status load and a short
+// sleep, ~1 s in all).  Every tile waited on is held by a running
+// workgroup, so a legitimate wait lasts microseconds; a wait past the bound
+// means a tile was never published.  It then sets *err (when given) and
+// goes on with the counts already seen: the prefix is at most the true one,
+// so the caller's writes stay inside its node's range, and the host, which
+// reads *err at its next synchronisation, fails the build instead of
+// hanging the queue.
+constexpr uint32_t kSpinLimit = 1u << 20;
+
 __device__ inline uint32_t lookback_exclusive_wave(uint64_t *status, uint32_t t, uint32_t k, uint32_t tag,
-                                                   uint32_t local, bool published = false) {
+                                                   uint32_t local, bool published = false,
+                                                   uint32_t *err = nullptr) {
   const uint32_t lane = lane_id();
   if (!published) lookback_publish(status, t, k, tag, local);
   if (k == 0) return 0;
   uint32_t prefix = 0;
   uint32_t first = t - 1;  // tile read by lane 0
   uint32_t remaining = k;  // earlier tiles of this node: t-k .. t-1
+  uint32_t spins = 0;
   while (true) {
     const bool inRange = lane < remaining;
     uint64_t w = 0;
@@ -109,14 +121,19 @@ __device__ inline uint32_t lookback_exclusive_wave(uint64_t *status, uint32_t t,
     const uint64_t notReady = __ballot(inRange && flag == 0);
     const uint32_t stop = incl ? (uint32_t)__builtin_ctzll(incl) : 63u;  // lanes 0..stop count
     const uint64_t upto = stop >= 63 ? ~0ull : ((1ull << (stop + 1)) - 1);
-    if (notReady & upto) {
+    const bool timedOut = ++spins > kSpinLimit;
+    if ((notReady & upto) && !timedOut) {
       __builtin_amdgcn_s_sleep(1);
       continue;
     }
-    uint32_t v = (inRange && lane <= stop) ? (uint32_t)w : 0u;
+    uint32_t v = (inRange && lane <= stop && flag != 0) ? (uint32_t)w : 0u;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
     prefix += v;
+    if (timedOut && (notReady & upto)) {
+      if (lane == 0 && err) atomicOr(err, 1u);
+      break;
+    }
     if (incl) break;
     first -= 64;
     remaining -= 64;
@@ -142,7 +159,7 @@ constexpr uint32_t kCountScanTile = 1024 * kCountScanPer;
 template <int BLOCK>
 __device__ inline void count_scan_part(const uint32_t *count, uint32_t *out, uint32_t n, uint32_t base,
                                        uint64_t *status, uint32_t tag, uint32_t blk, bool last, uint32_t *total,
-                                       uint32_t *smem, uint32_t sub) {
+                                       uint32_t *smem, uint32_t sub, uint32_t *err = nullptr) {
   const uint32_t i0 = blk * (BLOCK * kCountScanPer) + threadIdx.x * kCountScanPer;
   uint32_t v[kCountScanPer], sum = 0;
 #pragma unroll
@@ -153,7 +170,7 @@ __device__ inline void count_scan_part(const uint32_t *count, uint32_t *out, uin
   uint32_t tot;
   const uint32_t ex = block_excl_scan<BLOCK, true>(sum, smem, &tot);
   if (threadIdx.x < kWave) {
-    const uint32_t prefix = lookback_exclusive_wave(status, blk, blk, tag, tot);
+    const uint32_t prefix = lookback_exclusive_wave(status, blk, blk, tag, tot, false, err);
     if (threadIdx.x == 0) {
       smem[BLOCK / kWave + 1] = prefix;
       if (last && total) *total = base + prefix + tot;
@@ -169,9 +186,9 @@ __device__ inline void count_scan_part(const uint32_t *count, uint32_t *out, uin
 }
 __device__ inline void count_scan_tile(const uint32_t *count, uint32_t *out, uint32_t n, uint32_t base,
                                        uint64_t *status, uint32_t tag, uint32_t *total, uint32_t *smem,
-                                       uint32_t sub = 1) {
+                                       uint32_t sub = 1, uint32_t *err = nullptr) {
   count_scan_part<1024>(count, out, n, base, status, tag, blockIdx.x, blockIdx.x == gridDim.x - 1, total, smem,
-                        sub);
+                        sub, err);
 }
 
 // block-wide min/max reduction of 12 floats (left lo/hi, right lo/hi)

@@ -43,6 +43,7 @@ constexpr int kMaxLevels = 1024;
 enum {
   C_NUMVALID = 0, C_NODECOUNT = 1, C_JOBS = 2, C_JOBPRIMS = 3, C_SMALL = 4, C_CURJOBS = 5,
   C_NODESOUT = 6,  // node count after the thread phase
+  C_SPIN = 7,      // a look-back wait exceeded its bound (build_common.h)
   C_LEVEL0 = 8, C_LSTART0 = C_LEVEL0 + kMaxLevels + 4
 };
 
@@ -233,51 +234,6 @@ __device__ __forceinline__ uint32_t find_split(const K *keys, uint32_t b, uint32
   return lo;
 }
 
-// One tree level: each open node becomes a leaf or splits in two.
-template <typename K>
-__global__ __launch_bounds__(kBlock) void k_rlevel(RCtx c, const K *keys, int level) {
-  __shared__ uint32_t ss[kBlock / kWave + 1];
-  __shared__ uint32_t sNode, sOpen;
-  const uint32_t nOpen = c.ctr[C_LEVEL0 + level];
-  const uint32_t start = c.ctr[C_LSTART0 + level];
-  const uint32_t *open = c.levels + start;
-  uint32_t *next = c.levels + start + nOpen;  // the next level follows this one
-  if (blockIdx.x == 0 && threadIdx.x == 0) c.ctr[C_LSTART0 + level + 1] = start + nOpen;
-  for (uint32_t b0 = blockIdx.x * kBlock; b0 < nOpen; b0 += gridDim.x * kBlock) {
-    const uint32_t i = b0 + threadIdx.x;
-    uint32_t node = 0, split = 0;
-    uint2 r = make_uint2(0, 0);
-    bool doSplit = false;
-    if (i < nOpen) {
-      node = open[i];
-      r = c.ranges[node];
-      const uint32_t size = r.y - r.x;
-      if (size > c.T) {
-        split = find_split<K>(keys, r.x, r.y);
-        doSplit = true;
-      } else {
-        c.nodes[node].oc = pack_oc(r.x, size);
-      }
-    }
-    uint32_t tot;
-    const uint32_t ex = block_excl_scan<kBlock>(doSplit ? 1u : 0u, ss, &tot);
-    if (threadIdx.x == 0 && tot) {
-      sNode = atomicAdd(&c.ctr[C_NODECOUNT], 2 * tot);
-      sOpen = atomicAdd(&c.ctr[C_LEVEL0 + level + 1], 2 * tot);
-    }
-    __syncthreads();
-    if (doSplit) {
-      const uint32_t child = sNode + 2 * ex;
-      c.nodes[node].oc = pack_oc(child, 0);
-      c.ranges[child] = make_uint2(r.x, split);
-      c.ranges[child + 1] = make_uint2(split, r.y);
-      next[sOpen + 2 * ex] = child;
-      next[sOpen + 2 * ex + 1] = child + 1;
-    }
-    __syncthreads();
-  }
-}
-
 __global__ void k_rinit_levels(uint32_t *ctr) {
   for (int i = threadIdx.x; i < kMaxLevels + 2; i += blockDim.x) ctr[C_LEVEL0 + 1 + i] = 0;
   if (threadIdx.x < C_LEVEL0 - C_JOBS) ctr[C_JOBS + threadIdx.x] = 0;
@@ -336,6 +292,435 @@ static void refit_launch(const RCtx &c, Node *nodes, const uint32_t *primIDs, in
   if (top > 0) k_rrefit_top<<<1, 1024, 0, s>>>(c, nodes, primIDs, top);
 }
 
+// ------------------------------------------------ one-pass hierarchy
+// The tree findSplit/createNodes build top down (radix.h:446-570) is the
+// binary radix tree of the sorted keys with equal keys told apart by their
+// position, cut at ranges of <= T prims (leaves) and with every range of
+// equal keys split in the middle instead (radix.h:455-465).  Each internal
+// node of the radix tree is found independently from the keys around its
+// position (Karras, "Maximizing parallelism in the construction of BVHs,
+// octrees, and k-d trees", 2012): its range [first, last] and split gamma
+// (left = [first, gamma], right = [gamma + 1, last]).  A node whose range
+// has more than T prims and two different keys is a split node of the
+// reference's tree; a child range of equal keys with more than T prims
+// roots a middle-split subtree of mid_inner(size) inner nodes.
+//
+// Numbering: node 0 is the root and node 1 unused (radix.h:585-586); the
+// children of the s-th split node (in position order) are nodes 2 + 2s and
+// 3 + 2s; the pairs of the middle-split subtrees follow all of those, each
+// subtree's in level order.  Boxes are refitted bottom up in buckets: the
+// inner nodes of middle-split subtrees first, deepest first, then the split
+// nodes in decreasing order of the common key-prefix length of their range,
+// which grows strictly from a node to its split children, so every child
+// is final before its parent.
+
+// The sorted keys as one workgroup reads them: positions [lo, hi) from an
+// LDS copy, the rest from global memory.  delta(i, j) is the common prefix
+// length of keys i and j, positions breaking ties of equal keys (-1 out of
+// range).
+template <typename K>
+struct KeyWindow {
+  const K *g;
+  const K *w;
+  int64_t lo, hi, n;
+  __device__ __forceinline__ K at(int64_t j) const { return (j >= lo && j < hi) ? w[j - lo] : g[j]; }
+  __device__ __forceinline__ int delta(int64_t i, int64_t j) const {
+    if (j < 0 || j >= n) return -1;
+    const K a = at(i), b = at(j);
+    constexpr int kBits = 8 * (int)sizeof(K);
+    if (a == b) return kBits + __clz((uint32_t)(i ^ j));
+    return sizeof(K) == 8 ? __clzll((unsigned long long)(a ^ b)) : __clz((uint32_t)(a ^ b));
+  }
+};
+
+template <typename K>
+__device__ __forceinline__ int key_prefix(K a, K b) {
+  return sizeof(K) == 8 ? __clzll((unsigned long long)(a ^ b)) : __clz((uint32_t)(a ^ b));
+}
+
+// Shape of the middle-split subtree of m prims with leaves <= T (each inner
+// node of s prims has children of floor(s/2) and ceil(s/2)): the sizes at
+// depth d are q = m >> d or q + 1 (m mod 2^d of them), so every node above
+// depth D = min{d : q_d <= T} is inner, at depth D exactly the nodes of
+// T + 1 prims are (when q_D == T), and deeper nodes are leaves.
+struct MidShape {
+  uint32_t D, q, r, T;  // depth D, q_D, m mod 2^D, the leaf threshold
+  // inner nodes at depth t
+  __host__ __device__ uint32_t at(uint32_t t) const { return t < D ? 1u << t : (t == D && q == T) ? r : 0u; }
+};
+
+__host__ __device__ inline MidShape mid_shape(uint32_t m, uint32_t T) {
+  MidShape ms{0u, m, 0u, T};
+  while (ms.q > T) ms.q = m >> ++ms.D;
+  ms.r = ms.D ? m & ((1u << ms.D) - 1u) : 0u;
+  return ms;
+}
+
+// inner nodes of the middle-split subtree of m prims
+__host__ __device__ inline uint32_t mid_inner(uint32_t m, uint32_t T) {
+  const MidShape ms = mid_shape(m, T);
+  return (ms.D ? (1u << ms.D) - 1u : 0u) + (ms.q == T ? ms.r : 0u);
+}
+
+// Refit buckets: split nodes by the prefix length of their range (< key
+// bits), inner nodes of middle-split subtrees by key bits + 1 + depth; the
+// last bin counts the middle-split subtree roots.
+constexpr int kDeltaBins = 128;
+constexpr int kBinEqRoots = kDeltaBins - 1;
+// radix-tree nodes per workgroup of the hierarchy kernels (kBlock threads,
+// kKPer nodes each, strided); bucket slots are assigned per such block
+constexpr int kKPer = 8;
+constexpr uint32_t kKBlock = kBlock * kKPer;
+
+struct KNode {
+  uint32_t first, last, gamma, pad;
+};
+
+// An equal-key range of more than T prims: the root of a middle-split
+// subtree at node `pos`, prims [begin, begin + size), inner-node pairs from
+// node `pairBase` on.
+struct EqRoot {
+  uint32_t pos, begin, size, pairBase;
+};
+
+// boxes in sorted prim order: sorted[i] = boxes[primIDs[i]], one thread
+// per prim (the one random gather of the build, at full memory parallelism)
+__global__ __launch_bounds__(kBlock) void k_gather_sorted(const cubql_box3f *boxes, const uint32_t *primIDs,
+                                                          const uint32_t *nPtr, cubql_box3f *sorted) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= *nPtr) return;
+  const float2 *src = reinterpret_cast<const float2 *>(boxes + primIDs[i]);
+  float2 *dst = reinterpret_cast<float2 *>(sorted + i);
+  const float2 a = src[0], b = src[1], c = src[2];
+  dst[0] = a;
+  dst[1] = b;
+  dst[2] = c;
+}
+
+// box of prims [b, b + m) (sorted boxes)
+__device__ __forceinline__ void range_box(const cubql_box3f *sorted, uint32_t b, uint32_t m, Node &nd) {
+  for (int d = 0; d < 3; d++) { nd.lo[d] = INFINITY; nd.hi[d] = -INFINITY; }
+  for (uint32_t j = 0; j < m; j++) {
+    const cubql_box3f bx = load_box(sorted, b + j);
+    for (int d = 0; d < 3; d++) { nd.lo[d] = fminf(nd.lo[d], bx.lower[d]); nd.hi[d] = fmaxf(nd.hi[d], bx.upper[d]); }
+  }
+}
+
+// Radix-tree nodes: range, split, whether a node is a split node, the inner
+// nodes of the middle-split subtrees under it; the workgroup's bucket
+// counts go to row-per-bin blockHist[bin * numKBlocks + block].
+// keys a workgroup of k_karras stages in LDS: its nodes and kKWin on each
+// side (the searches of all but the few nodes with long ranges stay inside)
+constexpr int kKWin = 1024;
+
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_karras(const K *keys, const uint32_t *nPtr, uint32_t T, KNode *kn,
+                                                   uint32_t *splitFlag, uint32_t *eqPairs, uint32_t *blockHist,
+                                                   uint32_t numKBlocks) {
+  constexpr int kBits = 8 * (int)sizeof(K);
+  __shared__ uint32_t sh[kDeltaBins];
+  __shared__ K sKeys[kKBlock + 2 * kKWin];
+  const uint32_t n = *nPtr;
+  const int64_t N = n;
+  const int64_t b0 = (int64_t)blockIdx.x * kKBlock;
+  KeyWindow<K> kw{keys, sKeys, b0 - kKWin, b0 + kKBlock + kKWin, N};
+  if (kw.lo < 0) kw.lo = 0;
+  if (kw.hi > N) kw.hi = N;
+  for (int b = threadIdx.x; b < kDeltaBins; b += kBlock) sh[b] = 0;
+  for (int64_t j = kw.lo + threadIdx.x; j < kw.hi; j += kBlock) sKeys[j - kw.lo] = keys[j];
+  __syncthreads();
+  for (int r = 0; r < kKPer; r++) {
+    const int64_t i = b0 + r * kBlock + threadIdx.x;
+    if (i >= N - 1) break;
+    const int d = kw.delta(i, i + 1) - kw.delta(i, i - 1) > 0 ? 1 : -1;
+    const int dmin = kw.delta(i, i - d);
+    int64_t lmax = 2;
+    while (kw.delta(i, i + lmax * d) > dmin) lmax *= 2;
+    int64_t l = 0;
+    for (int64_t t = lmax / 2; t >= 1; t /= 2)
+      if (kw.delta(i, i + (l + t) * d) > dmin) l += t;
+    const int64_t j = i + l * d;
+    const int dn = kw.delta(i, j);
+    int64_t sp = 0;
+    for (int sh2 = 1;; sh2++) {
+      const int64_t t = (l + ((int64_t)1 << sh2) - 1) >> sh2;  // ceil(l / 2^sh2)
+      if (kw.delta(i, i + (sp + t) * d) > dn) sp += t;
+      if (t == 1) break;
+    }
+    const int64_t g = i + sp * d + (d < 0 ? -1 : 0);
+    const uint32_t first = (uint32_t)(i < j ? i : j), last = (uint32_t)(i < j ? j : i);
+    kn[i] = KNode{first, last, (uint32_t)g, 0u};
+    const bool split = last - first + 1 > T && kw.at(first) != kw.at(last);
+    splitFlag[i] = split ? 1u : 0u;
+    uint32_t eq = 0;
+    if (split) {
+      // (split nodes: the keys of the range differ, so dn < key bits)
+      atomicAdd(&sh[dn], 1u);
+      const uint32_t cb[2] = {first, (uint32_t)g + 1}, ce[2] = {(uint32_t)g, last};
+      for (int side = 0; side < 2; side++) {
+        const uint32_t m = ce[side] - cb[side] + 1;
+        if (m > T && kw.at(cb[side]) == kw.at(ce[side])) {
+          const MidShape ms = mid_shape(m, T);
+          for (uint32_t t = 0; t <= ms.D; t++)
+            if (const uint32_t c = ms.at(t)) atomicAdd(&sh[kBits + 1 + t], c);
+          eq += mid_inner(m, T);
+          atomicAdd(&sh[kBinEqRoots], 1u);
+        }
+      }
+    }
+    eqPairs[i] = eq;
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kDeltaBins; b += kBlock) blockHist[(size_t)b * numKBlocks + blockIdx.x] = sh[b];
+}
+
+// One workgroup per bin: exclusive scan of the bin's row over the blocks
+// (in place), the bin's total to binTotal[bin].
+__global__ __launch_bounds__(1024) void k_karras_binscan(uint32_t *blockHist, uint32_t numKBlocks,
+                                                         uint32_t *binTotal) {
+  __shared__ uint32_t ss[1024 / kWave + 1];
+  uint32_t *row = blockHist + (size_t)blockIdx.x * numKBlocks;
+  uint32_t run = 0;
+  for (uint32_t b0 = 0; b0 < numKBlocks; b0 += 1024) {
+    const uint32_t k = b0 + threadIdx.x;
+    const uint32_t v = k < numKBlocks ? row[k] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<1024>(v, ss, &tot);
+    if (k < numKBlocks) row[k] = run + ex;
+    run += tot;
+  }
+  if (threadIdx.x == 0) binTotal[blockIdx.x] = run;
+}
+
+// exclusive scan over the n - 1 radix-tree nodes (n = *nPtr valid prims)
+__global__ __launch_bounds__(1024) void k_karras_scan(const uint32_t *count, uint32_t *out, const uint32_t *nPtr,
+                                                      uint64_t *status, uint32_t tag, uint32_t *total,
+                                                      uint32_t *err) {
+  __shared__ uint32_t smem[1024 / kWave + 2];
+  const uint32_t n = *nPtr;
+  count_scan_tile(count, out, n ? n - 1 : 0u, 0u, status, tag, total, smem, 0u, err);
+}
+
+// info[3] = 1 when the root is not a split node: at most T valid prims, or
+// all of their keys equal
+template <typename K>
+__global__ void k_karras_info(const K *keys, const uint32_t *nPtr, uint32_t T, uint32_t *info) {
+  const uint32_t n = *nPtr;
+  info[3] = (n <= T || keys[0] == keys[n - 1]) ? 1u : 0u;
+}
+
+// The root alone: one leaf of at most T prims (one workgroup) ...
+__global__ __launch_bounds__(256) void k_karras_leaf_root(const cubql_box3f *boxes, const uint32_t *primIDs,
+                                                          const uint32_t *nPtr, Node *nodes) {
+  __shared__ float sh[6][256];
+  const uint32_t n = *nPtr;
+  float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t j = threadIdx.x; j < n; j += 256) {
+    const cubql_box3f bx = load_box(boxes, primIDs[j]);
+    for (int d = 0; d < 3; d++) { v[d] = fminf(v[d], bx.lower[d]); v[3 + d] = fmaxf(v[3 + d], bx.upper[d]); }
+  }
+  for (int k = 0; k < 6; k++) sh[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (uint32_t w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int k = 0; k < 6; k++)
+        sh[k][threadIdx.x] = k < 3 ? fminf(sh[k][threadIdx.x], sh[k][threadIdx.x + w])
+                                   : fmaxf(sh[k][threadIdx.x], sh[k][threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    Node nd;
+    for (int d = 0; d < 3; d++) { nd.lo[d] = sh[d][0]; nd.hi[d] = sh[3 + d][0]; }
+    nd.oc = pack_oc(0, n);
+    nodes[0] = nd;
+  }
+}
+
+// ... or one middle-split subtree over all prims
+__global__ void k_karras_root_eq(const uint32_t *nPtr, EqRoot *roots) { roots[0] = EqRoot{0u, 0u, *nPtr, 2u}; }
+
+// Children of every split node: leaves (with their boxes), split nodes
+// (their position), roots of middle-split subtrees; each split node's slot
+// goes into its prefix-length bucket at its block's offset.
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_karras_emit(const K *keys, const cubql_box3f *sorted, uint32_t n,
+                                                        uint32_t T,
+                                                        const KNode *kn, const uint32_t *splitFlag,
+                                                        const uint32_t *slot, const uint32_t *eqBase,
+                                                        uint32_t numSplit, const uint32_t *bucketStart,
+                                                        const uint32_t *blockOff, uint32_t numKBlocks,
+                                                        uint32_t *bucket, uint32_t *pos, Node *nodes,
+                                                        EqRoot *eqRoots) {
+  __shared__ uint32_t sRank[kDeltaBins];
+  for (int b = threadIdx.x; b < kDeltaBins; b += kBlock) sRank[b] = 0;
+  __syncthreads();
+  for (int r = 0; r < kKPer; r++) {
+    const uint32_t i = blockIdx.x * kKBlock + r * kBlock + threadIdx.x;
+    if (i >= n - 1) break;
+    if (!splitFlag[i]) continue;
+    const KNode k = kn[i];
+    const uint32_t s = slot[i];
+    const uint32_t base = 2 + 2 * s;
+    if (i == 0) {
+      Node nd;
+      for (int d = 0; d < 3; d++) { nd.lo[d] = INFINITY; nd.hi[d] = -INFINITY; }
+      nd.oc = pack_oc(2, 0);
+      nodes[0] = nd;
+      pos[0] = 0;
+    }
+    const int dn = key_prefix(keys[k.first], keys[k.last]);
+    bucket[bucketStart[dn] + blockOff[(size_t)dn * numKBlocks + blockIdx.x] + atomicAdd(&sRank[dn], 1u)] = s;
+    uint32_t eqPair = 2 + 2 * numSplit + 2 * eqBase[i];
+    const uint32_t cb[2] = {k.first, k.gamma + 1}, ce[2] = {k.gamma, k.last};
+    for (int side = 0; side < 2; side++) {
+      const uint32_t m = ce[side] - cb[side] + 1;
+      Node nd;
+      if (m <= T) {
+        range_box(sorted, cb[side], m, nd);
+        nd.oc = pack_oc(cb[side], m);
+      } else {
+        for (int d = 0; d < 3; d++) { nd.lo[d] = INFINITY; nd.hi[d] = -INFINITY; }
+        if (keys[cb[side]] == keys[ce[side]]) {
+          // equal keys: a middle-split subtree, built by k_karras_equal
+          nd.oc = pack_oc(eqPair, 0);
+          const uint32_t e = blockOff[(size_t)kBinEqRoots * numKBlocks + blockIdx.x] + atomicAdd(&sRank[kBinEqRoots], 1u);
+          eqRoots[e] = EqRoot{base + side, cb[side], m, eqPair};
+          eqPair += 2 * mid_inner(m, T);
+        } else {
+          // the child range is the radix-tree node at its split end
+          const uint32_t cs = slot[side == 0 ? k.gamma : k.gamma + 1];
+          nd.oc = pack_oc(2 + 2 * cs, 0);
+          pos[cs] = base + side;
+        }
+      }
+      nodes[base + side] = nd;
+    }
+  }
+}
+
+// Inner nodes of the middle-split subtrees.  Work item w of a subtree is
+// the level-order inner node w when w < 2^D - 1, else node k = w - (2^D - 1)
+// of depth D, inner when it holds T + 1 prims; the depth-D inner nodes are
+// numbered by rank, which is their start offset minus k q (every depth-D
+// node before k holds q or q + 1 prims).  Each inner node writes itself,
+// its leaf children with their boxes, and its position into its depth's
+// bucket (slots claimed by one atomic per wave and depth).
+__global__ __launch_bounds__(256) void k_karras_equal(const cubql_box3f *sorted, uint32_t T, const EqRoot *roots, uint32_t nr, int kBits,
+                                                      const uint32_t *bucketStart, uint32_t *bucketCursor,
+                                                      uint32_t *bucket, Node *nodes) {
+  const uint32_t lane = lane_id();
+  for (uint32_t r = blockIdx.x; r < nr; r += gridDim.x) {
+    const EqRoot er = roots[r];
+    const MidShape ms = mid_shape(er.size, T);
+    const uint32_t full = (1u << ms.D) - 1u;  // inner nodes above depth D
+    const uint32_t items = full + (ms.q == T ? full + 1u : 0u);
+    const uint32_t stride = gridDim.y * 256;
+    // every lane of a wave runs the same number of rounds (the wave-wide
+    // slot claim below needs all of them)
+    for (uint32_t w0 = blockIdx.y * 256 + (threadIdx.x & ~(kWave - 1u)); w0 < items; w0 += stride) {
+      const uint32_t w = w0 + lane;
+      bool inner = false;
+      uint32_t d = 0, pos = 0, pair = 0, b = 0, sz = 0;
+      if (w < items) {
+        uint32_t k;
+        if (w < full) {
+          d = 31u - (uint32_t)__clz(w + 1);
+          k = w + 1 - (1u << d);
+        } else {
+          d = ms.D;
+          k = w - full;
+        }
+        // range of node (d, k) from the root's halvings along k's bits
+        sz = er.size;
+        for (int j = (int)d - 1; j >= 0; j--) {
+          const uint32_t h = sz >> 1;
+          if ((k >> j) & 1u) {
+            b += h;
+            sz -= h;
+          } else {
+            sz = h;
+          }
+        }
+        inner = sz > T;  // else a leaf at depth D, written by its parent
+        if (inner) {
+          const uint32_t I = d < ms.D ? w : full + (b - k * ms.q);
+          pos = d == 0 ? er.pos : er.pairBase + 2 * ((1u << (d - 1)) - 1u + (k >> 1)) + (k & 1u);
+          pair = er.pairBase + 2 * I;
+          Node nd;
+          for (int dd = 0; dd < 3; dd++) { nd.lo[dd] = INFINITY; nd.hi[dd] = -INFINITY; }
+          nd.oc = pack_oc(pair, 0);
+          nodes[pos] = nd;
+          const uint32_t h = sz >> 1;
+          const uint32_t cs[2] = {h, sz - h}, cbeg[2] = {b, b + h};
+          for (int side = 0; side < 2; side++)
+            if (cs[side] <= T) {
+              Node lf;
+              range_box(sorted, er.begin + cbeg[side], cs[side], lf);
+              lf.oc = pack_oc(er.begin + cbeg[side], cs[side]);
+              nodes[pair + side] = lf;
+            }
+        }
+      }
+      // bucket slots: one atomic per distinct depth in the wave
+      uint64_t rem = __ballot(inner);
+      while (rem) {
+        const uint32_t leader = (uint32_t)__builtin_ctzll(rem);
+        const uint32_t dl = (uint32_t)__shfl((int)d, (int)leader, kWave);
+        const uint64_t grp = __ballot(inner && d == dl);
+        uint32_t slot0 = 0;
+        if (lane == leader) slot0 = atomicAdd(&bucketCursor[kBits + 1 + dl], (uint32_t)__popcll(grp));
+        slot0 = (uint32_t)__shfl((int)slot0, (int)leader, kWave);
+        if (inner && d == dl)
+          bucket[bucketStart[kBits + 1 + dl] + slot0 + (uint32_t)__popcll(grp & ((1ull << lane) - 1ull))] = pos;
+        rem &= ~grp;
+      }
+    }
+  }
+}
+
+// Refit of one bucket's inner nodes from their children, which are final
+// (leaves carry their boxes; inner children were refitted in an earlier
+// bucket).  Entries are split-node slots (node pos[s], children 2 + 2s) or,
+// for middle-split buckets, node positions.
+__device__ __forceinline__ void karras_refit_one(uint32_t e, bool slots, const uint32_t *pos, Node *nodes) {
+  uint32_t p, c0;
+  if (slots) {
+    p = pos[e];
+    c0 = 2 + 2 * e;
+  } else {
+    p = e;
+    c0 = (uint32_t)oc_offset(nodes[p].oc);
+  }
+  const Node l = nodes[c0], r = nodes[c0 + 1];
+  Node *me = &nodes[p];
+  for (int d = 0; d < 3; d++) {
+    me->lo[d] = fminf(l.lo[d], r.lo[d]);
+    me->hi[d] = fmaxf(l.hi[d], r.hi[d]);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_karras_refit(const uint32_t *bucket, uint32_t begin, uint32_t count,
+                                                         bool slots, const uint32_t *pos, Node *nodes) {
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t < count) karras_refit_one(bucket[begin + t], slots, pos, nodes);
+}
+
+// The narrow buckets in one workgroup, in order; its barriers order them
+// (one CU: a bucket's node writes are seen by the later buckets' reads).
+struct RefitRange {
+  uint32_t begin, count, slots, pad;
+};
+constexpr int kRefitTopBuckets = kDeltaBins;
+__global__ __launch_bounds__(1024) void k_karras_refit_top(const uint32_t *bucket, const RefitRange *ranges,
+                                                           int numRanges, const uint32_t *pos, Node *nodes) {
+  for (int r = 0; r < numRanges; r++) {
+    const RefitRange rr = ranges[r];
+    for (uint32_t t = threadIdx.x; t < rr.count; t += 1024)
+      karras_refit_one(bucket[rr.begin + t], rr.slots != 0, pos, nodes);
+    __syncthreads();
+  }
+}
+
 template <typename K>
 int build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, int bitsPerDim,
           hipStream_t s, const cubql_mem_resource *mem) {
@@ -355,9 +740,10 @@ int build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, in
   CUBQL_CHECK(scr.reserve(&keysA, n));
   CUBQL_CHECK(scr.reserve(&keysB, n));
   CUBQL_CHECK(scr.reserve(&idsB, n));
-  CUBQL_CHECK(scr.reserve(&c.nodes, 2 * (size_t)n + 2));
-  CUBQL_CHECK(scr.reserve(&c.ranges, 2 * (size_t)n + 2));
-  CUBQL_CHECK(scr.reserve(&c.levels, 2 * (size_t)n + 2));
+  // k_rscan's root writes (the hierarchy below does not use these)
+  CUBQL_CHECK(scr.reserve(&c.nodes, 2));
+  CUBQL_CHECK(scr.reserve(&c.ranges, 1));
+  CUBQL_CHECK(scr.reserve(&c.levels, 1));
   CUBQL_CHECK(scr.commit());
   uint32_t *primIDs = nullptr;
   CUBQL_CHECK(mem_alloc(mem, (void **)&primIDs, (size_t)n * sizeof(uint32_t), s));
@@ -380,60 +766,155 @@ int build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, in
   int st;
   if (sizeof(K) == 8)
     st = radix_sort_pairs_dev_u64((uint64_t *)keysA, primIDs, (uint64_t *)keysB, idsB, &c.ctr[C_NUMVALID],
-                                  n, keyBits, s, mem, (uint64_t **)&kRes, &vRes);
+                                  n, keyBits, s, mem, (uint64_t **)&kRes, &vRes, &c.ctr[C_SPIN]);
   else
     st = radix_sort_pairs_dev_u32((uint32_t *)keysA, primIDs, (uint32_t *)keysB, idsB, &c.ctr[C_NUMVALID],
-                                  n, keyBits, s, mem, (uint32_t **)&kRes, &vRes);
+                                  n, keyBits, s, mem, (uint32_t **)&kRes, &vRes, &c.ctr[C_SPIN]);
   if (st) return fail(st);
   if (vRes != primIDs) {
     if (hipMemcpyAsync(primIDs, vRes, (size_t)n * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
       return fail((set_error("id copy failed"), CUBQL_ERR_HIP));
   }
-  // hierarchy, level by level, with a completion check every few levels
-  int dev = 0, numCU = 256;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&numCU, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-    numCU = 256;
-  const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(div_up(n, kBlock), (uint32_t)numCU * 8));
-  std::vector<uint32_t> hc(C_LSTART0 + kMaxLevels + 8);
-  int level = 0;
+  // hierarchy in one pass over the radix-tree nodes, one host check for the
+  // node count, then the refit by prefix length
   int hs = prof_begin(s);
-  while (true) {
-    for (int i = 0; i < kLevelsPerSync && level < kMaxLevels; i++, level++) {
-      // level l has at most 2^l open nodes: the top levels launch a few
-      // workgroups instead of a chip-wide grid of idle ones
-      const uint64_t maxOpen = std::min<uint64_t>(level < 40 ? (1ull << level) : ~0ull, (uint64_t)n);
-      const uint32_t gridL = std::max<uint32_t>(1u, std::min<uint32_t>(grid, div_up(maxOpen, kBlock)));
-      k_rlevel<K><<<gridL, kBlock, 0, s>>>(c, kRes, level);
-    }
-    if (hipGetLastError() != hipSuccess) return fail((set_error("level launch failed"), CUBQL_ERR_HIP));
-    if (hipMemcpyAsync(hc.data(), c.ctr, sizeof(uint32_t) * hc.size(), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-      return fail((set_error("level sync failed"), CUBQL_ERR_HIP));
-    if (hc[C_LEVEL0 + level] == 0) break;
-    if (level >= kMaxLevels) return fail((set_error("radix hierarchy deeper than %d levels", kMaxLevels), CUBQL_ERR_INTERNAL));
-  }
+  constexpr int kBits = 8 * (int)sizeof(K);
+  const uint32_t scanTiles = div_up(std::max(n, 1u), kCountScanTile);
+  const uint32_t numKBlocks = div_up(std::max(n, 1u), kKBlock);
+  enum { M_SPLIT = 0, M_EQPAIRS = 1, M_ROOTEQ = 3, M_ERR = 4, M_WORDS = 8 };
+  KNode *kn = nullptr;
+  uint32_t *splitFlag = nullptr, *slotArr = nullptr, *eqPairs = nullptr, *eqBase = nullptr, *misc = nullptr;
+  uint32_t *hist = nullptr, *bStart = nullptr, *bCursor = nullptr, *bucket = nullptr, *pos = nullptr;
+  uint32_t *blockHist = nullptr;
+  uint64_t *kStatus = nullptr;
+  EqRoot *eqRoots = nullptr;
+  RefitRange *topRanges = nullptr;
+  cubql_box3f *sortedBoxes = nullptr;
+  ScratchArena hscr(mem, s);
+  const size_t n1 = std::max(n, 1u);
+  CUBQL_CHECK(hscr.reserve(&kn, n1));
+  CUBQL_CHECK(hscr.reserve(&splitFlag, n1));
+  CUBQL_CHECK(hscr.reserve(&slotArr, n1));
+  CUBQL_CHECK(hscr.reserve(&eqPairs, n1));
+  CUBQL_CHECK(hscr.reserve(&eqBase, n1));
+  CUBQL_CHECK(hscr.reserve(&pos, n1));
+  CUBQL_CHECK(hscr.reserve(&bucket, 2 * n1));  // split nodes + middle-split inner nodes, each < n
+  CUBQL_CHECK(hscr.reserve(&eqRoots, n1));
+  CUBQL_CHECK(hscr.reserve(&blockHist, (size_t)kDeltaBins * numKBlocks));
+  CUBQL_CHECK(hscr.reserve(&kStatus, scanTiles));
+  CUBQL_CHECK(hscr.reserve(&misc, M_WORDS + 3 * kDeltaBins));
+  CUBQL_CHECK(hscr.reserve(&topRanges, kRefitTopBuckets));
+  CUBQL_CHECK(hscr.reserve(&sortedBoxes, n1));
+  if (hscr.commit() != CUBQL_OK) return fail(CUBQL_ERR_ALLOC);
+  hist = misc + M_WORDS;
+  bStart = hist + kDeltaBins;
+  bCursor = bStart + kDeltaBins;
+  if (fill_u32(misc, 0u, M_WORDS + 3 * kDeltaBins, s) != CUBQL_OK ||
+      fill_u32((uint32_t *)kStatus, 0u, 2 * (size_t)scanTiles, s) != CUBQL_OK)
+    return fail(CUBQL_ERR_HIP);
+  // the valid count is on the device only: the grids are sized by n and
+  // the kernels stop at the count
+  const uint32_t *nv = &c.ctr[C_NUMVALID];
+  k_karras_info<K><<<1, 1, 0, s>>>(kRes, nv, T, misc);
+  k_karras<K><<<numKBlocks, kBlock, 0, s>>>(kRes, nv, T, kn, splitFlag, eqPairs, blockHist, numKBlocks);
+  k_gather_sorted<<<div_up((uint32_t)n1, kBlock), kBlock, 0, s>>>(boxes, primIDs, nv, sortedBoxes);
+  k_karras_binscan<<<kDeltaBins, 1024, 0, s>>>(blockHist, numKBlocks, hist);
+  k_karras_scan<<<scanTiles, 1024, 0, s>>>(splitFlag, slotArr, nv, kStatus, 1u, misc + M_SPLIT, misc + M_ERR);
+  k_karras_scan<<<scanTiles, 1024, 0, s>>>(eqPairs, eqBase, nv, kStatus, 2u, misc + M_EQPAIRS, misc + M_ERR);
+  if (hipGetLastError() != hipSuccess) return fail((set_error("hierarchy launch failed"), CUBQL_ERR_HIP));
+  std::vector<uint32_t> hm(M_WORDS + kDeltaBins);
+  uint32_t numValid = 0, spinWord = 0;
+  if (hipMemcpyAsync(hm.data(), misc, sizeof(uint32_t) * hm.size(), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(&numValid, nv, sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(&spinWord, &c.ctr[C_SPIN], sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return fail((set_error("hierarchy sync failed"), CUBQL_ERR_HIP));
   prof_end(hs, s, "radix_hierarchy");
-  const uint32_t numValid = hc[C_NUMVALID];
-  const uint32_t numNodes = numValid ? hc[C_NODECOUNT] : 0;
-  prof_set(hs, 8.0 * numNodes, numNodes);
+  prof_set(hs, 40.0 * numValid, numValid);
+  if (hm[M_ERR] || spinWord)
+    return fail((set_error("radix build: a look-back wait exceeded its bound"), CUBQL_ERR_INTERNAL));
+  uint32_t *hh = hm.data() + M_WORDS;
+  const bool rootEq = hm[M_ROOTEQ] != 0;
+  const bool rootLeaf = numValid <= T;
+  uint32_t numSplit = hm[M_SPLIT], eqInner = hm[M_EQPAIRS], numEqRoots = hh[kBinEqRoots];
+  if (rootEq) {
+    // no split node: the refit buckets are those of the one subtree
+    numSplit = 0;
+    numEqRoots = rootLeaf ? 0 : 1;
+    eqInner = rootLeaf ? 0 : mid_inner(numValid, T);
+    std::fill(hh, hh + kDeltaBins, 0u);
+    if (!rootLeaf) {
+      const MidShape ms = mid_shape(numValid, T);
+      for (uint32_t t = 0; t <= ms.D; t++) hh[kBits + 1 + t] = ms.at(t);
+    }
+  }
+  hh[kBinEqRoots] = 0;  // not a refit bucket
+  const uint64_t nodes64 = numValid == 0 ? 0 : 2 + 2 * (uint64_t)numSplit + 2 * (uint64_t)eqInner;
+  if (nodes64 > 0xffffffffull) return fail((set_error("radix build: node count overflows"), CUBQL_ERR_INTERNAL));
+  const uint32_t numNodes = (uint32_t)nodes64;
+  std::vector<uint32_t> hStart(kDeltaBins, 0);
+  for (int d = 0, run = 0; d < kDeltaBins; d++) {
+    hStart[d] = (uint32_t)run;
+    run += (int)hh[d];
+  }
+  // refit order: buckets from the last bin down; the narrow tail (every
+  // bucket after the last wide one) goes to one workgroup
+  constexpr uint32_t kRefitTopMax = 4096;
+  std::vector<RefitRange> order;
+  for (int d = kDeltaBins - 1; d >= 0; d--)
+    if (hh[d]) order.push_back(RefitRange{hStart[d], hh[d], d < kBits ? 1u : 0u, 0u});
+  size_t topFrom = order.size();
+  while (topFrom > 0 && order[topFrom - 1].count <= kRefitTopMax) topFrom--;
   Node *nodes = nullptr;
+  int es = prof_begin(s);
   st = mem_alloc(mem, (void **)&nodes, std::max<size_t>(numNodes, 2) * sizeof(Node), s);
   if (st) return fail(st);
-  if (numNodes &&
-      hipMemcpyAsync(nodes, c.nodes, (size_t)numNodes * sizeof(Node), hipMemcpyDeviceToDevice, s) != hipSuccess) {
+  auto failN = [&](int e) {
     mem_free(mem, nodes, s);
-    return fail((set_error("node copy failed"), CUBQL_ERR_HIP));
+    return fail(e);
+  };
+  // node 1 stays unused (zero), as in the reference (radix.h:585-586)
+  if (fill_u32((uint32_t *)(nodes + 1), 0u, sizeof(Node) / 4, s) != CUBQL_OK) return failN(CUBQL_ERR_HIP);
+  if (numValid) {
+    if (hipMemcpyAsync(bStart, hStart.data(), sizeof(uint32_t) * kDeltaBins, hipMemcpyHostToDevice, s) != hipSuccess ||
+        (order.size() > topFrom &&
+         hipMemcpyAsync(topRanges, order.data() + topFrom, sizeof(RefitRange) * (order.size() - topFrom),
+                        hipMemcpyHostToDevice, s) != hipSuccess))
+      return failN((set_error("bucket upload failed"), CUBQL_ERR_HIP));
+    if (rootLeaf) {
+      k_karras_leaf_root<<<1, 256, 0, s>>>(boxes, primIDs, nv, nodes);
+    } else if (rootEq) {
+      k_karras_root_eq<<<1, 1, 0, s>>>(nv, eqRoots);
+    } else {
+      k_karras_emit<K><<<numKBlocks, kBlock, 0, s>>>(kRes, sortedBoxes, numValid, T, kn, splitFlag, slotArr,
+                                                      eqBase, numSplit, bStart, blockHist, numKBlocks, bucket,
+                                                      pos, nodes, eqRoots);
+    }
+    if (numEqRoots) {
+      // a few workgroups per subtree when there are few subtrees
+      const uint32_t gx = std::min(numEqRoots, 16384u), gy = std::min(64u, std::max(1u, 1024u / numEqRoots));
+      k_karras_equal<<<dim3(gx, gy), 256, 0, s>>>(sortedBoxes, T, eqRoots, numEqRoots, kBits, bStart,
+                                                  bCursor, bucket, nodes);
+    }
+    if (hipGetLastError() != hipSuccess) return failN((set_error("hierarchy emit launch failed"), CUBQL_ERR_HIP));
   }
+  prof_end(es, s, "radix_emit");
+  prof_set(es, 32.0 * numNodes + 24.0 * numValid, numNodes);
   bvh->nodes = (cubql_node3f *)nodes;
   bvh->numNodes = numNodes;
   bvh->primIDs = primIDs;
   bvh->numPrims = numValid;
   int rs = prof_begin(s);
-  refit_launch(c, nodes, primIDs, level, hc, numCU, s);
+  if (numValid && !rootLeaf) {
+    for (size_t r = 0; r < topFrom; r++)
+      k_karras_refit<<<div_up(order[r].count, kBlock), kBlock, 0, s>>>(bucket, order[r].begin, order[r].count,
+                                                                        order[r].slots != 0, pos, nodes);
+    if (order.size() > topFrom)
+      k_karras_refit_top<<<1, 1024, 0, s>>>(bucket, topRanges, (int)(order.size() - topFrom), pos, nodes);
+  }
   st = hipGetLastError() == hipSuccess ? CUBQL_OK : (set_error("refit launch failed"), CUBQL_ERR_HIP);
   prof_end(rs, s, "refit");
-  prof_set(rs, 24.0 * numValid + 64.0 * numNodes, numNodes);
+  prof_set(rs, 96.0 * (numNodes / 2), numNodes / 2);
   if (prof_on()) {
     if (hipStreamSynchronize(s) == hipSuccess) prof_flush();
   }
@@ -747,7 +1228,7 @@ __global__ __launch_bounds__(kBlock) void k_thread_phase(RCtx c, uint32_t *keys)
 __global__ __launch_bounds__(1024) void k_thread_scan(RCtx c) {
   __shared__ uint32_t ss[1024 / kWave + 2];
   count_scan_tile(c.smallCount, c.smallBase, c.ctr[C_SMALL], c.ctr[C_NODECOUNT], c.scanStatus, 1u,
-                  &c.ctr[C_NODESOUT], ss);
+                  &c.ctr[C_NODESOUT], ss, 1u, &c.ctr[C_SPIN]);
 }
 
 __global__ __launch_bounds__(kWave) void k_thread_emit(RCtx c, Node *out) {
@@ -815,7 +1296,8 @@ int rebin_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t
   prof_set(ks, 2.0 * 24.0 * n + 8.0 * n, n);
   if (hipGetLastError() != hipSuccess) return fail((set_error("rebin key launch failed"), CUBQL_ERR_HIP));
   uint32_t *keys = nullptr, *vRes = nullptr;
-  int st = radix_sort_pairs_dev_u32(keysA, primIDs, keysB, idsB, &c.ctr[C_NUMVALID], n, 30, s, mem, &keys, &vRes);
+  int st = radix_sort_pairs_dev_u32(keysA, primIDs, keysB, idsB, &c.ctr[C_NUMVALID], n, 30, s, mem, &keys, &vRes,
+                                    &c.ctr[C_SPIN]);
   if (st) return fail(st);
   if (vRes != primIDs &&
       hipMemcpyAsync(primIDs, vRes, (size_t)n * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
@@ -845,6 +1327,7 @@ int rebin_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t
       if (level >= kMaxLevels) return fail((set_error("rebin hierarchy deeper than %d levels", kMaxLevels), CUBQL_ERR_INTERNAL));
       continue;
     }
+    if (hc[C_SPIN]) return fail((set_error("rebin build: a look-back wait exceeded its bound"), CUBQL_ERR_INTERNAL));
     if (hc[C_JOBS] == 0) { level = L; break; }
     if (L >= kMaxLevels - 1) return fail((set_error("rebin hierarchy deeper than %d levels", kMaxLevels), CUBQL_ERR_INTERNAL));
     // rebin round: the jobs become level L's open nodes with rebuilt keys
@@ -856,7 +1339,8 @@ int rebin_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t
     k_rebin_keys<<<grid, kBlock, 0, s>>>(c, rkA, rvA);
     uint64_t *rkRes = nullptr;
     uint32_t *rvRes = nullptr;
-    st = radix_sort_pairs_dev_u64(rkA, rvA, rkB, rvB, &c.ctr[C_JOBPRIMS], n, 32 + jobBits, s, mem, &rkRes, &rvRes);
+    st = radix_sort_pairs_dev_u64(rkA, rvA, rkB, rvB, &c.ctr[C_JOBPRIMS], n, 32 + jobBits, s, mem, &rkRes, &rvRes,
+                                  &c.ctr[C_SPIN]);
     if (st) return fail(st);
     k_rebin_writeback<<<grid, kBlock, 0, s>>>(c, keys, rkRes, rvRes);
     if (hipGetLastError() != hipSuccess) return fail((set_error("rebin round launch failed"), CUBQL_ERR_HIP));
@@ -872,6 +1356,7 @@ int rebin_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t
   if (hipMemcpyAsync(hc.data(), c.ctr, sizeof(uint32_t) * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return fail((set_error("rebin sync failed"), CUBQL_ERR_HIP));
+  if (hc[C_SPIN]) return fail((set_error("rebin build: a look-back wait exceeded its bound"), CUBQL_ERR_INTERNAL));
   const uint32_t numValid = hc[C_NUMVALID];
   const uint32_t numNodes = numValid ? (numSmall ? hc[C_NODESOUT] : hc[C_NODECOUNT]) : 0;
   prof_set(hs, 8.0 * numNodes, numNodes);

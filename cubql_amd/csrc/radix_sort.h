@@ -19,13 +19,18 @@ namespace cubql {
 
 // Sorts (kA, vA) of *nDev (device-side count, <= nMax) pairs on key bits
 // [0, keyBits).  (kB, vB) is the ping-pong buffer; *kRes / *vRes receive the
-// buffers holding the result (kA/vA when the pass count is even).
+// buffers holding the result (kA/vA when the pass count is even).  err: a
+// device word the caller checks after the sort; when given, the look-back
+// waits are bounded and a wait past the bound sets it (the result is then
+// invalid); when NULL they are not bounded.
 int radix_sort_pairs_dev_u32(uint32_t *kA, uint32_t *vA, uint32_t *kB, uint32_t *vB,
                              const uint32_t *nDev, uint32_t nMax, int keyBits, hipStream_t s,
-                             const cubql_mem_resource *mem, uint32_t **kRes, uint32_t **vRes);
+                             const cubql_mem_resource *mem, uint32_t **kRes, uint32_t **vRes,
+                             uint32_t *err = nullptr);
 int radix_sort_pairs_dev_u64(uint64_t *kA, uint32_t *vA, uint64_t *kB, uint32_t *vB,
                              const uint32_t *nDev, uint32_t nMax, int keyBits, hipStream_t s,
-                             const cubql_mem_resource *mem, uint64_t **kRes, uint32_t **vRes);
+                             const cubql_mem_resource *mem, uint64_t **kRes, uint32_t **vRes,
+                             uint32_t *err = nullptr);
 
 // Workspace-based variant (no allocation; capturable in a graph).
 size_t radix_sort_ws_bytes(uint32_t nMax, int keyBits);

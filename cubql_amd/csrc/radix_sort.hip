@@ -4,6 +4,8 @@
 #include "profile.h"
 #include "radix_sort.h"
 
+#include "build_common.h"
+
 namespace cubql {
 namespace rs {
 
@@ -19,38 +21,51 @@ __device__ __forceinline__ uint64_t st_word(uint32_t tag, uint32_t flag, uint32_
 }
 
 // Digit histograms of every pass over kHistSpan tiles of keys: one partial
-// histogram per workgroup (no global atomics); k_hist_reduce sums them.
-constexpr uint32_t kHistSpan = 16;
+// histogram per workgroup (no global atomics), counted in one LDS
+// sub-histogram per wave; k_hist_reduce sums them.
+constexpr uint32_t kHistSpan = 2;
 
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_hist(const K *keys, const uint32_t *nPtr, int passes,
                                                  uint32_t *tileHist) {
-  __shared__ uint32_t h[8][256];
+  __shared__ uint32_t h[kNW][8][256];
   const uint32_t n = *nPtr;
-  for (int i = threadIdx.x; i < 8 * 256; i += kBlock) (&h[0][0])[i] = 0;
+  for (int i = threadIdx.x; i < kNW * 8 * 256; i += kBlock) (&h[0][0][0])[i] = 0;
   __syncthreads();
+  const uint32_t w = threadIdx.x / kWave;
   const uint32_t base = blockIdx.x * kTile * kHistSpan;
   const uint32_t end = (uint32_t)min((uint64_t)n, (uint64_t)base + (uint64_t)kTile * kHistSpan);
   for (uint32_t i = base + threadIdx.x; i < end; i += kBlock) {
     const K key = keys[i];
-    for (int p = 0; p < passes; p++) atomicAdd(&h[p][(uint32_t)(key >> (8 * p)) & 255u], 1u);
+    for (int p = 0; p < passes; p++) atomicAdd(&h[w][p][(uint32_t)(key >> (8 * p)) & 255u], 1u);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < passes * 256; i += kBlock)
-    tileHist[(size_t)blockIdx.x * passes * 256 + i] = (&h[0][0])[i];
+  for (int i = threadIdx.x; i < passes * 256; i += kBlock) {
+    uint32_t v = 0;
+    for (int ww = 0; ww < kNW; ww++) v += (&h[ww][0][0])[i];
+    tileHist[(size_t)blockIdx.x * passes * 256 + i] = v;
+  }
 }
 
-// One workgroup per pass: sum the partial histograms, exclusive scan over
-// digits.
-__global__ __launch_bounds__(256) void k_hist_reduce(const uint32_t *tileHist, uint32_t numTiles,
-                                                     int passes, uint32_t *digitBase) {
+// Sum of the partial histograms: workgroup (pass, chunk) adds the rows of
+// its chunk of blocks, one atomic per digit.
+constexpr uint32_t kHistChunks = 32;
+__global__ __launch_bounds__(256) void k_hist_reduce(const uint32_t *tileHist, uint32_t numBlocks, int passes,
+                                                     uint32_t *digitCount) {
+  const int p = blockIdx.x;
+  const uint32_t per = div_up(numBlocks, kHistChunks);
+  const uint32_t t0 = blockIdx.y * per, t1 = min(numBlocks, t0 + per);
+  uint32_t sum = 0;
+  for (uint32_t t = t0; t < t1; t++) sum += tileHist[(size_t)t * passes * 256 + p * 256 + threadIdx.x];
+  if (sum) atomicAdd(&digitCount[p * 256 + threadIdx.x], sum);
+}
+
+// exclusive scan over the digits of each pass
+__global__ __launch_bounds__(256) void k_digit_scan(const uint32_t *digitCount, uint32_t *digitBase) {
   __shared__ uint32_t ss[256 / kWave + 1];
   const int p = blockIdx.x;
-  uint32_t sum = 0;
-  for (uint32_t t = 0; t < numTiles; t++) sum += tileHist[(size_t)t * passes * 256 + p * 256 + threadIdx.x];
   uint32_t tot;
-  uint32_t ex = block_excl_scan<256>(sum, ss, &tot);
-  digitBase[p * 256 + threadIdx.x] = ex;
+  digitBase[p * 256 + threadIdx.x] = block_excl_scan<256>(digitCount[p * 256 + threadIdx.x], ss, &tot);
 }
 
 template <typename K>
@@ -59,7 +74,7 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const K *__restrict__ kin,
                                                      K *__restrict__ kout, uint32_t *__restrict__ vout,
                                                      const uint32_t *nPtr, int shift, uint32_t tag,
                                                      const uint32_t *digitBase, uint64_t *status,
-                                                     uint32_t *ticket) {
+                                                     uint32_t *ticket, uint32_t *err) {
   __shared__ uint32_t sHist[kNW][256];
   __shared__ uint32_t sTileExcl[256];
   __shared__ uint32_t sGlob[256];
@@ -127,11 +142,18 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const K *__restrict__ kin,
     } else {
       __hip_atomic_store(my, st_word(tag, 1, tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       uint32_t j = t - 1;
+      uint32_t spins = 0;
       while (true) {
         const uint64_t wv = __hip_atomic_load(status + (size_t)j * 256 + dgt, __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT);
         const uint32_t wt = (uint32_t)(wv >> 34), wf = (uint32_t)(wv >> 32) & 3u;
         if (wt != tag || wf == 0) {
+          // bounded when the caller checks err (build_common.h, kSpinLimit):
+          // the prefix so far is at most the true one, so writes stay in range
+          if (err && ++spins > kSpinLimit) {
+            atomicOr(err, 1u);
+            break;
+          }
           __builtin_amdgcn_s_sleep(1);
           continue;
         }
@@ -166,31 +188,36 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const K *__restrict__ kin,
   }
 }
 
-// Scratch layout (bytes): status | tileHist | digitBase | tickets.
+// Scratch layout (bytes): status | tileHist | digitBase | digitCount |
+// tickets.
 size_t sort_ws_bytes(uint32_t nMax, int keyBits) {
   const int passes = (keyBits + 7) / 8;
   const size_t maxTiles = std::max<uint32_t>(1u, div_up(nMax, kTile));
-  return maxTiles * 256 * 8 + maxTiles * passes * 256 * 4 + passes * 256 * 4 + 64 + 256;
+  const size_t histBlocks = div_up((uint32_t)maxTiles, kHistSpan);
+  return maxTiles * 256 * 8 + histBlocks * passes * 256 * 4 + 2 * passes * 256 * 4 + 64 + 256;
 }
 
 template <typename K>
 int sort_pairs_ws(K *kA, uint32_t *vA, K *kB, uint32_t *vB, const uint32_t *nDev, uint32_t nMax,
-                  int keyBits, hipStream_t s, void *ws, K **kRes, uint32_t **vRes) {
+                  int keyBits, hipStream_t s, void *ws, K **kRes, uint32_t **vRes, uint32_t *err = nullptr) {
   const int passes = (keyBits + 7) / 8;
   const uint32_t maxTiles = std::max<uint32_t>(1u, div_up(nMax, kTile));
+  const uint32_t histBlocks = div_up(maxTiles, kHistSpan);
   char *p = (char *)ws;
   uint64_t *status = (uint64_t *)p;
   p += (size_t)maxTiles * 256 * 8;
   uint32_t *tileHist = (uint32_t *)p;
-  p += (size_t)maxTiles * passes * 256 * 4;
+  p += (size_t)histBlocks * passes * 256 * 4;
   uint32_t *digitBase = (uint32_t *)p;
+  p += (size_t)passes * 256 * 4;
+  uint32_t *digitCount = (uint32_t *)p;
   p += (size_t)passes * 256 * 4;
   uint32_t *tickets = (uint32_t *)p;
   CUBQL_CHECK(fill_u32((uint32_t *)status, 0u, (size_t)maxTiles * 512, s));
-  CUBQL_CHECK(fill_u32(tickets, 0u, 16, s));
-  const uint32_t histBlocks = div_up(maxTiles, kHistSpan);
+  CUBQL_CHECK(fill_u32(digitCount, 0u, (size_t)passes * 256 + 16, s));  // digit counts and the tickets after them
   k_hist<K><<<histBlocks, kBlock, 0, s>>>(kA, nDev, passes, tileHist);
-  k_hist_reduce<<<passes, 256, 0, s>>>(tileHist, histBlocks, passes, digitBase);
+  k_hist_reduce<<<dim3(passes, kHistChunks), 256, 0, s>>>(tileHist, histBlocks, passes, digitCount);
+  k_digit_scan<<<passes, 256, 0, s>>>(digitCount, digitBase);
   int dev = 0, numCU = 256;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&numCU, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -201,7 +228,7 @@ int sort_pairs_ws(K *kA, uint32_t *vA, K *kB, uint32_t *vB, const uint32_t *nDev
   for (int p = 0; p < passes; p++) {
     int ps = prof_begin(s);
     k_onesweep<K><<<grid, kBlock, 0, s>>>(ki, vi, ko, vo, nDev, 8 * p, (uint32_t)(p + 1),
-                                          digitBase + p * 256, status, tickets + p);
+                                          digitBase + p * 256, status, tickets + p, err);
     prof_end(ps, s, "radix_onesweep");
     prof_set(ps, (double)nMax * 2.0 * (sizeof(K) + 4), nMax);
     std::swap(ki, ko);
@@ -215,11 +242,12 @@ int sort_pairs_ws(K *kA, uint32_t *vA, K *kB, uint32_t *vB, const uint32_t *nDev
 
 template <typename K>
 int sort_pairs(K *kA, uint32_t *vA, K *kB, uint32_t *vB, const uint32_t *nDev, uint32_t nMax,
-               int keyBits, hipStream_t s, const cubql_mem_resource *mem, K **kRes, uint32_t **vRes) {
+               int keyBits, hipStream_t s, const cubql_mem_resource *mem, K **kRes, uint32_t **vRes,
+               uint32_t *err) {
   Scratch scr(mem, s);
   char *ws = nullptr;
   CUBQL_CHECK(scr.alloc(&ws, sort_ws_bytes(nMax, keyBits)));
-  return sort_pairs_ws<K>(kA, vA, kB, vB, nDev, nMax, keyBits, s, ws, kRes, vRes);
+  return sort_pairs_ws<K>(kA, vA, kB, vB, nDev, nMax, keyBits, s, ws, kRes, vRes, err);
 }
 
 }  // namespace rs
@@ -238,13 +266,13 @@ int radix_sort_pairs_ws_u64(uint64_t *kA, uint32_t *vA, uint64_t *kB, uint32_t *
 
 int radix_sort_pairs_dev_u32(uint32_t *kA, uint32_t *vA, uint32_t *kB, uint32_t *vB, const uint32_t *nDev,
                              uint32_t nMax, int keyBits, hipStream_t s, const cubql_mem_resource *mem,
-                             uint32_t **kRes, uint32_t **vRes) {
-  return rs::sort_pairs<uint32_t>(kA, vA, kB, vB, nDev, nMax, keyBits, s, mem, kRes, vRes);
+                             uint32_t **kRes, uint32_t **vRes, uint32_t *err) {
+  return rs::sort_pairs<uint32_t>(kA, vA, kB, vB, nDev, nMax, keyBits, s, mem, kRes, vRes, err);
 }
 int radix_sort_pairs_dev_u64(uint64_t *kA, uint32_t *vA, uint64_t *kB, uint32_t *vB, const uint32_t *nDev,
                              uint32_t nMax, int keyBits, hipStream_t s, const cubql_mem_resource *mem,
-                             uint64_t **kRes, uint32_t **vRes) {
-  return rs::sort_pairs<uint64_t>(kA, vA, kB, vB, nDev, nMax, keyBits, s, mem, kRes, vRes);
+                             uint64_t **kRes, uint32_t **vRes, uint32_t *err) {
+  return rs::sort_pairs<uint64_t>(kA, vA, kB, vB, nDev, nMax, keyBits, s, mem, kRes, vRes, err);
 }
 
 }  // namespace cubql
@@ -267,7 +295,7 @@ int sort_entry(K *keys, uint32_t *vals, K *keysTmp, uint32_t *valsTmp, uint32_t 
   CUBQL_CHECK(fill_u32(nDev, n, 1, s));
   K *kr = nullptr;
   uint32_t *vr = nullptr;
-  CUBQL_CHECK(rs::sort_pairs<K>(keys, vals, keysTmp, valsTmp, nDev, n, keyBits, s, nullptr, &kr, &vr));
+  CUBQL_CHECK(rs::sort_pairs<K>(keys, vals, keysTmp, valsTmp, nDev, n, keyBits, s, nullptr, &kr, &vr, nullptr));
   if (kr != keys) {
     CUBQL_HIP(hipMemcpyAsync(keys, kr, sizeof(K) * (size_t)n, hipMemcpyDeviceToDevice, s));
     CUBQL_HIP(hipMemcpyAsync(vals, vr, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, s));

@@ -69,7 +69,8 @@ struct Counters {
   uint32_t ticket;
   uint32_t numTiles;
   uint32_t numBinTiles;
-  uint32_t pad[2];
+  uint32_t spinFault;  // a look-back wait exceeded its bound (build_common.h)
+  uint32_t pad;
 };
 
 struct Seg {
@@ -320,6 +321,7 @@ __global__ __launch_bounds__(1024) void k_count_scan(Ctx c, uint32_t numBlocks) 
       k->rootCb[j] = enc_f(r);
     }
     k->numValid = run;
+    k->spinFault = 0;
     k->nodeCount = 2;
     k->numLarge[0] = k->numLarge[1] = 0;
     k->numSmall = 0;
@@ -566,7 +568,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
     uint32_t rank[kItems];
     const uint32_t nLeft = block_rank_striped<kBlock, kItems, true>(left, rank, sr);
     if (threadIdx.x < kWave) {
-      const uint32_t prefix = lookback_exclusive_wave(c.tileStatus, t, k, tag, nLeft);
+      const uint32_t prefix = lookback_exclusive_wave(c.tileStatus, t, k, tag, nLeft, false, &c.ctr->spinFault);
       if (threadIdx.x == 0) {
         sPrefix = prefix;
         if (k == nTilesSeg - 1) c.segLeft[s] = prefix + nLeft;
@@ -819,7 +821,7 @@ __global__ __launch_bounds__(kWave) void k_small(Ctx c) {
 __global__ __launch_bounds__(1024) void k_small_scan(Ctx c, uint32_t tag) {
   __shared__ uint32_t ss[1024 / kWave + 2];
   count_scan_tile(c.smallCount, c.smallBase, c.ctr->numSmall, c.ctr->nodeCount, c.tileStatus, tag,
-                  &c.ctr->numNodesOut, ss);
+                  &c.ctr->numNodesOut, ss, 1u, &c.ctr->spinFault);
 }
 
 __global__ __launch_bounds__(kWave) void k_small_emit(Ctx c, Node *out) {
@@ -943,6 +945,7 @@ int sah_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T
         hipStreamSynchronize(s) != hipSuccess)
       return fail((set_error("sah level sync failed"), CUBQL_ERR_HIP));
     const Counters *hc = (const Counters *)hw;
+    if (hc->spinFault) return fail((set_error("sah build: a look-back wait exceeded its bound"), CUBQL_ERR_INTERNAL));
     if (hc->numLarge[0] == 0 && hc->numLarge[1] == 0) break;
     if (level >= (int)maxLevels) return fail((set_error("SAH tree deeper than %u levels", maxLevels), CUBQL_ERR_INTERNAL));
   }
@@ -967,6 +970,8 @@ int sah_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T
   if (hipMemcpyAsync(hw, c.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return fail((set_error("sah final sync failed"), CUBQL_ERR_HIP));
+  if (((const Counters *)hw)->spinFault)
+    return fail((set_error("sah build: a look-back wait exceeded its bound"), CUBQL_ERR_INTERNAL));
   const uint32_t numNodes = numValid ? (numSmall ? ((const Counters *)hw)->numNodesOut : topCount) : 0;
   if (prof_on()) {
     // algorithmic bytes: binning reads one 32-byte item per prim of the

@@ -78,6 +78,7 @@ struct Counters {
   uint32_t numTiles;
   uint32_t numTop;    // entries in the top-node list
   uint32_t baseNode[2], baseSmall[2], baseTop[2];  // each pass's bases, by pass parity (emit)
+  uint32_t spinFault;  // a look-back wait exceeded its bound (build_common.h)
 };
 
 // An open node with > kMedium prims.  [begin, end) is its range in the CPU
@@ -546,7 +547,7 @@ __global__ __launch_bounds__(256) void k_plan(Ctx c, int pass, uint32_t scanBloc
   if (blockIdx.x < scanBlocks) {
     __shared__ uint32_t ss[256 / kWave + 2];
     count_scan_part<256>(c.tileLeft, c.tilePre, c.ctr->numTiles, 0u, c.tileStatus + 2 * (size_t)c.maxTiles,
-                         (uint32_t)pass + 1, blockIdx.x, false, nullptr, ss, 0u);
+                         (uint32_t)pass + 1, blockIdx.x, false, nullptr, ss, 0u, &c.ctr->spinFault);
     return;
   }
   const int par = pass & 1;
@@ -628,6 +629,7 @@ __global__ void k_init_acc(Ctx c, size_t nStatus, size_t nEmit) {
       k->rootCb[d + 3] = enc_f(-INFINITY);
     }
     k->allPoints = 1;
+    k->spinFault = 0;
   }
 }
 
@@ -812,7 +814,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_pair(Ctx c, int pass) {
       uint32_t pre = 0;
       if (look) {
         const uint32_t mine = wid == 0 ? ctot[0] : ctot[2];
-        pre = lookback_exclusive_wave(c.tileStatus + (size_t)wid * c.maxTiles, t, k, tag, mine, true);
+        pre = lookback_exclusive_wave(c.tileStatus + (size_t)wid * c.maxTiles, t, k, tag, mine, true,
+                                      &c.ctr->spinFault);
         if (lane == 0 && t == tileEnd - 1) c.classTot[4 * (size_t)seg + q] = pre + mine;
       }
       if (lane == 0) {
@@ -1018,7 +1021,8 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit2(Ctx c, int pass) {
 #pragma unroll
     for (int q = 1; q < kEmitCounts; q++) v = w == (uint32_t)q ? tt[q] : v;
     const uint32_t pre =
-        lookback_exclusive_wave(c.emitStatus + (size_t)w * c.maxEmitBlocks, blockIdx.x, blockIdx.x, tag, v);
+        lookback_exclusive_wave(c.emitStatus + (size_t)w * c.maxEmitBlocks, blockIdx.x, blockIdx.x, tag, v, false,
+                                &c.ctr->spinFault);
     if (lane_id() == 0) sPre[w] = pre;
   }
   sync_block_lds();
@@ -1748,7 +1752,9 @@ __global__ __launch_bounds__(kWave) void k_small(Ctx c, int pp) {
   __shared__ float4 sIt[kSmall];
   __shared__ uint32_t sFr[kSmall + 2];
   __shared__ MedFrame sMed[kMedium / (kSmall + 1) + 1];
-  if (c.ctr->numLarge[pp] != 0) return;
+  // (launched before the host has checked the passes: after a look-back
+  // wait past its bound the items are not trusted, so nothing is gathered)
+  if (c.ctr->numLarge[pp] != 0 || c.ctr->spinFault != 0) return;
   const bool pts = c.ctr->allPoints != 0;
   const uint32_t nSmall = c.ctr->numSmall;
   for (uint32_t w = blockIdx.x; w < nSmall; w += gridDim.x) {
@@ -1767,7 +1773,7 @@ __global__ __launch_bounds__(1024) void k_small_scan(Ctx c, uint32_t tag, int pp
   __shared__ uint32_t ss[1024 / kWave + 2];
   if (c.ctr->numLarge[pp] != 0) return;
   count_scan_tile(c.smallCount, c.smallBase, c.ctr->numSmall, c.ctr->nodeCount, c.tileStatus, tag,
-                  &c.ctr->numNodesOut, ss);
+                  &c.ctr->numNodesOut, ss, 1u, &c.ctr->spinFault);
 }
 // (one wave per segment, four segments a workgroup)
 __global__ __launch_bounds__(256) void k_small_emit(Ctx c, Node *out) {
@@ -1945,6 +1951,8 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
         hipStreamSynchronize(s) != hipSuccess)
       return fail((set_error("pass sync failed"), CUBQL_ERR_HIP));
     const Counters *hc = (const Counters *)hw;
+    if (hc->spinFault)
+      return fail((set_error("spatial-median build: a look-back wait exceeded its bound"), CUBQL_ERR_INTERNAL));
     // open segments of the next pass to run
     if (hc->numLarge[pass & 1] == 0) break;
     batch = kPassesPerSync;
@@ -1968,6 +1976,8 @@ int sm_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T,
     if (hipMemcpyAsync(hw, c.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       return fail((set_error("final sync failed"), CUBQL_ERR_HIP));
+    if (((const Counters *)hw)->spinFault)
+      return fail((set_error("spatial-median build: a look-back wait exceeded its bound"), CUBQL_ERR_INTERNAL));
     nodesOut = ((const Counters *)hw)->numNodesOut;
   }
   const uint32_t numNodes = numValid ? (numSmall ? nodesOut : topCount) : 0;

@@ -67,7 +67,7 @@ __global__ void k_flag(const Node *nodes, uint32_t n, const uint32_t *parent, in
 __global__ __launch_bounds__(1024) void k_scan(const uint32_t *flag, uint32_t *wideId, uint32_t n,
                                                uint64_t *status, uint32_t *total) {
   __shared__ uint32_t smem[1024 / kWave + 2];
-  count_scan_tile(flag, wideId, n, 0u, status, 1u, total, smem, 0u);
+  count_scan_tile(flag, wideId, n, 0u, status, 1u, total, smem, 0u, total + 1);
 }
 
 template <int W>
@@ -132,9 +132,10 @@ int collapse_wide(const cubql_bvh3f &bin, int width, cubql_wbvh3f *out, hipStrea
   CUBQL_CHECK(arena.reserve(&flag, n));
   CUBQL_CHECK(arena.reserve(&wideId, n));
   CUBQL_CHECK(arena.reserve(&status, scanTiles));
-  CUBQL_CHECK(arena.reserve(&total, 1));
+  CUBQL_CHECK(arena.reserve(&total, 2));  // wide node count, look-back fault word
   CUBQL_CHECK(arena.commit());
   CUBQL_CHECK(fill_u32(parent, wd::kNone, n, s));
+  CUBQL_CHECK(fill_u32(total, 0u, 2, s));
   CUBQL_CHECK(fill_u32((uint32_t *)status, 0u, 2 * (size_t)scanTiles, s));
   const uint32_t g = std::min<uint32_t>(div_up(n, 256), 16384);
   const Node *nodes = (const Node *)bin.nodes;
@@ -142,9 +143,14 @@ int collapse_wide(const cubql_bvh3f &bin, int width, cubql_wbvh3f *out, hipStrea
   wd::k_flag<<<g, 256, 0, s>>>(nodes, n, parent, logW, flag);
   wd::k_scan<<<scanTiles, 1024, 0, s>>>(flag, wideId, n, status, total);
   CUBQL_LAUNCH_CHECK();
-  uint32_t numWide = 0;
-  CUBQL_HIP(hipMemcpyAsync(&numWide, total, 4, hipMemcpyDeviceToHost, s));
+  uint32_t tw[2] = {0, 0};
+  CUBQL_HIP(hipMemcpyAsync(tw, total, 8, hipMemcpyDeviceToHost, s));
   CUBQL_HIP(hipStreamSynchronize(s));
+  if (tw[1]) {
+    set_error("wide collapse: a look-back wait exceeded its bound");
+    return CUBQL_ERR_INTERNAL;
+  }
+  const uint32_t numWide = tw[0];
   void *wn = nullptr;
   CUBQL_CHECK(mem_alloc(mem, &wn, (size_t)numWide * width * sizeof(cubql_wide_child3f), s));
   out->nodes = (cubql_wide_child3f *)wn;

@@ -133,7 +133,7 @@ def lib():
         "cubql_triangle_boxes_f3": (I, [V, V, U32, V, V]),
         "cubql_fcp_triangles_f3": (I, [P(BVH3f), V, V, V, U32, F, V, V, V, V]),
         "cubql_fcp_triangles_backward_f3": (I, [V, V, V, U32, V, V, V, V, V]),
-        "cubql_points_distance_backward_f3": (I, [V, V, U32, I, V, V, V, V, V]),
+        "cubql_points_distance_backward_f3": (I, [V, U32, V, U32, I, V, V, V, V, V, P(MemResource)]),
         "cubql_batch_chamfer_workspace_f3": (C.c_size_t, [U32, U32, U32]),
         "cubql_batch_chamfer_f3": (I, [V, V, U32, V, V, U32, U32, P(BatchParams), V, C.c_size_t,
                                        V, V, V, V, V, V, V, V]),
@@ -512,15 +512,18 @@ class triangles:  # noqa: N801 -- mirrors cuBQL::triangles
 
 
 def points_backward(data: torch.Tensor, queries: torch.Tensor, ids: torch.Tensor,
-                    grad_dist: torch.Tensor, grad_queries=None, grad_points=None, stream=None):
-    """Adjoint of point distances for fcp (ids [nq]) or kNN (ids [nq,k])."""
+                    grad_dist: torch.Tensor, grad_queries=None, grad_points=None, stream=None,
+                    mem: Optional["TorchMemory"] = None):
+    """Adjoint of point distances for fcp (ids [nq]) or kNN (ids [nq,k]);
+    grad_points accumulates, grad_queries is overwritten."""
     data = _as_points(data, "data")
     queries = _as_points(queries, "queries")
     nq = queries.shape[0]
     k = 1 if ids.dim() == 1 else ids.shape[1]
     _check(lib().cubql_points_distance_backward_f3(
-        _ptr(data), _ptr(queries), nq, k, _ptr(ids.contiguous()), _ptr(grad_dist.contiguous()),
-        _ptr(grad_queries), _ptr(grad_points), C.c_void_p(_stream_handle(stream))),
+        _ptr(data), data.shape[0], _ptr(queries), nq, k, _ptr(ids.contiguous()), _ptr(grad_dist.contiguous()),
+        _ptr(grad_queries), _ptr(grad_points), C.c_void_p(_stream_handle(stream)),
+        C.byref(mem.resource) if mem is not None else None),
         "points_backward")
     return grad_queries, grad_points
 

@@ -250,7 +250,8 @@ __device__ __forceinline__ bool fcp_forest(const Side &tgt, uint32_t root, float
 // queries of side `qs` against the forest of side `ts`, in Morton order of qs
 template <int BLOCK, int STACK>
 __global__ __launch_bounds__(BLOCK) void k_bquery(Side qs, Side ts, int32_t *nn, float *d2, uint32_t *ovf) {
-  const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
+  // queries sorted by (cloud, Morton key): one contiguous run per XCD
+  const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
   if (t >= qs.n) return;
   const uint32_t qi = qs.ids[t];
   const uint32_t cloud = (uint32_t)(qs.keys[t] >> kCloudShift);
@@ -289,30 +290,68 @@ __global__ __launch_bounds__(64) void k_bquery_deep(Side qs, Side ts, int32_t *n
   }
 }
 
-// adjoint of loss[c] = mean over the query cloud of d(q, nearest): the
-// query's own term and its neighbour's term are both scatter-added
+// adjoint of loss[c] = mean over the query cloud of d(q, nearest).
+// The query's own term goes to gradQ[qi] without an atomic: in the first
+// direction (ownFirst) every point of the side is a query exactly once, so
+// a store initialises gradQ; in the second only this lane touches
+// gradQ[qi] (the other lanes scatter into the other side's array), so a
+// plain read-modify-write adds to what the first direction scattered.  The
+// neighbour's term is scattered into gradT: queries run in Morton order, so
+// lanes next to each other often share their nearest point; a segmented
+// wave sum over runs of equal neighbour ids leaves one atomic per run.
 __global__ __launch_bounds__(256) void k_bback(Side qs, Side ts, const int32_t *nn, const float *d2,
-                                               float *loss, float *gradQ, float *gradT) {
+                                               float *loss, float *gradQ, float *gradT, bool ownFirst) {
   const uint32_t t = blockIdx.x * 256 + threadIdx.x;
   const bool act = t < qs.n;
+  const uint32_t lane = lane_id();
   uint32_t qi = 0, cloud = 0;
-  float contrib = 0.f;
+  int32_t p = -1;
+  float contrib = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
   if (act) {
     qi = qs.ids[t];
     cloud = (uint32_t)(qs.keys[t] >> kCloudShift);
-    const int32_t p = nn[qi];
+    p = nn[qi];
     if (p >= 0) {
       const float d = sqrtf(d2[qi]);
       const float g = 1.f / (float)(qs.off[cloud + 1] - qs.off[cloud]);
       contrib = g * d;
       if (d > 0.f) {
         const float s = g / d;
-        for (int k = 0; k < 3; k++) {
-          const float diff = qs.pts[3 * (size_t)qi + k] - ts.pts[3 * (size_t)p + k];
-          if (gradQ) atomicAdd(&gradQ[3 * (size_t)qi + k], s * diff);
-          if (gradT) atomicAdd(&gradT[3 * (size_t)p + k], -s * diff);
-        }
+        gx = s * (qs.pts[3 * (size_t)qi + 0] - ts.pts[3 * (size_t)p + 0]);
+        gy = s * (qs.pts[3 * (size_t)qi + 1] - ts.pts[3 * (size_t)p + 1]);
+        gz = s * (qs.pts[3 * (size_t)qi + 2] - ts.pts[3 * (size_t)p + 2]);
       }
+    }
+    if (gradQ) {
+      float *o = gradQ + 3 * (size_t)qi;
+      if (ownFirst) {
+        o[0] = gx; o[1] = gy; o[2] = gz;
+      } else {
+        o[0] += gx; o[1] += gy; o[2] += gz;
+      }
+    }
+  }
+  if (gradT) {
+    // segmented inclusive sums over runs of equal p (a run starts at lane 0
+    // or where p differs from the lane before)
+    const int32_t pPrev = __shfl_up(p, 1, kWave);
+    bool seen = lane == 0 || pPrev != p;
+    float sx = gx, sy = gy, sz = gz;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const float ux = __shfl_up(sx, o, kWave), uy = __shfl_up(sy, o, kWave), uz = __shfl_up(sz, o, kWave);
+      const bool us = __shfl_up((int)seen, o, kWave) != 0;
+      if (lane >= (uint32_t)o && !seen) {
+        sx += ux; sy += uy; sz += uz;
+        seen = us;
+      }
+    }
+    const int32_t pNext = __shfl_down(p, 1, kWave);
+    const bool runEnd = lane == kWave - 1 || pNext != p;
+    if (runEnd && p >= 0 && (sx != 0.f || sy != 0.f || sz != 0.f)) {
+      atomicAdd(&gradT[3 * (size_t)p + 0], -sx);
+      atomicAdd(&gradT[3 * (size_t)p + 1], -sy);
+      atomicAdd(&gradT[3 * (size_t)p + 2], -sz);
     }
   }
   if (loss) {
@@ -322,7 +361,7 @@ __global__ __launch_bounds__(256) void k_bback(Side qs, Side ts, const int32_t *
     if (uniform) {
       float sum = contrib;
       for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, kWave);
-      if (lane_id() == 0 && __any(act)) atomicAdd(&loss[c0], sum);
+      if (lane == 0 && __any(act)) atomicAdd(&loss[c0], sum);
     } else if (act && contrib != 0.f) {
       atomicAdd(&loss[cloud], contrib);
     }
@@ -486,7 +525,9 @@ int run(const Args &a, hipStream_t s) {
   prof_set(ps, (12.0 + 12.0 + 8.0 * 2 * 6 + 8.0 + 24.0) * (a.nSrc + a.nTgt), a.nSrc + a.nTgt);
   const uint32_t fg = 1024;
   if (a.loss) k_fill_f32<<<std::max(1u, std::min(div_up(a.numClouds, 256), fg)), 256, 0, s>>>(a.loss, 0.f, a.numClouds);
-  if (a.gradSrc) k_fill_f32<<<fg, 256, 0, s>>>(a.gradSrc, 0.f, 3 * (size_t)a.nSrc);
+  // gradSrc is initialised by the first backward direction's stores; an
+  // empty source side leaves it nothing to initialise
+  if (a.gradSrc && !a.nSrc) k_fill_f32<<<fg, 256, 0, s>>>(a.gradSrc, 0.f, 3 * (size_t)a.nSrc);
   if (a.gradTgt) k_fill_f32<<<fg, 256, 0, s>>>(a.gradTgt, 0.f, 3 * (size_t)a.nTgt);
   ps = prof_begin(s);
   CUBQL_CHECK(query_dir(side[0], side[1], a.srcNN, a.srcD2, tp.ovf, a.prm, s));
@@ -495,8 +536,12 @@ int run(const Args &a, hipStream_t s) {
   prof_set(ps, 20.0 * (a.nSrc + a.nTgt), a.nSrc + a.nTgt);
   ps = prof_begin(s);
   if (a.loss || a.gradSrc || a.gradTgt) {
-    if (a.nSrc) k_bback<<<div_up(a.nSrc, 256), 256, 0, s>>>(side[0], side[1], a.srcNN, a.srcD2, a.loss, a.gradSrc, a.gradTgt);
-    if (a.nTgt) k_bback<<<div_up(a.nTgt, 256), 256, 0, s>>>(side[1], side[0], a.tgtNN, a.tgtD2, a.loss, a.gradTgt, a.gradSrc);
+    if (a.nSrc)
+      k_bback<<<div_up(a.nSrc, 256), 256, 0, s>>>(side[0], side[1], a.srcNN, a.srcD2, a.loss, a.gradSrc, a.gradTgt,
+                                                  true);
+    if (a.nTgt)
+      k_bback<<<div_up(a.nTgt, 256), 256, 0, s>>>(side[1], side[0], a.tgtNN, a.tgtD2, a.loss, a.gradTgt, a.gradSrc,
+                                                  false);
   }
   prof_end(ps, s, "batch_backward");
   prof_set(ps, 68.0 * (a.nSrc + a.nTgt), a.nSrc + a.nTgt);

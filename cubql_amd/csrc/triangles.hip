@@ -21,6 +21,7 @@
 #include "cubql_amd/triangle.hip.h"
 #include "profile.h"
 #include "query_order.h"
+#include "radix_sort.h"
 
 namespace cubql {
 namespace tri {
@@ -61,6 +62,33 @@ __device__ __forceinline__ bool fcp_tri_one(const cubql_bvh3f &bvh, const int32_
   return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, bestD, prim, 0u, st);
 }
 
+// The same on a caller-provided traversal stack.
+template <typename StackT>
+__device__ __forceinline__ bool fcp_tri_one_on(StackT &stack, const cubql_bvh3f &bvh, const int32_t *tris,
+                                               const float *verts, F3 q, float sqrMax, int32_t &best, float &bestD,
+                                               F3 &bestP) {
+  best = -1;
+  bestD = sqrMax;
+  bestP = {0.f, 0.f, 0.f};
+  auto prim = [&](uint32_t t) -> float {
+    F3 a, b, c, P;
+    tri_verts(tris, verts, t, a, b, c);
+    float w[3];
+    const float d = tri_closest<false>(a, b, c, q, P, w);
+    if (d < bestD || (d == bestD && best >= 0 && (int32_t)t < best)) {
+      bestD = d;
+      best = (int32_t)t;
+      bestP = P;
+    }
+    return bestD;
+  };
+  NoStats st;
+  cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim, 0u, st);
+  return cubql_dev::for_each_prim_on(stack, bvh, q.x, q.y, q.z, bestD, prim, 0u, st);
+}
+
+
+
 __device__ __forceinline__ void write_result(uint32_t qi, int32_t best, float bestD, F3 P, int32_t *ids,
                                              float *d2, float *cp) {
   ids[qi] = best;
@@ -84,7 +112,10 @@ __global__ __launch_bounds__(kBlock) void k_fcp_tris(cubql_bvh3f bvh, const int3
   int32_t best;
   float bestD;
   F3 P;
-  if (!fcp_tri_one<kFastStack>(bvh, tris, verts, q, sqrMax, best, bestD, P)) {
+  // private stack: triangle queries on an SAH tree stack many far
+  // children, and a short LDS stack sent too many of them to the deep rerun
+  PrivateStack<kFastStack> stack;
+  if (!fcp_tri_one_on(stack, bvh, tris, verts, q, sqrMax, best, bestD, P)) {
     ovfList[atomicAdd(&ovf->count, 1u)] = qi;
     return;
   }
@@ -162,33 +193,98 @@ __global__ __launch_bounds__(kBlock) void k_tri_backward(const int32_t *tris, co
   }
 }
 
-// points: d = |q-p| for each (query, neighbour) entry; k entries per query.
+// points: d = |q-p| for each (query, neighbour) entry e = qi*k + j.  The
+// query gradient is summed per lane.  The point gradient is not scattered
+// here: when gv is given, each entry's point term -g (0 for skipped
+// entries) goes to gv[e] and its sort key to key[e] (the point id, or
+// numPoints for entries that contribute nothing), val[e] = e.
 __global__ __launch_bounds__(kBlock) void k_point_backward(const float *points, const float *queries,
                                                            uint32_t nq, int k, const int32_t *ids,
-                                                           const float *gradDist, float *gradQ,
-                                                           float *gradP) {
+                                                           const float *gradDist, float *gradQ, float *gv,
+                                                           uint32_t *key, uint32_t *val, uint32_t numPoints) {
   const uint32_t qi = blockIdx.x * kBlock + threadIdx.x;
   if (qi >= nq) return;
   const F3 q = ld3(queries, qi);
   F3 gq = {0.f, 0.f, 0.f};
   for (int j = 0; j < k; j++) {
-    const int32_t p = ids[(size_t)qi * k + j];
-    if (p < 0) continue;
-    const F3 diff = sub(q, ld3(points, (size_t)p));
-    const float d = sqrtf(dot(diff, diff));
-    if (!(d > 0.f)) continue;
-    const F3 g = mul(gradDist[(size_t)qi * k + j] / d, diff);
-    gq = add(gq, g);
-    if (gradP) {
-      atomicAdd(&gradP[3 * (size_t)p + 0], -g.x);
-      atomicAdd(&gradP[3 * (size_t)p + 1], -g.y);
-      atomicAdd(&gradP[3 * (size_t)p + 2], -g.z);
+    const size_t e = (size_t)qi * k + j;
+    const int32_t p = ids[e];
+    F3 g = {0.f, 0.f, 0.f};
+    bool live = false;
+    if (p >= 0) {
+      const F3 diff = sub(q, ld3(points, (size_t)p));
+      const float d = sqrtf(dot(diff, diff));
+      if (d > 0.f) {
+        g = mul(gradDist[e] / d, diff);
+        gq = add(gq, g);
+        live = true;
+      }
+    }
+    if (gv) {
+      gv[3 * e + 0] = -g.x;
+      gv[3 * e + 1] = -g.y;
+      gv[3 * e + 2] = -g.z;
+      key[e] = live && (uint32_t)p < numPoints ? (uint32_t)p : numPoints;
+      val[e] = (uint32_t)e;
     }
   }
   if (gradQ) {
     gradQ[3 * (size_t)qi] = gq.x;
     gradQ[3 * (size_t)qi + 1] = gq.y;
     gradQ[3 * (size_t)qi + 2] = gq.z;
+  }
+}
+
+// The point gradient from the entries sorted by point id: each wave sums
+// its runs of equal ids (contiguous after the sort) with a segmented scan;
+// a run inside the wave holds all of its point's entries, so it is added
+// with a plain read-modify-write, and only runs that touch the wave's first
+// or last lane (which may continue in the next or previous wave) use
+// atomics.  No float atomic per entry, and the sum order inside a run is
+// fixed.
+__global__ __launch_bounds__(kBlock) void k_point_scatter(const uint32_t *key, const uint32_t *val,
+                                                          const float *gv, uint32_t n, uint32_t numPoints,
+                                                          float *gradP) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t lane = lane_id();
+  const bool act = i < n;
+  const uint32_t p = act ? key[i] : numPoints;
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+  if (act && p < numPoints) {
+    const uint32_t e = val[i];
+    sx = gv[3 * (size_t)e + 0];
+    sy = gv[3 * (size_t)e + 1];
+    sz = gv[3 * (size_t)e + 2];
+  }
+  const uint32_t pPrev = (uint32_t)__shfl_up((int)p, 1, kWave);
+  const bool head = lane == 0 || pPrev != p;
+  bool seen = head;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const float ux = __shfl_up(sx, o, kWave), uy = __shfl_up(sy, o, kWave), uz = __shfl_up(sz, o, kWave);
+    const bool us = __shfl_up((int)seen, o, kWave) != 0;
+    if (lane >= (uint32_t)o && !seen) {
+      sx += ux; sy += uy; sz += uz;
+      seen = us;
+    }
+  }
+  const uint32_t pNext = (uint32_t)__shfl_down((int)p, 1, kWave);
+  const bool runEnd = lane == kWave - 1 || pNext != p;
+  // lane 0 of a run: the lane where the run starts (a ballot over heads)
+  const uint64_t heads = __ballot(head);
+  const uint64_t below = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+  const uint32_t runStart = 63u - (uint32_t)__builtin_clzll(below);
+  if (runEnd && p < numPoints) {
+    float *o = gradP + 3 * (size_t)p;
+    if (runStart == 0 || lane == kWave - 1) {
+      atomicAdd(&o[0], sx);
+      atomicAdd(&o[1], sy);
+      atomicAdd(&o[2], sz);
+    } else {
+      o[0] += sx;
+      o[1] += sy;
+      o[2] += sz;
+    }
   }
 }
 
@@ -274,22 +370,54 @@ extern "C" int cubql_fcp_triangles_backward_f3(const int32_t *d_tris, const floa
   return CUBQL_OK;
 }
 
-extern "C" int cubql_points_distance_backward_f3(const float *d_points, const float *d_queries,
+extern "C" int cubql_points_distance_backward_f3(const float *d_points, uint32_t numPoints, const float *d_queries,
                                                  uint32_t numQueries, int k, const int32_t *d_ids,
                                                  const float *d_gradDist, float *d_gradQueries,
-                                                 float *d_gradPoints, cubql_stream_t s) {
+                                                 float *d_gradPoints, cubql_stream_t stream,
+                                                 const cubql_mem_resource *mem) {
   clear_error();
   if (numQueries == 0) return CUBQL_OK;
   if (!d_points || !d_queries || !d_ids || !d_gradDist || k < 1) {
     set_error("cubql_points_distance_backward_f3: invalid argument");
     return CUBQL_ERR_INVALID_ARG;
   }
-  int ps = prof_begin((hipStream_t)s);
-  tri::k_point_backward<<<div_up(numQueries, tri::kBlock), tri::kBlock, 0, (hipStream_t)s>>>(
-      d_points, d_queries, numQueries, k, d_ids, d_gradDist, d_gradQueries, d_gradPoints);
-  prof_end(ps, (hipStream_t)s, "points_backward");
-  prof_set(ps, (12.0 + 12.0 + k * (4.0 + 4.0 + 12.0 + 24.0)) * numQueries, numQueries);
+  const size_t ne = (size_t)numQueries * (size_t)k;
+  if (ne > 0xffffffffull || numPoints > 0x7fffffffu) {
+    set_error("cubql_points_distance_backward_f3: %zu entries / %u points exceed the 32-bit entry index", ne,
+              numPoints);
+    return CUBQL_ERR_INVALID_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const bool scatter = d_gradPoints && numPoints;
+  ScratchArena scr(mem, s);
+  float *gv = nullptr;
+  uint32_t *kA = nullptr, *vA = nullptr, *kB = nullptr, *vB = nullptr, *cnt = nullptr;
+  if (scatter) {
+    CUBQL_CHECK(scr.reserve(&gv, 3 * ne));
+    CUBQL_CHECK(scr.reserve(&kA, ne));
+    CUBQL_CHECK(scr.reserve(&vA, ne));
+    CUBQL_CHECK(scr.reserve(&kB, ne));
+    CUBQL_CHECK(scr.reserve(&vB, ne));
+    CUBQL_CHECK(scr.reserve(&cnt, 1));
+    CUBQL_CHECK(scr.commit());
+    CUBQL_CHECK(fill_u32(cnt, (uint32_t)ne, 1, s));
+  }
+  int ps = prof_begin(s);
+  tri::k_point_backward<<<div_up(numQueries, tri::kBlock), tri::kBlock, 0, s>>>(
+      d_points, d_queries, numQueries, k, d_ids, d_gradDist, d_gradQueries, gv, kA, vA, numPoints);
   CUBQL_LAUNCH_CHECK();
+  if (scatter) {
+    // key bits: enough for the sentinel numPoints
+    int bits = 1;
+    while (bits < 32 && (numPoints >> bits) != 0) bits++;
+    uint32_t *kRes = nullptr, *vRes = nullptr;
+    CUBQL_CHECK(radix_sort_pairs_dev_u32(kA, vA, kB, vB, cnt, (uint32_t)ne, bits, s, mem, &kRes, &vRes));
+    tri::k_point_scatter<<<div_up((uint32_t)ne, tri::kBlock), tri::kBlock, 0, s>>>(kRes, vRes, gv, (uint32_t)ne,
+                                                                                  numPoints, d_gradPoints);
+    CUBQL_LAUNCH_CHECK();
+  }
+  prof_end(ps, s, "points_backward");
+  prof_set(ps, (12.0 + 12.0 + k * (4.0 + 4.0 + 12.0 + 24.0)) * numQueries, numQueries);
   return CUBQL_OK;
 }
 

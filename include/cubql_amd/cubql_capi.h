@@ -326,14 +326,19 @@ int cubql_fcp_triangles_backward_f3(const int32_t *d_tris, const float *d_verts,
    fcp (k = 1) or kNN call (no reference counterpart):
      gradQueries[i]  = sum_j g_ij (q_i - p)/d_ij          (overwritten)
      gradPoints[p]  += -g_ij (q_i - p)/d_ij               (accumulated)
-   ids < 0 and d = 0 contribute nothing.  Either output may be NULL. */
-int cubql_points_distance_backward_f3(const float *d_points,
+   ids < 0 and d = 0 contribute nothing.  Either output may be NULL.  The
+   point gradient is gathered per point (entries sorted by point id, one
+   sum per point) rather than scattered with a float atomic per entry; its
+   temporaries (28 bytes per entry plus the sort's) come from mem (NULL:
+   hipMallocAsync).  numQueries * k < 2^32. */
+int cubql_points_distance_backward_f3(const float *d_points, uint32_t numPoints,
                                       const float *d_queries,
                                       uint32_t numQueries, int k,
                                       const int32_t *d_ids,
                                       const float *d_gradDist,
                                       float *d_gradQueries,
-                                      float *d_gradPoints, cubql_stream_t s);
+                                      float *d_gradPoints, cubql_stream_t s,
+                                      const cubql_mem_resource *mem);
 
 /* ------------------------------------------------------------- batched */
 

@@ -8,6 +8,7 @@
 // distances the lowest primID wins (kNN lists are ordered by
 // (sqrDist, primID)).
 #include <mutex>
+#include <type_traits>
 
 #include "common.h"
 #include "cubql_amd/traversal.hip.h"
@@ -53,6 +54,32 @@ __device__ __forceinline__ bool fcp_one(const BVH &bvh, const float *points, flo
   return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, bestD, prim, 0u, st);
 }
 
+// The same over a binary tree on a caller-provided traversal stack.
+template <typename StackT>
+__device__ __forceinline__ bool fcp_one_on(StackT &stack, const cubql_bvh3f &bvh, const float *points, float3 q,
+                                           float sqrMax, int32_t &best, float &bestD) {
+  best = -1;
+  bestD = sqrMax;
+  auto prim = [&](uint32_t pid) -> float {
+    float3 p = load_pt(points, pid);
+    float d = cubql_dev::sqr_dist_pt(p.x, p.y, p.z, q.x, q.y, q.z);
+    if (d < bestD || (d == bestD && best >= 0 && (int32_t)pid < best)) {
+      bestD = d;
+      best = (int32_t)pid;
+    }
+    return bestD;
+  };
+  cubql_dev::NoStats st;
+  cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim, 0u, st);
+  return cubql_dev::for_each_prim_on(stack, bvh, q.x, q.y, q.z, bestD, prim, 0u, st);
+}
+
+// Entries of the fast kernel's LDS stack per lane: the seeded radius keeps
+// a closest-point query's stack shallow, and 8 entries x 8 bytes x 256 lanes
+// (16 KB a workgroup) leave the kernel at full occupancy; deeper queries
+// rerun in the deep kernel.
+constexpr int kLdsStack = 8;
+
 // fast path: 64-entry stack; queries that overflow go to the deep kernel.
 // BVH: cubql_bvh3f (binary) or cubql_dev::WideBVH<W>.
 template <typename BVH>
@@ -67,7 +94,15 @@ __global__ __launch_bounds__(kBlock) void k_fcp(BVH bvh, const float *points,
   float3 q = load_pt(queries, qi);
   int32_t best;
   float bestD;
-  if (!fcp_one<kFastStack>(bvh, points, q, sqrMax, best, bestD)) {
+  bool done;
+  if constexpr (std::is_same<BVH, cubql_bvh3f>::value) {
+    __shared__ uint2 sStack[kLdsStack * kBlock];
+    cubql_dev::LdsStack<kLdsStack, kBlock> stack(sStack + threadIdx.x);
+    done = fcp_one_on(stack, bvh, points, q, sqrMax, best, bestD);
+  } else {
+    done = fcp_one<kFastStack>(bvh, points, q, sqrMax, best, bestD);
+  }
+  if (!done) {
     uint32_t slot = atomicAdd(&ovf->count, 1u);
     ovfList[slot] = qi;
     return;

@@ -81,15 +81,37 @@ struct TraversalCounts {
   __device__ __forceinline__ void prim() { prims++; }
 };
 
+// Traversal stack storage for (node id, distance bits) entries; the
+// traversal keeps the stack pointer itself (in a register).  PrivateStack
+// is an array in the lane's private memory (the compiler places it in
+// scratch); LdsStack keeps N entries per lane in LDS, entry j of a lane at
+// base[j * STRIDE] (base = the workgroup's array + the lane's thread index,
+// STRIDE = the workgroup size), so a wave's accesses to one depth fall in
+// consecutive banks.
+template <int N>
+struct PrivateStack {
+  static constexpr int kCap = N;
+  uint2 e[N];
+  __device__ __forceinline__ void put(int i, uint2 v) { e[i] = v; }
+  __device__ __forceinline__ uint2 get(int i) const { return e[i]; }
+};
+template <int N, int STRIDE>
+struct LdsStack {
+  static constexpr int kCap = N;
+  uint2 *base;
+  __device__ __forceinline__ explicit LdsStack(uint2 *b) : base(b) {}
+  __device__ __forceinline__ void put(int i, uint2 v) { base[i * STRIDE] = v; }
+  __device__ __forceinline__ uint2 get(int i) const { return base[i * STRIDE]; }
+};
+
 // forEachLeaf.  leaf(const uint32_t *prims, uint32_t count) -> float.
 // Returns false if the stack overflowed (query incomplete).
 // `root` selects the subtree to walk (0 for a whole BinaryBVH; a cloud's
 // root node in a batched forest).
-template <int STACK, typename LeafFn, typename Stats>
-__device__ __forceinline__ bool for_each_leaf(const cubql_bvh3f &bvh, float qx, float qy, float qz,
-                                              float sqrMax, LeafFn &&leaf, uint32_t root, Stats &st) {
+template <typename StackT, typename LeafFn, typename Stats>
+__device__ __forceinline__ bool for_each_leaf_on(StackT &stack, const cubql_bvh3f &bvh, float qx, float qy,
+                                                 float qz, float sqrMax, LeafFn &&leaf, uint32_t root, Stats &st) {
   if (bvh.numNodes == 0) return true;
-  uint2 stack[STACK];
   int sp = 0;
   float cull = sqrMax;
   uint64_t node = node_oc(load_node(bvh.nodes, root));
@@ -107,8 +129,8 @@ __device__ __forceinline__ bool for_each_leaf(const cubql_bvh3f &bvh, float qx, 
       node = near0 ? node_oc(n0) : node_oc(n1);
       const float dfar = fmaxf(d0, d1);
       if (dfar <= cull) {
-        if (sp >= STACK) return false;
-        stack[sp++] = make_uint2(near0 ? i0 + 1 : i0, __float_as_uint(dfar));
+        if (sp >= StackT::kCap) return false;
+        stack.put(sp++, make_uint2(near0 ? i0 + 1 : i0, __float_as_uint(dfar)));
       }
     }
     if (haveLeaf) {
@@ -118,7 +140,7 @@ __device__ __forceinline__ bool for_each_leaf(const cubql_bvh3f &bvh, float qx, 
     }
     bool found = false;
     while (sp > 0) {
-      uint2 e = stack[--sp];
+      uint2 e = stack.get(--sp);
       if (__uint_as_float(e.y) <= cull) {
         node = node_oc(load_node(bvh.nodes, e.x));
         st.nodes(32);
@@ -128,6 +150,12 @@ __device__ __forceinline__ bool for_each_leaf(const cubql_bvh3f &bvh, float qx, 
     }
     if (!found) return true;
   }
+}
+template <int STACK, typename LeafFn, typename Stats>
+__device__ __forceinline__ bool for_each_leaf(const cubql_bvh3f &bvh, float qx, float qy, float qz,
+                                              float sqrMax, LeafFn &&leaf, uint32_t root, Stats &st) {
+  PrivateStack<STACK> stack;
+  return for_each_leaf_on(stack, bvh, qx, qy, qz, sqrMax, leaf, root, st);
 }
 template <int STACK, typename LeafFn>
 __device__ __forceinline__ bool for_each_leaf(const cubql_bvh3f &bvh, float qx, float qy, float qz,
@@ -179,6 +207,17 @@ __device__ __forceinline__ bool for_each_prim(const cubql_bvh3f &bvh, float qx, 
                                               float sqrMax, PrimFn &&prim, uint32_t root = 0) {
   NoStats st;
   return for_each_prim<STACK>(bvh, qx, qy, qz, sqrMax, prim, root, st);
+}
+// forEachPrim on a caller-provided stack (PrivateStack / LdsStack).
+template <typename StackT, typename PrimFn, typename Stats>
+__device__ __forceinline__ bool for_each_prim_on(StackT &stack, const cubql_bvh3f &bvh, float qx, float qy,
+                                                 float qz, float sqrMax, PrimFn &&prim, uint32_t root, Stats &st) {
+  auto leaf = [&](const uint32_t *ids, uint32_t cnt) -> float {
+    float r = INFINITY;
+    for (uint32_t i = 0; i < cnt; i++) r = fminf(r, prim(ids[i]));
+    return r;
+  };
+  return for_each_leaf_on(stack, bvh, qx, qy, qz, sqrMax, leaf, root, st);
 }
 
 }  // namespace cubql_dev

@@ -314,24 +314,69 @@ static void refit_launch(const RCtx &c, Node *nodes, const uint32_t *primIDs, in
 // which grows strictly from a node to its split children, so every child
 // is final before its parent.
 
-// The sorted keys as one workgroup reads them: positions [lo, hi) from an
-// LDS copy, the rest from global memory.  delta(i, j) is the common prefix
-// length of keys i and j, positions breaking ties of equal keys (-1 out of
-// range).
+// Readers of the sorted keys for the radix-tree search: from the
+// workgroup's LDS copy of positions [lo, hi) (a read outside sets `miss`
+// and returns 0; the node is then redone from global memory), or from
+// global memory.  Two separate code paths keep LDS reads ds_ and global
+// reads global_ loads (a merged read is a slower flat load).
 template <typename K>
-struct KeyWindow {
-  const K *g;
+struct LdsKeys {
   const K *w;
-  int64_t lo, hi, n;
-  __device__ __forceinline__ K at(int64_t j) const { return (j >= lo && j < hi) ? w[j - lo] : g[j]; }
-  __device__ __forceinline__ int delta(int64_t i, int64_t j) const {
-    if (j < 0 || j >= n) return -1;
-    const K a = at(i), b = at(j);
-    constexpr int kBits = 8 * (int)sizeof(K);
-    if (a == b) return kBits + __clz((uint32_t)(i ^ j));
-    return sizeof(K) == 8 ? __clzll((unsigned long long)(a ^ b)) : __clz((uint32_t)(a ^ b));
+  int64_t lo, hi;
+  bool miss = false;
+  template <typename I>
+  __device__ __forceinline__ K at(I j) {
+    if (j < lo || j >= hi) {
+      miss = true;
+      return (K)0;
+    }
+    return w[j - lo];
   }
 };
+template <typename K>
+struct GlobalKeys {
+  const K *g;
+  template <typename I>
+  __device__ __forceinline__ K at(I j) { return g[j]; }
+};
+
+// Common prefix length of key ki (at position i) and the key at j,
+// positions breaking ties of equal keys (-1 out of range).  I: the index
+// type (32-bit when n < 2^30, so that i + 2 * range never overflows).
+template <typename K, typename I, typename R>
+__device__ __forceinline__ int key_delta(R &rd, I n, I i, K ki, I j) {
+  if (j < 0 || j >= n) return -1;
+  const K b = rd.at(j);
+  constexpr int kBits = 8 * (int)sizeof(K);
+  if (ki == b) return kBits + __clz((uint32_t)(i ^ j));
+  return sizeof(K) == 8 ? __clzll((unsigned long long)(ki ^ b)) : __clz((uint32_t)(ki ^ b));
+}
+
+// Range [first, last] and split gamma of radix-tree node i (Karras 2012),
+// dn = the prefix length of the range.
+template <typename K, typename I, typename R>
+__device__ __forceinline__ void karras_node(R &rd, I n, I i, uint32_t &first, uint32_t &last, uint32_t &gamma,
+                                            int &dn) {
+  const K ki = rd.at(i);
+  const int d = key_delta<K, I>(rd, n, i, ki, i + 1) - key_delta<K, I>(rd, n, i, ki, i - 1) > 0 ? 1 : -1;
+  const int dmin = key_delta<K, I>(rd, n, i, ki, i - d);
+  I lmax = 2;
+  while (key_delta<K, I>(rd, n, i, ki, i + lmax * d) > dmin) lmax *= 2;
+  I l = 0;
+  for (I t = lmax / 2; t >= 1; t /= 2)
+    if (key_delta<K, I>(rd, n, i, ki, i + (l + t) * d) > dmin) l += t;
+  const I j = i + l * d;
+  dn = key_delta<K, I>(rd, n, i, ki, j);
+  I sp = 0;
+  for (int sh = 1;; sh++) {
+    const I t = (l + ((I)1 << sh) - 1) >> sh;  // ceil(l / 2^sh)
+    if (key_delta<K, I>(rd, n, i, ki, i + (sp + t) * d) > dn) sp += t;
+    if (t == 1) break;
+  }
+  gamma = (uint32_t)(i + sp * d + (d < 0 ? -1 : 0));
+  first = (uint32_t)(i < j ? i : j);
+  last = (uint32_t)(i < j ? j : i);
+}
 
 template <typename K>
 __device__ __forceinline__ int key_prefix(K a, K b) {
@@ -367,10 +412,12 @@ __host__ __device__ inline uint32_t mid_inner(uint32_t m, uint32_t T) {
 // last bin counts the middle-split subtree roots.
 constexpr int kDeltaBins = 128;
 constexpr int kBinEqRoots = kDeltaBins - 1;
-// radix-tree nodes per workgroup of the hierarchy kernels (kBlock threads,
-// kKPer nodes each, strided); bucket slots are assigned per such block
-constexpr int kKPer = 8;
-constexpr uint32_t kKBlock = kBlock * kKPer;
+// radix-tree nodes per workgroup of the hierarchy kernels (kKThreads
+// threads, kKPer nodes each, strided); bucket slots are assigned per such
+// block
+constexpr int kKThreads = 1024;
+constexpr int kKPer = 2;
+constexpr uint32_t kKBlock = kKThreads * kKPer;
 
 struct KNode {
   uint32_t first, last, gamma, pad;
@@ -383,95 +430,163 @@ struct EqRoot {
   uint32_t pos, begin, size, pairBase;
 };
 
-// boxes in sorted prim order: sorted[i] = boxes[primIDs[i]], one thread
-// per prim (the one random gather of the build, at full memory parallelism)
-__global__ __launch_bounds__(kBlock) void k_gather_sorted(const cubql_box3f *boxes, const uint32_t *primIDs,
-                                                          const uint32_t *nPtr, cubql_box3f *sorted) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= *nPtr) return;
-  const float2 *src = reinterpret_cast<const float2 *>(boxes + primIDs[i]);
-  float2 *dst = reinterpret_cast<float2 *>(sorted + i);
-  const float2 a = src[0], b = src[1], c = src[2];
-  dst[0] = a;
-  dst[1] = b;
-  dst[2] = c;
-}
-
-// box of prims [b, b + m) (sorted boxes)
-__device__ __forceinline__ void range_box(const cubql_box3f *sorted, uint32_t b, uint32_t m, Node &nd) {
+// box of sorted prims [b, b + m): their ids are contiguous, their boxes
+// are gathered (eight in flight at a time; for input in random order each
+// box costs a line read)
+__device__ __forceinline__ void range_box(const cubql_box3f *boxes, const uint32_t *primIDs, uint32_t b,
+                                          uint32_t m, Node &nd) {
   for (int d = 0; d < 3; d++) { nd.lo[d] = INFINITY; nd.hi[d] = -INFINITY; }
-  for (uint32_t j = 0; j < m; j++) {
-    const cubql_box3f bx = load_box(sorted, b + j);
-    for (int d = 0; d < 3; d++) { nd.lo[d] = fminf(nd.lo[d], bx.lower[d]); nd.hi[d] = fmaxf(nd.hi[d], bx.upper[d]); }
+  for (uint32_t j0 = 0; j0 < m; j0 += 8) {
+    uint32_t id[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (j0 + u < m) id[u] = primIDs[b + j0 + u];
+    cubql_box3f bx[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (j0 + u < m) bx[u] = load_box(boxes, id[u]);
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (j0 + u < m)
+        for (int d = 0; d < 3; d++) {
+          nd.lo[d] = fminf(nd.lo[d], bx[u].lower[d]);
+          nd.hi[d] = fmaxf(nd.hi[d], bx[u].upper[d]);
+        }
   }
 }
 
-// Radix-tree nodes: range, split, whether a node is a split node, the inner
-// nodes of the middle-split subtrees under it; the workgroup's bucket
-// counts go to row-per-bin blockHist[bin * numKBlocks + block].
+// The per-node outputs of the radix-tree pass: the node record, whether it
+// is a split node (more than T prims, two different keys), the inner nodes
+// of the middle-split subtrees under it, and its refit-bucket counts
+// (add(bin, count): into LDS in the block kernel, into the block's row in
+// global memory for the deferred nodes).
+template <typename K, typename Add>
+__device__ __forceinline__ void karras_emit_counts(const K *keys, uint32_t T, uint32_t i, uint32_t first,
+                                                   uint32_t last, uint32_t g, int dn, KNode *kn,
+                                                   uint32_t *splitFlag, uint32_t *eqPairs, Add &&add) {
+  constexpr int kBits = 8 * (int)sizeof(K);
+  kn[i] = KNode{first, last, g, 0u};
+  const bool split = last - first + 1 > T && keys[first] != keys[last];
+  splitFlag[i] = split ? 1u : 0u;
+  uint32_t eq = 0;
+  if (split) {
+    // (split nodes: the keys of the range differ, so dn < key bits)
+    add(dn, 1u);
+    const uint32_t cb[2] = {first, g + 1}, ce[2] = {g, last};
+    for (int side = 0; side < 2; side++) {
+      const uint32_t m = ce[side] - cb[side] + 1;
+      if (m > T && keys[cb[side]] == keys[ce[side]]) {
+        const MidShape ms = mid_shape(m, T);
+        for (uint32_t t = 0; t <= ms.D; t++)
+          if (const uint32_t c = ms.at(t)) add(kBits + 1 + (int)t, c);
+        eq += mid_inner(m, T);
+        add(kBinEqRoots, 1u);
+      }
+    }
+  }
+  eqPairs[i] = eq;
+}
+
 // keys a workgroup of k_karras stages in LDS: its nodes and kKWin on each
-// side (the searches of all but the few nodes with long ranges stay inside)
+// side.  A node whose search reads outside (one with a long range, about
+// one in five hundred) is deferred to k_karras_far.
 constexpr int kKWin = 1024;
 
 template <typename K>
-__global__ __launch_bounds__(kBlock) void k_karras(const K *keys, const uint32_t *nPtr, uint32_t T, KNode *kn,
+__global__ __launch_bounds__(kKThreads) void k_karras(const K *keys, const uint32_t *nPtr, uint32_t T, KNode *kn,
                                                    uint32_t *splitFlag, uint32_t *eqPairs, uint32_t *blockHist,
-                                                   uint32_t numKBlocks) {
-  constexpr int kBits = 8 * (int)sizeof(K);
+                                                   uint32_t numKBlocks, uint32_t *farList, uint32_t *farCount) {
   __shared__ uint32_t sh[kDeltaBins];
   __shared__ K sKeys[kKBlock + 2 * kKWin];
   const uint32_t n = *nPtr;
   const int64_t N = n;
   const int64_t b0 = (int64_t)blockIdx.x * kKBlock;
-  KeyWindow<K> kw{keys, sKeys, b0 - kKWin, b0 + kKBlock + kKWin, N};
-  if (kw.lo < 0) kw.lo = 0;
-  if (kw.hi > N) kw.hi = N;
-  for (int b = threadIdx.x; b < kDeltaBins; b += kBlock) sh[b] = 0;
-  for (int64_t j = kw.lo + threadIdx.x; j < kw.hi; j += kBlock) sKeys[j - kw.lo] = keys[j];
+  const int64_t wlo = b0 - kKWin < 0 ? 0 : b0 - kKWin;
+  const int64_t whi = b0 + kKBlock + kKWin > N ? N : b0 + kKBlock + kKWin;
+  for (int b = threadIdx.x; b < kDeltaBins; b += kKThreads) sh[b] = 0;
+  for (int64_t j = wlo + threadIdx.x; j < whi; j += kKThreads) sKeys[j - wlo] = keys[j];
   __syncthreads();
   for (int r = 0; r < kKPer; r++) {
-    const int64_t i = b0 + r * kBlock + threadIdx.x;
+    const int64_t i = b0 + r * kKThreads + threadIdx.x;
     if (i >= N - 1) break;
-    const int d = kw.delta(i, i + 1) - kw.delta(i, i - 1) > 0 ? 1 : -1;
-    const int dmin = kw.delta(i, i - d);
-    int64_t lmax = 2;
-    while (kw.delta(i, i + lmax * d) > dmin) lmax *= 2;
-    int64_t l = 0;
-    for (int64_t t = lmax / 2; t >= 1; t /= 2)
-      if (kw.delta(i, i + (l + t) * d) > dmin) l += t;
-    const int64_t j = i + l * d;
-    const int dn = kw.delta(i, j);
-    int64_t sp = 0;
-    for (int sh2 = 1;; sh2++) {
-      const int64_t t = (l + ((int64_t)1 << sh2) - 1) >> sh2;  // ceil(l / 2^sh2)
-      if (kw.delta(i, i + (sp + t) * d) > dn) sp += t;
-      if (t == 1) break;
+    uint32_t first, last, g;
+    int dn;
+    LdsKeys<K> lk{sKeys, wlo, whi};
+    if (N < (1 << 30)) {
+      karras_node<K, int32_t>(lk, (int32_t)N, (int32_t)i, first, last, g, dn);
+    } else {
+      karras_node<K, int64_t>(lk, N, i, first, last, g, dn);
     }
-    const int64_t g = i + sp * d + (d < 0 ? -1 : 0);
-    const uint32_t first = (uint32_t)(i < j ? i : j), last = (uint32_t)(i < j ? j : i);
-    kn[i] = KNode{first, last, (uint32_t)g, 0u};
-    const bool split = last - first + 1 > T && kw.at(first) != kw.at(last);
-    splitFlag[i] = split ? 1u : 0u;
-    uint32_t eq = 0;
-    if (split) {
-      // (split nodes: the keys of the range differ, so dn < key bits)
-      atomicAdd(&sh[dn], 1u);
-      const uint32_t cb[2] = {first, (uint32_t)g + 1}, ce[2] = {(uint32_t)g, last};
-      for (int side = 0; side < 2; side++) {
-        const uint32_t m = ce[side] - cb[side] + 1;
-        if (m > T && kw.at(cb[side]) == kw.at(ce[side])) {
-          const MidShape ms = mid_shape(m, T);
-          for (uint32_t t = 0; t <= ms.D; t++)
-            if (const uint32_t c = ms.at(t)) atomicAdd(&sh[kBits + 1 + t], c);
-          eq += mid_inner(m, T);
-          atomicAdd(&sh[kBinEqRoots], 1u);
-        }
-      }
+    // deferred nodes: one atomic per wave claims the wave's list slots
+    const uint64_t far = __ballot(lk.miss);
+    if (far) {
+      const uint32_t leader = (uint32_t)__builtin_ctzll(far);
+      uint32_t base = 0;
+      if (lane_id() == leader) base = atomicAdd(farCount, (uint32_t)__popcll(far));
+      base = (uint32_t)__shfl((int)base, (int)leader, kWave);
+      if (lk.miss) farList[base + (uint32_t)__popcll(far & ((1ull << lane_id()) - 1ull))] = (uint32_t)i;
     }
-    eqPairs[i] = eq;
+    if (lk.miss) continue;
+    karras_emit_counts<K>(keys, T, (uint32_t)i, first, last, g, dn, kn, splitFlag, eqPairs,
+                          [&](int bin, uint32_t c) { atomicAdd(&sh[bin], c); });
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < kDeltaBins; b += kBlock) blockHist[(size_t)b * numKBlocks + blockIdx.x] = sh[b];
+  for (int b = threadIdx.x; b < kDeltaBins; b += kKThreads) blockHist[(size_t)b * numKBlocks + blockIdx.x] = sh[b];
+}
+
+// Largest t in [0, tmax] with delta(i, i + t d) > thr, for one wave (all
+// lanes get it): delta falls as t grows (sorted keys), so the test is true
+// on a prefix of t.  Lane L first tests t = 2^L, then the bracket is cut in
+// 64 per round; a 2^24 range takes five rounds of one load per lane.
+template <typename K>
+__device__ __forceinline__ int64_t wave_last_true(const K *keys, int64_t n, int64_t i, K ki, int d, int thr,
+                                                  int64_t tmax) {
+  GlobalKeys<K> gk{keys};
+  const uint32_t lane = lane_id();
+  const int64_t t0 = lane < 62 ? ((int64_t)1 << lane) : (int64_t)1 << 62;
+  const bool ok0 = lane < 62 && t0 <= tmax && key_delta<K, int64_t>(gk, n, i, ki, i + t0 * d) > thr;
+  const uint32_t k = (uint32_t)__popcll(__ballot(ok0));
+  if (k == 0) return 0;
+  int64_t lo = (int64_t)1 << (k - 1);
+  int64_t hi = k < 62 ? ((int64_t)1 << k) : tmax + 1;
+  if (hi > tmax + 1) hi = tmax + 1;
+  while (hi - lo > 1) {
+    const int64_t step = (hi - lo + kWave - 1) / kWave;
+    const int64_t t = lo + (int64_t)(lane + 1) * step;
+    const bool ok = t < hi && key_delta<K, int64_t>(gk, n, i, ki, i + t * d) > thr;
+    lo += (int64_t)__popcll(__ballot(ok)) * step;
+    if (lo + step < hi) hi = lo + step;
+  }
+  return lo;
+}
+
+// The deferred nodes, one wave each, reading keys from global memory.
+template <typename K>
+__global__ __launch_bounds__(256) void k_karras_far(const K *keys, const uint32_t *nPtr, uint32_t T,
+                                                    const uint32_t *farList, const uint32_t *farCount, KNode *kn,
+                                                    uint32_t *splitFlag, uint32_t *eqPairs, uint32_t *blockHist,
+                                                    uint32_t numKBlocks) {
+  const int64_t N = *nPtr;
+  const uint32_t cnt = *farCount;
+  const uint32_t nw = gridDim.x * (256 / kWave);
+  for (uint32_t q = (blockIdx.x * 256 + threadIdx.x) / kWave; q < cnt; q += nw) {
+    const int64_t i = farList[q];
+    GlobalKeys<K> gk{keys};
+    const K ki = keys[i];
+    const int d = key_delta<K, int64_t>(gk, N, i, ki, i + 1) - key_delta<K, int64_t>(gk, N, i, ki, i - 1) > 0 ? 1 : -1;
+    const int dmin = key_delta<K, int64_t>(gk, N, i, ki, i - d);
+    const int64_t l = wave_last_true<K>(keys, N, i, ki, d, dmin, d > 0 ? N - 1 - i : i);
+    const int64_t j = i + l * d;
+    const int dn = key_delta<K, int64_t>(gk, N, i, ki, j);
+    const int64_t sp = wave_last_true<K>(keys, N, i, ki, d, dn, l);
+    const uint32_t g = (uint32_t)(i + sp * d + (d < 0 ? -1 : 0));
+    const uint32_t first = (uint32_t)(i < j ? i : j), last = (uint32_t)(i < j ? j : i);
+    if (lane_id() == 0) {
+      uint32_t *row = blockHist + (size_t)(i / kKBlock);
+      karras_emit_counts<K>(keys, T, (uint32_t)i, first, last, g, dn, kn, splitFlag, eqPairs,
+                            [&](int bin, uint32_t c) { atomicAdd(&row[(size_t)bin * numKBlocks], c); });
+    }
+  }
 }
 
 // One workgroup per bin: exclusive scan of the bin's row over the blocks
@@ -492,13 +607,51 @@ __global__ __launch_bounds__(1024) void k_karras_binscan(uint32_t *blockHist, ui
   if (threadIdx.x == 0) binTotal[blockIdx.x] = run;
 }
 
-// exclusive scan over the n - 1 radix-tree nodes (n = *nPtr valid prims)
-__global__ __launch_bounds__(1024) void k_karras_scan(const uint32_t *count, uint32_t *out, const uint32_t *nPtr,
-                                                      uint64_t *status, uint32_t tag, uint32_t *total,
-                                                      uint32_t *err) {
-  __shared__ uint32_t smem[1024 / kWave + 2];
+// Exclusive scans of two per-node counts over the n - 1 radix-tree nodes
+// (n = *nPtr valid prims) in one pass: one block scan for both, then waves
+// 0 and 1 run the two tile look-backs (status arrays stA, stB) at once.
+// listA[oA[i]] = i where cA[i] is 1 (the split nodes in position order).
+__global__ __launch_bounds__(1024) void k_karras_scan2(const uint32_t *cA, const uint32_t *cB, uint32_t *oA,
+                                                       uint32_t *oB, uint32_t *listA, const uint32_t *nPtr,
+                                                       uint64_t *stA, uint64_t *stB, uint32_t *totA,
+                                                       uint32_t *totB, uint32_t *err) {
+  __shared__ uint32_t smem[2 * (1024 / kWave + 1)];
+  __shared__ uint32_t sPre[2];
   const uint32_t n = *nPtr;
-  count_scan_tile(count, out, n ? n - 1 : 0u, 0u, status, tag, total, smem, 0u, err);
+  const uint32_t m = n ? n - 1 : 0u;
+  const uint32_t i0 = blockIdx.x * kCountScanTile + threadIdx.x * kCountScanPer;
+  uint32_t a[kCountScanPer], b[kCountScanPer];
+  uint32_t v[2] = {0u, 0u};
+#pragma unroll
+  for (uint32_t j = 0; j < kCountScanPer; j++) {
+    a[j] = i0 + j < m ? cA[i0 + j] : 0u;
+    b[j] = i0 + j < m ? cB[i0 + j] : 0u;
+    v[0] += a[j];
+    v[1] += b[j];
+  }
+  uint32_t ex[2], tot[2];
+  block_excl_scan_multi<1024, 2>(v, ex, tot, smem);
+  const uint32_t w = threadIdx.x / kWave;
+  if (w < 2) {
+    const uint32_t t = w == 0 ? tot[0] : tot[1];
+    const uint32_t p = lookback_exclusive_wave(w == 0 ? stA : stB, blockIdx.x, blockIdx.x, 1u, t, false, err);
+    if (lane_id() == 0) {
+      sPre[w] = p;
+      if (blockIdx.x == gridDim.x - 1) *(w == 0 ? totA : totB) = p + t;
+    }
+  }
+  __syncthreads();
+  uint32_t oa = sPre[0] + ex[0], ob = sPre[1] + ex[1];
+#pragma unroll
+  for (uint32_t j = 0; j < kCountScanPer; j++) {
+    if (i0 + j < m) {
+      oA[i0 + j] = oa;
+      oB[i0 + j] = ob;
+      if (a[j]) listA[oa] = i0 + j;
+    }
+    oa += a[j];
+    ob += b[j];
+  }
 }
 
 // info[3] = 1 when the root is not a split node: at most T valid prims, or
@@ -539,62 +692,70 @@ __global__ __launch_bounds__(256) void k_karras_leaf_root(const cubql_box3f *box
 // ... or one middle-split subtree over all prims
 __global__ void k_karras_root_eq(const uint32_t *nPtr, EqRoot *roots) { roots[0] = EqRoot{0u, 0u, *nPtr, 2u}; }
 
-// Children of every split node: leaves (with their boxes), split nodes
-// (their position), roots of middle-split subtrees; each split node's slot
-// goes into its prefix-length bucket at its block's offset.
+// Children of every split node (one thread per split node, from the list
+// of split nodes): leaves with their boxes, split nodes (their position),
+// roots of middle-split subtrees (claimed with one atomic per wave).  Each
+// split node's refit bucket is its prefix length: dnKey[s] = dn, dnVal[s]
+// = s, sorted by key afterwards.
 template <typename K>
-__global__ __launch_bounds__(kBlock) void k_karras_emit(const K *keys, const cubql_box3f *sorted, uint32_t n,
-                                                        uint32_t T,
-                                                        const KNode *kn, const uint32_t *splitFlag,
-                                                        const uint32_t *slot, const uint32_t *eqBase,
-                                                        uint32_t numSplit, const uint32_t *bucketStart,
-                                                        const uint32_t *blockOff, uint32_t numKBlocks,
-                                                        uint32_t *bucket, uint32_t *pos, Node *nodes,
-                                                        EqRoot *eqRoots) {
-  __shared__ uint32_t sRank[kDeltaBins];
-  for (int b = threadIdx.x; b < kDeltaBins; b += kBlock) sRank[b] = 0;
-  __syncthreads();
-  for (int r = 0; r < kKPer; r++) {
-    const uint32_t i = blockIdx.x * kKBlock + r * kBlock + threadIdx.x;
-    if (i >= n - 1) break;
-    if (!splitFlag[i]) continue;
+__global__ __launch_bounds__(kBlock) void k_karras_emit(const K *keys, const cubql_box3f *boxes,
+                                                        const uint32_t *primIDs, uint32_t T,
+                                                        const KNode *kn, const uint32_t *splitList,
+                                                        uint32_t numSplit, const uint32_t *slot,
+                                                        const uint32_t *eqPairs, const uint32_t *eqBase,
+                                                        uint32_t *dnKey, uint32_t *dnVal, uint32_t *pos,
+                                                        Node *nodes, EqRoot *eqRoots, uint32_t *eqRootCursor) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  const bool active = s < numSplit;
+  uint32_t eqCount = 0;
+  EqRoot er[2];
+  if (active) {
+    const uint32_t i = splitList[s];
     const KNode k = kn[i];
-    const uint32_t s = slot[i];
     const uint32_t base = 2 + 2 * s;
-    if (i == 0) {
+    if (s == 0) {
+      // the root (node 0, the first split node)
       Node nd;
       for (int d = 0; d < 3; d++) { nd.lo[d] = INFINITY; nd.hi[d] = -INFINITY; }
       nd.oc = pack_oc(2, 0);
       nodes[0] = nd;
       pos[0] = 0;
     }
-    const int dn = key_prefix(keys[k.first], keys[k.last]);
-    bucket[bucketStart[dn] + blockOff[(size_t)dn * numKBlocks + blockIdx.x] + atomicAdd(&sRank[dn], 1u)] = s;
-    uint32_t eqPair = 2 + 2 * numSplit + 2 * eqBase[i];
+    dnKey[s] = (uint32_t)key_prefix(keys[k.first], keys[k.last]);
+    dnVal[s] = s;
+    uint32_t eqPair = eqPairs[i] ? 2 + 2 * numSplit + 2 * eqBase[i] : 0u;
     const uint32_t cb[2] = {k.first, k.gamma + 1}, ce[2] = {k.gamma, k.last};
     for (int side = 0; side < 2; side++) {
       const uint32_t m = ce[side] - cb[side] + 1;
-      Node nd;
       if (m <= T) {
-        range_box(sorted, cb[side], m, nd);
+        Node nd;
+        range_box(boxes, primIDs, cb[side], m, nd);
         nd.oc = pack_oc(cb[side], m);
+        nodes[base + side] = nd;
+      } else if (keys[cb[side]] == keys[ce[side]]) {
+        // equal keys: a middle-split subtree, built by k_karras_equal,
+        // which also writes its root node
+        er[eqCount++] = EqRoot{base + side, cb[side], m, eqPair};
+        eqPair += 2 * mid_inner(m, T);
       } else {
+        // the child range is the radix-tree node at its split end
+        const uint32_t cs = slot[side == 0 ? k.gamma : k.gamma + 1];
+        Node nd;
         for (int d = 0; d < 3; d++) { nd.lo[d] = INFINITY; nd.hi[d] = -INFINITY; }
-        if (keys[cb[side]] == keys[ce[side]]) {
-          // equal keys: a middle-split subtree, built by k_karras_equal
-          nd.oc = pack_oc(eqPair, 0);
-          const uint32_t e = blockOff[(size_t)kBinEqRoots * numKBlocks + blockIdx.x] + atomicAdd(&sRank[kBinEqRoots], 1u);
-          eqRoots[e] = EqRoot{base + side, cb[side], m, eqPair};
-          eqPair += 2 * mid_inner(m, T);
-        } else {
-          // the child range is the radix-tree node at its split end
-          const uint32_t cs = slot[side == 0 ? k.gamma : k.gamma + 1];
-          nd.oc = pack_oc(2 + 2 * cs, 0);
-          pos[cs] = base + side;
-        }
+        nd.oc = pack_oc(2 + 2 * cs, 0);
+        nodes[base + side] = nd;
+        pos[cs] = base + side;
       }
-      nodes[base + side] = nd;
     }
+  }
+  // equal-key subtree roots: one atomic per wave claims the wave's slots
+  const uint32_t incl = wave_incl_scan(eqCount);
+  const uint32_t waveTot = (uint32_t)__shfl((int)incl, kWave - 1, kWave);
+  if (waveTot) {
+    uint32_t e0 = 0;
+    if (lane_id() == kWave - 1) e0 = atomicAdd(eqRootCursor, waveTot);
+    e0 = (uint32_t)__shfl((int)e0, kWave - 1, kWave) + incl - eqCount;
+    for (uint32_t q = 0; q < eqCount; q++) eqRoots[e0 + q] = er[q];
   }
 }
 
@@ -605,7 +766,7 @@ __global__ __launch_bounds__(kBlock) void k_karras_emit(const K *keys, const cub
 // node before k holds q or q + 1 prims).  Each inner node writes itself,
 // its leaf children with their boxes, and its position into its depth's
 // bucket (slots claimed by one atomic per wave and depth).
-__global__ __launch_bounds__(256) void k_karras_equal(const cubql_box3f *sorted, uint32_t T, const EqRoot *roots, uint32_t nr, int kBits,
+__global__ __launch_bounds__(256) void k_karras_equal(const cubql_box3f *boxes, const uint32_t *primIDs, uint32_t T, const EqRoot *roots, uint32_t nr, int kBits,
                                                       const uint32_t *bucketStart, uint32_t *bucketCursor,
                                                       uint32_t *bucket, Node *nodes) {
   const uint32_t lane = lane_id();
@@ -655,7 +816,7 @@ __global__ __launch_bounds__(256) void k_karras_equal(const cubql_box3f *sorted,
           for (int side = 0; side < 2; side++)
             if (cs[side] <= T) {
               Node lf;
-              range_box(sorted, er.begin + cbeg[side], cs[side], lf);
+              range_box(boxes, primIDs, er.begin + cbeg[side], cs[side], lf);
               lf.oc = pack_oc(er.begin + cbeg[side], cs[side]);
               nodes[pair + side] = lf;
             }
@@ -699,24 +860,28 @@ __device__ __forceinline__ void karras_refit_one(uint32_t e, bool slots, const u
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_karras_refit(const uint32_t *bucket, uint32_t begin, uint32_t count,
+__global__ __launch_bounds__(kBlock) void k_karras_refit(const uint32_t *list, uint32_t begin, uint32_t count,
                                                          bool slots, const uint32_t *pos, Node *nodes) {
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-  if (t < count) karras_refit_one(bucket[begin + t], slots, pos, nodes);
+  if (t < count) karras_refit_one(list[begin + t], slots, pos, nodes);
 }
 
 // The narrow buckets in one workgroup, in order; its barriers order them
 // (one CU: a bucket's node writes are seen by the later buckets' reads).
+// Split-node buckets (slots) are ranges of the slots sorted by prefix
+// length; middle-split buckets are ranges of node positions.
 struct RefitRange {
   uint32_t begin, count, slots, pad;
 };
 constexpr int kRefitTopBuckets = kDeltaBins;
-__global__ __launch_bounds__(1024) void k_karras_refit_top(const uint32_t *bucket, const RefitRange *ranges,
-                                                           int numRanges, const uint32_t *pos, Node *nodes) {
+__global__ __launch_bounds__(1024) void k_karras_refit_top(const uint32_t *slotList, const uint32_t *posList,
+                                                           const RefitRange *ranges, int numRanges,
+                                                           const uint32_t *pos, Node *nodes) {
   for (int r = 0; r < numRanges; r++) {
     const RefitRange rr = ranges[r];
+    const uint32_t *list = rr.slots ? slotList : posList;
     for (uint32_t t = threadIdx.x; t < rr.count; t += 1024)
-      karras_refit_one(bucket[rr.begin + t], rr.slots != 0, pos, nodes);
+      karras_refit_one(list[rr.begin + t], rr.slots != 0, pos, nodes);
     __syncthreads();
   }
 }
@@ -781,15 +946,15 @@ int build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, in
   constexpr int kBits = 8 * (int)sizeof(K);
   const uint32_t scanTiles = div_up(std::max(n, 1u), kCountScanTile);
   const uint32_t numKBlocks = div_up(std::max(n, 1u), kKBlock);
-  enum { M_SPLIT = 0, M_EQPAIRS = 1, M_ROOTEQ = 3, M_ERR = 4, M_WORDS = 8 };
+  enum { M_SPLIT = 0, M_EQPAIRS = 1, M_FAR = 2, M_ROOTEQ = 3, M_ERR = 4, M_EQCURSOR = 5, M_WORDS = 8 };
   KNode *kn = nullptr;
   uint32_t *splitFlag = nullptr, *slotArr = nullptr, *eqPairs = nullptr, *eqBase = nullptr, *misc = nullptr;
   uint32_t *hist = nullptr, *bStart = nullptr, *bCursor = nullptr, *bucket = nullptr, *pos = nullptr;
+  uint32_t *splitList = nullptr, *dnKey = nullptr, *dnVal = nullptr, *dnKeyB = nullptr, *dnValB = nullptr;
   uint32_t *blockHist = nullptr;
   uint64_t *kStatus = nullptr;
   EqRoot *eqRoots = nullptr;
   RefitRange *topRanges = nullptr;
-  cubql_box3f *sortedBoxes = nullptr;
   ScratchArena hscr(mem, s);
   const size_t n1 = std::max(n, 1u);
   CUBQL_CHECK(hscr.reserve(&kn, n1));
@@ -798,29 +963,38 @@ int build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, in
   CUBQL_CHECK(hscr.reserve(&eqPairs, n1));
   CUBQL_CHECK(hscr.reserve(&eqBase, n1));
   CUBQL_CHECK(hscr.reserve(&pos, n1));
-  CUBQL_CHECK(hscr.reserve(&bucket, 2 * n1));  // split nodes + middle-split inner nodes, each < n
+  CUBQL_CHECK(hscr.reserve(&bucket, n1));  // middle-split inner nodes (< n)
+  CUBQL_CHECK(hscr.reserve(&splitList, n1));
+  CUBQL_CHECK(hscr.reserve(&dnKey, n1));
+  CUBQL_CHECK(hscr.reserve(&dnVal, n1));
+  CUBQL_CHECK(hscr.reserve(&dnKeyB, n1));
+  CUBQL_CHECK(hscr.reserve(&dnValB, n1));
   CUBQL_CHECK(hscr.reserve(&eqRoots, n1));
   CUBQL_CHECK(hscr.reserve(&blockHist, (size_t)kDeltaBins * numKBlocks));
-  CUBQL_CHECK(hscr.reserve(&kStatus, scanTiles));
+  CUBQL_CHECK(hscr.reserve(&kStatus, 2 * (size_t)scanTiles));
   CUBQL_CHECK(hscr.reserve(&misc, M_WORDS + 3 * kDeltaBins));
   CUBQL_CHECK(hscr.reserve(&topRanges, kRefitTopBuckets));
-  CUBQL_CHECK(hscr.reserve(&sortedBoxes, n1));
+  uint32_t *farList = nullptr;
+  CUBQL_CHECK(hscr.reserve(&farList, n1));
   if (hscr.commit() != CUBQL_OK) return fail(CUBQL_ERR_ALLOC);
   hist = misc + M_WORDS;
   bStart = hist + kDeltaBins;
   bCursor = bStart + kDeltaBins;
   if (fill_u32(misc, 0u, M_WORDS + 3 * kDeltaBins, s) != CUBQL_OK ||
-      fill_u32((uint32_t *)kStatus, 0u, 2 * (size_t)scanTiles, s) != CUBQL_OK)
+      fill_u32((uint32_t *)kStatus, 0u, 4 * (size_t)scanTiles, s) != CUBQL_OK)
     return fail(CUBQL_ERR_HIP);
   // the valid count is on the device only: the grids are sized by n and
   // the kernels stop at the count
   const uint32_t *nv = &c.ctr[C_NUMVALID];
   k_karras_info<K><<<1, 1, 0, s>>>(kRes, nv, T, misc);
-  k_karras<K><<<numKBlocks, kBlock, 0, s>>>(kRes, nv, T, kn, splitFlag, eqPairs, blockHist, numKBlocks);
-  k_gather_sorted<<<div_up((uint32_t)n1, kBlock), kBlock, 0, s>>>(boxes, primIDs, nv, sortedBoxes);
+  k_karras<K><<<numKBlocks, kKThreads, 0, s>>>(kRes, nv, T, kn, splitFlag, eqPairs, blockHist, numKBlocks, farList,
+                                               misc + M_FAR);
+  k_karras_far<K><<<1024, 256, 0, s>>>(kRes, nv, T, farList, misc + M_FAR, kn, splitFlag, eqPairs, blockHist,
+                                       numKBlocks);
   k_karras_binscan<<<kDeltaBins, 1024, 0, s>>>(blockHist, numKBlocks, hist);
-  k_karras_scan<<<scanTiles, 1024, 0, s>>>(splitFlag, slotArr, nv, kStatus, 1u, misc + M_SPLIT, misc + M_ERR);
-  k_karras_scan<<<scanTiles, 1024, 0, s>>>(eqPairs, eqBase, nv, kStatus, 2u, misc + M_EQPAIRS, misc + M_ERR);
+  k_karras_scan2<<<scanTiles, 1024, 0, s>>>(splitFlag, eqPairs, slotArr, eqBase, splitList, nv, kStatus,
+                                            kStatus + scanTiles,
+                                            misc + M_SPLIT, misc + M_EQPAIRS, misc + M_ERR);
   if (hipGetLastError() != hipSuccess) return fail((set_error("hierarchy launch failed"), CUBQL_ERR_HIP));
   std::vector<uint32_t> hm(M_WORDS + kDeltaBins);
   uint32_t numValid = 0, spinWord = 0;
@@ -852,13 +1026,21 @@ int build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, in
   const uint64_t nodes64 = numValid == 0 ? 0 : 2 + 2 * (uint64_t)numSplit + 2 * (uint64_t)eqInner;
   if (nodes64 > 0xffffffffull) return fail((set_error("radix build: node count overflows"), CUBQL_ERR_INTERNAL));
   const uint32_t numNodes = (uint32_t)nodes64;
+  // bucket starts: split-node buckets are ranges of the slots sorted by
+  // prefix length (ascending); middle-split buckets are ranges of `bucket`
   std::vector<uint32_t> hStart(kDeltaBins, 0);
-  for (int d = 0, run = 0; d < kDeltaBins; d++) {
-    hStart[d] = (uint32_t)run;
-    run += (int)hh[d];
+  for (int d = 0, runS = 0, runE = 0; d < kDeltaBins; d++) {
+    if (d < kBits) {
+      hStart[d] = (uint32_t)runS;
+      runS += (int)hh[d];
+    } else {
+      hStart[d] = (uint32_t)runE;
+      runE += (int)hh[d];
+    }
   }
-  // refit order: buckets from the last bin down; the narrow tail (every
-  // bucket after the last wide one) goes to one workgroup
+  // refit order: middle-split buckets deepest first, then split-node
+  // buckets by decreasing prefix length; the narrow tail (every bucket
+  // after the last wide one) goes to one workgroup
   constexpr uint32_t kRefitTopMax = 4096;
   std::vector<RefitRange> order;
   for (int d = kDeltaBins - 1; d >= 0; d--)
@@ -873,6 +1055,7 @@ int build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, in
     mem_free(mem, nodes, s);
     return fail(e);
   };
+  uint32_t *slotsByDn = dnVal;
   // node 1 stays unused (zero), as in the reference (radix.h:585-586)
   if (fill_u32((uint32_t *)(nodes + 1), 0u, sizeof(Node) / 4, s) != CUBQL_OK) return failN(CUBQL_ERR_HIP);
   if (numValid) {
@@ -886,14 +1069,21 @@ int build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, in
     } else if (rootEq) {
       k_karras_root_eq<<<1, 1, 0, s>>>(nv, eqRoots);
     } else {
-      k_karras_emit<K><<<numKBlocks, kBlock, 0, s>>>(kRes, sortedBoxes, numValid, T, kn, splitFlag, slotArr,
-                                                      eqBase, numSplit, bStart, blockHist, numKBlocks, bucket,
-                                                      pos, nodes, eqRoots);
+      k_karras_emit<K><<<div_up(numSplit, kBlock), kBlock, 0, s>>>(kRes, boxes, primIDs, T, kn, splitList, numSplit,
+                                                                   slotArr, eqPairs, eqBase, dnKey, dnVal, pos,
+                                                                   nodes, eqRoots, misc + M_EQCURSOR);
+      // split nodes into prefix-length order (one 8-bit pass; lengths < 64).
+      // No host check follows, so its look-back waits are the unbounded
+      // kind (every tile waited on is held by a running workgroup).
+      uint32_t *kOut = nullptr;
+      st = radix_sort_pairs_dev_u32(dnKey, dnVal, dnKeyB, dnValB, misc + M_SPLIT, numSplit, 8, s, mem, &kOut,
+                                    &slotsByDn, nullptr);
+      if (st) return failN(st);
     }
     if (numEqRoots) {
       // a few workgroups per subtree when there are few subtrees
       const uint32_t gx = std::min(numEqRoots, 16384u), gy = std::min(64u, std::max(1u, 1024u / numEqRoots));
-      k_karras_equal<<<dim3(gx, gy), 256, 0, s>>>(sortedBoxes, T, eqRoots, numEqRoots, kBits, bStart,
+      k_karras_equal<<<dim3(gx, gy), 256, 0, s>>>(boxes, primIDs, T, eqRoots, numEqRoots, kBits, bStart,
                                                   bCursor, bucket, nodes);
     }
     if (hipGetLastError() != hipSuccess) return failN((set_error("hierarchy emit launch failed"), CUBQL_ERR_HIP));
@@ -907,10 +1097,12 @@ int build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T, in
   int rs = prof_begin(s);
   if (numValid && !rootLeaf) {
     for (size_t r = 0; r < topFrom; r++)
-      k_karras_refit<<<div_up(order[r].count, kBlock), kBlock, 0, s>>>(bucket, order[r].begin, order[r].count,
+      k_karras_refit<<<div_up(order[r].count, kBlock), kBlock, 0, s>>>(order[r].slots ? slotsByDn : bucket,
+                                                                        order[r].begin, order[r].count,
                                                                         order[r].slots != 0, pos, nodes);
     if (order.size() > topFrom)
-      k_karras_refit_top<<<1, 1024, 0, s>>>(bucket, topRanges, (int)(order.size() - topFrom), pos, nodes);
+      k_karras_refit_top<<<1, 1024, 0, s>>>(slotsByDn, bucket, topRanges, (int)(order.size() - topFrom), pos,
+                                            nodes);
   }
   st = hipGetLastError() == hipSuccess ? CUBQL_OK : (set_error("refit launch failed"), CUBQL_ERR_HIP);
   prof_end(rs, s, "refit");

@@ -5,8 +5,8 @@
 // keys (uint32 or uint64) with uint32 values, ascending, stable.
 //
 // Design: 8-bit digits.  One histogram pass computes the digit counts of
-// every sort pass at once (per-tile histograms, summed without global
-// atomics).  Each sort pass is then a single read + single write of the
+// every sort pass at once (per-workgroup histograms, summed in chunks with
+// one atomic per digit and chunk).  Each sort pass is then a single read + single write of the
 // data: a workgroup claims a 4096-key tile (ticket order, so it only waits
 // on tiles that running workgroups hold), ranks keys per digit with wave64
 // ballots, resolves each digit's global offset by a decoupled look-back over

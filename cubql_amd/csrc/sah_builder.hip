@@ -113,7 +113,24 @@ struct Ctx {
   uint32_t *smallCount, *smallBase;
   uint32_t *levelItems;
   uint32_t maxLarge;
+  uint32_t maxChunks;      // 1024-segment chunks of the emit
+  uint64_t *emitStatus;    // [6][maxChunks] look-back words of the emit's counts
 };
+
+// The partition and binning tilings of a level's open segments: tile
+// offsets per segment and the totals, and the partition's tile tickets
+// cleared.  Written by the emit of the level before (k_count_scan for the
+// root).
+__device__ __forceinline__ void level_plan_totals(const Ctx &c, uint32_t level, uint32_t nSeg, uint32_t tiles,
+                                                  uint32_t btiles, uint32_t items) {
+  c.tileStart[nSeg] = tiles;
+  c.binStart[nSeg] = btiles;
+  c.ctr->numTiles = tiles;
+  c.ctr->numBinTiles = btiles;
+  c.ctr->ticket = 0;
+  for (uint32_t cc = 0; cc < kTicketCounters; cc++) c.tickets[kTicketStride * cc] = 0;
+  c.levelItems[level] = items;
+}
 
 __device__ __forceinline__ Item make_item(const cubql_box3f &b, uint32_t id) {
   Item it;
@@ -335,9 +352,15 @@ __global__ __launch_bounds__(1024) void k_count_scan(Ctx c, uint32_t numBlocks) 
       for (int j = 0; j < 6; j++) s.cb[j] = k->rootCb[j];
       c.segs[0][0] = s;
       k->numLarge[0] = 1;
-    } else if (run > 0) {
-      c.small[0] = SmallSeg{0, run, 0, 0};
-      k->numSmall = 1;
+      c.tileStart[0] = 0;
+      c.binStart[0] = 0;
+      level_plan_totals(c, 0u, 1u, div_up(run, kTile), div_up(run, kBinTile), run);
+    } else {
+      if (run > 0) {
+        c.small[0] = SmallSeg{0, run, 0, 0};
+        k->numSmall = 1;
+      }
+      level_plan_totals(c, 0u, 0u, 0u, 0u, 0u);
     }
   }
 }
@@ -373,53 +396,32 @@ __global__ __launch_bounds__(kBlock) void k_write_items(Ctx c) {
 }
 
 // ------------------------------------------------------------- level pass
-// Plan (single workgroup): tile offsets of both tilings, child-bound reset.
-__global__ __launch_bounds__(1024) void k_plan(Ctx c, int level) {
-  __shared__ uint32_t ss[1024 / kWave + 1];
-  const int par = level & 1;
-  const uint32_t n = c.ctr->numLarge[par];
-  const Seg *segs = c.segs[par];
-  uint32_t run = 0, runB = 0, items = 0;
-  for (uint32_t b0 = 0; b0 < n; b0 += 1024) {
-    const uint32_t s = b0 + threadIdx.x;
-    uint32_t tiles = 0, btiles = 0, size = 0;
-    if (s < n) {
-      const Seg g = segs[s];
-      size = g.end - g.begin;
-      tiles = div_up(size, kTile);
-      btiles = div_up(size, kBinTile);
-      c.segLeft[s] = 0;
-      for (int j = 0; j < 3; j++) {
-        for (uint32_t cp = 0; cp < kCbCopies; cp++) {
-          int32_t *cb = c.childCb + (size_t)cp * 12 * c.maxLarge;
-          cb[(2 * s) * 6 + j] = enc_f(INFINITY);
-          cb[(2 * s) * 6 + j + 3] = enc_f(-INFINITY);
-          cb[(2 * s + 1) * 6 + j] = enc_f(INFINITY);
-          cb[(2 * s + 1) * 6 + j + 3] = enc_f(-INFINITY);
-        }
-      }
-    }
-    uint32_t tot, totB, totItems;
-    const uint32_t ex = block_excl_scan<1024>(tiles, ss, &tot);
-    const uint32_t exB = block_excl_scan<1024>(btiles, ss, &totB);
-    block_excl_scan<1024>(size, ss, &totItems);
-    if (s < n) {
-      c.tileStart[s] = run + ex;
-      c.binStart[s] = runB + exB;
-    }
-    run += tot;
-    runB += totB;
-    items += totItems;
+// The emit runs one 1024-thread workgroup per 1024 open segments (a chunk;
+// chunk ch is workgroup ch, so every chunk waited on is held by a workgroup
+// the dispatcher started earlier): K counts per segment, block scans, and
+// the chunks' offsets by decoupled look-back, one wave per count at once.
+// The status words carry the level as their tag, so they are cleared once
+// per build.  The chunk's exclusive offsets land in pre[] (LDS).
+template <int K>
+__device__ __forceinline__ void chunk_scan(const uint32_t (&v)[K], uint32_t (&ex)[K], uint32_t (&tot)[K],
+                                           uint64_t *status, uint32_t maxChunks, uint32_t ch, uint32_t tag,
+                                           uint32_t *err, uint32_t *smem, uint32_t *pre) {
+  block_excl_scan_multi<1024, K>(v, ex, tot, smem);
+  const uint32_t w = threadIdx.x / kWave;
+  if (w < (uint32_t)K) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int q = 0; q < K; q++) t = (uint32_t)q == w ? tot[q] : t;
+    const uint32_t p = lookback_exclusive_wave(status + (size_t)w * maxChunks, ch, ch, tag, t, false, err);
+    if (lane_id() == 0) pre[w] = p;
   }
-  if (threadIdx.x == 0) {
-    c.tileStart[n] = run;
-    c.binStart[n] = runB;
-    c.ctr->numTiles = run;
-    c.ctr->numBinTiles = runB;
-    c.ctr->ticket = 0;
-    for (uint32_t cc = 0; cc < kTicketCounters; cc++) c.tickets[kTicketStride * cc] = 0;
-    c.levelItems[level] = items;
-  }
+  __syncthreads();
+}
+
+// identity of the child-bound accumulators (min words, then max words)
+__global__ void k_cb_init(int32_t *cb, size_t nWords) {
+  for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < nWords; w += (size_t)gridDim.x * blockDim.x)
+    cb[w] = (w % 6) < 3 ? enc_f(INFINITY) : enc_f(-INFINITY);
 }
 
 __device__ __forceinline__ uint32_t find_seg(const uint32_t *start, uint32_t nSeg, uint32_t t) {
@@ -598,7 +600,10 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
       const int j = threadIdx.x % 6;
       const bool nonEmpty = side == 0 ? (nLeft > 0) : (nLeft < te - tb);
       if (nonEmpty) {
-        int32_t *dcb = &c.childCb[((size_t)(blockIdx.x % kCbCopies) * 2 * c.maxLarge + 2 * s + side) * 6 + j];
+        // (copy t mod the segment's copy count: copies spread a wide
+        // segment's atomics, a narrow one uses only as many as its tiles)
+        const uint32_t nCopies = min((uint32_t)kCbCopies, nTilesSeg);
+        int32_t *dcb = &c.childCb[((size_t)(t % nCopies) * 2 * c.maxLarge + 2 * s + side) * 6 + j];
         if (j < 3) atomicMin(dcb, enc_f(v[threadIdx.x]));
         else atomicMax(dcb, enc_f(v[threadIdx.x]));
       }
@@ -607,80 +612,99 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ctx c, int level) {
   }
 }
 
-// Emit (single workgroup): child ids for split nodes in segment order,
-// next-level large segments and small segments.
+// Emit: child ids for split nodes in segment order, next-level large
+// segments with their tilings (the next level's plan) and small segments,
+// offsets by the chunk scan; the child-bound accumulators it reads are reset
+// for the next level.  Every chunk reads the node and small-segment counts
+// before it publishes its look-back words, and the last chunk, whose
+// look-back waits for all of those, updates them.
 __global__ __launch_bounds__(1024) void k_emit(Ctx c, int level) {
-  __shared__ uint32_t ss[1024 / kWave + 1];
-  __shared__ uint32_t sBase, sSmall;
+  constexpr int K = 6;  // split, large children, small children, tiles, binning tiles, items
+  __shared__ uint32_t smem[K * (1024 / kWave + 1)];
+  __shared__ uint32_t sPre[K];
   const int par = level & 1;
   const uint32_t n = c.ctr->numLarge[par];
+  const uint32_t nChunks = n ? div_up(n, 1024u) : 1u;
+  const uint32_t ch = blockIdx.x;
+  if (ch >= nChunks) return;
+  const uint32_t nodeBase = c.ctr->nodeCount, smallBase = c.ctr->numSmall;
   const Seg *segs = c.segs[par];
   Seg *next = c.segs[par ^ 1];
-  if (threadIdx.x == 0) {
-    sBase = c.ctr->nodeCount;
-    sSmall = c.ctr->numSmall;
-  }
-  __syncthreads();
-  const uint32_t nodeBase = sBase, smallBase = sSmall;
-  uint32_t runSplit = 0, runLarge = 0, runSmall = 0;
-  for (uint32_t b0 = 0; b0 < n; b0 += 1024) {
-    const uint32_t s = b0 + threadIdx.x;
-    uint32_t isSplit = 0, nLarge = 0, nSmallC = 0;
-    Seg g;
-    uint32_t cb[2], ce[2];
-    bool large[2] = {false, false};
-    if (s < n) {
-      g = segs[s];
-      if (c.segMode[s] != kModeLeaf) {
-        isSplit = 1;
-        const uint32_t mid = g.begin + c.segLeft[s];
-        cb[0] = g.begin; ce[0] = mid;
-        cb[1] = mid; ce[1] = g.end;
-        for (int side = 0; side < 2; side++) {
-          const uint32_t sz = ce[side] - cb[side];
-          large[side] = sz > (uint32_t)kSmall && sz > c.T;
-          if (large[side]) nLarge++;
-          else nSmallC++;
-        }
-      }
-    }
-    uint32_t totSp, totL, totS;
-    const uint32_t exSp = block_excl_scan<1024>(isSplit, ss, &totSp);
-    const uint32_t exL = block_excl_scan<1024>(nLarge, ss, &totL);
-    const uint32_t exS = block_excl_scan<1024>(nSmallC, ss, &totS);
-    if (isSplit) {
-      const uint32_t child = nodeBase + 2 * (runSplit + exSp);
-      c.nodes[g.node].oc = pack_oc(child, 0);
-      uint32_t li = runLarge + exL, si = runSmall + exS;
+  const uint32_t s = ch * 1024 + threadIdx.x;
+  uint32_t v[K] = {0u, 0u, 0u, 0u, 0u, 0u};
+  Seg g;
+  uint32_t cb[2], ce[2];
+  bool large[2] = {false, false};
+  int32_t ccb[2][6];
+  if (s < n) {
+    g = segs[s];
+    if (c.segMode[s] != kModeLeaf) {
+      v[0] = 1;
+      const uint32_t mid = g.begin + c.segLeft[s];
+      cb[0] = g.begin; ce[0] = mid;
+      cb[1] = mid; ce[1] = g.end;
       for (int side = 0; side < 2; side++) {
+        const uint32_t sz = ce[side] - cb[side];
+        large[side] = sz > (uint32_t)kSmall && sz > c.T;
         if (large[side]) {
-          Seg ns;
-          ns.begin = cb[side]; ns.end = ce[side]; ns.node = child + side; ns.pad = 0;
-          ns.pad2[0] = ns.pad2[1] = 0;
-          for (int j = 0; j < 6; j++) {
-            int32_t v = c.childCb[(2 * s + side) * 6 + j];
-            for (uint32_t cp = 1; cp < kCbCopies; cp++) {
-              const int32_t w = c.childCb[((size_t)cp * 2 * c.maxLarge + 2 * s + side) * 6 + j];
-              v = j < 3 ? min(v, w) : max(v, w);
-            }
-            ns.cb[j] = v;
-          }
-          next[li++] = ns;
+          v[1]++;
+          v[3] += div_up(sz, kTile);
+          v[4] += div_up(sz, kBinTile);
+          v[5] += sz;
         } else {
-          c.small[smallBase + si] = SmallSeg{cb[side], ce[side], child + side, (uint32_t)(par ^ 1)};
-          si++;
+          v[2]++;
         }
       }
     }
-    runSplit += totSp;
-    runLarge += totL;
-    runSmall += totS;
+    // child bounds (the union over the copies the partition used: one per
+    // tile up to kCbCopies), and those copies reset to the identity for
+    // the next level
+    const uint32_t nCopies = min((uint32_t)kCbCopies, div_up(g.end - g.begin, kTile));
+    for (int side = 0; side < 2; side++)
+      for (int j = 0; j < 6; j++) {
+        int32_t w = j < 3 ? enc_f(INFINITY) : enc_f(-INFINITY);
+        const int32_t idw = w;
+        for (uint32_t cp = 0; cp < nCopies; cp++) {
+          int32_t *p = &c.childCb[((size_t)cp * 2 * c.maxLarge + 2 * s + side) * 6 + j];
+          const int32_t u = *p;
+          w = j < 3 ? min(w, u) : max(w, u);
+          *p = idw;
+        }
+        ccb[side][j] = w;
+      }
   }
-  if (threadIdx.x == 0) {
-    c.ctr->nodeCount = nodeBase + 2 * runSplit;
-    c.ctr->numSmall = smallBase + runSmall;
-    c.ctr->numLarge[par ^ 1] = runLarge;
+  uint32_t ex[K], tot[K];
+  chunk_scan<K>(v, ex, tot, c.emitStatus, c.maxChunks, ch, (uint32_t)level + 1, &c.ctr->spinFault, smem, sPre);
+  if (v[0]) {
+    const uint32_t child = nodeBase + 2 * (sPre[0] + ex[0]);
+    c.nodes[g.node].oc = pack_oc(child, 0);
+    uint32_t li = sPre[1] + ex[1], si = sPre[2] + ex[2];
+    uint32_t ti = sPre[3] + ex[3], bi = sPre[4] + ex[4];
+    for (int side = 0; side < 2; side++) {
+      if (large[side]) {
+        Seg ns;
+        ns.begin = cb[side]; ns.end = ce[side]; ns.node = child + side; ns.pad = 0;
+        ns.pad2[0] = ns.pad2[1] = 0;
+        for (int j = 0; j < 6; j++) ns.cb[j] = ccb[side][j];
+        next[li] = ns;
+        c.tileStart[li] = ti;
+        c.binStart[li] = bi;
+        ti += div_up(ce[side] - cb[side], kTile);
+        bi += div_up(ce[side] - cb[side], kBinTile);
+        li++;
+      } else {
+        c.small[smallBase + si] = SmallSeg{cb[side], ce[side], child + side, (uint32_t)(par ^ 1)};
+        si++;
+      }
+    }
+  }
+  if (ch == nChunks - 1 && threadIdx.x == 0) {
+    c.ctr->nodeCount = nodeBase + 2 * (sPre[0] + tot[0]);
+    c.ctr->numSmall = smallBase + sPre[2] + tot[2];
+    const uint32_t nNext = sPre[1] + tot[1];
+    c.ctr->numLarge[par ^ 1] = nNext;
     c.ctr->numLarge[par] = 0;
+    level_plan_totals(c, (uint32_t)level + 1, nNext, sPre[3] + tot[3], sPre[4] + tot[4], sPre[5] + tot[5]);
   }
 }
 
@@ -893,6 +917,8 @@ int sah_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T
   CUBQL_CHECK(scr.reserve(&c.smallCount, (size_t)n + 1));
   CUBQL_CHECK(scr.reserve(&c.smallBase, (size_t)n + 1));
   CUBQL_CHECK(scr.reserve(&c.levelItems, maxLevels + 2));
+  c.maxChunks = div_up(c.maxLarge, 1024u);
+  CUBQL_CHECK(scr.reserve(&c.emitStatus, 6 * (size_t)c.maxChunks));
   CUBQL_CHECK(scr.commit());
 
   uint32_t *primIDs = nullptr;
@@ -903,7 +929,9 @@ int sah_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T
     return st;
   };
   int st = fill_u32((uint32_t *)c.tileStatus, 0u, 2 * (size_t)maxTiles, s);
+  if (!st) st = fill_u32((uint32_t *)c.emitStatus, 0u, 12 * (size_t)c.maxChunks, s);
   if (st) return fail(st);
+  k_cb_init<<<256, 256, 0, s>>>(c.childCb, 12 * (size_t)c.maxLarge * kCbCopies);
   if (n) {
     k_count<<<numBlocks, kBlock, 0, s>>>(c);
     k_count_scan<<<1, 1024, 0, s>>>(c, numBlocks);
@@ -928,7 +956,6 @@ int sah_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T
   std::vector<std::pair<int, int>> binSlots, partSlots;
   while (n) {
     for (int i = 0; i < kLevelsPerSync && level < (int)maxLevels; i++, level++) {
-      k_plan<<<1, 1024, 0, s>>>(c, level);
       int pb = prof_begin(s);
       k_bin<<<binGrid, kBlock, 0, s>>>(c, level);
       prof_end(pb, s, "sah_bin");
@@ -938,7 +965,7 @@ int sah_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T
       k_partition<<<partGrid, kBlock, 0, s>>>(c, level);
       prof_end(pp, s, "sah_partition");
       if (pp >= 0) partSlots.push_back({pp, level});
-      k_emit<<<1, 1024, 0, s>>>(c, level);
+      k_emit<<<c.maxChunks, 1024, 0, s>>>(c, level);
     }
     if (hipGetLastError() != hipSuccess) return fail((set_error("sah level launch failed"), CUBQL_ERR_HIP));
     if (hipMemcpyAsync(hw, c.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess ||

@@ -176,7 +176,8 @@ def bench_batched(cq, dev, num_clouds: int, plain_budget_s: float, tune: bool):
     torch.cuda.synchronize()
     maxc = int(max(ss.max(), ts.max()))
     if tune:
-        cands = [(T, b, st) for T in (4, 8, 16) for b in (128, 256) for st in (32, 64)]
+        # stack depths 8 and 16 run the traversal stack in LDS, 64 in scratch
+        cands = [(T, b, st) for T in (4, 8, 16) for b in (128, 256) for st in (8, 16, 64)]
         params, times = tune_batch(src, so, tgt, to, cands, reps=2, max_cloud_size=maxc)
     else:
         params, times = cq.BatchParams(8, 256, 64, maxc), {}

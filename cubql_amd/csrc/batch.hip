@@ -221,9 +221,11 @@ __global__ void k_brefit_level(Side sd, Temp tp, int level) {
   }
 }
 
-template <int STACK>
-__device__ __forceinline__ bool fcp_forest(const Side &tgt, uint32_t root, float3 q, int32_t &best,
-                                           float &bestD) {
+// The closest point of one query in the tree at `root`, on a caller's
+// traversal stack (a private array, or LDS for the shallow stacks).
+template <typename StackT>
+__device__ __forceinline__ bool fcp_forest_on(StackT &stack, const Side &tgt, uint32_t root, float3 q,
+                                              int32_t &best, float &bestD) {
   best = -1;
   bestD = INFINITY;
   cubql_bvh3f bvh;
@@ -243,8 +245,15 @@ __device__ __forceinline__ bool fcp_forest(const Side &tgt, uint32_t root, float
   };
   // seed the radius with the nearest-child leaf (visited again below,
   // changing nothing)
-  cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim, root);
-  return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, bestD, prim, root);
+  cubql_dev::NoStats st;
+  cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim, root, st);
+  return cubql_dev::for_each_prim_on(stack, bvh, q.x, q.y, q.z, bestD, prim, root, st);
+}
+template <int STACK>
+__device__ __forceinline__ bool fcp_forest(const Side &tgt, uint32_t root, float3 q, int32_t &best,
+                                           float &bestD) {
+  cubql_dev::PrivateStack<STACK> stack;
+  return fcp_forest_on(stack, tgt, root, q, best, bestD);
 }
 
 // queries of side `qs` against the forest of side `ts`, in Morton order of qs
@@ -260,7 +269,17 @@ __global__ __launch_bounds__(BLOCK) void k_bquery(Side qs, Side ts, int32_t *nn,
   float bestD = INFINITY;
   if (root != kNoRoot) {
     const float3 q = make_float3(qs.pts[3 * (size_t)qi], qs.pts[3 * (size_t)qi + 1], qs.pts[3 * (size_t)qi + 2]);
-    if (!fcp_forest<STACK>(ts, root, q, best, bestD)) {
+    bool done;
+    if constexpr (STACK <= 16) {
+      // shallow stacks live in LDS (at most 32 KB a workgroup), deeper
+      // ones in the lane's private memory
+      __shared__ uint2 sStack[STACK * BLOCK];
+      cubql_dev::LdsStack<STACK, BLOCK> stack(sStack + threadIdx.x);
+      done = fcp_forest_on(stack, ts, root, q, best, bestD);
+    } else {
+      done = fcp_forest<STACK>(ts, root, q, best, bestD);
+    }
+    if (!done) {
       ovf[2 + atomicAdd(&ovf[0], 1u)] = t;
       return;
     }

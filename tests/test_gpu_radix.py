@@ -65,6 +65,13 @@ def _inputs(case):
         # one key for every prim: the whole tree is one middle-split subtree
         P = np.tile(rng.random((1, 3), dtype=np.float32), (300_000, 1))
         return np.concatenate([P, P], 1), None
+    if case == "manydupes":
+        # 20K distinct positions, 10 prims each: thousands of equal-key
+        # subtrees hanging under split nodes
+        base = rng.random((20_000, 3), dtype=np.float32)
+        P = base[np.repeat(np.arange(20_000), 10)]
+        P = P[rng.permutation(len(P))]
+        return np.concatenate([P, P], 1), None
     if case == "clustered":
         P = np.concatenate([rng.normal(0, 1e-4, (5000, 3)), rng.normal(5, 1, (5000, 3)),
                             rng.normal(-1e5, 1, (30, 3))]).astype(np.float32)
@@ -79,7 +86,7 @@ def _inputs(case):
 
 
 @pytest.mark.parametrize("bits", [10, 21])
-@pytest.mark.parametrize("case", ["uniform", "dupes", "identical", "clustered", "boxes_invalid"])
+@pytest.mark.parametrize("case", ["uniform", "dupes", "manydupes", "identical", "clustered", "boxes_invalid"])
 @pytest.mark.parametrize("T", [1, 8])
 def test_radix_builder_equals_oracle(cq, oracle_mod, bits, case, T):
     print(f"[gpu] radix bits={bits} case={case} T={T}", flush=True)

@@ -190,6 +190,177 @@ __device__ __forceinline__ bool knn_one(const BVH &bvh, const float *points, flo
   return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, sqrMax, prim, 0u, st);
 }
 
+// Far siblings of the nearest leaf's last kSeedSibs ancestors whose nearest
+// leaves seed a kNN query over a binary tree.
+constexpr int kSeedSibs = 4;
+
+// kNN over a binary tree on a caller-provided (short) stack.  Before the
+// traversal proper, the query visits the nearest leaf (descending along the
+// nearer child, ties to the second) and then, while its list is not full,
+// the nearest leaf under each far sibling met on the last kSeedSibs steps of
+// that descent (deepest first).  The traversal then starts at the root with
+// the list's radius (the k-th distance once the list is full), so that on
+// the way down it stacks only far children that can still hold a candidate,
+// and it skips the seeded leaves, whose prims are already in the list.  The
+// result is knn_one's: the list holds the k smallest (distance, primID)
+// pairs under sqrMax whatever the visiting order, and every leaf within the
+// radius is still visited exactly once.  One loop visits every leaf, so the
+// insertion code is inlined once.
+template <int K, typename StackT, typename Stats = cubql_dev::NoStats>
+__device__ __forceinline__ bool knn_seeded_on(StackT &stack, const cubql_bvh3f &bvh, const float *points,
+                                              float3 q, int k, float sqrMax, int32_t (&ids)[K], float (&ds)[K],
+                                              int &found, Stats &&st = Stats()) {
+  using namespace cubql_dev;
+  found = 0;
+#pragma unroll
+  for (int j = 0; j < K; j++) { ids[j] = -1; ds[j] = INFINITY; }
+  float cull = sqrMax;
+  float kthD = INFINITY;
+  int32_t kthId = 0x7fffffff;
+  // far siblings of the first descent (sib[0] the deepest) and the offsets
+  // of the seeded leaves (a leaf's first primID slot names it), the latest
+  // in seeded[0]; every index is static, so both stay in registers
+  uint32_t sib[kSeedSibs], seeded[kSeedSibs + 1];
+#pragma unroll
+  for (int j = 0; j < kSeedSibs; j++) sib[j] = 0u;
+#pragma unroll
+  for (int j = 0; j <= kSeedSibs; j++) seeded[j] = 0xffffffffu;
+  int nsib = 0;
+  bool trav = false, record = true, visit = true;
+  int sp = 0;
+  uint64_t node = node_oc(load_node(bvh.nodes, 0));
+  while (true) {
+    if (!trav) {
+      // seeding descent to the nearest leaf (the first one records the far
+      // siblings)
+      while (oc_cnt(node) == 0) {
+        const uint32_t i0 = oc_off(node);
+        const QNode n0 = load_node(bvh.nodes, i0);
+        const QNode n1 = load_node(bvh.nodes, i0 + 1);
+        st.nodes(64);
+        const bool near0 = sqr_dist_box(q.x, q.y, q.z, n0) < sqr_dist_box(q.x, q.y, q.z, n1);
+        node = near0 ? node_oc(n0) : node_oc(n1);
+        if (record) {
+#pragma unroll
+          for (int j = kSeedSibs - 1; j > 0; j--) sib[j] = sib[j - 1];
+          sib[0] = near0 ? i0 + 1 : i0;
+          nsib = min(nsib + 1, kSeedSibs);
+        }
+      }
+      record = false;
+    }
+    if (visit) {
+      const uint32_t off = oc_off(node), cnt = oc_cnt(node);
+      bool skip = false;
+      if (trav) {
+#pragma unroll
+        for (int j = 0; j <= kSeedSibs; j++) skip |= off == seeded[j];
+      } else {
+#pragma unroll
+        for (int j = kSeedSibs; j > 0; j--) seeded[j] = seeded[j - 1];
+        seeded[0] = off;
+      }
+      if (!skip)
+        for (uint32_t c = 0; c < cnt; c++) {
+          const uint32_t pid = bvh.primIDs[off + c];
+          st.prim();
+          const float3 p = load_pt(points, pid);
+          const float d = sqr_dist_pt(p.x, p.y, p.z, q.x, q.y, q.z);
+          const int32_t id = (int32_t)pid;
+          if (!(d < sqrMax)) continue;
+          if (found == k) {
+            // reject unless (d,id) < (ds[k-1], ids[k-1])
+            if (d > kthD || (d == kthD && id > kthId)) continue;
+          } else {
+            found++;
+          }
+          int32_t carryI = id;
+          float carryD = d;
+#pragma unroll
+          for (int j = 0; j < K; j++) {
+            if (j < found) {
+              const bool greater = ds[j] > carryD || (ds[j] == carryD && ids[j] > carryI);
+              if (greater) {
+                const float td = ds[j];
+                const int32_t ti = ids[j];
+                ds[j] = carryD;
+                ids[j] = carryI;
+                carryD = td;
+                carryI = ti;
+              }
+            }
+          }
+          if (found == k) {
+#pragma unroll
+            for (int j = 0; j < K; j++)
+              if (j == k - 1) {
+                kthD = ds[j];
+                kthId = ids[j];
+              }
+            cull = kthD;
+          }
+        }
+    }
+    if (!trav) {
+      if (nsib > 0 && found < k) {
+        // next seed: the nearest leaf under the deepest unused sibling
+        const uint32_t from = sib[0];
+#pragma unroll
+        for (int j = 0; j + 1 < kSeedSibs; j++) sib[j] = sib[j + 1];
+        nsib--;
+        node = node_oc(load_node(bvh.nodes, from));
+        st.nodes(32);
+        visit = true;
+        continue;
+      }
+      trav = true;
+      node = node_oc(load_node(bvh.nodes, 0));
+    } else {
+      bool popped = false;
+      while (sp > 0) {
+        const uint2 e = stack.get(--sp);
+        if (__uint_as_float(e.y) <= cull) {
+          node = node_oc(load_node(bvh.nodes, e.x));
+          st.nodes(32);
+          popped = true;
+          break;
+        }
+      }
+      if (!popped) return true;
+    }
+    // traversal descent: near child first, far child stacked if within the
+    // radius (the rules of for_each_leaf_on)
+    visit = true;
+    while (oc_cnt(node) == 0) {
+      const uint32_t i0 = oc_off(node);
+      const QNode n0 = load_node(bvh.nodes, i0);
+      const QNode n1 = load_node(bvh.nodes, i0 + 1);
+      st.nodes(64);
+      const float d0 = sqr_dist_box(q.x, q.y, q.z, n0);
+      const float d1 = sqr_dist_box(q.x, q.y, q.z, n1);
+      if (fminf(d0, d1) > cull) { visit = false; break; }
+      const bool near0 = d0 < d1;
+      node = near0 ? node_oc(n0) : node_oc(n1);
+      const float dfar = fmaxf(d0, d1);
+      if (dfar <= cull) {
+        if (sp >= StackT::kCap) return false;
+        stack.put(sp++, make_uint2(near0 ? i0 + 1 : i0, __float_as_uint(dfar)));
+      }
+    }
+  }
+}
+
+// kNN lists of up to kKnnSeededMax entries over a binary tree take the
+// seeded traversal on an LDS stack of kKnnLdsStack entries per lane (32 KB a
+// workgroup; deeper queries rerun in the deep kernel).  Longer lists keep
+// the unseeded traversal on a private stack: on the clustered Gaussian-
+// mixture workload (2M points, k = 16) the LDS versions were slower (16
+// entries: 264 Mq/s, 24 entries: 224 Mq/s, against 293 Mq/s), the queries
+// that overflow the LDS stack being the slowest ones, which the deep rerun
+// then no longer overlaps with the others.
+constexpr int kKnnSeededMax = 8;
+constexpr int kKnnLdsStack = 16;
+
 template <int K, typename BVH>
 __global__ __launch_bounds__(kBlock) void k_knn(BVH bvh, const float *points,
                                                 const float *queries, uint32_t nq, int k,
@@ -204,7 +375,15 @@ __global__ __launch_bounds__(kBlock) void k_knn(BVH bvh, const float *points,
   int32_t ids[K];
   float ds[K];
   int found;
-  if (!knn_one<K, kFastStack>(bvh, points, q, k, sqrMax, ids, ds, found)) {
+  bool done;
+  if constexpr (std::is_same<BVH, cubql_bvh3f>::value && K <= kKnnSeededMax) {
+    __shared__ uint2 sStack[kKnnLdsStack * kBlock];
+    cubql_dev::LdsStack<kKnnLdsStack, kBlock> stack(sStack + threadIdx.x);
+    done = knn_seeded_on<K>(stack, bvh, points, q, k, sqrMax, ids, ds, found);
+  } else {
+    done = knn_one<K, kFastStack>(bvh, points, q, k, sqrMax, ids, ds, found);
+  }
+  if (!done) {
     uint32_t slot = atomicAdd(&ovf->count, 1u);
     ovfList[slot] = qi;
     return;
@@ -231,7 +410,14 @@ __global__ __launch_bounds__(64) void k_knn_deep(BVH bvh, const float *points,
     int32_t ids[K];
     float ds[K];
     int found;
-    if (!knn_one<K, kDeepStack>(bvh, points, q, k, sqrMax, ids, ds, found)) {
+    bool done;
+    if constexpr (std::is_same<BVH, cubql_bvh3f>::value && K <= kKnnSeededMax) {
+      cubql_dev::PrivateStack<kDeepStack> stack;
+      done = knn_seeded_on<K>(stack, bvh, points, q, k, sqrMax, ids, ds, found);
+    } else {
+      done = knn_one<K, kDeepStack>(bvh, points, q, k, sqrMax, ids, ds, found);
+    }
+    if (!done) {
       // deeper than the deep stack: marked incomplete, not "none found"
       atomicAdd(&ovf->deepFailed, 1u);
       found = -1;
@@ -266,7 +452,9 @@ int launch_knn(const BVH &bvh, const float *points, const float *queries, uint32
                                                   ovf, ovfList, order);
   prof_end(ps, s, "knn_points");
   prof_set(ps, (12.0 + 8.0 * k + 4.0) * nq, nq);
-  k_knn_deep<K, BVH><<<64, 64, 0, s>>>(bvh, points, queries, k, sqrMax, ids, d, found, ovf, ovfList);
+  // one workgroup per CU: the overflow list holds a few percent of the
+  // queries on clustered input
+  k_knn_deep<K, BVH><<<256, 64, 0, s>>>(bvh, points, queries, k, sqrMax, ids, d, found, ovf, ovfList);
   CUBQL_LAUNCH_CHECK();
   return CUBQL_OK;
 }
@@ -577,7 +765,13 @@ __global__ __launch_bounds__(64) void k_points_stats(cubql_bvh3f bvh, const floa
       int32_t ids[K];
       float ds[K];
       int found;
-      knn_one<K, kDeepStack>(bvh, points, q, k, sqrMax, ids, ds, found, st);
+      // the traversal k_knn runs, on the deep stack
+      if constexpr (K <= kKnnSeededMax) {
+        cubql_dev::PrivateStack<kDeepStack> stack;
+        knn_seeded_on<K>(stack, bvh, points, q, k, sqrMax, ids, ds, found, st);
+      } else {
+        knn_one<K, kDeepStack>(bvh, points, q, k, sqrMax, ids, ds, found, st);
+      }
     }
   }
   stats_flush(st, nq, out);

@@ -12,7 +12,7 @@ Cases: uniform points (no rebinning), tight clusters beside far outliers
 (broad-phase rebins), exact duplicates (position keys), ranges around the
 32-prim thread-phase limit, invalid boxes, and the C4 workload: 2,000,000
 points of a 64-component Gaussian mixture with 1% outliers at 1e6, kNN k=16
-on a sample against the oracle, and the mean-kNN-distance backward against
+and k=8 on a sample against the oracle, and the mean-kNN-distance backward against
 the FP64 closed form (rule in tests/gradtol.py).
 """
 import numpy as np
@@ -115,6 +115,14 @@ def test_gpu_rebin_c4_workload(cq, oracle_mod):
     # closed form at the kernel's ids (tolerance rule in tests/gradtol.py)
     check_point_grads(oracle_mod, P, Q, ids_h, np.full((len(Q), 16), 1.0 / (len(Q) * 16)),
                       gq.cpu().numpy(), gp.cpu().numpy(), label="C4 kNN16 mean")
+    # k = 8 takes the seeded traversal on the LDS stack; the outlier queries
+    # overflow it and rerun in the deep kernel
+    ids8, d28, nf8 = cq.points.findKNN(bvh, dP, dQ, 8)
+    torch.cuda.synchronize()
+    oi8, od8, on8 = oracle_mod.knn(tree, P, Q[sel], 8)
+    assert np.array_equal(ids8.cpu().numpy()[sel], oi8)
+    assert np.array_equal(d28.cpu().numpy()[sel], od8)
+    assert np.array_equal(nf8.cpu().numpy()[sel], on8)
     bvh.free()
 
 

@@ -80,7 +80,8 @@ __device__ __forceinline__ bool fcp_one_on(StackT &stack, const cubql_bvh3f &bvh
 // rerun in the deep kernel.
 constexpr int kLdsStack = 8;
 
-// fast path: 64-entry stack; queries that overflow go to the deep kernel.
+// fast path: an 8-entry LDS stack over a binary tree, a 64-entry private
+// stack over a wide one; queries that overflow go to the deep kernel.
 // BVH: cubql_bvh3f (binary) or cubql_dev::WideBVH<W>.
 template <typename BVH>
 __global__ __launch_bounds__(kBlock) void k_fcp(BVH bvh, const float *points,

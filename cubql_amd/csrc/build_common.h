@@ -153,6 +153,31 @@ __device__ inline uint32_t lookback_exclusive_wave(uint64_t *status, uint32_t t,
 // total to *total.  smem: 1024/64 + 2 words.
 constexpr uint32_t kCountScanPer = 8;
 constexpr uint32_t kCountScanTile = 1024 * kCountScanPer;
+// A thread's kCountScanPer consecutive entries [i0, i0 + 8) of an array of
+// n: two 16-byte words when all are in range and aligned, else one by one
+// (absent entries read as 0, and are not stored).
+__device__ __forceinline__ void load_per(const uint32_t *p, uint32_t i0, uint32_t n, uint32_t (&v)[kCountScanPer]) {
+  static_assert(kCountScanPer == 8, "two 16-byte words");
+  if (i0 + kCountScanPer <= n && ((uintptr_t)(p + i0) & 15u) == 0) {
+    const uint4 x = reinterpret_cast<const uint4 *>(p + i0)[0];
+    const uint4 y = reinterpret_cast<const uint4 *>(p + i0)[1];
+    v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+    v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < kCountScanPer; j++) v[j] = i0 + j < n ? p[i0 + j] : 0u;
+  }
+}
+__device__ __forceinline__ void store_per(uint32_t *p, uint32_t i0, uint32_t n, const uint32_t (&v)[kCountScanPer]) {
+  if (i0 + kCountScanPer <= n && ((uintptr_t)(p + i0) & 15u) == 0) {
+    reinterpret_cast<uint4 *>(p + i0)[0] = make_uint4(v[0], v[1], v[2], v[3]);
+    reinterpret_cast<uint4 *>(p + i0)[1] = make_uint4(v[4], v[5], v[6], v[7]);
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < kCountScanPer; j++)
+      if (i0 + j < n) p[i0 + j] = v[j];
+  }
+}
 // One BLOCK-thread workgroup's part (tile blk of BLOCK*kCountScanPer
 // entries) of such a scan; returns the total over all tiles to the last
 // tile's thread 0 through *total when total is non-null.
@@ -162,9 +187,10 @@ __device__ inline void count_scan_part(const uint32_t *count, uint32_t *out, uin
                                        uint32_t *smem, uint32_t sub, uint32_t *err = nullptr) {
   const uint32_t i0 = blk * (BLOCK * kCountScanPer) + threadIdx.x * kCountScanPer;
   uint32_t v[kCountScanPer], sum = 0;
+  load_per(count, i0, n, v);
 #pragma unroll
   for (uint32_t j = 0; j < kCountScanPer; j++) {
-    v[j] = i0 + j < n ? count[i0 + j] - sub : 0u;
+    v[j] = i0 + j < n ? v[j] - sub : 0u;
     sum += v[j];
   }
   uint32_t tot;
@@ -178,11 +204,13 @@ __device__ inline void count_scan_part(const uint32_t *count, uint32_t *out, uin
   }
   sync_block_lds();
   uint32_t o = base + smem[BLOCK / kWave + 1] + ex;
+  uint32_t r[kCountScanPer];
 #pragma unroll
   for (uint32_t j = 0; j < kCountScanPer; j++) {
-    if (i0 + j < n) out[i0 + j] = o;
+    r[j] = o;
     o += v[j];
   }
+  store_per(out, i0, n, r);
 }
 __device__ inline void count_scan_tile(const uint32_t *count, uint32_t *out, uint32_t n, uint32_t base,
                                        uint64_t *status, uint32_t tag, uint32_t *total, uint32_t *smem,

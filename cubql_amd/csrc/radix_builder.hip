@@ -622,10 +622,10 @@ __global__ __launch_bounds__(1024) void k_karras_scan2(const uint32_t *cA, const
   const uint32_t i0 = blockIdx.x * kCountScanTile + threadIdx.x * kCountScanPer;
   uint32_t a[kCountScanPer], b[kCountScanPer];
   uint32_t v[2] = {0u, 0u};
+  load_per(cA, i0, m, a);
+  load_per(cB, i0, m, b);
 #pragma unroll
   for (uint32_t j = 0; j < kCountScanPer; j++) {
-    a[j] = i0 + j < m ? cA[i0 + j] : 0u;
-    b[j] = i0 + j < m ? cB[i0 + j] : 0u;
     v[0] += a[j];
     v[1] += b[j];
   }
@@ -642,16 +642,17 @@ __global__ __launch_bounds__(1024) void k_karras_scan2(const uint32_t *cA, const
   }
   __syncthreads();
   uint32_t oa = sPre[0] + ex[0], ob = sPre[1] + ex[1];
+  uint32_t ra[kCountScanPer], rb[kCountScanPer];
 #pragma unroll
   for (uint32_t j = 0; j < kCountScanPer; j++) {
-    if (i0 + j < m) {
-      oA[i0 + j] = oa;
-      oB[i0 + j] = ob;
-      if (a[j]) listA[oa] = i0 + j;
-    }
+    ra[j] = oa;
+    rb[j] = ob;
+    if (i0 + j < m && a[j]) listA[oa] = i0 + j;
     oa += a[j];
     ob += b[j];
   }
+  store_per(oA, i0, m, ra);
+  store_per(oB, i0, m, rb);
 }
 
 // info[3] = 1 when the root is not a split node: at most T valid prims, or

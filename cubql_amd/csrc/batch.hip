@@ -243,11 +243,11 @@ __device__ __forceinline__ bool fcp_forest_on(StackT &stack, const Side &tgt, ui
     }
     return bestD;
   };
-  // seed the radius with the nearest-child leaf (visited again below,
-  // changing nothing)
+  // seed the radius with the nearest-child leaf, which the traversal then
+  // leaves out
   cubql_dev::NoStats st;
-  cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim, root, st);
-  return cubql_dev::for_each_prim_on(stack, bvh, q.x, q.y, q.z, bestD, prim, root, st);
+  const uint32_t seed = cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim, root, st);
+  return cubql_dev::for_each_prim_except_on(stack, bvh, q.x, q.y, q.z, bestD, prim, seed, root, st);
 }
 template <int STACK>
 __device__ __forceinline__ bool fcp_forest(const Side &tgt, uint32_t root, float3 q, int32_t &best,

@@ -48,10 +48,10 @@ __device__ __forceinline__ bool fcp_one(const BVH &bvh, const float *points, flo
     }
     return bestD;
   };
-  // seed the radius with the nearest-child leaf (visited again below,
-  // changing nothing)
-  cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim, 0u, st);
-  return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, bestD, prim, 0u, st);
+  // seed the radius with the nearest-child leaf, which the traversal then
+  // leaves out
+  const uint32_t seed = cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim, 0u, st);
+  return cubql_dev::for_each_prim_except<STACK>(bvh, q.x, q.y, q.z, bestD, prim, seed, 0u, st);
 }
 
 // The same over a binary tree on a caller-provided traversal stack.
@@ -70,8 +70,8 @@ __device__ __forceinline__ bool fcp_one_on(StackT &stack, const cubql_bvh3f &bvh
     return bestD;
   };
   cubql_dev::NoStats st;
-  cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim, 0u, st);
-  return cubql_dev::for_each_prim_on(stack, bvh, q.x, q.y, q.z, bestD, prim, 0u, st);
+  const uint32_t seed = cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim, 0u, st);
+  return cubql_dev::for_each_prim_except_on(stack, bvh, q.x, q.y, q.z, bestD, prim, seed, 0u, st);
 }
 
 // Entries of the fast kernel's LDS stack per lane: the seeded radius keeps

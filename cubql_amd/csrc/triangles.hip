@@ -56,10 +56,10 @@ __device__ __forceinline__ bool fcp_tri_one(const cubql_bvh3f &bvh, const int32_
     }
     return bestD;
   };
-  // seed the radius with the nearest-child leaf (visited again below,
-  // changing nothing)
-  cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim, 0u, st);
-  return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, bestD, prim, 0u, st);
+  // seed the radius with the nearest-child leaf, which the traversal then
+  // leaves out
+  const uint32_t seed = cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim, 0u, st);
+  return cubql_dev::for_each_prim_except<STACK>(bvh, q.x, q.y, q.z, bestD, prim, seed, 0u, st);
 }
 
 // The same on a caller-provided traversal stack.
@@ -83,8 +83,8 @@ __device__ __forceinline__ bool fcp_tri_one_on(StackT &stack, const cubql_bvh3f 
     return bestD;
   };
   NoStats st;
-  cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim, 0u, st);
-  return cubql_dev::for_each_prim_on(stack, bvh, q.x, q.y, q.z, bestD, prim, 0u, st);
+  const uint32_t seed = cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim, 0u, st);
+  return cubql_dev::for_each_prim_except_on(stack, bvh, q.x, q.y, q.z, bestD, prim, seed, 0u, st);
 }
 
 

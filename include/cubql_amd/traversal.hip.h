@@ -168,11 +168,14 @@ __device__ __forceinline__ bool for_each_leaf(const cubql_bvh3f &bvh, float qx, 
 // the traversal orders them) to one leaf and calls prim(primID) for its
 // prims.  A closest-point query that runs this first can start the full
 // traversal with the radius it found, so that on the way down it stacks only
-// far children that can still hold a closer prim.
+// far children that can still hold a closer prim.  Returns the leaf's first
+// primID slot (kNoLeaf for an empty tree), which for_each_prim_except
+// leaves out.
+constexpr uint32_t kNoLeaf = 0xffffffffu;
 template <typename PrimFn, typename Stats>
-__device__ __forceinline__ void visit_nearest_leaf(const cubql_bvh3f &bvh, float qx, float qy, float qz,
-                                                   PrimFn &&prim, uint32_t root, Stats &st) {
-  if (bvh.numNodes == 0) return;
+__device__ __forceinline__ uint32_t visit_nearest_leaf(const cubql_bvh3f &bvh, float qx, float qy, float qz,
+                                                       PrimFn &&prim, uint32_t root, Stats &st) {
+  if (bvh.numNodes == 0) return kNoLeaf;
   uint64_t node = node_oc(load_node(bvh.nodes, root));
   while (oc_cnt(node) == 0) {
     const uint32_t i0 = oc_off(node);
@@ -183,12 +186,13 @@ __device__ __forceinline__ void visit_nearest_leaf(const cubql_bvh3f &bvh, float
   }
   const uint32_t *ids = bvh.primIDs + oc_off(node);
   for (uint32_t j = 0; j < oc_cnt(node); j++) prim(ids[j]);
+  return oc_off(node);
 }
 template <typename PrimFn>
-__device__ __forceinline__ void visit_nearest_leaf(const cubql_bvh3f &bvh, float qx, float qy, float qz,
-                                                   PrimFn &&prim, uint32_t root = 0) {
+__device__ __forceinline__ uint32_t visit_nearest_leaf(const cubql_bvh3f &bvh, float qx, float qy, float qz,
+                                                       PrimFn &&prim, uint32_t root = 0) {
   NoStats st;
-  visit_nearest_leaf(bvh, qx, qy, qz, prim, root, st);
+  return visit_nearest_leaf(bvh, qx, qy, qz, prim, root, st);
 }
 
 // forEachPrim: prim(uint32_t primID) -> float (new squared cull radius).
@@ -218,6 +222,37 @@ __device__ __forceinline__ bool for_each_prim_on(StackT &stack, const cubql_bvh3
     return r;
   };
   return for_each_leaf_on(stack, bvh, qx, qy, qz, sqrMax, leaf, root, st);
+}
+
+// forEachPrim without the leaf whose first primID slot is `skip` (the leaf
+// visit_nearest_leaf returned: the caller has tested its prims, and testing
+// a prim again changes no closest-point result).  A left-out leaf leaves the
+// radius as it is.
+template <typename PrimFn>
+struct PrimsExcept {
+  const uint32_t *primIDs;
+  uint32_t skip;
+  PrimFn &prim;
+  __device__ __forceinline__ float operator()(const uint32_t *ids, uint32_t cnt) const {
+    if ((uint32_t)(ids - primIDs) == skip) return INFINITY;
+    float r = INFINITY;
+    for (uint32_t i = 0; i < cnt; i++) r = fminf(r, prim(ids[i]));
+    return r;
+  }
+};
+template <typename StackT, typename PrimFn, typename Stats>
+__device__ __forceinline__ bool for_each_prim_except_on(StackT &stack, const cubql_bvh3f &bvh, float qx, float qy,
+                                                        float qz, float sqrMax, PrimFn &&prim, uint32_t skip,
+                                                        uint32_t root, Stats &st) {
+  PrimsExcept<PrimFn> leaf{bvh.primIDs, skip, prim};
+  return for_each_leaf_on(stack, bvh, qx, qy, qz, sqrMax, leaf, root, st);
+}
+template <int STACK, typename PrimFn, typename Stats>
+__device__ __forceinline__ bool for_each_prim_except(const cubql_bvh3f &bvh, float qx, float qy, float qz,
+                                                     float sqrMax, PrimFn &&prim, uint32_t skip, uint32_t root,
+                                                     Stats &st) {
+  PrivateStack<STACK> stack;
+  return for_each_prim_except_on(stack, bvh, qx, qy, qz, sqrMax, prim, skip, root, st);
 }
 
 }  // namespace cubql_dev
@@ -355,30 +390,31 @@ __device__ __forceinline__ bool for_each_leaf(const WideBVH<W> &bvh, float qx, f
 // Descends along the nearest child to one leaf and calls prim(primID) for
 // its prims (the wide counterpart of visit_nearest_leaf).
 template <int W, typename PrimFn, typename Stats>
-__device__ __forceinline__ void visit_nearest_leaf(const WideBVH<W> &bvh, float qx, float qy, float qz,
-                                                   PrimFn &&prim, uint32_t root, Stats &st) {
-  if (bvh.numNodes == 0) return;
+__device__ __forceinline__ uint32_t visit_nearest_leaf(const WideBVH<W> &bvh, float qx, float qy, float qz,
+                                                       PrimFn &&prim, uint32_t root, Stats &st) {
+  if (bvh.numNodes == 0) return kNoLeaf;
   uint32_t node = root;
   for (int guard = 0; guard < (1 << 20); guard++) {
     uint64_t key[W];
     wide_child_keys<W>(bvh, node, qx, qy, qz, INFINITY, key);
     st.nodes(32 * W);
-    if (key[0] == ~0ull) return;
+    if (key[0] == ~0ull) return kNoLeaf;
     const uint32_t payload = (uint32_t)key[0];
     if (payload & 0x80000000u) {
       const uint64_t bits = wide_leaf_bits(bvh, payload, Log2<W>::value);
       const uint32_t *ids = bvh.primIDs + wc_off(bits);
       for (uint32_t j = 0; j < wc_cnt(bits); j++) prim(ids[j]);
-      return;
+      return wc_off(bits);
     }
     node = payload;
   }
+  return kNoLeaf;
 }
 template <int W, typename PrimFn>
-__device__ __forceinline__ void visit_nearest_leaf(const WideBVH<W> &bvh, float qx, float qy, float qz,
-                                                   PrimFn &&prim, uint32_t root = 0) {
+__device__ __forceinline__ uint32_t visit_nearest_leaf(const WideBVH<W> &bvh, float qx, float qy, float qz,
+                                                       PrimFn &&prim, uint32_t root = 0) {
   NoStats st;
-  visit_nearest_leaf(bvh, qx, qy, qz, prim, root, st);
+  return visit_nearest_leaf(bvh, qx, qy, qz, prim, root, st);
 }
 
 template <int STACK, int W, typename PrimFn, typename Stats>
@@ -396,6 +432,14 @@ __device__ __forceinline__ bool for_each_prim(const WideBVH<W> &bvh, float qx, f
                                               PrimFn &&prim, uint32_t root = 0) {
   NoStats st;
   return for_each_prim<STACK>(bvh, qx, qy, qz, sqrMax, prim, root, st);
+}
+// (the wide counterpart of the binary for_each_prim_except)
+template <int STACK, int W, typename PrimFn, typename Stats>
+__device__ __forceinline__ bool for_each_prim_except(const WideBVH<W> &bvh, float qx, float qy, float qz,
+                                                     float sqrMax, PrimFn &&prim, uint32_t skip, uint32_t root,
+                                                     Stats &st) {
+  PrimsExcept<PrimFn> leaf{bvh.primIDs, skip, prim};
+  return for_each_leaf<STACK>(bvh, qx, qy, qz, sqrMax, leaf, root, st);
 }
 
 }  // namespace cubql_dev

@@ -2,9 +2,9 @@
 counters behind the query rooflines.
 
 Closed forms:
-* a tree that is one leaf of n points: closest-point queries load no node
-  pair and test 2n prims each (the seeding descent visits the leaf, then the
-  traversal visits it again); kNN queries test n prims each;
+* a tree that is one leaf of n points: closest-point and kNN queries load
+  no node pair and test n prims each (the seeding descent visits the leaf;
+  the traversal then leaves it out);
 * a two-leaf tree (n in (T, 2T]): every query loads the root's child pair
   at least once.
 Bounds on a real tree: at least one prim tested per query, node bytes a
@@ -30,7 +30,7 @@ def test_stats_single_leaf(cq):
     Q = torch.from_numpy(rng.random((100, 3), dtype=np.float32)).cuda()
     bvh = cq.gpuBuilder(cq.points_to_boxes(P), cq.BuildConfig(makeLeafThreshold=8))
     st = cq.query_stats(bvh, Q, data=P, k=0)
-    assert st == {"node_bytes": 0, "prims": 2 * 6 * 100, "queries": 100}
+    assert st == {"node_bytes": 0, "prims": 6 * 100, "queries": 100}
     st = cq.query_stats(bvh, Q, data=P, k=4)
     assert st == {"node_bytes": 0, "prims": 6 * 100, "queries": 100}
     bvh.free()

@@ -15,8 +15,14 @@ __device__ __forceinline__ uint32_t spread10(uint32_t v) {
   return x;
 }
 
-// 30-bit Morton key of each query in the cells of the root box (queries
-// outside it clamp to the border cells); value = query index.
+// Bits of the Morton key the queries are sorted by: the top 24 of the 30
+// (3 sort passes, cells of 1/256 of the root box per axis; 10M queries
+// leave less than one per cell, so the last 6 bits change little).
+constexpr int kOrderBits = 24;
+
+// Top kOrderBits of the 30-bit Morton key of each query in the cells of the
+// root box (queries outside it clamp to the border cells); value = query
+// index.
 __global__ void k_query_keys(const cubql_box3f *rootBox, const float *queries, uint32_t nq, uint32_t *keys,
                              uint32_t *vals, uint32_t *count) {
   const cubql_box3f bounds = *rootBox;
@@ -30,7 +36,7 @@ __global__ void k_query_keys(const cubql_box3f *rootBox, const float *queries, u
       const float f = fmaxf((queries[3 * (size_t)i + d] - bounds.lower[d]) * scale, 0.f);
       cell[d] = f < 1023.f ? (uint32_t)f : 1023u;
     }
-    keys[i] = spread10(cell[0]) | (spread10(cell[1]) << 1) | (spread10(cell[2]) << 2);
+    keys[i] = (spread10(cell[0]) | (spread10(cell[1]) << 1) | (spread10(cell[2]) << 2)) >> (30 - kOrderBits);
     vals[i] = i;
   }
 }
@@ -39,7 +45,7 @@ __global__ void k_query_keys(const cubql_box3f *rootBox, const float *queries, u
 
 size_t query_order_ws_bytes(uint32_t nq) {
   const size_t arr = ((size_t)nq * 4 + 255) & ~(size_t)255;
-  return 4 * arr + 256 + radix_sort_ws_bytes(nq, 30);
+  return 4 * arr + 256 + radix_sort_ws_bytes(nq, qo::kOrderBits);
 }
 
 int query_order_ws(const cubql_bvh3f &bvh, const float *queries, uint32_t nq, hipStream_t s, void *ws,
@@ -60,7 +66,7 @@ int query_order_box_ws(const cubql_box3f *rootBox, const float *queries, uint32_
   qo::k_query_keys<<<std::min<uint32_t>(div_up(nq, 256), 4096), 256, 0, s>>>(rootBox, queries, nq, kA, vA, cnt);
   CUBQL_LAUNCH_CHECK();
   uint32_t *kRes = nullptr, *vRes = nullptr;
-  CUBQL_CHECK(radix_sort_pairs_ws_u32(kA, vA, kB, vB, cnt, nq, 30, s, sortWs, &kRes, &vRes));
+  CUBQL_CHECK(radix_sort_pairs_ws_u32(kA, vA, kB, vB, cnt, nq, qo::kOrderBits, s, sortWs, &kRes, &vRes));
   *order = vRes;
   return CUBQL_OK;
 }

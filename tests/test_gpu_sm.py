@@ -197,6 +197,15 @@ def test_gpu_empty_and_tiny_inputs(cq, oracle_mod):
         ids, d2 = cq.points.findClosest(bvh, dP, dP)
         assert np.array_equal(ids.cpu().numpy(), np.arange(n, dtype=np.int32))
         assert np.all(d2.cpu().numpy() == 0)
+        # kNN with k up to 8 (the seeded traversal on the LDS stack), also
+        # with fewer prims than k, against brute force; queries off the points
+        Q = np.random.default_rng(n + 100).random((300, 3), dtype=np.float32)
+        for k in (1, 4, 8):
+            kid, kd, nf = cq.points.findKNN(bvh, dP, torch.from_numpy(Q).cuda(), k)
+            bki, bkd = oracle_mod.knn_brute(P, Q, k)
+            assert np.array_equal(kid.cpu().numpy(), bki), (n, k)
+            assert np.array_equal(kd.cpu().numpy(), bkd), (n, k)
+            assert np.all(nf.cpu().numpy() == min(n, k)), (n, k)
         bvh.free()
     # all-invalid input: no valid prims -> every query reports not found
     boxes = torch.tensor([[1, 1, 1, 0, 0, 0]] * 5, dtype=torch.float32, device="cuda")

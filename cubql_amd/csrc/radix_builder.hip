@@ -1234,17 +1234,41 @@ __device__ __forceinline__ uint32_t find_job(const uint32_t *start, uint32_t J, 
   return lo;
 }
 
-// Centroid domain of every job, over all rebin prims at once: lanes of a
-// wave whose prims all belong to one job (the common case: jobs are
-// contiguous and large) reduce in registers and issue one atomic per bound;
-// mixed waves fall back to per-lane atomics.  jobDom holds encoded bounds.
+// Centroid domain of every job, over all rebin prims at once.  Each wave
+// walks a contiguous run of 64-prim groups and keeps the bounds of its
+// current job in registers while its groups stay in that job (jobs are
+// contiguous and usually large), adding them to the job's domain (one
+// atomic per bound) when the job changes and at the end of the run; a group
+// that straddles jobs adds its prims lane by lane.  So a job takes about
+// one atomic per wave and bound, not one per group: the domains of large
+// jobs are the same few words, and same-address atomics serialise.
+// jobDom holds encoded bounds.
 __global__ __launch_bounds__(kBlock) void k_rebin_grow(RCtx c) {
   const uint32_t J = c.ctr[C_CURJOBS];
   const uint32_t P = c.ctr[C_JOBPRIMS];
   const uint32_t lane = lane_id();
-  const uint32_t stride = gridDim.x * kBlock;
-  for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~(uint32_t)(kWave - 1)); base < P; base += stride) {
-    const uint32_t t = base + lane;
+  const uint32_t nWaves = gridDim.x * (kBlock / kWave);
+  const uint32_t w = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  const uint32_t groups = div_up(P, (uint32_t)kWave);
+  const uint32_t per = div_up(groups, nWaves);
+  const uint32_t g0 = min(groups, w * per), g1 = min(groups, g0 + per);
+  float acc[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  uint32_t accJob = 0xffffffffu;  // (wave-uniform)
+  auto flush = [&]() {
+    if (accJob == 0xffffffffu) return;
+    float v[6];
+    for (int d = 0; d < 3; d++) {
+      v[d] = wave_min_dpp(acc[d]);
+      v[d + 3] = wave_max_dpp(acc[d + 3]);
+    }
+    if (lane < 6) {
+      const float x = lane == 0 ? v[0] : lane == 1 ? v[1] : lane == 2 ? v[2] : lane == 3 ? v[3] : lane == 4 ? v[4] : v[5];
+      if (lane < 3) atomicMin(&c.jobDom[6 * (size_t)accJob + lane], enc_f(x));
+      else atomicMax(&c.jobDom[6 * (size_t)accJob + lane], enc_f(x));
+    }
+  };
+  for (uint32_t g = g0; g < g1; g++) {
+    const uint32_t t = g * kWave + lane;
     const bool active = t < P;
     uint32_t j = 0;
     float ce[3] = {0.f, 0.f, 0.f};
@@ -1252,26 +1276,26 @@ __global__ __launch_bounds__(kBlock) void k_rebin_grow(RCtx c) {
       j = find_job(c.jobStart, J, t);
       center3(load_box(c.boxes, c.ids[c.ranges[c.jobs[j]].x + (t - c.jobStart[j])]), ce);
     }
-    const uint32_t j0 = __builtin_amdgcn_readfirstlane(j);  // lane 0 is active when base < P
-    int32_t *dom = c.jobDom + 6 * (size_t)j;
+    const uint32_t j0 = __builtin_amdgcn_readfirstlane(j);  // lane 0 is active: g * 64 < P
     if (__ballot(active && j != j0) == 0) {
-      float v[6];
-      for (int d = 0; d < 3; d++) {
-        v[d] = wave_min_dpp(active ? ce[d] : INFINITY);
-        v[d + 3] = wave_max_dpp(active ? ce[d] : -INFINITY);
+      if (j0 != accJob) {
+        flush();
+        for (int d = 0; d < 3; d++) { acc[d] = INFINITY; acc[d + 3] = -INFINITY; }
+        accJob = j0;
       }
-      if (lane < 6) {
-        const float x = lane == 0 ? v[0] : lane == 1 ? v[1] : lane == 2 ? v[2] : lane == 3 ? v[3] : lane == 4 ? v[4] : v[5];
-        if (lane < 3) atomicMin(&c.jobDom[6 * (size_t)j0 + lane], enc_f(x));
-        else atomicMax(&c.jobDom[6 * (size_t)j0 + lane], enc_f(x));
+      for (int d = 0; d < 3; d++) {
+        acc[d] = fminf(acc[d], active ? ce[d] : INFINITY);
+        acc[d + 3] = fmaxf(acc[d + 3], active ? ce[d] : -INFINITY);
       }
     } else if (active) {
+      int32_t *dom = c.jobDom + 6 * (size_t)j;
       for (int d = 0; d < 3; d++) {
         atomicMin(&dom[d], enc_f(ce[d]));
         atomicMax(&dom[d + 3], enc_f(ce[d]));
       }
     }
   }
+  flush();
 }
 
 // Per job: quantizer of its domain and whether to key by local index: the

@@ -954,6 +954,7 @@ int sah_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T
   const uint32_t selGrid = std::min<uint32_t>(c.maxLarge, (uint32_t)numCU * 8);
   int level = 0;
   std::vector<std::pair<int, int>> binSlots, partSlots;
+  bool synced = false;  // hw holds the counters after the last level batch
   while (n) {
     for (int i = 0; i < kLevelsPerSync && level < (int)maxLevels; i++, level++) {
       int pb = prof_begin(s);
@@ -973,11 +974,13 @@ int sah_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t T
       return fail((set_error("sah level sync failed"), CUBQL_ERR_HIP));
     const Counters *hc = (const Counters *)hw;
     if (hc->spinFault) return fail((set_error("sah build: a look-back wait exceeded its bound"), CUBQL_ERR_INTERNAL));
+    synced = true;
     if (hc->numLarge[0] == 0 && hc->numLarge[1] == 0) break;
     if (level >= (int)maxLevels) return fail((set_error("SAH tree deeper than %u levels", maxLevels), CUBQL_ERR_INTERNAL));
   }
-  if (hipMemcpyAsync(hw, c.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess)
+  // (after the level loop the counters are already on the host)
+  if (!synced && (hipMemcpyAsync(hw, c.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                  hipStreamSynchronize(s) != hipSuccess))
     return fail((set_error("sah sync failed"), CUBQL_ERR_HIP));
   const Counters hc0 = *(const Counters *)hw;
   const uint32_t numSmall = hc0.numSmall, numValid = hc0.numValid;

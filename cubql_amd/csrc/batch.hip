@@ -36,6 +36,7 @@ namespace bt {
 
 constexpr int kCloudShift = 30;
 constexpr uint32_t kNoRoot = 0xffffffffu;
+constexpr uint32_t kNoSlot = 0xffffffffu;
 
 struct Side {
   const float *pts;
@@ -44,6 +45,9 @@ struct Side {
   // persistent (kept for the queries)
   uint64_t *keys;       // sorted keys
   uint32_t *ids;        // sorted global point indices = primIDs of the forest
+  float4 *spts;         // points in sorted order: x, y, z, global index (bits)
+  uint32_t *nnS;        // per sorted query: sorted slot of its closest point
+  float *d2S;           //   on the other side (kNoSlot if none) and its distance
   Node *nodes;          // capacity 2n + 2
   uint32_t *rootOf;     // numClouds
   uint32_t *ctr;        // counters, see C_* (C_WORDS words)
@@ -102,6 +106,16 @@ __global__ __launch_bounds__(256) void k_bkeys(Side sd, uint32_t numClouds) {
     const uint32_t m = spread10(cell[0]) | (spread10(cell[1]) << 1) | (spread10(cell[2]) << 2);
     sd.keys[i] = ((uint64_t)c << kCloudShift) | m;
     sd.ids[i] = i;
+  }
+}
+
+// points in sorted order, with their global index in w: a leaf's prims are
+// then one contiguous run for the refit and the queries
+__global__ __launch_bounds__(256) void k_bgather(Side sd) {
+  for (uint32_t t = blockIdx.x * 256 + threadIdx.x; t < sd.n; t += gridDim.x * 256) {
+    const uint32_t i = sd.ids[t];
+    sd.spts[t] = make_float4(sd.pts[3 * (size_t)i], sd.pts[3 * (size_t)i + 1], sd.pts[3 * (size_t)i + 2],
+                             __uint_as_float(i));
   }
 }
 
@@ -206,11 +220,11 @@ __global__ void k_brefit_level(Side sd, Temp tp, int level) {
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     if (cnt) {
       for (uint32_t j = 0; j < cnt; j++) {
-        const size_t p = sd.ids[o + j];
+        const float4 p = sd.spts[o + j];
+        const float x[3] = {p.x, p.y, p.z};
         for (int d = 0; d < 3; d++) {
-          const float x = sd.pts[3 * p + d];
-          lo[d] = fminf(lo[d], x);
-          hi[d] = fmaxf(hi[d], x);
+          lo[d] = fminf(lo[d], x[d]);
+          hi[d] = fmaxf(hi[d], x[d]);
         }
       }
     } else {
@@ -222,38 +236,50 @@ __global__ void k_brefit_level(Side sd, Temp tp, int level) {
 }
 
 // The closest point of one query in the tree at `root`, on a caller's
-// traversal stack (a private array, or LDS for the shallow stacks).
+// traversal stack (a private array, or LDS for the shallow stacks): its
+// global index (lowest among equal distances), its sorted slot and distance.
+// Leaf prims are read from the sorted points, a contiguous run per leaf.
 template <typename StackT>
 __device__ __forceinline__ bool fcp_forest_on(StackT &stack, const Side &tgt, uint32_t root, float3 q,
-                                              int32_t &best, float &bestD) {
+                                              int32_t &best, uint32_t &bestSlot, float &bestD) {
   best = -1;
+  bestSlot = kNoSlot;
   bestD = INFINITY;
   cubql_bvh3f bvh;
   bvh.nodes = (cubql_node3f *)tgt.nodes;
   bvh.numNodes = 2;
-  bvh.primIDs = tgt.ids;
+  bvh.primIDs = tgt.ids;  // only for leaf slot offsets: ids are not read
   bvh.numPrims = tgt.n;
-  const float *pts = tgt.pts;
-  auto prim = [&](uint32_t pid) -> float {
-    const float px = pts[3 * (size_t)pid], py = pts[3 * (size_t)pid + 1], pz = pts[3 * (size_t)pid + 2];
-    const float d = cubql_dev::sqr_dist_pt(px, py, pz, q.x, q.y, q.z);
-    if (d < bestD || (d == bestD && best >= 0 && (int32_t)pid < best)) {
+  const float4 *spts = tgt.spts;
+  auto test = [&](uint32_t slot) {
+    const float4 p = spts[slot];
+    const float d = cubql_dev::sqr_dist_pt(p.x, p.y, p.z, q.x, q.y, q.z);
+    const int32_t pid = (int32_t)__float_as_uint(p.w);
+    if (d < bestD || (d == bestD && best >= 0 && pid < best)) {
       bestD = d;
-      best = (int32_t)pid;
+      best = pid;
+      bestSlot = slot;
     }
-    return bestD;
   };
   // seed the radius with the nearest-child leaf, which the traversal then
   // leaves out
   cubql_dev::NoStats st;
-  const uint32_t seed = cubql_dev::visit_nearest_leaf(bvh, q.x, q.y, q.z, prim, root, st);
-  return cubql_dev::for_each_prim_except_on(stack, bvh, q.x, q.y, q.z, bestD, prim, seed, root, st);
+  const uint64_t seedLeaf = cubql_dev::nearest_leaf(bvh, q.x, q.y, q.z, root, st);
+  const uint32_t seed = cubql_dev::oc_off(seedLeaf);
+  for (uint32_t j = 0; j < cubql_dev::oc_cnt(seedLeaf); j++) test(seed + j);
+  auto leaf = [&](const uint32_t *ids, uint32_t cnt) -> float {
+    const uint32_t off = (uint32_t)(ids - tgt.ids);
+    if (off == seed) return INFINITY;
+    for (uint32_t j = 0; j < cnt; j++) test(off + j);
+    return bestD;
+  };
+  return cubql_dev::for_each_leaf_on(stack, bvh, q.x, q.y, q.z, bestD, leaf, root, st);
 }
 template <int STACK>
 __device__ __forceinline__ bool fcp_forest(const Side &tgt, uint32_t root, float3 q, int32_t &best,
-                                           float &bestD) {
+                                           uint32_t &bestSlot, float &bestD) {
   cubql_dev::PrivateStack<STACK> stack;
-  return fcp_forest_on(stack, tgt, root, q, best, bestD);
+  return fcp_forest_on(stack, tgt, root, q, best, bestSlot, bestD);
 }
 
 // queries of side `qs` against the forest of side `ts`, in Morton order of qs
@@ -262,22 +288,24 @@ __global__ __launch_bounds__(BLOCK) void k_bquery(Side qs, Side ts, int32_t *nn,
   // queries sorted by (cloud, Morton key): one contiguous run per XCD
   const uint32_t t = xcd_chunk(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
   if (t >= qs.n) return;
-  const uint32_t qi = qs.ids[t];
+  const float4 qv = qs.spts[t];
+  const uint32_t qi = __float_as_uint(qv.w);
   const uint32_t cloud = (uint32_t)(qs.keys[t] >> kCloudShift);
   const uint32_t root = ts.rootOf[cloud];
   int32_t best = -1;
+  uint32_t bestSlot = kNoSlot;
   float bestD = INFINITY;
   if (root != kNoRoot) {
-    const float3 q = make_float3(qs.pts[3 * (size_t)qi], qs.pts[3 * (size_t)qi + 1], qs.pts[3 * (size_t)qi + 2]);
+    const float3 q = make_float3(qv.x, qv.y, qv.z);
     bool done;
     if constexpr (STACK <= 16) {
       // shallow stacks live in LDS (at most 32 KB a workgroup), deeper
       // ones in the lane's private memory
       __shared__ uint2 sStack[STACK * BLOCK];
       cubql_dev::LdsStack<STACK, BLOCK> stack(sStack + threadIdx.x);
-      done = fcp_forest_on(stack, ts, root, q, best, bestD);
+      done = fcp_forest_on(stack, ts, root, q, best, bestSlot, bestD);
     } else {
-      done = fcp_forest<STACK>(ts, root, q, best, bestD);
+      done = fcp_forest<STACK>(ts, root, q, best, bestSlot, bestD);
     }
     if (!done) {
       ovf[2 + atomicAdd(&ovf[0], 1u)] = t;
@@ -286,26 +314,33 @@ __global__ __launch_bounds__(BLOCK) void k_bquery(Side qs, Side ts, int32_t *nn,
   }
   nn[qi] = best;
   d2[qi] = bestD;
+  qs.nnS[t] = bestSlot;
+  qs.d2S[t] = bestD;
 }
 
 __global__ __launch_bounds__(64) void k_bquery_deep(Side qs, Side ts, int32_t *nn, float *d2, uint32_t *ovf) {
   const uint32_t n = ovf[0];
   for (uint32_t i = blockIdx.x * 64 + threadIdx.x; i < n; i += gridDim.x * 64) {
     const uint32_t t = ovf[2 + i];
-    const uint32_t qi = qs.ids[t];
+    const float4 qv = qs.spts[t];
+    const uint32_t qi = __float_as_uint(qv.w);
     const uint32_t cloud = (uint32_t)(qs.keys[t] >> kCloudShift);
-    const float3 q = make_float3(qs.pts[3 * (size_t)qi], qs.pts[3 * (size_t)qi + 1], qs.pts[3 * (size_t)qi + 2]);
+    const float3 q = make_float3(qv.x, qv.y, qv.z);
     int32_t best;
+    uint32_t bestSlot;
     float bestD;
-    if (!fcp_forest<1024>(ts, ts.rootOf[cloud], q, best, bestD)) {
+    if (!fcp_forest<1024>(ts, ts.rootOf[cloud], q, best, bestSlot, bestD)) {
       // deeper than the deep stack (a cloud's Morton tree is at most 46
       // levels deep, so this does not happen): marked incomplete
       atomicAdd(&ovf[1], 1u);
       best = -2;
+      bestSlot = kNoSlot;
       bestD = INFINITY;
     }
     nn[qi] = best;
     d2[qi] = bestD;
+    qs.nnS[t] = bestSlot;
+    qs.d2S[t] = bestD;
   }
 }
 
@@ -318,8 +353,9 @@ __global__ __launch_bounds__(64) void k_bquery_deep(Side qs, Side ts, int32_t *n
 // neighbour's term is scattered into gradT: queries run in Morton order, so
 // lanes next to each other often share their nearest point; a segmented
 // wave sum over runs of equal neighbour ids leaves one atomic per run.
-__global__ __launch_bounds__(256) void k_bback(Side qs, Side ts, const int32_t *nn, const float *d2,
-                                               float *loss, float *gradQ, float *gradT, bool ownFirst) {
+// The query, its result and the neighbour are read by sorted position.
+__global__ __launch_bounds__(256) void k_bback(Side qs, Side ts, float *loss, float *gradQ, float *gradT,
+                                               bool ownFirst) {
   const uint32_t t = blockIdx.x * 256 + threadIdx.x;
   const bool act = t < qs.n;
   const uint32_t lane = lane_id();
@@ -327,18 +363,21 @@ __global__ __launch_bounds__(256) void k_bback(Side qs, Side ts, const int32_t *
   int32_t p = -1;
   float contrib = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
   if (act) {
-    qi = qs.ids[t];
+    const float4 qv = qs.spts[t];
+    qi = __float_as_uint(qv.w);
     cloud = (uint32_t)(qs.keys[t] >> kCloudShift);
-    p = nn[qi];
-    if (p >= 0) {
-      const float d = sqrtf(d2[qi]);
+    const uint32_t ps = qs.nnS[t];
+    if (ps != kNoSlot) {
+      const float4 pv = ts.spts[ps];
+      p = (int32_t)__float_as_uint(pv.w);
+      const float d = sqrtf(qs.d2S[t]);
       const float g = 1.f / (float)(qs.off[cloud + 1] - qs.off[cloud]);
       contrib = g * d;
       if (d > 0.f) {
         const float s = g / d;
-        gx = s * (qs.pts[3 * (size_t)qi + 0] - ts.pts[3 * (size_t)p + 0]);
-        gy = s * (qs.pts[3 * (size_t)qi + 1] - ts.pts[3 * (size_t)p + 1]);
-        gz = s * (qs.pts[3 * (size_t)qi + 2] - ts.pts[3 * (size_t)p + 2]);
+        gx = s * (qv.x - pv.x);
+        gy = s * (qv.y - pv.y);
+        gz = s * (qv.z - pv.z);
       }
     }
     if (gradQ) {
@@ -403,7 +442,7 @@ struct Layout {
 };
 
 struct Plan {
-  size_t keys[2], ids[2], nodes[2], roots[2], ctr[2];
+  size_t keys[2], ids[2], spts[2], nnS[2], d2S[2], nodes[2], roots[2], ctr[2];
   size_t keysB, valsB, ranges, levels, sortWs, ovf;
   size_t total;
   uint32_t maxLevels;
@@ -418,6 +457,9 @@ Plan make_plan(uint32_t numClouds, uint32_t nSrc, uint32_t nTgt) {
     const size_t n = std::max(ns[s], 1u);
     p.keys[s] = L.take(8 * n);
     p.ids[s] = L.take(4 * n);
+    p.spts[s] = L.take(16 * n);
+    p.nnS[s] = L.take(4 * n);
+    p.d2S[s] = L.take(4 * n);
     p.nodes[s] = L.take(32 * (2 * n + 2));
     p.roots[s] = L.take(4 * (size_t)std::max(numClouds, 1u));
     p.ctr[s] = L.take(4 * C_WORDS);
@@ -467,6 +509,8 @@ int build_side(Side &sd, Temp &tp, const Args &a, const Plan &pl, hipStream_t s,
     CUBQL_HIP(hipMemcpyAsync(sd.keys, kr, 8 * (size_t)sd.n, hipMemcpyDeviceToDevice, s));
     CUBQL_HIP(hipMemcpyAsync(sd.ids, vr, 4 * (size_t)sd.n, hipMemcpyDeviceToDevice, s));
   }
+  const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(div_up(sd.n, 256), (uint32_t)numCU * 4));
+  if (sd.n) k_bgather<<<grid, 256, 0, s>>>(sd);
   k_binit<<<std::max(1u, std::min<uint32_t>(div_up(a.numClouds, 256), 64)), 256, 0, s>>>(sd, tp, a.numClouds);
   k_bsingle_root<<<1, 64, 0, s>>>(sd);
   // depth bound: cloud bits + 30 Morton bits + mid splits of equal keys
@@ -474,7 +518,6 @@ int build_side(Side &sd, Temp &tp, const Args &a, const Plan &pl, hipStream_t s,
   const uint32_t maxCloud = a.prm.maxCloudSize ? a.prm.maxCloudSize : std::max(sd.n, 1u);
   while ((1ull << midBits) < maxCloud) midBits++;
   const int levels = std::min(cbits + kCloudShift + midBits + 2, kMaxLevels);
-  const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(div_up(sd.n, 256), (uint32_t)numCU * 4));
   const uint32_t T = (uint32_t)std::max(1, std::min(a.prm.leafSize, 65535));
   for (int l = 0; l < levels; l++) k_blevel<<<grid, 256, 0, s>>>(sd, tp, T, l);
   for (int l = levels - 1; l >= 0; l--) k_brefit_level<<<grid, 256, 0, s>>>(sd, tp, l);
@@ -525,6 +568,9 @@ int run(const Args &a, hipStream_t s) {
     side[i].n = ns[i];
     side[i].keys = (uint64_t *)(a.ws + pl.keys[i]);
     side[i].ids = (uint32_t *)(a.ws + pl.ids[i]);
+    side[i].spts = (float4 *)(a.ws + pl.spts[i]);
+    side[i].nnS = (uint32_t *)(a.ws + pl.nnS[i]);
+    side[i].d2S = (float *)(a.ws + pl.d2S[i]);
     side[i].nodes = (Node *)(a.ws + pl.nodes[i]);
     side[i].rootOf = (uint32_t *)(a.ws + pl.roots[i]);
     side[i].ctr = (uint32_t *)(a.ws + pl.ctr[i]);
@@ -541,7 +587,7 @@ int run(const Args &a, hipStream_t s) {
   CUBQL_CHECK(build_side(side[0], tp, a, pl, s, numCU));
   CUBQL_CHECK(build_side(side[1], tp, a, pl, s, numCU));
   prof_end(ps, s, "batch_build");
-  prof_set(ps, (12.0 + 12.0 + 8.0 * 2 * 6 + 8.0 + 24.0) * (a.nSrc + a.nTgt), a.nSrc + a.nTgt);
+  prof_set(ps, (12.0 + 12.0 + 8.0 * 2 * 6 + 8.0 + 24.0 + 32.0) * (a.nSrc + a.nTgt), a.nSrc + a.nTgt);
   const uint32_t fg = 1024;
   if (a.loss) k_fill_f32<<<std::max(1u, std::min(div_up(a.numClouds, 256), fg)), 256, 0, s>>>(a.loss, 0.f, a.numClouds);
   // gradSrc is initialised by the first backward direction's stores; an
@@ -556,11 +602,9 @@ int run(const Args &a, hipStream_t s) {
   ps = prof_begin(s);
   if (a.loss || a.gradSrc || a.gradTgt) {
     if (a.nSrc)
-      k_bback<<<div_up(a.nSrc, 256), 256, 0, s>>>(side[0], side[1], a.srcNN, a.srcD2, a.loss, a.gradSrc, a.gradTgt,
-                                                  true);
+      k_bback<<<div_up(a.nSrc, 256), 256, 0, s>>>(side[0], side[1], a.loss, a.gradSrc, a.gradTgt, true);
     if (a.nTgt)
-      k_bback<<<div_up(a.nTgt, 256), 256, 0, s>>>(side[1], side[0], a.tgtNN, a.tgtD2, a.loss, a.gradTgt, a.gradSrc,
-                                                  false);
+      k_bback<<<div_up(a.nTgt, 256), 256, 0, s>>>(side[1], side[0], a.loss, a.gradTgt, a.gradSrc, false);
   }
   prof_end(ps, s, "batch_backward");
   prof_set(ps, 68.0 * (a.nSrc + a.nTgt), a.nSrc + a.nTgt);

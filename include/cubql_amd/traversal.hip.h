@@ -172,10 +172,11 @@ __device__ __forceinline__ bool for_each_leaf(const cubql_bvh3f &bvh, float qx, 
 // primID slot (kNoLeaf for an empty tree), which for_each_prim_except
 // leaves out.
 constexpr uint32_t kNoLeaf = 0xffffffffu;
-template <typename PrimFn, typename Stats>
-__device__ __forceinline__ uint32_t visit_nearest_leaf(const cubql_bvh3f &bvh, float qx, float qy, float qz,
-                                                       PrimFn &&prim, uint32_t root, Stats &st) {
-  if (bvh.numNodes == 0) return kNoLeaf;
+// The offset/count word of the leaf that descent ends in (the tree must not
+// be empty).
+template <typename Stats>
+__device__ __forceinline__ uint64_t nearest_leaf(const cubql_bvh3f &bvh, float qx, float qy, float qz,
+                                                 uint32_t root, Stats &st) {
   uint64_t node = node_oc(load_node(bvh.nodes, root));
   while (oc_cnt(node) == 0) {
     const uint32_t i0 = oc_off(node);
@@ -184,6 +185,13 @@ __device__ __forceinline__ uint32_t visit_nearest_leaf(const cubql_bvh3f &bvh, f
     st.nodes(64);
     node = sqr_dist_box(qx, qy, qz, n0) < sqr_dist_box(qx, qy, qz, n1) ? node_oc(n0) : node_oc(n1);
   }
+  return node;
+}
+template <typename PrimFn, typename Stats>
+__device__ __forceinline__ uint32_t visit_nearest_leaf(const cubql_bvh3f &bvh, float qx, float qy, float qz,
+                                                       PrimFn &&prim, uint32_t root, Stats &st) {
+  if (bvh.numNodes == 0) return kNoLeaf;
+  const uint64_t node = nearest_leaf(bvh, qx, qy, qz, root, st);
   const uint32_t *ids = bvh.primIDs + oc_off(node);
   for (uint32_t j = 0; j < oc_cnt(node); j++) prim(ids[j]);
   return oc_off(node);

@@ -552,6 +552,12 @@ class ChamferBatch:
         self.params = params or BatchParams()
         dev = self.src.device
         ns, nt = self.src.shape[0], self.tgt.shape[0]
+        for name, off, n in (("src_off", self.src_off, ns), ("tgt_off", self.tgt_off, nt)):
+            o = off.cpu()
+            if (self.num_clouds < 0 or o.numel() != self.num_clouds + 1 or int(o[0]) != 0
+                    or int(o[-1]) != n or bool((o[1:] < o[:-1]).any())):
+                raise CubqlError(f"{name} must hold {max(self.num_clouds, 0) + 1} non-decreasing "
+                                 f"offsets from 0 to {n}")
         wsb = lib().cubql_batch_chamfer_workspace_f3(self.num_clouds, ns, nt)
         self.ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
         self.src_nn = torch.empty(ns, dtype=torch.int32, device=dev)

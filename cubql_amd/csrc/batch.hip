@@ -176,8 +176,16 @@ __global__ __launch_bounds__(256) void k_blevel(Side sd, Temp tp, uint32_t T, in
       node = open[i];
       r = tp.ranges[node];
       const uint32_t size = r.y - r.x;
-      multi = (sd.keys[r.x] >> kCloudShift) != (sd.keys[r.y - 1] >> kCloudShift);
-      if (multi || size > T) {
+      const uint32_t c0 = (uint32_t)(sd.keys[r.x] >> kCloudShift), c1 = (uint32_t)(sd.keys[r.y - 1] >> kCloudShift);
+      multi = c0 != c1;
+      if (multi) {
+        // the highest differing key bit is a cloud bit: the first key at or
+        // above the search key find_split64 would look for is the first point
+        // of cloud cs, which the offsets give without a search
+        const uint32_t cs = c1 & (~0u << (31 - __clz(c0 ^ c1)));
+        split = sd.off[cs];
+        doSplit = true;
+      } else if (size > T) {
         split = find_split64(sd.keys, r.x, r.y);
         doSplit = true;
       } else {

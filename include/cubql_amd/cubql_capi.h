@@ -363,7 +363,8 @@ size_t cubql_batch_chamfer_workspace_f3(uint32_t numClouds, uint32_t numSrc,
 /* Batched bidirectional closest point + Chamfer adjoint over numClouds
    independent cloud pairs (no reference counterpart).  Cloud c holds source
    points [srcOffsets[c], srcOffsets[c+1]) and target points likewise
-   (device arrays of numClouds+1 entries).  Builds one 30-bit Morton tree per
+   (device arrays of numClouds+1 entries that start at 0, end at numSrc or
+   numTgt and never decrease: the clouds split each side).  Builds one 30-bit Morton tree per
    cloud and side (each identical to cubql_radix30_builder_f3 on that cloud
    with makeLeafThreshold = leafSize), then for every source point the
    closest target point of the same cloud (global index, -1 if the cloud has

@@ -141,6 +141,13 @@ def test_batch_edge_cases(cq, oracle_mod):
     torch.cuda.synchronize()
     bi, _ = oracle_mod.fcp_brute(Th[:9], Sh[:5])
     assert np.array_equal(B1.src_nn.cpu().numpy(), bi)
+    # offsets that do not split a side into clouds are refused on the host
+    # (no kernel runs)
+    bad = [torch.tensor(o, dtype=torch.int32, device="cuda") for o in ([1, 5], [0, 4], [0, 6, 5])]
+    for o in bad:
+        with pytest.raises(cq.CubqlError):
+            cq.ChamferBatch(S[:5], o, T[:9], torch.tensor([0, 9] + [9] * (o.numel() - 2), dtype=torch.int32,
+                                                          device="cuda"))
 
 
 def _check_loss_and_grads(oracle_mod, B, src, so, tgt, to, clouds):

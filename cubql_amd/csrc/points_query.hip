@@ -74,6 +74,47 @@ __device__ __forceinline__ bool fcp_one_on(StackT &stack, const cubql_bvh3f &bvh
   return cubql_dev::for_each_prim_except_on(stack, bvh, q.x, q.y, q.z, bestD, prim, seed, 0u, st);
 }
 
+// The same with the leaf prims read from a copy of the points in primID-slot
+// order (x, y, z and the primID's bits): a leaf's prims are one contiguous
+// run instead of a primID load and a gather each.  Tests the same prims in
+// the same order, so the result is the same.
+template <typename StackT>
+__device__ __forceinline__ bool fcp_sorted_on(StackT &stack, const cubql_bvh3f &bvh, const float4 *spts, float3 q,
+                                              float sqrMax, int32_t &best, float &bestD) {
+  best = -1;
+  bestD = sqrMax;
+  auto test = [&](uint32_t slot) {
+    const float4 p = spts[slot];
+    const float d = cubql_dev::sqr_dist_pt(p.x, p.y, p.z, q.x, q.y, q.z);
+    const int32_t pid = (int32_t)__float_as_uint(p.w);
+    if (d < bestD || (d == bestD && best >= 0 && pid < best)) {
+      bestD = d;
+      best = pid;
+    }
+  };
+  cubql_dev::NoStats st;
+  const uint64_t seedLeaf = cubql_dev::nearest_leaf(bvh, q.x, q.y, q.z, 0u, st);
+  const uint32_t seed = cubql_dev::oc_off(seedLeaf);
+  for (uint32_t j = 0; j < cubql_dev::oc_cnt(seedLeaf); j++) test(seed + j);
+  auto leaf = [&](const uint32_t *ids, uint32_t cnt) -> float {
+    const uint32_t off = (uint32_t)(ids - bvh.primIDs);
+    if (off == seed) return INFINITY;
+    for (uint32_t j = 0; j < cnt; j++) test(off + j);
+    return bestD;
+  };
+  return cubql_dev::for_each_leaf_on(stack, bvh, q.x, q.y, q.z, bestD, leaf, 0u, st);
+}
+
+// points in primID-slot order for fcp_sorted_on
+__global__ __launch_bounds__(256) void k_slot_points(const uint32_t *primIDs, uint32_t numPrims, const float *points,
+                                                     float4 *spts) {
+  for (uint32_t t = blockIdx.x * 256 + threadIdx.x; t < numPrims; t += gridDim.x * 256) {
+    const uint32_t i = primIDs[t];
+    const float3 p = load_pt(points, i);
+    spts[t] = make_float4(p.x, p.y, p.z, __uint_as_float(i));
+  }
+}
+
 // Entries of the fast kernel's LDS stack per lane: the seeded radius keeps
 // a closest-point query's stack shallow, and 8 entries x 8 bytes x 256 lanes
 // (16 KB a workgroup) leave the kernel at full occupancy; deeper queries
@@ -83,8 +124,9 @@ constexpr int kLdsStack = 8;
 // fast path: an 8-entry LDS stack over a binary tree, a 64-entry private
 // stack over a wide one; queries that overflow go to the deep kernel.
 // BVH: cubql_bvh3f (binary) or cubql_dev::WideBVH<W>.
-template <typename BVH>
-__global__ __launch_bounds__(kBlock) void k_fcp(BVH bvh, const float *points,
+// kSorted (binary trees): leaf prims from spts, the points in primID-slot order.
+template <typename BVH, bool kSorted = false>
+__global__ __launch_bounds__(kBlock) void k_fcp(BVH bvh, const float *points, const float4 *spts,
                                                 const float *queries, uint32_t nq, float sqrMax,
                                                 int32_t *outIds, float *outD, Overflow *ovf,
                                                 uint32_t *ovfList, const uint32_t *order) {
@@ -99,7 +141,8 @@ __global__ __launch_bounds__(kBlock) void k_fcp(BVH bvh, const float *points,
   if constexpr (std::is_same<BVH, cubql_bvh3f>::value) {
     __shared__ uint2 sStack[kLdsStack * kBlock];
     cubql_dev::LdsStack<kLdsStack, kBlock> stack(sStack + threadIdx.x);
-    done = fcp_one_on(stack, bvh, points, q, sqrMax, best, bestD);
+    if constexpr (kSorted) done = fcp_sorted_on(stack, bvh, spts, q, sqrMax, best, bestD);
+    else done = fcp_one_on(stack, bvh, points, q, sqrMax, best, bestD);
   } else {
     done = fcp_one<kFastStack>(bvh, points, q, sqrMax, best, bestD);
   }
@@ -134,8 +177,10 @@ __global__ __launch_bounds__(64) void k_fcp_deep(BVH bvh, const float *points,
 }
 
 // kNN with a register-resident candidate list of K entries (K >= k).
-template <int K, int STACK, typename BVH, typename Stats = cubql_dev::NoStats>
-__device__ __forceinline__ bool knn_one(const BVH &bvh, const float *points, float3 q,
+// kSorted (binary trees): leaf prims from spts, the points in primID-slot
+// order, a contiguous run per leaf (the same prims in the same order).
+template <int K, int STACK, bool kSorted = false, typename BVH, typename Stats = cubql_dev::NoStats>
+__device__ __forceinline__ bool knn_one(const BVH &bvh, const float *points, const float4 *spts, float3 q,
                                         int k, float sqrMax, int32_t (&ids)[K], float (&ds)[K],
                                         int &found, Stats &&st = Stats()) {
   found = 0;
@@ -147,9 +192,8 @@ __device__ __forceinline__ bool knn_one(const BVH &bvh, const float *points, flo
   // select to read the runtime-indexed k-th entry
   float kthD = INFINITY;
   int32_t kthId = 0x7fffffff;
-  auto prim = [&](uint32_t pid) -> float {
+  auto insert = [&](float3 p, uint32_t pid) -> float {
     st.prim();
-    float3 p = load_pt(points, pid);
     float d = cubql_dev::sqr_dist_pt(p.x, p.y, p.z, q.x, q.y, q.z);
     int32_t id = (int32_t)pid;
     if (!(d < sqrMax)) return cull;
@@ -188,7 +232,21 @@ __device__ __forceinline__ bool knn_one(const BVH &bvh, const float *points, flo
     }
     return cull;
   };
-  return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, sqrMax, prim, 0u, st);
+  if constexpr (kSorted) {
+    auto leaf = [&](const uint32_t *idp, uint32_t cnt) -> float {
+      const uint32_t off = (uint32_t)(idp - bvh.primIDs);
+      float r = INFINITY;
+      for (uint32_t c = 0; c < cnt; c++) {
+        const float4 v = spts[off + c];
+        r = fminf(r, insert(make_float3(v.x, v.y, v.z), __float_as_uint(v.w)));
+      }
+      return r;
+    };
+    return cubql_dev::for_each_leaf<STACK>(bvh, q.x, q.y, q.z, sqrMax, leaf, 0u, st);
+  } else {
+    auto prim = [&](uint32_t pid) -> float { return insert(load_pt(points, pid), pid); };
+    return cubql_dev::for_each_prim<STACK>(bvh, q.x, q.y, q.z, sqrMax, prim, 0u, st);
+  }
 }
 
 // Far siblings of the nearest leaf's last kSeedSibs ancestors whose nearest
@@ -207,10 +265,11 @@ constexpr int kSeedSibs = 4;
 // pairs under sqrMax whatever the visiting order, and every leaf within the
 // radius is still visited exactly once.  One loop visits every leaf, so the
 // insertion code is inlined once.
-template <int K, typename StackT, typename Stats = cubql_dev::NoStats>
+template <int K, bool kSorted = false, typename StackT, typename Stats = cubql_dev::NoStats>
 __device__ __forceinline__ bool knn_seeded_on(StackT &stack, const cubql_bvh3f &bvh, const float *points,
-                                              float3 q, int k, float sqrMax, int32_t (&ids)[K], float (&ds)[K],
-                                              int &found, Stats &&st = Stats()) {
+                                              const float4 *spts, float3 q, int k, float sqrMax,
+                                              int32_t (&ids)[K], float (&ds)[K], int &found,
+                                              Stats &&st = Stats()) {
   using namespace cubql_dev;
   found = 0;
 #pragma unroll
@@ -263,9 +322,17 @@ __device__ __forceinline__ bool knn_seeded_on(StackT &stack, const cubql_bvh3f &
       }
       if (!skip)
         for (uint32_t c = 0; c < cnt; c++) {
-          const uint32_t pid = bvh.primIDs[off + c];
+          uint32_t pid;
+          float3 p;
+          if constexpr (kSorted) {
+            const float4 v = spts[off + c];
+            pid = __float_as_uint(v.w);
+            p = make_float3(v.x, v.y, v.z);
+          } else {
+            pid = bvh.primIDs[off + c];
+            p = load_pt(points, pid);
+          }
           st.prim();
-          const float3 p = load_pt(points, pid);
           const float d = sqr_dist_pt(p.x, p.y, p.z, q.x, q.y, q.z);
           const int32_t id = (int32_t)pid;
           if (!(d < sqrMax)) continue;
@@ -362,8 +429,9 @@ __device__ __forceinline__ bool knn_seeded_on(StackT &stack, const cubql_bvh3f &
 constexpr int kKnnSeededMax = 8;
 constexpr int kKnnLdsStack = 16;
 
-template <int K, typename BVH>
-__global__ __launch_bounds__(kBlock) void k_knn(BVH bvh, const float *points,
+// kSorted (binary trees): leaf prims from spts, the points in primID-slot order.
+template <int K, typename BVH, bool kSorted = false>
+__global__ __launch_bounds__(kBlock) void k_knn(BVH bvh, const float *points, const float4 *spts,
                                                 const float *queries, uint32_t nq, int k,
                                                 float sqrMax, int32_t *outIds, float *outD,
                                                 int32_t *outFound, Overflow *ovf,
@@ -380,9 +448,9 @@ __global__ __launch_bounds__(kBlock) void k_knn(BVH bvh, const float *points,
   if constexpr (std::is_same<BVH, cubql_bvh3f>::value && K <= kKnnSeededMax) {
     __shared__ uint2 sStack[kKnnLdsStack * kBlock];
     cubql_dev::LdsStack<kKnnLdsStack, kBlock> stack(sStack + threadIdx.x);
-    done = knn_seeded_on<K>(stack, bvh, points, q, k, sqrMax, ids, ds, found);
+    done = knn_seeded_on<K, kSorted>(stack, bvh, points, spts, q, k, sqrMax, ids, ds, found);
   } else {
-    done = knn_one<K, kFastStack>(bvh, points, q, k, sqrMax, ids, ds, found);
+    done = knn_one<K, kFastStack, kSorted>(bvh, points, spts, q, k, sqrMax, ids, ds, found);
   }
   if (!done) {
     uint32_t slot = atomicAdd(&ovf->count, 1u);
@@ -414,9 +482,9 @@ __global__ __launch_bounds__(64) void k_knn_deep(BVH bvh, const float *points,
     bool done;
     if constexpr (std::is_same<BVH, cubql_bvh3f>::value && K <= kKnnSeededMax) {
       cubql_dev::PrivateStack<kDeepStack> stack;
-      done = knn_seeded_on<K>(stack, bvh, points, q, k, sqrMax, ids, ds, found);
+      done = knn_seeded_on<K>(stack, bvh, points, nullptr, q, k, sqrMax, ids, ds, found);
     } else {
-      done = knn_one<K, kDeepStack>(bvh, points, q, k, sqrMax, ids, ds, found);
+      done = knn_one<K, kDeepStack>(bvh, points, nullptr, q, k, sqrMax, ids, ds, found);
     }
     if (!done) {
       // deeper than the deep stack: marked incomplete, not "none found"
@@ -447,10 +515,22 @@ __global__ void k_fill_notfound(uint32_t nq, int k, int32_t *ids, float *d, int3
 template <int K, typename BVH>
 int launch_knn(const BVH &bvh, const float *points, const float *queries, uint32_t nq,
                int k, float sqrMax, int32_t *ids, float *d, int32_t *found, Overflow *ovf,
-               uint32_t *ovfList, const uint32_t *order, hipStream_t s) {
+               uint32_t *ovfList, const uint32_t *order, float4 *spts, hipStream_t s) {
   int ps = prof_begin(s);
-  k_knn<K, BVH><<<div_up(nq, kBlock), kBlock, 0, s>>>(bvh, points, queries, nq, k, sqrMax, ids, d, found,
-                                                  ovf, ovfList, order);
+  bool sorted = false;
+  if constexpr (std::is_same<BVH, cubql_bvh3f>::value) {
+    if (spts) {
+      k_slot_points<<<std::max(1u, std::min(div_up(bvh.numPrims, 256u), 2048u)), 256, 0, s>>>(
+          bvh.primIDs, bvh.numPrims, points, spts);
+      k_knn<K, BVH, true><<<div_up(nq, kBlock), kBlock, 0, s>>>(bvh, points, spts, queries, nq, k, sqrMax, ids, d,
+                                                                found, ovf, ovfList, order);
+      sorted = true;
+    }
+  }
+  if (!sorted) {
+    k_knn<K, BVH><<<div_up(nq, kBlock), kBlock, 0, s>>>(bvh, points, nullptr, queries, nq, k, sqrMax, ids, d, found,
+                                                        ovf, ovfList, order);
+  }
   prof_end(ps, s, "knn_points");
   prof_set(ps, (12.0 + 8.0 * k + 4.0) * nq, nq);
   // one workgroup per CU: the overflow list holds a few percent of the
@@ -514,11 +594,13 @@ const cubql_box3f *root_box_ws(const cubql_dev::WideBVH<W> &bvh, hipStream_t s, 
 }
 
 // Closest-point queries on a caller-provided workspace of fcp_ws_bytes(nq)
-// bytes; allocates nothing.
+// bytes; allocates nothing.  With spts (binary trees: numPrims float4 slots)
+// the points are first copied into primID-slot order and the leaves read
+// them from there.
 template <typename BVH>
 int fcp_run_ws(const BVH &bvh, const float *d_points, const float *d_queries, uint32_t numQueries,
                float sqrMaxSearchDist, int32_t *d_outIDs, float *d_outSqrDist, hipStream_t s, char *ws,
-               const char *profName) {
+               const char *profName, float4 *spts = nullptr) {
   if (bvh.numNodes == 0) {
     k_fill_notfound<<<256, 256, 0, s>>>(numQueries, 1, d_outIDs, d_outSqrDist, nullptr);
     CUBQL_LAUNCH_CHECK();
@@ -535,8 +617,22 @@ int fcp_run_ws(const BVH &bvh, const float *d_points, const float *d_queries, ui
     CUBQL_CHECK(query_order_box_ws(rootBox, d_queries, numQueries, s,
                                    ws + kWsList + align256((size_t)numQueries * 4), &order));
   int ps = prof_begin(s);
-  k_fcp<BVH><<<div_up(numQueries, kBlock), kBlock, 0, s>>>(bvh, d_points, d_queries, numQueries, sqrMaxSearchDist,
-                                                           d_outIDs, d_outSqrDist, ovf, ovfList, order);
+  bool sorted = false;
+  if constexpr (std::is_same<BVH, cubql_bvh3f>::value) {
+    if (spts) {
+      k_slot_points<<<std::max(1u, std::min(div_up(bvh.numPrims, 256u), 2048u)), 256, 0, s>>>(
+          bvh.primIDs, bvh.numPrims, d_points, spts);
+      k_fcp<BVH, true><<<div_up(numQueries, kBlock), kBlock, 0, s>>>(bvh, d_points, spts, d_queries, numQueries,
+                                                                     sqrMaxSearchDist, d_outIDs, d_outSqrDist, ovf,
+                                                                     ovfList, order);
+      sorted = true;
+    }
+  }
+  if (!sorted) {
+    k_fcp<BVH><<<div_up(numQueries, kBlock), kBlock, 0, s>>>(bvh, d_points, nullptr, d_queries, numQueries,
+                                                             sqrMaxSearchDist, d_outIDs, d_outSqrDist, ovf, ovfList,
+                                                             order);
+  }
   prof_end(ps, s, profName);
   // compulsory bytes only: query in, id + distance out
   prof_set(ps, 20.0 * numQueries, numQueries);
@@ -552,8 +648,15 @@ int fcp_run(const BVH &bvh, const float *d_points, const float *d_queries, uint3
   Scratch scr(nullptr, s);
   char *ws = nullptr;
   if (bvh.numNodes) CUBQL_CHECK(scr.alloc(&ws, fcp_ws_bytes(numQueries)));
+  // the slot-ordered copy pays when the queries test at least about as many
+  // prims as the copy moves
+  float4 *spts = nullptr;
+  if constexpr (std::is_same<BVH, cubql_bvh3f>::value) {
+    if (bvh.numNodes && bvh.numPrims && numQueries >= kQueryOrderMin && (uint64_t)numQueries * 8 >= bvh.numPrims)
+      CUBQL_CHECK(scr.alloc(&spts, bvh.numPrims));
+  }
   return fcp_run_ws(bvh, d_points, d_queries, numQueries, sqrMaxSearchDist, d_outIDs, d_outSqrDist, s, ws,
-                    profName);
+                    profName, spts);
 }
 
 template <typename BVH>
@@ -578,11 +681,18 @@ int knn_run(const BVH &bvh, const float *d_points, const float *d_queries, uint3
     CUBQL_CHECK(scr.alloc(&w, query_order_ws_bytes(numQueries)));
     CUBQL_CHECK(query_order_box_ws(rootBox, d_queries, numQueries, s, w, &order));
   }
-  if (k <= 4) return launch_knn<4>(bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
-  if (k <= 8) return launch_knn<8>(bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
-  if (k <= 16) return launch_knn<16>(bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
-  if (k <= 32) return launch_knn<32>(bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
-  return launch_knn<64>(bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, s);
+  // the slot-ordered copy pays when the queries test at least about as many
+  // prims as the copy moves
+  float4 *spts = nullptr;
+  if constexpr (std::is_same<BVH, cubql_bvh3f>::value) {
+    if (bvh.numPrims && numQueries >= kQueryOrderMin && (uint64_t)numQueries * k * 2 >= bvh.numPrims)
+      CUBQL_CHECK(scr.alloc(&spts, bvh.numPrims));
+  }
+  if (k <= 4) return launch_knn<4>(bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, spts, s);
+  if (k <= 8) return launch_knn<8>(bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, spts, s);
+  if (k <= 16) return launch_knn<16>(bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, spts, s);
+  if (k <= 32) return launch_knn<32>(bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, spts, s);
+  return launch_knn<64>(bvh, d_points, d_queries, numQueries, k, sqrMaxSearchDist, d_outIDs, d_outSqrDist, d_numFound, ovf, ovfList, order, spts, s);
 }
 
 template <int W>
@@ -769,9 +879,9 @@ __global__ __launch_bounds__(64) void k_points_stats(cubql_bvh3f bvh, const floa
       // the traversal k_knn runs, on the deep stack
       if constexpr (K <= kKnnSeededMax) {
         cubql_dev::PrivateStack<kDeepStack> stack;
-        knn_seeded_on<K>(stack, bvh, points, q, k, sqrMax, ids, ds, found, st);
+        knn_seeded_on<K>(stack, bvh, points, nullptr, q, k, sqrMax, ids, ds, found, st);
       } else {
-        knn_one<K, kDeepStack>(bvh, points, q, k, sqrMax, ids, ds, found, st);
+        knn_one<K, kDeepStack>(bvh, points, nullptr, q, k, sqrMax, ids, ds, found, st);
       }
     }
   }

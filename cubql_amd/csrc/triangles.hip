@@ -88,6 +88,55 @@ __device__ __forceinline__ bool fcp_tri_one_on(StackT &stack, const cubql_bvh3f 
 }
 
 
+// The same with the leaf triangles read from a copy of the mesh in
+// primID-slot order (three float4 per slot: a and the triangle id's bits, b,
+// c): a leaf's triangles are one contiguous run instead of a primID, index
+// and vertex gather chain each.  Tests the same triangles in the same order
+// with the same vertices, so the result is the same.
+template <typename StackT>
+__device__ __forceinline__ bool fcp_tri_sorted_on(StackT &stack, const cubql_bvh3f &bvh, const float4 *stris, F3 q,
+                                                  float sqrMax, int32_t &best, float &bestD, F3 &bestP) {
+  best = -1;
+  bestD = sqrMax;
+  bestP = {0.f, 0.f, 0.f};
+  auto test = [&](uint32_t slot) {
+    const float4 va = stris[3 * (size_t)slot], vb = stris[3 * (size_t)slot + 1], vc = stris[3 * (size_t)slot + 2];
+    const F3 a = {va.x, va.y, va.z}, b = {vb.x, vb.y, vb.z}, c = {vc.x, vc.y, vc.z};
+    const int32_t t = (int32_t)__float_as_uint(va.w);
+    F3 P;
+    float w[3];
+    const float d = tri_closest<false>(a, b, c, q, P, w);
+    if (d < bestD || (d == bestD && best >= 0 && t < best)) {
+      bestD = d;
+      best = t;
+      bestP = P;
+    }
+  };
+  NoStats st;
+  const uint64_t seedLeaf = nearest_leaf(bvh, q.x, q.y, q.z, 0u, st);
+  const uint32_t seed = oc_off(seedLeaf);
+  for (uint32_t j = 0; j < oc_cnt(seedLeaf); j++) test(seed + j);
+  auto leaf = [&](const uint32_t *ids, uint32_t cnt) -> float {
+    const uint32_t off = (uint32_t)(ids - bvh.primIDs);
+    if (off == seed) return INFINITY;
+    for (uint32_t j = 0; j < cnt; j++) test(off + j);
+    return bestD;
+  };
+  return for_each_leaf_on(stack, bvh, q.x, q.y, q.z, bestD, leaf, 0u, st);
+}
+
+// the mesh in primID-slot order for fcp_tri_sorted_on
+__global__ __launch_bounds__(256) void k_slot_tris(const uint32_t *primIDs, uint32_t numPrims, const int32_t *tris,
+                                                   const float *verts, float4 *stris) {
+  for (uint32_t s = blockIdx.x * 256 + threadIdx.x; s < numPrims; s += gridDim.x * 256) {
+    const uint32_t t = primIDs[s];
+    F3 a, b, c;
+    tri_verts(tris, verts, t, a, b, c);
+    stris[3 * (size_t)s] = make_float4(a.x, a.y, a.z, __uint_as_float(t));
+    stris[3 * (size_t)s + 1] = make_float4(b.x, b.y, b.z, 0.f);
+    stris[3 * (size_t)s + 2] = make_float4(c.x, c.y, c.z, 0.f);
+  }
+}
 
 __device__ __forceinline__ void write_result(uint32_t qi, int32_t best, float bestD, F3 P, int32_t *ids,
                                              float *d2, float *cp) {
@@ -100,8 +149,11 @@ __device__ __forceinline__ void write_result(uint32_t qi, int32_t best, float be
   }
 }
 
+// kSorted: the leaves read stris, the mesh in primID-slot order
+template <bool kSorted>
 __global__ __launch_bounds__(kBlock) void k_fcp_tris(cubql_bvh3f bvh, const int32_t *tris, const float *verts,
-                                                     const float *queries, uint32_t nq, float sqrMax,
+                                                     const float4 *stris, const float *queries, uint32_t nq,
+                                                     float sqrMax,
                                                      int32_t *ids, float *d2, float *cp, Overflow *ovf,
                                                      uint32_t *ovfList, const uint32_t *order) {
   // Morton-ordered queries: one contiguous run of them per XCD
@@ -115,7 +167,10 @@ __global__ __launch_bounds__(kBlock) void k_fcp_tris(cubql_bvh3f bvh, const int3
   // private stack: triangle queries on an SAH tree stack many far
   // children, and a short LDS stack sent too many of them to the deep rerun
   PrivateStack<kFastStack> stack;
-  if (!fcp_tri_one_on(stack, bvh, tris, verts, q, sqrMax, best, bestD, P)) {
+  bool done;
+  if constexpr (kSorted) done = fcp_tri_sorted_on(stack, bvh, stris, q, sqrMax, best, bestD, P);
+  else done = fcp_tri_one_on(stack, bvh, tris, verts, q, sqrMax, best, bestD, P);
+  if (!done) {
     ovfList[atomicAdd(&ovf->count, 1u)] = qi;
     return;
   }
@@ -338,10 +393,23 @@ extern "C" int cubql_fcp_triangles_f3(const cubql_bvh3f *bvh, const int32_t *d_t
   CUBQL_CHECK(fill_u32((uint32_t *)ovf, 0u, 2, s));
   const uint32_t *order = nullptr;
   CUBQL_CHECK(query_order(*bvh, d_queries, numQueries, s, scr, &order));
+  // the slot-ordered copy of the mesh pays when the queries test at least
+  // about as many triangles as the copy moves
+  float4 *stris = nullptr;
+  if (bvh->numPrims && numQueries >= kQueryOrderMin && (uint64_t)numQueries * 8 >= bvh->numPrims)
+    CUBQL_CHECK(scr.alloc(&stris, 3 * (size_t)bvh->numPrims));
   int ps = prof_begin(s);
-  tri::k_fcp_tris<<<div_up(numQueries, tri::kBlock), tri::kBlock, 0, s>>>(
-      *bvh, d_tris, d_verts, d_queries, numQueries, sqrMax, d_outTriIDs, d_outSqrDist, d_outClosest, ovf,
-      ovfList, order);
+  if (stris) {
+    tri::k_slot_tris<<<std::max(1u, std::min(div_up(bvh->numPrims, 256u), 2048u)), 256, 0, s>>>(
+        bvh->primIDs, bvh->numPrims, d_tris, d_verts, stris);
+    tri::k_fcp_tris<true><<<div_up(numQueries, tri::kBlock), tri::kBlock, 0, s>>>(
+        *bvh, d_tris, d_verts, stris, d_queries, numQueries, sqrMax, d_outTriIDs, d_outSqrDist, d_outClosest, ovf,
+        ovfList, order);
+  } else {
+    tri::k_fcp_tris<false><<<div_up(numQueries, tri::kBlock), tri::kBlock, 0, s>>>(
+        *bvh, d_tris, d_verts, nullptr, d_queries, numQueries, sqrMax, d_outTriIDs, d_outSqrDist, d_outClosest,
+        ovf, ovfList, order);
+  }
   prof_end(ps, s, "fcp_triangles");
   prof_set(ps, 20.0 * numQueries, numQueries);
   tri::k_fcp_tris_deep<<<64, 64, 0, s>>>(*bvh, d_tris, d_verts, d_queries, sqrMax, d_outTriIDs,

@@ -1550,13 +1550,17 @@ int rebin_build(cubql_bvh3f *bvh, const cubql_box3f *boxes, uint32_t n, uint32_t
     // rebin round: the jobs become level L's open nodes with rebuilt keys
     rounds++;
     const uint32_t J = hc[C_JOBS];
+    // job ids are below J: the sort needs no more key bits than that
+    int roundJobBits = 1;
+    while ((1ull << roundJobBits) < (uint64_t)J) roundJobBits++;
+    roundJobBits = std::min(roundJobBits, jobBits);
     k_rebin_setup<<<1, 1024, 0, s>>>(c, L);
     k_rebin_grow<<<grid, kBlock, 0, s>>>(c);
     k_rebin_finalize<<<std::max<uint32_t>(1u, std::min<uint32_t>(div_up(J, kBlock), 1024u)), kBlock, 0, s>>>(c);
     k_rebin_keys<<<grid, kBlock, 0, s>>>(c, rkA, rvA);
     uint64_t *rkRes = nullptr;
     uint32_t *rvRes = nullptr;
-    st = radix_sort_pairs_dev_u64(rkA, rvA, rkB, rvB, &c.ctr[C_JOBPRIMS], n, 32 + jobBits, s, mem, &rkRes, &rvRes,
+    st = radix_sort_pairs_dev_u64(rkA, rvA, rkB, rvB, &c.ctr[C_JOBPRIMS], n, 32 + roundJobBits, s, mem, &rkRes, &rvRes,
                                   &c.ctr[C_SPIN]);
     if (st) return fail(st);
     k_rebin_writeback<<<grid, kBlock, 0, s>>>(c, keys, rkRes, rvRes);
